@@ -1,0 +1,116 @@
+"""Oracle-side reading of a `picomax` argument list.
+
+Test infrastructure.  Restates the reference's option handling closely enough
+to drive the oracle on the golden cases: first-occurrence token lookup
+(inputParser, src/pmx.hpp:190-245), inputProcess (src/pmx_io.cpp:423-503),
+init_crystal (src/pmx_io.cpp:108-310) and init_mater (src/pmx_io.cpp:315-417).
+`run(args)` returns what the CLI would write: the .dat array and the .gpt array.
+"""
+import math
+import os
+
+import numpy as np
+
+import orc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+
+class Args:
+    def __init__(self, tokens):
+        self.t = list(tokens)
+
+    def has(self, opt):
+        return opt in self.t
+
+    def val(self, opt):
+        if opt in self.t:
+            i = self.t.index(opt)
+            if i + 1 < len(self.t):
+                return self.t[i + 1]
+        return ""
+
+    def vec(self, opt):
+        return self.val(opt).split(",")
+
+    def int(self, opt, default):
+        return int(self.val(opt)) if self.has(opt) else default
+
+    def float(self, opt, default):
+        return float(self.val(opt)) if self.has(opt) else default
+
+
+def _pairs(s):
+    out = {}
+    for p in s.split(","):
+        k, v = p.split(":")
+        out[float(k)] = float(v)
+    return out
+
+
+def build(tokens):
+    """System + run parameters for an argument list (golden paths resolved)."""
+    tokens = [t.replace("{golden}", GOLDEN) for t in tokens]
+    A = Args(tokens)
+    name = A.val("-crystal")
+    a = float(A.val("-a"))
+    f = A.float("-f", math.sqrt(8.0 / 3.0))
+    u = A.float("-u", 3.0 / 8.0)
+    encut = A.float("-encut", 200.0)
+    gcut = math.sqrt(encut * (a / 2.0 / orc.PI) ** 2 / orc.KCON)
+    gcut = A.float("-gcut", gcut)
+    vg = [float(x) for x in A.vec("-vg")]
+    kw = {}
+    if A.has("-use_uniform_us_so"):
+        kw["uniform_us_so"] = A.float("-uniform_us_so_val", 0.0)
+    elif A.has("-us_so"):
+        kw["us_so"] = _pairs(A.val("-us_so"))
+    if A.has("-ua_so_val_g0"):
+        kw["ua_so_val_g0"] = float(A.val("-ua_so_val_g0"))
+    if A.has("-ua_so"):
+        kw["ua_so"] = _pairs(A.val("-ua_so"))
+    neps = A.int("-neps", 1)
+    sysm = orc.System(name, a=a, f=f, u=u, vg=vg, gcut=gcut, neps=neps, **kw)
+    p = dict(switch=A.val("-switch"), nband=A.int("-nband", 8), nvalence=A.int("-nvalence", 4),
+             neps=neps, kk=A.int("-kk", 0), delta=A.int("-delta", 0),
+             epsilon=A.float("-epsilon", 0.1))
+    p["refpoint"] = [float(x) for x in A.vec("-refpoint")] if A.has("-refpoint") else [0.0, 0.0, 0.0]
+    gz = [float(x) for x in A.vec("-gammazero")] if A.has("-gammazero") else [0.0, 0.0, 0.0]
+    if A.has("-qvec"):
+        v = [float(x) for x in A.vec("-qvec")]
+        p["Q"] = np.array(v, float).reshape(-1, 3)
+    elif A.has("-qpath"):
+        p["Q"] = orc.qpath(name, A.vec("-qpath"), [int(x) for x in A.vec("-qnum")], f, gz)
+    else:
+        p["Q"] = np.zeros((0, 3))
+    if A.has("-nfreq") and A.has("-dfreq"):
+        p["nfreq"] = int(A.val("-nfreq"))
+        p["dfreq"] = float(A.val("-dfreq"))
+    if A.has("-kpointfile"):
+        p["K"], p["KW"] = orc.read_kpoints(A.val("-kpointfile"))
+    return sysm, p
+
+
+def run(tokens):
+    """(dat, gpt) the reference CLI writes for these arguments, from the oracle."""
+    s, p = build(tokens)
+    off = s.ref_energy(p["refpoint"], p["nvalence"])
+    if p["switch"] == "band":
+        return s.bands(p["Q"], p["nband"], off).ravel(), None
+    re, im = s.chi(p["K"], p["KW"], p["Q"], p["nband"], off, p["delta"], p["epsilon"],
+                   p["nfreq"], p["dfreq"], p["kk"])
+    return np.concatenate([re, im]), s.G[:p["neps"]].copy()
+
+
+def slice_error(got, ref, nslice):
+    """Max error per slice normalised by the slice's max |value| (Re and Im
+    halves together), with a floor of 1e-12 of the global max so that slices
+    that are identically zero in the reference do not divide by zero."""
+    n = len(ref) // 2
+    gr, gi = got[:n].reshape(nslice, -1), got[n:].reshape(nslice, -1)
+    rr, ri = ref[:n].reshape(nslice, -1), ref[n:].reshape(nslice, -1)
+    scale = np.maximum(np.abs(rr).max(1), np.abs(ri).max(1))
+    scale = np.maximum(scale, 1e-12 * max(np.abs(ref).max(), 1e-300))
+    err = np.maximum(np.abs(gr - rr).max(1), np.abs(gi - ri).max(1)) / scale
+    return float(err.max())

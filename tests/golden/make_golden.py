@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures under tests/golden/ by running the reference CLI.
+
+The reference binary is compiled from its own sources where they lie
+(`make -C oracle ref`, flags of src/compile_cmd.sh:15-17) into
+oracle/_ref/picomax_ref.  This script runs it on small inputs and keeps only
+its output files (.dat, .gpt) plus the k-point files it read, together with
+manifest.json: the exact argument list of every case and the sha256 of every
+output, so each fixture can be regenerated and audited.
+
+Case selection follows the reference's hazards: Si runs use the L point as
+energy reference with -nvalence 8 (the zero then sits mid-gap) and an -nband
+at which no degenerate multiplet is cut on the grid, because the reference's
+chi depends on Eigen's arbitrary eigenvector choice when NBAND splits a
+multiplet (src/pmx_chi.cpp:391-411).  Every chi case also keeps at least as
+many conduction states at each k as valence states at k+q: the reference's
+clean-up frees C_k rows by the valence count (src/pmx_chi.cpp:697-703) and
+aborts otherwise.  tests/test_oracle.py checks both
+conditions on every chi fixture with the oracle.
+
+Usage: python3 tests/golden/make_golden.py [--ref oracle/_ref/picomax_ref]
+"""
+import argparse
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import orc  # noqa: E402
+from picomax_amd import kpoints  # noqa: E402
+
+SI_VG = "3,4,8,11,-0.21,0,0.04,0.08,-0.21,0,0.04,0.08"
+# zinc-blende with antisymmetric form factors (GaAs-like): V1,2 = V_S +- V_A
+GAAS_VG = "3,4,8,11,-0.1407,0.05,0.0,0.0577,-0.2807,-0.05,0.0,0.0277"
+SI = ["-crystal", "zb", "-a", "5.43", "-vg", SI_VG]
+GC59 = ["-gcut", "3.4641016151377544"]
+# trigonal Te, first 6 |G|^2 shells of f = 1.33; three identical atoms
+TE_KEYS = ["0.565323", "1.333333", "1.898656", "2.261292", "3.594626", "4.0"]
+TE_V = ["-0.2105", "-0.1322", "0.0451", "0.0214", "0.0127", "-0.0093"]
+TE_VG = ",".join(TE_KEYS + TE_V * 3)
+TE = ["-crystal", "te", "-a", "4.45", "-f", "1.33", "-u", "0.26", "-vg", TE_VG]
+TE_US = ",".join("%s:%s" % kv for kv in zip(TE_KEYS[:3], ["0.0152", "-0.0061", "0.0083"]))
+TE_UA = ",".join("%s:%s" % kv for kv in zip(TE_KEYS[:3], ["0.0117", "0.0042", "-0.0071"]))
+WZ_KEYS = ["0.378233", "1.333333", "1.51293", "1.711566", "2.846264", "3.404093"]
+WZ_V = ["-0.3101", "-0.1810", "0.0902", "0.0415", "0.0254", "-0.0122"]
+WZ_V2 = ["-0.2401", "0.1110", "0.0502", "-0.0315", "0.0054", "0.0222"]
+WZ_VG = ",".join(WZ_KEYS + WZ_V + WZ_V + WZ_V2 + WZ_V2)
+WZ = ["-crystal", "wz", "-a", "3.19", "-f", "1.626", "-u", "0.375", "-vg", WZ_VG]
+
+TE_K = [[0.1311, 0.0527, 0.2113], [-0.2471, 0.1902, -0.0817], [0.3149, -0.0733, 0.1267]]
+WZ_K = [[0.0713, 0.1429, 0.0811], [-0.1931, 0.0512, -0.1513]]
+
+CASES = {
+    # C1 band, exactly as BASELINE's first config (H4: offset at Gamma, nvalence 4)
+    "si_band_c1": dict(args=["-switch", "band"] + SI + GC59 +
+                       ["-nband", "16", "-nvalence", "4", "-qpath", "L,G,X", "-qnum", "60,60"],
+                       keep=[".dat"]),
+    # C1 chi: 8^3 Gamma grid, NEPS 9, 200 w, Gaussian 0.1, KK on; -nband 30 cuts no multiplet
+    "si_chi_c1": dict(args=["-switch", "epsij"] + SI + GC59 +
+                      ["-nband", "30", "-nvalence", "8", "-refpoint", "0.5,0.5,0.5",
+                       "-qvec", "0.01,0,0", "-kpointfile", "{golden}/kpoints_zb_8.txt",
+                       "-neps", "9", "-nfreq", "200", "-dfreq", "0.05", "-delta", "0",
+                       "-epsilon", "0.1", "-kk", "1"],
+                      kgrid=("zb", 8), keep=[".dat", ".gpt"]),
+    # Lorentzian window (delta 1), two q, KK on, 4^3 grid
+    "si_chi_lor": dict(args=["-switch", "epsij"] + SI + GC59 +
+                       ["-nband", "28", "-nvalence", "8", "-refpoint", "0.5,0.5,0.5",
+                        "-qvec", "0.01,0,0,0.0,0.02,0.01", "-kpointfile", "{golden}/kpoints_zb_4.txt",
+                        "-neps", "4", "-nfreq", "400", "-dfreq", "0.02", "-delta", "1",
+                        "-epsilon", "0.05", "-kk", "1"],
+                       kgrid=("zb", 4), keep=[".dat", ".gpt"]),
+    # triangle broadening, no KK
+    "si_chi_tri": dict(args=["-switch", "epsij"] + SI + GC59 +
+                       ["-nband", "36", "-nvalence", "8", "-refpoint", "0.5,0.5,0.5",
+                        "-qvec", "0.02,0.01,0", "-kpointfile", "{golden}/kpoints_zb_4.txt",
+                        "-neps", "3", "-nfreq", "150", "-dfreq", "0.1", "-delta", "2",
+                        "-epsilon", "0.3", "-kk", "0"],
+                       kgrid=("zb", 4), keep=[".dat", ".gpt"]),
+    # rectangle broadening, KK on
+    "si_chi_rect": dict(args=["-switch", "epsij"] + SI + GC59 +
+                        ["-nband", "28", "-nvalence", "8", "-refpoint", "0.5,0.5,0.5",
+                         "-qvec", "0.01,0.0,0.0", "-kpointfile", "{golden}/kpoints_zb_4.txt",
+                         "-neps", "2", "-nfreq", "150", "-dfreq", "0.1", "-delta", "3",
+                         "-epsilon", "0.25", "-kk", "1"],
+                        kgrid=("zb", 4), keep=[".dat", ".gpt"]),
+    # complex H(k): zinc-blende with antisymmetric form factors
+    "gaas_band": dict(args=["-switch", "band", "-crystal", "zb", "-a", "5.65", "-vg", GAAS_VG,
+                            "-gcut", "3.4641016151377544", "-nband", "20", "-nvalence", "8",
+                            "-refpoint", "0,0,0", "-qpath", "L,G,X,W,K,G", "-qnum", "8,10,4,3,9"],
+                      keep=[".dat"]),
+    # trigonal Te with SOC maps (symmetric + antisymmetric), band path
+    "te_soc_band": dict(args=["-switch", "band"] + TE +
+                        ["-gcut", "2.6", "-us_so", TE_US, "-ua_so", TE_UA, "-nband", "48",
+                         "-nvalence", "18", "-refpoint", "0,0,0",
+                         "-qpath", "G,A,H,K,G,M,L", "-qnum", "4,4,4,4,4,4"],
+                        keep=[".dat"]),
+    # Te SOC chi, q-path with gammazero (NQ = 3), Lorentzian, KK on
+    "te_soc_chi": dict(args=["-switch", "epsij"] + TE +
+                       ["-gcut", "2.0", "-us_so", TE_US, "-ua_so", TE_UA, "-ua_so_val_g0", "0.0031",
+                        "-nband", "48", "-nvalence", "18", "-refpoint", "0,0,0",
+                        "-qpath", "G,A", "-qnum", "2", "-gammazero", "0,0,0.001",
+                        "-kpointfile", "{golden}/kpoints_te_3.txt", "-neps", "3",
+                        "-nfreq", "120", "-dfreq", "0.1", "-delta", "1", "-epsilon", "0.15", "-kk", "1"],
+                       kfile=("kpoints_te_3.txt", TE_K), keep=[".dat", ".gpt"]),
+    # uniform symmetric SOC: H(k) uses the uniform value, socCurrent the (empty) map
+    "te_usoc_chi": dict(args=["-switch", "epsij"] + TE +
+                        ["-gcut", "2.0", "-use_uniform_us_so", "-uniform_us_so_val", "0.02",
+                         "-ua_so", TE_UA, "-nband", "48", "-nvalence", "18", "-refpoint", "0,0,0",
+                         "-qvec", "0,0.013,0.02", "-kpointfile", "{golden}/kpoints_te_3.txt",
+                         "-neps", "4", "-nfreq", "100", "-dfreq", "0.12", "-delta", "0",
+                         "-epsilon", "0.2", "-kk", "0"],
+                        kfile=("kpoints_te_3.txt", TE_K), keep=[".dat", ".gpt"]),
+    # wurtzite band path (4-atom cell, complex H)
+    "wz_band": dict(args=["-switch", "band"] + WZ +
+                    ["-gcut", "2.2", "-nband", "24", "-nvalence", "16", "-refpoint", "0,0,0",
+                     "-qpath", "M,G,A,L,H,K", "-qnum", "5,5,5,5,5"],
+                    keep=[".dat"]),
+    # full G order of hexagonal cells (.gpt with NEPS = NPW): pins the std::sort tie order
+    "te_gpt": dict(args=["-switch", "epsij"] + TE +
+                   ["-gcut", "2.6", "-nband", "48", "-nvalence", "18", "-qvec", "0.01,0,0",
+                    "-kpointfile", "{golden}/kpoints_te_1.txt", "-neps", "97", "-nfreq", "2",
+                    "-dfreq", "0.1", "-delta", "0", "-epsilon", "0.1", "-kk", "0"],
+                   kfile=("kpoints_te_1.txt", TE_K[:1]), keep=[".gpt"]),
+    "wz_gpt": dict(args=["-switch", "epsij"] + WZ +
+                   ["-gcut", "2.2", "-nband", "40", "-nvalence", "16", "-qvec", "0.01,0,0",
+                    "-kpointfile", "{golden}/kpoints_wz_1.txt", "-neps", "67", "-nfreq", "2",
+                    "-dfreq", "0.1", "-delta", "0", "-epsilon", "0.1", "-kk", "0"],
+                   kfile=("kpoints_wz_1.txt", WZ_K[:1]), keep=[".gpt"]),
+    "wz_chi": dict(args=["-switch", "epsij"] + WZ +
+                   ["-gcut", "2.2", "-nband", "40", "-nvalence", "16", "-refpoint", "0,0,0",
+                    "-qvec", "0.0,0.0,0.012", "-kpointfile", "{golden}/kpoints_wz_2.txt",
+                    "-neps", "5", "-nfreq", "160", "-dfreq", "0.1", "-delta", "0",
+                    "-epsilon", "0.2", "-kk", "1"],
+                   kfile=("kpoints_wz_2.txt", WZ_K), keep=[".dat", ".gpt"]),
+}
+
+
+def sha256(path):
+    h = hashlib.sha256()
+    with open(path, "rb") as fh:
+        h.update(fh.read())
+    return h.hexdigest()
+
+
+def prepare_kfiles(case):
+    if "kgrid" in case:
+        cr, n = case["kgrid"]
+        name = "kpoints_%s_%d.txt" % (cr, n)
+        path = os.path.join(HERE, name)
+        if not os.path.exists(path):
+            kpoints.write_kpoints(path, kpoints.gamma_grid(cr, n))
+    if "kfile" in case:
+        name, K = case["kfile"]
+        path = os.path.join(HERE, name)
+        if not os.path.exists(path):
+            kpoints.write_kpoints(path, np.array(K, float))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default=os.path.join(ROOT, "oracle", "_ref", "picomax_ref"))
+    ap.add_argument("--only", nargs="*")
+    opt = ap.parse_args()
+    if not os.path.exists(opt.ref):
+        sys.exit("reference binary missing: run `make -C oracle ref`")
+    man_path = os.path.join(HERE, "manifest.json")
+    manifest = json.load(open(man_path)) if os.path.exists(man_path) else {}
+    env = dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8"))
+    for name, case in CASES.items():
+        if opt.only and name not in opt.only:
+            continue
+        prepare_kfiles(case)
+        args = [a.replace("{golden}", HERE) for a in case["args"]]
+        with tempfile.TemporaryDirectory() as tmp:
+            cmd = [opt.ref] + args + ["-wdir", tmp, "-outputfile", name]
+            r = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+            if r.returncode != 0:
+                sys.exit("%s failed:\n%s" % (name, r.stdout[-2000:]))
+            entry = {"args": [a.replace(HERE, "{golden}") for a in case["args"]], "files": {}}
+            for ext in case["keep"]:
+                src = os.path.join(tmp, name + ext)
+                dst = os.path.join(HERE, name + ext)
+                with open(src, "rb") as fi, open(dst, "wb") as fo:
+                    fo.write(fi.read())
+                entry["files"][name + ext] = sha256(dst)
+        manifest[name] = entry
+        print("wrote", name, sorted(entry["files"]))
+    with open(man_path, "w") as fh:
+        json.dump(manifest, fh, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()
