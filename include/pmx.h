@@ -1,0 +1,183 @@
+/*
+ * pmx.h -- C-ABI of libpmx, the MI355X (gfx950) implementation of PicoMax's
+ * EPM band-structure and chi(q,w) path.
+ *
+ * The reference has no library interface: its CLI calls C++ functions on
+ * global state.  Each entry point below replaces one of those functions (the
+ * citation names it); the `picomax` CLI, the Python wrapper and the tests all
+ * go through this header.
+ *
+ * Conventions
+ *   - every function returns 0 on success or a negative PMX_E* code and never
+ *     aborts; pmx_last_error() gives the message (thread-local);
+ *   - arrays are plain host pointers owned by the caller; complex arrays are
+ *     interleaved (re, im) doubles; lengths are given explicitly;
+ *   - a pmx_ctx owns one HIP device's buffers and stream and is used from
+ *     one host thread at a time; calls are synchronous on return;
+ *   - output layouts equal the reference's file layouts, so the CLI writes
+ *     them verbatim.
+ */
+#ifndef PMX_H
+#define PMX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PMX_OK 0
+#define PMX_EINVAL (-1)   /* bad argument or inconsistent sizes            */
+#define PMX_EHIP (-2)     /* HIP runtime error                               */
+#define PMX_ENOMEM (-3)   /* device allocation failed                        */
+#define PMX_ECONV (-4)    /* eigensolver did not converge                    */
+#define PMX_ESTATE (-5)   /* call order (no system set, ...)                 */
+#define PMX_EIO (-6)      /* file could not be read or written               */
+#define PMX_EBANDS (-7)   /* no sign change of E within -nband (see pmx_chi) */
+
+/* precision of the chi path (the eigensolve is FP64 in every mode) */
+#define PMX_PREC_FP64 0    /* FP64 throughout                                */
+#define PMX_PREC_FP32 1    /* rho + rank update FP32 MFMA, FP64 k-reduction  */
+#define PMX_PREC_BF16X3 2  /* split-BF16 (hi+lo, 3 products) MFMA, FP32 acc  */
+#define PMX_PREC_DIRECT 3  /* FP64, per-transition windowed sum (no binning) */
+
+typedef struct pmx_ctx pmx_ctx;
+
+/* Everything HamiltonianEPM reads (src/pmx_epm.cpp:56-153) and chi_tensor
+ * reads besides the k/q sets (src/pmx_chi.cpp:319-713). */
+typedef struct {
+    int npw;               /* NPW                                            */
+    const double *G;       /* [npw][3] G vectors in 2pi/a, generate_gpt order */
+    int neps;              /* NEPS local-field G (the first neps of G)       */
+    int natom;             /* NATOM                                          */
+    const double *tau;     /* [natom][3] atomic basis in a                   */
+    int nv;                /* number of form-factor shells                   */
+    const double *g2;      /* [nv] |G|^2 keys (2pi/a)^2                      */
+    const double *vg;      /* [natom][nv] form factors, Ry                   */
+    double g2max;          /* g2[nv-1] + 1e-2 (init_mater)                   */
+    int n_us;              /* symmetric SOC map, ascending keys              */
+    const double *us_key;
+    const double *us_val;
+    int n_ua;              /* antisymmetric SOC map, ascending keys          */
+    const double *ua_key;
+    const double *ua_val;
+    int use_uniform_us;    /* -use_uniform_us_so                             */
+    double uniform_us;     /* -uniform_us_so_val, Ry                         */
+    double a;              /* lattice constant, Angstrom                     */
+    double bz_volume;      /* lattice::bz_volume, (2pi/a)^3                  */
+} pmx_system;
+
+/* Inputs of chi_tensor (src/pmx_chi.cpp:319-713). */
+typedef struct {
+    int nk;                /* NKPT                                           */
+    const double *K;       /* [nk][3] k-points, 2pi/a                        */
+    const double *KW;      /* [nk] weights (already divided by NKPT0)        */
+    int nq;                /* NQ                                             */
+    const double *Q;       /* [nq][3] q-vectors, 2pi/a                       */
+    int nband;             /* NBAND (spinor count)                           */
+    double offset;         /* env::energyoffset, eV                          */
+    int delta;             /* 0 Gaussian, 1 Lorentzian, 2 triangle, 3 rect   */
+    double epsilon;        /* EPSILON, eV                                    */
+    int nfreq;             /* NFREQ; w_f = f * dfreq                         */
+    double dfreq;          /* eV                                             */
+    int kk;                /* apply kramerskronigtransform                   */
+    int precision;         /* PMX_PREC_*                                     */
+} pmx_chi_params;
+
+/* ---------------------------------------------------------------- context */
+int pmx_create(int device, pmx_ctx **out);
+void pmx_destroy(pmx_ctx *ctx);
+const char *pmx_last_error(void);
+/* library version string */
+const char *pmx_version(void);
+
+/* Uploads the system, builds the plane-wave index map loci on the device
+ * (replaces generate_loci, src/pmx_basis.cpp:142-179) and the k-independent
+ * form-factor / SOC pair tables used by H(k) (HamiltonianEPM,
+ * src/pmx_epm.cpp:56-153; pseudopotential, :171-203; get_soc_form_factor,
+ * :22-50). */
+int pmx_set_system(pmx_ctx *ctx, const pmx_system *sys);
+
+/* Replaces only the form factors vg [natom][nv] (Ry) of the current system. */
+int pmx_set_form_factors(pmx_ctx *ctx, const double *vg);
+
+/* Copies the device loci table [neps][npw] (index or -1). */
+int pmx_get_loci(pmx_ctx *ctx, int *loci);
+
+/* setRefEnergy (src/pmx_epm.cpp:393-428): offset = (e[nv-1] + e[nv]) / 2 at
+ * Kref, or 0 when nvalence is out of range. */
+int pmx_ref_energy(pmx_ctx *ctx, const double *Kref, int nvalence, double *offset);
+
+/* The band loop of empiricalpseudopotentialmethod (src/pmx_epm.cpp:225-248):
+ * eband[q][n] = e_n(Q_q) - offset for n < nband, 0 for n >= 2*npw. */
+int pmx_bands(pmx_ctx *ctx, int nq, const double *Q, int nband, double offset, double *eband);
+
+/* Batched FP64 Hermitian eigensolve of H(k) (HamiltonianEPM +
+ * SelfAdjointEigenSolver, src/pmx_epm.cpp:232; src/pmx_chi.cpp:388, 476):
+ * lowest nband spinor eigenvalues evals [nk][nband] and, when evecs is not
+ * NULL, their eigenvectors evecs [nk][nband][2*npw] complex, spinor rows
+ * interleaved (2p = up, 2p+1 = down). */
+int pmx_eig_batch(pmx_ctx *ctx, int nk, const double *K, int nband, double *evals, double *evecs);
+
+/* chi_tensor (src/pmx_chi.cpp:319-713) including the optional KK pass.
+ * ReX and ImX each hold 9*nq*neps*(neps+1)/2*nfreq doubles in the order
+ * [i][j][q][m][n<=m][f] of export_Xij (src/pmx_io.cpp:720-741). */
+int pmx_chi(pmx_ctx *ctx, const pmx_chi_params *p, double *ReX, double *ImX);
+
+/* kramerskronigtransform (src/pmx_chi.cpp:272-309) applied in place to nel
+ * contiguous elements of nfreq points each (w_f = f * dfreq). */
+int pmx_kk(pmx_ctx *ctx, int nel, int nfreq, double dfreq, double *ReX, double *ImX);
+
+/* Local-field inversion (new stage; the reference only has it in MATLAB,
+ * include/matlab_wrapper/+picomax/+eps/macroscopic.m:23-38, 87-89):
+ * eps_mn = delta_mn + chi^LL_mn (i = j = 0 block, lower triangle + Hermitian
+ * fill), per (q, f): epsM = 1 / [eps^-1]_00, elf = -Im(1/epsM).
+ * ReX/ImX as returned by pmx_chi; epsM [nq][nfreq] complex, elf [nq][nfreq];
+ * epsinv [nq][nfreq][neps][neps] complex may be NULL. */
+int pmx_epsinv(pmx_ctx *ctx, int nq, int neps, int nfreq, const double *ReX, const double *ImX,
+               double *epsM, double *elf, double *epsinv);
+
+/* Hellmann-Feynman adjoint of the band energies (no reference counterpart):
+ * given upstream gradients gE [nk][nband] of eband (band energies before the
+ * offset), returns gvg [natom][nv] = sum_k,n gE[k][n] dE_n(k)/dvg. */
+int pmx_grad_bands(pmx_ctx *ctx, int nk, const double *K, int nband, const double *gE, double *gvg);
+
+/* --------------------------------------------------------- host set-up
+ * The reference's CLI set-up (inputParser src/pmx.hpp:190-245, inputProcess
+ * src/pmx_io.cpp:423-503, init_crystal :108-310, init_mater :315-417,
+ * generate_gpt src/pmx_basis.cpp:29-81, import_kpt :200-245) as a parsed,
+ * opaque input object.  Host only; needs no GPU. */
+typedef struct pmx_input pmx_input;
+
+int pmx_input_parse(int argc, const char *const *argv, pmx_input **out);
+void pmx_input_free(pmx_input *in);
+/* pointers into the input's own storage, valid until pmx_input_free */
+int pmx_input_system(const pmx_input *in, pmx_system *out);
+
+typedef struct {
+    const char *sw;        /* -switch value ("" if absent)                   */
+    const char *outputfile;
+    const char *wdir;
+    int debug;
+    int nband, nvalence, neps, kk, delta, nfreq, npw, natom, nq, nk, kpoint;
+    double epsilon, dfreq, gcut, encut, a, bz_volume;
+    double refpoint[3];
+    const double *Q;       /* [nq][3]                                        */
+    const double *K;       /* [nk][3] (NULL unless a k-point file was read)  */
+    const double *KW;      /* [nk]                                           */
+} pmx_run_info;
+
+int pmx_input_info(const pmx_input *in, pmx_run_info *out);
+
+/* generate_gpt on its own: up to max_out G vectors sorted like the reference;
+ * returns NPW (>= 0) or an error code. */
+int pmx_generate_gpt(const double *recip, int dim, double gcut, int max_out, double *G_out);
+
+/* The CLI body after parsing: runs -switch band | epsij (epsLL: nothing, as
+ * the reference's main) on ctx, prints the reference's progress lines to
+ * stdout and writes ${wdir}/${outputfile}.dat (+ .gpt for epsij).
+ * precision selects the chi path (PMX_PREC_*). */
+int pmx_run(pmx_ctx *ctx, const pmx_input *in, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

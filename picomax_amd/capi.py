@@ -1,0 +1,260 @@
+"""ctypes binding of libpmx (include/pmx.h).
+
+`lib()` loads picomax_amd/lib/libpmx.so and fails loudly when it is missing:
+there is no CPU fallback for any compute entry point.  `Context` wraps a
+pmx_ctx on one GPU; `Input` wraps a parsed command line.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libpmx.so")
+CLI_PATH = os.path.join(PKG, "bin", "picomax")
+
+PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "direct": 3}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+# every symbol include/pmx.h declares
+EXPORTS = ["pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_set_system",
+           "pmx_set_form_factors", "pmx_get_loci", "pmx_ref_energy", "pmx_bands", "pmx_eig_batch",
+           "pmx_chi", "pmx_kk", "pmx_epsinv", "pmx_grad_bands", "pmx_input_parse", "pmx_input_free",
+           "pmx_input_system", "pmx_input_info", "pmx_generate_gpt", "pmx_run"]
+
+
+class PmxSystem(ctypes.Structure):
+    _fields_ = [("npw", ctypes.c_int), ("G", _dp), ("neps", ctypes.c_int),
+                ("natom", ctypes.c_int), ("tau", _dp),
+                ("nv", ctypes.c_int), ("g2", _dp), ("vg", _dp), ("g2max", ctypes.c_double),
+                ("n_us", ctypes.c_int), ("us_key", _dp), ("us_val", _dp),
+                ("n_ua", ctypes.c_int), ("ua_key", _dp), ("ua_val", _dp),
+                ("use_uniform_us", ctypes.c_int), ("uniform_us", ctypes.c_double),
+                ("a", ctypes.c_double), ("bz_volume", ctypes.c_double)]
+
+
+class PmxChiParams(ctypes.Structure):
+    _fields_ = [("nk", ctypes.c_int), ("K", _dp), ("KW", _dp), ("nq", ctypes.c_int), ("Q", _dp),
+                ("nband", ctypes.c_int), ("offset", ctypes.c_double), ("delta", ctypes.c_int),
+                ("epsilon", ctypes.c_double), ("nfreq", ctypes.c_int), ("dfreq", ctypes.c_double),
+                ("kk", ctypes.c_int), ("precision", ctypes.c_int)]
+
+
+class PmxRunInfo(ctypes.Structure):
+    _fields_ = [("sw", ctypes.c_char_p), ("outputfile", ctypes.c_char_p), ("wdir", ctypes.c_char_p),
+                ("debug", ctypes.c_int), ("nband", ctypes.c_int), ("nvalence", ctypes.c_int),
+                ("neps", ctypes.c_int), ("kk", ctypes.c_int), ("delta", ctypes.c_int),
+                ("nfreq", ctypes.c_int), ("npw", ctypes.c_int), ("natom", ctypes.c_int),
+                ("nq", ctypes.c_int), ("nk", ctypes.c_int), ("kpoint", ctypes.c_int),
+                ("epsilon", ctypes.c_double), ("dfreq", ctypes.c_double), ("gcut", ctypes.c_double),
+                ("encut", ctypes.c_double), ("a", ctypes.c_double), ("bz_volume", ctypes.c_double),
+                ("refpoint", ctypes.c_double * 3), ("Q", _dp), ("K", _dp), ("KW", _dp)]
+
+
+class PmxError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PmxError("libpmx.so is not built (%s); run `python -m picomax_amd.build`" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.pmx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    L.pmx_destroy.argtypes = [vp]
+    L.pmx_destroy.restype = None
+    L.pmx_last_error.restype = ctypes.c_char_p
+    L.pmx_version.restype = ctypes.c_char_p
+    L.pmx_set_system.argtypes = [vp, ctypes.POINTER(PmxSystem)]
+    L.pmx_set_form_factors.argtypes = [vp, _dp]
+    L.pmx_get_loci.argtypes = [vp, _ip]
+    L.pmx_ref_energy.argtypes = [vp, _dp, ctypes.c_int, _dp]
+    L.pmx_bands.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, _dp]
+    L.pmx_eig_batch.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
+    L.pmx_chi.argtypes = [vp, ctypes.POINTER(PmxChiParams), _dp, _dp]
+    L.pmx_kk.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp]
+    L.pmx_epsinv.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.pmx_grad_bands.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
+    L.pmx_input_parse.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(vp)]
+    L.pmx_input_free.argtypes = [vp]
+    L.pmx_input_free.restype = None
+    L.pmx_input_system.argtypes = [vp, ctypes.POINTER(PmxSystem)]
+    L.pmx_input_info.argtypes = [vp, ctypes.POINTER(PmxRunInfo)]
+    L.pmx_generate_gpt.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]
+    L.pmx_run.argtypes = [vp, vp, ctypes.c_int]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise PmxError("libpmx error %d: %s" % (rc, lib().pmx_last_error().decode()))
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    return a.reshape(shape) if shape is not None else a
+
+
+class Input:
+    """A parsed `picomax` command line (host only, no GPU needed)."""
+
+    def __init__(self, args):
+        args = ["picomax"] + [str(a) for a in args]
+        arr = (ctypes.c_char_p * len(args))(*[a.encode() for a in args])
+        h = ctypes.c_void_p()
+        _check(lib().pmx_input_parse(len(args), arr, ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pmx_input_free(self.h)
+            self.h = None
+
+    def system(self):
+        s = PmxSystem()
+        _check(lib().pmx_input_system(self.h, ctypes.byref(s)))
+        return s
+
+    def info(self):
+        r = PmxRunInfo()
+        _check(lib().pmx_input_info(self.h, ctypes.byref(r)))
+        return r
+
+    @property
+    def G(self):
+        s = self.system()
+        return np.ctypeslib.as_array(s.G, shape=(s.npw, 3)).copy()
+
+    @property
+    def Q(self):
+        r = self.info()
+        return np.ctypeslib.as_array(r.Q, shape=(r.nq, 3)).copy() if r.nq else np.zeros((0, 3))
+
+    @property
+    def K(self):
+        r = self.info()
+        if not r.nk:
+            return np.zeros((0, 3)), np.zeros(0)
+        return (np.ctypeslib.as_array(r.K, shape=(r.nk, 3)).copy(),
+                np.ctypeslib.as_array(r.KW, shape=(r.nk,)).copy())
+
+
+class Context:
+    """A pmx_ctx on one GPU (default device 0)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().pmx_create(int(device), ctypes.byref(h)))
+        self.h = h
+        self.sys = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pmx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_system(self, system, keep=None):
+        """system: a PmxSystem (e.g. Input.system()); keep: object owning its arrays."""
+        _check(lib().pmx_set_system(self.h, ctypes.byref(system)))
+        self.sys = (system, keep)
+        self.npw = system.npw
+        self.neps = system.neps
+        self.natom = system.natom
+        self.nv = system.nv
+
+    def set_form_factors(self, vg):
+        vg = _f64(vg, (self.natom, self.nv))
+        _check(lib().pmx_set_form_factors(self.h, _p(vg)))
+
+    def loci(self):
+        out = np.zeros((self.neps, self.npw), np.int32)
+        _check(lib().pmx_get_loci(self.h, out.ctypes.data_as(_ip)))
+        return out
+
+    def ref_energy(self, refpoint, nvalence):
+        out = np.zeros(1)
+        _check(lib().pmx_ref_energy(self.h, _p(_f64(refpoint, (3,))), int(nvalence), _p(out)))
+        return float(out[0])
+
+    def bands(self, Q, nband, offset=0.0):
+        Q = _f64(Q).reshape(-1, 3)
+        out = np.zeros((len(Q), nband))
+        _check(lib().pmx_bands(self.h, len(Q), _p(Q), int(nband), float(offset), _p(out)))
+        return out
+
+    def eig(self, K, nband, vectors=False):
+        K = _f64(K).reshape(-1, 3)
+        ev = np.zeros((len(K), nband))
+        vec = np.zeros((len(K), nband, 2 * self.npw), complex) if vectors else None
+        _check(lib().pmx_eig_batch(self.h, len(K), _p(K), int(nband), _p(ev),
+                                   _p(vec.view(np.float64)) if vectors else None))
+        return (ev, vec) if vectors else ev
+
+    def chi(self, K, KW, Q, nband, offset, delta, epsilon, nfreq, dfreq, kk=0, precision="fp64"):
+        K = _f64(K).reshape(-1, 3)
+        KW = _f64(KW).reshape(-1)
+        Q = _f64(Q).reshape(-1, 3)
+        p = PmxChiParams(len(K), _p(K), _p(KW), len(Q), _p(Q), int(nband), float(offset), int(delta),
+                         float(epsilon), int(nfreq), float(dfreq), int(kk),
+                         PREC[precision] if isinstance(precision, str) else int(precision))
+        ne2 = self.neps * (self.neps + 1) // 2
+        n = 9 * len(Q) * ne2 * nfreq
+        re = np.zeros(n)
+        im = np.zeros(n)
+        _check(lib().pmx_chi(self.h, ctypes.byref(p), _p(re), _p(im)))
+        return re, im
+
+    def kk(self, ReX, ImX, dfreq):
+        re = _f64(ReX).copy()
+        im = _f64(ImX).copy()
+        nfreq = re.shape[-1]
+        nel = re.size // nfreq
+        _check(lib().pmx_kk(self.h, nel, nfreq, float(dfreq), _p(re), _p(im)))
+        return re, im
+
+    def epsinv(self, ReX, ImX, nq, nfreq, full=False):
+        ReX = _f64(ReX)
+        ImX = _f64(ImX)
+        eM = np.zeros((nq, nfreq), complex)
+        elf = np.zeros((nq, nfreq))
+        inv = np.zeros((nq, nfreq, self.neps, self.neps), complex) if full else None
+        _check(lib().pmx_epsinv(self.h, int(nq), self.neps, int(nfreq), _p(ReX), _p(ImX),
+                                _p(eM.view(np.float64)), _p(elf),
+                                _p(inv.view(np.float64)) if full else None))
+        return (eM, elf, inv) if full else (eM, elf)
+
+    def grad_bands(self, K, nband, gE):
+        K = _f64(K).reshape(-1, 3)
+        gE = _f64(gE, (len(K), nband))
+        out = np.zeros((self.natom, self.nv))
+        _check(lib().pmx_grad_bands(self.h, len(K), _p(K), int(nband), _p(gE), _p(out)))
+        return out
+
+    def run(self, inp, precision="fp64"):
+        _check(lib().pmx_run(self.h, inp.h, PREC[precision]))
+
+
+def generate_gpt(recip, dim, gcut):
+    recip = _f64(recip, (3, 3))
+    cap = (2 * int(np.ceil(gcut)) + 1) ** 3
+    out = np.zeros((cap, 3))
+    n = lib().pmx_generate_gpt(_p(recip), int(dim), float(gcut), cap, _p(out))
+    if n < 0:
+        _check(n)
+    return out[:n].copy()

@@ -1,0 +1,880 @@
+/*
+ * pmx_chi.hip -- chi^{ij}_{GG'}(q, w) on gfx950: replaces chi_tensor
+ * (src/pmx_chi.cpp:319-713) and kramerskronigtransform (:272-309).
+ *
+ * Per q the reference does: eigensolves at k and k+q, the oscillator
+ * strengths rho (4 spin channels, :501-548), then for every stored element
+ * (i, j, m, n<=m) and transition t=(k,c,v) the windowed sum
+ *     Im chi(f) += kw reO_t delta(dE_t - w_f),  Re chi(f) -= kw imO_t delta(..)
+ * over f in [f0_t - wh, f0_t + wh], f0_t = int((dE_t - w0)/df) (:573-628),
+ * then scales by SCALEFACTOR/w^2 (:631-634) and optionally applies KK.
+ *
+ * Here the transitions are binned by f0 itself, so each transition lands in
+ * exactly the window the reference gives it, and delta is expanded around the
+ * bin centre c_b = w0 + (f0 + 1/2) df in powers of s = (dE - c_b)/(df/2):
+ *     delta(dE - w_f) = sum_n K_n(f0 - f) s^n,
+ *     K_n(D) = (df/2)^n delta^(n)((D + 1/2) df) / n!.
+ * The windowed sum becomes, for every bin b and order n < NT, a Hermitian
+ * rank-k update  M_{b,n} += sum_{t in b} w_t s_t^n X_t X_t^H  (X_t the
+ * 3*NEPS-vector of rho_{i,m}) followed by a banded convolution over bins
+ *     Z(f) = sum_{b=f-wh}^{f+wh} sum_n M_{b,n} K_n(b - f).
+ * NT is chosen from the ratio (df/2)/epsilon so that the truncated series is
+ * below 1e-11 (FP64) relative to the peak; Gaussian and Lorentzian only.  The
+ * triangle and rectangle models (not smooth) and PMX_PREC_DIRECT use the
+ * per-transition windowed sum.
+ *
+ * Spin: without SOC the reference's spinor sum over each Kramers-degenerate
+ * pair equals 2x the spatial sum (the cross terms of reO cancel over the pair,
+ * src/pmx_chi.cpp:602-604), so X_t = rho_spatial with weight 2 kw.  With SOC,
+ *     Im chi <- Re(M1),  Re chi <- -(Im M1 + Im M2)/2,
+ *     M1 = sum (A A^H - B B^H),  M2 = sum (C C^H - E E^H),
+ *     A = up+dn, B = ud+du, C = up-dn, E = ud-du,
+ * which equals the reference's reO / imO (including its cross-spin terms in
+ * reO only) summed per transition.
+ */
+#include "pmx_device.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+namespace pmx {
+
+namespace {
+
+__device__ __forceinline__ zd cmul(zd a, zd b) { return zd{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ zd cmulc(zd a, zd b) { return zd{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y}; } /* a conj(b) */
+__device__ __forceinline__ zd cconj(zd a) { return zd{a.x, -a.y}; }
+
+constexpr int MAX_NT = 20;
+constexpr int TILE = 16;
+
+/* ------------------------------------------------------------------
+ * transition tables: per k the conduction/valence counts (spinor), energies
+ * ------------------------------------------------------------------ */
+struct TransArgs {
+    int nk, nb_s, nband, spatial;
+    const double *evk, *evkq; /* [nk][nb_s] solve-basis eigenvalues */
+    double offset;
+    int *nc, *nv;             /* [nk] spinor counts */
+    int *bad;                 /* set when a count is undefined */
+};
+
+/* classification of chi_tensor (src/pmx_chi.cpp:391-397, 477-483) */
+__global__ void classify_kernel(TransArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nk) return;
+    int nc = -1, nv = -1;
+    for (int c = 0; c < a.nband; c++) {
+        int st = a.nband - 1 - c;
+        double e = a.evk[(size_t)k * a.nb_s + (a.spatial ? st / 2 : st)] - a.offset;
+        if (e <= 0) {
+            nc = c;
+            break;
+        }
+    }
+    for (int v = 0; v < a.nband; v++) {
+        double e = a.evkq[(size_t)k * a.nb_s + (a.spatial ? v / 2 : v)] - a.offset;
+        if (e > 0) {
+            nv = v;
+            break;
+        }
+    }
+    if (nc < 0 || nv < 0) atomicOr(a.bad, 1);
+    a.nc[k] = nc < 0 ? 0 : nc;
+    a.nv[k] = nv < 0 ? 0 : nv;
+}
+
+/* ------------------------------------------------------------------
+ * oscillator strengths rho (src/pmx_chi.cpp:501-548), one block per k.
+ * Item layout: X[(tbase + c*nvx + v) * nch + ch][a], a = 3 m + i, where
+ * nvx/ncx are the counts in the solve basis (spatial = spinor/2).
+ * ------------------------------------------------------------------ */
+struct RhoArgs {
+    int nk, N, npw, neps, nb_s, nband, spatial, nch;
+    const zd *vk, *vkq;       /* [nk][nb_s][N] */
+    const double *evk, *evkq; /* [nk][nb_s] */
+    double offset;
+    const int *nc, *nv, *tbase; /* spinor counts, item base per k (in transitions) */
+    const int *loci;          /* [neps][npw] */
+    const zd *wA;             /* [neps][3][npw] conj(u_mi).(G_p + Q/2 + G_m/2) */
+    const zd *uc;             /* [neps][3][3] conj(u_mi,d) */
+    const zd *wS;             /* [neps][3][npw] conj(u_mi).socCurrent(G_p, G_loci) (SOC only) */
+    const double *K;          /* [nk][3] */
+    const double *KW;         /* [nk] */
+    zd *X;                    /* items */
+    double *dE;               /* [transitions] */
+    double *wt;               /* [transitions] kw (x2 spatial) */
+};
+
+__global__ void __launch_bounds__(256) rho_kernel(RhoArgs a) {
+    const int k = blockIdx.x;
+    if (k >= a.nk) return;
+    const int ncs = a.spatial ? a.nc[k] / 2 : a.nc[k];
+    const int nvs = a.spatial ? a.nv[k] / 2 : a.nv[k];
+    const int npair = ncs * nvs;
+    if (npair == 0) return;
+    const size_t tb = a.tbase[k];
+    const double Kx = a.K[3 * k], Ky = a.K[3 * k + 1], Kz = a.K[3 * k + 2];
+    const int D = 3 * a.neps;
+    const zd *vk = a.vk + (size_t)k * a.nb_s * a.N;
+    const zd *vkq = a.vkq + (size_t)k * a.nb_s * a.N;
+    /* energies and weights of the transitions */
+    for (int t = threadIdx.x; t < npair; t += blockDim.x) {
+        const int c = t / nvs, v = t % nvs;
+        int cst = a.spatial ? (a.nband / 2 - 1 - c) : (a.nband - 1 - c);
+        double ec = a.evk[(size_t)k * a.nb_s + cst] - a.offset;
+        double ev = a.evkq[(size_t)k * a.nb_s + v] - a.offset;
+        a.dE[tb + t] = ec - ev;
+        a.wt[tb + t] = a.KW[k] * (a.spatial ? 2.0 : 1.0);
+    }
+    for (int job = threadIdx.x; job < npair * a.neps; job += blockDim.x) {
+        const int m = job / npair, t = job % npair;
+        const int c = t / nvs, v = t % nvs;
+        const int cst = a.spatial ? (a.nband / 2 - 1 - c) : (a.nband - 1 - c);
+        const zd *ck = vk + (size_t)cst * a.N;
+        const zd *cq = vkq + (size_t)v * a.N;
+        const int *lm = a.loci + (size_t)m * a.npw;
+        const zd *wA = a.wA + (size_t)m * 3 * a.npw;
+        if (a.spatial) {
+            zd S{0, 0}, R0{0, 0}, R1{0, 0}, R2{0, 0};
+            for (int p = 0; p < a.npw; p++) {
+                const int l = lm[p];
+                if (l < 0) continue;
+                zd pr = cmul(cconj(ck[p]), cq[l]);
+                S.x += pr.x; S.y += pr.y;
+                zd w0 = wA[p], w1 = wA[a.npw + p], w2 = wA[2 * a.npw + p];
+                zd t0 = cmul(pr, w0), t1 = cmul(pr, w1), t2 = cmul(pr, w2);
+                R0.x += t0.x; R0.y += t0.y;
+                R1.x += t1.x; R1.y += t1.y;
+                R2.x += t2.x; R2.y += t2.y;
+            }
+            zd R[3] = {R0, R1, R2};
+            zd *out = a.X + (tb + t) * (size_t)D;
+            for (int i = 0; i < 3; i++) {
+                const zd *u = a.uc + ((size_t)m * 3 + i) * 3;
+                zd bk{u[0].x * Kx + u[1].x * Ky + u[2].x * Kz, u[0].y * Kx + u[1].y * Ky + u[2].y * Kz};
+                zd r = R[i];
+                zd bs = cmul(bk, S);
+                out[3 * m + i] = zd{r.x + bs.x, r.y + bs.y};
+            }
+        } else {
+            /* spinor rows 2p (up), 2p+1 (down) */
+            const zd *wS = a.wS + (size_t)m * 3 * a.npw;
+            zd Su{0, 0}, Sd{0, 0};
+            zd Ru[3] = {{0, 0}, {0, 0}, {0, 0}}, Rd[3] = {{0, 0}, {0, 0}, {0, 0}};
+            zd Qud[3] = {{0, 0}, {0, 0}, {0, 0}}, Qdu[3] = {{0, 0}, {0, 0}, {0, 0}};
+            for (int p = 0; p < a.npw; p++) {
+                const int l = lm[p];
+                if (l < 0) continue;
+                zd cu = cconj(ck[2 * p]), cd = cconj(ck[2 * p + 1]);
+                zd qu = cq[2 * l], qd = cq[2 * l + 1];
+                zd puu = cmul(cu, qu), pdd = cmul(cd, qd), pud = cmul(cu, qd), pdu = cmul(cd, qu);
+                Su.x += puu.x; Su.y += puu.y;
+                Sd.x += pdd.x; Sd.y += pdd.y;
+                for (int i = 0; i < 3; i++) {
+                    zd w = wA[i * a.npw + p], s = wS[i * a.npw + p];
+                    zd t1 = cmul(puu, w), t2 = cmul(pdd, w), t3 = cmul(pud, s), t4 = cmul(pdu, s);
+                    Ru[i].x += t1.x; Ru[i].y += t1.y;
+                    Rd[i].x += t2.x; Rd[i].y += t2.y;
+                    Qud[i].x += t3.x; Qud[i].y += t3.y;
+                    Qdu[i].x += t4.x; Qdu[i].y += t4.y;
+                }
+            }
+            zd *out = a.X + (tb + t) * (size_t)4 * D;
+            for (int i = 0; i < 3; i++) {
+                const zd *u = a.uc + ((size_t)m * 3 + i) * 3;
+                zd bk{u[0].x * Kx + u[1].x * Ky + u[2].x * Kz, u[0].y * Kx + u[1].y * Ky + u[2].y * Kz};
+                zd bu = cmul(bk, Su), bd = cmul(bk, Sd);
+                zd up{Ru[i].x + bu.x, Ru[i].y + bu.y}, dn{Rd[i].x + bd.x, Rd[i].y + bd.y};
+                zd ud = Qud[i], du = Qdu[i];
+                out[0 * D + 3 * m + i] = zd{up.x + dn.x, up.y + dn.y}; /* A */
+                out[1 * D + 3 * m + i] = zd{ud.x + du.x, ud.y + du.y}; /* B */
+                out[2 * D + 3 * m + i] = zd{up.x - dn.x, up.y - dn.y}; /* C */
+                out[3 * D + 3 * m + i] = zd{ud.x - du.x, ud.y - du.y}; /* E */
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------
+ * binning: f0 = int((dE - w0)/df) exactly as the reference, bin = f0 + wh;
+ * transitions whose window misses [0, nfreq) are dropped.
+ * ------------------------------------------------------------------ */
+__global__ void bin_kernel(int nt, const double *dE, double w0, double df, int wh, int nbins, int *bin, int *hist) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    int f0 = int((dE[t] - w0) / df);
+    int b = f0 + wh;
+    if (b < 0 || b >= nbins) b = -1;
+    bin[t] = b;
+    if (b >= 0) atomicAdd(&hist[b], 1);
+}
+
+__global__ void scatter_kernel(int nt, const int *bin, int *cursor, int *perm) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    int b = bin[t];
+    if (b < 0) return;
+    int pos = atomicAdd(&cursor[b], 1);
+    perm[pos] = t;
+}
+
+/* ------------------------------------------------------------------
+ * moments: M[b][n][tile pair][16x16] += sum_{t in b} w s^n X_a conj(X_a')
+ * (SIMT FP64 reference kernel; one block per (tile pair, bin))
+ * Items: for SOC each transition carries channels ch (A, B, C, E): M1 takes
+ * A (+) and B (-), M2 takes C (+) and E (-).
+ * ------------------------------------------------------------------ */
+struct MomArgs {
+    int D, nch, ntile, npairs, nbins, NT;
+    const zd *X;
+    const double *dE, *wt;
+    const int *perm, *start; /* sorted transition ids; start[b] .. start[b+1] */
+    double w0, df;
+    int acc_index;           /* 0: M1, 1: M2 */
+    zd *M;                   /* [nbins][NT][npairs][256] */
+};
+
+__device__ __forceinline__ void tile_pair(int idx, int &ti, int &tj) {
+    /* idx over ti >= tj, row-major in ti */
+    int t = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+    while ((t + 1) * (t + 2) / 2 <= idx) t++;
+    while (t * (t + 1) / 2 > idx) t--;
+    ti = t;
+    tj = idx - t * (t + 1) / 2;
+}
+
+__global__ void __launch_bounds__(256) moment_simt_kernel(MomArgs a) {
+    const int pairi = blockIdx.x, b = blockIdx.y;
+    const int t0 = a.start[b], t1 = a.start[b + 1];
+    if (t0 == t1) return;
+    int ti, tj;
+    tile_pair(pairi, ti, tj);
+    const int r = threadIdx.x / TILE, cc = threadIdx.x % TILE;
+    const int ar = ti * TILE + r, ac = tj * TILE + cc;
+    __shared__ zd xa[32][TILE], xb[32][TILE];
+    __shared__ double ws[32], ss[32];
+    zd acc[MAX_NT];
+#pragma unroll
+    for (int n = 0; n < MAX_NT; n++) acc[n] = zd{0, 0};
+    const double h = 0.5 * a.df;
+    const double cb = a.w0 + (b - 0 + 0.5) * a.df; /* centre of bin b relative to the wh shift below */
+    (void)cb;
+    const int nsig = 2; /* channels per accumulator in SOC: (+, -) */
+    for (int base = t0; base < t1; base += 32 / (a.nch == 1 ? 1 : nsig)) {
+        /* stage up to 32 items: spatial = 32 transitions, SOC = 16 transitions x 2 channels */
+        const int per = (a.nch == 1) ? 32 : 16;
+        const int cnt = min(per, t1 - base);
+        for (int e = threadIdx.x; e < 32 * TILE * 2; e += blockDim.x) {
+            int item = e / (2 * TILE), which = (e / TILE) & 1, col = e % TILE;
+            int tl = (a.nch == 1) ? item : item / 2;
+            int ch = (a.nch == 1) ? 0 : (2 * a.acc_index + (item & 1));
+            zd v{0, 0};
+            if (tl < cnt) {
+                int t = a.perm[base + tl];
+                int aa = (which == 0 ? ti : tj) * TILE + col;
+                if (aa < a.D) v = a.X[((size_t)t * a.nch + ch) * a.D + aa];
+            }
+            if (which == 0) xa[item][col] = v; else xb[item][col] = v;
+        }
+        for (int e = threadIdx.x; e < 32; e += blockDim.x) {
+            int tl = (a.nch == 1) ? e : e / 2;
+            double w = 0.0, s = 0.0;
+            if (tl < cnt) {
+                int t = a.perm[base + tl];
+                w = a.wt[t];
+                if (a.nch != 1 && (e & 1)) w = -w;
+                int f0 = int((a.dE[t] - a.w0) / a.df);
+                double cen = a.w0 + (f0 + 0.5) * a.df;
+                s = (a.dE[t] - cen) / h;
+            }
+            ws[e] = w;
+            ss[e] = s;
+        }
+        __syncthreads();
+        const int nitems = (a.nch == 1) ? cnt : 2 * cnt;
+        for (int it = 0; it < nitems; it++) {
+            zd p = cmulc(xa[it][r], xb[it][cc]);
+            double w = ws[it], s = ss[it];
+            zd q{p.x * w, p.y * w};
+#pragma unroll
+            for (int n = 0; n < MAX_NT; n++) {
+                if (n < a.NT) {
+                    acc[n].x += q.x;
+                    acc[n].y += q.y;
+                    q.x *= s;
+                    q.y *= s;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (ar < a.D && ac < a.D) {
+        for (int n = 0; n < a.NT; n++) {
+            zd *dst = a.M + (((size_t)b * a.NT + n) * a.npairs + pairi) * 256 + threadIdx.x;
+            dst->x += acc[n].x;
+            dst->y += acc[n].y;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------
+ * direct windowed sum (PMX_PREC_DIRECT, triangle, rect): one block per
+ * (tile pair, bin); Z[f][pair][256] += sum_t w delta(dE_t - w_f) X X^H
+ * ------------------------------------------------------------------ */
+struct DirArgs {
+    int D, nch, npairs, nbins, wh, nfreq, delta, acc_index;
+    double eps, w0, df;
+    const zd *X;
+    const double *dE, *wt;
+    const int *perm, *start;
+    zd *Z; /* [nfreq][npairs][256] */
+};
+
+__device__ double delta_fn(int code, double eps, double x) {
+    const double PI = PMX_PI;
+    switch (code) {
+    case 0: return exp(-0.5 * x * x / (eps * eps)) / (eps * sqrt(2.0 * PI));
+    case 1: return eps / PI / (eps * eps + x * x);
+    case 2: return (fabs(x) < 0.5 * eps) ? 4.0 / eps * (0.5 - fabs(x) / eps) : 0.0;
+    case 3: return (fabs(x) < 0.5 * eps) ? 1.0 / eps : 0.0;
+    }
+    return 0.0;
+}
+
+__global__ void __launch_bounds__(256) direct_kernel(DirArgs a) {
+    const int pairi = blockIdx.x, b = blockIdx.y;
+    const int t0 = a.start[b], t1 = a.start[b + 1];
+    if (t0 == t1) return;
+    int ti, tj;
+    tile_pair(pairi, ti, tj);
+    const int r = threadIdx.x / TILE, cc = threadIdx.x % TILE;
+    const int ar = ti * TILE + r, ac = tj * TILE + cc;
+    const int f0 = b - a.wh;
+    const int fmin = max(0, f0 - a.wh), fmax = min(a.nfreq - 1, f0 + a.wh);
+    __shared__ zd xa[32][TILE], xb[32][TILE];
+    __shared__ double ws[32], de[32];
+    for (int f = fmin; f <= fmax; f++) {
+        zd acc{0, 0};
+        const double wf = a.w0 + a.df * f;
+        for (int base = t0; base < t1; base += (a.nch == 1 ? 32 : 16)) {
+            const int per = (a.nch == 1) ? 32 : 16;
+            const int cnt = min(per, t1 - base);
+            __syncthreads();
+            for (int e = threadIdx.x; e < 32 * TILE * 2; e += blockDim.x) {
+                int item = e / (2 * TILE), which = (e / TILE) & 1, col = e % TILE;
+                int tl = (a.nch == 1) ? item : item / 2;
+                int ch = (a.nch == 1) ? 0 : (2 * a.acc_index + (item & 1));
+                zd v{0, 0};
+                if (tl < cnt) {
+                    int t = a.perm[base + tl];
+                    int aa = (which == 0 ? ti : tj) * TILE + col;
+                    if (aa < a.D) v = a.X[((size_t)t * a.nch + ch) * a.D + aa];
+                }
+                if (which == 0) xa[item][col] = v; else xb[item][col] = v;
+            }
+            for (int e = threadIdx.x; e < 32; e += blockDim.x) {
+                int tl = (a.nch == 1) ? e : e / 2;
+                double w = 0.0, d = 0.0;
+                if (tl < cnt) {
+                    int t = a.perm[base + tl];
+                    w = a.wt[t];
+                    if (a.nch != 1 && (e & 1)) w = -w;
+                    d = a.dE[t];
+                }
+                ws[e] = w * delta_fn(a.delta, a.eps, d - wf);
+                de[e] = d;
+            }
+            __syncthreads();
+            const int nitems = (a.nch == 1) ? cnt : 2 * cnt;
+            for (int it = 0; it < nitems; it++) {
+                zd p = cmulc(xa[it][r], xb[it][cc]);
+                acc.x += p.x * ws[it];
+                acc.y += p.y * ws[it];
+            }
+        }
+        if (ar < a.D && ac < a.D) {
+            /* bins f0 - wh .. f0 + wh all add into frequency f: atomics */
+            zd *dst = a.Z + ((size_t)f * a.npairs + pairi) * 256 + threadIdx.x;
+            atomicAdd(&dst->x, acc.x);
+            atomicAdd(&dst->y, acc.y);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------
+ * convolution over bins: Z(f) = sum_{D=-wh..wh} sum_n M[f+D+wh][n] K[D+wh][n]
+ * one block per (tile pair, block of 16 frequencies)
+ * ------------------------------------------------------------------ */
+struct ConvArgs {
+    int npairs, nbins, NT, wh, nfreq;
+    const zd *M;
+    const double *Kt; /* [2wh+1][NT] */
+    zd *Z;            /* [nfreq][npairs][256] */
+};
+
+constexpr int CONV_F = 16;
+
+__global__ void __launch_bounds__(256) conv_kernel(ConvArgs a) {
+    const int pairi = blockIdx.x, fb = blockIdx.y * CONV_F;
+    zd acc[CONV_F];
+#pragma unroll
+    for (int j = 0; j < CONV_F; j++) acc[j] = zd{0, 0};
+    const int W = 2 * a.wh + 1;
+    /* bins touching f in [fb, fb+CONV_F): b in [fb, fb + CONV_F - 1 + 2wh] */
+    const int bmax = min(a.nbins - 1, fb + CONV_F - 1 + 2 * a.wh);
+    for (int b = fb; b <= bmax; b++) {
+        for (int n = 0; n < a.NT; n++) {
+            zd m = a.M[(((size_t)b * a.NT + n) * a.npairs + pairi) * 256 + threadIdx.x];
+            if (m.x == 0.0 && m.y == 0.0) continue;
+#pragma unroll
+            for (int j = 0; j < CONV_F; j++) {
+                const int f = fb + j;
+                const int Dd = b - a.wh - f; /* f0 - f */
+                if (f < a.nfreq && Dd >= -a.wh && Dd <= a.wh) {
+                    double kv = a.Kt[(Dd + a.wh) * a.NT + n];
+                    acc[j].x += m.x * kv;
+                    acc[j].y += m.y * kv;
+                }
+            }
+        }
+    }
+    (void)W;
+#pragma unroll
+    for (int j = 0; j < CONV_F; j++) {
+        const int f = fb + j;
+        if (f < a.nfreq) {
+            zd *dst = a.Z + ((size_t)f * a.npairs + pairi) * 256 + threadIdx.x;
+            dst->x += acc[j].x;
+            dst->y += acc[j].y;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------
+ * packing: Z tiles -> ReX/ImX [i][j][q][m][n<=m][f] with SCALEFACTOR/w^2
+ * ------------------------------------------------------------------ */
+struct PackArgs {
+    int D, neps, npairs, nfreq, nq, q, soc;
+    double SF, dfreq, w0;
+    const zd *Z1, *Z2;
+    double *ReX, *ImX;
+};
+
+__device__ __forceinline__ zd zfetch(const zd *Z, int f, int npairs, int a, int b) {
+    /* Hermitian: element (a, b) of the full matrix from lower tiles */
+    bool cj = false;
+    if (a < b) {
+        int t = a; a = b; b = t;
+        cj = true;
+    }
+    const int ti = a / TILE, tj = b / TILE;
+    int pairi = ti * (ti + 1) / 2 + tj;
+    zd v = Z[((size_t)f * npairs + pairi) * 256 + (a % TILE) * TILE + (b % TILE)];
+    return cj ? cconj(v) : v;
+}
+
+__global__ void pack_kernel(PackArgs a) {
+    const int ne2 = a.neps * (a.neps + 1) / 2;
+    const size_t total = (size_t)9 * ne2 * a.nfreq;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int f = idx % a.nfreq;
+        const size_t el = idx / a.nfreq;
+        const int ij = el / ne2, e2 = el % ne2;
+        const int i = ij / 3, j = ij % 3;
+        int m = (int)((sqrt(8.0 * e2 + 1.0) - 1.0) * 0.5);
+        while ((m + 1) * (m + 2) / 2 <= e2) m++;
+        while (m * (m + 1) / 2 > e2) m--;
+        const int n = e2 - m * (m + 1) / 2;
+        const int ra = 3 * m + i, rb = 3 * n + j;
+        zd z1 = zfetch(a.Z1, f, a.npairs, ra, rb);
+        double imv = z1.x;
+        double rev = a.soc ? -0.5 * (z1.y + zfetch(a.Z2, f, a.npairs, ra, rb).y) : -z1.y;
+        const double w = a.w0 + a.dfreq * f;
+        const double inv = (f == 0) ? 0.0 : 1.0 / (w * w);
+        const size_t dst = (((size_t)ij * a.nq + a.q) * ne2 + e2) * a.nfreq + f;
+        a.ImX[dst] = a.SF * (imv * inv);
+        a.ReX[dst] = a.SF * (rev * inv);
+    }
+}
+
+/* kramerskronigtransform (src/pmx_chi.cpp:272-309), one thread per (element, f):
+ * ReX += ds sum_{s!=f} w_s ImX_s/(w_s^2 - w_f^2), ImX += ds sum_{s!=f} w_s ReX0_s/(..) */
+__global__ void kk_kernel(int nel, int nfreq, const double *Kk, const double *Re0, const double *Im0, double *Re,
+                          double *Im) {
+    const size_t total = (size_t)nel * nfreq;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int f = idx % nfreq;
+        const size_t el = idx / nfreq;
+        const double *r0 = Re0 + el * nfreq, *i0 = Im0 + el * nfreq;
+        double ar = 0.0, ai = 0.0;
+        for (int s = 0; s < nfreq; s++) {
+            double kv = Kk[(size_t)s * nfreq + f];
+            ar += kv * i0[s];
+            ai += kv * r0[s];
+        }
+        Re[idx] = r0[f] + ar;
+        Im[idx] = i0[f] + ai;
+    }
+}
+
+__global__ void kkmat_kernel(int nfreq, double dfreq, double *Kk) {
+    const size_t total = (size_t)nfreq * nfreq;
+    const double ds = dfreq * 2.0 / PMX_PI;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int s = idx / nfreq, f = idx % nfreq;
+        const double ws = dfreq * s, wf = dfreq * f;
+        Kk[idx] = (s == f) ? 0.0 : ds * (ws / (ws * ws - wf * wf));
+    }
+}
+
+/* host: Taylor kernel table K_n(D) = h^n delta^(n)((D+1/2) df)/n! */
+void taylor_table(int delta, double eps, double df, int wh, int NT, std::vector<double> &Kt) {
+    const int W = 2 * wh + 1;
+    Kt.assign((size_t)W * NT, 0.0);
+    const double h = 0.5 * df;
+    for (int Dd = -wh; Dd <= wh; Dd++) {
+        const double x = (Dd + 0.5) * df; /* c_b - w_f */
+        double *row = &Kt[(size_t)(Dd + wh) * NT];
+        if (delta == 1) {
+            /* (1/pi) Im[(-1)^n / (x - i eps)^(n+1)] h^n */
+            std::complex<double> z(x, -eps), inv = 1.0 / z, pw = inv;
+            double hn = 1.0, sg = 1.0;
+            for (int n = 0; n < NT; n++) {
+                row[n] = sg * hn * pw.imag() / PMX_PI;
+                pw *= inv;
+                hn *= h;
+                sg = -sg;
+            }
+        } else {
+            /* a_{n+1} = -(x a_n + a_{n-1}) / (eps^2 (n+1)), a_0 = g(x) */
+            double am1 = 0.0, a0 = std::exp(-0.5 * x * x / (eps * eps)) / (eps * std::sqrt(2.0 * PMX_PI));
+            double hn = 1.0;
+            for (int n = 0; n < NT; n++) {
+                row[n] = hn * a0;
+                double a1 = -(x * a0 + am1) / (eps * eps * (n + 1));
+                am1 = a0;
+                a0 = a1;
+                hn *= h;
+            }
+        }
+    }
+}
+
+int window_half(int delta, double eps, double df) {
+    int wg = std::max(1, int(6.0 * eps / df));
+    int wl = std::max(1, int(50.0 * eps / df));
+    return (delta == 1) ? wl : wg;
+}
+
+/* Taylor order for a truncation below tol relative to the peak: with
+ * r = (df/2)/epsilon the Lorentzian terms fall like r^n (its poles sit at
+ * distance >= epsilon from every expansion point), the Gaussian ones like
+ * r^n / sqrt(n!) (Hermite-function bound). */
+int taylor_order(int delta, double eps, double df, double tol) {
+    const double r = 0.5 * df / eps;
+    if (r >= 0.6) return -1;
+    double term = 1.0;
+    for (int n = 1; n <= MAX_NT; n++) {
+        term *= (delta == 0) ? r / std::sqrt((double)n) : r;
+        if (term < tol) return n;
+    }
+    return -1;
+}
+
+template <class T> int upload(hipStream_t s, DBuf<T> &b, const std::vector<T> &h) {
+    int rc = b.ensure(h.size());
+    if (rc) return rc;
+    if (!h.empty()) PMX_CHECK_HIP(hipMemcpyAsync(b.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+    return 0;
+}
+
+} // namespace
+
+struct ChiWork {
+    DBuf<double> Kd, KQd, KWd, evk, evkq, dEd, wtd, Ktd, Kkd, Re0, Im0, ReD, ImD;
+    DBuf<zd> vk, vkq, Xd, M1, M2, Z1, Z2, wAd, ucd, wSd;
+    DBuf<int> nc, nv, tbase, bind, hist, start, cursor, perm, bad;
+};
+
+} // namespace pmx
+
+using namespace pmx;
+
+extern "C" int pmx_chi(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double *ImX) {
+    if (!c || !p || !ReX || !ImX) return set_error(PMX_EINVAL, "null pointer");
+    if (!c->has_system) return set_error(PMX_ESTATE, "no system set");
+    if (p->nfreq <= 0 || !(p->dfreq > 0)) return set_error(PMX_EINVAL, "nfreq > 0 and dfreq > 0 required");
+    if (p->delta < 0 || p->delta > 3) return set_error(PMX_EINVAL, "delta must be 0..3");
+    if (p->nband <= 0 || p->nband > 2 * c->npw) return set_error(PMX_EINVAL, "nband must be in [1, 2 npw]");
+    if (p->nk < 0 || p->nq < 0) return set_error(PMX_EINVAL, "negative sizes");
+    PMX_CHECK_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int neps = c->neps, npw = c->npw, D = 3 * neps;
+    const int ne2 = neps * (neps + 1) / 2;
+    const size_t per_q = (size_t)9 * ne2 * p->nfreq;
+    const bool spatial = !c->soc && (p->nband % 2 == 0);
+    if (!c->soc && !spatial) return set_error(PMX_EINVAL, "odd -nband without SOC splits a Kramers pair; use an even -nband");
+    const int N = basis_size(c);
+    const int nb_s = spatial ? p->nband / 2 : p->nband;
+    const int nch = spatial ? 1 : 4;
+    const double SF = 4.0 * c->bz * std::pow(PMX_PI, 5) * std::pow(PMX_HBAR, 4) * std::pow(PMX_E, 2) /
+                      (std::pow(PMX_EV, 3) * std::pow(PMX_EMASS, 2) * std::pow(c->a * PMX_ANGSTROM, 5));
+    const double w0 = 0.0, df = p->dfreq;
+    const int wh = window_half(p->delta, p->epsilon, df);
+    const int nbins = p->nfreq + 2 * wh;
+    int NT = -1;
+    bool direct = p->precision == PMX_PREC_DIRECT || p->delta >= 2;
+    if (!direct) {
+        NT = taylor_order(p->delta, p->epsilon, df, 1e-11);
+        if (NT < 0 || NT > MAX_NT) direct = true;
+    }
+    const int ntile = (D + TILE - 1) / TILE;
+    const int npairs = ntile * (ntile + 1) / 2;
+
+    ChiWork w;
+    int rc;
+    if (p->nk == 0) {
+        std::fill(ReX, ReX + per_q * p->nq, 0.0);
+        std::fill(ImX, ImX + per_q * p->nq, 0.0);
+        return PMX_OK;
+    }
+    /* ---- phase A: eigenpairs at k (src/pmx_chi.cpp:379-413) */
+    std::vector<double> Kh(p->K, p->K + 3 * p->nk);
+    std::vector<double> KWh(p->KW, p->KW + p->nk);
+    if ((rc = upload(st, w.Kd, Kh)) || (rc = upload(st, w.KWd, KWh))) return rc;
+    if ((rc = w.evk.ensure((size_t)p->nk * nb_s)) || (rc = w.vk.ensure((size_t)p->nk * nb_s * N))) return rc;
+    if ((rc = eig_device(c, p->nk, w.Kd.p, nb_s, nb_s, w.evk.p, w.vk.p))) return rc;
+    if ((rc = w.evkq.ensure((size_t)p->nk * nb_s)) || (rc = w.vkq.ensure((size_t)p->nk * nb_s * N))) return rc;
+    if ((rc = w.nc.ensure(p->nk)) || (rc = w.nv.ensure(p->nk)) || (rc = w.tbase.ensure(p->nk + 1)) ||
+        (rc = w.bad.ensure(1)))
+        return rc;
+    if ((rc = w.M1.ensure((size_t)nbins * std::max(NT, 1) * npairs * 256))) return rc;
+    if ((rc = w.Z1.ensure((size_t)p->nfreq * npairs * 256))) return rc;
+    if (!spatial) {
+        if ((rc = w.M2.ensure((size_t)nbins * std::max(NT, 1) * npairs * 256))) return rc;
+        if ((rc = w.Z2.ensure((size_t)p->nfreq * npairs * 256))) return rc;
+    }
+    std::vector<double> Kt;
+    if (!direct) {
+        taylor_table(p->delta, p->epsilon, df, wh, NT, Kt);
+        if ((rc = upload(st, w.Ktd, Kt))) return rc;
+    }
+    if (p->kk) {
+        if ((rc = w.Kkd.ensure((size_t)p->nfreq * p->nfreq))) return rc;
+        hipLaunchKernelGGL(kkmat_kernel, dim3(256), dim3(256), 0, st, p->nfreq, df, w.Kkd.p);
+        if ((rc = w.Re0.ensure(per_q)) || (rc = w.Im0.ensure(per_q))) return rc;
+    }
+    if ((rc = w.ReD.ensure(per_q)) || (rc = w.ImD.ensure(per_q))) return rc;
+
+    for (int q = 0; q < p->nq; q++) {
+        const double Qx = p->Q[3 * q], Qy = p->Q[3 * q + 1], Qz = p->Q[3 * q + 2];
+        /* ---- per-q tables: uvec_LT (src/pmx_basis.cpp:830-874), weights */
+        std::vector<zd> uc((size_t)neps * 9), wA((size_t)neps * 3 * npw), wS;
+        if (!spatial) wS.assign((size_t)neps * 3 * npw, zd{0, 0});
+        for (int m = 0; m < neps; m++) {
+            const double *Gm = &c->G[3 * m];
+            double vx = Qx + Gm[0], vy = Qy + Gm[1], vz = Qz + Gm[2];
+            double nv = std::sqrt(vx * vx + vy * vy + vz * vz);
+            double L[3] = {0, 0, 1};
+            if (nv != 0) { L[0] = vx / nv; L[1] = vy / nv; L[2] = vz / nv; }
+            double T[3];
+            if (vy != 0) { T[0] = -vy; T[1] = vx; T[2] = 0; }
+            else if (vz != 0) { T[0] = vz; T[1] = 0; T[2] = -vx; }
+            else { T[0] = 0; T[1] = 1; T[2] = 0; }
+            double nt = std::sqrt(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]);
+            for (double &x : T) x /= nt;
+            double lt = L[0] * T[0] + L[1] * T[1] + L[2] * T[2];
+            for (int d = 0; d < 3; d++) T[d] -= lt * L[d];
+            nt = std::sqrt(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]);
+            for (double &x : T) x /= nt;
+            double P[3] = {L[1] * T[2] - L[2] * T[1], L[2] * T[0] - L[0] * T[2], L[0] * T[1] - L[1] * T[0]};
+            double np_ = std::sqrt(P[0] * P[0] + P[1] * P[1] + P[2] * P[2]);
+            for (double &x : P) x /= np_;
+            const double r2 = std::sqrt(2.0);
+            std::complex<double> u[3][3];
+            for (int d = 0; d < 3; d++) {
+                u[0][d] = L[d];
+                u[1][d] = std::complex<double>(T[d], P[d]) / r2;
+                u[2][d] = std::complex<double>(T[d], -P[d]) / r2;
+            }
+            for (int i = 0; i < 3; i++)
+                for (int d = 0; d < 3; d++) uc[((size_t)m * 3 + i) * 3 + d] = zd{u[i][d].real(), -u[i][d].imag()};
+            for (int pp = 0; pp < npw; pp++) {
+                const double *Gp = &c->G[3 * pp];
+                double o[3] = {Gp[0] + 0.5 * Qx + 0.5 * Gm[0], Gp[1] + 0.5 * Qy + 0.5 * Gm[1], Gp[2] + 0.5 * Qz + 0.5 * Gm[2]};
+                for (int i = 0; i < 3; i++) {
+                    std::complex<double> acc = 0.0;
+                    for (int d = 0; d < 3; d++) acc += std::conj(u[i][d]) * o[d];
+                    wA[((size_t)m * 3 + i) * npw + pp] = zd{acc.real(), acc.imag()};
+                }
+                if (!spatial) {
+                    int l = c->loci_h[(size_t)m * npw + pp];
+                    if (l < 0) continue;
+                    /* socCurrent(G_p, G_l) with the map lambda_S (src/pmx_chi.cpp:203-266) */
+                    const double *Gl = &c->G[3 * l];
+                    double qx = Gl[0] - Gp[0], qy = Gl[1] - Gp[1], qz = Gl[2] - Gp[2];
+                    std::complex<double> vso = c->vso_chi_at(qx, qy, qz);
+                    std::complex<double> I(0, 1);
+                    std::complex<double> sv[3] = {I * vso * (qy + I * qz), I * vso * (qz - qx), I * vso * (-I * qx - qy)};
+                    for (int i = 0; i < 3; i++) {
+                        std::complex<double> acc = 0.0;
+                        for (int d = 0; d < 3; d++) acc += std::conj(u[i][d]) * sv[d];
+                        wS[((size_t)m * 3 + i) * npw + pp] = zd{acc.real(), acc.imag()};
+                    }
+                }
+            }
+        }
+        if ((rc = upload(st, w.ucd, uc)) || (rc = upload(st, w.wAd, wA))) return rc;
+        if (!spatial && (rc = upload(st, w.wSd, wS))) return rc;
+
+        /* ---- phase B: eigenpairs at k+q */
+        std::vector<double> KQ(3 * (size_t)p->nk);
+        for (int k = 0; k < p->nk; k++) {
+            KQ[3 * k] = Kh[3 * k] + Qx;
+            KQ[3 * k + 1] = Kh[3 * k + 1] + Qy;
+            KQ[3 * k + 2] = Kh[3 * k + 2] + Qz;
+        }
+        if ((rc = upload(st, w.KQd, KQ))) return rc;
+        if ((rc = eig_device(c, p->nk, w.KQd.p, nb_s, nb_s, w.evkq.p, w.vkq.p))) return rc;
+
+        /* ---- classification + transition offsets */
+        PMX_CHECK_HIP(hipMemsetAsync(w.bad.p, 0, sizeof(int), st));
+        TransArgs ta{p->nk, nb_s, p->nband, spatial ? 1 : 0, w.evk.p, w.evkq.p, p->offset, w.nc.p, w.nv.p, w.bad.p};
+        hipLaunchKernelGGL(classify_kernel, dim3((p->nk + 255) / 256), dim3(256), 0, st, ta);
+        std::vector<int> nch_h(p->nk), nvh(p->nk), tb(p->nk + 1);
+        int bad = 0;
+        PMX_CHECK_HIP(hipMemcpyAsync(nch_h.data(), w.nc.p, sizeof(int) * p->nk, hipMemcpyDeviceToHost, st));
+        PMX_CHECK_HIP(hipMemcpyAsync(nvh.data(), w.nv.p, sizeof(int) * p->nk, hipMemcpyDeviceToHost, st));
+        PMX_CHECK_HIP(hipMemcpyAsync(&bad, w.bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        PMX_CHECK_HIP(hipStreamSynchronize(st));
+        if (bad)
+            return set_error(PMX_EBANDS, "at some k no level at or below the energy zero lies within -nband, "
+                                         "or at some k+q none above it (the reference's NBAND_C/NBAND_V are undefined)");
+        tb[0] = 0;
+        for (int k = 0; k < p->nk; k++) {
+            int a = spatial ? nch_h[k] / 2 : nch_h[k], b = spatial ? nvh[k] / 2 : nvh[k];
+            tb[k + 1] = tb[k] + a * b;
+        }
+        const int ntr = tb[p->nk];
+        if ((rc = upload(st, w.tbase, tb))) return rc;
+        if ((rc = w.Xd.ensure((size_t)std::max(ntr, 1) * nch * D))) return rc;
+        if ((rc = w.dEd.ensure(std::max(ntr, 1))) || (rc = w.wtd.ensure(std::max(ntr, 1)))) return rc;
+
+        /* ---- rho */
+        RhoArgs ra{};
+        ra.nk = p->nk; ra.N = N; ra.npw = npw; ra.neps = neps; ra.nb_s = nb_s; ra.nband = p->nband;
+        ra.spatial = spatial ? 1 : 0; ra.nch = nch;
+        ra.vk = w.vk.p; ra.vkq = w.vkq.p; ra.evk = w.evk.p; ra.evkq = w.evkq.p; ra.offset = p->offset;
+        ra.nc = w.nc.p; ra.nv = w.nv.p; ra.tbase = w.tbase.p; ra.loci = c->dLoci.p;
+        ra.wA = w.wAd.p; ra.uc = w.ucd.p; ra.wS = spatial ? nullptr : w.wSd.p;
+        ra.K = w.Kd.p; ra.KW = w.KWd.p; ra.X = w.Xd.p; ra.dE = w.dEd.p; ra.wt = w.wtd.p;
+        hipLaunchKernelGGL(rho_kernel, dim3(p->nk), dim3(256), 0, st, ra);
+        PMX_CHECK_HIP(hipGetLastError());
+
+        /* ---- bin by f0 (counting sort) */
+        if ((rc = w.bind.ensure(std::max(ntr, 1))) || (rc = w.perm.ensure(std::max(ntr, 1))) ||
+            (rc = w.hist.ensure(nbins)) || (rc = w.start.ensure(nbins + 1)) || (rc = w.cursor.ensure(nbins)))
+            return rc;
+        PMX_CHECK_HIP(hipMemsetAsync(w.hist.p, 0, sizeof(int) * nbins, st));
+        if (ntr > 0)
+            hipLaunchKernelGGL(bin_kernel, dim3((ntr + 255) / 256), dim3(256), 0, st, ntr, w.dEd.p, w0, df, wh, nbins,
+                               w.bind.p, w.hist.p);
+        std::vector<int> hist(nbins), start(nbins + 1);
+        PMX_CHECK_HIP(hipMemcpyAsync(hist.data(), w.hist.p, sizeof(int) * nbins, hipMemcpyDeviceToHost, st));
+        PMX_CHECK_HIP(hipStreamSynchronize(st));
+        start[0] = 0;
+        for (int b = 0; b < nbins; b++) start[b + 1] = start[b] + hist[b];
+        if ((rc = upload(st, w.start, start))) return rc;
+        PMX_CHECK_HIP(hipMemcpyAsync(w.cursor.p, w.start.p, sizeof(int) * nbins, hipMemcpyDeviceToDevice, st));
+        if (ntr > 0)
+            hipLaunchKernelGGL(scatter_kernel, dim3((ntr + 255) / 256), dim3(256), 0, st, ntr, w.bind.p, w.cursor.p,
+                               w.perm.p);
+        /* the order within a bin only changes the summation order; sort it
+         * on the host for run-to-run determinism */
+        std::vector<int> perm(start[nbins]);
+        if (!perm.empty()) {
+            PMX_CHECK_HIP(hipMemcpyAsync(perm.data(), w.perm.p, sizeof(int) * perm.size(), hipMemcpyDeviceToHost, st));
+            PMX_CHECK_HIP(hipStreamSynchronize(st));
+            for (int b = 0; b < nbins; b++) std::sort(perm.begin() + start[b], perm.begin() + start[b + 1]);
+            if ((rc = upload(st, w.perm, perm))) return rc;
+        }
+
+        /* ---- accumulate */
+        const int nacc = spatial ? 1 : 2;
+        for (int acc = 0; acc < nacc; acc++) {
+            DBuf<zd> &Z = acc == 0 ? w.Z1 : w.Z2;
+            DBuf<zd> &M = acc == 0 ? w.M1 : w.M2;
+            PMX_CHECK_HIP(hipMemsetAsync(Z.p, 0, sizeof(zd) * (size_t)p->nfreq * npairs * 256, st));
+            if (direct) {
+                DirArgs da{D, nch, npairs, nbins, wh, p->nfreq, p->delta, acc, p->epsilon, w0, df,
+                           w.Xd.p, w.dEd.p, w.wtd.p, w.perm.p, w.start.p, Z.p};
+                hipLaunchKernelGGL(direct_kernel, dim3(npairs, nbins), dim3(256), 0, st, da);
+            } else {
+                PMX_CHECK_HIP(hipMemsetAsync(M.p, 0, sizeof(zd) * (size_t)nbins * NT * npairs * 256, st));
+                MomArgs ma{D, nch, ntile, npairs, nbins, NT, w.Xd.p, w.dEd.p, w.wtd.p, w.perm.p, w.start.p,
+                           w0, df, acc, M.p};
+                hipLaunchKernelGGL(moment_simt_kernel, dim3(npairs, nbins), dim3(256), 0, st, ma);
+                ConvArgs ca{npairs, nbins, NT, wh, p->nfreq, M.p, w.Ktd.p, Z.p};
+                hipLaunchKernelGGL(conv_kernel, dim3(npairs, (p->nfreq + CONV_F - 1) / CONV_F), dim3(256), 0, st, ca);
+            }
+            PMX_CHECK_HIP(hipGetLastError());
+        }
+
+        /* ---- scale + pack (+ KK) */
+        double *dstRe = p->kk ? w.Re0.p : w.ReD.p, *dstIm = p->kk ? w.Im0.p : w.ImD.p;
+        PackArgs pa{D, neps, npairs, p->nfreq, 1, 0, spatial ? 0 : 1, SF, df, w0, w.Z1.p,
+                    spatial ? w.Z1.p : w.Z2.p, dstRe, dstIm};
+        hipLaunchKernelGGL(pack_kernel, dim3(1024), dim3(256), 0, st, pa);
+        if (p->kk)
+            hipLaunchKernelGGL(kk_kernel, dim3(2048), dim3(256), 0, st, 9 * ne2, p->nfreq, w.Kkd.p, w.Re0.p, w.Im0.p,
+                               w.ReD.p, w.ImD.p);
+        PMX_CHECK_HIP(hipGetLastError());
+        std::vector<double> hre(per_q), him(per_q);
+        PMX_CHECK_HIP(hipMemcpyAsync(hre.data(), w.ReD.p, sizeof(double) * per_q, hipMemcpyDeviceToHost, st));
+        PMX_CHECK_HIP(hipMemcpyAsync(him.data(), w.ImD.p, sizeof(double) * per_q, hipMemcpyDeviceToHost, st));
+        PMX_CHECK_HIP(hipStreamSynchronize(st));
+        /* scatter the per-q block [ij][e2][f] into [ij][q][e2][f] */
+        const size_t blk = (size_t)ne2 * p->nfreq;
+        for (int ij = 0; ij < 9; ij++) {
+            std::memcpy(ReX + ((size_t)ij * p->nq + q) * blk, hre.data() + (size_t)ij * blk, sizeof(double) * blk);
+            std::memcpy(ImX + ((size_t)ij * p->nq + q) * blk, him.data() + (size_t)ij * blk, sizeof(double) * blk);
+        }
+    }
+    return PMX_OK;
+}
+
+extern "C" int pmx_kk(pmx_ctx *c, int nel, int nfreq, double dfreq, double *ReX, double *ImX) {
+    if (!c || !ReX || !ImX) return set_error(PMX_EINVAL, "null pointer");
+    if (nel < 0 || nfreq <= 0) return set_error(PMX_EINVAL, "bad sizes");
+    PMX_CHECK_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const size_t n = (size_t)nel * nfreq;
+    DBuf<double> Kk, r0, i0, r, i;
+    int rc;
+    if ((rc = Kk.ensure((size_t)nfreq * nfreq)) || (rc = r0.ensure(n)) || (rc = i0.ensure(n)) || (rc = r.ensure(n)) ||
+        (rc = i.ensure(n)))
+        return rc;
+    PMX_CHECK_HIP(hipMemcpyAsync(r0.p, ReX, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    PMX_CHECK_HIP(hipMemcpyAsync(i0.p, ImX, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(kkmat_kernel, dim3(256), dim3(256), 0, st, nfreq, dfreq, Kk.p);
+    hipLaunchKernelGGL(kk_kernel, dim3(2048), dim3(256), 0, st, nel, nfreq, Kk.p, r0.p, i0.p, r.p, i.p);
+    PMX_CHECK_HIP(hipGetLastError());
+    PMX_CHECK_HIP(hipMemcpyAsync(ReX, r.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    PMX_CHECK_HIP(hipMemcpyAsync(ImX, i.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    PMX_CHECK_HIP(hipStreamSynchronize(st));
+    return PMX_OK;
+}
+
+extern "C" int pmx_epsinv(pmx_ctx *c, int nq, int neps, int nfreq, const double *ReX, const double *ImX,
+                          double *epsM, double *elf, double *epsinv) {
+    (void)c; (void)nq; (void)neps; (void)nfreq; (void)ReX; (void)ImX; (void)epsM; (void)elf; (void)epsinv;
+    return set_error(PMX_ESTATE, "pmx_epsinv: not built yet");
+}
+
+extern "C" int pmx_grad_bands(pmx_ctx *c, int nk, const double *K, int nband, const double *gE, double *gvg) {
+    (void)c; (void)nk; (void)K; (void)nband; (void)gE; (void)gvg;
+    return set_error(PMX_ESTATE, "pmx_grad_bands: not built yet");
+}

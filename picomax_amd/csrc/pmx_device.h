@@ -1,0 +1,107 @@
+/*
+ * pmx_device.h -- device-side types shared by the libpmx HIP sources.
+ */
+#ifndef PMX_DEVICE_H
+#define PMX_DEVICE_H
+
+#include <hip/hip_runtime.h>
+
+#include "pmx_internal.h"
+
+#include <complex>
+#include <string>
+#include <vector>
+
+namespace pmx {
+
+/* complex double on the device (interleaved re, im like the C-ABI arrays) */
+struct zd {
+    double x, y;
+};
+
+/* H(k) assembly modes of the eigensolver */
+enum HMode {
+    HM_SPATIAL_COMPLEX = 0, /* no SOC: H0 (NPW x NPW), complex Hermitian          */
+    HM_SPATIAL_REAL = 1,    /* no SOC, inversion centre found: H0 real symmetric  */
+    HM_SPINOR = 2           /* SOC: full 2NPW x 2NPW spinor H                      */
+};
+
+#define PMX_CHECK_HIP(call)                                                                   \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return set_error(PMX_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+/* device buffer with RAII */
+template <class T> struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int ensure(size_t count) {
+        if (count <= n && p) return 0;
+        release();
+        if (count == 0) count = 1;
+        if (hipMalloc((void **)&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            return set_error(PMX_ENOMEM, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+        return 0;
+    }
+};
+
+} // namespace pmx
+
+/* the context behind pmx_ctx */
+struct pmx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    size_t lds_max = 0;
+    bool has_system = false;
+
+    /* system */
+    int npw = 0, neps = 0, natom = 0, nv = 0;
+    double a = 0, bz = 0, g2max = 0, kin = 0;
+    bool soc = false;
+    bool real_gauge = false;
+    std::vector<double> G, tau, g2, vg;
+    std::vector<int> shell;              /* [npw][npw] form-factor shell or -1 */
+    std::vector<pmx::zd> sfac;           /* [npw][npw][natom] e^{-i2pi dG.tau}  */
+    std::vector<double> phase;           /* [npw] real-gauge phases            */
+    std::vector<pmx::zd> vso;            /* [npw][npw] SOC strength per pair   */
+    std::vector<int> loci_h;             /* [neps][npw]                        */
+    std::vector<double> us_k, us_v, ua_k, ua_v; /* SOC maps, ascending keys   */
+
+    /* socCurrent's strength for q = G_j - G_i (src/pmx_chi.cpp:203-237): the
+     * symmetric part always from the map, even with -use_uniform_us_so */
+    std::complex<double> vso_chi_at(double qx, double qy, double qz) const;
+
+    pmx::DBuf<double> dG, dPhase, dVreal, dVg;
+    pmx::DBuf<pmx::zd> dVpair, dVso, dSfac;
+    pmx::DBuf<int> dShell, dLoci;
+
+    /* eigensolver scratch */
+    pmx::DBuf<double> dK, dEval, dScratch;
+    pmx::DBuf<pmx::zd> dEvec, dZscr;
+    pmx::DBuf<int> dStatus;
+};
+
+namespace pmx {
+/* eigensolve of nmat H(K) on the context's stream; K is a device array */
+int eig_device(pmx_ctx *ctx, int nmat, const double *dKpts, int nb, int nvec, double *dEvals, zd *dEvecs);
+/* basis size of the solve (NPW spatial or 2 NPW spinor) */
+inline int basis_size(const pmx_ctx *c) { return c->soc ? 2 * c->npw : c->npw; }
+int rebuild_vtable(pmx_ctx *ctx);
+} // namespace pmx
+
+#endif

@@ -1,0 +1,462 @@
+/*
+ * pmx_input.cpp -- host-side set-up of a picomax run: option parsing, crystal
+ * and material set-up, the plane-wave G table, q-paths and k-point import.
+ *
+ * Behaviour follows the reference's set-up code, function by function:
+ *   option lookup        inputParser        src/pmx.hpp:190-245
+ *   run options          inputProcess       src/pmx_io.cpp:423-503
+ *   lattice / q-vectors  init_crystal       src/pmx_io.cpp:108-310
+ *   form factors, SOC    init_mater         src/pmx_io.cpp:315-417
+ *   G table              generate_gpt       src/pmx_basis.cpp:29-81
+ *                        generate_gpt_rect  src/pmx_basis.cpp:86-132
+ *   k-point file         import_kpt         src/pmx_basis.cpp:200-245
+ *   symmetry points      highsympoint_*     src/pmx_basis.cpp:722-788
+ * Errors are returned as codes with a message instead of abort(); the CLI
+ * turns the reference's abort paths back into abort().
+ */
+#include "pmx_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <map>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace pmx {
+
+namespace {
+
+/* first-occurrence token lookup of inputParser */
+struct Tokens {
+    std::vector<std::string> t;
+    bool has(const std::string &o) const { return std::find(t.begin(), t.end(), o) != t.end(); }
+    std::string val(const std::string &o) const {
+        auto it = std::find(t.begin(), t.end(), o);
+        if (it != t.end() && ++it != t.end()) return *it;
+        return std::string();
+    }
+    /* comma split exactly as vectorOption: getline while the stream is good */
+    std::vector<std::string> vec(const std::string &o) const {
+        std::stringstream ss(val(o));
+        std::vector<std::string> v;
+        while (ss.good()) {
+            std::string s;
+            std::getline(ss, s, ',');
+            v.push_back(s);
+        }
+        return v;
+    }
+    void fill_int(const std::string &o, int &v) const {
+        if (has(o)) v = std::stoi(val(o));
+    }
+    void fill_double(const std::string &o, double &v) const {
+        if (has(o)) v = std::stod(val(o));
+    }
+};
+
+V3 hs_fcc(const std::string &s) {
+    static const std::unordered_map<std::string, V3> m{
+        {"G", {0, 0, 0}}, {"X", {1, 0, 0}}, {"L", {0.5, 0.5, 0.5}},
+        {"W", {1, 0.5, 0}}, {"U", {1, 0.25, 0.25}}, {"K", {0.75, 0.75, 0}}};
+    return m.at(s);
+}
+
+V3 hs_hex(const std::string &s, double f) {
+    if (s == "G") return {0, 0, 0};
+    if (s == "A") return {0, 0, 0.5 / f};
+    if (s == "H") return {2.0 / 3.0, 0, 0.5 / f};
+    if (s == "K") return {2.0 / 3.0, 0, 0};
+    if (s == "L") return {0.5, -0.5 / std::sqrt(3), 0.5 / f};
+    if (s == "M") return {0.5, -0.5 / std::sqrt(3), 0};
+    throw std::out_of_range("unknown high-symmetry point " + s);
+}
+
+V3 hs_hex2d(const std::string &s) {
+    if (s == "G") return {0, 0, 0};
+    if (s == "M") return {1.0 / 3.0, 0, 0};
+    if (s == "K") return {1.0 / 3.0, 1.0 / 3.0 / std::sqrt(3), 0};
+    if (s == "Kp") return {1.0 / 3.0, -1.0 / 3.0 / std::sqrt(3), 0};
+    throw std::out_of_range("unknown high-symmetry point " + s);
+}
+
+double norm3(const V3 &v) { return std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+
+/* "G^2:value" pairs of -us_so / -ua_so into an ordered map (later duplicate
+ * keys overwrite, as operator[] does); malformed pairs are skipped with a
+ * message on stderr, as the reference does. */
+void parse_pairs(const std::vector<std::string> &pairs, const char *flag, std::map<double, double> &out) {
+    for (const std::string &p : pairs) {
+        std::stringstream ss(p);
+        std::string k, v;
+        if (std::getline(ss, k, ':') && std::getline(ss, v)) {
+            try {
+                out[std::stod(k)] = std::stod(v);
+            } catch (const std::exception &) {
+                fprintf(stderr, "Invalid argument for %s pair: %s. Format G^2:value expected.\n", flag,
+                        p.c_str());
+            }
+        } else {
+            fprintf(stderr, "Invalid format for %s pair: %s. Format G^2:value expected.\n", flag, p.c_str());
+        }
+    }
+}
+
+} // namespace
+
+int generate_gpt_host(const double *recip, int dim, double gcut, bool rect, std::vector<double> &G) {
+    const double tol = 1e-6;
+    int imax = (int)std::ceil(gcut);
+    int N = (int)std::pow(2 * imax + 1, dim);
+    std::vector<V3> cand;
+    cand.reserve(N);
+    const V3 b1{recip[0], recip[1], recip[2]}, b2{recip[3], recip[4], recip[5]},
+        b3{recip[6], recip[7], recip[8]};
+    if (dim == 3) {
+        for (int i1 = -imax; i1 <= imax; i1++)
+            for (int i2 = -imax; i2 <= imax; i2++)
+                for (int i3 = -imax; i3 <= imax; i3++) {
+                    const double c1 = i1, c2 = i2, c3 = i3;
+                    cand.push_back({c1 * b1.x + c2 * b2.x + c3 * b3.x, c1 * b1.y + c2 * b2.y + c3 * b3.y,
+                                    c1 * b1.z + c2 * b2.z + c3 * b3.z});
+                }
+    } else if (dim == 2) {
+        for (int i1 = -imax; i1 <= imax; i1++)
+            for (int i2 = -imax; i2 <= imax; i2++) {
+                const double c1 = i1, c2 = i2;
+                cand.push_back({c1 * b1.x + c2 * b2.x, c1 * b1.y + c2 * b2.y, c1 * b1.z + c2 * b2.z});
+            }
+    } else {
+        return set_error(PMX_EINVAL, "dimension must be 2 or 3");
+    }
+    std::vector<double> gn(N);
+    for (int i = 0; i < N; i++) gn[i] = norm3(cand[i]);
+    /* same comparator on the same index sequence: libstdc++ std::sort then
+     * reproduces the reference's tie order within each shell */
+    std::vector<int> idx(N);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](int A, int B) { return gn[A] < gn[B]; });
+    std::vector<double> gs = gn;
+    std::sort(gs.begin(), gs.end(), std::less<double>());
+    int npw = N;
+    if (!rect) {
+        for (int i = 0; i < N; i++)
+            if (gs[i] > gcut + tol) {
+                npw = i;
+                break;
+            }
+    }
+    G.resize((size_t)3 * npw);
+    for (int i = 0; i < npw; i++) {
+        G[3 * i] = cand[idx[i]].x;
+        G[3 * i + 1] = cand[idx[i]].y;
+        G[3 * i + 2] = cand[idx[i]].z;
+    }
+    return npw;
+}
+
+int input_parse(int argc, const char *const *argv, Input &in) {
+    Tokens T;
+    for (int i = 1; i < argc; i++) T.t.push_back(argv[i]);
+    try {
+        /* ---- inputProcess */
+        T.fill_int("-debug", in.debug);
+        in.outputfile = T.has("-outputfile") ? T.val("-outputfile") : std::string("picomax_output");
+        if (T.has("-wdir"))
+            in.wdir = T.val("-wdir");
+        else
+            in.wdir = std::filesystem::current_path().string();
+        T.fill_int("-logfile", in.logfile);
+        if (T.has("-kpointfile")) {
+            in.kpointfile = T.val("-kpointfile");
+            in.kpoint = 0;
+        } else {
+            T.fill_int("-kpoint", in.kpoint);
+            T.fill_int("-kpointorder", in.kpointorder);
+        }
+        T.fill_int("-nband", in.nband);
+        T.fill_int("-neps", in.neps);
+        T.fill_int("-nvalence", in.nvalence);
+        if (T.has("-nfreq") && T.has("-dfreq")) {
+            in.nfreq = std::stoi(T.val("-nfreq"));
+            in.dfreq = std::stod(T.val("-dfreq"));
+            in.has_freq = true;
+        }
+        T.fill_int("-kk", in.kk);
+        T.fill_double("-epsilon", in.epsilon);
+        T.fill_int("-delta", in.delta);
+        if (T.has("-refpoint")) {
+            auto v = T.vec("-refpoint");
+            if (v.size() != 3) return set_error(PMX_EINVAL, "check the number of refpoint!");
+            in.refpoint = {std::stod(v[0]), std::stod(v[1]), std::stod(v[2])};
+        }
+        in.sw = T.has("-switch") ? T.val("-switch") : std::string();
+
+        /* ---- init_crystal */
+        T.fill_int("-dim", in.dim);
+        if (T.has("-crystal")) in.crystal = T.val("-crystal");
+        if (!T.has("-a")) return set_error(PMX_EINVAL, "lattice constant (a) is a mandatory argument!");
+        in.a = std::stod(T.val("-a"));
+        T.fill_double("-f", in.f);
+        T.fill_double("-u", in.u);
+        const double r3 = std::sqrt(3);
+        const std::string &c = in.crystal;
+        double *R = in.recip;
+        auto setR = [&](V3 a1, V3 a2, V3 a3) {
+            R[0] = a1.x; R[1] = a1.y; R[2] = a1.z;
+            R[3] = a2.x; R[4] = a2.y; R[5] = a2.z;
+            R[6] = a3.x; R[7] = a3.y; R[8] = a3.z;
+        };
+        in.tau.clear();
+        if (c == "zb" || c == "rs") {
+            setR({-1, 1, 1}, {1, -1, 1}, {1, 1, -1});
+            in.bz_volume = 4.0;
+            in.tau = {{0, 0, 0}, c == "zb" ? V3{0.25, 0.25, 0.25} : V3{0.5, 0, 0}};
+        } else if (c == "wz" || c == "te") {
+            setR({1, -1 / r3, 0}, {1, 1 / r3, 0}, {0, 0, 1.0 / in.f});
+            V3 b1{R[0], R[1], R[2]}, b2{R[3], R[4], R[5]}, b3{R[6], R[7], R[8]};
+            V3 cr{b1.y * b2.z - b1.z * b2.y, b1.z * b2.x - b1.x * b2.z, b1.x * b2.y - b1.y * b2.x};
+            in.bz_volume = std::fabs(cr.x * b3.x + cr.y * b3.y + cr.z * b3.z);
+            if (c == "wz")
+                in.tau = {{0.5, 0.5 / r3, 0}, {0.5, -0.5 / r3, 0.5 * in.f}, {0.5, 0.5 / r3, in.u * in.f},
+                          {0.5, -0.5 / r3, (0.5 + in.u) * in.f}};
+            else
+                in.tau = {{0, 0.5 * r3 * in.u, -1.0 / 3.0 * in.f}, {1.5 * in.u, 0, 0},
+                          {0, -0.5 * r3 * in.u, 1.0 / 3.0 * in.f}};
+        } else if (c == "hex") {
+            in.dim = 2;
+            setR({1.0 / 3.0, -r3 / 3.0, 0}, {1.0 / 3.0, r3 / 3.0, 0}, {0, 0, 0});
+            in.bz_volume = std::fabs(R[0] * R[3] + R[1] * R[4] + R[2] * R[5]);
+            in.tau = {{0, 0, 0}, {0.5, 0, 5 * r3}};
+        } else {
+            return set_error(PMX_EINVAL, "unknown -crystal '" + c + "' (zb, rs, wz, te, hex)");
+        }
+        T.fill_int("-gsym", in.gsym);
+        T.fill_double("-encut", in.encut);
+        in.gcut = std::sqrt(in.encut * std::pow(in.a / 2.0 / PMX_PI, 2) / PMX_KCON);
+        T.fill_double("-gcut", in.gcut);
+        if (T.has("-gammazero")) {
+            auto v = T.vec("-gammazero");
+            if (v.size() != 3) return set_error(PMX_EINVAL, "check the number of gammazero!");
+            in.gammazero = {std::stod(v[0]), std::stod(v[1]), std::stod(v[2])};
+        }
+        in.Q.clear();
+        if (T.has("-qvec")) {
+            auto v = T.vec("-qvec");
+            if (v.size() % 3 != 0) return set_error(PMX_EINVAL, "check the number of qvec!");
+            for (size_t i = 0; i < v.size(); i += 3)
+                in.Q.push_back({std::stod(v[i]), std::stod(v[i + 1]), std::stod(v[i + 2])});
+        } else if (T.has("-qpath") != T.has("-qnum")) {
+            return set_error(PMX_EINVAL, "specify both -qpath and -qnum!");
+        } else if (T.has("-qpath")) {
+            in.qpath = T.vec("-qpath");
+            for (auto &s : T.vec("-qnum")) in.qnum.push_back(std::stoi(s));
+            V3 v0{}, v1{};
+            for (size_t i = 0; i < in.qnum.size(); i++) {
+                if (i + 1 >= in.qpath.size()) return set_error(PMX_EINVAL, "-qpath needs one more point than -qnum");
+                if (c == "zb" || c == "rs") {
+                    v0 = hs_fcc(in.qpath[i]);
+                    v1 = hs_fcc(in.qpath[i + 1]);
+                } else if (c == "wz" || c == "te") {
+                    v0 = hs_hex(in.qpath[i], in.f);
+                    v1 = hs_hex(in.qpath[i + 1], in.f);
+                } else {
+                    v0 = hs_hex2d(in.qpath[i]);
+                    v1 = hs_hex2d(in.qpath[i + 1]);
+                }
+                V3 dv{v1.x - v0.x, v1.y - v0.y, v1.z - v0.z};
+                int n = in.qnum[i];
+                for (int j = 0; j < n; j++) {
+                    double t = double(j) / double(n);
+                    V3 q{v0.x + t * dv.x, v0.y + t * dv.y, v0.z + t * dv.z};
+                    if (norm3(q) == 0) q = {q.x + in.gammazero.x, q.y + in.gammazero.y, q.z + in.gammazero.z};
+                    in.Q.push_back(q);
+                }
+            }
+            V3 q = v1;
+            if (norm3(q) == 0) q = {q.x + in.gammazero.x, q.y + in.gammazero.y, q.z + in.gammazero.z};
+            in.Q.push_back(q);
+        }
+
+        /* ---- init_mater */
+        const int natom = (int)in.tau.size();
+        if (T.has("-vg")) {
+            auto v = T.vec("-vg");
+            if (v.size() % (natom + 1) != 0) return set_error(PMX_EINVAL, "check the number of arguments for -vg!");
+            int nv = (int)v.size() / (natom + 1);
+            in.g2.resize(nv);
+            in.vg.assign((size_t)natom * nv, 0.0);
+            for (int i = 0; i < nv; i++) {
+                in.g2[i] = std::stod(v[i]);
+                for (int j = 0; j < natom; j++) in.vg[(size_t)j * nv + i] = std::stod(v[(j + 1) * nv + i]);
+            }
+            in.g2max = in.g2[nv - 1] + 1e-2;
+        }
+        if (T.has("-use_uniform_us_so")) {
+            in.use_uniform_us = 1;
+            if (T.has("-uniform_us_so_val")) {
+                try {
+                    in.uniform_us = std::stod(T.val("-uniform_us_so_val"));
+                } catch (const std::exception &) {
+                    fprintf(stderr, "Invalid argument for -uniform_us_so_val: Must be a number.\n");
+                }
+            } else {
+                fprintf(stderr, "Warning: -use_uniform_us_so is set, but -uniform_us_so_val is not provided. "
+                                "Defaulting to 0.0 Ry.\n");
+                in.uniform_us = 0.0;
+            }
+        } else if (T.has("-us_so")) {
+            parse_pairs(T.vec("-us_so"), "-us_so", in.us);
+        }
+        if (T.has("-ua_so_val_g0")) {
+            try {
+                in.ua[0.0] = std::stod(T.val("-ua_so_val_g0"));
+            } catch (const std::exception &) {
+                fprintf(stderr, "Invalid argument for -ua_so_val_g0: Must be a number.\n");
+            }
+        }
+        if (T.has("-ua_so")) parse_pairs(T.vec("-ua_so"), "-ua_so", in.ua);
+    } catch (const std::exception &e) {
+        return set_error(PMX_EINVAL, std::string("invalid option value: ") + e.what());
+    }
+
+    /* ---- generate_gpt / generate_gpt_rect */
+    int npw = generate_gpt_host(in.recip, in.dim, in.gcut, in.gsym == 0, in.G);
+    if (npw < 0) return npw;
+    in.npw = npw;
+
+    /* ---- k-points */
+    if (in.kpoint == 0) {
+        int rc = import_kpt(in.kpointfile, in.K, in.KW);
+        if (rc) return rc;
+    } else if (in.kpoint > 0) {
+        return set_error(PMX_EINVAL, "-kpoint " + std::to_string(in.kpoint) +
+                                         " grid generation is not provided; pass -kpointfile "
+                                         "(picomax_amd.kpoints writes Gamma-centred grids)");
+    }
+    return PMX_OK;
+}
+
+int import_kpt(const std::string &path, std::vector<V3> &K, std::vector<double> &KW) {
+    std::ifstream f(path);
+    if (!f.is_open()) return set_error(PMX_EIO, "cannot open k-point file " + path);
+    std::string line;
+    int i = 0, nk0 = 1, nk = 0;
+    K.clear();
+    KW.clear();
+    while (std::getline(f, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        if (i == 0) {
+            std::stringstream(line) >> nk0;
+            i++;
+        } else if (i == 1) {
+            std::stringstream(line) >> nk;
+            K.assign(nk, V3{0, 0, 0});
+            KW.assign(nk, 0.0);
+            i++;
+        } else if (i - 2 < nk) {
+            std::stringstream ss(line);
+            ss >> K[i - 2].x >> K[i - 2].y >> K[i - 2].z >> KW[i - 2];
+            KW[i - 2] /= nk0;
+            i++;
+        }
+    }
+    return PMX_OK;
+}
+
+} // namespace pmx
+
+extern "C" {
+
+int pmx_generate_gpt(const double *recip, int dim, double gcut, int max_out, double *G_out) {
+    std::vector<double> G;
+    int npw = pmx::generate_gpt_host(recip, dim, gcut, false, G);
+    if (npw < 0) return npw;
+    for (int i = 0; i < npw && i < max_out; i++)
+        for (int d = 0; d < 3; d++) G_out[3 * i + d] = G[3 * i + d];
+    return npw;
+}
+
+int pmx_input_parse(int argc, const char *const *argv, pmx_input **out) {
+    if (!out) return pmx::set_error(PMX_EINVAL, "null output pointer");
+    auto *in = new pmx::Input();
+    int rc = pmx::input_parse(argc, argv, *in);
+    if (rc) {
+        delete in;
+        *out = nullptr;
+        return rc;
+    }
+    in->finalize();
+    *out = reinterpret_cast<pmx_input *>(in);
+    return PMX_OK;
+}
+
+void pmx_input_free(pmx_input *in) { delete reinterpret_cast<pmx::Input *>(in); }
+
+int pmx_input_system(const pmx_input *pin, pmx_system *s) {
+    if (!pin || !s) return pmx::set_error(PMX_EINVAL, "null pointer");
+    const pmx::Input &in = *reinterpret_cast<const pmx::Input *>(pin);
+    std::memset(s, 0, sizeof(*s));
+    s->npw = in.npw;
+    s->G = in.G.data();
+    s->neps = in.neps;
+    s->natom = (int)in.tau.size();
+    s->tau = in.tau_flat.data();
+    s->nv = (int)in.g2.size();
+    s->g2 = in.g2.data();
+    s->vg = in.vg.data();
+    s->g2max = in.g2max;
+    s->n_us = (int)in.us_k.size();
+    s->us_key = in.us_k.data();
+    s->us_val = in.us_v.data();
+    s->n_ua = (int)in.ua_k.size();
+    s->ua_key = in.ua_k.data();
+    s->ua_val = in.ua_v.data();
+    s->use_uniform_us = in.use_uniform_us;
+    s->uniform_us = in.uniform_us;
+    s->a = in.a;
+    s->bz_volume = in.bz_volume;
+    return PMX_OK;
+}
+
+int pmx_input_info(const pmx_input *pin, pmx_run_info *r) {
+    if (!pin || !r) return pmx::set_error(PMX_EINVAL, "null pointer");
+    const pmx::Input &in = *reinterpret_cast<const pmx::Input *>(pin);
+    std::memset(r, 0, sizeof(*r));
+    r->sw = in.sw.c_str();
+    r->outputfile = in.outputfile.c_str();
+    r->wdir = in.wdir.c_str();
+    r->debug = in.debug;
+    r->nband = in.nband;
+    r->nvalence = in.nvalence;
+    r->neps = in.neps;
+    r->kk = in.kk;
+    r->delta = in.delta;
+    r->nfreq = in.has_freq ? in.nfreq : 0;
+    r->npw = in.npw;
+    r->natom = (int)in.tau.size();
+    r->nq = (int)in.Q.size();
+    r->nk = (int)in.K.size();
+    r->kpoint = in.kpoint;
+    r->epsilon = in.epsilon;
+    r->dfreq = in.dfreq;
+    r->gcut = in.gcut;
+    r->encut = in.encut;
+    r->a = in.a;
+    r->bz_volume = in.bz_volume;
+    r->refpoint[0] = in.refpoint.x;
+    r->refpoint[1] = in.refpoint.y;
+    r->refpoint[2] = in.refpoint.z;
+    r->Q = in.Q_flat.data();
+    r->K = in.K.empty() ? nullptr : in.K_flat.data();
+    r->KW = in.KW.empty() ? nullptr : in.KW.data();
+    return PMX_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,119 @@
+"""GPU parity: the HIP path through the C-ABI against the reference's golden
+outputs and the FP64 oracle.
+
+Bars: band energies within 1e-10 eV; chi within 1e-8 of each (i,j,q,m,n)
+slice's max |value| (FP64 paths); .gpt bytes identical; loci identical.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import cli_args
+import orc
+from picomax_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+BAND_CASES = ["si_band_c1", "gaas_band", "te_soc_band", "wz_band"]
+CHI_CASES = ["si_chi_c1", "si_chi_lor", "si_chi_tri", "si_chi_rect", "te_soc_chi", "te_usoc_chi", "wz_chi"]
+
+
+def _expand(args):
+    return [a.replace("{golden}", cli_args.GOLDEN) for a in args]
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+def _nslice(args):
+    s, p = cli_args.build(args)
+    return 9 * len(p["Q"]) * p["neps"] * (p["neps"] + 1) // 2
+
+
+@pytest.mark.parametrize("case", BAND_CASES + CHI_CASES)
+def test_cli_matches_reference(manifest, case, tmp_path):
+    args = _expand(manifest[case]["args"])
+    _say("CLI case %s" % case)
+    r = subprocess.run([capi.CLI_PATH] + args + ["-wdir", str(tmp_path), "-outputfile", case],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:]
+    dat = np.fromfile(str(tmp_path / (case + ".dat")), dtype="<f8")
+    ref = np.fromfile(os.path.join(cli_args.GOLDEN, case + ".dat"), dtype="<f8")
+    assert dat.shape == ref.shape
+    if case in BAND_CASES:
+        assert np.abs(dat - ref).max() < 1e-10
+    else:
+        err = cli_args.slice_error(dat, ref, _nslice(manifest[case]["args"]))
+        _say("  %s slice error %.3e" % (case, err))
+        assert err < 1e-8
+        gpt = open(str(tmp_path / (case + ".gpt")), "rb").read()
+        assert gpt == open(os.path.join(cli_args.GOLDEN, case + ".gpt"), "rb").read()
+
+
+@pytest.mark.parametrize("case", ["te_gpt", "wz_gpt", "si_chi_c1", "te_soc_chi"])
+def test_loci_matches_oracle(manifest, ctx, case):
+    args = _expand(manifest[case]["args"])
+    _say("loci %s" % case)
+    inp = capi.Input(args)
+    ctx.set_system(inp.system(), inp)
+    s, p = cli_args.build(manifest[case]["args"])
+    assert np.array_equal(ctx.loci(), s.loci)
+
+
+@pytest.mark.parametrize("case", ["si_band_c1", "gaas_band", "te_soc_band", "wz_band"])
+def test_eigenpairs_against_oracle_hamiltonian(manifest, ctx, case):
+    args = _expand(manifest[case]["args"])
+    _say("eigenpairs %s" % case)
+    inp = capi.Input(args)
+    ctx.set_system(inp.system(), inp)
+    s, p = cli_args.build(manifest[case]["args"])
+    rng = np.random.default_rng(7)
+    K = rng.uniform(-0.6, 0.6, (6, 3))
+    K[0] = 0.0  # Gamma: exact degeneracies
+    nb = min(16, 2 * s.npw)
+    ev, vec = ctx.eig(K, nb, vectors=True)
+    for k in range(len(K)):
+        H = s.hamiltonian(K[k])
+        w = np.linalg.eigvalsh(H)[:nb]
+        assert np.abs(ev[k] - w).max() < 1e-10
+        V = vec[k].T  # columns
+        res = H @ V - V * ev[k]
+        assert np.abs(res).max() < 1e-9
+        assert np.abs(V.conj().T @ V - np.eye(nb)).max() < 1e-10
+
+
+def test_spatial_path_equals_spinor_oracle_random(ctx):
+    """Random zinc-blende form factors, no inversion centre: the complex
+    spatial solve + factor 2 equals the oracle's full spinor chi."""
+    _say("random zb chi fp64")
+    rng = np.random.default_rng(11)
+    vs = rng.uniform(-0.3, 0.3, 4)
+    va = rng.uniform(-0.1, 0.1, 4)
+    vg = [3, 4, 8, 11] + list(vs + va) + list(vs - va)
+    args = ["-crystal", "zb", "-a", "5.6", "-vg", ",".join("%.17g" % x for x in vg), "-gcut", "3.0", "-neps", "4"]
+    inp = capi.Input(args)
+    ctx.set_system(inp.system(), inp)
+    s = orc.System("zb", a=5.6, vg=vg, gcut=3.0, neps=4)
+    K = rng.uniform(-0.5, 0.5, (5, 3))
+    KW = np.full(5, 0.2)
+    off = s.ref_energy((0.5, 0.5, 0.5), 8)
+    assert abs(ctx.ref_energy((0.5, 0.5, 0.5), 8) - off) < 1e-11
+    Q = [[0.013, -0.007, 0.002]]
+    for delta, eps, df, nf in [(1, 0.08, 0.04, 300), (0, 0.12, 0.05, 260)]:
+        re0, im0 = s.chi(K, KW, Q, 24, off, delta, eps, nf, df, 1)
+        re1, im1 = ctx.chi(K, KW, Q, 24, off, delta, eps, nf, df, 1, "fp64")
+        err = cli_args.slice_error(np.concatenate([re1, im1]), np.concatenate([re0, im0]), 9 * 10)
+        _say("  delta %d err %.3e" % (delta, err))
+        assert err < 1e-8
