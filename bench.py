@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Benchmark: chi(q, w) k-points per second on one MI355X.
+
+Workload (the largest single-GPU configuration): Si zinc-blende, a = 5.43 A,
+form factors V(sqrt3) = -0.21, V(sqrt8) = 0.04, V(sqrt11) = 0.08 Ry, 137 plane
+waves (-gcut sqrt24), Gamma-centred 24^3 k-grid (13,824 k and 13,824 k+q
+eigensolves), q = 0.01 [100], -nband 36 (4 valence -> 14 conduction spatial
+bands; 32 would cut the Lambda_3 pair on this grid), 59 local-field G, 1000
+frequencies 0..19.98 eV, Lorentzian 0.05 eV, Kramers-Kronig on.
+
+A step is one full pmx_chi_device call: both eigensolve batches, rho, the
+binned rank update, the frequency convolution, scaling and KK, with the
+outputs left in HBM.  `value` is the mixed-precision path's k-points/s; the
+FP64 path is timed in the same run for the ratio and as the reference for the
+eps_M error.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--precision fp32]
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from picomax_amd import capi, kpoints  # noqa: E402
+
+SI_VG = "3,4,8,11,-0.21,0,0.04,0.08,-0.21,0,0.04,0.08"
+GCUT = "4.898979485566356"  # sqrt(24) -> 137 plane waves
+NEPS, NBAND, NVALENCE = 59, 36, 8
+NFREQ, DFREQ, DELTA, EPS, KK = 1000, 0.02, 1, 0.05, 1
+QVEC = [0.01, 0.0, 0.0]
+REFPOINT = [0.5, 0.5, 0.5]
+NGRID = 24
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+WORKLOAD = ("Si zb a=5.43, 137 PW (gcut sqrt24), 24^3 Gamma k-grid (13824 k + 13824 k+q eigensolves), "
+            "q=0.01[100], nband 36 (4v->14c), NEPS 59, 1000 w (df 0.02), Lorentzian 0.05, KK")
+
+
+def cli_args():
+    return ["-crystal", "zb", "-a", "5.43", "-vg", SI_VG, "-gcut", GCUT, "-neps", str(NEPS)]
+
+
+class Hip:
+    """hipMalloc / hipFree / hipDeviceSynchronize from the HIP runtime libpmx uses."""
+
+    def __init__(self):
+        capi.lib()  # loads libamdhip64 as a dependency
+        self.h = ctypes.CDLL("libamdhip64.so")
+        self.h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.h.hipFree.argtypes = [ctypes.c_void_p]
+        self.h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+
+    def malloc(self, nbytes):
+        p = ctypes.c_void_p()
+        if self.h.hipMalloc(ctypes.byref(p), nbytes) != 0:
+            raise RuntimeError("hipMalloc failed")
+        return p.value
+
+    def free(self, p):
+        self.h.hipFree(ctypes.c_void_p(p))
+
+    def sync(self):
+        if self.h.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
+
+    def d2h(self, dst_np, src):
+        if self.h.hipMemcpy(dst_np.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(src), dst_np.nbytes, 2) != 0:
+            raise RuntimeError("hipMemcpy failed")
+
+
+def run_steps(ctx, hip, params, dre, dim, steps, warmup):
+    for _ in range(warmup):
+        ctx.chi_device(params, dre, dim)
+    hip.sync()
+    ctx.profile(True)
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.chi_device(params, dre, dim)
+    hip.sync()
+    t1 = time.perf_counter()
+    prof = ctx.profile_report()
+    ctx.profile(False)
+    return (t1 - t0) / steps, prof
+
+
+def eps_block(hip, dre, dim, n_block):
+    re = np.zeros(n_block)
+    im = np.zeros(n_block)
+    hip.d2h(re, dre)
+    hip.d2h(im, dim)
+    return re, im
+
+
+def cpu_baseline(workdir):
+    """The reference CLI (compiled from its sources into oracle/_ref) on a
+    bounded sample of the same workload: every 512th k-point of the 24^3 grid
+    (27 points, weight 512 of NKPT0 = 13824), timed on the host's cores."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "picomax_ref")
+    K = kpoints.gamma_grid("zb", NGRID)[::512]
+    kfile = os.path.join(workdir, "ksub.txt")
+    kpoints.write_kpoints(kfile, K, weights=np.full(len(K), 512.0), nkpt0=NGRID ** 3)
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
+    args = ["-switch", "epsij"] + cli_args() + [
+        "-nband", str(NBAND), "-nvalence", str(NVALENCE), "-refpoint", ",".join(map(str, REFPOINT)),
+        "-qvec", ",".join(map(str, QVEC)), "-kpointfile", kfile, "-nfreq", str(NFREQ), "-dfreq", str(DFREQ),
+        "-delta", str(DELTA), "-epsilon", str(EPS), "-kk", str(KK), "-wdir", workdir, "-outputfile", "cpu"]
+    kind = "reference"
+    if not os.path.exists(exe):
+        return None
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+    t0 = time.perf_counter()
+    r = subprocess.run([exe] + args, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    t1 = time.perf_counter()
+    if r.returncode != 0:
+        return None
+    return {"value": len(K) / (t1 - t0), "unit": "k-pts/s", "cores": cores, "kind": kind,
+            "sample": "%d of %d k-points (every 512th of the 24^3 grid, weights renormalised), same C2 "
+                      "inputs, reference CLI built from its sources with g++ -O3 -fopenmp -ffast-math, "
+                      "%.1f s wall" % (len(K), NGRID ** 3, t1 - t0)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3"])
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--grid", type=int, default=NGRID)
+    opt = ap.parse_args()
+    if opt.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (k-sharding is described "
+              "in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    inp = capi.Input(cli_args())
+    ctx = capi.Context(0)
+    ctx.set_system(inp.system(), inp)
+    K = kpoints.gamma_grid("zb", opt.grid)
+    KW = np.full(len(K), 1.0 / len(K))
+    off = ctx.ref_energy(REFPOINT, NVALENCE)
+    hip = Hip()
+    ne2 = NEPS * (NEPS + 1) // 2
+    n = 9 * ne2 * NFREQ
+    dre, dim = hip.malloc(8 * n), hip.malloc(8 * n)
+    nk = len(K)
+
+    pm = ctx.chi_params(K, KW, [QVEC], NBAND, off, DELTA, EPS, NFREQ, DFREQ, KK, opt.precision)
+    t_mix, prof_mix = run_steps(ctx, hip, pm, dre, dim, opt.steps, opt.warmup)
+    re_m, im_m = eps_block(hip, dre, dim, ne2 * NFREQ)
+
+    p64 = ctx.chi_params(K, KW, [QVEC], NBAND, off, DELTA, EPS, NFREQ, DFREQ, KK, "fp64")
+    t_64, prof_64 = run_steps(ctx, hip, p64, dre, dim, opt.fp64_steps, 1)
+    re_d, im_d = eps_block(hip, dre, dim, ne2 * NFREQ)
+    hip.free(dre)
+    hip.free(dim)
+
+    eM_m, elf_m = ctx.epsinv(re_m, im_m, 1, NFREQ)
+    eM_d, elf_d = ctx.epsinv(re_d, im_d, 1, NFREQ)
+    sel = slice(1, None)  # w = 0 is identically 1 + 0i in both
+    eps_err = float(np.max(np.abs(eM_m[0, sel] - eM_d[0, sel]) / np.abs(eM_d[0, sel])))
+    elf_err = float(np.max(np.abs(elf_m[0, sel] - elf_d[0, sel])) / np.max(np.abs(elf_d[0, sel])))
+    im_err = float(np.max(np.abs(eM_m[0, sel].imag - eM_d[0, sel].imag)) / np.max(np.abs(eM_d[0, sel].imag)))
+
+    # dominant kernel of the mixed step
+    dom = max(prof_mix.items(), key=lambda kv: kv[1]["ms"])
+    name, rec = dom
+    avg_ms = rec["ms"] / rec["count"]
+    achieved = rec["flops"] / rec["count"] / (avg_ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS["fp32"] if name.endswith("f32") else (
+        PEAK_TFLOPS["bf16x3"] if "bf16" in name else PEAK_TFLOPS["fp64"])
+    roof = {"kernel": name, "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4), "traffic": None, "avg_ms": round(avg_ms, 4),
+            "launches": rec["count"]}
+    heev = [v for k, v in prof_mix.items() if k.startswith("heev")]
+    heev_ms = sum(v["ms"] for v in heev) / opt.steps
+    eigs_per_step = 2 * nk
+
+    cpu = None
+    if not opt.no_cpu:
+        with tempfile.TemporaryDirectory() as td:
+            cpu = cpu_baseline(td)
+
+    out = {
+        "metric": "chi(q,w) k-pts/s mixed vs FP64 GPU at eps_M rel err <=1e-4; eigensolves/s",
+        "value": round(nk / t_mix, 2),
+        "unit": "k-pts/s",
+        "n_gpus": 1,
+        "steps": opt.steps,
+        "warmup": opt.warmup,
+        "ms_per_step": round(1e3 * t_mix, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 rank update + f64 eigensolve" if opt.precision == "fp32" else "bf16x3 rank update + f64 eigensolve",
+        "data": "synthetic (Gamma-centred k-grid, published Si form factors)",
+        "config": {"workload": WORKLOAD, "precision": opt.precision, "nk": nk, "nq": 1},
+        "fp64_value": round(nk / t_64, 2),
+        "fp64_ms_per_step": round(1e3 * t_64, 3),
+        "speedup_vs_fp64": round(t_64 / t_mix, 3),
+        "epsM_rel_err": eps_err,
+        "elf_rel_err": elf_err,
+        "im_epsM_rel_err": im_err,
+        "eigensolves_per_s": round(eigs_per_step / (heev_ms * 1e-3), 1) if heev_ms > 0 else None,
+        "kernels_ms_per_step": {k: round(v["ms"] / opt.steps, 3) for k, v in prof_mix.items()},
+        "fp64_kernels_ms_per_step": {k: round(v["ms"] / opt.fp64_steps, 3) for k, v in prof_64.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

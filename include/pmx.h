@@ -122,6 +122,18 @@ int pmx_eig_batch(pmx_ctx *ctx, int nk, const double *K, int nband, double *eval
  * [i][j][q][m][n<=m][f] of export_Xij (src/pmx_io.cpp:720-741). */
 int pmx_chi(pmx_ctx *ctx, const pmx_chi_params *p, double *ReX, double *ImX);
 
+/* pmx_chi with ReX / ImX in device memory (hipMalloc'd by the caller, same
+ * layout and sizes): nothing crosses PCIe but the k / q lists. */
+int pmx_chi_device(pmx_ctx *ctx, const pmx_chi_params *p, double *dReX, double *dImX);
+
+/* Per-kernel timing with HIP events on the context's stream (no reference
+ * counterpart).  enable != 0 starts recording (and clears), 0 stops.
+ * pmx_profile_report writes a JSON object
+ * {"kernel": {"count": n, "ms": total, "flops": f, "bytes": b}, ...}
+ * into buf (NUL-terminated, truncated to len). */
+int pmx_profile(pmx_ctx *ctx, int enable);
+int pmx_profile_report(pmx_ctx *ctx, char *buf, int len);
+
 /* kramerskronigtransform (src/pmx_chi.cpp:272-309) applied in place to nel
  * contiguous elements of nfreq points each (w_f = f * dfreq). */
 int pmx_kk(pmx_ctx *ctx, int nel, int nfreq, double dfreq, double *ReX, double *ImX);

@@ -21,7 +21,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 # every symbol include/pmx.h declares
 EXPORTS = ["pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_set_system",
            "pmx_set_form_factors", "pmx_get_loci", "pmx_ref_energy", "pmx_bands", "pmx_eig_batch",
-           "pmx_chi", "pmx_kk", "pmx_epsinv", "pmx_grad_bands", "pmx_input_parse", "pmx_input_free",
+           "pmx_chi", "pmx_chi_device", "pmx_profile", "pmx_profile_report", "pmx_kk", "pmx_epsinv", "pmx_grad_bands", "pmx_input_parse", "pmx_input_free",
            "pmx_input_system", "pmx_input_info", "pmx_generate_gpt", "pmx_run"]
 
 
@@ -80,6 +80,9 @@ def lib():
     L.pmx_bands.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, _dp]
     L.pmx_eig_batch.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
     L.pmx_chi.argtypes = [vp, ctypes.POINTER(PmxChiParams), _dp, _dp]
+    L.pmx_chi_device.argtypes = [vp, ctypes.POINTER(PmxChiParams), ctypes.c_void_p, ctypes.c_void_p]
+    L.pmx_profile.argtypes = [vp, ctypes.c_int]
+    L.pmx_profile_report.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
     L.pmx_kk.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp]
     L.pmx_epsinv.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
     L.pmx_grad_bands.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
@@ -219,6 +222,31 @@ class Context:
         im = np.zeros(n)
         _check(lib().pmx_chi(self.h, ctypes.byref(p), _p(re), _p(im)))
         return re, im
+
+    def chi_device(self, params, dre_ptr, dim_ptr):
+        """params: PmxChiParams (see chi_params); dre_ptr/dim_ptr: device addresses."""
+        _check(lib().pmx_chi_device(self.h, ctypes.byref(params), ctypes.c_void_p(dre_ptr),
+                                    ctypes.c_void_p(dim_ptr)))
+
+    def chi_params(self, K, KW, Q, nband, offset, delta, epsilon, nfreq, dfreq, kk=0, precision="fp64"):
+        """PmxChiParams whose arrays stay alive as attributes of the returned object."""
+        K = _f64(K).reshape(-1, 3)
+        KW = _f64(KW).reshape(-1)
+        Q = _f64(Q).reshape(-1, 3)
+        p = PmxChiParams(len(K), _p(K), _p(KW), len(Q), _p(Q), int(nband), float(offset), int(delta),
+                         float(epsilon), int(nfreq), float(dfreq), int(kk),
+                         PREC[precision] if isinstance(precision, str) else int(precision))
+        p._keep = (K, KW, Q)
+        return p
+
+    def profile(self, enable=True):
+        _check(lib().pmx_profile(self.h, 1 if enable else 0))
+
+    def profile_report(self):
+        import json
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(lib().pmx_profile_report(self.h, buf, len(buf)))
+        return json.loads(buf.value.decode())
 
     def kk(self, ReX, ImX, dfreq):
         re = _f64(ReX).copy()

@@ -126,6 +126,35 @@ int rebuild_vtable(pmx_ctx *c) {
 
 using namespace pmx;
 
+Prof::Prof(pmx_ctx *ctx, const char *name, double flops, double bytes) : c(ctx) {
+    if (!c || !c->prof) return;
+    pmx_ctx::ProfEvent e{name, nullptr, nullptr, flops, bytes};
+    if (hipEventCreate(&e.a) != hipSuccess || hipEventCreate(&e.b) != hipSuccess) return;
+    (void)hipEventRecord(e.a, c->stream);
+    c->prof_pending.push_back(e);
+    idx = (int)c->prof_pending.size() - 1;
+}
+
+Prof::~Prof() {
+    if (idx >= 0) (void)hipEventRecord(c->prof_pending[idx].b, c->stream);
+}
+
+static void prof_drain(pmx_ctx *c) {
+    for (auto &e : c->prof_pending) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(e.b) == hipSuccess && hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+            auto &acc = c->prof_acc[e.name];
+            acc.count++;
+            acc.ms += ms;
+            acc.flops += e.flops;
+            acc.bytes += e.bytes;
+        }
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    c->prof_pending.clear();
+}
+
 std::complex<double> pmx_ctx::vso_chi_at(double qx, double qy, double qz) const {
     double q2 = qx * qx + qy * qy + qz * qz;
     double la = soc_lookup(ua_k, ua_v, q2), ls = soc_lookup(us_k, us_v, q2);
@@ -319,6 +348,34 @@ int pmx_set_system(pmx_ctx *c, const pmx_system *s) {
     PMX_CHECK_HIP(hipMemcpyAsync(c->loci_h.data(), c->dLoci.p, sizeof(int) * s->neps * n, hipMemcpyDeviceToHost, c->stream));
     PMX_CHECK_HIP(hipStreamSynchronize(c->stream));
     c->has_system = true;
+    return PMX_OK;
+}
+
+int pmx_profile(pmx_ctx *c, int enable) {
+    if (!c) return set_error(PMX_EINVAL, "null context");
+    PMX_CHECK_HIP(hipSetDevice(c->device));
+    prof_drain(c);
+    if (enable) c->prof_acc.clear();
+    c->prof = enable != 0;
+    return PMX_OK;
+}
+
+int pmx_profile_report(pmx_ctx *c, char *buf, int len) {
+    if (!c || !buf || len <= 0) return set_error(PMX_EINVAL, "null pointer");
+    PMX_CHECK_HIP(hipSetDevice(c->device));
+    prof_drain(c);
+    std::string js = "{";
+    bool first = true;
+    char tmp[512];
+    for (auto &kv : c->prof_acc) {
+        std::snprintf(tmp, sizeof(tmp), "%s\"%s\": {\"count\": %ld, \"ms\": %.9g, \"flops\": %.9g, \"bytes\": %.9g}",
+                      first ? "" : ", ", kv.first.c_str(), kv.second.count, kv.second.ms, kv.second.flops,
+                      kv.second.bytes);
+        js += tmp;
+        first = false;
+    }
+    js += "}";
+    std::snprintf(buf, len, "%s", js.c_str());
     return PMX_OK;
 }
 

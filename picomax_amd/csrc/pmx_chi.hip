@@ -47,7 +47,10 @@ __device__ __forceinline__ zd cmul(zd a, zd b) { return zd{a.x * b.x - a.y * b.y
 __device__ __forceinline__ zd cmulc(zd a, zd b) { return zd{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y}; } /* a conj(b) */
 __device__ __forceinline__ zd cconj(zd a) { return zd{a.x, -a.y}; }
 
-constexpr int MAX_NT = 20;
+typedef double dx4 __attribute__((ext_vector_type(4)));
+typedef float fx4 __attribute__((ext_vector_type(4)));
+
+constexpr int MAX_NT = 20; /* 4 waves x 5 orders in the MFMA kernel */
 constexpr int TILE = 16;
 
 /* ------------------------------------------------------------------
@@ -94,19 +97,40 @@ __global__ void classify_kernel(TransArgs a) {
 struct RhoArgs {
     int nk, N, npw, neps, nb_s, nband, spatial, nch;
     const zd *vk, *vkq;       /* [nk][nb_s][N] */
-    const double *evk, *evkq; /* [nk][nb_s] */
-    double offset;
-    const int *nc, *nv, *tbase; /* spinor counts, item base per k (in transitions) */
+    const int *nc, *nv, *tbase; /* spinor counts, transition base per k */
+    const int *pos;           /* transition -> sorted row, -1 if dropped */
     const int *loci;          /* [neps][npw] */
     const zd *wA;             /* [neps][3][npw] conj(u_mi).(G_p + Q/2 + G_m/2) */
     const zd *uc;             /* [neps][3][3] conj(u_mi,d) */
     const zd *wS;             /* [neps][3][npw] conj(u_mi).socCurrent(G_p, G_loci) (SOC only) */
     const double *K;          /* [nk][3] */
-    const double *KW;         /* [nk] */
-    zd *X;                    /* items */
-    double *dE;               /* [transitions] */
-    double *wt;               /* [transitions] kw (x2 spatial) */
+    zd *X;                    /* FP64 rows (or null) */
+    float2 *Xf;               /* FP32 rows (or null) */
 };
+
+__device__ __forceinline__ void put_item(const RhoArgs &a, size_t row, int ch, int col, zd v) {
+    const size_t idx = (row * a.nch + ch) * (size_t)(3 * a.neps) + col;
+    if (a.Xf) a.Xf[idx] = make_float2((float)v.x, (float)v.y);
+    else a.X[idx] = v;
+}
+
+/* transition energies dE = E_c(k) - E_v(k+q) and weights (src/pmx_chi.cpp:587-590) */
+__global__ void dE_kernel(int nk, int nb_s, int nband, int spatial, const double *evk, const double *evkq,
+                          double offset, const int *nc, const int *nv, const int *tbase, const double *KW,
+                          double *dE, double *wt) {
+    const int k = blockIdx.x;
+    const int ncs = spatial ? nc[k] / 2 : nc[k];
+    const int nvs = spatial ? nv[k] / 2 : nv[k];
+    const size_t tb = tbase[k];
+    for (int t = threadIdx.x; t < ncs * nvs; t += blockDim.x) {
+        const int c = t / nvs, v = t % nvs;
+        const int cst = spatial ? (nband / 2 - 1 - c) : (nband - 1 - c);
+        double ec = evk[(size_t)k * nb_s + cst] - offset;
+        double ev = evkq[(size_t)k * nb_s + v] - offset;
+        dE[tb + t] = ec - ev;
+        wt[tb + t] = KW[k] * (spatial ? 2.0 : 1.0);
+    }
+}
 
 __global__ void __launch_bounds__(256) rho_kernel(RhoArgs a) {
     const int k = blockIdx.x;
@@ -117,20 +141,12 @@ __global__ void __launch_bounds__(256) rho_kernel(RhoArgs a) {
     if (npair == 0) return;
     const size_t tb = a.tbase[k];
     const double Kx = a.K[3 * k], Ky = a.K[3 * k + 1], Kz = a.K[3 * k + 2];
-    const int D = 3 * a.neps;
     const zd *vk = a.vk + (size_t)k * a.nb_s * a.N;
     const zd *vkq = a.vkq + (size_t)k * a.nb_s * a.N;
-    /* energies and weights of the transitions */
-    for (int t = threadIdx.x; t < npair; t += blockDim.x) {
-        const int c = t / nvs, v = t % nvs;
-        int cst = a.spatial ? (a.nband / 2 - 1 - c) : (a.nband - 1 - c);
-        double ec = a.evk[(size_t)k * a.nb_s + cst] - a.offset;
-        double ev = a.evkq[(size_t)k * a.nb_s + v] - a.offset;
-        a.dE[tb + t] = ec - ev;
-        a.wt[tb + t] = a.KW[k] * (a.spatial ? 2.0 : 1.0);
-    }
     for (int job = threadIdx.x; job < npair * a.neps; job += blockDim.x) {
         const int m = job / npair, t = job % npair;
+        const int row = a.pos[tb + t];
+        if (row < 0) continue;
         const int c = t / nvs, v = t % nvs;
         const int cst = a.spatial ? (a.nband / 2 - 1 - c) : (a.nband - 1 - c);
         const zd *ck = vk + (size_t)cst * a.N;
@@ -151,13 +167,11 @@ __global__ void __launch_bounds__(256) rho_kernel(RhoArgs a) {
                 R2.x += t2.x; R2.y += t2.y;
             }
             zd R[3] = {R0, R1, R2};
-            zd *out = a.X + (tb + t) * (size_t)D;
             for (int i = 0; i < 3; i++) {
                 const zd *u = a.uc + ((size_t)m * 3 + i) * 3;
                 zd bk{u[0].x * Kx + u[1].x * Ky + u[2].x * Kz, u[0].y * Kx + u[1].y * Ky + u[2].y * Kz};
-                zd r = R[i];
                 zd bs = cmul(bk, S);
-                out[3 * m + i] = zd{r.x + bs.x, r.y + bs.y};
+                put_item(a, row, 0, 3 * m + i, zd{R[i].x + bs.x, R[i].y + bs.y});
             }
         } else {
             /* spinor rows 2p (up), 2p+1 (down) */
@@ -174,25 +188,25 @@ __global__ void __launch_bounds__(256) rho_kernel(RhoArgs a) {
                 Su.x += puu.x; Su.y += puu.y;
                 Sd.x += pdd.x; Sd.y += pdd.y;
                 for (int i = 0; i < 3; i++) {
-                    zd w = wA[i * a.npw + p], s = wS[i * a.npw + p];
-                    zd t1 = cmul(puu, w), t2 = cmul(pdd, w), t3 = cmul(pud, s), t4 = cmul(pdu, s);
+                    zd w = wA[i * a.npw + p], sw = wS[i * a.npw + p];
+                    zd t1 = cmul(puu, w), t2 = cmul(pdd, w), t3 = cmul(pud, sw), t4 = cmul(pdu, sw);
                     Ru[i].x += t1.x; Ru[i].y += t1.y;
                     Rd[i].x += t2.x; Rd[i].y += t2.y;
                     Qud[i].x += t3.x; Qud[i].y += t3.y;
                     Qdu[i].x += t4.x; Qdu[i].y += t4.y;
                 }
             }
-            zd *out = a.X + (tb + t) * (size_t)4 * D;
             for (int i = 0; i < 3; i++) {
                 const zd *u = a.uc + ((size_t)m * 3 + i) * 3;
                 zd bk{u[0].x * Kx + u[1].x * Ky + u[2].x * Kz, u[0].y * Kx + u[1].y * Ky + u[2].y * Kz};
                 zd bu = cmul(bk, Su), bd = cmul(bk, Sd);
                 zd up{Ru[i].x + bu.x, Ru[i].y + bu.y}, dn{Rd[i].x + bd.x, Rd[i].y + bd.y};
                 zd ud = Qud[i], du = Qdu[i];
-                out[0 * D + 3 * m + i] = zd{up.x + dn.x, up.y + dn.y}; /* A */
-                out[1 * D + 3 * m + i] = zd{ud.x + du.x, ud.y + du.y}; /* B */
-                out[2 * D + 3 * m + i] = zd{up.x - dn.x, up.y - dn.y}; /* C */
-                out[3 * D + 3 * m + i] = zd{ud.x - du.x, ud.y - du.y}; /* E */
+                const int col = 3 * m + i;
+                put_item(a, row, 0, col, zd{up.x + dn.x, up.y + dn.y}); /* A */
+                put_item(a, row, 1, col, zd{ud.x + du.x, ud.y + du.y}); /* B */
+                put_item(a, row, 2, col, zd{up.x - dn.x, up.y - dn.y}); /* C */
+                put_item(a, row, 3, col, zd{ud.x - du.x, ud.y - du.y}); /* E */
             }
         }
     }
@@ -221,24 +235,23 @@ __global__ void scatter_kernel(int nt, const int *bin, int *cursor, int *perm) {
     perm[pos] = t;
 }
 
-/* ------------------------------------------------------------------
- * moments: M[b][n][tile pair][16x16] += sum_{t in b} w s^n X_a conj(X_a')
- * (SIMT FP64 reference kernel; one block per (tile pair, bin))
- * Items: for SOC each transition carries channels ch (A, B, C, E): M1 takes
- * A (+) and B (-), M2 takes C (+) and E (-).
- * ------------------------------------------------------------------ */
-struct MomArgs {
-    int D, nch, ntile, npairs, nbins, NT;
-    const zd *X;
-    const double *dE, *wt;
-    const int *perm, *start; /* sorted transition ids; start[b] .. start[b+1] */
-    double w0, df;
-    int acc_index;           /* 0: M1, 1: M2 */
-    zd *M;                   /* [nbins][NT][npairs][256] */
-};
+/* sorted-row attributes: pos[t], weight, and the scaled offset s of dE from
+ * its bin centre w0 + (f0 + 1/2) df, in units of df/2 */
+__global__ void sorted_attr_kernel(int nsorted, const int *perm, const double *dE, const double *wt, double w0,
+                                   double df, int *pos, double *ws, double *ss, double *es) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsorted) return;
+    const int t = perm[i];
+    pos[t] = i;
+    const double e = dE[t];
+    const int f0 = int((e - w0) / df);
+    ws[i] = wt[t];
+    es[i] = e;
+    ss[i] = (e - (w0 + (f0 + 0.5) * df)) / (0.5 * df);
+}
 
+/* tile pair index over ti >= tj, row-major in ti */
 __device__ __forceinline__ void tile_pair(int idx, int &ti, int &tj) {
-    /* idx over ti >= tj, row-major in ti */
     int t = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
     while ((t + 1) * (t + 2) / 2 <= idx) t++;
     while (t * (t + 1) / 2 > idx) t--;
@@ -246,76 +259,112 @@ __device__ __forceinline__ void tile_pair(int idx, int &ti, int &tj) {
     tj = idx - t * (t + 1) / 2;
 }
 
-__global__ void __launch_bounds__(256) moment_simt_kernel(MomArgs a) {
+/* ------------------------------------------------------------------
+ * moments on MFMA.  Block = 4 waves = one 16x16 output tile (tile pair) of one
+ * bin; wave w owns orders n in [w*NPW, w*NPW + NPW).  Real embedding of the
+ * complex rank-k update over sorted rows r of the bin (K = 2 per row):
+ *   Re M += A Bre,  Im M += A Bim,  A[a][(r,0)] = Re x_a, A[a][(r,1)] = Im x_a,
+ *   Bre[(r,0)][b] = w Re x_b, Bre[(r,1)][b] = w Im x_b,
+ *   Bim[(r,0)][b] = -w Im x_b, Bim[(r,1)][b] = w Re x_b,  times s_r^n.
+ * v_mfma_*_16x16x4 lane layout: A(i = l%16, k = l/16), B(k = l/16, j = l%16);
+ * D(j = l%16) with i = 4(l/16) + v for f32 and i = l/16 + 4v for f64
+ * (measured on gfx950).
+ * ------------------------------------------------------------------ */
+
+template <class T> struct MfmaT;
+template <> struct MfmaT<double> {
+    typedef dx4 V;
+    typedef zd C;
+    static __device__ __forceinline__ V mma(double a, double b, V c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <> struct MfmaT<float> {
+    typedef fx4 V;
+    typedef float2 C;
+    static __device__ __forceinline__ V mma(float a, float b, V c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+struct MomArgs2 {
+    int D, nch, npairs, nbins, NT, npw_wave, acc_index;
+    const void *X;          /* sorted rows [row][nch][D] complex of T */
+    const double *ws, *ss;  /* sorted rows */
+    const int *start;
+    zd *M;                  /* [nbins][NT][npairs][256] FP64 (accumulated) */
+};
+
+template <class T> __global__ void __launch_bounds__(256) moment_mfma_kernel(MomArgs2 a) {
+    typedef typename MfmaT<T>::V V;
+    typedef typename MfmaT<T>::C C;
+    constexpr int MAXW = 5;
     const int pairi = blockIdx.x, b = blockIdx.y;
-    const int t0 = a.start[b], t1 = a.start[b + 1];
-    if (t0 == t1) return;
+    const int r0 = a.start[b], r1 = a.start[b + 1];
+    if (r0 == r1) return;
     int ti, tj;
     tile_pair(pairi, ti, tj);
-    const int r = threadIdx.x / TILE, cc = threadIdx.x % TILE;
-    const int ar = ti * TILE + r, ac = tj * TILE + cc;
-    __shared__ zd xa[32][TILE], xb[32][TILE];
-    __shared__ double ws[32], ss[32];
-    zd acc[MAX_NT];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n0 = wave * a.npw_wave;
+    const int nw = min(a.npw_wave, a.NT - n0);
+    if (nw <= 0) return;
+    const int ia = ti * TILE + (lane & 15), ib = tj * TILE + (lane & 15);
+    const int kq = lane >> 4;
+    const C *X = reinterpret_cast<const C *>(a.X);
+    const int nitems = (a.nch == 1) ? 1 : 2; /* SOC: (+) and (-) channel per accumulator */
+    V accr[MAXW], acci[MAXW];
 #pragma unroll
-    for (int n = 0; n < MAX_NT; n++) acc[n] = zd{0, 0};
-    const double h = 0.5 * a.df;
-    const double cb = a.w0 + (b - 0 + 0.5) * a.df; /* centre of bin b relative to the wh shift below */
-    (void)cb;
-    const int nsig = 2; /* channels per accumulator in SOC: (+, -) */
-    for (int base = t0; base < t1; base += 32 / (a.nch == 1 ? 1 : nsig)) {
-        /* stage up to 32 items: spatial = 32 transitions, SOC = 16 transitions x 2 channels */
-        const int per = (a.nch == 1) ? 32 : 16;
-        const int cnt = min(per, t1 - base);
-        for (int e = threadIdx.x; e < 32 * TILE * 2; e += blockDim.x) {
-            int item = e / (2 * TILE), which = (e / TILE) & 1, col = e % TILE;
-            int tl = (a.nch == 1) ? item : item / 2;
-            int ch = (a.nch == 1) ? 0 : (2 * a.acc_index + (item & 1));
-            zd v{0, 0};
-            if (tl < cnt) {
-                int t = a.perm[base + tl];
-                int aa = (which == 0 ? ti : tj) * TILE + col;
-                if (aa < a.D) v = a.X[((size_t)t * a.nch + ch) * a.D + aa];
-            }
-            if (which == 0) xa[item][col] = v; else xb[item][col] = v;
-        }
-        for (int e = threadIdx.x; e < 32; e += blockDim.x) {
-            int tl = (a.nch == 1) ? e : e / 2;
-            double w = 0.0, s = 0.0;
-            if (tl < cnt) {
-                int t = a.perm[base + tl];
-                w = a.wt[t];
-                if (a.nch != 1 && (e & 1)) w = -w;
-                int f0 = int((a.dE[t] - a.w0) / a.df);
-                double cen = a.w0 + (f0 + 0.5) * a.df;
-                s = (a.dE[t] - cen) / h;
-            }
-            ws[e] = w;
-            ss[e] = s;
-        }
-        __syncthreads();
-        const int nitems = (a.nch == 1) ? cnt : 2 * cnt;
-        for (int it = 0; it < nitems; it++) {
-            zd p = cmulc(xa[it][r], xb[it][cc]);
-            double w = ws[it], s = ss[it];
-            zd q{p.x * w, p.y * w};
-#pragma unroll
-            for (int n = 0; n < MAX_NT; n++) {
-                if (n < a.NT) {
-                    acc[n].x += q.x;
-                    acc[n].y += q.y;
-                    q.x *= s;
-                    q.y *= s;
-                }
-            }
-        }
-        __syncthreads();
+    for (int j = 0; j < MAXW; j++) {
+        accr[j] = V{0, 0, 0, 0};
+        acci[j] = V{0, 0, 0, 0};
     }
-    if (ar < a.D && ac < a.D) {
-        for (int n = 0; n < a.NT; n++) {
-            zd *dst = a.M + (((size_t)b * a.NT + n) * a.npairs + pairi) * 256 + threadIdx.x;
-            dst->x += acc[n].x;
-            dst->y += acc[n].y;
+    const int K2 = 2 * nitems; /* K entries per row */
+    const int kend = (r1 - r0) * K2;
+    for (int kb = 0; kb < kend; kb += 4) {
+        const int kk = kb + kq;
+        T av = 0, bre = 0, bim = 0, sv = 0;
+        if (kk < kend) {
+            const int row = r0 + kk / K2;
+            const int sub = kk % K2;
+            const int item = sub >> 1, part = sub & 1;
+            const int ch = (a.nch == 1) ? 0 : (2 * a.acc_index + item);
+            const C *xr = X + ((size_t)row * a.nch + ch) * a.D;
+            C xa = (ia < a.D) ? xr[ia] : C{0, 0};
+            C xb = (ib < a.D) ? xr[ib] : C{0, 0};
+            T w = (T)a.ws[row];
+            if (item) w = -w;
+            sv = (T)a.ss[row];
+            av = part ? xa.y : xa.x;
+            bre = w * (part ? xb.y : xb.x);
+            bim = w * (part ? xb.x : -xb.y);
+            T sc = 1;
+            for (int j = 0; j < n0; j++) sc *= sv;
+            bre *= sc;
+            bim *= sc;
+        }
+#pragma unroll
+        for (int j = 0; j < MAXW; j++) {
+            if (j < nw) {
+                accr[j] = MfmaT<T>::mma(av, bre, accr[j]);
+                acci[j] = MfmaT<T>::mma(av, bim, acci[j]);
+                bre *= sv;
+                bim *= sv;
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < MAXW; j++) {
+        if (j < nw) {
+            zd *dst = a.M + (((size_t)b * a.NT + n0 + j) * a.npairs + pairi) * 256;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = MfmaT<T>::drow(lane, v), col = lane & 15;
+                zd &d = dst[row * TILE + col];
+                d.x += (double)accr[j][v];
+                d.y += (double)acci[j][v];
+            }
         }
     }
 }
@@ -327,10 +376,10 @@ __global__ void __launch_bounds__(256) moment_simt_kernel(MomArgs a) {
 struct DirArgs {
     int D, nch, npairs, nbins, wh, nfreq, delta, acc_index;
     double eps, w0, df;
-    const zd *X;
-    const double *dE, *wt;
-    const int *perm, *start;
-    zd *Z; /* [nfreq][npairs][256] */
+    const zd *X;              /* sorted FP64 rows */
+    const double *es, *ws;    /* sorted dE and weights */
+    const int *start;
+    zd *Z;                    /* [nfreq][npairs][256] */
 };
 
 __device__ double delta_fn(int code, double eps, double x) {
@@ -355,12 +404,12 @@ __global__ void __launch_bounds__(256) direct_kernel(DirArgs a) {
     const int f0 = b - a.wh;
     const int fmin = max(0, f0 - a.wh), fmax = min(a.nfreq - 1, f0 + a.wh);
     __shared__ zd xa[32][TILE], xb[32][TILE];
-    __shared__ double ws[32], de[32];
+    __shared__ double ws[32];
+    const int per = (a.nch == 1) ? 32 : 16;
     for (int f = fmin; f <= fmax; f++) {
         zd acc{0, 0};
         const double wf = a.w0 + a.df * f;
-        for (int base = t0; base < t1; base += (a.nch == 1 ? 32 : 16)) {
-            const int per = (a.nch == 1) ? 32 : 16;
+        for (int base = t0; base < t1; base += per) {
             const int cnt = min(per, t1 - base);
             __syncthreads();
             for (int e = threadIdx.x; e < 32 * TILE * 2; e += blockDim.x) {
@@ -369,23 +418,20 @@ __global__ void __launch_bounds__(256) direct_kernel(DirArgs a) {
                 int ch = (a.nch == 1) ? 0 : (2 * a.acc_index + (item & 1));
                 zd v{0, 0};
                 if (tl < cnt) {
-                    int t = a.perm[base + tl];
                     int aa = (which == 0 ? ti : tj) * TILE + col;
-                    if (aa < a.D) v = a.X[((size_t)t * a.nch + ch) * a.D + aa];
+                    if (aa < a.D) v = a.X[((size_t)(base + tl) * a.nch + ch) * a.D + aa];
                 }
                 if (which == 0) xa[item][col] = v; else xb[item][col] = v;
             }
             for (int e = threadIdx.x; e < 32; e += blockDim.x) {
                 int tl = (a.nch == 1) ? e : e / 2;
-                double w = 0.0, d = 0.0;
+                double w = 0.0;
                 if (tl < cnt) {
-                    int t = a.perm[base + tl];
-                    w = a.wt[t];
+                    w = a.ws[base + tl];
                     if (a.nch != 1 && (e & 1)) w = -w;
-                    d = a.dE[t];
+                    w *= delta_fn(a.delta, a.eps, a.es[base + tl] - wf);
                 }
-                ws[e] = w * delta_fn(a.delta, a.eps, d - wf);
-                de[e] = d;
+                ws[e] = w;
             }
             __syncthreads();
             const int nitems = (a.nch == 1) ? cnt : 2 * cnt;
@@ -494,7 +540,7 @@ __global__ void pack_kernel(PackArgs a) {
         double rev = a.soc ? -0.5 * (z1.y + zfetch(a.Z2, f, a.npairs, ra, rb).y) : -z1.y;
         const double w = a.w0 + a.dfreq * f;
         const double inv = (f == 0) ? 0.0 : 1.0 / (w * w);
-        const size_t dst = (((size_t)ij * a.nq + a.q) * ne2 + e2) * a.nfreq + f;
+        const size_t dst = (((size_t)ij * a.nq + a.q) * ne2 + e2) * a.nfreq + f; /* nq = 1, q = 0: compact */
         a.ImX[dst] = a.SF * (imv * inv);
         a.ReX[dst] = a.SF * (rev * inv);
     }
@@ -502,22 +548,142 @@ __global__ void pack_kernel(PackArgs a) {
 
 /* kramerskronigtransform (src/pmx_chi.cpp:272-309), one thread per (element, f):
  * ReX += ds sum_{s!=f} w_s ImX_s/(w_s^2 - w_f^2), ImX += ds sum_{s!=f} w_s ReX0_s/(..) */
-__global__ void kk_kernel(int nel, int nfreq, const double *Kk, const double *Re0, const double *Im0, double *Re,
-                          double *Im) {
+/* KK as an FP64 MFMA GEMM over the frequency axis: with the fixed matrix
+ * Kk[s][f] = ds w_s / (w_s^2 - w_f^2) (s != f), per element row
+ *   Re'[f] = Re0[f] + sum_s Im0[s] Kk[s][f],  Im'[f] = Im0[f] + sum_s Re0[s] Kk[s][f].
+ * Block = 4 waves, each a 16-row x 16-frequency tile of both outputs. */
+__global__ void __launch_bounds__(256) kk_gemm_kernel(int M, int NF, const double *Re0, const double *Im0,
+                                                      const double *Kk, double *Re, double *Im, int ne2, int nq,
+                                                      int q) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row0 = (blockIdx.y * 4 + wave) * 16, col0 = blockIdx.x * 16;
+    if (row0 >= M) return;
+    const int ar = row0 + (lane & 15), kq = lane >> 4, bc = col0 + (lane & 15);
+    dx4 accR = {0, 0, 0, 0}, accI = {0, 0, 0, 0};
+    for (int k0 = 0; k0 < NF; k0 += 4) {
+        const int k = k0 + kq;
+        double a1 = 0.0, a2 = 0.0, b = 0.0;
+        if (k < NF) {
+            if (ar < M) {
+                a1 = Im0[(size_t)ar * NF + k];
+                a2 = Re0[(size_t)ar * NF + k];
+            }
+            if (bc < NF) b = Kk[(size_t)k * NF + bc];
+        }
+        accR = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b, accR, 0, 0, 0);
+        accI = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b, accI, 0, 0, 0);
+    }
+    for (int v = 0; v < 4; v++) {
+        const int r = row0 + (lane >> 4) + 4 * v, cidx = col0 + (lane & 15); /* f64 D layout */
+        if (r >= M || cidx >= NF) continue;
+        const size_t ij = r / ne2, e2 = r % ne2;
+        const size_t o = ((ij * nq + q) * ne2 + e2) * NF + cidx;
+        Re[o] = Re0[(size_t)r * NF + cidx] + accR[v];
+        Im[o] = Im0[(size_t)r * NF + cidx] + accI[v];
+    }
+}
+
+__global__ void place_kernel(int nel, int nfreq, int ne2, int nq, int q, const double *Re0, const double *Im0,
+                             double *Re, double *Im) {
     const size_t total = (size_t)nel * nfreq;
     for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
         const int f = idx % nfreq;
         const size_t el = idx / nfreq;
-        const double *r0 = Re0 + el * nfreq, *i0 = Im0 + el * nfreq;
-        double ar = 0.0, ai = 0.0;
-        for (int s = 0; s < nfreq; s++) {
-            double kv = Kk[(size_t)s * nfreq + f];
-            ar += kv * i0[s];
-            ai += kv * r0[s];
-        }
-        Re[idx] = r0[f] + ar;
-        Im[idx] = i0[f] + ai;
+        const size_t ij = el / ne2, e2 = el % ne2;
+        const size_t o = ((ij * nq + q) * ne2 + e2) * nfreq + f;
+        Re[o] = Re0[idx];
+        Im[o] = Im0[idx];
     }
+}
+
+/* eps_mn = delta_mn + chi^{00}_mn (n <= m stored, eps_nm = conj(eps_mn)),
+ * Gauss-Jordan with partial pivoting on [eps | I], one block per (q, f);
+ * epsM = 1/[eps^-1]_00, ELF = -Im [eps^-1]_00 (macroscopic.m:23-38, 87-89) */
+__global__ void __launch_bounds__(256) epsinv_kernel(int N, int nq, int nfreq, const double *ReX, const double *ImX,
+                                                     zd *scratch, zd *epsM, double *elf, zd *inv_out) {
+    const int mat = blockIdx.x;
+    const int q = mat / nfreq, f = mat % nfreq;
+    const int ne2 = N * (N + 1) / 2;
+    const int W = 2 * N;
+    zd *A = scratch + (size_t)blockIdx.x * N * W;
+    __shared__ int piv;
+    __shared__ double best[256];
+    __shared__ int bidx[256];
+    for (int e = threadIdx.x; e < N * W; e += blockDim.x) {
+        const int r = e / W, cidx = e % W;
+        zd v{0, 0};
+        if (cidx < N) {
+            int m = r, n = cidx;
+            bool cj = false;
+            if (n > m) { int t = m; m = n; n = t; cj = true; }
+            const size_t src = ((size_t)q * ne2 + (size_t)m * (m + 1) / 2 + n) * nfreq + f; /* ij = 0 block */
+            v = zd{ReX[src], cj ? -ImX[src] : ImX[src]};
+            if (r == cidx) v.x += 1.0;
+        } else if (cidx - N == r) {
+            v = zd{1.0, 0.0};
+        }
+        A[e] = v;
+    }
+    __syncthreads();
+    for (int col = 0; col < N; col++) {
+        double bv = -1.0;
+        int bi = col;
+        for (int r = col + threadIdx.x; r < N; r += blockDim.x) {
+            zd v = A[(size_t)r * W + col];
+            double mag = v.x * v.x + v.y * v.y;
+            if (mag > bv) { bv = mag; bi = r; }
+        }
+        best[threadIdx.x] = bv;
+        bidx[threadIdx.x] = bi;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double b0 = -1.0;
+            int i0 = col;
+            for (int t = 0; t < blockDim.x; t++)
+                if (best[t] > b0) { b0 = best[t]; i0 = bidx[t]; }
+            piv = i0;
+        }
+        __syncthreads();
+        const int pr = piv;
+        if (pr != col)
+            for (int cidx = threadIdx.x; cidx < W; cidx += blockDim.x) {
+                zd t = A[(size_t)pr * W + cidx];
+                A[(size_t)pr * W + cidx] = A[(size_t)col * W + cidx];
+                A[(size_t)col * W + cidx] = t;
+            }
+        __syncthreads();
+        const zd pv = A[(size_t)col * W + col];
+        const double den = pv.x * pv.x + pv.y * pv.y;
+        const zd ip{pv.x / den, -pv.y / den};
+        __syncthreads();
+        for (int cidx = threadIdx.x; cidx < W; cidx += blockDim.x)
+            A[(size_t)col * W + cidx] = cmul(A[(size_t)col * W + cidx], ip);
+        __syncthreads();
+        for (int e = threadIdx.x; e < N * W; e += blockDim.x) {
+            const int r = e / W, cidx = e % W;
+            if (r == col) continue;
+            const zd fct = A[(size_t)r * W + col];
+            if (cidx == col) continue;
+            const zd t = cmul(fct, A[(size_t)col * W + cidx]);
+            A[e].x -= t.x;
+            A[e].y -= t.y;
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < N; r += blockDim.x)
+            if (r != col) A[(size_t)r * W + col] = zd{0, 0};
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const zd g = A[N]; /* [eps^-1]_00 */
+        const double den = g.x * g.x + g.y * g.y;
+        epsM[(size_t)q * nfreq + f] = zd{g.x / den, -g.y / den};
+        elf[(size_t)q * nfreq + f] = -g.y;
+    }
+    if (inv_out)
+        for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+            const int r = e / N, cidx = e % N;
+            inv_out[((size_t)mat * N + r) * N + cidx] = A[(size_t)r * W + N + cidx];
+        }
 }
 
 __global__ void kkmat_kernel(int nfreq, double dfreq, double *Kk) {
@@ -595,16 +761,18 @@ template <class T> int upload(hipStream_t s, DBuf<T> &b, const std::vector<T> &h
 
 struct ChiWork {
     DBuf<double> Kd, KQd, KWd, evk, evkq, dEd, wtd, Ktd, Kkd, Re0, Im0, ReD, ImD;
+    DBuf<double> ws, ss, es;
     DBuf<zd> vk, vkq, Xd, M1, M2, Z1, Z2, wAd, ucd, wSd;
-    DBuf<int> nc, nv, tbase, bind, hist, start, cursor, perm, bad;
+    DBuf<float2> Xf;
+    DBuf<int> nc, nv, tbase, bind, hist, start, cursor, perm, pos, bad;
 };
 
 } // namespace pmx
 
 using namespace pmx;
 
-extern "C" int pmx_chi(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double *ImX) {
-    if (!c || !p || !ReX || !ImX) return set_error(PMX_EINVAL, "null pointer");
+static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm) {
+    if (!c || !p || !dRe || !dIm) return set_error(PMX_EINVAL, "null pointer");
     if (!c->has_system) return set_error(PMX_ESTATE, "no system set");
     if (p->nfreq <= 0 || !(p->dfreq > 0)) return set_error(PMX_EINVAL, "nfreq > 0 and dfreq > 0 required");
     if (p->delta < 0 || p->delta > 3) return set_error(PMX_EINVAL, "delta must be 0..3");
@@ -628,7 +796,8 @@ extern "C" int pmx_chi(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double 
     int NT = -1;
     bool direct = p->precision == PMX_PREC_DIRECT || p->delta >= 2;
     if (!direct) {
-        NT = taylor_order(p->delta, p->epsilon, df, 1e-11);
+        NT = taylor_order(p->delta, p->epsilon, df,
+                          (p->precision == PMX_PREC_FP64) ? 1e-11 : 1e-6);
         if (NT < 0 || NT > MAX_NT) direct = true;
     }
     const int ntile = (D + TILE - 1) / TILE;
@@ -637,8 +806,9 @@ extern "C" int pmx_chi(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double 
     ChiWork w;
     int rc;
     if (p->nk == 0) {
-        std::fill(ReX, ReX + per_q * p->nq, 0.0);
-        std::fill(ImX, ImX + per_q * p->nq, 0.0);
+        PMX_CHECK_HIP(hipMemsetAsync(dRe, 0, sizeof(double) * per_q * p->nq, st));
+        PMX_CHECK_HIP(hipMemsetAsync(dIm, 0, sizeof(double) * per_q * p->nq, st));
+        PMX_CHECK_HIP(hipStreamSynchronize(st));
         return PMX_OK;
     }
     /* ---- phase A: eigenpairs at k (src/pmx_chi.cpp:379-413) */
@@ -665,9 +835,8 @@ extern "C" int pmx_chi(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double 
     if (p->kk) {
         if ((rc = w.Kkd.ensure((size_t)p->nfreq * p->nfreq))) return rc;
         hipLaunchKernelGGL(kkmat_kernel, dim3(256), dim3(256), 0, st, p->nfreq, df, w.Kkd.p);
-        if ((rc = w.Re0.ensure(per_q)) || (rc = w.Im0.ensure(per_q))) return rc;
     }
-    if ((rc = w.ReD.ensure(per_q)) || (rc = w.ImD.ensure(per_q))) return rc;
+    if ((rc = w.Re0.ensure(per_q)) || (rc = w.Im0.ensure(per_q))) return rc;
 
     for (int q = 0; q < p->nq; q++) {
         const double Qx = p->Q[3 * q], Qy = p->Q[3 * q + 1], Qz = p->Q[3 * q + 2];
@@ -760,25 +929,18 @@ extern "C" int pmx_chi(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double 
         }
         const int ntr = tb[p->nk];
         if ((rc = upload(st, w.tbase, tb))) return rc;
-        if ((rc = w.Xd.ensure((size_t)std::max(ntr, 1) * nch * D))) return rc;
-        if ((rc = w.dEd.ensure(std::max(ntr, 1))) || (rc = w.wtd.ensure(std::max(ntr, 1)))) return rc;
-
-        /* ---- rho */
-        RhoArgs ra{};
-        ra.nk = p->nk; ra.N = N; ra.npw = npw; ra.neps = neps; ra.nb_s = nb_s; ra.nband = p->nband;
-        ra.spatial = spatial ? 1 : 0; ra.nch = nch;
-        ra.vk = w.vk.p; ra.vkq = w.vkq.p; ra.evk = w.evk.p; ra.evkq = w.evkq.p; ra.offset = p->offset;
-        ra.nc = w.nc.p; ra.nv = w.nv.p; ra.tbase = w.tbase.p; ra.loci = c->dLoci.p;
-        ra.wA = w.wAd.p; ra.uc = w.ucd.p; ra.wS = spatial ? nullptr : w.wSd.p;
-        ra.K = w.Kd.p; ra.KW = w.KWd.p; ra.X = w.Xd.p; ra.dE = w.dEd.p; ra.wt = w.wtd.p;
-        hipLaunchKernelGGL(rho_kernel, dim3(p->nk), dim3(256), 0, st, ra);
-        PMX_CHECK_HIP(hipGetLastError());
-
-        /* ---- bin by f0 (counting sort) */
-        if ((rc = w.bind.ensure(std::max(ntr, 1))) || (rc = w.perm.ensure(std::max(ntr, 1))) ||
-            (rc = w.hist.ensure(nbins)) || (rc = w.start.ensure(nbins + 1)) || (rc = w.cursor.ensure(nbins)))
+        const int ntr1 = std::max(ntr, 1);
+        if ((rc = w.dEd.ensure(ntr1)) || (rc = w.wtd.ensure(ntr1)) || (rc = w.bind.ensure(ntr1)) ||
+            (rc = w.perm.ensure(ntr1)) || (rc = w.pos.ensure(ntr1)) || (rc = w.ws.ensure(ntr1)) ||
+            (rc = w.ss.ensure(ntr1)) || (rc = w.es.ensure(ntr1)) || (rc = w.hist.ensure(nbins)) ||
+            (rc = w.start.ensure(nbins + 1)) || (rc = w.cursor.ensure(nbins)))
             return rc;
+        hipLaunchKernelGGL(dE_kernel, dim3(p->nk), dim3(256), 0, st, p->nk, nb_s, p->nband, spatial ? 1 : 0, w.evk.p,
+                           w.evkq.p, p->offset, w.nc.p, w.nv.p, w.tbase.p, w.KWd.p, w.dEd.p, w.wtd.p);
+
+        /* ---- bin by f0 (counting sort); rows of a bin become contiguous */
         PMX_CHECK_HIP(hipMemsetAsync(w.hist.p, 0, sizeof(int) * nbins, st));
+        PMX_CHECK_HIP(hipMemsetAsync(w.pos.p, 0xff, sizeof(int) * ntr1, st));
         if (ntr > 0)
             hipLaunchKernelGGL(bin_kernel, dim3((ntr + 255) / 256), dim3(256), 0, st, ntr, w.dEd.p, w0, df, wh, nbins,
                                w.bind.p, w.hist.p);
@@ -787,20 +949,49 @@ extern "C" int pmx_chi(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double 
         PMX_CHECK_HIP(hipStreamSynchronize(st));
         start[0] = 0;
         for (int b = 0; b < nbins; b++) start[b + 1] = start[b] + hist[b];
+        const int nsorted = start[nbins];
         if ((rc = upload(st, w.start, start))) return rc;
         PMX_CHECK_HIP(hipMemcpyAsync(w.cursor.p, w.start.p, sizeof(int) * nbins, hipMemcpyDeviceToDevice, st));
         if (ntr > 0)
             hipLaunchKernelGGL(scatter_kernel, dim3((ntr + 255) / 256), dim3(256), 0, st, ntr, w.bind.p, w.cursor.p,
                                w.perm.p);
-        /* the order within a bin only changes the summation order; sort it
-         * on the host for run-to-run determinism */
-        std::vector<int> perm(start[nbins]);
-        if (!perm.empty()) {
-            PMX_CHECK_HIP(hipMemcpyAsync(perm.data(), w.perm.p, sizeof(int) * perm.size(), hipMemcpyDeviceToHost, st));
+        if (nsorted > 0) {
+            /* transition order inside a bin only reorders the sum; sorting
+             * it keeps the result identical run to run */
+            std::vector<int> perm(nsorted);
+            PMX_CHECK_HIP(hipMemcpyAsync(perm.data(), w.perm.p, sizeof(int) * nsorted, hipMemcpyDeviceToHost, st));
             PMX_CHECK_HIP(hipStreamSynchronize(st));
             for (int b = 0; b < nbins; b++) std::sort(perm.begin() + start[b], perm.begin() + start[b + 1]);
             if ((rc = upload(st, w.perm, perm))) return rc;
+            hipLaunchKernelGGL(sorted_attr_kernel, dim3((nsorted + 255) / 256), dim3(256), 0, st, nsorted, w.perm.p,
+                               w.dEd.p, w.wtd.p, w0, df, w.pos.p, w.ws.p, w.ss.p, w.es.p);
         }
+
+        /* ---- rho into sorted rows */
+        const bool fp32 = !direct && (p->precision == PMX_PREC_FP32 || p->precision == PMX_PREC_BF16X3);
+        const size_t xrows = (size_t)std::max(nsorted, 1) * nch * D;
+        if (fp32) {
+            if ((rc = w.Xf.ensure(xrows))) return rc;
+        } else if ((rc = w.Xd.ensure(xrows))) {
+            return rc;
+        }
+        RhoArgs ra{};
+        ra.nk = p->nk; ra.N = N; ra.npw = npw; ra.neps = neps; ra.nb_s = nb_s; ra.nband = p->nband;
+        ra.spatial = spatial ? 1 : 0; ra.nch = nch;
+        ra.vk = w.vk.p; ra.vkq = w.vkq.p;
+        ra.nc = w.nc.p; ra.nv = w.nv.p; ra.tbase = w.tbase.p; ra.pos = w.pos.p; ra.loci = c->dLoci.p;
+        ra.wA = w.wAd.p; ra.uc = w.ucd.p; ra.wS = spatial ? nullptr : w.wSd.p;
+        ra.K = w.Kd.p;
+        ra.X = fp32 ? nullptr : w.Xd.p;
+        ra.Xf = fp32 ? w.Xf.p : nullptr;
+        if (nsorted > 0) {
+            /* algorithmic work of the gather-GEMM (src/pmx_chi.cpp:534-544): 8 flops per complex MAC,
+             * 3 weighted (+1 plain) sums per (m, p) and transition; bytes = the eigenvector rows read */
+            const double macs = (double)nsorted * neps * npw * (spatial ? 4.0 : 16.0);
+            Prof pf(c, "rho", 8.0 * macs, 16.0 * (double)N * (2 * nb_s) * p->nk);
+            hipLaunchKernelGGL(rho_kernel, dim3(p->nk), dim3(256), 0, st, ra);
+        }
+        PMX_CHECK_HIP(hipGetLastError());
 
         /* ---- accumulate */
         const int nacc = spatial ? 1 : 2;
@@ -808,41 +999,76 @@ extern "C" int pmx_chi(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double 
             DBuf<zd> &Z = acc == 0 ? w.Z1 : w.Z2;
             DBuf<zd> &M = acc == 0 ? w.M1 : w.M2;
             PMX_CHECK_HIP(hipMemsetAsync(Z.p, 0, sizeof(zd) * (size_t)p->nfreq * npairs * 256, st));
+            if (nsorted == 0) continue;
             if (direct) {
                 DirArgs da{D, nch, npairs, nbins, wh, p->nfreq, p->delta, acc, p->epsilon, w0, df,
-                           w.Xd.p, w.dEd.p, w.wtd.p, w.perm.p, w.start.p, Z.p};
+                           w.Xd.p, w.es.p, w.ws.p, w.start.p, Z.p};
+                Prof pf(c, "direct", 8.0 * 9.0 * ne2 * nsorted * (2 * wh + 1) * (spatial ? 1 : 2), 0.0);
                 hipLaunchKernelGGL(direct_kernel, dim3(npairs, nbins), dim3(256), 0, st, da);
             } else {
                 PMX_CHECK_HIP(hipMemsetAsync(M.p, 0, sizeof(zd) * (size_t)nbins * NT * npairs * 256, st));
-                MomArgs ma{D, nch, ntile, npairs, nbins, NT, w.Xd.p, w.dEd.p, w.wtd.p, w.perm.p, w.start.p,
-                           w0, df, acc, M.p};
-                hipLaunchKernelGGL(moment_simt_kernel, dim3(npairs, nbins), dim3(256), 0, st, ma);
-                ConvArgs ca{npairs, nbins, NT, wh, p->nfreq, M.p, w.Ktd.p, Z.p};
-                hipLaunchKernelGGL(conv_kernel, dim3(npairs, (p->nfreq + CONV_F - 1) / CONV_F), dim3(256), 0, st, ca);
+                MomArgs2 ma{D, nch, npairs, nbins, NT, (NT + 3) / 4, acc, fp32 ? (const void *)w.Xf.p : (const void *)w.Xd.p,
+                            w.ws.p, w.ss.p, w.start.p, M.p};
+                {
+                    /* algorithmic: 8 flops per complex MAC for each stored element
+                     * (i,j,m,n<=m), order n and rank-1 term (1 per transition, 2 with SOC) */
+                    const double fl = 8.0 * 9.0 * ne2 * NT * (double)nsorted * (spatial ? 1 : 2);
+                    const double by = (double)nsorted * nch * D * (fp32 ? 8.0 : 16.0) +
+                                      16.0 * (double)nbins * NT * npairs * 256 * 2;
+                    Prof pf(c, fp32 ? "moment_mfma_f32" : "moment_mfma_f64", fl, by);
+                    if (fp32)
+                        hipLaunchKernelGGL(moment_mfma_kernel<float>, dim3(npairs, nbins), dim3(256), 0, st, ma);
+                    else
+                        hipLaunchKernelGGL(moment_mfma_kernel<double>, dim3(npairs, nbins), dim3(256), 0, st, ma);
+                }
+                {
+                    Prof pf(c, "conv", 4.0 * 9.0 * ne2 * p->nfreq * (2 * wh + 1) * NT, 0.0);
+                    ConvArgs ca{npairs, nbins, NT, wh, p->nfreq, M.p, w.Ktd.p, Z.p};
+                    hipLaunchKernelGGL(conv_kernel, dim3(npairs, (p->nfreq + CONV_F - 1) / CONV_F), dim3(256), 0, st,
+                                       ca);
+                }
             }
             PMX_CHECK_HIP(hipGetLastError());
         }
 
         /* ---- scale + pack (+ KK) */
-        double *dstRe = p->kk ? w.Re0.p : w.ReD.p, *dstIm = p->kk ? w.Im0.p : w.ImD.p;
-        PackArgs pa{D, neps, npairs, p->nfreq, 1, 0, spatial ? 0 : 1, SF, df, w0, w.Z1.p,
-                    spatial ? w.Z1.p : w.Z2.p, dstRe, dstIm};
-        hipLaunchKernelGGL(pack_kernel, dim3(1024), dim3(256), 0, st, pa);
-        if (p->kk)
-            hipLaunchKernelGGL(kk_kernel, dim3(2048), dim3(256), 0, st, 9 * ne2, p->nfreq, w.Kkd.p, w.Re0.p, w.Im0.p,
-                               w.ReD.p, w.ImD.p);
-        PMX_CHECK_HIP(hipGetLastError());
-        std::vector<double> hre(per_q), him(per_q);
-        PMX_CHECK_HIP(hipMemcpyAsync(hre.data(), w.ReD.p, sizeof(double) * per_q, hipMemcpyDeviceToHost, st));
-        PMX_CHECK_HIP(hipMemcpyAsync(him.data(), w.ImD.p, sizeof(double) * per_q, hipMemcpyDeviceToHost, st));
-        PMX_CHECK_HIP(hipStreamSynchronize(st));
-        /* scatter the per-q block [ij][e2][f] into [ij][q][e2][f] */
-        const size_t blk = (size_t)ne2 * p->nfreq;
-        for (int ij = 0; ij < 9; ij++) {
-            std::memcpy(ReX + ((size_t)ij * p->nq + q) * blk, hre.data() + (size_t)ij * blk, sizeof(double) * blk);
-            std::memcpy(ImX + ((size_t)ij * p->nq + q) * blk, him.data() + (size_t)ij * blk, sizeof(double) * blk);
+        {
+            Prof pf(c, "pack", 0.0, 16.0 * per_q);
+            PackArgs pa{D, neps, npairs, p->nfreq, 1, 0, spatial ? 0 : 1, SF, df, w0, w.Z1.p,
+                        spatial ? w.Z1.p : w.Z2.p, w.Re0.p, w.Im0.p};
+            hipLaunchKernelGGL(pack_kernel, dim3(1024), dim3(256), 0, st, pa);
         }
+        if (p->kk) {
+            Prof pf(c, "kk", 4.0 * per_q * p->nfreq, 8.0 * 4 * per_q);
+            const int M = 9 * ne2;
+            hipLaunchKernelGGL(kk_gemm_kernel, dim3((p->nfreq + 15) / 16, (M + 63) / 64), dim3(256), 0, st, M,
+                               p->nfreq, w.Re0.p, w.Im0.p, w.Kkd.p, dRe, dIm, ne2, p->nq, q);
+        } else {
+            hipLaunchKernelGGL(place_kernel, dim3(1024), dim3(256), 0, st, 9 * ne2, p->nfreq, ne2, p->nq, q, w.Re0.p,
+                               w.Im0.p, dRe, dIm);
+        }
+        PMX_CHECK_HIP(hipGetLastError());
     }
+    PMX_CHECK_HIP(hipStreamSynchronize(st));
+    return PMX_OK;
+}
+
+extern "C" int pmx_chi_device(pmx_ctx *c, const pmx_chi_params *p, double *dReX, double *dImX) {
+    return chi_run(c, p, dReX, dImX);
+}
+
+extern "C" int pmx_chi(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double *ImX) {
+    if (!c || !p || !ReX || !ImX) return set_error(PMX_EINVAL, "null pointer");
+    if (!c->has_system) return set_error(PMX_ESTATE, "no system set");
+    if (p->nq < 0 || p->nfreq <= 0) return set_error(PMX_EINVAL, "bad sizes");
+    PMX_CHECK_HIP(hipSetDevice(c->device));
+    const size_t n = (size_t)9 * p->nq * c->neps * (c->neps + 1) / 2 * p->nfreq;
+    DBuf<double> r, i;
+    int rc;
+    if ((rc = r.ensure(n)) || (rc = i.ensure(n))) return rc;
+    if ((rc = chi_run(c, p, r.p, i.p))) return rc;
+    PMX_CHECK_HIP(hipMemcpy(ReX, r.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    PMX_CHECK_HIP(hipMemcpy(ImX, i.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     return PMX_OK;
 }
 
@@ -860,7 +1086,8 @@ extern "C" int pmx_kk(pmx_ctx *c, int nel, int nfreq, double dfreq, double *ReX,
     PMX_CHECK_HIP(hipMemcpyAsync(r0.p, ReX, sizeof(double) * n, hipMemcpyHostToDevice, st));
     PMX_CHECK_HIP(hipMemcpyAsync(i0.p, ImX, sizeof(double) * n, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(kkmat_kernel, dim3(256), dim3(256), 0, st, nfreq, dfreq, Kk.p);
-    hipLaunchKernelGGL(kk_kernel, dim3(2048), dim3(256), 0, st, nel, nfreq, Kk.p, r0.p, i0.p, r.p, i.p);
+    hipLaunchKernelGGL(kk_gemm_kernel, dim3((nfreq + 15) / 16, (nel + 63) / 64), dim3(256), 0, st, nel, nfreq, r0.p,
+                       i0.p, Kk.p, r.p, i.p, nel, 1, 0);
     PMX_CHECK_HIP(hipGetLastError());
     PMX_CHECK_HIP(hipMemcpyAsync(ReX, r.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     PMX_CHECK_HIP(hipMemcpyAsync(ImX, i.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
@@ -870,8 +1097,34 @@ extern "C" int pmx_kk(pmx_ctx *c, int nel, int nfreq, double dfreq, double *ReX,
 
 extern "C" int pmx_epsinv(pmx_ctx *c, int nq, int neps, int nfreq, const double *ReX, const double *ImX,
                           double *epsM, double *elf, double *epsinv) {
-    (void)c; (void)nq; (void)neps; (void)nfreq; (void)ReX; (void)ImX; (void)epsM; (void)elf; (void)epsinv;
-    return set_error(PMX_ESTATE, "pmx_epsinv: not built yet");
+    if (!c || !ReX || !ImX || !epsM || !elf) return set_error(PMX_EINVAL, "null pointer");
+    if (nq <= 0 || neps <= 0 || nfreq <= 0) return set_error(PMX_EINVAL, "bad sizes");
+    PMX_CHECK_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const size_t ne2 = (size_t)neps * (neps + 1) / 2;
+    const size_t nblk = (size_t)nq * ne2 * nfreq; /* the i = j = 0 block leads the layout */
+    const int nmat = nq * nfreq;
+    DBuf<double> re, im, el;
+    DBuf<zd> scr, em, inv;
+    int rc;
+    if ((rc = re.ensure(nblk)) || (rc = im.ensure(nblk)) || (rc = el.ensure(nmat)) || (rc = em.ensure(nmat)) ||
+        (rc = scr.ensure((size_t)nmat * neps * 2 * neps)))
+        return rc;
+    if (epsinv && (rc = inv.ensure((size_t)nmat * neps * neps))) return rc;
+    PMX_CHECK_HIP(hipMemcpyAsync(re.p, ReX, sizeof(double) * nblk, hipMemcpyHostToDevice, st));
+    PMX_CHECK_HIP(hipMemcpyAsync(im.p, ImX, sizeof(double) * nblk, hipMemcpyHostToDevice, st));
+    {
+        Prof pf(c, "epsinv", 8.0 * neps * neps * 2.0 * neps * nmat, 0.0);
+        hipLaunchKernelGGL(epsinv_kernel, dim3(nmat), dim3(256), 0, st, neps, nq, nfreq, re.p, im.p, scr.p, em.p, el.p,
+                           epsinv ? inv.p : nullptr);
+    }
+    PMX_CHECK_HIP(hipGetLastError());
+    PMX_CHECK_HIP(hipMemcpyAsync(epsM, em.p, sizeof(zd) * nmat, hipMemcpyDeviceToHost, st));
+    PMX_CHECK_HIP(hipMemcpyAsync(elf, el.p, sizeof(double) * nmat, hipMemcpyDeviceToHost, st));
+    if (epsinv)
+        PMX_CHECK_HIP(hipMemcpyAsync(epsinv, inv.p, sizeof(zd) * nmat * neps * neps, hipMemcpyDeviceToHost, st));
+    PMX_CHECK_HIP(hipStreamSynchronize(st));
+    return PMX_OK;
 }
 
 extern "C" int pmx_grad_bands(pmx_ctx *c, int nk, const double *K, int nband, const double *gE, double *gvg) {

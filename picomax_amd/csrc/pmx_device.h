@@ -9,6 +9,7 @@
 #include "pmx_internal.h"
 
 #include <complex>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -90,13 +91,35 @@ struct pmx_ctx {
     pmx::DBuf<pmx::zd> dVpair, dVso, dSfac;
     pmx::DBuf<int> dShell, dLoci;
 
+    /* per-kernel HIP-event timing (pmx_profile) */
+    struct ProfEvent {
+        std::string name;
+        hipEvent_t a, b;
+        double flops, bytes;
+    };
+    struct ProfAcc {
+        long count = 0;
+        double ms = 0, flops = 0, bytes = 0;
+    };
+    bool prof = false;
+    std::vector<ProfEvent> prof_pending;
+    std::map<std::string, ProfAcc> prof_acc;
+
     /* eigensolver scratch */
-    pmx::DBuf<double> dK, dEval, dScratch;
+    pmx::DBuf<double> dK, dEval, dScratch, dEigWork, dEigRef;
     pmx::DBuf<pmx::zd> dEvec, dZscr;
     pmx::DBuf<int> dStatus;
 };
 
 namespace pmx {
+/* records [start, end) events around the launches in its scope when
+ * profiling is on; flops/bytes are the algorithmic counts of that launch */
+struct Prof {
+    pmx_ctx *c;
+    int idx = -1;
+    Prof(pmx_ctx *ctx, const char *name, double flops = 0.0, double bytes = 0.0);
+    ~Prof();
+};
 /* eigensolve of nmat H(K) on the context's stream; K is a device array */
 int eig_device(pmx_ctx *ctx, int nmat, const double *dKpts, int nb, int nvec, double *dEvals, zd *dEvecs);
 /* basis size of the solve (NPW spatial or 2 NPW spinor) */

@@ -6,22 +6,24 @@
  * src/pmx_chi.cpp:388, 476).  The reference diagonalises the full spinor
  * matrix and keeps the lowest NBAND pairs; only those are computed here.
  *
- * One workgroup owns one matrix at a time (grid-stride over the batch):
- *   1. assemble the lower triangle of H(k) into packed column-major storage,
- *      in LDS when it fits (N(N+1)/2 elements: real N <= ~190, complex
- *      N <= 137 on the 160 KB gfx950 LDS) else in a per-workgroup global slab;
- *   2. Householder reduction to a real symmetric tridiagonal T (the zhetd2
- *      scheme: H_k = I - tau v v^H, v_0 = 1, beta real), each step one
- *      column-oriented pass for the Hermitian mat-vec (every stored element
- *      read once, contributions to the row part kept in per-wave partial
- *      vectors) and one pass for the rank-2 update of the lower triangle;
- *   3. the lowest nb eigenvalues of T by 8-point multisection on Sturm
- *      counts, one 8-lane group per eigenvalue;
- *   4. eigenvectors of T for the lowest nvec eigenvalues by inverse iteration
- *      (LU with partial pivoting of T - lambda I), modified Gram-Schmidt
- *      inside clusters of eigenvalues closer than 1e-5 ||T||, one thread per
- *      cluster;
- *   5. back-transformation z = H_0 H_1 ... H_{N-2} y, one wave per vector.
+ * Four batched kernels per chunk of matrices:
+ *   tridiag   one workgroup per matrix: assemble the lower triangle of H(k)
+ *             in packed column-major storage (LDS when it fits: real N up to
+ *             ~190, complex N up to ~130 on the 160 KB gfx950 LDS; else a
+ *             per-workgroup global slab) and reduce it to a real symmetric
+ *             tridiagonal T by Householder reflections (the zhetd2 scheme:
+ *             H_k = I - tau v v^H, v_0 = 1, beta real); each step is one
+ *             column pass for the Hermitian mat-vec (every stored element read
+ *             once, the transposed contributions kept in per-wave partials)
+ *             and one pass for the rank-2 update;
+ *   bisect    one thread per (matrix, eigenvalue index): bisection on Sturm
+ *             counts of T, structure-of-arrays so consecutive threads read
+ *             consecutive matrices;
+ *   invit     one thread per cluster (eigenvalues closer than 1e-5 ||T||):
+ *             inverse iteration, each solve a fused LU-with-partial-pivoting
+ *             + forward pass and a back substitution, modified Gram-Schmidt
+ *             inside the cluster; structure-of-arrays scratch;
+ *   backtrans one workgroup per matrix: z = H_0 H_1 ... H_{N-2} y in LDS.
  * The no-SOC matrix is H0 (x) I2 in the reference; here H0 (NPW x NPW) is
  * solved and the caller duplicates eigenvalues.  When the crystal has an
  * inversion centre s and no SOC, H0 is made real by the diagonal gauge
@@ -30,6 +32,7 @@
  */
 #include "pmx_device.h"
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 
@@ -39,7 +42,6 @@ namespace {
 
 constexpr int EIG_THREADS = 256;
 constexpr int EIG_WAVES = EIG_THREADS / 64;
-constexpr int MS_GROUP = 8; /* lanes per eigenvalue in the multisection */
 
 /* ---------------------------------------------------------- arithmetic */
 __device__ __forceinline__ double cj(double a) { return a; }
@@ -87,11 +89,12 @@ struct EigArgs {
     const zd *Vso;        /* [npw][npw] */
     const double *phase;  /* [npw] */
     int nb, nvec;
-    double *evals;        /* [nmat][nb] */
-    zd *evecs;            /* [nmat][nvec][N] or null */
-    char *scratch;        /* per-workgroup slab */
-    size_t slab;          /* bytes per workgroup */
-    int *status;          /* [nmat] 0 ok */
+    double *dS, *eS;      /* SoA [N][nmat] diagonal / off-diagonal of T */
+    double *gb;           /* [nmat][4]: Gershgorin lower, upper, pivmin, ||T|| */
+    void *vref;           /* [nmat][P] reflectors (strictly below the sub-diagonal) */
+    void *tauv;           /* [nmat][N] Householder scalars */
+    char *scratch;        /* per-workgroup slab (matrix storage when not in LDS) */
+    size_t slab;
 };
 
 /* H(k) element (r, c), r >= c, of the assembly mode */
@@ -137,20 +140,6 @@ template <> __device__ __forceinline__ zd h_elem<zd>(const EigArgs &a, const dou
     return sx - isy; /* (0, 1) */
 }
 
-/* Sturm count: number of eigenvalues of T below x */
-__device__ int sturm_count(const double *d, const double *e2, int N, double x, double pivmin) {
-    int cnt = 0;
-    double q = d[0] - x;
-    if (fabs(q) < pivmin) q = -pivmin;
-    cnt += q < 0.0;
-    for (int i = 1; i < N; i++) {
-        q = (d[i] - x) - e2[i - 1] / q;
-        if (fabs(q) < pivmin) q = -pivmin;
-        cnt += q < 0.0;
-    }
-    return cnt;
-}
-
 /* deterministic start vector entry in [-1, 1) */
 __device__ __forceinline__ double hash_unit(unsigned a, unsigned b) {
     unsigned h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u;
@@ -163,14 +152,13 @@ __device__ __forceinline__ double hash_unit(unsigned a, unsigned b) {
 }
 
 template <class T, bool LDS>
-__global__ void __launch_bounds__(EIG_THREADS) heev_kernel(EigArgs a) {
+__global__ void __launch_bounds__(EIG_THREADS) tridiag_kernel(EigArgs a) {
     extern __shared__ double smem_d[];
     const int N = a.N;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t P = (size_t)N * (N + 1) / 2;
 
-    /* LDS carve-up: [A packed (LDS case)] [d N][e N][e2 N][tau N (T)][v N (T)][w N (T)]
-     * [part EIG_WAVES*N (T)][red 2*EIG_WAVES][misc] */
+    /* LDS: [A packed (LDS case)] [d N][e N] [tau N][v N][w N][part WAVES*N] (T) [red] */
     char *sp = reinterpret_cast<char *>(smem_d);
     T *A;
     if (LDS) {
@@ -181,8 +169,6 @@ __global__ void __launch_bounds__(EIG_THREADS) heev_kernel(EigArgs a) {
     }
     double *d = reinterpret_cast<double *>(sp); sp += sizeof(double) * N;
     double *e = reinterpret_cast<double *>(sp); sp += sizeof(double) * N;
-    double *e2 = reinterpret_cast<double *>(sp); sp += sizeof(double) * N;
-    double *lam = reinterpret_cast<double *>(sp); sp += sizeof(double) * (a.nb + 1);
     T *tauv = reinterpret_cast<T *>(sp); sp += sizeof(T) * N;
     T *vv = reinterpret_cast<T *>(sp); sp += sizeof(T) * N;
     T *ww = reinterpret_cast<T *>(sp); sp += sizeof(T) * N;
@@ -191,14 +177,9 @@ __global__ void __launch_bounds__(EIG_THREADS) heev_kernel(EigArgs a) {
     __shared__ double sh_scal[8];
     __shared__ int sh_int[4];
 
-    /* global per-workgroup slab after A (non-LDS) or from the start (LDS) */
-    char *gs = a.scratch + (size_t)blockIdx.x * a.slab + (LDS ? 0 : ((P * sizeof(T) + 255) / 256) * 256);
-    double *Y = reinterpret_cast<double *>(gs);                 /* [nvec][N] tridiagonal eigvecs */
-    double *lu = Y + (size_t)a.nvec * N;                        /* [nvec][5][N] LU work          */
-    T *Z = reinterpret_cast<T *>(lu + (size_t)a.nvec * 5 * N);  /* [nvec][N] back-transform      */
-
     for (int mat = blockIdx.x; mat < a.nmat; mat += gridDim.x) {
         double Kv[3] = {a.K[3 * mat], a.K[3 * mat + 1], a.K[3 * mat + 2]};
+        T *vref = a.vref ? reinterpret_cast<T *>(a.vref) + (size_t)mat * P : nullptr;
         /* 1. assemble lower triangle */
         for (int c = wave; c < N; c += EIG_WAVES)
             for (int r = c + lane; r < N; r += 64) A[pk(r, c, N)] = h_elem<T>(a, Kv, r, c);
@@ -207,7 +188,6 @@ __global__ void __launch_bounds__(EIG_THREADS) heev_kernel(EigArgs a) {
         /* 2. Householder tridiagonalisation */
         for (int k = 0; k + 1 < N; k++) {
             const int L = N - k - 1;
-            /* ||x[1:]||^2 of column k below the sub-diagonal */
             double s = 0.0;
             for (int r = k + 2 + tid; r < N; r += EIG_THREADS) s += nrm2(A[pk(r, k, N)]);
             s = wave_sum(s);
@@ -231,7 +211,7 @@ __global__ void __launch_bounds__(EIG_THREADS) heev_kernel(EigArgs a) {
                 e[k] = beta;
                 tauv[k] = tau;
                 vv[0] = from_real<T>(1.0);
-                ww[0] = scal; /* scale factor parked in ww[0] */
+                ww[0] = scal;
                 sh_int[0] = (re(tau) == 0.0 && im(tau) == 0.0) ? 0 : 1;
             }
             __syncthreads();
@@ -241,13 +221,14 @@ __global__ void __launch_bounds__(EIG_THREADS) heev_kernel(EigArgs a) {
             for (int i = 1 + tid; i < L; i += EIG_THREADS) {
                 T x = A[pk(k + 1 + i, k, N)] * scal;
                 vv[i] = x;
-                A[pk(k + 1 + i, k, N)] = x; /* reflector stored below the sub-diagonal */
+                A[pk(k + 1 + i, k, N)] = x;
+                if (vref) vref[pk(k + 1 + i, k, N)] = x;
             }
             __syncthreads();
             if (!active) continue;
-            /* p = tau * A22 v, column pass: wave per column c */
             for (int i = tid; i < L * EIG_WAVES; i += EIG_THREADS) part[i] = zero<T>();
             __syncthreads();
+            /* p = tau * A22 v, column pass: every stored element read once */
             for (int cc = wave; cc < L; cc += EIG_WAVES) {
                 const int c = k + 1 + cc;
                 const T vc = vv[cc];
@@ -270,7 +251,6 @@ __global__ void __launch_bounds__(EIG_THREADS) heev_kernel(EigArgs a) {
                 }
             }
             __syncthreads();
-            /* p = tau * sum of partials; pv = p^H v */
             double pr = 0.0, pi = 0.0;
             for (int i = tid; i < L; i += EIG_THREADS) {
                 T p = zero<T>();
@@ -321,8 +301,12 @@ __global__ void __launch_bounds__(EIG_THREADS) heev_kernel(EigArgs a) {
         for (int i = tid; i < N; i += EIG_THREADS) d[i] = re(A[pk(i, i, N)]);
         if (tid == 0) e[N - 1] = 0.0;
         __syncthreads();
-
-        /* 3. lowest nb eigenvalues of (d, e) by multisection */
+        /* outputs: T in SoA, Householder scalars, Gershgorin bounds */
+        for (int i = tid; i < N; i += EIG_THREADS) {
+            a.dS[(size_t)i * a.nmat + mat] = d[i];
+            a.eS[(size_t)i * a.nmat + mat] = e[i];
+            if (a.tauv) reinterpret_cast<T *>(a.tauv)[(size_t)mat * N + i] = (i + 1 < N) ? tauv[i] : zero<T>();
+        }
         if (tid == 0) {
             double gl = d[0], gu = d[0], emax2 = 0.0;
             for (int i = 0; i < N; i++) {
@@ -334,180 +318,211 @@ __global__ void __launch_bounds__(EIG_THREADS) heev_kernel(EigArgs a) {
             double tnorm = fmax(fabs(gl), fabs(gu));
             gl -= 2.0 * DBL_EPSILON * tnorm * N + 1e-300;
             gu += 2.0 * DBL_EPSILON * tnorm * N + 1e-300;
-            sh_scal[2] = gl;
-            sh_scal[3] = gu;
-            sh_scal[4] = DBL_MIN * fmax(1.0, emax2);
-            sh_scal[5] = tnorm;
+            a.gb[4 * mat] = gl;
+            a.gb[4 * mat + 1] = gu;
+            a.gb[4 * mat + 2] = DBL_MIN * fmax(1.0, emax2);
+            a.gb[4 * mat + 3] = tnorm;
         }
-        for (int i = tid; i + 1 < N; i += EIG_THREADS) e2[i] = e[i] * e[i];
-        __syncthreads();
-        {
-            const double gl = sh_scal[2], gu = sh_scal[3], pivmin = sh_scal[4];
-            const int grp = tid / MS_GROUP, g = tid % MS_GROUP;
-            const int ngrp = EIG_THREADS / MS_GROUP;
-            for (int j0 = 0; j0 < a.nb; j0 += ngrp) {
-                const int j = j0 + grp;
-                double lo = gl, hi = gu;
-                for (int it = 0; it < 40; it++) {
-                    double x = lo + (hi - lo) * (double)(g + 1) / (double)(MS_GROUP + 1);
-                    int c = (j < a.nb && j < N) ? sturm_count(d, e2, N, x, pivmin) : 0;
-                    unsigned long long ball = __ballot(c <= j);
-                    int sh = (lane / MS_GROUP) * MS_GROUP;
-                    int m = __popcll((ball >> sh) & ((1ull << MS_GROUP) - 1));
-                    double xlo = __shfl(x, (lane & ~(MS_GROUP - 1)) + (m > 0 ? m - 1 : 0), 64);
-                    double xhi = __shfl(x, (lane & ~(MS_GROUP - 1)) + (m < MS_GROUP ? m : MS_GROUP - 1), 64);
-                    if (m > 0) lo = xlo;
-                    if (m < MS_GROUP) hi = xhi;
-                    if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin) break;
-                }
-                if (g == 0 && j < a.nb) lam[j] = (j < N) ? 0.5 * (lo + hi) : 0.0;
-            }
-        }
-        __syncthreads();
-        for (int j = tid; j < a.nb; j += EIG_THREADS)
-            a.evals[(size_t)mat * a.nb + j] = (j < N) ? lam[j] : 0.0;
-
-        if (a.nvec > 0) {
-            /* 4. inverse iteration, one thread per cluster of close eigenvalues */
-            const double ortol = 1e-5 * sh_scal[5];
-            const int nvec = a.nvec < N ? a.nvec : N;
-            for (int j = tid; j < nvec; j += EIG_THREADS) {
-                if (j > 0 && lam[j] - lam[j - 1] <= ortol) continue; /* not a cluster head */
-                int jend = j + 1;
-                while (jend < nvec && lam[jend] - lam[jend - 1] <= ortol) jend++;
-                for (int jj = j; jj < jend; jj++) {
-                    double *y = Y + (size_t)jj * N;
-                    double *dl = lu + (size_t)jj * 5 * N, *dd = dl + N, *du = dd + N, *du2 = du + N;
-                    int *ip = reinterpret_cast<int *>(du2 + N);
-                    const double lamj = lam[jj];
-                    const double eps_piv = DBL_EPSILON * sh_scal[5];
-                    /* LU with partial pivoting of T - lam I (tridiagonal) */
-                    for (int i = 0; i < N; i++) {
-                        dd[i] = d[i] - lamj;
-                        if (i + 1 < N) {
-                            dl[i] = e[i];
-                            du[i] = e[i];
-                        }
-                        du2[i] = 0.0;
-                        ip[i] = i;
-                    }
-                    for (int i = 0; i + 1 < N; i++) {
-                        if (fabs(dd[i]) >= fabs(dl[i])) {
-                            if (dd[i] == 0.0) dd[i] = eps_piv;
-                            double fact = dl[i] / dd[i];
-                            dl[i] = fact;
-                            dd[i + 1] -= fact * du[i];
-                        } else {
-                            double fact = dd[i] / dl[i];
-                            dd[i] = dl[i];
-                            dl[i] = fact;
-                            double t = du[i];
-                            du[i] = dd[i + 1];
-                            dd[i + 1] = t - fact * dd[i + 1];
-                            if (i + 2 < N) {
-                                du2[i] = du[i + 1];
-                                du[i + 1] = -fact * du[i + 1];
-                            }
-                            ip[i] = i + 1;
-                        }
-                    }
-                    if (dd[N - 1] == 0.0) dd[N - 1] = eps_piv;
-                    for (int i = 0; i < N; i++) y[i] = hash_unit(jj + 1, i + 1);
-                    for (int it = 0; it < 3; it++) {
-                        /* solve L U x = y (dgttrs order) */
-                        for (int i = 0; i + 1 < N; i++) {
-                            if (ip[i] == i) {
-                                y[i + 1] -= dl[i] * y[i];
-                            } else {
-                                double t = y[i];
-                                y[i] = y[i + 1];
-                                y[i + 1] = t - dl[i] * y[i + 1];
-                            }
-                        }
-                        y[N - 1] /= dd[N - 1];
-                        if (N > 1) y[N - 2] = (y[N - 2] - du[N - 2] * y[N - 1]) / dd[N - 2];
-                        for (int i = N - 3; i >= 0; i--) y[i] = (y[i] - du[i] * y[i + 1] - du2[i] * y[i + 2]) / dd[i];
-                        /* orthogonalise against the cluster's earlier vectors, normalise */
-                        for (int pj = j; pj < jj; pj++) {
-                            const double *yp = Y + (size_t)pj * N;
-                            double s2 = 0.0;
-                            for (int i = 0; i < N; i++) s2 += yp[i] * y[i];
-                            for (int i = 0; i < N; i++) y[i] -= s2 * yp[i];
-                        }
-                        double nn = 0.0, mx = 0.0;
-                        for (int i = 0; i < N; i++) mx = fmax(mx, fabs(y[i]));
-                        double sc = 1.0 / mx;
-                        for (int i = 0; i < N; i++) {
-                            y[i] *= sc;
-                            nn += y[i] * y[i];
-                        }
-                        sc = 1.0 / sqrt(nn);
-                        for (int i = 0; i < N; i++) y[i] *= sc;
-                    }
-                }
-            }
-            __syncthreads();
-            /* 5. back-transform, wave per vector: z = H_0 (H_1 (... y)) */
-            for (int jj = wave; jj < nvec; jj += EIG_WAVES) {
-                T *z = Z + (size_t)jj * N;
-                const double *y = Y + (size_t)jj * N;
-                for (int i = lane; i < N; i += 64) z[i] = from_real<T>(y[i]);
-                for (int k = N - 2; k >= 0; k--) {
-                    const T tau = tauv[k];
-                    if (re(tau) == 0.0 && im(tau) == 0.0) continue;
-                    const int L = N - k - 1;
-                    double sr = 0.0, si = 0.0;
-                    for (int i = lane; i < L; i += 64) {
-                        T v = (i == 0) ? from_real<T>(1.0) : A[pk(k + 1 + i, k, N)];
-                        T t = cj(v) * z[k + 1 + i];
-                        sr += re(t);
-                        si += im(t);
-                    }
-                    sr = wave_sum(sr);
-                    si = wave_sum(si);
-                    T sdot;
-                    if constexpr (sizeof(T) == sizeof(double)) sdot = sr; else sdot = zd{sr, si};
-                    sdot = tau * sdot;
-                    for (int i = lane; i < L; i += 64) {
-                        T v = (i == 0) ? from_real<T>(1.0) : A[pk(k + 1 + i, k, N)];
-                        z[k + 1 + i] -= v * sdot;
-                    }
-                }
-                /* write, restoring the gauge phase on the real path */
-                if (a.evecs) {
-                    zd *out = a.evecs + ((size_t)mat * a.nvec + jj) * N;
-                    for (int i = lane; i < N; i += 64) {
-                        zd o;
-                        if constexpr (sizeof(T) == sizeof(double)) {
-                            double s, c;
-                            sincos(a.phase[i], &s, &c);
-                            o = zd{c * z[i], s * z[i]};
-                        } else {
-                            o = z[i];
-                        }
-                        out[i] = o;
-                    }
-                }
-            }
-        }
-        if (tid == 0) a.status[mat] = 0;
         __syncthreads();
     }
 }
 
-size_t lds_bytes(int N, int nb, size_t elem, bool lds_matrix) {
+/* lowest nb eigenvalues of every T by bisection on Sturm counts, one thread
+ * per (matrix, index); consecutive threads take consecutive matrices so the
+ * SoA reads of d and e coalesce */
+__global__ void bisect_kernel(int N, int nmat, int nb, const double *dS, const double *eS, const double *gb,
+                              double *lamS, double *evals) {
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (t >= (size_t)nmat * nb) return;
+    const int mat = t % nmat, j = t / nmat;
+    double lo = gb[4 * mat], hi = gb[4 * mat + 1];
+    const double pivmin = gb[4 * mat + 2];
+    for (int it = 0; it < 80; it++) {
+        const double mid = 0.5 * (lo + hi);
+        if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin || mid == lo || mid == hi) break;
+        int cnt = 0;
+        double q = dS[mat] - mid;
+        if (fabs(q) < pivmin) q = -pivmin;
+        cnt += q < 0.0;
+        for (int i = 1; i < N; i++) {
+            const double ei = eS[(size_t)(i - 1) * nmat + mat];
+            q = (dS[(size_t)i * nmat + mat] - mid) - (ei * ei) / q;
+            if (fabs(q) < pivmin) q = -pivmin;
+            cnt += q < 0.0;
+        }
+        if (cnt <= j) lo = mid; else hi = mid;
+    }
+    const double lam = (j < N) ? 0.5 * (lo + hi) : 0.0;
+    lamS[(size_t)j * nmat + mat] = lam;
+    evals[(size_t)mat * nb + j] = lam;
+}
+
+/* eigenvectors of T for the lowest nvec eigenvalues by inverse iteration:
+ * one thread per cluster of eigenvalues closer than 1e-5 ||T|| (the head
+ * thread takes the cluster in order, modified Gram-Schmidt against earlier
+ * members each iteration).  Each solve of (T - lambda I) x = y is one forward
+ * pass of LU with partial pivoting fused with the forward substitution, then
+ * the back substitution; all per-thread arrays are SoA [.][N][slot]. */
+__global__ void invit_kernel(int N, int nmat, int nvec, const double *dS, const double *eS, const double *gb,
+                             const double *lamS, double *Y, double *work) {
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (t >= (size_t)nmat * nvec) return;
+    const int mat = t % nmat, j = t / nmat;
+    const double tnorm = gb[4 * mat + 3];
+    const double ortol = 1e-5 * tnorm;
+    auto lam = [&](int jj) { return lamS[(size_t)jj * nmat + mat]; };
+    if (j > 0 && lam(j) - lam(j - 1) <= ortol) return; /* not a cluster head */
+    int jend = j + 1;
+    while (jend < nvec && lam(jend) - lam(jend - 1) <= ortol) jend++;
+    const size_t slots = (size_t)nmat * nvec;
+    double *dd = work + t, *du = work + (size_t)N * slots + t, *du2 = work + (size_t)2 * N * slots + t;
+    auto at = [&](double *base, int i) -> double & { return base[(size_t)i * slots]; };
+    auto D = [&](int i) { return dS[(size_t)i * nmat + mat]; };
+    auto E = [&](int i) { return eS[(size_t)i * nmat + mat]; };
+    const double eps_piv = DBL_EPSILON * tnorm;
+    for (int jj = j; jj < jend; jj++) {
+        double *y = Y + (size_t)jj * N * nmat + mat; /* element i at y[i * nmat] */
+        auto Yv = [&](int i) -> double & { return y[(size_t)i * nmat]; };
+        const double lm = lam(jj);
+        for (int i = 0; i < N; i++) Yv(i) = hash_unit(jj + 1, i + 1);
+        for (int it = 0; it < 3; it++) {
+            /* forward: factor T - lm I with partial pivoting, transform y */
+            double cd = D(0) - lm, cdu = (N > 1) ? E(0) : 0.0, cb = Yv(0);
+            for (int i = 0; i + 1 < N; i++) {
+                const double ndl = E(i), nd = D(i + 1) - lm, ndu = (i + 2 < N) ? E(i + 1) : 0.0, nb_ = Yv(i + 1);
+                if (fabs(cd) >= fabs(ndl)) {
+                    if (cd == 0.0) cd = eps_piv;
+                    const double f = ndl / cd;
+                    at(dd, i) = cd; at(du, i) = cdu; at(du2, i) = 0.0; Yv(i) = cb;
+                    cd = nd - f * cdu;
+                    cdu = ndu;
+                    cb = nb_ - f * cb;
+                } else {
+                    const double f = cd / ndl;
+                    at(dd, i) = ndl; at(du, i) = nd; at(du2, i) = ndu; Yv(i) = nb_;
+                    const double ncd = cdu - f * nd, ncdu = -f * ndu, ncb = cb - f * nb_;
+                    cd = ncd;
+                    cdu = ncdu;
+                    cb = ncb;
+                }
+            }
+            if (cd == 0.0) cd = eps_piv;
+            at(dd, N - 1) = cd;
+            Yv(N - 1) = cb;
+            /* back substitution */
+            double x2 = 0.0, x1 = Yv(N - 1) / at(dd, N - 1);
+            Yv(N - 1) = x1;
+            for (int i = N - 2; i >= 0; i--) {
+                const double x = (Yv(i) - at(du, i) * x1 - at(du2, i) * x2) / at(dd, i);
+                Yv(i) = x;
+                x2 = x1;
+                x1 = x;
+            }
+            /* orthogonalise against the cluster's earlier vectors, normalise */
+            for (int pj = j; pj < jj; pj++) {
+                const double *yp = Y + (size_t)pj * N * nmat + mat;
+                double s2 = 0.0;
+                for (int i = 0; i < N; i++) s2 += yp[(size_t)i * nmat] * Yv(i);
+                for (int i = 0; i < N; i++) Yv(i) -= s2 * yp[(size_t)i * nmat];
+            }
+            double mx = 0.0;
+            for (int i = 0; i < N; i++) mx = fmax(mx, fabs(Yv(i)));
+            const double sc = 1.0 / mx;
+            double nn = 0.0;
+            for (int i = 0; i < N; i++) {
+                const double v = Yv(i) * sc;
+                Yv(i) = v;
+                nn += v * v;
+            }
+            const double sc2 = 1.0 / sqrt(nn);
+            for (int i = 0; i < N; i++) Yv(i) *= sc2;
+        }
+    }
+}
+
+/* z_j = H_0 H_1 ... H_{N-2} y_j, one workgroup per matrix, vectors in LDS */
+template <class T>
+__global__ void __launch_bounds__(EIG_THREADS) backtrans_kernel(int N, int nmat, int nvec, const double *Y,
+                                                                 const void *vref_, const void *tau_,
+                                                                 const double *phase, zd *evecs) {
+    extern __shared__ double smem_d[];
+    T *Z = reinterpret_cast<T *>(smem_d);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const size_t P = (size_t)N * (N + 1) / 2;
+    for (int mat = blockIdx.x; mat < nmat; mat += gridDim.x) {
+        const T *vref = reinterpret_cast<const T *>(vref_) + (size_t)mat * P;
+        const T *tau = reinterpret_cast<const T *>(tau_) + (size_t)mat * N;
+        for (int idx = tid; idx < nvec * N; idx += EIG_THREADS) {
+            const int jj = idx / N, i = idx % N;
+            Z[idx] = from_real<T>(Y[((size_t)jj * N + i) * nmat + mat]);
+        }
+        __syncthreads();
+        for (int jj = wave; jj < nvec; jj += EIG_WAVES) {
+            T *z = Z + (size_t)jj * N;
+            for (int k = N - 2; k >= 0; k--) {
+                const T tk = tau[k];
+                if (re(tk) == 0.0 && im(tk) == 0.0) continue;
+                const int L = N - k - 1;
+                double sr = 0.0, si = 0.0;
+                for (int i = lane; i < L; i += 64) {
+                    T v = (i == 0) ? from_real<T>(1.0) : vref[pk(k + 1 + i, k, N)];
+                    T t2 = cj(v) * z[k + 1 + i];
+                    sr += re(t2);
+                    si += im(t2);
+                }
+                sr = wave_sum(sr);
+                si = wave_sum(si);
+                T sdot;
+                if constexpr (sizeof(T) == sizeof(double)) sdot = sr; else sdot = zd{sr, si};
+                sdot = tk * sdot;
+                for (int i = lane; i < L; i += 64) {
+                    T v = (i == 0) ? from_real<T>(1.0) : vref[pk(k + 1 + i, k, N)];
+                    z[k + 1 + i] -= v * sdot;
+                }
+            }
+            zd *out = evecs + ((size_t)mat * nvec + jj) * N;
+            for (int i = lane; i < N; i += 64) {
+                zd o;
+                if constexpr (sizeof(T) == sizeof(double)) {
+                    double sn, cs;
+                    sincos(phase[i], &sn, &cs);
+                    o = zd{cs * z[i], sn * z[i]};
+                } else {
+                    o = z[i];
+                }
+                out[i] = o;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+size_t lds_bytes(int N, size_t elem, bool lds_matrix) {
     size_t P = (size_t)N * (N + 1) / 2;
     size_t b = lds_matrix ? ((P * elem + 15) / 16) * 16 : 0;
-    b += sizeof(double) * (3 * N + nb + 1);
+    b += sizeof(double) * 2 * N;
     b += elem * (3 * N + (size_t)N * EIG_WAVES);
     b += sizeof(double) * 4 * EIG_WAVES;
     return b;
 }
 
-template <class T, bool LDS> int launch(pmx_ctx *c, EigArgs &a, int grid, size_t lds) {
-    auto kern = heev_kernel<T, LDS>;
-    if (lds > 65536) PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+template <class T, bool LDS> int launch_tridiag(pmx_ctx *c, EigArgs &a, int grid, size_t lds) {
+    auto kern = tridiag_kernel<T, LDS>;
+    if (lds > 65536)
+        PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(EIG_THREADS), lds, c->stream, a);
+    PMX_CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, const double *Y, const void *vref,
+                                        const void *tau, const double *phase, zd *evecs) {
+    const size_t lds = sizeof(T) * (size_t)nvec * N;
+    auto kern = backtrans_kernel<T>;
+    if (lds > 65536)
+        PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int grid = std::min(nmat, c->num_cu * 4);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(EIG_THREADS), lds, c->stream, N, nmat, nvec, Y, vref, tau, phase, evecs);
     PMX_CHECK_HIP(hipGetLastError());
     return 0;
 }
@@ -519,44 +534,76 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
     const int N = basis_size(c);
     if (nb > N) nb = N;
     if (nvec > nb) nvec = nb;
-    EigArgs a{};
-    a.N = N;
-    a.nmat = nmat;
-    a.mode = c->soc ? HM_SPINOR : (c->real_gauge ? HM_SPATIAL_REAL : HM_SPATIAL_COMPLEX);
-    a.npw = c->npw;
-    a.K = dKpts;
-    a.G = c->dG.p;
-    a.kin = c->kin;
-    a.Vpair = c->dVpair.p;
-    a.Vreal = c->dVreal.p;
-    a.Vso = c->dVso.p;
-    a.phase = c->dPhase.p;
-    a.nb = nb;
-    a.nvec = dEvecs ? nvec : 0;
-    a.evals = dEvals;
-    a.evecs = dEvecs;
-    const bool real = (a.mode == HM_SPATIAL_REAL);
+    if (!dEvecs) nvec = 0;
+    const int mode = c->soc ? HM_SPINOR : (c->real_gauge ? HM_SPATIAL_REAL : HM_SPATIAL_COMPLEX);
+    const bool real = (mode == HM_SPATIAL_REAL);
     const size_t elem = real ? sizeof(double) : sizeof(zd);
-    const size_t lds_cap = c->lds_max;
-    bool lds_matrix = lds_bytes(N, nb, elem, true) <= lds_cap;
-    size_t lds = lds_bytes(N, nb, elem, lds_matrix);
-    if (lds > lds_cap) return set_error(PMX_EINVAL, "basis too large for the eigensolver's LDS work arrays");
     const size_t P = (size_t)N * (N + 1) / 2;
-    size_t slab = (lds_matrix ? 0 : ((P * elem + 255) / 256) * 256) +
-                  (size_t)a.nvec * N * sizeof(double) * 6 + (size_t)a.nvec * N * elem + 256;
-    slab = ((slab + 255) / 256) * 256;
-    int grid = nmat;
-    int max_grid = c->num_cu * (lds_matrix ? 1 : 2) * 2;
-    if (grid > max_grid) grid = max_grid;
-    int rc = c->dScratch.ensure((slab * grid + 7) / 8);
-    if (rc) return rc;
-    rc = c->dStatus.ensure(nmat);
-    if (rc) return rc;
-    a.scratch = reinterpret_cast<char *>(c->dScratch.p);
-    a.slab = slab;
-    a.status = c->dStatus.p;
-    if (real) return lds_matrix ? launch<double, true>(c, a, grid, lds) : launch<double, false>(c, a, grid, lds);
-    return lds_matrix ? launch<zd, true>(c, a, grid, lds) : launch<zd, false>(c, a, grid, lds);
+    const bool lds_matrix = lds_bytes(N, elem, true) <= c->lds_max;
+    const size_t lds = lds_bytes(N, elem, lds_matrix);
+    if (lds > c->lds_max) return set_error(PMX_EINVAL, "basis too large for the eigensolver's LDS work arrays");
+    if (nvec > 0 && sizeof(zd) * (size_t)nvec * N > c->lds_max)
+        return set_error(PMX_EINVAL, "too many eigenvectors for the back-transformation's LDS");
+    /* chunk the batch so that reflectors + SoA scratch stay near 4 GB */
+    const size_t per_mat = (nvec > 0 ? P * elem + N * elem : 0) + sizeof(double) * (size_t)N * (2 + nb) +
+                           sizeof(double) * (size_t)nvec * N * 4 + 64;
+    int chunk = (int)std::max<size_t>(64, std::min<size_t>((size_t)nmat, (size_t)4e9 / per_mat));
+    const size_t slab = lds_matrix ? 0 : ((P * elem + 255) / 256) * 256;
+    const int grid_max = c->num_cu * (lds_matrix ? 1 : 2) * 2;
+    int rc;
+    const size_t cs = (size_t)chunk;
+    const size_t need = cs * N * 2 + cs * 4 + cs * nb + (nvec > 0 ? cs * nvec * N * 4 : 0);
+    if ((rc = c->dEigWork.ensure(need))) return rc;
+    if (nvec > 0 && (rc = c->dEigRef.ensure((cs * (P + N) * elem + 7) / 8))) return rc;
+    if (!lds_matrix && (rc = c->dScratch.ensure((slab * grid_max + 7) / 8))) return rc;
+    double *dS = c->dEigWork.p, *eS = dS + cs * N, *gb = eS + cs * N, *lamS = gb + cs * 4;
+    double *Y = lamS + cs * nb, *work = Y + cs * nvec * N;
+    char *vref = reinterpret_cast<char *>(c->dEigRef.p);
+    char *tauv = vref + cs * P * elem;
+    const double n3 = (double)N * N * N, n2 = (double)N * N;
+    for (int m0 = 0; m0 < nmat; m0 += chunk) {
+        const int nm = std::min(chunk, nmat - m0);
+        EigArgs a{};
+        a.N = N; a.nmat = nm; a.mode = mode; a.npw = c->npw;
+        a.K = dKpts + 3 * (size_t)m0; a.G = c->dG.p; a.kin = c->kin;
+        a.Vpair = c->dVpair.p; a.Vreal = c->dVreal.p; a.Vso = c->dVso.p; a.phase = c->dPhase.p;
+        a.nb = nb; a.nvec = nvec;
+        a.dS = dS; a.eS = eS; a.gb = gb;
+        a.vref = nvec > 0 ? vref : nullptr;
+        a.tauv = nvec > 0 ? tauv : nullptr;
+        a.scratch = reinterpret_cast<char *>(c->dScratch.p);
+        a.slab = slab;
+        const int grid = std::min(nm, grid_max);
+        {
+            Prof pf(c, real ? "tridiag_real" : "tridiag_complex", nm * (real ? (4.0 / 3.0) * n3 : (16.0 / 3.0) * n3), 0.0);
+            if (real) rc = lds_matrix ? launch_tridiag<double, true>(c, a, grid, lds) : launch_tridiag<double, false>(c, a, grid, lds);
+            else rc = lds_matrix ? launch_tridiag<zd, true>(c, a, grid, lds) : launch_tridiag<zd, false>(c, a, grid, lds);
+            if (rc) return rc;
+        }
+        {
+            Prof pf(c, "bisect", 0.0, 0.0);
+            const size_t nt = (size_t)nm * nb;
+            hipLaunchKernelGGL(bisect_kernel, dim3((nt + 255) / 256), dim3(256), 0, c->stream, N, nm, nb, dS, eS, gb,
+                               lamS, dEvals + (size_t)m0 * nb);
+            PMX_CHECK_HIP(hipGetLastError());
+        }
+        if (nvec > 0) {
+            {
+                Prof pf(c, "invit", 0.0, 0.0);
+                const size_t nt = (size_t)nm * nvec;
+                hipLaunchKernelGGL(invit_kernel, dim3((nt + 255) / 256), dim3(256), 0, c->stream, N, nm, nvec, dS, eS,
+                                   gb, lamS, Y, work);
+                PMX_CHECK_HIP(hipGetLastError());
+            }
+            {
+                Prof pf(c, real ? "backtrans_real" : "backtrans_complex", nm * (real ? 2.0 : 8.0) * n2 * nvec, 0.0);
+                rc = real ? launch_backtrans<double>(c, N, nm, nvec, Y, vref, tauv, c->dPhase.p, dEvecs + (size_t)m0 * nvec * N)
+                          : launch_backtrans<zd>(c, N, nm, nvec, Y, vref, tauv, c->dPhase.p, dEvecs + (size_t)m0 * nvec * N);
+                if (rc) return rc;
+            }
+        }
+    }
+    return 0;
 }
 
 } // namespace pmx

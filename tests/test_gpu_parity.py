@@ -117,3 +117,51 @@ def test_spatial_path_equals_spinor_oracle_random(ctx):
         err = cli_args.slice_error(np.concatenate([re1, im1]), np.concatenate([re0, im0]), 9 * 10)
         _say("  delta %d err %.3e" % (delta, err))
         assert err < 1e-8
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("direct", 1e-8)])
+def test_chi_precision_modes(manifest, ctx, precision, tol):
+    """The mixed FP32 path within 1e-4 of each slice's max, the direct FP64
+    windowed sum within 1e-8, on the C1 fixture."""
+    case = "si_chi_c1"
+    args = manifest[case]["args"]
+    _say("chi %s on %s" % (precision, case))
+    inp = capi.Input(_expand(args))
+    ctx.set_system(inp.system(), inp)
+    s, p = cli_args.build(args)
+    off = ctx.ref_energy(p["refpoint"], p["nvalence"])
+    re, im = ctx.chi(p["K"], p["KW"], p["Q"], p["nband"], off, p["delta"], p["epsilon"], p["nfreq"], p["dfreq"],
+                     p["kk"], precision)
+    ref = np.fromfile(os.path.join(cli_args.GOLDEN, case + ".dat"), dtype="<f8")
+    err = cli_args.slice_error(np.concatenate([re, im]), ref, _nslice(args))
+    _say("  %s slice error %.3e" % (precision, err))
+    assert err < tol
+
+
+def test_epsinv_against_numpy(manifest, ctx):
+    case = "si_chi_lor"
+    args = manifest[case]["args"]
+    _say("epsinv %s" % case)
+    inp = capi.Input(_expand(args))
+    ctx.set_system(inp.system(), inp)
+    s, p = cli_args.build(args)
+    ref = np.fromfile(os.path.join(cli_args.GOLDEN, case + ".dat"), dtype="<f8")
+    n = len(ref) // 2
+    re, im = ref[:n], ref[n:]
+    nq, neps, nf = len(p["Q"]), p["neps"], p["nfreq"]
+    eM, elf, inv = ctx.epsinv(re, im, nq, nf, full=True)
+    ne2 = neps * (neps + 1) // 2
+    blk = (re + 1j * im)[:nq * ne2 * nf].reshape(nq, ne2, nf)
+    for q in range(nq):
+        for f in range(0, nf, 37):
+            E = np.zeros((neps, neps), complex)
+            for m in range(neps):
+                for nn in range(m + 1):
+                    E[m, nn] = blk[q, m * (m + 1) // 2 + nn, f]
+                    if m != nn:
+                        E[nn, m] = np.conj(E[m, nn])
+            E += np.eye(neps)
+            Ei = np.linalg.inv(E)
+            assert np.abs(inv[q, f] - Ei).max() < 1e-12 * max(1.0, np.abs(Ei).max())
+            assert abs(eM[q, f] - 1.0 / Ei[0, 0]) < 1e-12 * abs(eM[q, f])
+            assert abs(elf[q, f] + Ei[0, 0].imag) < 1e-13 * max(1.0, abs(Ei[0, 0]))
