@@ -370,6 +370,124 @@ template <class T> __global__ void __launch_bounds__(256) moment_mfma_kernel(Mom
 }
 
 /* ------------------------------------------------------------------
+ * split-BF16 rank update (PMX_PREC_BF16X3): every operand x = hi + lo with
+ * hi = bf16(x), lo = bf16(x - hi); a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi,
+ * three v_mfma_f32_16x16x32_bf16 per product, FP32 accumulation.  Same tile
+ * and real embedding as moment_mfma_kernel, K = 32 per instruction: lane l
+ * holds K entries 8(l/16) .. 8(l/16)+7 of A (row l%16) and of B (column
+ * l%16); D rows 4(l/16)+v.  The s^n scaling is applied to B in FP32 and B is
+ * re-split for every order n.
+ * ------------------------------------------------------------------ */
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+    unsigned u = __float_as_uint(x);
+    unsigned r = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u; /* round to nearest even */
+    float h = __uint_as_float(r);
+    float l = x - h;
+    unsigned ul = __float_as_uint(l);
+    unsigned rl = (ul + 0x7FFFu + ((ul >> 16) & 1u)) >> 16;
+    hi = __builtin_bit_cast(__bf16, (unsigned short)(r >> 16));
+    lo = __builtin_bit_cast(__bf16, (unsigned short)rl);
+}
+
+__global__ void __launch_bounds__(256) moment_bf16x3_kernel(MomArgs2 a) {
+    constexpr int MAXW = 5;
+    const int pairi = blockIdx.x, b = blockIdx.y;
+    const int r0 = a.start[b], r1 = a.start[b + 1];
+    if (r0 == r1) return;
+    int ti, tj;
+    tile_pair(pairi, ti, tj);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n0 = wave * a.npw_wave;
+    const int nw = min(a.npw_wave, a.NT - n0);
+    if (nw <= 0) return;
+    const int ia = ti * TILE + (lane & 15), ib = tj * TILE + (lane & 15);
+    const int kg = lane >> 4;
+    const float2 *X = reinterpret_cast<const float2 *>(a.X);
+    const int nitems = (a.nch == 1) ? 1 : 2;
+    const int K2 = 2 * nitems;
+    const int kend = (r1 - r0) * K2;
+    fx4 accr[MAXW], acci[MAXW];
+#pragma unroll
+    for (int j = 0; j < MAXW; j++) {
+        accr[j] = fx4{0, 0, 0, 0};
+        acci[j] = fx4{0, 0, 0, 0};
+    }
+    for (int kb = 0; kb < kend; kb += 32) {
+        bf8 ahi, alo;
+        float bre[8], bim[8], sv[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int kk = kb + 8 * kg + e;
+            float av = 0.f, br = 0.f, bi = 0.f, s = 0.f;
+            if (kk < kend) {
+                const int row = r0 + kk / K2, sub = kk % K2;
+                const int item = sub >> 1, part = sub & 1;
+                const int ch = (a.nch == 1) ? 0 : (2 * a.acc_index + item);
+                const float2 *xr = X + ((size_t)row * a.nch + ch) * a.D;
+                float2 xa = (ia < a.D) ? xr[ia] : make_float2(0.f, 0.f);
+                float2 xb = (ib < a.D) ? xr[ib] : make_float2(0.f, 0.f);
+                float w = (float)a.ws[row];
+                if (item) w = -w;
+                s = (float)a.ss[row];
+                av = part ? xa.y : xa.x;
+                br = w * (part ? xb.y : xb.x);
+                bi = w * (part ? xb.x : -xb.y);
+                float sc = 1.f;
+                for (int j = 0; j < n0; j++) sc *= s;
+                br *= sc;
+                bi *= sc;
+            }
+            __bf16 h, l;
+            split_bf16(av, h, l);
+            ahi[e] = h;
+            alo[e] = l;
+            bre[e] = br;
+            bim[e] = bi;
+            sv[e] = s;
+        }
+#pragma unroll
+        for (int j = 0; j < MAXW; j++) {
+            if (j < nw) {
+                bf8 brh, brl, bih, bil;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    __bf16 h, l;
+                    split_bf16(bre[e], h, l);
+                    brh[e] = h;
+                    brl[e] = l;
+                    split_bf16(bim[e], h, l);
+                    bih[e] = h;
+                    bil[e] = l;
+                    bre[e] *= sv[e];
+                    bim[e] *= sv[e];
+                }
+                accr[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, brh, accr[j], 0, 0, 0);
+                accr[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, brl, accr[j], 0, 0, 0);
+                accr[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, brh, accr[j], 0, 0, 0);
+                acci[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bih, acci[j], 0, 0, 0);
+                acci[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bil, acci[j], 0, 0, 0);
+                acci[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bih, acci[j], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < MAXW; j++) {
+        if (j < nw) {
+            zd *dst = a.M + (((size_t)b * a.NT + n0 + j) * a.npairs + pairi) * 256;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = 4 * (lane >> 4) + v, col = lane & 15;
+                zd &d = dst[row * TILE + col];
+                d.x += (double)accr[j][v];
+                d.y += (double)acci[j][v];
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------
  * direct windowed sum (PMX_PREC_DIRECT, triangle, rect): one block per
  * (tile pair, bin); Z[f][pair][256] += sum_t w delta(dE_t - w_f) X X^H
  * ------------------------------------------------------------------ */
@@ -461,41 +579,39 @@ struct ConvArgs {
     zd *Z;            /* [nfreq][npairs][256] */
 };
 
-constexpr int CONV_F = 16;
-
-__global__ void __launch_bounds__(256) conv_kernel(ConvArgs a) {
-    const int pairi = blockIdx.x, fb = blockIdx.y * CONV_F;
-    zd acc[CONV_F];
-#pragma unroll
-    for (int j = 0; j < CONV_F; j++) acc[j] = zd{0, 0};
-    const int W = 2 * a.wh + 1;
-    /* bins touching f in [fb, fb+CONV_F): b in [fb, fb + CONV_F - 1 + 2wh] */
-    const int bmax = min(a.nbins - 1, fb + CONV_F - 1 + 2 * a.wh);
-    for (int b = fb; b <= bmax; b++) {
-        for (int n = 0; n < a.NT; n++) {
-            zd m = a.M[(((size_t)b * a.NT + n) * a.npairs + pairi) * 256 + threadIdx.x];
-            if (m.x == 0.0 && m.y == 0.0) continue;
-#pragma unroll
-            for (int j = 0; j < CONV_F; j++) {
-                const int f = fb + j;
-                const int Dd = b - a.wh - f; /* f0 - f */
-                if (f < a.nfreq && Dd >= -a.wh && Dd <= a.wh) {
-                    double kv = a.Kt[(Dd + a.wh) * a.NT + n];
-                    acc[j].x += m.x * kv;
-                    acc[j].y += m.y * kv;
-                }
-            }
+/* Z(f) = sum_{D=-wh..wh} sum_n M[f+D+wh][n] K[D+wh][n] as an FP64 MFMA GEMM: for a block of 16 frequencies
+ * Z[f][e] = sum_{(b,n)} Kc[f][(b,n)] M[(b,n)][e], Kc[f][(b,n)] = K_n(b - wh - f)
+ * inside the window and 0 outside; b runs over the bins any f of the block
+ * can see.  Block = 4 waves x 16 consecutive tile elements e. */
+__global__ void __launch_bounds__(256) conv_mfma_kernel(ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int E = a.npairs * 256;
+    const int e0 = (blockIdx.x * 4 + wave) * 16;
+    if (e0 >= E) return;
+    const int fb = blockIdx.y * 16;
+    const int fa = fb + (lane & 15);        /* A row: frequency */
+    const int eb = e0 + (lane & 15);        /* B column: element */
+    const int kq = lane >> 4;
+    const int b_lo = fb, b_hi = min(a.nbins - 1, fb + 15 + 2 * a.wh);
+    const int kcount = (b_hi - b_lo + 1) * a.NT;
+    dx4 accR = {0, 0, 0, 0}, accI = {0, 0, 0, 0};
+    for (int k0 = 0; k0 < kcount; k0 += 4) {
+        const int kk = k0 + kq;
+        double av = 0.0, br = 0.0, bi = 0.0;
+        if (kk < kcount) {
+            const int b = b_lo + kk / a.NT, n = kk % a.NT;
+            const int Dd = b - a.wh - fa;
+            if (fa < a.nfreq && Dd >= -a.wh && Dd <= a.wh) av = a.Kt[(Dd + a.wh) * a.NT + n];
+            const zd m = a.M[((size_t)b * a.NT + n) * E + eb];
+            br = m.x;
+            bi = m.y;
         }
+        accR = __builtin_amdgcn_mfma_f64_16x16x4f64(av, br, accR, 0, 0, 0);
+        accI = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bi, accI, 0, 0, 0);
     }
-    (void)W;
-#pragma unroll
-    for (int j = 0; j < CONV_F; j++) {
-        const int f = fb + j;
-        if (f < a.nfreq) {
-            zd *dst = a.Z + ((size_t)f * a.npairs + pairi) * 256 + threadIdx.x;
-            dst->x += acc[j].x;
-            dst->y += acc[j].y;
-        }
+    for (int v = 0; v < 4; v++) {
+        const int f = fb + (lane >> 4) + 4 * v; /* f64 D layout */
+        if (f < a.nfreq) a.Z[(size_t)f * E + eb] = zd{accR[v], accI[v]};
     }
 }
 
@@ -1015,8 +1131,11 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                     const double fl = 8.0 * 9.0 * ne2 * NT * (double)nsorted * (spatial ? 1 : 2);
                     const double by = (double)nsorted * nch * D * (fp32 ? 8.0 : 16.0) +
                                       16.0 * (double)nbins * NT * npairs * 256 * 2;
-                    Prof pf(c, fp32 ? "moment_mfma_f32" : "moment_mfma_f64", fl, by);
-                    if (fp32)
+                    const bool bf = p->precision == PMX_PREC_BF16X3;
+                    Prof pf(c, bf ? "moment_bf16x3" : (fp32 ? "moment_mfma_f32" : "moment_mfma_f64"), fl, by);
+                    if (bf)
+                        hipLaunchKernelGGL(moment_bf16x3_kernel, dim3(npairs, nbins), dim3(256), 0, st, ma);
+                    else if (fp32)
                         hipLaunchKernelGGL(moment_mfma_kernel<float>, dim3(npairs, nbins), dim3(256), 0, st, ma);
                     else
                         hipLaunchKernelGGL(moment_mfma_kernel<double>, dim3(npairs, nbins), dim3(256), 0, st, ma);
@@ -1024,8 +1143,8 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                 {
                     Prof pf(c, "conv", 4.0 * 9.0 * ne2 * p->nfreq * (2 * wh + 1) * NT, 0.0);
                     ConvArgs ca{npairs, nbins, NT, wh, p->nfreq, M.p, w.Ktd.p, Z.p};
-                    hipLaunchKernelGGL(conv_kernel, dim3(npairs, (p->nfreq + CONV_F - 1) / CONV_F), dim3(256), 0, st,
-                                       ca);
+                    hipLaunchKernelGGL(conv_mfma_kernel, dim3((npairs * 256 + 63) / 64, (p->nfreq + 15) / 16),
+                                       dim3(256), 0, st, ca);
                 }
             }
             PMX_CHECK_HIP(hipGetLastError());
@@ -1127,7 +1246,76 @@ extern "C" int pmx_epsinv(pmx_ctx *c, int nq, int neps, int nfreq, const double 
     return PMX_OK;
 }
 
+/* Hellmann-Feynman adjoint of the band energies: one block per (k, band);
+ * dE/dvg[s][sh] = 2 Re sum_{i>j, shell_ij = sh} conj(psi_i) dH_ij psi_j with
+ * dH_ij/dvg[s][sh] = Ry2eV e^{-i 2pi (G_i - G_j).tau_s} / NATOM (the
+ * derivative of pseudopotential(), src/pmx_epm.cpp:171-203; it multiplies the
+ * identity in spin, src/pmx_epm.cpp:84), weighted by the upstream gradient. */
+__global__ void __launch_bounds__(256) hf_grad_kernel(int npw, int N, int natom, int nv, int nb, int soc,
+                                                      const zd *evecs, const int *shell, const zd *sfac,
+                                                      const double *gw, double *gvg) {
+    extern __shared__ double acc[]; /* [natom * nv] */
+    const int kn = blockIdx.x;
+    const double g = gw[kn];
+    for (int i = threadIdx.x; i < natom * nv; i += blockDim.x) acc[i] = 0.0;
+    __syncthreads();
+    if (g != 0.0) {
+        const zd *psi = evecs + (size_t)kn * N;
+        const size_t npair = (size_t)npw * npw;
+        for (size_t id = threadIdx.x; id < npair; id += blockDim.x) {
+            const int i = id / npw, j = id % npw;
+            if (i <= j) continue;
+            const int sh = shell[id];
+            if (sh < 0) continue;
+            /* sum over spin of conj(psi_i,s) psi_j,s */
+            zd pr;
+            if (soc) {
+                zd a = cmul(cconj(psi[2 * i]), psi[2 * j]), b = cmul(cconj(psi[2 * i + 1]), psi[2 * j + 1]);
+                pr = zd{a.x + b.x, a.y + b.y};
+            } else {
+                pr = cmul(cconj(psi[i]), psi[j]);
+            }
+            for (int sa = 0; sa < natom; sa++) {
+                zd f = sfac[id * natom + sa];
+                atomicAdd(&acc[sa * nv + sh], pr.x * f.x - pr.y * f.y);
+            }
+        }
+    }
+    __syncthreads();
+    const double scale = 2.0 * g * PMX_RY2EV / natom;
+    for (int i = threadIdx.x; i < natom * nv; i += blockDim.x)
+        if (acc[i] != 0.0) atomicAdd(&gvg[i], scale * acc[i]);
+}
+
 extern "C" int pmx_grad_bands(pmx_ctx *c, int nk, const double *K, int nband, const double *gE, double *gvg) {
-    (void)c; (void)nk; (void)K; (void)nband; (void)gE; (void)gvg;
-    return set_error(PMX_ESTATE, "pmx_grad_bands: not built yet");
+    if (!c || !K || !gE || !gvg) return set_error(PMX_EINVAL, "null pointer");
+    if (!c->has_system) return set_error(PMX_ESTATE, "no system set");
+    if (nk <= 0 || nband <= 0 || nband > 2 * c->npw) return set_error(PMX_EINVAL, "bad sizes");
+    PMX_CHECK_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int N = basis_size(c);
+    const int nb_s = c->soc ? nband : (nband + 1) / 2;
+    /* upstream gradient per solve-basis state: spinor partners share one spatial state */
+    std::vector<double> gw((size_t)nk * nb_s, 0.0);
+    for (int k = 0; k < nk; k++)
+        for (int n = 0; n < nband; n++) gw[(size_t)k * nb_s + (c->soc ? n : n / 2)] += gE[(size_t)k * nband + n];
+    DBuf<double> Kd, ev, gwd, out;
+    DBuf<zd> vec;
+    int rc;
+    if ((rc = Kd.ensure(3 * (size_t)nk)) || (rc = ev.ensure((size_t)nk * nb_s)) || (rc = gwd.ensure(gw.size())) ||
+        (rc = out.ensure((size_t)c->natom * c->nv)) || (rc = vec.ensure((size_t)nk * nb_s * N)))
+        return rc;
+    PMX_CHECK_HIP(hipMemcpyAsync(Kd.p, K, sizeof(double) * 3 * nk, hipMemcpyHostToDevice, st));
+    PMX_CHECK_HIP(hipMemcpyAsync(gwd.p, gw.data(), sizeof(double) * gw.size(), hipMemcpyHostToDevice, st));
+    PMX_CHECK_HIP(hipMemsetAsync(out.p, 0, sizeof(double) * c->natom * c->nv, st));
+    if ((rc = eig_device(c, nk, Kd.p, nb_s, nb_s, ev.p, vec.p))) return rc;
+    {
+        Prof pf(c, "hf_grad", 0.0, 0.0);
+        hipLaunchKernelGGL(hf_grad_kernel, dim3(nk * nb_s), dim3(256), sizeof(double) * c->natom * c->nv, st, c->npw,
+                           N, c->natom, c->nv, nb_s, c->soc ? 1 : 0, vec.p, c->dShell.p, c->dSfac.p, gwd.p, out.p);
+        PMX_CHECK_HIP(hipGetLastError());
+    }
+    PMX_CHECK_HIP(hipMemcpyAsync(gvg, out.p, sizeof(double) * c->natom * c->nv, hipMemcpyDeviceToHost, st));
+    PMX_CHECK_HIP(hipStreamSynchronize(st));
+    return PMX_OK;
 }
