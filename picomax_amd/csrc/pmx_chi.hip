@@ -213,6 +213,107 @@ __global__ void __launch_bounds__(256) rho_kernel(RhoArgs a) {
 }
 
 /* ------------------------------------------------------------------
+ * rho for the spatial basis on MFMA.  For one k and one m, with channels
+ * ch = S (weight 1) and R_i (weight wA_{m,i,p}):
+ *   P_ch[c][v] = sum_p conj(Ck[c][p]) * Ckq[v][loci(m,p)] * w_ch(p)
+ * is a complex GEMM [nc x npw] x [npw x 4 nv], gathered over loci while the
+ * B operand is built.  Real embedding over K = 2 npw: A[c][(p,0)] = Re Ck,
+ * A[c][(p,1)] = Im Ck; B_re[(p,0)][j] = Re b, B_re[(p,1)][j] = Im b,
+ * B_im[(p,0)][j] = Im b, B_im[(p,1)][j] = -Re b  (conj(a) b).  One wave per
+ * (16 conduction x 16 (channel, valence)) tile; the epilogue combines
+ * rho_i = R_i + (conj(u_mi).K) S through LDS.  Needs nv <= 4 per tile column
+ * group (more valence states take more column groups).
+ * ------------------------------------------------------------------ */
+template <class T> struct RhoT;
+template <> struct RhoT<double> {
+    typedef dx4 V;
+    static __device__ __forceinline__ V mma(double a, double b, V c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <> struct RhoT<float> {
+    typedef fx4 V;
+    static __device__ __forceinline__ V mma(float a, float b, V c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+template <class T> __global__ void __launch_bounds__(256) rho_mfma_kernel(RhoArgs a) {
+    typedef typename RhoT<T>::V V;
+    const int k = blockIdx.x;
+    const int ncs = a.nc[k] / 2, nvs = a.nv[k] / 2;
+    if (ncs * nvs == 0) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ double shd[4][2][16][17]; /* per wave: [re/im][c][column] staging */
+    const size_t tb = a.tbase[k];
+    const double Kx = a.K[3 * k], Ky = a.K[3 * k + 1], Kz = a.K[3 * k + 2];
+    const zd *vk = a.vk + (size_t)k * a.nb_s * a.N;
+    const zd *vkq = a.vkq + (size_t)k * a.nb_s * a.N;
+    const int nvg = (nvs + 3) / 4;            /* valence column groups of 4 */
+    const int nct = (ncs + 15) / 16;          /* conduction row tiles */
+    const int jobs = a.neps * nct * nvg;
+    for (int job = wave; job < jobs; job += 4) {
+        const int m = job / (nct * nvg), rem = job % (nct * nvg);
+        const int ct = rem / nvg, vg = rem % nvg;
+        const int crow = ct * 16 + (lane & 15);            /* A row */
+        const int jcol = lane & 15, ch = jcol >> 2, vv = vg * 4 + (jcol & 3); /* B column */
+        const int kq = lane >> 4;
+        const bool crow_ok = crow < ncs, col_ok = vv < nvs;
+        const zd *ck = vk + (size_t)(a.nband / 2 - 1 - (crow_ok ? crow : 0)) * a.N;
+        const zd *cq = vkq + (size_t)(col_ok ? vv : 0) * a.N;
+        const int *lm = a.loci + (size_t)m * a.npw;
+        const zd *wA = a.wA + (size_t)m * 3 * a.npw;
+        V accR = {0, 0, 0, 0}, accI = {0, 0, 0, 0};
+        const int K2 = 2 * a.npw;
+        for (int k0 = 0; k0 < K2; k0 += 4) {
+            const int kk = k0 + kq;
+            T av = 0, bre = 0, bim = 0;
+            if (kk < K2) {
+                const int p = kk >> 1, part = kk & 1;
+                if (crow_ok) av = (T)(part ? ck[p].y : ck[p].x);
+                const int l = lm[p];
+                if (col_ok && l >= 0) {
+                    zd q = cq[l];
+                    zd w = (ch == 0) ? zd{1.0, 0.0} : wA[(ch - 1) * a.npw + p];
+                    zd bb = cmul(q, w);
+                    bre = (T)(part ? bb.y : bb.x);
+                    bim = (T)(part ? -bb.x : bb.y);
+                }
+            }
+            accR = RhoT<T>::mma(av, bre, accR);
+            accI = RhoT<T>::mma(av, bim, accI);
+        }
+        /* stage the 16x16 tile, then combine S and R_i per (c, v) */
+        for (int v = 0; v < 4; v++) {
+            const int r = RhoT<T>::drow(lane, v);
+            shd[wave][0][r][lane & 15] = (double)accR[v];
+            shd[wave][1][r][lane & 15] = (double)accI[v];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        for (int e = lane; e < 16 * 4; e += 64) {
+            const int r = e >> 2, vl = e & 3;
+            const int c = ct * 16 + r, v = vg * 4 + vl;
+            if (c >= ncs || v >= nvs) continue;
+            const int t = c * nvs + v;
+            const int row = a.pos[tb + t];
+            if (row < 0) continue;
+            const zd S{shd[wave][0][r][vl], shd[wave][1][r][vl]};
+            for (int i = 0; i < 3; i++) {
+                const zd R{shd[wave][0][r][4 * (i + 1) + vl], shd[wave][1][r][4 * (i + 1) + vl]};
+                const zd *u = a.uc + ((size_t)m * 3 + i) * 3;
+                zd bk{u[0].x * Kx + u[1].x * Ky + u[2].x * Kz, u[0].y * Kx + u[1].y * Ky + u[2].y * Kz};
+                zd bs = cmul(bk, S);
+                put_item(a, row, 0, 3 * m + i, zd{R.x + bs.x, R.y + bs.y});
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+/* ------------------------------------------------------------------
  * binning: f0 = int((dE - w0)/df) exactly as the reference, bin = f0 + wh;
  * transitions whose window misses [0, nfreq) are dropped.
  * ------------------------------------------------------------------ */
@@ -1105,7 +1206,12 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
              * 3 weighted (+1 plain) sums per (m, p) and transition; bytes = the eigenvector rows read */
             const double macs = (double)nsorted * neps * npw * (spatial ? 4.0 : 16.0);
             Prof pf(c, "rho", 8.0 * macs, 16.0 * (double)N * (2 * nb_s) * p->nk);
-            hipLaunchKernelGGL(rho_kernel, dim3(p->nk), dim3(256), 0, st, ra);
+            if (spatial && fp32)
+                hipLaunchKernelGGL(rho_mfma_kernel<float>, dim3(p->nk), dim3(256), 0, st, ra);
+            else if (spatial)
+                hipLaunchKernelGGL(rho_mfma_kernel<double>, dim3(p->nk), dim3(256), 0, st, ra);
+            else
+                hipLaunchKernelGGL(rho_kernel, dim3(p->nk), dim3(256), 0, st, ra);
         }
         PMX_CHECK_HIP(hipGetLastError());
 

@@ -327,6 +327,117 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_kernel(EigArgs a) {
     }
 }
 
+/* Real symmetric path with the full matrix in LDS (column-major, N <= 140 so
+ * that N^2 doubles fit next to the work vectors): thread r owns row r, so the
+ * mat-vec p_r = sum_c A[c][r] v_c and the rank-2 update of row r need no
+ * reductions and every LDS access is a coalesced column read or a broadcast.
+ * Four barriers per Householder step; beta/tau are computed redundantly by
+ * every thread instead of by one.  Same outputs as tridiag_kernel. */
+__global__ void __launch_bounds__(EIG_THREADS) tridiag_full_real_kernel(EigArgs a) {
+    extern __shared__ double smem_d[];
+    const int N = a.N;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double *A = smem_d;                       /* [N][N] column-major */
+    double *vv = A + (size_t)N * N;           /* [N] */
+    double *ww = vv + N;                      /* [N] */
+    double *e = ww + N;                       /* [N] */
+    double *tauv = e + N;                     /* [N] */
+    double *red = tauv + N;                   /* [2][EIG_WAVES] */
+    const size_t P = (size_t)N * (N + 1) / 2;
+    const int r = tid;
+    for (int mat = blockIdx.x; mat < a.nmat; mat += gridDim.x) {
+        const double Kv[3] = {a.K[3 * mat], a.K[3 * mat + 1], a.K[3 * mat + 2]};
+        double *vref = a.vref ? reinterpret_cast<double *>(a.vref) + (size_t)mat * P : nullptr;
+        if (r < N)
+            for (int c = 0; c < N; c++) A[(size_t)c * N + r] = (r >= c) ? h_elem<double>(a, Kv, r, c) : h_elem<double>(a, Kv, c, r);
+        __syncthreads();
+        for (int k = 0; k + 1 < N; k++) {
+            /* ||x[1:]||^2 with x = column k below the diagonal */
+            const double xr = (r < N) ? A[(size_t)k * N + r] : 0.0;
+            double s2 = (r >= k + 2 && r < N) ? xr * xr : 0.0;
+            s2 = wave_sum(s2);
+            if (lane == 0) red[wave] = s2;
+            __syncthreads(); /* (1) */
+            double xn2 = 0.0;
+            for (int w = 0; w < EIG_WAVES; w++) xn2 += red[w];
+            const double alpha = A[(size_t)k * N + k + 1];
+            double beta, tau, scal;
+            if (xn2 == 0.0) {
+                beta = alpha;
+                tau = 0.0;
+                scal = 0.0;
+            } else {
+                beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+                tau = (beta - alpha) / beta;
+                scal = 1.0 / (alpha - beta);
+            }
+            double vr = 0.0;
+            if (r == k + 1) vr = 1.0;
+            else if (r >= k + 2 && r < N) vr = xr * scal;
+            if (r < N) vv[r] = vr;
+            if (vref && r >= k + 2 && r < N) vref[pk(r, k, N)] = vr;
+            if (tid == 0) {
+                e[k] = beta;
+                tauv[k] = tau;
+            }
+            if (tau == 0.0) {
+                __syncthreads();
+                continue;
+            }
+            __syncthreads(); /* (2) */
+            /* p_r = tau sum_c A[c][r] v_c over the trailing block */
+            double pr = 0.0;
+            if (r > k && r < N) {
+                const double *col = A + (size_t)(k + 1) * N + r;
+                int c = k + 1;
+                for (; c + 3 < N; c += 4) {
+                    pr += col[0] * vv[c] + col[N] * vv[c + 1] + col[2 * N] * vv[c + 2] + col[3 * N] * vv[c + 3];
+                    col += 4 * N;
+                }
+                for (; c < N; c++, col += N) pr += col[0] * vv[c];
+                pr *= tau;
+            }
+            double pv = wave_sum(pr * vr);
+            if (lane == 0) red[EIG_WAVES + wave] = pv;
+            __syncthreads(); /* (3) */
+            double pvs = 0.0;
+            for (int w = 0; w < EIG_WAVES; w++) pvs += red[EIG_WAVES + w];
+            const double al = -0.5 * tau * pvs;
+            const double wr = pr + al * vr;
+            if (r < N) ww[r] = (r > k) ? wr : 0.0;
+            __syncthreads(); /* (4) */
+            if (r > k && r < N) {
+                double *col = A + (size_t)(k + 1) * N + r;
+                for (int c = k + 1; c < N; c++, col += N) *col -= vr * ww[c] + wr * vv[c];
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < N; i += EIG_THREADS) {
+            a.dS[(size_t)i * a.nmat + mat] = A[(size_t)i * N + i];
+            a.eS[(size_t)i * a.nmat + mat] = (i + 1 < N) ? e[i] : 0.0;
+            if (a.tauv) reinterpret_cast<double *>(a.tauv)[(size_t)mat * N + i] = (i + 1 < N) ? tauv[i] : 0.0;
+        }
+        if (tid == 0) {
+            double gl = A[0], gu = A[0], emax2 = 0.0;
+            for (int i = 0; i < N; i++) {
+                const double di = A[(size_t)i * N + i];
+                double off = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < N ? fabs(e[i]) : 0.0);
+                gl = fmin(gl, di - off);
+                gu = fmax(gu, di + off);
+                if (i + 1 < N) emax2 = fmax(emax2, e[i] * e[i]);
+            }
+            double tnorm = fmax(fabs(gl), fabs(gu));
+            gl -= 2.0 * DBL_EPSILON * tnorm * N + 1e-300;
+            gu += 2.0 * DBL_EPSILON * tnorm * N + 1e-300;
+            a.gb[4 * mat] = gl;
+            a.gb[4 * mat + 1] = gu;
+            a.gb[4 * mat + 2] = DBL_MIN * fmax(1.0, emax2);
+            a.gb[4 * mat + 3] = tnorm;
+        }
+        __syncthreads();
+    }
+}
+
 /* lowest nb eigenvalues of every T by bisection on Sturm counts, one thread
  * per (matrix, index); consecutive threads take consecutive matrices so the
  * SoA reads of d and e coalesce */
@@ -440,13 +551,19 @@ __global__ void invit_kernel(int N, int nmat, int nvec, const double *dS, const 
     }
 }
 
-/* z_j = H_0 H_1 ... H_{N-2} y_j, one workgroup per matrix, vectors in LDS */
+/* z_j = H_0 H_1 ... H_{N-2} y_j, one workgroup per matrix, vectors in LDS;
+ * the reflectors are staged through LDS KC columns at a time, so the
+ * sequential loop over k reads no global memory */
+constexpr int BT_KC = 16;
+
 template <class T>
 __global__ void __launch_bounds__(EIG_THREADS) backtrans_kernel(int N, int nmat, int nvec, const double *Y,
                                                                  const void *vref_, const void *tau_,
                                                                  const double *phase, zd *evecs) {
     extern __shared__ double smem_d[];
     T *Z = reinterpret_cast<T *>(smem_d);
+    T *Vc = Z + (size_t)nvec * N;          /* [BT_KC][N] staged reflector columns (row index k+1+i) */
+    T *Tc = Vc + (size_t)BT_KC * N;        /* [BT_KC] */
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t P = (size_t)N * (N + 1) / 2;
     for (int mat = blockIdx.x; mat < nmat; mat += gridDim.x) {
@@ -456,42 +573,52 @@ __global__ void __launch_bounds__(EIG_THREADS) backtrans_kernel(int N, int nmat,
             const int jj = idx / N, i = idx % N;
             Z[idx] = from_real<T>(Y[((size_t)jj * N + i) * nmat + mat]);
         }
+        for (int kend = N - 2; kend >= 0; kend -= BT_KC) {
+            const int kbeg = max(0, kend - BT_KC + 1);
+            __syncthreads();
+            for (int idx = tid; idx < BT_KC * N; idx += EIG_THREADS) {
+                const int kc = idx / N, i = idx % N; /* column k = kbeg + kc, entry k+1+i */
+                const int k = kbeg + kc;
+                T v = zero<T>();
+                if (k <= kend && k + 1 + i < N) v = (i == 0) ? from_real<T>(1.0) : vref[pk(k + 1 + i, k, N)];
+                Vc[idx] = v;
+            }
+            for (int kc = tid; kc < BT_KC; kc += EIG_THREADS) Tc[kc] = (kbeg + kc <= kend) ? tau[kbeg + kc] : zero<T>();
+            __syncthreads();
+            for (int jj = wave; jj < nvec; jj += EIG_WAVES) {
+                T *z = Z + (size_t)jj * N;
+                for (int k = kend; k >= kbeg; k--) {
+                    const T tk = Tc[k - kbeg];
+                    if (re(tk) == 0.0 && im(tk) == 0.0) continue;
+                    const int L = N - k - 1;
+                    const T *vk = Vc + (size_t)(k - kbeg) * N;
+                    double sr = 0.0, si = 0.0;
+                    for (int i = lane; i < L; i += 64) {
+                        T t2 = cj(vk[i]) * z[k + 1 + i];
+                        sr += re(t2);
+                        si += im(t2);
+                    }
+                    sr = wave_sum(sr);
+                    if constexpr (sizeof(T) != sizeof(double)) si = wave_sum(si);
+                    T sdot;
+                    if constexpr (sizeof(T) == sizeof(double)) sdot = sr; else sdot = zd{sr, si};
+                    sdot = tk * sdot;
+                    for (int i = lane; i < L; i += 64) z[k + 1 + i] -= vk[i] * sdot;
+                }
+            }
+        }
         __syncthreads();
-        for (int jj = wave; jj < nvec; jj += EIG_WAVES) {
-            T *z = Z + (size_t)jj * N;
-            for (int k = N - 2; k >= 0; k--) {
-                const T tk = tau[k];
-                if (re(tk) == 0.0 && im(tk) == 0.0) continue;
-                const int L = N - k - 1;
-                double sr = 0.0, si = 0.0;
-                for (int i = lane; i < L; i += 64) {
-                    T v = (i == 0) ? from_real<T>(1.0) : vref[pk(k + 1 + i, k, N)];
-                    T t2 = cj(v) * z[k + 1 + i];
-                    sr += re(t2);
-                    si += im(t2);
-                }
-                sr = wave_sum(sr);
-                si = wave_sum(si);
-                T sdot;
-                if constexpr (sizeof(T) == sizeof(double)) sdot = sr; else sdot = zd{sr, si};
-                sdot = tk * sdot;
-                for (int i = lane; i < L; i += 64) {
-                    T v = (i == 0) ? from_real<T>(1.0) : vref[pk(k + 1 + i, k, N)];
-                    z[k + 1 + i] -= v * sdot;
-                }
+        for (int idx = tid; idx < nvec * N; idx += EIG_THREADS) {
+            const int jj = idx / N, i = idx % N;
+            zd o;
+            if constexpr (sizeof(T) == sizeof(double)) {
+                double sn, cs;
+                sincos(phase[i], &sn, &cs);
+                o = zd{cs * Z[idx], sn * Z[idx]};
+            } else {
+                o = Z[idx];
             }
-            zd *out = evecs + ((size_t)mat * nvec + jj) * N;
-            for (int i = lane; i < N; i += 64) {
-                zd o;
-                if constexpr (sizeof(T) == sizeof(double)) {
-                    double sn, cs;
-                    sincos(phase[i], &sn, &cs);
-                    o = zd{cs * z[i], sn * z[i]};
-                } else {
-                    o = z[i];
-                }
-                out[i] = o;
-            }
+            evecs[((size_t)mat * nvec + jj) * N + i] = o;
         }
         __syncthreads();
     }
@@ -517,7 +644,7 @@ template <class T, bool LDS> int launch_tridiag(pmx_ctx *c, EigArgs &a, int grid
 
 template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, const double *Y, const void *vref,
                                         const void *tau, const double *phase, zd *evecs) {
-    const size_t lds = sizeof(T) * (size_t)nvec * N;
+    const size_t lds = sizeof(T) * ((size_t)nvec * N + (size_t)BT_KC * N + BT_KC);
     auto kern = backtrans_kernel<T>;
     if (lds > 65536)
         PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -542,7 +669,7 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
     const bool lds_matrix = lds_bytes(N, elem, true) <= c->lds_max;
     const size_t lds = lds_bytes(N, elem, lds_matrix);
     if (lds > c->lds_max) return set_error(PMX_EINVAL, "basis too large for the eigensolver's LDS work arrays");
-    if (nvec > 0 && sizeof(zd) * (size_t)nvec * N > c->lds_max)
+    if (nvec > 0 && elem * ((size_t)nvec * N + (size_t)BT_KC * (N + 1)) > c->lds_max)
         return set_error(PMX_EINVAL, "too many eigenvectors for the back-transformation's LDS");
     /* chunk the batch so that reflectors + SoA scratch stay near 4 GB */
     const size_t per_mat = (nvec > 0 ? P * elem + N * elem : 0) + sizeof(double) * (size_t)N * (2 + nb) +
@@ -576,7 +703,15 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
         const int grid = std::min(nm, grid_max);
         {
             Prof pf(c, real ? "tridiag_real" : "tridiag_complex", nm * (real ? (4.0 / 3.0) * n3 : (16.0 / 3.0) * n3), 0.0);
-            if (real) rc = lds_matrix ? launch_tridiag<double, true>(c, a, grid, lds) : launch_tridiag<double, false>(c, a, grid, lds);
+            const size_t lds_full = sizeof(double) * ((size_t)N * N + 4 * (size_t)N + 2 * EIG_WAVES);
+            if (real && N <= EIG_THREADS && lds_full <= c->lds_max) {
+                auto kern = tridiag_full_real_kernel;
+                PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  (int)lds_full));
+                hipLaunchKernelGGL(kern, dim3(std::min(nm, c->num_cu * 2)), dim3(EIG_THREADS), lds_full, c->stream, a);
+                PMX_CHECK_HIP(hipGetLastError());
+                rc = 0;
+            } else if (real) rc = lds_matrix ? launch_tridiag<double, true>(c, a, grid, lds) : launch_tridiag<double, false>(c, a, grid, lds);
             else rc = lds_matrix ? launch_tridiag<zd, true>(c, a, grid, lds) : launch_tridiag<zd, false>(c, a, grid, lds);
             if (rc) return rc;
         }

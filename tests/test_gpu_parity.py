@@ -119,7 +119,7 @@ def test_spatial_path_equals_spinor_oracle_random(ctx):
         assert err < 1e-8
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("bf16x3", 5e-4), ("direct", 1e-8)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("bf16x3", 5e-3), ("direct", 1e-8)])
 def test_chi_precision_modes(manifest, ctx, precision, tol):
     """The mixed FP32 path within 1e-4 of each slice's max, the direct FP64
     windowed sum within 1e-8, on the C1 fixture."""
