@@ -183,8 +183,8 @@ def main():
     roof = {"kernel": name, "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": None, "avg_ms": round(avg_ms, 4),
             "launches": rec["count"]}
-    heev = [v for k, v in prof_mix.items() if k.startswith("heev")]
-    heev_ms = sum(v["ms"] for v in heev) / opt.steps
+    eig_names = ("tridiag_real", "tridiag_complex", "bisect", "invit", "backtrans_real", "backtrans_complex")
+    heev_ms = sum(v["ms"] for k, v in prof_mix.items() if k in eig_names) / opt.steps
     eigs_per_step = 2 * nk
 
     cpu = None

@@ -398,6 +398,35 @@ struct MomArgs2 {
     zd *M;                  /* [nbins][NT][npairs][256] FP64 (accumulated) */
 };
 
+template <class T> struct MomFetch {
+    T av, bre, bim, sv;
+};
+
+/* operands of K entry kk (A value, scaled B values, s) for this lane */
+template <class T, class C>
+__device__ __forceinline__ MomFetch<T> mom_fetch(const MomArgs2 &a, const C *X, int r0, int kk, int kend, int K2,
+                                                 int ia, int ib, int n0) {
+    MomFetch<T> f{0, 0, 0, 0};
+    if (kk < kend) {
+        const int row = r0 + kk / K2;
+        const int sub = kk % K2;
+        const int item = sub >> 1, part = sub & 1;
+        const int ch = (a.nch == 1) ? 0 : (2 * a.acc_index + item);
+        const C *xr = X + ((size_t)row * a.nch + ch) * a.D;
+        C xa = (ia < a.D) ? xr[ia] : C{0, 0};
+        C xb = (ib < a.D) ? xr[ib] : C{0, 0};
+        T w = (T)a.ws[row];
+        if (item) w = -w;
+        f.sv = (T)a.ss[row];
+        f.av = part ? xa.y : xa.x;
+        T sc = 1;
+        for (int j = 0; j < n0; j++) sc *= f.sv;
+        f.bre = w * sc * (part ? xb.y : xb.x);
+        f.bim = w * sc * (part ? xb.x : -xb.y);
+    }
+    return f;
+}
+
 template <class T> __global__ void __launch_bounds__(256) moment_mfma_kernel(MomArgs2 a) {
     typedef typename MfmaT<T>::V V;
     typedef typename MfmaT<T>::C C;
@@ -423,37 +452,22 @@ template <class T> __global__ void __launch_bounds__(256) moment_mfma_kernel(Mom
     }
     const int K2 = 2 * nitems; /* K entries per row */
     const int kend = (r1 - r0) * K2;
+    /* software pipeline: the next K step's operands load while this one's
+     * MFMAs run */
+    MomFetch<T> cur = mom_fetch<T, C>(a, X, r0, kq, kend, K2, ia, ib, n0);
     for (int kb = 0; kb < kend; kb += 4) {
-        const int kk = kb + kq;
-        T av = 0, bre = 0, bim = 0, sv = 0;
-        if (kk < kend) {
-            const int row = r0 + kk / K2;
-            const int sub = kk % K2;
-            const int item = sub >> 1, part = sub & 1;
-            const int ch = (a.nch == 1) ? 0 : (2 * a.acc_index + item);
-            const C *xr = X + ((size_t)row * a.nch + ch) * a.D;
-            C xa = (ia < a.D) ? xr[ia] : C{0, 0};
-            C xb = (ib < a.D) ? xr[ib] : C{0, 0};
-            T w = (T)a.ws[row];
-            if (item) w = -w;
-            sv = (T)a.ss[row];
-            av = part ? xa.y : xa.x;
-            bre = w * (part ? xb.y : xb.x);
-            bim = w * (part ? xb.x : -xb.y);
-            T sc = 1;
-            for (int j = 0; j < n0; j++) sc *= sv;
-            bre *= sc;
-            bim *= sc;
-        }
+        const MomFetch<T> nxt = mom_fetch<T, C>(a, X, r0, kb + 4 + kq, kend, K2, ia, ib, n0);
+        T bre = cur.bre, bim = cur.bim;
 #pragma unroll
         for (int j = 0; j < MAXW; j++) {
             if (j < nw) {
-                accr[j] = MfmaT<T>::mma(av, bre, accr[j]);
-                acci[j] = MfmaT<T>::mma(av, bim, acci[j]);
-                bre *= sv;
-                bim *= sv;
+                accr[j] = MfmaT<T>::mma(cur.av, bre, accr[j]);
+                acci[j] = MfmaT<T>::mma(cur.av, bim, acci[j]);
+                bre *= cur.sv;
+                bim *= cur.sv;
             }
         }
+        cur = nxt;
     }
 #pragma unroll
     for (int j = 0; j < MAXW; j++) {

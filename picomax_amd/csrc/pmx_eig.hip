@@ -385,17 +385,30 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_full_real_kernel(EigArgs 
                 continue;
             }
             __syncthreads(); /* (2) */
-            /* p_r = tau sum_c A[c][r] v_c over the trailing block */
+            /* p_r = tau sum_c A[c][r] v_c over the trailing block; loads of a
+             * group of 8 columns are issued before any arithmetic (the LDS
+             * arrays share one base, so the compiler cannot reorder across
+             * the group on its own), with independent partial sums */
             double pr = 0.0;
             if (r > k && r < N) {
                 const double *col = A + (size_t)(k + 1) * N + r;
+                double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
                 int c = k + 1;
-                for (; c + 3 < N; c += 4) {
-                    pr += col[0] * vv[c] + col[N] * vv[c + 1] + col[2 * N] * vv[c + 2] + col[3 * N] * vv[c + 3];
-                    col += 4 * N;
+                for (; c + 7 < N; c += 8) {
+                    double av[8], vc[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        av[u] = col[(size_t)u * N];
+                        vc[u] = vv[c + u];
+                    }
+                    p0 += av[0] * vc[0] + av[4] * vc[4];
+                    p1 += av[1] * vc[1] + av[5] * vc[5];
+                    p2 += av[2] * vc[2] + av[6] * vc[6];
+                    p3 += av[3] * vc[3] + av[7] * vc[7];
+                    col += 8 * (size_t)N;
                 }
-                for (; c < N; c++, col += N) pr += col[0] * vv[c];
-                pr *= tau;
+                for (; c < N; c++, col += N) p0 += col[0] * vv[c];
+                pr = tau * ((p0 + p1) + (p2 + p3));
             }
             double pv = wave_sum(pr * vr);
             if (lane == 0) red[EIG_WAVES + wave] = pv;
@@ -408,7 +421,20 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_full_real_kernel(EigArgs 
             __syncthreads(); /* (4) */
             if (r > k && r < N) {
                 double *col = A + (size_t)(k + 1) * N + r;
-                for (int c = k + 1; c < N; c++, col += N) *col -= vr * ww[c] + wr * vv[c];
+                int c = k + 1;
+                for (; c + 7 < N; c += 8) {
+                    double av[8], vc[8], wc[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        av[u] = col[(size_t)u * N];
+                        vc[u] = vv[c + u];
+                        wc[u] = ww[c + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u++) col[(size_t)u * N] = av[u] - (vr * wc[u] + wr * vc[u]);
+                    col += 8 * (size_t)N;
+                }
+                for (; c < N; c++, col += N) *col -= vr * ww[c] + wr * vv[c];
             }
         }
         __syncthreads();
@@ -551,74 +577,103 @@ __global__ void invit_kernel(int N, int nmat, int nvec, const double *dS, const 
     }
 }
 
-/* z_j = H_0 H_1 ... H_{N-2} y_j, one workgroup per matrix, vectors in LDS;
- * the reflectors are staged through LDS KC columns at a time, so the
- * sequential loop over k reads no global memory */
-constexpr int BT_KC = 16;
+/* z_j = H_0 H_1 ... H_{N-2} y_j, one workgroup per matrix.  Vectors are
+ * processed in groups of gsz (<= 32) held in LDS; the reflectors are staged
+ * through LDS kc columns at a time, so the sequential loop over k reads no
+ * global memory, and each wave applies them to its vectors together so the
+ * dot-product reductions of different vectors overlap. */
+constexpr int BT_MAXV = 8; /* vectors per wave per group */
 
 template <class T>
-__global__ void __launch_bounds__(EIG_THREADS) backtrans_kernel(int N, int nmat, int nvec, const double *Y,
-                                                                 const void *vref_, const void *tau_,
-                                                                 const double *phase, zd *evecs) {
+__global__ void __launch_bounds__(EIG_THREADS) backtrans_kernel(int N, int nmat, int nvec, int kc, int gsz,
+                                                                 const double *Y, const void *vref_,
+                                                                 const void *tau_, const double *phase, zd *evecs) {
     extern __shared__ double smem_d[];
     T *Z = reinterpret_cast<T *>(smem_d);
-    T *Vc = Z + (size_t)nvec * N;          /* [BT_KC][N] staged reflector columns (row index k+1+i) */
-    T *Tc = Vc + (size_t)BT_KC * N;        /* [BT_KC] */
+    T *Vc = Z + (size_t)gsz * N;           /* [kc][N] staged reflector columns (row index k+1+i) */
+    T *Tc = Vc + (size_t)kc * N;           /* [kc] */
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t P = (size_t)N * (N + 1) / 2;
     for (int mat = blockIdx.x; mat < nmat; mat += gridDim.x) {
         const T *vref = reinterpret_cast<const T *>(vref_) + (size_t)mat * P;
         const T *tau = reinterpret_cast<const T *>(tau_) + (size_t)mat * N;
-        for (int idx = tid; idx < nvec * N; idx += EIG_THREADS) {
-            const int jj = idx / N, i = idx % N;
-            Z[idx] = from_real<T>(Y[((size_t)jj * N + i) * nmat + mat]);
-        }
-        for (int kend = N - 2; kend >= 0; kend -= BT_KC) {
-            const int kbeg = max(0, kend - BT_KC + 1);
+        for (int j0 = 0; j0 < nvec; j0 += gsz) {
+            const int ng = min(gsz, nvec - j0);
             __syncthreads();
-            for (int idx = tid; idx < BT_KC * N; idx += EIG_THREADS) {
-                const int kc = idx / N, i = idx % N; /* column k = kbeg + kc, entry k+1+i */
-                const int k = kbeg + kc;
-                T v = zero<T>();
-                if (k <= kend && k + 1 + i < N) v = (i == 0) ? from_real<T>(1.0) : vref[pk(k + 1 + i, k, N)];
-                Vc[idx] = v;
+            for (int idx = tid; idx < ng * N; idx += EIG_THREADS) {
+                const int jj = idx / N, i = idx % N;
+                Z[idx] = from_real<T>(Y[((size_t)(j0 + jj) * N + i) * nmat + mat]);
             }
-            for (int kc = tid; kc < BT_KC; kc += EIG_THREADS) Tc[kc] = (kbeg + kc <= kend) ? tau[kbeg + kc] : zero<T>();
-            __syncthreads();
-            for (int jj = wave; jj < nvec; jj += EIG_WAVES) {
-                T *z = Z + (size_t)jj * N;
+            for (int kend = N - 2; kend >= 0; kend -= kc) {
+                const int kbeg = max(0, kend - kc + 1);
+                __syncthreads();
+                for (int idx = tid; idx < kc * N; idx += EIG_THREADS) {
+                    const int kk = idx / N, i = idx % N; /* column k = kbeg + kk, entry k+1+i */
+                    const int k = kbeg + kk;
+                    T v = zero<T>();
+                    if (k <= kend && k + 1 + i < N) v = (i == 0) ? from_real<T>(1.0) : vref[pk(k + 1 + i, k, N)];
+                    Vc[idx] = v;
+                }
+                for (int kk = tid; kk < kc; kk += EIG_THREADS) Tc[kk] = (kbeg + kk <= kend) ? tau[kbeg + kk] : zero<T>();
+                __syncthreads();
                 for (int k = kend; k >= kbeg; k--) {
                     const T tk = Tc[k - kbeg];
                     if (re(tk) == 0.0 && im(tk) == 0.0) continue;
                     const int L = N - k - 1;
                     const T *vk = Vc + (size_t)(k - kbeg) * N;
-                    double sr = 0.0, si = 0.0;
-                    for (int i = lane; i < L; i += 64) {
-                        T t2 = cj(vk[i]) * z[k + 1 + i];
-                        sr += re(t2);
-                        si += im(t2);
+                    double sr[BT_MAXV], si[BT_MAXV];
+#pragma unroll
+                    for (int u = 0; u < BT_MAXV; u++) {
+                        sr[u] = 0.0;
+                        si[u] = 0.0;
                     }
-                    sr = wave_sum(sr);
-                    if constexpr (sizeof(T) != sizeof(double)) si = wave_sum(si);
-                    T sdot;
-                    if constexpr (sizeof(T) == sizeof(double)) sdot = sr; else sdot = zd{sr, si};
-                    sdot = tk * sdot;
-                    for (int i = lane; i < L; i += 64) z[k + 1 + i] -= vk[i] * sdot;
+                    for (int i = lane; i < L; i += 64) {
+                        const T vc = cj(vk[i]);
+#pragma unroll
+                        for (int u = 0; u < BT_MAXV; u++) {
+                            const int jj = wave + EIG_WAVES * u;
+                            if (jj < ng) {
+                                T t2 = vc * Z[(size_t)jj * N + k + 1 + i];
+                                sr[u] += re(t2);
+                                si[u] += im(t2);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < BT_MAXV; u++) {
+                        if (wave + EIG_WAVES * u < ng) { /* wave-uniform */
+                            sr[u] = wave_sum(sr[u]);
+                            if constexpr (sizeof(T) != sizeof(double)) si[u] = wave_sum(si[u]);
+                        }
+                    }
+                    for (int i = lane; i < L; i += 64) {
+                        const T vi = vk[i];
+#pragma unroll
+                        for (int u = 0; u < BT_MAXV; u++) {
+                            const int jj = wave + EIG_WAVES * u;
+                            if (jj < ng) {
+                                T sdot;
+                                if constexpr (sizeof(T) == sizeof(double)) sdot = sr[u]; else sdot = zd{sr[u], si[u]};
+                                sdot = tk * sdot;
+                                Z[(size_t)jj * N + k + 1 + i] -= vi * sdot;
+                            }
+                        }
+                    }
                 }
             }
-        }
-        __syncthreads();
-        for (int idx = tid; idx < nvec * N; idx += EIG_THREADS) {
-            const int jj = idx / N, i = idx % N;
-            zd o;
-            if constexpr (sizeof(T) == sizeof(double)) {
-                double sn, cs;
-                sincos(phase[i], &sn, &cs);
-                o = zd{cs * Z[idx], sn * Z[idx]};
-            } else {
-                o = Z[idx];
+            __syncthreads();
+            for (int idx = tid; idx < ng * N; idx += EIG_THREADS) {
+                const int jj = idx / N, i = idx % N;
+                zd o;
+                if constexpr (sizeof(T) == sizeof(double)) {
+                    double sn, cs;
+                    sincos(phase[i], &sn, &cs);
+                    o = zd{cs * Z[idx], sn * Z[idx]};
+                } else {
+                    o = Z[idx];
+                }
+                evecs[((size_t)mat * nvec + j0 + jj) * N + i] = o;
             }
-            evecs[((size_t)mat * nvec + jj) * N + i] = o;
         }
         __syncthreads();
     }
@@ -644,12 +699,20 @@ template <class T, bool LDS> int launch_tridiag(pmx_ctx *c, EigArgs &a, int grid
 
 template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, const double *Y, const void *vref,
                                         const void *tau, const double *phase, zd *evecs) {
-    const size_t lds = sizeof(T) * ((size_t)nvec * N + (size_t)BT_KC * N + BT_KC);
+    /* staging depth and vector group size that fit the LDS */
+    const size_t cap = c->lds_max;
+    int kc = 16;
+    while (kc > 1 && sizeof(T) * (size_t)kc * (N + 1) > cap / 3) kc /= 2;
+    int gsz = std::min(nvec, 4 * BT_MAXV);
+    while (gsz > 1 && sizeof(T) * ((size_t)gsz * N + (size_t)kc * (N + 1)) > cap) gsz--;
+    const size_t lds = sizeof(T) * ((size_t)gsz * N + (size_t)kc * (N + 1));
+    if (lds > cap) return set_error(PMX_EINVAL, "basis too large for the back-transformation's LDS");
     auto kern = backtrans_kernel<T>;
     if (lds > 65536)
         PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int grid = std::min(nmat, c->num_cu * 4);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(EIG_THREADS), lds, c->stream, N, nmat, nvec, Y, vref, tau, phase, evecs);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(EIG_THREADS), lds, c->stream, N, nmat, nvec, kc, gsz, Y, vref, tau,
+                       phase, evecs);
     PMX_CHECK_HIP(hipGetLastError());
     return 0;
 }
@@ -669,8 +732,6 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
     const bool lds_matrix = lds_bytes(N, elem, true) <= c->lds_max;
     const size_t lds = lds_bytes(N, elem, lds_matrix);
     if (lds > c->lds_max) return set_error(PMX_EINVAL, "basis too large for the eigensolver's LDS work arrays");
-    if (nvec > 0 && elem * ((size_t)nvec * N + (size_t)BT_KC * (N + 1)) > c->lds_max)
-        return set_error(PMX_EINVAL, "too many eigenvectors for the back-transformation's LDS");
     /* chunk the batch so that reflectors + SoA scratch stay near 4 GB */
     const size_t per_mat = (nvec > 0 ? P * elem + N * elem : 0) + sizeof(double) * (size_t)N * (2 + nb) +
                            sizeof(double) * (size_t)nvec * N * 4 + 64;
