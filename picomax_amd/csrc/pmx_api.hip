@@ -97,10 +97,27 @@ __global__ void vtable_kernel(const int *shell, const zd *sfac, const double *vg
     }
     Vpair[idx] = v;
     if (Vreal) {
-        double t = -(phase[i] - phase[j]);
+        /* from the lower element (ii >= jj) for both (i, j) and (j, i), so
+         * the table is exactly symmetric */
+        const int ii = max(i, j), jj = min(i, j);
+        const size_t lo = (size_t)ii * npw + jj;
+        zd u{0.0, 0.0};
+        const int shl = shell[lo];
+        if (shl >= 0) {
+            double ax = 0.0, ay = 0.0;
+            for (int s = 0; s < natom; s++) {
+                zd f = sfac[lo * natom + s];
+                ax += f.x * vg[s * nv + shl];
+                ay += f.y * vg[s * nv + shl];
+            }
+            ax /= (double)natom;
+            ay /= (double)natom;
+            u = zd{ax * PMX_RY2EV, ay * PMX_RY2EV};
+        }
+        double t = -(phase[ii] - phase[jj]);
         double s, c;
         sincos(t, &s, &c);
-        Vreal[idx] = v.x * c - v.y * s;
+        Vreal[idx] = u.x * c - u.y * s;
     }
 }
 

@@ -464,6 +464,206 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_full_real_kernel(EigArgs 
     }
 }
 
+/* Register-tiled Householder tridiagonalisation of a real symmetric H.
+ * The lower triangle is cut into TT x TT tiles, one tile per thread, held in
+ * VGPRs for the whole reduction; LDS carries only vectors (the current
+ * column, p) and the per-tile matvec partials, so each step costs 4 TT^2 FMAs
+ * per thread and three barriers.  Tiles are numbered by block column J
+ * descending: the columns retire in order, so whole waves go idle together
+ * and their SIMD slots pass to the other workgroup on the CU.
+ *
+ * Step k (x = column k of the updated matrix, in LDS):
+ *   v, tau, beta from x (every wave reduces |x|^2 itself, bit-identically);
+ *   tile (I, J) writes A_IJ v_J into slot (I, J) and A_IJ^T v_I into (J, I);
+ *   row r sums its slots: p = tau A v;  w = p - (tau/2)(p.v) v;
+ *   A_IJ -= v_I w_J^T + w_I v_J^T; the tiles of column k+1 write it out. */
+constexpr int TT = 8;       /* tile edge */
+constexpr int TS = TT + 2;  /* LDS stride of one block of a vector (bank spread) */
+constexpr int TILE_MAX_THREADS = 256;
+
+__device__ __forceinline__ int tvi(int r) { return (r / TT) * TS + (r % TT); }
+
+__global__ void __launch_bounds__(TILE_MAX_THREADS, 2) tridiag_tile_real_kernel(EigArgs a, int nblk) {
+    extern __shared__ double smem_d[];
+    const int N = a.N, NR = nblk * TT;
+    const int nvp = nblk * TS, nblkp = nblk | 1;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double *xk0 = smem_d, *xk1 = xk0 + nvp, *pp = xk1 + nvp;
+    double *vw = pp + nvp + (size_t)wave * nvp;                       /* this wave's copy of v */
+    double *part = pp + nvp + (size_t)(blockDim.x >> 6) * nvp;        /* [contributing block C][row block R][TS] */
+    double *dd = part + (size_t)nblk * nblkp * TS;                    /* [N] */
+    double *ee = dd + N, *tt = ee + N;
+    const int ntile = nblk * (nblk + 1) / 2;
+    const int mat = blockIdx.x;
+    const size_t P = (size_t)N * (N + 1) / 2;
+
+    /* tile of this thread: the m-th group holds block column J = nblk-1-m, rows I = J..nblk-1 */
+    int I = 0, J = 0;
+    const bool has_tile = tid < ntile;
+    if (has_tile) {
+        int m = (int)((sqrt(8.0 * tid + 1.0) - 1.0) * 0.5);
+        while (m * (m + 1) / 2 > tid) m--;
+        while ((m + 1) * (m + 2) / 2 <= tid) m++;
+        J = nblk - 1 - m;
+        I = J + (tid - m * (m + 1) / 2);
+    }
+    const bool diag = I == J;
+    const int r0 = I * TT, c0 = J * TT;
+    const double Kv[3] = {a.K[3 * mat], a.K[3 * mat + 1], a.K[3 * mat + 2]};
+    double *vref = a.vref ? reinterpret_cast<double *>(a.vref) + (size_t)mat * P : nullptr;
+
+    double A[TT][TT];
+#pragma unroll
+    for (int i = 0; i < TT; i++) {
+        const int r = r0 + i;
+        const double *row = a.Vreal + (size_t)r * a.npw + c0; /* Vreal is symmetric */
+#pragma unroll
+        for (int j = 0; j < TT; j++) A[i][j] = (has_tile && r < N && c0 + j < N) ? row[j] : 0.0;
+    }
+    if (diag)
+#pragma unroll
+        for (int i = 0; i < TT; i++)
+            if (r0 + i < N) A[i][i] = h_elem<double>(a, Kv, r0 + i, r0 + i);
+    if (has_tile && J == 0)
+#pragma unroll
+        for (int i = 0; i < TT; i++) xk0[tvi(r0 + i)] = A[i][0];
+    __syncthreads();
+    int cur = 0;
+    for (int k = 0; k + 1 < N; k++) {
+        const double *x = cur ? xk1 : xk0;
+        double *xn = cur ? xk0 : xk1;
+        /* Householder vector of column k; every wave forms it in its own copy */
+        double s2 = 0.0;
+        for (int r = k + 2 + lane; r < N; r += 64) {
+            const double t = x[tvi(r)];
+            s2 += t * t;
+        }
+        s2 = wave_sum(s2);
+        const double alpha = x[tvi(k + 1)];
+        double beta, tau, scal;
+        if (s2 == 0.0) {
+            beta = alpha;
+            tau = 0.0;
+            scal = 0.0;
+        } else {
+            beta = -copysign(sqrt(alpha * alpha + s2), alpha);
+            tau = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+        }
+        const int cmin = (k + 1) / TT; /* first block column with a column > k */
+        const bool active = has_tile && J >= cmin && tau != 0.0;
+        if (tau != 0.0) {
+            /* rows of blocks >= cmin hold column k (rows below are stale) */
+            for (int r = cmin * TT + lane; r < NR; r += 64) {
+                const double xr = x[tvi(r)];
+                vw[tvi(r)] = (r == k + 1) ? 1.0 : ((r > k + 1 && r < N) ? xr * scal : 0.0);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (active) {
+            double vI[TT], vJ[TT], pJ[TT];
+#pragma unroll
+            for (int i = 0; i < TT; i++) {
+                vI[i] = vw[tvi(r0 + i)];
+                vJ[i] = vw[tvi(c0 + i)];
+                pJ[i] = 0.0;
+            }
+            double *sI = part + ((size_t)J * nblkp + I) * TS;
+#pragma unroll
+            for (int i = 0; i < TT; i++) {
+                double pi = 0.0;
+#pragma unroll
+                for (int j = 0; j < TT; j++) {
+                    pi = fma(A[i][j], vJ[j], pi);
+                    pJ[j] = fma(A[i][j], vI[i], pJ[j]);
+                }
+                sI[i] = pi;
+            }
+            if (!diag) {
+                double *sJ = part + ((size_t)I * nblkp + J) * TS;
+#pragma unroll
+                for (int j = 0; j < TT; j++) sJ[j] = pJ[j];
+            }
+        }
+        __syncthreads(); /* (1) partials */
+        if (tau != 0.0 && tid < NR) {
+            const int r = tid, R = r / TT, i = r % TT;
+            double p = 0.0;
+            if (r > k && r < N) {
+                for (int C = cmin; C < nblk; C++) p += part[((size_t)C * nblkp + R) * TS + i];
+                p *= tau;
+                if (vref && r >= k + 2) vref[pk(r, k, N)] = vw[tvi(r)];
+            }
+            pp[tvi(r)] = p;
+        }
+        if (tid == 0) {
+            dd[k] = x[tvi(k)];
+            ee[k] = beta;
+            tt[k] = tau;
+        }
+        __syncthreads(); /* (2) p */
+        if (tau != 0.0) {
+            double pv = 0.0;
+            for (int r = k + 1 + lane; r < N; r += 64) pv = fma(pp[tvi(r)], vw[tvi(r)], pv);
+            pv = wave_sum(pv);
+            const double al = -0.5 * tau * pv;
+            if (active) {
+                double vI[TT], vJ[TT], wI[TT], wJ[TT];
+#pragma unroll
+                for (int i = 0; i < TT; i++) {
+                    vI[i] = vw[tvi(r0 + i)];
+                    vJ[i] = vw[tvi(c0 + i)];
+                    wI[i] = fma(al, vI[i], pp[tvi(r0 + i)]);
+                    wJ[i] = fma(al, vJ[i], pp[tvi(c0 + i)]);
+                }
+#pragma unroll
+                for (int i = 0; i < TT; i++)
+#pragma unroll
+                    for (int j = 0; j < TT; j++) A[i][j] = fma(-wI[i], vJ[j], fma(-vI[i], wJ[j], A[i][j]));
+            }
+        }
+        /* column k+1 for the next step */
+        if (has_tile && J == (k + 1) / TT) {
+            const int jc = (k + 1) % TT;
+#pragma unroll
+            for (int i = 0; i < TT; i++) {
+                double v = A[i][0];
+#pragma unroll
+                for (int j = 1; j < TT; j++) v = (j == jc) ? A[i][j] : v;
+                xn[tvi(r0 + i)] = v;
+            }
+        }
+        __syncthreads(); /* (3) next column */
+        cur ^= 1;
+    }
+    if (tid == 0) dd[N - 1] = (cur ? xk1 : xk0)[tvi(N - 1)];
+    __syncthreads();
+    for (int i = tid; i < N; i += blockDim.x) {
+        a.dS[(size_t)i * a.nmat + mat] = dd[i];
+        a.eS[(size_t)i * a.nmat + mat] = (i + 1 < N) ? ee[i] : 0.0;
+        if (a.tauv) reinterpret_cast<double *>(a.tauv)[(size_t)mat * N + i] = (i + 1 < N) ? tt[i] : 0.0;
+    }
+    if (tid == 0) {
+        double gl = dd[0], gu = dd[0], emax2 = 0.0;
+        for (int i = 0; i < N; i++) {
+            const double di = dd[i];
+            double off = (i > 0 ? fabs(ee[i - 1]) : 0.0) + (i + 1 < N ? fabs(ee[i]) : 0.0);
+            gl = fmin(gl, di - off);
+            gu = fmax(gu, di + off);
+            if (i + 1 < N) emax2 = fmax(emax2, ee[i] * ee[i]);
+        }
+        double tnorm = fmax(fabs(gl), fabs(gu));
+        gl -= 2.0 * DBL_EPSILON * tnorm * N + 1e-300;
+        gu += 2.0 * DBL_EPSILON * tnorm * N + 1e-300;
+        a.gb[4 * mat] = gl;
+        a.gb[4 * mat + 1] = gu;
+        a.gb[4 * mat + 2] = DBL_MIN * fmax(1.0, emax2);
+        a.gb[4 * mat + 3] = tnorm;
+    }
+}
+
 /* lowest nb eigenvalues of every T by bisection on Sturm counts, one thread
  * per (matrix, index); consecutive threads take consecutive matrices so the
  * SoA reads of d and e coalesce */
@@ -679,6 +879,139 @@ __global__ void __launch_bounds__(EIG_THREADS) backtrans_kernel(int N, int nmat,
     }
 }
 
+/* Blocked back-transformation of the real path, z = H_0 H_1 ... H_{N-2} y,
+ * in the compact WY form: BW consecutive reflectors multiply to
+ * H_kb ... H_kb+BW-1 = I - V T V^T, with T upper triangular from the
+ * forward recurrence T_ii = tau_i, T_{j,i} = -tau_i sum_m T_{j,m} (V^T V)_{m,i},
+ * and act as Z -= V (T (V^T Z)).  V^T V, V^T Z, T W and the update of Z are
+ * FP64 MFMA 16x16x4 products out of LDS; blocks run from the last to the
+ * first.  One workgroup (4 waves) per matrix, matrices mapped so that the
+ * workgroups of one XCD take consecutive matrices. */
+typedef double dx4 __attribute__((ext_vector_type(4)));
+constexpr int BW = 16;
+constexpr int VS = BW + 1;
+
+__device__ __forceinline__ int xcd_remap(int b, int G) {
+    const int x = b & 7, s = b >> 3;
+    int start = 0;
+    for (int y = 0; y < x; y++) start += (G - y + 7) >> 3;
+    return start + s;
+}
+
+__device__ __forceinline__ dx4 mma64(double a, double b, dx4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__global__ void __launch_bounds__(256) backtrans_wy_kernel(int N, int nmat, int nvec, int NV, const double *Y,
+                                                            const double *vref_all, const double *tau_all,
+                                                            const double *phase, zd *evecs) {
+    extern __shared__ double smem_d[];
+    const int Np = (N + 15) & ~15, ZS = NV + 1;
+    double *Zs = smem_d;                  /* [Np][ZS] */
+    double *Vs = Zs + (size_t)Np * ZS;    /* [Np][VS] */
+    double *Gs = Vs + (size_t)Np * VS;    /* [BW][VS] */
+    double *Ts = Gs + BW * VS;            /* [BW][VS] */
+    double *Ws = Ts + BW * VS;            /* [BW][ZS] */
+    double *taus = Ws + BW * ZS;          /* [BW] */
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, l16 = lane & 15, l4 = lane >> 4;
+    const int mat = xcd_remap(blockIdx.x, gridDim.x);
+    const size_t P = (size_t)N * (N + 1) / 2;
+    const double *vref = vref_all + (size_t)mat * P;
+    const double *tau = tau_all + (size_t)mat * N;
+    for (int idx = tid; idx < Np * NV; idx += 256) {
+        const int i = idx % Np, j = idx / Np;
+        Zs[i * ZS + j] = (i < N && j < nvec) ? Y[((size_t)j * N + i) * nmat + mat] : 0.0;
+    }
+    const int nref = N - 1, nct = NV / 16, nrt = Np / 16;
+    for (int bi = (nref + BW - 1) / BW - 1; bi >= 0; bi--) {
+        const int kb = bi * BW;
+        __syncthreads(); /* Z loaded or updated; V free */
+        for (int idx = tid; idx < Np * BW; idx += 256) {
+            const int r = idx % Np, kk = idx / Np, k = kb + kk;
+            double v = 0.0;
+            if (k < nref) {
+                if (r == k + 1) v = 1.0;
+                else if (r > k + 1 && r < N) v = vref[pk(r, k, N)];
+            }
+            Vs[r * VS + kk] = v;
+        }
+        if (tid < BW) taus[tid] = (kb + tid < nref) ? tau[kb + tid] : 0.0;
+        __syncthreads();
+        /* tile 0: G = V^T V; tile 1+c: W_c = V^T Z_c (sums over the rows) */
+        for (int t = wave; t <= nct; t += 4) {
+            const int cb = (t - 1) * 16;
+            dx4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+            for (int s = 0; s < Np / 4; s += 2) {
+                const int ra = 4 * s + l4, rb = ra + 4;
+                const double a0 = Vs[ra * VS + l16], a1 = Vs[rb * VS + l16];
+                const double b0 = (t == 0) ? a0 : Zs[ra * ZS + cb + l16];
+                const double b1 = (t == 0) ? a1 : Zs[rb * ZS + cb + l16];
+                acc0 = mma64(a0, b0, acc0);
+                acc1 = mma64(a1, b1, acc1);
+            }
+            double *out = (t == 0) ? Gs : Ws + cb;
+            const int os = (t == 0) ? VS : ZS;
+#pragma unroll
+            for (int v = 0; v < 4; v++) out[(l4 + 4 * v) * os + l16] = acc0[v] + acc1[v];
+        }
+        __syncthreads();
+        if (wave == 0 && lane < BW) {
+            /* row j = lane of T */
+            const int j = lane;
+            double trow[BW];
+#pragma unroll
+            for (int i = 0; i < BW; i++) {
+                double sacc = 0.0;
+#pragma unroll
+                for (int m = 0; m < i; m++) sacc = fma(trow[m], Gs[m * VS + i], sacc);
+                trow[i] = (i == j) ? taus[i] : ((i > j) ? -taus[i] * sacc : 0.0);
+            }
+#pragma unroll
+            for (int i = 0; i < BW; i++) Ts[j * VS + i] = trow[i];
+        }
+        __syncthreads();
+        /* W_c <- -(T W_c), in place (a wave reads all of its column block first) */
+        for (int cc = wave; cc < nct; cc += 4) {
+            dx4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < BW / 4; s++) {
+                const int kk = 4 * s + l4;
+                acc = mma64(Ts[l16 * VS + kk], Ws[kk * ZS + cc * 16 + l16], acc);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) Ws[(l4 + 4 * v) * ZS + cc * 16 + l16] = -acc[v];
+        }
+        __syncthreads();
+        /* Z += V W */
+        for (int t = wave; t < nrt * nct; t += 4) {
+            const int rt = t % nrt, cc = t / nrt;
+            dx4 acc;
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[v] = Zs[(rt * 16 + l4 + 4 * v) * ZS + cc * 16 + l16];
+#pragma unroll
+            for (int s = 0; s < BW / 4; s++) {
+                const int kk = 4 * s + l4;
+                acc = mma64(Vs[(rt * 16 + l16) * VS + kk], Ws[kk * ZS + cc * 16 + l16], acc);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) Zs[(rt * 16 + l4 + 4 * v) * ZS + cc * 16 + l16] = acc[v];
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < nvec * N; idx += 256) {
+        const int j = idx / N, i = idx % N;
+        double sn, cs;
+        sincos(phase[i], &sn, &cs);
+        const double z = Zs[i * ZS + j];
+        evecs[((size_t)mat * nvec + j) * N + i] = zd{cs * z, sn * z};
+    }
+}
+
+size_t wy_lds_bytes(int N, int nvec) {
+    const size_t Np = (N + 15) & ~15, NV = (nvec + 15) & ~15;
+    return sizeof(double) * (Np * (NV + 1) + Np * VS + 2 * BW * VS + BW * (NV + 1) + BW);
+}
+
 size_t lds_bytes(int N, size_t elem, bool lds_matrix) {
     size_t P = (size_t)N * (N + 1) / 2;
     size_t b = lds_matrix ? ((P * elem + 15) / 16) * 16 : 0;
@@ -699,6 +1032,17 @@ template <class T, bool LDS> int launch_tridiag(pmx_ctx *c, EigArgs &a, int grid
 
 template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, const double *Y, const void *vref,
                                         const void *tau, const double *phase, zd *evecs) {
+    if (sizeof(T) == sizeof(double) && N >= 2 && wy_lds_bytes(N, nvec) <= c->lds_max) {
+        const size_t lds = wy_lds_bytes(N, nvec);
+        if (lds > 65536)
+            PMX_CHECK_HIP(hipFuncSetAttribute((const void *)backtrans_wy_kernel,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(backtrans_wy_kernel, dim3(nmat), dim3(256), lds, c->stream, N, nmat, nvec,
+                           (nvec + 15) & ~15, Y, reinterpret_cast<const double *>(vref),
+                           reinterpret_cast<const double *>(tau), phase, evecs);
+        PMX_CHECK_HIP(hipGetLastError());
+        return 0;
+    }
     /* staging depth and vector group size that fit the LDS */
     const size_t cap = c->lds_max;
     int kc = 16;
@@ -765,7 +1109,16 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
         {
             Prof pf(c, real ? "tridiag_real" : "tridiag_complex", nm * (real ? (4.0 / 3.0) * n3 : (16.0 / 3.0) * n3), 0.0);
             const size_t lds_full = sizeof(double) * ((size_t)N * N + 4 * (size_t)N + 2 * EIG_WAVES);
-            if (real && N <= EIG_THREADS && lds_full <= c->lds_max) {
+            const int nblk = (N + TT - 1) / TT;
+            const int tile_threads = (std::max(nblk * (nblk + 1) / 2, nblk * TT) + 63) / 64 * 64;
+            if (real && tile_threads <= TILE_MAX_THREADS) {
+                const size_t lds_tile = sizeof(double) * ((2 + 1 + tile_threads / 64) * (size_t)nblk * TS +
+                                                          (size_t)nblk * (nblk | 1) * TS + 3 * (size_t)N);
+                hipLaunchKernelGGL(tridiag_tile_real_kernel, dim3(nm), dim3(tile_threads), lds_tile, c->stream, a,
+                                   nblk);
+                PMX_CHECK_HIP(hipGetLastError());
+                rc = 0;
+            } else if (real && N <= EIG_THREADS && lds_full <= c->lds_max) {
                 auto kern = tridiag_full_real_kernel;
                 PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                   (int)lds_full));
