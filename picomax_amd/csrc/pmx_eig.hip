@@ -76,6 +76,28 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+/* Sum over a full wave (every lane active) by DPP within rows of 16 and
+ * four lane reads; every lane gets the same bits, in every wave alike. */
+template <int CTRL> __device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double wave_sum_full(double v) {
+    v += dpp_f64<0xB1>(v);  /* quad_perm [1,0,3,2] */
+    v += dpp_f64<0x4E>(v);  /* quad_perm [2,3,0,1] */
+    v += dpp_f64<0x141>(v); /* row_half_mirror */
+    v += dpp_f64<0x140>(v); /* row_mirror */
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
 /* packed column-major lower storage: element (i, j), i >= j */
 __device__ __forceinline__ int pk(int i, int j, int N) { return j * (2 * N - j + 1) / 2 + (i - j); }
 
@@ -538,7 +560,7 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, 2) tridiag_tile_real_kernel(
             const double t = x[tvi(r)];
             s2 += t * t;
         }
-        s2 = wave_sum(s2);
+        s2 = wave_sum_full(s2);
         const double alpha = x[tvi(k + 1)];
         double beta, tau, scal;
         if (s2 == 0.0) {
@@ -592,7 +614,13 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, 2) tridiag_tile_real_kernel(
             const int r = tid, R = r / TT, i = r % TT;
             double p = 0.0;
             if (r > k && r < N) {
-                for (int C = cmin; C < nblk; C++) p += part[((size_t)C * nblkp + R) * TS + i];
+                /* every slot is read (independent loads); those of retired blocks hold stale data */
+                const double *col = part + (size_t)R * TS + i;
+#pragma unroll 6
+                for (int C = 0; C < nblk; C++) {
+                    const double t = col[(size_t)C * nblkp * TS];
+                    p += (C >= cmin) ? t : 0.0;
+                }
                 p *= tau;
                 if (vref && r >= k + 2) vref[pk(r, k, N)] = vw[tvi(r)];
             }
@@ -607,7 +635,7 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, 2) tridiag_tile_real_kernel(
         if (tau != 0.0) {
             double pv = 0.0;
             for (int r = k + 1 + lane; r < N; r += 64) pv = fma(pp[tvi(r)], vw[tvi(r)], pv);
-            pv = wave_sum(pv);
+            pv = wave_sum_full(pv);
             const double al = -0.5 * tau * pv;
             if (active) {
                 double vI[TT], vJ[TT], wI[TT], wJ[TT];
