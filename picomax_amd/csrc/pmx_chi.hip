@@ -317,32 +317,98 @@ template <class T> __global__ void __launch_bounds__(256) rho_mfma_kernel(RhoArg
  * binning: f0 = int((dE - w0)/df) exactly as the reference, bin = f0 + wh;
  * transitions whose window misses [0, nfreq) are dropped.
  * ------------------------------------------------------------------ */
-__global__ void bin_kernel(int nt, const double *dE, double w0, double df, int wh, int nbins, int *bin, int *hist) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    int f0 = int((dE[t] - w0) / df);
-    int b = f0 + wh;
-    if (b < 0 || b >= nbins) b = -1;
-    bin[t] = b;
-    if (b >= 0) atomicAdd(&hist[b], 1);
+/* Stable counting sort of the transitions by frequency bin, on the device.
+ * Transitions are cut into chunks of SB; the rows of a bin end up in
+ * ascending transition order, so the sums over a bin do not depend on
+ * scheduling.  (1) per-chunk bin counts; (2) per bin, the exclusive prefix
+ * of those counts over the chunks, and the bin total; (3) bin starts;
+ * (4) each transition's row = start + chunk prefix + rank among the equal
+ * bins before it in its chunk, and the row attributes: weight, dE and the
+ * offset s of dE from its bin centre w0 + (f0 + 1/2) df in units of df/2. */
+constexpr int SB = 256;
+
+__global__ void __launch_bounds__(SB) bin_count_kernel(int nt, const double *dE, double w0, double df, int wh,
+                                                       int nbins, int *bin, int *cnt) {
+    extern __shared__ int hcount[];
+    for (int i = threadIdx.x; i < nbins; i += SB) hcount[i] = 0;
+    __syncthreads();
+    const int t = blockIdx.x * SB + threadIdx.x;
+    if (t < nt) {
+        const int f0 = int((dE[t] - w0) / df);
+        int b = f0 + wh;
+        if (b < 0 || b >= nbins) b = -1;
+        bin[t] = b;
+        if (b >= 0) atomicAdd(&hcount[b], 1);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbins; i += SB) cnt[(size_t)blockIdx.x * nbins + i] = hcount[i];
 }
 
-__global__ void scatter_kernel(int nt, const int *bin, int *cursor, int *perm) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    int b = bin[t];
-    if (b < 0) return;
-    int pos = atomicAdd(&cursor[b], 1);
-    perm[pos] = t;
+/* one workgroup per bin: cnt[c][b] <- sum_{c' < c} cnt[c'][b]; total[b] */
+__global__ void __launch_bounds__(256) bin_prefix_kernel(int nchunk, int nbins, int *cnt, int *total) {
+    __shared__ int part[256];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int per = (nchunk + 255) / 256, c0 = tid * per, c1 = min(nchunk, c0 + per);
+    int s = 0;
+    for (int c = c0; c < c1; c++) s += cnt[(size_t)c * nbins + b];
+    part[tid] = s;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) { /* inclusive Hillis-Steele scan */
+        const int v = tid >= o ? part[tid - o] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = part[tid] - s;
+    for (int c = c0; c < c1; c++) {
+        const size_t i = (size_t)c * nbins + b;
+        const int v = cnt[i];
+        cnt[i] = run;
+        run += v;
+    }
+    if (tid == 255) total[b] = part[255];
 }
 
-/* sorted-row attributes: pos[t], weight, and the scaled offset s of dE from
- * its bin centre w0 + (f0 + 1/2) df, in units of df/2 */
-__global__ void sorted_attr_kernel(int nsorted, const int *perm, const double *dE, const double *wt, double w0,
-                                   double df, int *pos, double *ws, double *ss, double *es) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nsorted) return;
-    const int t = perm[i];
+/* start[b] = sum_{b' < b} total[b'], start[nbins] = number of binned rows */
+__global__ void __launch_bounds__(256) bin_start_kernel(int nbins, const int *total, int *start) {
+    __shared__ int part[256];
+    const int tid = threadIdx.x;
+    const int per = (nbins + 255) / 256, b0 = tid * per, b1 = min(nbins, b0 + per);
+    int s = 0;
+    for (int b = b0; b < b1; b++) s += total[b];
+    part[tid] = s;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const int v = tid >= o ? part[tid - o] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = part[tid] - s;
+    for (int b = b0; b < b1; b++) {
+        start[b] = run;
+        run += total[b];
+    }
+    if (tid == 255) start[nbins] = part[255];
+}
+
+__global__ void __launch_bounds__(SB) bin_place_kernel(int nt, int nbins, const int *bin, const int *cnt,
+                                                       const int *start, const double *dE, const double *wt,
+                                                       double w0, double df, int *pos, double *ws, double *ss,
+                                                       double *es) {
+    __shared__ int bl[SB];
+    const int t = blockIdx.x * SB + threadIdx.x;
+    const int b = t < nt ? bin[t] : -1;
+    bl[threadIdx.x] = b;
+    __syncthreads();
+    if (t >= nt) return;
+    if (b < 0) {
+        pos[t] = -1;
+        return;
+    }
+    int rank = 0;
+    for (int j = 0; j < (int)threadIdx.x; j++) rank += (bl[j] == b);
+    const int i = start[b] + cnt[(size_t)blockIdx.x * nbins + b] + rank;
     pos[t] = i;
     const double e = dE[t];
     const int f0 = int((e - w0) / df);
@@ -395,7 +461,8 @@ struct MomArgs2 {
     const void *X;          /* sorted rows [row][nch][D] complex of T */
     const double *ws, *ss;  /* sorted rows */
     const int *start;
-    zd *M;                  /* [nbins][NT][npairs][256] FP64 (accumulated) */
+    void *M;                /* [nbins][NT][npairs][256] complex: FP64 (zd), or FP32 (float2) if m32 */
+    int m32;
 };
 
 template <class T> struct MomFetch {
@@ -432,8 +499,7 @@ template <class T> __global__ void __launch_bounds__(256) moment_mfma_kernel(Mom
     typedef typename MfmaT<T>::C C;
     constexpr int MAXW = 5;
     const int pairi = blockIdx.x, b = blockIdx.y;
-    const int r0 = a.start[b], r1 = a.start[b + 1];
-    if (r0 == r1) return;
+    const int r0 = a.start[b], r1 = a.start[b + 1]; /* an empty bin writes zero moments */
     int ti, tj;
     tile_pair(pairi, ti, tj);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -472,13 +538,14 @@ template <class T> __global__ void __launch_bounds__(256) moment_mfma_kernel(Mom
 #pragma unroll
     for (int j = 0; j < MAXW; j++) {
         if (j < nw) {
-            zd *dst = a.M + (((size_t)b * a.NT + n0 + j) * a.npairs + pairi) * 256;
+            const size_t o = (((size_t)b * a.NT + n0 + j) * a.npairs + pairi) * 256;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int row = MfmaT<T>::drow(lane, v), col = lane & 15;
-                zd &d = dst[row * TILE + col];
-                d.x += (double)accr[j][v];
-                d.y += (double)acci[j][v];
+                if (a.m32)
+                    reinterpret_cast<float2 *>(a.M)[o + row * TILE + col] = float2{(float)accr[j][v], (float)acci[j][v]};
+                else
+                    reinterpret_cast<zd *>(a.M)[o + row * TILE + col] = zd{(double)accr[j][v], (double)acci[j][v]};
             }
         }
     }
@@ -509,8 +576,7 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
 __global__ void __launch_bounds__(256) moment_bf16x3_kernel(MomArgs2 a) {
     constexpr int MAXW = 5;
     const int pairi = blockIdx.x, b = blockIdx.y;
-    const int r0 = a.start[b], r1 = a.start[b + 1];
-    if (r0 == r1) return;
+    const int r0 = a.start[b], r1 = a.start[b + 1]; /* an empty bin writes zero moments */
     int ti, tj;
     tile_pair(pairi, ti, tj);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -590,13 +656,14 @@ __global__ void __launch_bounds__(256) moment_bf16x3_kernel(MomArgs2 a) {
 #pragma unroll
     for (int j = 0; j < MAXW; j++) {
         if (j < nw) {
-            zd *dst = a.M + (((size_t)b * a.NT + n0 + j) * a.npairs + pairi) * 256;
+            const size_t o = (((size_t)b * a.NT + n0 + j) * a.npairs + pairi) * 256;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int row = 4 * (lane >> 4) + v, col = lane & 15;
-                zd &d = dst[row * TILE + col];
-                d.x += (double)accr[j][v];
-                d.y += (double)acci[j][v];
+                if (a.m32)
+                    reinterpret_cast<float2 *>(a.M)[o + row * TILE + col] = float2{accr[j][v], acci[j][v]};
+                else
+                    reinterpret_cast<zd *>(a.M)[o + row * TILE + col] = zd{(double)accr[j][v], (double)acci[j][v]};
             }
         }
     }
@@ -689,45 +756,72 @@ __global__ void __launch_bounds__(256) direct_kernel(DirArgs a) {
  * ------------------------------------------------------------------ */
 struct ConvArgs {
     int npairs, nbins, NT, wh, nfreq;
-    const zd *M;
-    const double *Kt; /* [2wh+1][NT] */
+    const void *M;    /* [nbins][NT][npairs][256] complex of T */
+    const void *Kt;   /* [2wh+1][NT] of T */
     zd *Z;            /* [nfreq][npairs][256] */
 };
 
-/* Z(f) = sum_{D=-wh..wh} sum_n M[f+D+wh][n] K[D+wh][n] as an FP64 MFMA GEMM: for a block of 16 frequencies
- * Z[f][e] = sum_{(b,n)} Kc[f][(b,n)] M[(b,n)][e], Kc[f][(b,n)] = K_n(b - wh - f)
- * inside the window and 0 outside; b runs over the bins any f of the block
- * can see.  Block = 4 waves x 16 consecutive tile elements e. */
-__global__ void __launch_bounds__(256) conv_mfma_kernel(ConvArgs a) {
+/* Z(f, e) = sum_{b, n} Kt[b - wh - f][n] M[b][n][e]: a GEMM whose A operand
+ * is the banded Taylor-kernel table.  A wave owns 16 elements and 4 blocks of
+ * 16 frequencies, so every M value it loads feeds 4 MFMA pairs; k steps that
+ * miss a block's band are skipped for that block.  Kt is staged in LDS.  T =
+ * double (FP64 MFMA 16x16x4) or float (FP32 MFMA 16x16x4, the mixed paths). */
+constexpr int CONV_FB = 4; /* frequency blocks of 16 per wave */
+template <class T> __global__ void __launch_bounds__(256) conv_mfma_kernel(ConvArgs a) {
+    typedef typename MfmaT<T>::V V;
+    typedef typename MfmaT<T>::C C;
+    extern __shared__ double smem_c[];
+    T *kt = reinterpret_cast<T *>(smem_c);
+    const int W = 2 * a.wh + 1;
+    for (int i = threadIdx.x; i < W * a.NT; i += 256) kt[i] = reinterpret_cast<const T *>(a.Kt)[i];
+    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int E = a.npairs * 256;
     const int e0 = (blockIdx.x * 4 + wave) * 16;
     if (e0 >= E) return;
-    const int fb = blockIdx.y * 16;
-    const int fa = fb + (lane & 15);        /* A row: frequency */
-    const int eb = e0 + (lane & 15);        /* B column: element */
+    const int fb0 = blockIdx.y * 16 * CONV_FB;
+    const int eb = e0 + (lane & 15); /* B column: element */
     const int kq = lane >> 4;
-    const int b_lo = fb, b_hi = min(a.nbins - 1, fb + 15 + 2 * a.wh);
+    const int b_lo = fb0, b_hi = min(a.nbins - 1, fb0 + 16 * CONV_FB - 1 + 2 * a.wh);
     const int kcount = (b_hi - b_lo + 1) * a.NT;
-    dx4 accR = {0, 0, 0, 0}, accI = {0, 0, 0, 0};
+    const C *M = reinterpret_cast<const C *>(a.M);
+    V accR[CONV_FB], accI[CONV_FB];
+#pragma unroll
+    for (int q = 0; q < CONV_FB; q++) {
+        accR[q] = V{0, 0, 0, 0};
+        accI[q] = V{0, 0, 0, 0};
+    }
     for (int k0 = 0; k0 < kcount; k0 += 4) {
         const int kk = k0 + kq;
-        double av = 0.0, br = 0.0, bi = 0.0;
+        T br = 0, bi = 0;
+        int b = b_lo, n = 0;
         if (kk < kcount) {
-            const int b = b_lo + kk / a.NT, n = kk % a.NT;
-            const int Dd = b - a.wh - fa;
-            if (fa < a.nfreq && Dd >= -a.wh && Dd <= a.wh) av = a.Kt[(Dd + a.wh) * a.NT + n];
-            const zd m = a.M[((size_t)b * a.NT + n) * E + eb];
+            b = b_lo + kk / a.NT;
+            n = kk % a.NT;
+            const C m = M[((size_t)b * a.NT + n) * E + eb];
             br = m.x;
             bi = m.y;
         }
-        accR = __builtin_amdgcn_mfma_f64_16x16x4f64(av, br, accR, 0, 0, 0);
-        accI = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bi, accI, 0, 0, 0);
+        /* bins of this k step (wave-uniform range) */
+        const int bs0 = b_lo + k0 / a.NT, bs1 = b_lo + min(k0 + 3, kcount - 1) / a.NT;
+#pragma unroll
+        for (int q = 0; q < CONV_FB; q++) {
+            const int fq = fb0 + 16 * q;
+            if (bs1 < fq || bs0 > fq + 15 + 2 * a.wh) continue; /* outside this block's band */
+            const int fa = fq + (lane & 15);                      /* A row: frequency */
+            const int Dd = b - a.wh - fa;
+            const T av = (kk < kcount && fa < a.nfreq && Dd >= -a.wh && Dd <= a.wh) ? kt[(Dd + a.wh) * a.NT + n] : T(0);
+            accR[q] = MfmaT<T>::mma(av, br, accR[q]);
+            accI[q] = MfmaT<T>::mma(av, bi, accI[q]);
+        }
     }
-    for (int v = 0; v < 4; v++) {
-        const int f = fb + (lane >> 4) + 4 * v; /* f64 D layout */
-        if (f < a.nfreq) a.Z[(size_t)f * E + eb] = zd{accR[v], accI[v]};
-    }
+#pragma unroll
+    for (int q = 0; q < CONV_FB; q++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int f = fb0 + 16 * q + MfmaT<T>::drow(lane, v);
+            if (f < a.nfreq) a.Z[(size_t)f * E + eb] = zd{(double)accR[q][v], (double)accI[q][v]};
+        }
 }
 
 /* ------------------------------------------------------------------
@@ -995,7 +1089,8 @@ struct ChiWork {
     DBuf<double> ws, ss, es;
     DBuf<zd> vk, vkq, Xd, M1, M2, Z1, Z2, wAd, ucd, wSd;
     DBuf<float2> Xf;
-    DBuf<int> nc, nv, tbase, bind, hist, start, cursor, perm, pos, bad;
+    DBuf<float> Ktf;
+    DBuf<int> nc, nv, tbase, bind, hist, start, cnt, pos, bad;
 };
 
 } // namespace pmx
@@ -1062,6 +1157,8 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
     if (!direct) {
         taylor_table(p->delta, p->epsilon, df, wh, NT, Kt);
         if ((rc = upload(st, w.Ktd, Kt))) return rc;
+        std::vector<float> Ktf(Kt.begin(), Kt.end());
+        if ((rc = upload(st, w.Ktf, Ktf))) return rc;
     }
     if (p->kk) {
         if ((rc = w.Kkd.ensure((size_t)p->nfreq * p->nfreq))) return rc;
@@ -1162,45 +1259,29 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         if ((rc = upload(st, w.tbase, tb))) return rc;
         const int ntr1 = std::max(ntr, 1);
         if ((rc = w.dEd.ensure(ntr1)) || (rc = w.wtd.ensure(ntr1)) || (rc = w.bind.ensure(ntr1)) ||
-            (rc = w.perm.ensure(ntr1)) || (rc = w.pos.ensure(ntr1)) || (rc = w.ws.ensure(ntr1)) ||
-            (rc = w.ss.ensure(ntr1)) || (rc = w.es.ensure(ntr1)) || (rc = w.hist.ensure(nbins)) ||
-            (rc = w.start.ensure(nbins + 1)) || (rc = w.cursor.ensure(nbins)))
+            (rc = w.pos.ensure(ntr1)) || (rc = w.ws.ensure(ntr1)) || (rc = w.ss.ensure(ntr1)) ||
+            (rc = w.es.ensure(ntr1)) || (rc = w.hist.ensure(nbins)) || (rc = w.start.ensure(nbins + 1)))
             return rc;
         hipLaunchKernelGGL(dE_kernel, dim3(p->nk), dim3(256), 0, st, p->nk, nb_s, p->nband, spatial ? 1 : 0, w.evk.p,
                            w.evkq.p, p->offset, w.nc.p, w.nv.p, w.tbase.p, w.KWd.p, w.dEd.p, w.wtd.p);
 
-        /* ---- bin by f0 (counting sort); rows of a bin become contiguous */
-        PMX_CHECK_HIP(hipMemsetAsync(w.hist.p, 0, sizeof(int) * nbins, st));
-        PMX_CHECK_HIP(hipMemsetAsync(w.pos.p, 0xff, sizeof(int) * ntr1, st));
-        if (ntr > 0)
-            hipLaunchKernelGGL(bin_kernel, dim3((ntr + 255) / 256), dim3(256), 0, st, ntr, w.dEd.p, w0, df, wh, nbins,
-                               w.bind.p, w.hist.p);
-        std::vector<int> hist(nbins), start(nbins + 1);
-        PMX_CHECK_HIP(hipMemcpyAsync(hist.data(), w.hist.p, sizeof(int) * nbins, hipMemcpyDeviceToHost, st));
-        PMX_CHECK_HIP(hipStreamSynchronize(st));
-        start[0] = 0;
-        for (int b = 0; b < nbins; b++) start[b + 1] = start[b] + hist[b];
-        const int nsorted = start[nbins];
-        if ((rc = upload(st, w.start, start))) return rc;
-        PMX_CHECK_HIP(hipMemcpyAsync(w.cursor.p, w.start.p, sizeof(int) * nbins, hipMemcpyDeviceToDevice, st));
-        if (ntr > 0)
-            hipLaunchKernelGGL(scatter_kernel, dim3((ntr + 255) / 256), dim3(256), 0, st, ntr, w.bind.p, w.cursor.p,
-                               w.perm.p);
-        if (nsorted > 0) {
-            /* transition order inside a bin only reorders the sum; sorting
-             * it keeps the result identical run to run */
-            std::vector<int> perm(nsorted);
-            PMX_CHECK_HIP(hipMemcpyAsync(perm.data(), w.perm.p, sizeof(int) * nsorted, hipMemcpyDeviceToHost, st));
-            PMX_CHECK_HIP(hipStreamSynchronize(st));
-            for (int b = 0; b < nbins; b++) std::sort(perm.begin() + start[b], perm.begin() + start[b + 1]);
-            if ((rc = upload(st, w.perm, perm))) return rc;
-            hipLaunchKernelGGL(sorted_attr_kernel, dim3((nsorted + 255) / 256), dim3(256), 0, st, nsorted, w.perm.p,
-                               w.dEd.p, w.wtd.p, w0, df, w.pos.p, w.ws.p, w.ss.p, w.es.p);
+        /* ---- bin by f0 (stable counting sort on the device); rows of a bin become contiguous */
+        const int nchunk = std::max((ntr + SB - 1) / SB, 1);
+        if ((rc = w.cnt.ensure((size_t)nchunk * nbins))) return rc;
+        if (ntr > 0) {
+            hipLaunchKernelGGL(bin_count_kernel, dim3(nchunk), dim3(SB), sizeof(int) * nbins, st, ntr, w.dEd.p, w0, df,
+                               wh, nbins, w.bind.p, w.cnt.p);
+            hipLaunchKernelGGL(bin_prefix_kernel, dim3(nbins), dim3(256), 0, st, nchunk, nbins, w.cnt.p, w.hist.p);
+            hipLaunchKernelGGL(bin_start_kernel, dim3(1), dim3(256), 0, st, nbins, w.hist.p, w.start.p);
+            hipLaunchKernelGGL(bin_place_kernel, dim3(nchunk), dim3(SB), 0, st, ntr, nbins, w.bind.p, w.cnt.p,
+                               w.start.p, w.dEd.p, w.wtd.p, w0, df, w.pos.p, w.ws.p, w.ss.p, w.es.p);
+        } else {
+            PMX_CHECK_HIP(hipMemsetAsync(w.start.p, 0, sizeof(int) * (nbins + 1), st));
         }
 
         /* ---- rho into sorted rows */
         const bool fp32 = !direct && (p->precision == PMX_PREC_FP32 || p->precision == PMX_PREC_BF16X3);
-        const size_t xrows = (size_t)std::max(nsorted, 1) * nch * D;
+        const size_t xrows = (size_t)ntr1 * nch * D; /* binned rows <= transitions */
         if (fp32) {
             if ((rc = w.Xf.ensure(xrows))) return rc;
         } else if ((rc = w.Xd.ensure(xrows))) {
@@ -1215,10 +1296,10 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         ra.K = w.Kd.p;
         ra.X = fp32 ? nullptr : w.Xd.p;
         ra.Xf = fp32 ? w.Xf.p : nullptr;
-        if (nsorted > 0) {
+        if (ntr > 0) {
             /* algorithmic work of the gather-GEMM (src/pmx_chi.cpp:534-544): 8 flops per complex MAC,
              * 3 weighted (+1 plain) sums per (m, p) and transition; bytes = the eigenvector rows read */
-            const double macs = (double)nsorted * neps * npw * (spatial ? 4.0 : 16.0);
+            const double macs = (double)ntr * neps * npw * (spatial ? 4.0 : 16.0);
             Prof pf(c, "rho", 8.0 * macs, 16.0 * (double)N * (2 * nb_s) * p->nk);
             if (spatial && fp32)
                 hipLaunchKernelGGL(rho_mfma_kernel<float>, dim3(p->nk), dim3(256), 0, st, ra);
@@ -1234,22 +1315,22 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         for (int acc = 0; acc < nacc; acc++) {
             DBuf<zd> &Z = acc == 0 ? w.Z1 : w.Z2;
             DBuf<zd> &M = acc == 0 ? w.M1 : w.M2;
-            PMX_CHECK_HIP(hipMemsetAsync(Z.p, 0, sizeof(zd) * (size_t)p->nfreq * npairs * 256, st));
-            if (nsorted == 0) continue;
+            if (direct || ntr == 0)
+                PMX_CHECK_HIP(hipMemsetAsync(Z.p, 0, sizeof(zd) * (size_t)p->nfreq * npairs * 256, st));
+            if (ntr == 0) continue;
             if (direct) {
                 DirArgs da{D, nch, npairs, nbins, wh, p->nfreq, p->delta, acc, p->epsilon, w0, df,
                            w.Xd.p, w.es.p, w.ws.p, w.start.p, Z.p};
-                Prof pf(c, "direct", 8.0 * 9.0 * ne2 * nsorted * (2 * wh + 1) * (spatial ? 1 : 2), 0.0);
+                Prof pf(c, "direct", 8.0 * 9.0 * ne2 * ntr * (2 * wh + 1) * (spatial ? 1 : 2), 0.0);
                 hipLaunchKernelGGL(direct_kernel, dim3(npairs, nbins), dim3(256), 0, st, da);
             } else {
-                PMX_CHECK_HIP(hipMemsetAsync(M.p, 0, sizeof(zd) * (size_t)nbins * NT * npairs * 256, st));
                 MomArgs2 ma{D, nch, npairs, nbins, NT, (NT + 3) / 4, acc, fp32 ? (const void *)w.Xf.p : (const void *)w.Xd.p,
-                            w.ws.p, w.ss.p, w.start.p, M.p};
+                            w.ws.p, w.ss.p, w.start.p, M.p, fp32 ? 1 : 0};
                 {
                     /* algorithmic: 8 flops per complex MAC for each stored element
                      * (i,j,m,n<=m), order n and rank-1 term (1 per transition, 2 with SOC) */
-                    const double fl = 8.0 * 9.0 * ne2 * NT * (double)nsorted * (spatial ? 1 : 2);
-                    const double by = (double)nsorted * nch * D * (fp32 ? 8.0 : 16.0) +
+                    const double fl = 8.0 * 9.0 * ne2 * NT * (double)ntr * (spatial ? 1 : 2);
+                    const double by = (double)ntr * nch * D * (fp32 ? 8.0 : 16.0) +
                                       16.0 * (double)nbins * NT * npairs * 256 * 2;
                     const bool bf = p->precision == PMX_PREC_BF16X3;
                     Prof pf(c, bf ? "moment_bf16x3" : (fp32 ? "moment_mfma_f32" : "moment_mfma_f64"), fl, by);
@@ -1261,10 +1342,22 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                         hipLaunchKernelGGL(moment_mfma_kernel<double>, dim3(npairs, nbins), dim3(256), 0, st, ma);
                 }
                 {
-                    Prof pf(c, "conv", 4.0 * 9.0 * ne2 * p->nfreq * (2 * wh + 1) * NT, 0.0);
-                    ConvArgs ca{npairs, nbins, NT, wh, p->nfreq, M.p, w.Ktd.p, Z.p};
-                    hipLaunchKernelGGL(conv_mfma_kernel, dim3((npairs * 256 + 63) / 64, (p->nfreq + 15) / 16),
-                                       dim3(256), 0, st, ca);
+                    Prof pf(c, fp32 ? "conv_f32" : "conv_f64", 4.0 * 9.0 * ne2 * p->nfreq * (2 * wh + 1) * NT, 0.0);
+                    const dim3 grid((npairs * 256 + 63) / 64, (p->nfreq + 16 * CONV_FB - 1) / (16 * CONV_FB));
+                    const size_t W = 2 * (size_t)wh + 1;
+                    const size_t lds_c = (fp32 ? sizeof(float) : sizeof(double)) * W * NT;
+                    if (lds_c > c->lds_max) return set_error(PMX_EINVAL, "broadening window too wide for the convolution's LDS table; use -precision direct");
+                    if (lds_c > 65536) {
+                        const void *kf = fp32 ? (const void *)conv_mfma_kernel<float> : (const void *)conv_mfma_kernel<double>;
+                        PMX_CHECK_HIP(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_c));
+                    }
+                    if (fp32) {
+                        ConvArgs ca{npairs, nbins, NT, wh, p->nfreq, M.p, w.Ktf.p, Z.p};
+                        hipLaunchKernelGGL(conv_mfma_kernel<float>, grid, dim3(256), sizeof(float) * W * NT, st, ca);
+                    } else {
+                        ConvArgs ca{npairs, nbins, NT, wh, p->nfreq, M.p, w.Ktd.p, Z.p};
+                        hipLaunchKernelGGL(conv_mfma_kernel<double>, grid, dim3(256), sizeof(double) * W * NT, st, ca);
+                    }
                 }
             }
             PMX_CHECK_HIP(hipGetLastError());

@@ -693,25 +693,38 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, 2) tridiag_tile_real_kernel(
 }
 
 /* lowest nb eigenvalues of every T by bisection on Sturm counts, one thread
- * per (matrix, index); consecutive threads take consecutive matrices so the
- * SoA reads of d and e coalesce */
-__global__ void bisect_kernel(int N, int nmat, int nb, const double *dS, const double *eS, const double *gb,
-                              double *lamS, double *evals) {
-    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (t >= (size_t)nmat * nb) return;
-    const int mat = t % nmat, j = t / nmat;
+ * per (matrix, index).  A workgroup takes mpw consecutive matrices and
+ * stages their d and e^2 in LDS once, so the ~50 counts per eigenvalue read
+ * LDS rather than re-streaming T from L2. */
+__global__ void __launch_bounds__(256) bisect_kernel(int N, int nmat, int nb, int mpw, const double *dS,
+                                                     const double *eS, const double *gb, double *lamS, double *evals) {
+    extern __shared__ double smem_d[];
+    double *dl = smem_d;                    /* [mpw][N] */
+    double *e2l = dl + (size_t)mpw * N;     /* [mpw][N]: e_{i-1}^2 at i */
+    const int m0 = blockIdx.x * mpw;
+    const int nm = min(mpw, nmat - m0);
+    for (int idx = threadIdx.x; idx < nm * N; idx += blockDim.x) {
+        const int i = idx / nm, ml = idx % nm;
+        dl[ml * N + i] = dS[(size_t)i * nmat + m0 + ml];
+        const double e = (i > 0) ? eS[(size_t)(i - 1) * nmat + m0 + ml] : 0.0;
+        e2l[ml * N + i] = e * e;
+    }
+    __syncthreads();
+    const int ml = threadIdx.x / nb, j = threadIdx.x % nb;
+    if (ml >= nm) return;
+    const int mat = m0 + ml;
+    const double *d = dl + ml * N, *e2 = e2l + ml * N;
     double lo = gb[4 * mat], hi = gb[4 * mat + 1];
     const double pivmin = gb[4 * mat + 2];
     for (int it = 0; it < 80; it++) {
         const double mid = 0.5 * (lo + hi);
         if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin || mid == lo || mid == hi) break;
         int cnt = 0;
-        double q = dS[mat] - mid;
+        double q = d[0] - mid;
         if (fabs(q) < pivmin) q = -pivmin;
         cnt += q < 0.0;
         for (int i = 1; i < N; i++) {
-            const double ei = eS[(size_t)(i - 1) * nmat + mat];
-            q = (dS[(size_t)i * nmat + mat] - mid) - (ei * ei) / q;
+            q = (d[i] - mid) - e2[i] / q;
             if (fabs(q) < pivmin) q = -pivmin;
             cnt += q < 0.0;
         }
@@ -749,12 +762,14 @@ __global__ void invit_kernel(int N, int nmat, int nvec, const double *dS, const 
         double *y = Y + (size_t)jj * N * nmat + mat; /* element i at y[i * nmat] */
         auto Yv = [&](int i) -> double & { return y[(size_t)i * nmat]; };
         const double lm = lam(jj);
-        for (int i = 0; i < N; i++) Yv(i) = hash_unit(jj + 1, i + 1);
-        for (int it = 0; it < 3; it++) {
+        /* two solves from a deterministic pseudo-random start (generated in
+         * the first forward pass); the growth per solve is 1/|lambda error| */
+        for (int it = 0; it < 2; it++) {
             /* forward: factor T - lm I with partial pivoting, transform y */
-            double cd = D(0) - lm, cdu = (N > 1) ? E(0) : 0.0, cb = Yv(0);
+            double cd = D(0) - lm, cdu = (N > 1) ? E(0) : 0.0, cb = it ? Yv(0) : hash_unit(jj + 1, 1);
             for (int i = 0; i + 1 < N; i++) {
-                const double ndl = E(i), nd = D(i + 1) - lm, ndu = (i + 2 < N) ? E(i + 1) : 0.0, nb_ = Yv(i + 1);
+                const double ndl = E(i), nd = D(i + 1) - lm, ndu = (i + 2 < N) ? E(i + 1) : 0.0;
+                const double nb_ = it ? Yv(i + 1) : hash_unit(jj + 1, i + 2);
                 if (fabs(cd) >= fabs(ndl)) {
                     if (cd == 0.0) cd = eps_piv;
                     const double f = ndl / cd;
@@ -773,31 +788,38 @@ __global__ void invit_kernel(int N, int nmat, int nvec, const double *dS, const 
             }
             if (cd == 0.0) cd = eps_piv;
             at(dd, N - 1) = cd;
-            Yv(N - 1) = cb;
-            /* back substitution */
-            double x2 = 0.0, x1 = Yv(N - 1) / at(dd, N - 1);
+            /* back substitution, accumulating |x|^2 */
+            double x2 = 0.0, x1 = cb / cd;
             Yv(N - 1) = x1;
+            double nn = x1 * x1;
             for (int i = N - 2; i >= 0; i--) {
                 const double x = (Yv(i) - at(du, i) * x1 - at(du2, i) * x2) / at(dd, i);
                 Yv(i) = x;
+                nn += x * x;
                 x2 = x1;
                 x1 = x;
             }
-            /* orthogonalise against the cluster's earlier vectors, normalise */
-            for (int pj = j; pj < jj; pj++) {
-                const double *yp = Y + (size_t)pj * N * nmat + mat;
-                double s2 = 0.0;
-                for (int i = 0; i < N; i++) s2 += yp[(size_t)i * nmat] * Yv(i);
-                for (int i = 0; i < N; i++) Yv(i) -= s2 * yp[(size_t)i * nmat];
+            /* orthogonalise against the cluster's earlier vectors */
+            if (jj > j) {
+                for (int pj = j; pj < jj; pj++) {
+                    const double *yp = Y + (size_t)pj * N * nmat + mat;
+                    double s2 = 0.0;
+                    for (int i = 0; i < N; i++) s2 += yp[(size_t)i * nmat] * Yv(i);
+                    for (int i = 0; i < N; i++) Yv(i) -= s2 * yp[(size_t)i * nmat];
+                }
+                nn = 0.0;
+                for (int i = 0; i < N; i++) nn += Yv(i) * Yv(i);
             }
-            double mx = 0.0;
-            for (int i = 0; i < N; i++) mx = fmax(mx, fabs(Yv(i)));
-            const double sc = 1.0 / mx;
-            double nn = 0.0;
-            for (int i = 0; i < N; i++) {
-                const double v = Yv(i) * sc;
-                Yv(i) = v;
-                nn += v * v;
+            if (!(nn < 1e280)) { /* rescale by the largest entry first */
+                double mx = 0.0;
+                for (int i = 0; i < N; i++) mx = fmax(mx, fabs(Yv(i)));
+                const double sc = 1.0 / mx;
+                nn = 0.0;
+                for (int i = 0; i < N; i++) {
+                    const double v = Yv(i) * sc;
+                    Yv(i) = v;
+                    nn += v * v;
+                }
             }
             const double sc2 = 1.0 / sqrt(nn);
             for (int i = 0; i < N; i++) Yv(i) *= sc2;
@@ -1159,9 +1181,15 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
         }
         {
             Prof pf(c, "bisect", 0.0, 0.0);
-            const size_t nt = (size_t)nm * nb;
-            hipLaunchKernelGGL(bisect_kernel, dim3((nt + 255) / 256), dim3(256), 0, c->stream, N, nm, nb, dS, eS, gb,
-                               lamS, dEvals + (size_t)m0 * nb);
+            int mpw = std::max(1, 256 / nb);
+            while (mpw > 1 && sizeof(double) * 2 * (size_t)mpw * N > c->lds_max) mpw--;
+            const size_t lds_b = sizeof(double) * 2 * (size_t)mpw * N;
+            if (nb > 256 || lds_b > c->lds_max) return set_error(PMX_EINVAL, "too many eigenvalues or too large a basis for bisection");
+            if (lds_b > 65536)
+                PMX_CHECK_HIP(hipFuncSetAttribute((const void *)bisect_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  (int)lds_b));
+            hipLaunchKernelGGL(bisect_kernel, dim3((nm + mpw - 1) / mpw), dim3(256), lds_b, c->stream, N, nm, nb, mpw,
+                               dS, eS, gb, lamS, dEvals + (size_t)m0 * nb);
             PMX_CHECK_HIP(hipGetLastError());
         }
         if (nvec > 0) {
