@@ -14,7 +14,7 @@ outputs left in HBM.  `value` is the mixed-precision path's k-points/s; the
 FP64 path is timed in the same run for the ratio and as the reference for the
 eps_M error.
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--precision fp32]
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--precision bf16x3|fp32]
 """
 import argparse
 import ctypes
@@ -134,7 +134,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3"])
+    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x3"])
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--grid", type=int, default=NGRID)
@@ -159,6 +159,10 @@ def main():
     pm = ctx.chi_params(K, KW, [QVEC], NBAND, off, DELTA, EPS, NFREQ, DFREQ, KK, opt.precision)
     t_mix, prof_mix = run_steps(ctx, hip, pm, dre, dim, opt.steps, opt.warmup)
     re_m, im_m = eps_block(hip, dre, dim, ne2 * NFREQ)
+
+    other = "fp32" if opt.precision == "bf16x3" else "bf16x3"
+    po = ctx.chi_params(K, KW, [QVEC], NBAND, off, DELTA, EPS, NFREQ, DFREQ, KK, other)
+    t_other, _ = run_steps(ctx, hip, po, dre, dim, 1, 1)
 
     p64 = ctx.chi_params(K, KW, [QVEC], NBAND, off, DELTA, EPS, NFREQ, DFREQ, KK, "fp64")
     t_64, prof_64 = run_steps(ctx, hip, p64, dre, dim, opt.fp64_steps, 1)
@@ -207,6 +211,7 @@ def main():
         "data": "synthetic (Gamma-centred k-grid, published Si form factors)",
         "config": {"workload": WORKLOAD, "precision": opt.precision, "nk": nk, "nq": 1},
         "fp64_value": round(nk / t_64, 2),
+        other + "_value": round(nk / t_other, 2),
         "fp64_ms_per_step": round(1e3 * t_64, 3),
         "speedup_vs_fp64": round(t_64 / t_mix, 3),
         "epsM_rel_err": eps_err,

@@ -562,87 +562,110 @@ template <class T> __global__ void __launch_bounds__(256) moment_mfma_kernel(Mom
  * ------------------------------------------------------------------ */
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
+/* x = hi + lo with hi = bf16(x), lo = bf16(x - hi), both rounded to nearest
+ * even by the gfx950 conversion (v_cvt_pk_bf16_f32) */
 __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
-    unsigned u = __float_as_uint(x);
-    unsigned r = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u; /* round to nearest even */
-    float h = __uint_as_float(r);
-    float l = x - h;
-    unsigned ul = __float_as_uint(l);
-    unsigned rl = (ul + 0x7FFFu + ((ul >> 16) & 1u)) >> 16;
-    hi = __builtin_bit_cast(__bf16, (unsigned short)(r >> 16));
-    lo = __builtin_bit_cast(__bf16, (unsigned short)rl);
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
 }
 
+/* one workgroup per (tile pair, bin); the 4 waves take interleaved K steps
+ * of the bin and every order n, so each K element is loaded and its B side
+ * split once; the orders scale the A side, A_n = s^n a, re-split per order.
+ * The wave partials are summed in LDS at the end. */
+constexpr int MOM_MAXNT = 12; /* largest order count of the split-BF16 kernel */
+/* NI items per row: 1 (spatial), 2 (SOC: + and - channel); MAXNT >= NT orders */
+template <int NI, int MAXNT>
 __global__ void __launch_bounds__(256) moment_bf16x3_kernel(MomArgs2 a) {
-    constexpr int MAXW = 5;
+    __shared__ fx4 red[2][64][2 * MAXNT / 2];
     const int pairi = blockIdx.x, b = blockIdx.y;
     const int r0 = a.start[b], r1 = a.start[b + 1]; /* an empty bin writes zero moments */
     int ti, tj;
     tile_pair(pairi, ti, tj);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int n0 = wave * a.npw_wave;
-    const int nw = min(a.npw_wave, a.NT - n0);
-    if (nw <= 0) return;
+    const int NT = a.NT;
     const int ia = ti * TILE + (lane & 15), ib = tj * TILE + (lane & 15);
     const int kg = lane >> 4;
     const float2 *X = reinterpret_cast<const float2 *>(a.X);
-    const int nitems = (a.nch == 1) ? 1 : 2;
-    const int K2 = 2 * nitems;
+    constexpr int K2 = 2 * NI, RPL = 8 / K2; /* K entries per row; rows per lane per K step */
     const int kend = (r1 - r0) * K2;
-    fx4 accr[MAXW], acci[MAXW];
+    fx4 accr[MAXNT], acci[MAXNT];
 #pragma unroll
-    for (int j = 0; j < MAXW; j++) {
+    for (int j = 0; j < MAXNT; j++) {
         accr[j] = fx4{0, 0, 0, 0};
         acci[j] = fx4{0, 0, 0, 0};
     }
-    for (int kb = 0; kb < kend; kb += 32) {
-        bf8 ahi, alo;
-        float bre[8], bim[8], sv[8];
+    /* raw operands of one K step: RPL rows per lane; the next step's are
+     * loaded while this one's MFMAs run */
+    struct Raw {
+        float w[RPL], s[RPL];
+        float2 xa[RPL][NI], xb[RPL][NI];
+    };
+    auto fetch = [&](int kb, Raw &r) {
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int kk = kb + 8 * kg + e;
-            float av = 0.f, br = 0.f, bi = 0.f, s = 0.f;
-            if (kk < kend) {
-                const int row = r0 + kk / K2, sub = kk % K2;
-                const int item = sub >> 1, part = sub & 1;
-                const int ch = (a.nch == 1) ? 0 : (2 * a.acc_index + item);
-                const float2 *xr = X + ((size_t)row * a.nch + ch) * a.D;
-                float2 xa = (ia < a.D) ? xr[ia] : make_float2(0.f, 0.f);
-                float2 xb = (ib < a.D) ? xr[ib] : make_float2(0.f, 0.f);
-                float w = (float)a.ws[row];
-                if (item) w = -w;
-                s = (float)a.ss[row];
-                av = part ? xa.y : xa.x;
-                br = w * (part ? xb.y : xb.x);
-                bi = w * (part ? xb.x : -xb.y);
-                float sc = 1.f;
-                for (int j = 0; j < n0; j++) sc *= s;
-                br *= sc;
-                bi *= sc;
+        for (int rr = 0; rr < RPL; rr++) {
+            const int kk0 = kb + 8 * kg + rr * K2; /* a row's K entries are all in or all out */
+            r.w[rr] = 0.f;
+            r.s[rr] = 0.f;
+#pragma unroll
+            for (int it = 0; it < NI; it++) {
+                r.xa[rr][it] = make_float2(0.f, 0.f);
+                r.xb[rr][it] = make_float2(0.f, 0.f);
             }
-            __bf16 h, l;
-            split_bf16(av, h, l);
-            ahi[e] = h;
-            alo[e] = l;
-            bre[e] = br;
-            bim[e] = bi;
-            sv[e] = s;
+            if (kk0 < kend) {
+                const int row = r0 + kk0 / K2;
+                r.w[rr] = (float)a.ws[row];
+                r.s[rr] = (float)a.ss[row];
+#pragma unroll
+                for (int it = 0; it < NI; it++) {
+                    const int ch = (NI == 1) ? 0 : (2 * a.acc_index + it);
+                    const float2 *xr = X + ((size_t)row * a.nch + ch) * a.D;
+                    if (ia < a.D) r.xa[rr][it] = xr[ia];
+                    if (ib < a.D) r.xb[rr][it] = xr[ib];
+                }
+            }
+        }
+    };
+    Raw cur, nxt;
+    fetch(32 * wave, cur);
+    for (int kb = 32 * wave; kb < kend; kb += 128) {
+        fetch(kb + 128, nxt);
+        float av[8], sv[8];
+        bf8 brh, brl, bih, bil;
+#pragma unroll
+        for (int rr = 0; rr < RPL; rr++) {
+            const float w = cur.w[rr], s = cur.s[rr];
+            const float2 *xa = cur.xa[rr], *xb = cur.xb[rr];
+#pragma unroll
+            for (int it = 0; it < NI; it++)
+#pragma unroll
+                for (int part = 0; part < 2; part++) {
+                    const int e = rr * K2 + 2 * it + part;
+                    const float wi = it ? -w : w;
+                    const float br = wi * (part ? xb[it].y : xb[it].x);
+                    const float bi = wi * (part ? xb[it].x : -xb[it].y);
+                    av[e] = part ? xa[it].y : xa[it].x;
+                    sv[e] = s;
+                    __bf16 h, l;
+                    split_bf16(br, h, l);
+                    brh[e] = h;
+                    brl[e] = l;
+                    split_bf16(bi, h, l);
+                    bih[e] = h;
+                    bil[e] = l;
+                }
         }
 #pragma unroll
-        for (int j = 0; j < MAXW; j++) {
-            if (j < nw) {
-                bf8 brh, brl, bih, bil;
+        for (int j = 0; j < MAXNT; j++) {
+            if (j < NT) {
+                bf8 ahi, alo;
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
                     __bf16 h, l;
-                    split_bf16(bre[e], h, l);
-                    brh[e] = h;
-                    brl[e] = l;
-                    split_bf16(bim[e], h, l);
-                    bih[e] = h;
-                    bil[e] = l;
-                    bre[e] *= sv[e];
-                    bim[e] *= sv[e];
+                    split_bf16(av[e], h, l);
+                    ahi[e] = h;
+                    alo[e] = l;
+                    av[e] *= sv[e];
                 }
                 accr[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, brh, accr[j], 0, 0, 0);
                 accr[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, brl, accr[j], 0, 0, 0);
@@ -652,11 +675,38 @@ __global__ void __launch_bounds__(256) moment_bf16x3_kernel(MomArgs2 a) {
                 acci[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bih, acci[j], 0, 0, 0);
             }
         }
+        cur = nxt;
     }
+    /* sum the four waves' partials: waves 2, 3 -> 0, 1, then 1 -> 0, in
+     * halves of the orders to bound the LDS */
+    for (int h0 = 0; h0 < NT; h0 += MAXNT / 2) {
+        const int hn = min(MAXNT / 2, NT - h0);
+        for (int stage = 2; stage >= 1; stage >>= 1) {
+            __syncthreads();
+            if (wave >= stage && wave < 2 * stage) {
 #pragma unroll
-    for (int j = 0; j < MAXW; j++) {
-        if (j < nw) {
-            const size_t o = (((size_t)b * a.NT + n0 + j) * a.npairs + pairi) * 256;
+                for (int j = 0; j < MAXNT / 2; j++)
+                    if (j < hn) {
+                        red[wave - stage][lane][2 * j] = accr[h0 + j];
+                        red[wave - stage][lane][2 * j + 1] = acci[h0 + j];
+                    }
+            }
+            __syncthreads();
+            if (wave < stage) {
+#pragma unroll
+                for (int j = 0; j < MAXNT / 2; j++)
+                    if (j < hn) {
+                        accr[h0 + j] += red[wave][lane][2 * j];
+                        acci[h0 + j] += red[wave][lane][2 * j + 1];
+                    }
+            }
+        }
+    }
+    if (wave != 0) return;
+#pragma unroll
+    for (int j = 0; j < MAXNT; j++) {
+        if (j < NT) {
+            const size_t o = (((size_t)b * NT + j) * a.npairs + pairi) * 256;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int row = 4 * (lane >> 4) + v, col = lane & 15;
@@ -1332,10 +1382,16 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                     const double fl = 8.0 * 9.0 * ne2 * NT * (double)ntr * (spatial ? 1 : 2);
                     const double by = (double)ntr * nch * D * (fp32 ? 8.0 : 16.0) +
                                       16.0 * (double)nbins * NT * npairs * 256 * 2;
-                    const bool bf = p->precision == PMX_PREC_BF16X3;
+                    const bool bf = p->precision == PMX_PREC_BF16X3 && NT <= MOM_MAXNT; /* else the FP32 MFMA kernel */
                     Prof pf(c, bf ? "moment_bf16x3" : (fp32 ? "moment_mfma_f32" : "moment_mfma_f64"), fl, by);
-                    if (bf)
-                        hipLaunchKernelGGL(moment_bf16x3_kernel, dim3(npairs, nbins), dim3(256), 0, st, ma);
+                    if (bf && nch == 1 && NT <= 9)
+                        hipLaunchKernelGGL((moment_bf16x3_kernel<1, 9>), dim3(npairs, nbins), dim3(256), 0, st, ma);
+                    else if (bf && nch == 1)
+                        hipLaunchKernelGGL((moment_bf16x3_kernel<1, MOM_MAXNT>), dim3(npairs, nbins), dim3(256), 0, st, ma);
+                    else if (bf && NT <= 9)
+                        hipLaunchKernelGGL((moment_bf16x3_kernel<2, 9>), dim3(npairs, nbins), dim3(256), 0, st, ma);
+                    else if (bf)
+                        hipLaunchKernelGGL((moment_bf16x3_kernel<2, MOM_MAXNT>), dim3(npairs, nbins), dim3(256), 0, st, ma);
                     else if (fp32)
                         hipLaunchKernelGGL(moment_mfma_kernel<float>, dim3(npairs, nbins), dim3(256), 0, st, ma);
                     else
