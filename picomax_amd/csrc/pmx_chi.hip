@@ -240,17 +240,37 @@ template <> struct RhoT<float> {
     static __device__ __forceinline__ int drow(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
+template <class T> struct Cplx;
+template <> struct Cplx<double> { typedef zd type; };
+template <> struct Cplx<float> { typedef float2 type; };
+
+/* The eigenvector rows of this k (conduction at k, valence at k+q) are
+ * staged in LDS once; the loci and weight loads of the next K step are
+ * issued before this step's MFMAs. */
 template <class T> __global__ void __launch_bounds__(256) rho_mfma_kernel(RhoArgs a) {
     typedef typename RhoT<T>::V V;
+    typedef typename Cplx<T>::type CT;
     const int k = blockIdx.x;
     const int ncs = a.nc[k] / 2, nvs = a.nv[k] / 2;
     if (ncs * nvs == 0) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     __shared__ double shd[4][2][16][17]; /* per wave: [re/im][c][column] staging */
+    extern __shared__ double rho_dyn[];
+    CT *sck = reinterpret_cast<CT *>(rho_dyn); /* [ncs][N]: conduction rows at k, c = 0 is the highest */
+    CT *scq = sck + (size_t)ncs * a.N;          /* [nvs][N]: valence rows at k+q */
     const size_t tb = a.tbase[k];
     const double Kx = a.K[3 * k], Ky = a.K[3 * k + 1], Kz = a.K[3 * k + 2];
     const zd *vk = a.vk + (size_t)k * a.nb_s * a.N;
     const zd *vkq = a.vkq + (size_t)k * a.nb_s * a.N;
+    for (int idx = threadIdx.x; idx < ncs * a.N; idx += 256) {
+        const zd v = vk[(size_t)(a.nband / 2 - 1 - idx / a.N) * a.N + idx % a.N];
+        sck[idx] = CT{(T)v.x, (T)v.y};
+    }
+    for (int idx = threadIdx.x; idx < nvs * a.N; idx += 256) {
+        const zd v = vkq[idx];
+        scq[idx] = CT{(T)v.x, (T)v.y};
+    }
+    __syncthreads();
     const int nvg = (nvs + 3) / 4;            /* valence column groups of 4 */
     const int nct = (ncs + 15) / 16;          /* conduction row tiles */
     const int jobs = a.neps * nct * nvg;
@@ -261,25 +281,41 @@ template <class T> __global__ void __launch_bounds__(256) rho_mfma_kernel(RhoArg
         const int jcol = lane & 15, ch = jcol >> 2, vv = vg * 4 + (jcol & 3); /* B column */
         const int kq = lane >> 4;
         const bool crow_ok = crow < ncs, col_ok = vv < nvs;
-        const zd *ck = vk + (size_t)(a.nband / 2 - 1 - (crow_ok ? crow : 0)) * a.N;
-        const zd *cq = vkq + (size_t)(col_ok ? vv : 0) * a.N;
+        const CT *ck = sck + (size_t)(crow_ok ? crow : 0) * a.N;
+        const CT *cq = scq + (size_t)(col_ok ? vv : 0) * a.N;
         const int *lm = a.loci + (size_t)m * a.npw;
-        const zd *wA = a.wA + (size_t)m * 3 * a.npw;
+        const zd *wA = a.wA + (size_t)m * 3 * a.npw + (size_t)(ch > 0 ? ch - 1 : 0) * a.npw;
         V accR = {0, 0, 0, 0}, accI = {0, 0, 0, 0};
         const int K2 = 2 * a.npw;
+        /* operands of K step kk = k0 + kq: p = kk / 2 */
+        int l_n = -1;
+        zd w_n{1.0, 0.0};
+        {
+            const int p = kq >> 1;
+            if (kq < K2) {
+                l_n = lm[p];
+                if (ch > 0) w_n = wA[p];
+            }
+        }
         for (int k0 = 0; k0 < K2; k0 += 4) {
             const int kk = k0 + kq;
+            const int l = l_n;
+            const zd w = w_n;
+            if (kk + 4 < K2) {
+                const int pn = (kk + 4) >> 1;
+                l_n = lm[pn];
+                if (ch > 0) w_n = wA[pn];
+            }
             T av = 0, bre = 0, bim = 0;
             if (kk < K2) {
                 const int p = kk >> 1, part = kk & 1;
-                if (crow_ok) av = (T)(part ? ck[p].y : ck[p].x);
-                const int l = lm[p];
+                if (crow_ok) av = part ? ck[p].y : ck[p].x;
                 if (col_ok && l >= 0) {
-                    zd q = cq[l];
-                    zd w = (ch == 0) ? zd{1.0, 0.0} : wA[(ch - 1) * a.npw + p];
-                    zd bb = cmul(q, w);
-                    bre = (T)(part ? bb.y : bb.x);
-                    bim = (T)(part ? -bb.x : bb.y);
+                    const CT q = cq[l];
+                    const T wx = (T)w.x, wy = (T)w.y;
+                    const T bx = q.x * wx - q.y * wy, by = q.x * wy + q.y * wx;
+                    bre = part ? by : bx;
+                    bim = part ? -bx : by;
                 }
             }
             accR = RhoT<T>::mma(av, bre, accR);
@@ -1351,10 +1387,19 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
              * 3 weighted (+1 plain) sums per (m, p) and transition; bytes = the eigenvector rows read */
             const double macs = (double)ntr * neps * npw * (spatial ? 4.0 : 16.0);
             Prof pf(c, "rho", 8.0 * macs, 16.0 * (double)N * (2 * nb_s) * p->nk);
-            if (spatial && fp32)
-                hipLaunchKernelGGL(rho_mfma_kernel<float>, dim3(p->nk), dim3(256), 0, st, ra);
-            else if (spatial)
-                hipLaunchKernelGGL(rho_mfma_kernel<double>, dim3(p->nk), dim3(256), 0, st, ra);
+            /* LDS for one k's conduction and valence rows; the largest k sets the size */
+            int maxrows = 0;
+            for (int kk = 0; kk < p->nk; kk++) maxrows = std::max(maxrows, nch_h[kk] / 2 + nvh[kk] / 2);
+            const size_t lds_r = (fp32 ? sizeof(float2) : sizeof(zd)) * (size_t)maxrows * N;
+            const bool rho_lds = spatial && lds_r + sizeof(double) * 4 * 2 * 16 * 17 <= c->lds_max;
+            if (rho_lds && lds_r > 65536 - sizeof(double) * 4 * 2 * 16 * 17) {
+                const void *kf = fp32 ? (const void *)rho_mfma_kernel<float> : (const void *)rho_mfma_kernel<double>;
+                PMX_CHECK_HIP(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_r));
+            }
+            if (rho_lds && fp32)
+                hipLaunchKernelGGL(rho_mfma_kernel<float>, dim3(p->nk), dim3(256), lds_r, st, ra);
+            else if (rho_lds)
+                hipLaunchKernelGGL(rho_mfma_kernel<double>, dim3(p->nk), dim3(256), lds_r, st, ra);
             else
                 hipLaunchKernelGGL(rho_kernel, dim3(p->nk), dim3(256), 0, st, ra);
         }
