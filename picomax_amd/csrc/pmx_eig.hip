@@ -550,8 +550,14 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, 2) tridiag_tile_real_kernel(
 #pragma unroll
         for (int i = 0; i < TT; i++) xk0[tvi(r0 + i)] = A[i][0];
     __syncthreads();
-    int cur = 0;
-    for (int k = 0; k + 1 < N; k++) {
+    /* k + 1 = kb1 + s, unrolled over s: the tile column s and the buffer
+     * parity k & 1 are constants in each copy of the body */
+    for (int kb1 = 0; kb1 < N; kb1 += TT) {
+#pragma unroll
+    for (int s = 0; s < TT; s++) {
+        const int k = kb1 + s - 1;
+        if (k < 0 || k + 1 >= N) continue;
+        const int cur = (s + 1) & 1; /* = k & 1, as kb1 is even */
         const double *x = cur ? xk1 : xk0;
         double *xn = cur ? xk0 : xk1;
         /* Householder vector of column k; every wave forms it in its own copy */
@@ -638,35 +644,34 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, 2) tridiag_tile_real_kernel(
             pv = wave_sum_full(pv);
             const double al = -0.5 * tau * pv;
             if (active) {
-                double vI[TT], vJ[TT], wI[TT], wJ[TT];
+                /* two rank-1 halves: A -= v_I w_J^T, then A -= w_I v_J^T */
+                double u[TT], z[TT];
 #pragma unroll
-                for (int i = 0; i < TT; i++) {
-                    vI[i] = vw[tvi(r0 + i)];
-                    vJ[i] = vw[tvi(c0 + i)];
-                    wI[i] = fma(al, vI[i], pp[tvi(r0 + i)]);
-                    wJ[i] = fma(al, vJ[i], pp[tvi(c0 + i)]);
-                }
+                for (int j = 0; j < TT; j++) z[j] = fma(al, vw[tvi(c0 + j)], pp[tvi(c0 + j)]);
+#pragma unroll
+                for (int i = 0; i < TT; i++) u[i] = vw[tvi(r0 + i)];
 #pragma unroll
                 for (int i = 0; i < TT; i++)
 #pragma unroll
-                    for (int j = 0; j < TT; j++) A[i][j] = fma(-wI[i], vJ[j], fma(-vI[i], wJ[j], A[i][j]));
+                    for (int j = 0; j < TT; j++) A[i][j] = fma(-u[i], z[j], A[i][j]);
+#pragma unroll
+                for (int j = 0; j < TT; j++) z[j] = vw[tvi(c0 + j)];
+#pragma unroll
+                for (int i = 0; i < TT; i++) u[i] = fma(al, vw[tvi(r0 + i)], pp[tvi(r0 + i)]);
+#pragma unroll
+                for (int i = 0; i < TT; i++)
+#pragma unroll
+                    for (int j = 0; j < TT; j++) A[i][j] = fma(-u[i], z[j], A[i][j]);
             }
         }
-        /* column k+1 for the next step */
-        if (has_tile && J == (k + 1) / TT) {
-            const int jc = (k + 1) % TT;
+        /* column k+1 = kb1 + s for the next step */
+        if (has_tile && c0 == kb1)
 #pragma unroll
-            for (int i = 0; i < TT; i++) {
-                double v = A[i][0];
-#pragma unroll
-                for (int j = 1; j < TT; j++) v = (j == jc) ? A[i][j] : v;
-                xn[tvi(r0 + i)] = v;
-            }
-        }
+            for (int i = 0; i < TT; i++) xn[tvi(r0 + i)] = A[i][s];
         __syncthreads(); /* (3) next column */
-        cur ^= 1;
     }
-    if (tid == 0) dd[N - 1] = (cur ? xk1 : xk0)[tvi(N - 1)];
+    }
+    if (tid == 0) dd[N - 1] = ((N & 1) ? xk0 : xk1)[tvi(N - 1)]; /* column N-1 was written at step N-2 */
     __syncthreads();
     for (int i = tid; i < N; i += blockDim.x) {
         a.dS[(size_t)i * a.nmat + mat] = dd[i];
