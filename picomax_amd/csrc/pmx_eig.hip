@@ -499,13 +499,15 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_full_real_kernel(EigArgs 
  *   tile (I, J) writes A_IJ v_J into slot (I, J) and A_IJ^T v_I into (J, I);
  *   row r sums its slots: p = tau A v;  w = p - (tau/2)(p.v) v;
  *   A_IJ -= v_I w_J^T + w_I v_J^T; the tiles of column k+1 write it out. */
-constexpr int TT = 8;       /* tile edge */
-constexpr int TS = TT + 2;  /* LDS stride of one block of a vector (bank spread) */
 constexpr int TILE_MAX_THREADS = 256;
 
-__device__ __forceinline__ int tvi(int r) { return (r / TT) * TS + (r % TT); }
-
-__global__ void __launch_bounds__(TILE_MAX_THREADS, 2) tridiag_tile_real_kernel(EigArgs a, int nblk) {
+/* TT: tile edge; TS = TT + 2: LDS stride of one block of a vector (bank
+ * spread); WPE: waves per SIMD the register budget is cut for (8x8 tiles
+ * leave room for 2, 7x7 tiles for 3) */
+template <int TT, int WPE>
+__global__ void __launch_bounds__(TILE_MAX_THREADS, WPE) tridiag_tile_real_kernel(EigArgs a, int nblk) {
+    constexpr int TS = TT + 2;
+    auto tvi = [](int r) { return (r / TT) * TS + (r % TT); };
     extern __shared__ double smem_d[];
     const int N = a.N, NR = nblk * TT;
     const int nvp = nblk * TS, nblkp = nblk | 1;
@@ -515,6 +517,7 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, 2) tridiag_tile_real_kernel(
     double *part = pp + nvp + (size_t)(blockDim.x >> 6) * nvp;        /* [contributing block C][row block R][TS] */
     double *dd = part + (size_t)nblk * nblkp * TS;                    /* [N] */
     double *ee = dd + N, *tt = ee + N;
+    double *ring = tt + N; /* [8][NR]: reflectors of the last 8 steps, stored to global together */
     const int ntile = nblk * (nblk + 1) / 2;
     const int mat = blockIdx.x;
     const size_t P = (size_t)N * (N + 1) / 2;
@@ -531,6 +534,7 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, 2) tridiag_tile_real_kernel(
     }
     const bool diag = I == J;
     const int r0 = I * TT, c0 = J * TT;
+    const int ib = I * TS, jb = J * TS; /* LDS offsets of blocks I and J of a vector */
     const double Kv[3] = {a.K[3 * mat], a.K[3 * mat + 1], a.K[3 * mat + 2]};
     double *vref = a.vref ? reinterpret_cast<double *>(a.vref) + (size_t)mat * P : nullptr;
 
@@ -548,16 +552,17 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, 2) tridiag_tile_real_kernel(
             if (r0 + i < N) A[i][i] = h_elem<double>(a, Kv, r0 + i, r0 + i);
     if (has_tile && J == 0)
 #pragma unroll
-        for (int i = 0; i < TT; i++) xk0[tvi(r0 + i)] = A[i][0];
+        for (int i = 0; i < TT; i++) xk0[(ib + i)] = A[i][0];
     __syncthreads();
-    /* k + 1 = kb1 + s, unrolled over s: the tile column s and the buffer
-     * parity k & 1 are constants in each copy of the body */
-    for (int kb1 = 0; kb1 < N; kb1 += TT) {
+    /* k + 1 = kb1 + s, unrolled over s: the tile column s is a constant in
+     * each copy of the body */
+    constexpr int UNR = (TT % 2 == 0) ? TT : 1; /* odd tiles: no unrolling (register budget) */
+    for (int kb1 = 0; kb1 < N; kb1 += UNR) {
 #pragma unroll
-    for (int s = 0; s < TT; s++) {
+    for (int s = 0; s < UNR; s++) {
         const int k = kb1 + s - 1;
         if (k < 0 || k + 1 >= N) continue;
-        const int cur = (s + 1) & 1; /* = k & 1, as kb1 is even */
+        const int cur = (TT % 2 == 0) ? ((s + 1) & 1) : (k & 1); /* k & 1, a constant when TT is even */
         const double *x = cur ? xk1 : xk0;
         double *xn = cur ? xk0 : xk1;
         /* Householder vector of column k; every wave forms it in its own copy */
@@ -594,8 +599,8 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, 2) tridiag_tile_real_kernel(
             double vI[TT], vJ[TT], pJ[TT];
 #pragma unroll
             for (int i = 0; i < TT; i++) {
-                vI[i] = vw[tvi(r0 + i)];
-                vJ[i] = vw[tvi(c0 + i)];
+                vI[i] = vw[(ib + i)];
+                vJ[i] = vw[(jb + i)];
                 pJ[i] = 0.0;
             }
             double *sI = part + ((size_t)J * nblkp + I) * TS;
@@ -628,10 +633,11 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, 2) tridiag_tile_real_kernel(
                     p += (C >= cmin) ? t : 0.0;
                 }
                 p *= tau;
-                if (vref && r >= k + 2) vref[pk(r, k, N)] = vw[tvi(r)];
             }
             pp[tvi(r)] = p;
         }
+        /* reflector of this step (zero when tau = 0: the step is the identity) */
+        if (vref && tid >= k + 2 && tid < N) ring[(k & 7) * NR + tid] = (tau != 0.0) ? vw[tvi(tid)] : 0.0;
         if (tid == 0) {
             dd[k] = x[tvi(k)];
             ee[k] = beta;
@@ -647,17 +653,17 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, 2) tridiag_tile_real_kernel(
                 /* two rank-1 halves: A -= v_I w_J^T, then A -= w_I v_J^T */
                 double u[TT], z[TT];
 #pragma unroll
-                for (int j = 0; j < TT; j++) z[j] = fma(al, vw[tvi(c0 + j)], pp[tvi(c0 + j)]);
+                for (int j = 0; j < TT; j++) z[j] = fma(al, vw[(jb + j)], pp[(jb + j)]);
 #pragma unroll
-                for (int i = 0; i < TT; i++) u[i] = vw[tvi(r0 + i)];
+                for (int i = 0; i < TT; i++) u[i] = vw[(ib + i)];
 #pragma unroll
                 for (int i = 0; i < TT; i++)
 #pragma unroll
                     for (int j = 0; j < TT; j++) A[i][j] = fma(-u[i], z[j], A[i][j]);
 #pragma unroll
-                for (int j = 0; j < TT; j++) z[j] = vw[tvi(c0 + j)];
+                for (int j = 0; j < TT; j++) z[j] = vw[(jb + j)];
 #pragma unroll
-                for (int i = 0; i < TT; i++) u[i] = fma(al, vw[tvi(r0 + i)], pp[tvi(r0 + i)]);
+                for (int i = 0; i < TT; i++) u[i] = fma(al, vw[(ib + i)], pp[(ib + i)]);
 #pragma unroll
                 for (int i = 0; i < TT; i++)
 #pragma unroll
@@ -665,10 +671,30 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, 2) tridiag_tile_real_kernel(
             }
         }
         /* column k+1 = kb1 + s for the next step */
-        if (has_tile && c0 == kb1)
+        if (UNR == TT) {
+            if (has_tile && c0 == kb1)
 #pragma unroll
-            for (int i = 0; i < TT; i++) xn[tvi(r0 + i)] = A[i][s];
+                for (int i = 0; i < TT; i++) xn[(ib + i)] = A[i][s % TT];
+        } else if (has_tile && J == (k + 1) / TT) {
+            const int jc = (k + 1) % TT;
+#pragma unroll
+            for (int i = 0; i < TT; i++) {
+                double v = A[i][0];
+#pragma unroll
+                for (int j = 1; j < TT; j++) v = (j == jc) ? A[i][j] : v;
+                xn[(ib + i)] = v;
+            }
+        }
         __syncthreads(); /* (3) next column */
+        if (vref && ((k & 7) == 7 || k + 2 == N)) {
+            /* reflector columns k0..k (rows kk+2..N-1 of column kk are contiguous) */
+            const int k0 = k & ~7;
+            for (int kk = k0; kk <= k; kk++) {
+                const int len = N - kk - 2;
+                double *dst = vref + pk(kk + 2, kk, N);
+                for (int i = tid; i < len; i += blockDim.x) dst[i] = ring[(kk & 7) * NR + kk + 2 + i];
+            }
+        }
     }
     }
     if (tid == 0) dd[N - 1] = ((N & 1) ? xk0 : xk1)[tvi(N - 1)]; /* column N-1 was written at step N-2 */
@@ -1164,13 +1190,23 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
         {
             Prof pf(c, real ? "tridiag_real" : "tridiag_complex", nm * (real ? (4.0 / 3.0) * n3 : (16.0 / 3.0) * n3), 0.0);
             const size_t lds_full = sizeof(double) * ((size_t)N * N + 4 * (size_t)N + 2 * EIG_WAVES);
-            const int nblk = (N + TT - 1) / TT;
-            const int tile_threads = (std::max(nblk * (nblk + 1) / 2, nblk * TT) + 63) / 64 * 64;
-            if (real && tile_threads <= TILE_MAX_THREADS) {
-                const size_t lds_tile = sizeof(double) * ((2 + 1 + tile_threads / 64) * (size_t)nblk * TS +
-                                                          (size_t)nblk * (nblk | 1) * TS + 3 * (size_t)N);
-                hipLaunchKernelGGL(tridiag_tile_real_kernel, dim3(nm), dim3(tile_threads), lds_tile, c->stream, a,
-                                   nblk);
+            auto tile_threads = [&](int tt) {
+                const int nb_ = (N + tt - 1) / tt;
+                return (std::max(nb_ * (nb_ + 1) / 2, nb_ * tt) + 63) / 64 * 64;
+            };
+            auto tile_lds = [&](int tt) {
+                const int nb_ = (N + tt - 1) / tt, ts = tt + 2;
+                return sizeof(double) * ((3 + tile_threads(tt) / 64) * (size_t)nb_ * ts +
+                                         (size_t)nb_ * (nb_ | 1) * ts + 3 * (size_t)N + 8 * (size_t)nb_ * tt);
+            };
+            if (real && tile_threads(7) <= TILE_MAX_THREADS) {
+                hipLaunchKernelGGL((tridiag_tile_real_kernel<7, 3>), dim3(nm), dim3(tile_threads(7)), tile_lds(7),
+                                   c->stream, a, (N + 6) / 7);
+                PMX_CHECK_HIP(hipGetLastError());
+                rc = 0;
+            } else if (real && tile_threads(8) <= TILE_MAX_THREADS) {
+                hipLaunchKernelGGL((tridiag_tile_real_kernel<8, 2>), dim3(nm), dim3(tile_threads(8)), tile_lds(8),
+                                   c->stream, a, (N + 7) / 8);
                 PMX_CHECK_HIP(hipGetLastError());
                 rc = 0;
             } else if (real && N <= EIG_THREADS && lds_full <= c->lds_max) {
