@@ -959,39 +959,53 @@ __global__ void pack_kernel(PackArgs a) {
 
 /* kramerskronigtransform (src/pmx_chi.cpp:272-309), one thread per (element, f):
  * ReX += ds sum_{s!=f} w_s ImX_s/(w_s^2 - w_f^2), ImX += ds sum_{s!=f} w_s ReX0_s/(..) */
-/* KK as an FP64 MFMA GEMM over the frequency axis: with the fixed matrix
+/* Kramers-Kronig as a GEMM over the frequency axis: with the fixed matrix
  * Kk[s][f] = ds w_s / (w_s^2 - w_f^2) (s != f), per element row
- *   Re'[f] = Re0[f] + sum_s Im0[s] Kk[s][f],  Im'[f] = Im0[f] + sum_s Re0[s] Kk[s][f].
- * Block = 4 waves, each a 16-row x 16-frequency tile of both outputs. */
+ *   Re'[f] = Re0[f] + sum_s Im0[s] Kk[s][f],  Im'[f] = Im0[f] + sum_s Re0[s] Kk[s][f],
+ * placed in file order.  A wave computes 16 rows x KK_CT
+ * column tiles of 16, so each A value it loads feeds KK_CT MFMA pairs.  T =
+ * double (FP64 MFMA) or float (FP32 MFMA, mixed paths; Kk then in FP32). */
+constexpr int KK_CT = 4;
+template <class T>
 __global__ void __launch_bounds__(256) kk_gemm_kernel(int M, int NF, const double *Re0, const double *Im0,
-                                                      const double *Kk, double *Re, double *Im, int ne2, int nq,
-                                                      int q) {
+                                                      const T *Kk, double *Re, double *Im, int ne2, int nq, int q) {
+    typedef typename MfmaT<T>::V V;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int row0 = (blockIdx.y * 4 + wave) * 16, col0 = blockIdx.x * 16;
+    const int row0 = (blockIdx.y * 4 + wave) * 16, col0 = blockIdx.x * 16 * KK_CT;
     if (row0 >= M) return;
-    const int ar = row0 + (lane & 15), kq = lane >> 4, bc = col0 + (lane & 15);
-    dx4 accR = {0, 0, 0, 0}, accI = {0, 0, 0, 0};
+    const int ar = row0 + (lane & 15), kq = lane >> 4;
+    V accR[KK_CT], accI[KK_CT];
+#pragma unroll
+    for (int c = 0; c < KK_CT; c++) {
+        accR[c] = V{0, 0, 0, 0};
+        accI[c] = V{0, 0, 0, 0};
+    }
     for (int k0 = 0; k0 < NF; k0 += 4) {
         const int k = k0 + kq;
-        double a1 = 0.0, a2 = 0.0, b = 0.0;
-        if (k < NF) {
-            if (ar < M) {
-                a1 = Im0[(size_t)ar * NF + k];
-                a2 = Re0[(size_t)ar * NF + k];
-            }
-            if (bc < NF) b = Kk[(size_t)k * NF + bc];
+        T a1 = 0, a2 = 0;
+        if (k < NF && ar < M) {
+            a1 = (T)Im0[(size_t)ar * NF + k];
+            a2 = (T)Re0[(size_t)ar * NF + k];
         }
-        accR = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b, accR, 0, 0, 0);
-        accI = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b, accI, 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < KK_CT; c++) {
+            const int bc = col0 + 16 * c + (lane & 15);
+            const T b = (k < NF && bc < NF) ? Kk[(size_t)k * NF + bc] : T(0);
+            accR[c] = MfmaT<T>::mma(a1, b, accR[c]);
+            accI[c] = MfmaT<T>::mma(a2, b, accI[c]);
+        }
     }
-    for (int v = 0; v < 4; v++) {
-        const int r = row0 + (lane >> 4) + 4 * v, cidx = col0 + (lane & 15); /* f64 D layout */
-        if (r >= M || cidx >= NF) continue;
-        const size_t ij = r / ne2, e2 = r % ne2;
-        const size_t o = ((ij * nq + q) * ne2 + e2) * NF + cidx;
-        Re[o] = Re0[(size_t)r * NF + cidx] + accR[v];
-        Im[o] = Im0[(size_t)r * NF + cidx] + accI[v];
-    }
+#pragma unroll
+    for (int c = 0; c < KK_CT; c++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int r = row0 + MfmaT<T>::drow(lane, v), cidx = col0 + 16 * c + (lane & 15);
+            if (r >= M || cidx >= NF) continue;
+            const size_t ij = r / ne2, e2 = r % ne2;
+            const size_t o = ((ij * nq + q) * ne2 + e2) * NF + cidx;
+            Re[o] = Re0[(size_t)r * NF + cidx] + (double)accR[c][v];
+            Im[o] = Im0[(size_t)r * NF + cidx] + (double)accI[c][v];
+        }
 }
 
 __global__ void place_kernel(int nel, int nfreq, int ne2, int nq, int q, const double *Re0, const double *Im0,
@@ -1097,13 +1111,13 @@ __global__ void __launch_bounds__(256) epsinv_kernel(int N, int nq, int nfreq, c
         }
 }
 
-__global__ void kkmat_kernel(int nfreq, double dfreq, double *Kk) {
+template <class T> __global__ void kkmat_kernel(int nfreq, double dfreq, T *Kk) {
     const size_t total = (size_t)nfreq * nfreq;
     const double ds = dfreq * 2.0 / PMX_PI;
     for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
         const int s = idx / nfreq, f = idx % nfreq;
         const double ws = dfreq * s, wf = dfreq * f;
-        Kk[idx] = (s == f) ? 0.0 : ds * (ws / (ws * ws - wf * wf));
+        Kk[idx] = (T)((s == f) ? 0.0 : ds * (ws / (ws * ws - wf * wf)));
     }
 }
 
@@ -1175,7 +1189,7 @@ struct ChiWork {
     DBuf<double> ws, ss, es;
     DBuf<zd> vk, vkq, Xd, M1, M2, Z1, Z2, wAd, ucd, wSd;
     DBuf<float2> Xf;
-    DBuf<float> Ktf;
+    DBuf<float> Ktf, Kkf;
     DBuf<int> nc, nv, tbase, bind, hist, start, cnt, pos, bad;
 };
 
@@ -1248,7 +1262,9 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
     }
     if (p->kk) {
         if ((rc = w.Kkd.ensure((size_t)p->nfreq * p->nfreq))) return rc;
-        hipLaunchKernelGGL(kkmat_kernel, dim3(256), dim3(256), 0, st, p->nfreq, df, w.Kkd.p);
+        hipLaunchKernelGGL(kkmat_kernel<double>, dim3(256), dim3(256), 0, st, p->nfreq, df, w.Kkd.p);
+        if ((rc = w.Kkf.ensure((size_t)p->nfreq * p->nfreq))) return rc;
+        hipLaunchKernelGGL(kkmat_kernel<float>, dim3(256), dim3(256), 0, st, p->nfreq, df, w.Kkf.p);
     }
     if ((rc = w.Re0.ensure(per_q)) || (rc = w.Im0.ensure(per_q))) return rc;
 
@@ -1474,8 +1490,13 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         if (p->kk) {
             Prof pf(c, "kk", 4.0 * per_q * p->nfreq, 8.0 * 4 * per_q);
             const int M = 9 * ne2;
-            hipLaunchKernelGGL(kk_gemm_kernel, dim3((p->nfreq + 15) / 16, (M + 63) / 64), dim3(256), 0, st, M,
-                               p->nfreq, w.Re0.p, w.Im0.p, w.Kkd.p, dRe, dIm, ne2, p->nq, q);
+            const dim3 grid((p->nfreq + 16 * KK_CT - 1) / (16 * KK_CT), (M + 63) / 64);
+            if (fp32)
+                hipLaunchKernelGGL(kk_gemm_kernel<float>, grid, dim3(256), 0, st, M, p->nfreq, w.Re0.p, w.Im0.p,
+                                   w.Kkf.p, dRe, dIm, ne2, p->nq, q);
+            else
+                hipLaunchKernelGGL(kk_gemm_kernel<double>, grid, dim3(256), 0, st, M, p->nfreq, w.Re0.p, w.Im0.p,
+                                   w.Kkd.p, dRe, dIm, ne2, p->nq, q);
         } else {
             hipLaunchKernelGGL(place_kernel, dim3(1024), dim3(256), 0, st, 9 * ne2, p->nfreq, ne2, p->nq, q, w.Re0.p,
                                w.Im0.p, dRe, dIm);
@@ -1518,9 +1539,9 @@ extern "C" int pmx_kk(pmx_ctx *c, int nel, int nfreq, double dfreq, double *ReX,
         return rc;
     PMX_CHECK_HIP(hipMemcpyAsync(r0.p, ReX, sizeof(double) * n, hipMemcpyHostToDevice, st));
     PMX_CHECK_HIP(hipMemcpyAsync(i0.p, ImX, sizeof(double) * n, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(kkmat_kernel, dim3(256), dim3(256), 0, st, nfreq, dfreq, Kk.p);
-    hipLaunchKernelGGL(kk_gemm_kernel, dim3((nfreq + 15) / 16, (nel + 63) / 64), dim3(256), 0, st, nel, nfreq, r0.p,
-                       i0.p, Kk.p, r.p, i.p, nel, 1, 0);
+    hipLaunchKernelGGL(kkmat_kernel<double>, dim3(256), dim3(256), 0, st, nfreq, dfreq, Kk.p);
+    hipLaunchKernelGGL(kk_gemm_kernel<double>, dim3((nfreq + 16 * KK_CT - 1) / (16 * KK_CT), (nel + 63) / 64),
+                       dim3(256), 0, st, nel, nfreq, r0.p, i0.p, Kk.p, r.p, i.p, nel, 1, 0);
     PMX_CHECK_HIP(hipGetLastError());
     PMX_CHECK_HIP(hipMemcpyAsync(ReX, r.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     PMX_CHECK_HIP(hipMemcpyAsync(ImX, i.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));

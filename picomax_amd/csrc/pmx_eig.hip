@@ -971,6 +971,7 @@ __global__ void __launch_bounds__(EIG_THREADS) backtrans_kernel(int N, int nmat,
 typedef double dx4 __attribute__((ext_vector_type(4)));
 constexpr int BW = 16;
 constexpr int VS = BW + 1;
+constexpr int BT_VL = 16; /* reflector values per thread: Np <= 256 */
 
 __device__ __forceinline__ int xcd_remap(int b, int G) {
     const int x = b & 7, s = b >> 3;
@@ -999,25 +1000,54 @@ __global__ void __launch_bounds__(256) backtrans_wy_kernel(int N, int nmat, int 
     const size_t P = (size_t)N * (N + 1) / 2;
     const double *vref = vref_all + (size_t)mat * P;
     const double *tau = tau_all + (size_t)mat * N;
-    for (int idx = tid; idx < Np * NV; idx += 256) {
-        const int i = idx % Np, j = idx / Np;
-        Zs[i * ZS + j] = (i < N && j < nvec) ? Y[((size_t)j * N + i) * nmat + mat] : 0.0;
+    /* Y -> Z in chunks of 16 loads per thread, all in flight before the stores */
+    for (int base = 0; base < Np * NV; base += 16 * 256) {
+        double y[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int idx = base + u * 256 + tid;
+            const int i = idx % Np, j = idx / Np;
+            y[u] = (idx < Np * NV && i < N && j < nvec) ? Y[((size_t)j * N + i) * nmat + mat] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int idx = base + u * 256 + tid;
+            if (idx < Np * NV) Zs[(idx % Np) * ZS + idx / Np] = y[u];
+        }
     }
     const int nref = N - 1, nct = NV / 16, nrt = Np / 16;
-    for (int bi = (nref + BW - 1) / BW - 1; bi >= 0; bi--) {
-        const int kb = bi * BW;
-        __syncthreads(); /* Z loaded or updated; V free */
-        for (int idx = tid; idx < Np * BW; idx += 256) {
+    /* the reflector block bi is held in registers (BT_VL values per thread,
+     * Np * BW <= 256 * BT_VL) while block bi + 1 is applied */
+    double vreg[BT_VL];
+    double treg = 0.0;
+    auto load_v = [&](int kb) {
+#pragma unroll
+        for (int u = 0; u < BT_VL; u++) {
+            const int idx = u * 256 + tid;
             const int r = idx % Np, kk = idx / Np, k = kb + kk;
             double v = 0.0;
-            if (k < nref) {
+            if (idx < Np * BW && k < nref) {
                 if (r == k + 1) v = 1.0;
                 else if (r > k + 1 && r < N) v = vref[pk(r, k, N)];
             }
-            Vs[r * VS + kk] = v;
+            vreg[u] = v;
         }
-        if (tid < BW) taus[tid] = (kb + tid < nref) ? tau[kb + tid] : 0.0;
-        __syncthreads();
+        if (tid < BW) treg = (kb + tid < nref) ? tau[kb + tid] : 0.0;
+    };
+    auto store_v = [&]() {
+#pragma unroll
+        for (int u = 0; u < BT_VL; u++) {
+            const int idx = u * 256 + tid;
+            if (idx < Np * BW) Vs[(idx % Np) * VS + idx / Np] = vreg[u];
+        }
+        if (tid < BW) taus[tid] = treg;
+    };
+    const int nbw = (nref + BW - 1) / BW;
+    load_v((nbw - 1) * BW);
+    __syncthreads(); /* Z stored */
+    store_v();
+    for (int bi = nbw - 1; bi >= 0; bi--) {
+        __syncthreads(); /* V and taus of block bi stored */
         /* tile 0: G = V^T V; tile 1+c: W_c = V^T Z_c (sums over the rows) */
         for (int t = wave; t <= nct; t += 4) {
             const int cb = (t - 1) * 16;
@@ -1062,6 +1092,7 @@ __global__ void __launch_bounds__(256) backtrans_wy_kernel(int N, int nmat, int 
 #pragma unroll
             for (int v = 0; v < 4; v++) Ws[(l4 + 4 * v) * ZS + cc * 16 + l16] = -acc[v];
         }
+        if (bi > 0) load_v((bi - 1) * BW); /* in flight during the update */
         __syncthreads();
         /* Z += V W */
         for (int t = wave; t < nrt * nct; t += 4) {
@@ -1076,6 +1107,10 @@ __global__ void __launch_bounds__(256) backtrans_wy_kernel(int N, int nmat, int 
             }
 #pragma unroll
             for (int v = 0; v < 4; v++) Zs[(rt * 16 + l4 + 4 * v) * ZS + cc * 16 + l16] = acc[v];
+        }
+        if (bi > 0) {
+            __syncthreads(); /* V of block bi no longer read */
+            store_v();
         }
     }
     __syncthreads();
@@ -1113,7 +1148,7 @@ template <class T, bool LDS> int launch_tridiag(pmx_ctx *c, EigArgs &a, int grid
 
 template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, const double *Y, const void *vref,
                                         const void *tau, const double *phase, zd *evecs) {
-    if (sizeof(T) == sizeof(double) && N >= 2 && wy_lds_bytes(N, nvec) <= c->lds_max) {
+    if (sizeof(T) == sizeof(double) && N >= 2 && N <= 16 * BT_VL && wy_lds_bytes(N, nvec) <= c->lds_max) {
         const size_t lds = wy_lds_bytes(N, nvec);
         if (lds > 65536)
             PMX_CHECK_HIP(hipFuncSetAttribute((const void *)backtrans_wy_kernel,
