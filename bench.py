@@ -138,6 +138,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--grid", type=int, default=NGRID)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="first run one device-to-device copy of this many bytes (a plain copy kernel of known "
+                         "size, to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     opt = ap.parse_args()
     if opt.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (k-sharding is described "
@@ -146,6 +149,14 @@ def main():
 
     inp = capi.Input(cli_args())
     ctx = capi.Context(0)
+    if opt.copy_bytes > 0:
+        hc = Hip()
+        src, dst = hc.malloc(opt.copy_bytes), hc.malloc(opt.copy_bytes)
+        if hc.h.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), opt.copy_bytes, 3) != 0:
+            raise RuntimeError("hipMemcpy failed")
+        hc.sync()
+        hc.free(src)
+        hc.free(dst)
     ctx.set_system(inp.system(), inp)
     K = kpoints.gamma_grid("zb", opt.grid)
     KW = np.full(len(K), 1.0 / len(K))

@@ -534,7 +534,11 @@ template <class T> __global__ void __launch_bounds__(256) moment_mfma_kernel(Mom
     typedef typename MfmaT<T>::V V;
     typedef typename MfmaT<T>::C C;
     constexpr int MAXW = 5;
-    const int pairi = blockIdx.x, b = blockIdx.y;
+    /* workgroup l runs on XCD l % 8: bins go round-robin over the XCDs and all
+     * tile pairs of a bin run on the same one, sharing its rows in that L2 */
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int pairi = slot % a.npairs, b = xcd + 8 * (slot / a.npairs);
+    if (b >= a.nbins) return;
     const int r0 = a.start[b], r1 = a.start[b + 1]; /* an empty bin writes zero moments */
     int ti, tj;
     tile_pair(pairi, ti, tj);
@@ -613,8 +617,12 @@ constexpr int MOM_MAXNT = 12; /* largest order count of the split-BF16 kernel */
 /* NI items per row: 1 (spatial), 2 (SOC: + and - channel); MAXNT >= NT orders */
 template <int NI, int MAXNT>
 __global__ void __launch_bounds__(256) moment_bf16x3_kernel(MomArgs2 a) {
-    __shared__ fx4 red[2][64][2 * MAXNT / 2];
-    const int pairi = blockIdx.x, b = blockIdx.y;
+    __shared__ fx4 red[2][64][2 * ((MAXNT + 1) / 2)];
+    /* workgroup l runs on XCD l % 8: bins go round-robin over the XCDs and all
+     * tile pairs of a bin run on the same one, sharing its rows in that L2 */
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int pairi = slot % a.npairs, b = xcd + 8 * (slot / a.npairs);
+    if (b >= a.nbins) return;
     const int r0 = a.start[b], r1 = a.start[b + 1]; /* an empty bin writes zero moments */
     int ti, tj;
     tile_pair(pairi, ti, tj);
@@ -715,13 +723,16 @@ __global__ void __launch_bounds__(256) moment_bf16x3_kernel(MomArgs2 a) {
     }
     /* sum the four waves' partials: waves 2, 3 -> 0, 1, then 1 -> 0, in
      * halves of the orders to bound the LDS */
-    for (int h0 = 0; h0 < NT; h0 += MAXNT / 2) {
-        const int hn = min(MAXNT / 2, NT - h0);
+    constexpr int HALF = (MAXNT + 1) / 2;
+#pragma unroll
+    for (int h0 = 0; h0 < MAXNT; h0 += HALF) { /* h0 is a constant in each copy: the accumulators stay in registers */
+        if (h0 >= NT) break;
+        const int hn = min(HALF, NT - h0);
         for (int stage = 2; stage >= 1; stage >>= 1) {
             __syncthreads();
             if (wave >= stage && wave < 2 * stage) {
 #pragma unroll
-                for (int j = 0; j < MAXNT / 2; j++)
+                for (int j = 0; j < HALF; j++)
                     if (j < hn) {
                         red[wave - stage][lane][2 * j] = accr[h0 + j];
                         red[wave - stage][lane][2 * j + 1] = acci[h0 + j];
@@ -730,7 +741,7 @@ __global__ void __launch_bounds__(256) moment_bf16x3_kernel(MomArgs2 a) {
             __syncthreads();
             if (wave < stage) {
 #pragma unroll
-                for (int j = 0; j < MAXNT / 2; j++)
+                for (int j = 0; j < HALF; j++)
                     if (j < hn) {
                         accr[h0 + j] += red[wave][lane][2 * j];
                         acci[h0 + j] += red[wave][lane][2 * j + 1];
@@ -1446,17 +1457,17 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                     const bool bf = p->precision == PMX_PREC_BF16X3 && NT <= MOM_MAXNT; /* else the FP32 MFMA kernel */
                     Prof pf(c, bf ? "moment_bf16x3" : (fp32 ? "moment_mfma_f32" : "moment_mfma_f64"), fl, by);
                     if (bf && nch == 1 && NT <= 9)
-                        hipLaunchKernelGGL((moment_bf16x3_kernel<1, 9>), dim3(npairs, nbins), dim3(256), 0, st, ma);
+                        hipLaunchKernelGGL((moment_bf16x3_kernel<1, 9>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
                     else if (bf && nch == 1)
-                        hipLaunchKernelGGL((moment_bf16x3_kernel<1, MOM_MAXNT>), dim3(npairs, nbins), dim3(256), 0, st, ma);
+                        hipLaunchKernelGGL((moment_bf16x3_kernel<1, MOM_MAXNT>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
                     else if (bf && NT <= 9)
-                        hipLaunchKernelGGL((moment_bf16x3_kernel<2, 9>), dim3(npairs, nbins), dim3(256), 0, st, ma);
+                        hipLaunchKernelGGL((moment_bf16x3_kernel<2, 9>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
                     else if (bf)
-                        hipLaunchKernelGGL((moment_bf16x3_kernel<2, MOM_MAXNT>), dim3(npairs, nbins), dim3(256), 0, st, ma);
+                        hipLaunchKernelGGL((moment_bf16x3_kernel<2, MOM_MAXNT>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
                     else if (fp32)
-                        hipLaunchKernelGGL(moment_mfma_kernel<float>, dim3(npairs, nbins), dim3(256), 0, st, ma);
+                        hipLaunchKernelGGL(moment_mfma_kernel<float>, dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
                     else
-                        hipLaunchKernelGGL(moment_mfma_kernel<double>, dim3(npairs, nbins), dim3(256), 0, st, ma);
+                        hipLaunchKernelGGL(moment_mfma_kernel<double>, dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
                 }
                 {
                     Prof pf(c, fp32 ? "conv_f32" : "conv_f64", 4.0 * 9.0 * ne2 * p->nfreq * (2 * wh + 1) * NT, 0.0);

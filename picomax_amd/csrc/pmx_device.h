@@ -125,6 +125,16 @@ int eig_device(pmx_ctx *ctx, int nmat, const double *dKpts, int nb, int nvec, do
 /* basis size of the solve (NPW spatial or 2 NPW spinor) */
 inline int basis_size(const pmx_ctx *c) { return c->soc ? 2 * c->npw : c->npw; }
 int rebuild_vtable(pmx_ctx *ctx);
+
+/* Workgroup b of a 1-D grid of G is dispatched to XCD b % 8; this maps it to
+ * a work index so that each XCD takes one contiguous range of indices, and
+ * neighbouring work items (which share operands) meet in one XCD's L2. */
+__device__ __forceinline__ int xcd_remap(int b, int G) {
+    const int x = b & 7, s = b >> 3;
+    int start = 0;
+    for (int y = 0; y < x; y++) start += (G - y + 7) >> 3;
+    return start + s;
+}
 } // namespace pmx
 
 #endif

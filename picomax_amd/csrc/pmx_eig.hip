@@ -973,12 +973,6 @@ constexpr int BW = 16;
 constexpr int VS = BW + 1;
 constexpr int BT_VL = 16; /* reflector values per thread: Np <= 256 */
 
-__device__ __forceinline__ int xcd_remap(int b, int G) {
-    const int x = b & 7, s = b >> 3;
-    int start = 0;
-    for (int y = 0; y < x; y++) start += (G - y + 7) >> 3;
-    return start + s;
-}
 
 __device__ __forceinline__ dx4 mma64(double a, double b, dx4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
