@@ -982,7 +982,12 @@ __global__ void __launch_bounds__(256) kk_gemm_kernel(int M, int NF, const doubl
                                                       const T *Kk, double *Re, double *Im, int ne2, int nq, int q) {
     typedef typename MfmaT<T>::V V;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int row0 = (blockIdx.y * 4 + wave) * 16, col0 = blockIdx.x * 16 * KK_CT;
+    /* workgroup l runs on XCD l % 8: row groups go round-robin over the XCDs
+     * and all column groups of a row group run on the same XCD (A rows in L2) */
+    const int ncg = (NF + 16 * KK_CT - 1) / (16 * KK_CT);
+    const int slot = blockIdx.x >> 3;
+    const int rg = (blockIdx.x & 7) + 8 * (slot / ncg), cg = slot % ncg;
+    const int row0 = (rg * 4 + wave) * 16, col0 = cg * 16 * KK_CT;
     if (row0 >= M) return;
     const int ar = row0 + (lane & 15), kq = lane >> 4;
     V accR[KK_CT], accI[KK_CT];
@@ -1501,7 +1506,7 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         if (p->kk) {
             Prof pf(c, "kk", 4.0 * per_q * p->nfreq, 8.0 * 4 * per_q);
             const int M = 9 * ne2;
-            const dim3 grid((p->nfreq + 16 * KK_CT - 1) / (16 * KK_CT), (M + 63) / 64);
+            const dim3 grid(8 * (((M + 63) / 64 + 7) / 8) * ((p->nfreq + 16 * KK_CT - 1) / (16 * KK_CT)));
             if (fp32)
                 hipLaunchKernelGGL(kk_gemm_kernel<float>, grid, dim3(256), 0, st, M, p->nfreq, w.Re0.p, w.Im0.p,
                                    w.Kkf.p, dRe, dIm, ne2, p->nq, q);
@@ -1551,8 +1556,9 @@ extern "C" int pmx_kk(pmx_ctx *c, int nel, int nfreq, double dfreq, double *ReX,
     PMX_CHECK_HIP(hipMemcpyAsync(r0.p, ReX, sizeof(double) * n, hipMemcpyHostToDevice, st));
     PMX_CHECK_HIP(hipMemcpyAsync(i0.p, ImX, sizeof(double) * n, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(kkmat_kernel<double>, dim3(256), dim3(256), 0, st, nfreq, dfreq, Kk.p);
-    hipLaunchKernelGGL(kk_gemm_kernel<double>, dim3((nfreq + 16 * KK_CT - 1) / (16 * KK_CT), (nel + 63) / 64),
-                       dim3(256), 0, st, nel, nfreq, r0.p, i0.p, Kk.p, r.p, i.p, nel, 1, 0);
+    hipLaunchKernelGGL(kk_gemm_kernel<double>,
+                       dim3(8 * (((nel + 63) / 64 + 7) / 8) * ((nfreq + 16 * KK_CT - 1) / (16 * KK_CT))), dim3(256),
+                       0, st, nel, nfreq, r0.p, i0.p, Kk.p, r.p, i.p, nel, 1, 0);
     PMX_CHECK_HIP(hipGetLastError());
     PMX_CHECK_HIP(hipMemcpyAsync(ReX, r.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     PMX_CHECK_HIP(hipMemcpyAsync(ImX, i.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
