@@ -226,6 +226,8 @@ void pmx_destroy(pmx_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    pmx::chi_work_free(c->chi);
+    c->chi = nullptr;
     (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -1209,6 +1209,8 @@ struct ChiWork {
     DBuf<int> nc, nv, tbase, bind, hist, start, cnt, pos, bad;
 };
 
+void chi_work_free(ChiWork *w) { delete w; }
+
 } // namespace pmx
 
 using namespace pmx;
@@ -1245,7 +1247,8 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
     const int ntile = (D + TILE - 1) / TILE;
     const int npairs = ntile * (ntile + 1) / 2;
 
-    ChiWork w;
+    if (!c->chi) c->chi = new ChiWork();
+    ChiWork &w = *c->chi; /* buffers grow to the largest call and are reused */
     int rc;
     if (p->nk == 0) {
         PMX_CHECK_HIP(hipMemsetAsync(dRe, 0, sizeof(double) * per_q * p->nq, st));

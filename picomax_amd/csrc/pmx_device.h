@@ -63,6 +63,11 @@ template <class T> struct DBuf {
 } // namespace pmx
 
 /* the context behind pmx_ctx */
+namespace pmx {
+struct ChiWork;
+void chi_work_free(ChiWork *w);
+} // namespace pmx
+
 struct pmx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -104,6 +109,9 @@ struct pmx_ctx {
     bool prof = false;
     std::vector<ProfEvent> prof_pending;
     std::map<std::string, ProfAcc> prof_acc;
+
+    /* chi work buffers, kept between calls (allocated on first use) */
+    pmx::ChiWork *chi = nullptr;
 
     /* eigensolver scratch */
     pmx::DBuf<double> dK, dEval, dScratch, dEigWork, dEigRef;
