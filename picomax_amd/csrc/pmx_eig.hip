@@ -556,7 +556,7 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, WPE) tridiag_tile_real_kerne
     __syncthreads();
     /* k + 1 = kb1 + s, unrolled over s: the tile column s is a constant in
      * each copy of the body */
-    constexpr int UNR = (TT % 2 == 0) ? TT : 1; /* odd tiles: no unrolling (register budget) */
+    constexpr int UNR = TT;
     for (int kb1 = 0; kb1 < N; kb1 += UNR) {
 #pragma unroll
     for (int s = 0; s < UNR; s++) {
