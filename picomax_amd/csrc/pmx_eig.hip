@@ -755,7 +755,12 @@ __global__ void __launch_bounds__(256) bisect_kernel(int N, int nmat, int nb, in
         if (fabs(q) < pivmin) q = -pivmin;
         cnt += q < 0.0;
         for (int i = 1; i < N; i++) {
-            q = (d[i] - mid) - e2[i] / q;
+            /* e^2 / q as e^2 * (1/q), the reciprocal from v_rcp_f64 and one
+             * Newton step (a few ulp; the count is that of a T perturbed at
+             * that level) */
+            double r = __builtin_amdgcn_rcp(q);
+            r = fma(fma(-q, r, 1.0), r, r);
+            q = fma(-e2[i], r, d[i] - mid);
             if (fabs(q) < pivmin) q = -pivmin;
             cnt += q < 0.0;
         }
