@@ -1047,37 +1047,55 @@ __global__ void __launch_bounds__(256) backtrans_wy_kernel(int N, int nmat, int 
     store_v();
     for (int bi = nbw - 1; bi >= 0; bi--) {
         __syncthreads(); /* V and taus of block bi stored */
-        /* tile 0: G = V^T V; tile 1+c: W_c = V^T Z_c (sums over the rows) */
-        for (int t = wave; t <= nct; t += 4) {
-            const int cb = (t - 1) * 16;
-            dx4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-            for (int s = 0; s < Np / 4; s += 2) {
-                const int ra = 4 * s + l4, rb = ra + 4;
-                const double a0 = Vs[ra * VS + l16], a1 = Vs[rb * VS + l16];
-                const double b0 = (t == 0) ? a0 : Zs[ra * ZS + cb + l16];
-                const double b1 = (t == 0) ? a1 : Zs[rb * ZS + cb + l16];
-                acc0 = mma64(a0, b0, acc0);
-                acc1 = mma64(a1, b1, acc1);
-            }
-            double *out = (t == 0) ? Gs : Ws + cb;
-            const int os = (t == 0) ? VS : ZS;
+        /* wave 0: G = V^T V, then T from G (lanes 0..15), while waves 1..3
+         * form W_c = V^T Z_c (sums over the rows) */
+        if (wave == 0) {
+            dx4 acc[4];
 #pragma unroll
-            for (int v = 0; v < 4; v++) out[(l4 + 4 * v) * os + l16] = acc0[v] + acc1[v];
-        }
-        __syncthreads();
-        if (wave == 0 && lane < BW) {
-            /* row j = lane of T */
-            const int j = lane;
-            double trow[BW];
+            for (int u = 0; u < 4; u++) acc[u] = dx4{0.0, 0.0, 0.0, 0.0};
+            for (int s0 = 0; s0 < Np / 4; s0 += 4) { /* Np / 4 is a multiple of 4 */
 #pragma unroll
-            for (int i = 0; i < BW; i++) {
-                double sacc = 0.0;
-#pragma unroll
-                for (int m = 0; m < i; m++) sacc = fma(trow[m], Gs[m * VS + i], sacc);
-                trow[i] = (i == j) ? taus[i] : ((i > j) ? -taus[i] * sacc : 0.0);
+                for (int u = 0; u < 4; u++) {
+                    const double av = Vs[(4 * (s0 + u) + l4) * VS + l16];
+                    acc[u] = mma64(av, av, acc[u]);
+                }
             }
 #pragma unroll
-            for (int i = 0; i < BW; i++) Ts[j * VS + i] = trow[i];
+            for (int v = 0; v < 4; v++) Gs[(l4 + 4 * v) * VS + l16] = (acc[0][v] + acc[1][v]) + (acc[2][v] + acc[3][v]);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane < BW) {
+                /* row j = lane of T */
+                const int j = lane;
+                double trow[BW];
+#pragma unroll
+                for (int i = 0; i < BW; i++) {
+                    double sacc = 0.0;
+#pragma unroll
+                    for (int m = 0; m < i; m++) sacc = fma(trow[m], Gs[m * VS + i], sacc);
+                    trow[i] = (i == j) ? taus[i] : ((i > j) ? -taus[i] * sacc : 0.0);
+                }
+#pragma unroll
+                for (int i = 0; i < BW; i++) Ts[j * VS + i] = trow[i];
+            }
+        } else {
+            for (int cc = wave - 1; cc < nct; cc += 3) {
+                const int cb = cc * 16;
+                dx4 acc[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) acc[u] = dx4{0.0, 0.0, 0.0, 0.0};
+                for (int s0 = 0; s0 < Np / 4; s0 += 4) {
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int r = 4 * (s0 + u) + l4;
+                        acc[u] = mma64(Vs[r * VS + l16], Zs[r * ZS + cb + l16], acc[u]);
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    Ws[(l4 + 4 * v) * ZS + cb + l16] = (acc[0][v] + acc[1][v]) + (acc[2][v] + acc[3][v]);
+            }
         }
         __syncthreads();
         /* W_c <- -(T W_c), in place (a wave reads all of its column block first) */
