@@ -101,6 +101,21 @@ def eps_block(hip, dre, dim, n_block):
     return re, im
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated), or None if it has no entry."""
+    path = os.path.join(ROOT, "profiles", "c2_traffic_fetch_write.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
+                    return int(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(workdir):
     """The reference CLI (compiled from its sources into oracle/_ref) on a
     bounded sample of the same workload: every 512th k-point of the 24^3 grid
@@ -196,7 +211,7 @@ def main():
     peak = PEAK_TFLOPS["fp32"] if name.endswith("f32") else (
         PEAK_TFLOPS["bf16x3"] if "bf16" in name else PEAK_TFLOPS["fp64"])
     roof = {"kernel": name, "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4), "traffic": None, "avg_ms": round(avg_ms, 4),
+            "frac": round(achieved / peak, 4), "traffic": measured_traffic(name), "avg_ms": round(avg_ms, 4),
             "launches": rec["count"]}
     eig_names = ("tridiag_real", "tridiag_complex", "bisect", "invit", "backtrans_real", "backtrans_complex")
     heev_ms = sum(v["ms"] for k, v in prof_mix.items() if k in eig_names) / opt.steps
