@@ -1261,7 +1261,8 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
     std::vector<double> KWh(p->KW, p->KW + p->nk);
     if ((rc = upload(st, w.Kd, Kh)) || (rc = upload(st, w.KWd, KWh))) return rc;
     if ((rc = w.evk.ensure((size_t)p->nk * nb_s)) || (rc = w.vk.ensure((size_t)p->nk * nb_s * N))) return rc;
-    if ((rc = eig_device(c, p->nk, w.Kd.p, nb_s, nb_s, w.evk.p, w.vk.p))) return rc;
+    /* rho reads conduction rows at k and valence rows at k+q only */
+    if ((rc = eig_device(c, p->nk, w.Kd.p, nb_s, nb_s, w.evk.p, w.vk.p, 1, p->offset))) return rc;
     if ((rc = w.evkq.ensure((size_t)p->nk * nb_s)) || (rc = w.vkq.ensure((size_t)p->nk * nb_s * N))) return rc;
     if ((rc = w.nc.ensure(p->nk)) || (rc = w.nv.ensure(p->nk)) || (rc = w.tbase.ensure(p->nk + 1)) ||
         (rc = w.bad.ensure(1)))
@@ -1356,7 +1357,7 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
             KQ[3 * k + 2] = Kh[3 * k + 2] + Qz;
         }
         if ((rc = upload(st, w.KQd, KQ))) return rc;
-        if ((rc = eig_device(c, p->nk, w.KQd.p, nb_s, nb_s, w.evkq.p, w.vkq.p))) return rc;
+        if ((rc = eig_device(c, p->nk, w.KQd.p, nb_s, nb_s, w.evkq.p, w.vkq.p, 2, p->offset))) return rc;
 
         /* ---- classification + transition offsets */
         PMX_CHECK_HIP(hipMemsetAsync(w.bad.p, 0, sizeof(int), st));

@@ -116,7 +116,7 @@ struct pmx_ctx {
     /* eigensolver scratch */
     pmx::DBuf<double> dK, dEval, dScratch, dEigWork, dEigRef;
     pmx::DBuf<pmx::zd> dEvec, dZscr;
-    pmx::DBuf<int> dStatus;
+    pmx::DBuf<int> dStatus, dVrange;
 };
 
 namespace pmx {
@@ -128,8 +128,12 @@ struct Prof {
     Prof(pmx_ctx *ctx, const char *name, double flops = 0.0, double bytes = 0.0);
     ~Prof();
 };
-/* eigensolve of nmat H(K) on the context's stream; K is a device array */
-int eig_device(pmx_ctx *ctx, int nmat, const double *dKpts, int nb, int nvec, double *dEvals, zd *dEvecs);
+/* eigensolve of nmat H(K) on the context's stream; K is a device array.
+ * Eigenvalues of the lowest nb states; eigenvectors of the lowest nvec, or,
+ * with vsel = 1 / 2, only of those above / at or below voff (the others are
+ * left unwritten; honoured on the real path, else all nvec are formed) */
+int eig_device(pmx_ctx *ctx, int nmat, const double *dKpts, int nb, int nvec, double *dEvals, zd *dEvecs,
+               int vsel = 0, double voff = 0.0);
 /* basis size of the solve (NPW spatial or 2 NPW spinor) */
 inline int basis_size(const pmx_ctx *c) { return c->soc ? 2 * c->npw : c->npw; }
 int rebuild_vtable(pmx_ctx *ctx);

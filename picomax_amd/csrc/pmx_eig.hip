@@ -771,6 +771,19 @@ __global__ void __launch_bounds__(256) bisect_kernel(int N, int nmat, int nb, in
     evals[(size_t)mat * nb + j] = lam;
 }
 
+/* per matrix, the range [j0, j0 + cnt) of the lowest nvec eigenvalues whose
+ * vectors are wanted: vsel 1 keeps lambda > voff (conduction states), vsel 2
+ * keeps lambda <= voff (valence states); the eigenvalues are ascending, so
+ * either set is contiguous */
+__global__ void vrange_kernel(int nmat, int nvec, int vsel, double voff, const double *lamS, int *vr) {
+    const int mat = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mat >= nmat) return;
+    int nbelow = 0;
+    for (int j = 0; j < nvec; j++) nbelow += (lamS[(size_t)j * nmat + mat] <= voff);
+    vr[2 * mat] = (vsel == 1) ? nbelow : 0;
+    vr[2 * mat + 1] = (vsel == 1) ? nvec - nbelow : nbelow;
+}
+
 /* eigenvectors of T for the lowest nvec eigenvalues by inverse iteration:
  * one thread per cluster of eigenvalues closer than 1e-5 ||T|| (the head
  * thread takes the cluster in order, modified Gram-Schmidt against earlier
@@ -778,7 +791,7 @@ __global__ void __launch_bounds__(256) bisect_kernel(int N, int nmat, int nb, in
  * pass of LU with partial pivoting fused with the forward substitution, then
  * the back substitution; all per-thread arrays are SoA [.][N][slot]. */
 __global__ void invit_kernel(int N, int nmat, int nvec, const double *dS, const double *eS, const double *gb,
-                             const double *lamS, double *Y, double *work) {
+                             const double *lamS, const int *vr, double *Y, double *work) {
     const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (t >= (size_t)nmat * nvec) return;
     const int mat = t % nmat, j = t / nmat;
@@ -788,6 +801,7 @@ __global__ void invit_kernel(int N, int nmat, int nvec, const double *dS, const 
     if (j > 0 && lam(j) - lam(j - 1) <= ortol) return; /* not a cluster head */
     int jend = j + 1;
     while (jend < nvec && lam(jend) - lam(jend - 1) <= ortol) jend++;
+    if (vr && (jend <= vr[2 * mat] || j >= vr[2 * mat] + vr[2 * mat + 1])) return; /* no wanted vector */
     const size_t slots = (size_t)nmat * nvec;
     double *dd = work + t, *du = work + (size_t)N * slots + t, *du2 = work + (size_t)2 * N * slots + t;
     auto at = [&](double *base, int i) -> double & { return base[(size_t)i * slots]; };
@@ -985,7 +999,7 @@ __device__ __forceinline__ dx4 mma64(double a, double b, dx4 c) {
 
 __global__ void __launch_bounds__(256) backtrans_wy_kernel(int N, int nmat, int nvec, int NV, const double *Y,
                                                             const double *vref_all, const double *tau_all,
-                                                            const double *phase, zd *evecs) {
+                                                            const double *phase, const int *vr, zd *evecs) {
     extern __shared__ double smem_d[];
     const int Np = (N + 15) & ~15, ZS = NV + 1;
     double *Zs = smem_d;                  /* [Np][ZS] */
@@ -999,6 +1013,8 @@ __global__ void __launch_bounds__(256) backtrans_wy_kernel(int N, int nmat, int 
     const size_t P = (size_t)N * (N + 1) / 2;
     const double *vref = vref_all + (size_t)mat * P;
     const double *tau = tau_all + (size_t)mat * N;
+    /* the wanted vectors j0 .. j0 + cnt - 1 become columns 0 .. cnt - 1 of Z */
+    const int j0 = vr ? vr[2 * mat] : 0, cnt = vr ? vr[2 * mat + 1] : nvec;
     /* Y -> Z in chunks of 16 loads per thread, all in flight before the stores */
     for (int base = 0; base < Np * NV; base += 16 * 256) {
         double y[16];
@@ -1006,7 +1022,7 @@ __global__ void __launch_bounds__(256) backtrans_wy_kernel(int N, int nmat, int 
         for (int u = 0; u < 16; u++) {
             const int idx = base + u * 256 + tid;
             const int i = idx % Np, j = idx / Np;
-            y[u] = (idx < Np * NV && i < N && j < nvec) ? Y[((size_t)j * N + i) * nmat + mat] : 0.0;
+            y[u] = (idx < Np * NV && i < N && j < cnt) ? Y[((size_t)(j0 + j) * N + i) * nmat + mat] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < 16; u++) {
@@ -1014,7 +1030,7 @@ __global__ void __launch_bounds__(256) backtrans_wy_kernel(int N, int nmat, int 
             if (idx < Np * NV) Zs[(idx % Np) * ZS + idx / Np] = y[u];
         }
     }
-    const int nref = N - 1, nct = NV / 16, nrt = Np / 16;
+    const int nref = N - 1, nct = (cnt + 15) / 16, nrt = Np / 16; /* column tiles holding wanted vectors */
     /* the reflector block bi is held in registers (BT_VL values per thread,
      * Np * BW <= 256 * BT_VL) while block bi + 1 is applied */
     double vreg[BT_VL];
@@ -1131,12 +1147,12 @@ __global__ void __launch_bounds__(256) backtrans_wy_kernel(int N, int nmat, int 
         }
     }
     __syncthreads();
-    for (int idx = tid; idx < nvec * N; idx += 256) {
+    for (int idx = tid; idx < cnt * N; idx += 256) {
         const int j = idx / N, i = idx % N;
         double sn, cs;
         sincos(phase[i], &sn, &cs);
         const double z = Zs[i * ZS + j];
-        evecs[((size_t)mat * nvec + j) * N + i] = zd{cs * z, sn * z};
+        evecs[((size_t)mat * nvec + j0 + j) * N + i] = zd{cs * z, sn * z};
     }
 }
 
@@ -1163,16 +1179,18 @@ template <class T, bool LDS> int launch_tridiag(pmx_ctx *c, EigArgs &a, int grid
     return 0;
 }
 
+bool wy_usable(pmx_ctx *c, int N, int nvec) { return N >= 2 && N <= 16 * BT_VL && wy_lds_bytes(N, nvec) <= c->lds_max; }
+
 template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, const double *Y, const void *vref,
-                                        const void *tau, const double *phase, zd *evecs) {
-    if (sizeof(T) == sizeof(double) && N >= 2 && N <= 16 * BT_VL && wy_lds_bytes(N, nvec) <= c->lds_max) {
+                                        const void *tau, const double *phase, const int *vr, zd *evecs) {
+    if (sizeof(T) == sizeof(double) && wy_usable(c, N, nvec)) {
         const size_t lds = wy_lds_bytes(N, nvec);
         if (lds > 65536)
             PMX_CHECK_HIP(hipFuncSetAttribute((const void *)backtrans_wy_kernel,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(backtrans_wy_kernel, dim3(nmat), dim3(256), lds, c->stream, N, nmat, nvec,
                            (nvec + 15) & ~15, Y, reinterpret_cast<const double *>(vref),
-                           reinterpret_cast<const double *>(tau), phase, evecs);
+                           reinterpret_cast<const double *>(tau), phase, vr, evecs);
         PMX_CHECK_HIP(hipGetLastError());
         return 0;
     }
@@ -1196,7 +1214,8 @@ template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, c
 
 } // namespace
 
-int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, double *dEvals, zd *dEvecs) {
+int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, double *dEvals, zd *dEvecs, int vsel,
+               double voff) {
     if (nmat <= 0) return 0;
     const int N = basis_size(c);
     if (nb > N) nb = N;
@@ -1226,6 +1245,10 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
     char *vref = reinterpret_cast<char *>(c->dEigRef.p);
     char *tauv = vref + cs * P * elem;
     const double n3 = (double)N * N * N, n2 = (double)N * N;
+    /* selected vectors only on the real path's blocked back-transformation */
+    if (!(real && wy_usable(c, N, nvec))) vsel = 0;
+    if (nvec > 0 && vsel && (rc = c->dVrange.ensure(2 * cs))) return rc;
+    int *vr = (nvec > 0 && vsel) ? c->dVrange.p : nullptr;
     for (int m0 = 0; m0 < nmat; m0 += chunk) {
         const int nm = std::min(chunk, nmat - m0);
         EigArgs a{};
@@ -1288,15 +1311,20 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
         if (nvec > 0) {
             {
                 Prof pf(c, "invit", 0.0, 0.0);
+                if (vr)
+                    hipLaunchKernelGGL(vrange_kernel, dim3((nm + 255) / 256), dim3(256), 0, c->stream, nm, nvec, vsel,
+                                       voff, lamS, vr);
                 const size_t nt = (size_t)nm * nvec;
                 hipLaunchKernelGGL(invit_kernel, dim3((nt + 255) / 256), dim3(256), 0, c->stream, N, nm, nvec, dS, eS,
-                                   gb, lamS, Y, work);
+                                   gb, lamS, vr, Y, work);
                 PMX_CHECK_HIP(hipGetLastError());
             }
             {
                 Prof pf(c, real ? "backtrans_real" : "backtrans_complex", nm * (real ? 2.0 : 8.0) * n2 * nvec, 0.0);
-                rc = real ? launch_backtrans<double>(c, N, nm, nvec, Y, vref, tauv, c->dPhase.p, dEvecs + (size_t)m0 * nvec * N)
-                          : launch_backtrans<zd>(c, N, nm, nvec, Y, vref, tauv, c->dPhase.p, dEvecs + (size_t)m0 * nvec * N);
+                rc = real ? launch_backtrans<double>(c, N, nm, nvec, Y, vref, tauv, c->dPhase.p, vr,
+                                                     dEvecs + (size_t)m0 * nvec * N)
+                          : launch_backtrans<zd>(c, N, nm, nvec, Y, vref, tauv, c->dPhase.p, nullptr,
+                                                 dEvecs + (size_t)m0 * nvec * N);
                 if (rc) return rc;
             }
         }
