@@ -888,17 +888,20 @@ template <class T> __global__ void __launch_bounds__(256) conv_mfma_kernel(ConvA
         accR[q] = V{0, 0, 0, 0};
         accI[q] = V{0, 0, 0, 0};
     }
+    /* M operand of K step kk = k0 + kq: row kk of the flattened (bin, order)
+     * axis; loaded two K steps ahead.  The table index of A is
+     * (b - fa) NT + n = kk + (b_lo - fa) NT, so no division is needed. */
+    auto mload = [&](int kk) -> C {
+        C m{0, 0};
+        if (kk < kcount) m = M[((size_t)b_lo * a.NT + kk) * E + eb];
+        return m;
+    };
+    C m0 = mload(kq), m1 = mload(4 + kq);
+    const int WN = W * a.NT;
     for (int k0 = 0; k0 < kcount; k0 += 4) {
         const int kk = k0 + kq;
-        T br = 0, bi = 0;
-        int b = b_lo, n = 0;
-        if (kk < kcount) {
-            b = b_lo + kk / a.NT;
-            n = kk % a.NT;
-            const C m = M[((size_t)b * a.NT + n) * E + eb];
-            br = m.x;
-            bi = m.y;
-        }
+        const C m2 = mload(k0 + 8 + kq);
+        const T br = m0.x, bi = m0.y;
         /* bins of this k step (wave-uniform range) */
         const int bs0 = b_lo + k0 / a.NT, bs1 = b_lo + min(k0 + 3, kcount - 1) / a.NT;
 #pragma unroll
@@ -906,11 +909,13 @@ template <class T> __global__ void __launch_bounds__(256) conv_mfma_kernel(ConvA
             const int fq = fb0 + 16 * q;
             if (bs1 < fq || bs0 > fq + 15 + 2 * a.wh) continue; /* outside this block's band */
             const int fa = fq + (lane & 15);                      /* A row: frequency */
-            const int Dd = b - a.wh - fa;
-            const T av = (kk < kcount && fa < a.nfreq && Dd >= -a.wh && Dd <= a.wh) ? kt[(Dd + a.wh) * a.NT + n] : T(0);
+            const int ti = kk + (b_lo - fa) * a.NT;
+            const T av = (kk < kcount && fa < a.nfreq && ti >= 0 && ti < WN) ? kt[ti] : T(0);
             accR[q] = MfmaT<T>::mma(av, br, accR[q]);
             accI[q] = MfmaT<T>::mma(av, bi, accI[q]);
         }
+        m0 = m1;
+        m1 = m2;
     }
 #pragma unroll
     for (int q = 0; q < CONV_FB; q++)
@@ -996,20 +1001,34 @@ __global__ void __launch_bounds__(256) kk_gemm_kernel(int M, int NF, const doubl
         accR[c] = V{0, 0, 0, 0};
         accI[c] = V{0, 0, 0, 0};
     }
-    for (int k0 = 0; k0 < NF; k0 += 4) {
-        const int k = k0 + kq;
-        T a1 = 0, a2 = 0;
+    /* operands of K step k = k0 + kq, loaded one step ahead */
+    struct Ops {
+        T a1, a2, b[KK_CT];
+    };
+    auto load = [&](int k) {
+        Ops o;
+        o.a1 = 0;
+        o.a2 = 0;
         if (k < NF && ar < M) {
-            a1 = (T)Im0[(size_t)ar * NF + k];
-            a2 = (T)Re0[(size_t)ar * NF + k];
+            o.a1 = (T)Im0[(size_t)ar * NF + k];
+            o.a2 = (T)Re0[(size_t)ar * NF + k];
         }
 #pragma unroll
         for (int c = 0; c < KK_CT; c++) {
             const int bc = col0 + 16 * c + (lane & 15);
-            const T b = (k < NF && bc < NF) ? Kk[(size_t)k * NF + bc] : T(0);
-            accR[c] = MfmaT<T>::mma(a1, b, accR[c]);
-            accI[c] = MfmaT<T>::mma(a2, b, accI[c]);
+            o.b[c] = (k < NF && bc < NF) ? Kk[(size_t)k * NF + bc] : T(0);
         }
+        return o;
+    };
+    Ops cur = load(kq);
+    for (int k0 = 0; k0 < NF; k0 += 4) {
+        const Ops nxt = load(k0 + 4 + kq);
+#pragma unroll
+        for (int c = 0; c < KK_CT; c++) {
+            accR[c] = MfmaT<T>::mma(cur.a1, cur.b[c], accR[c]);
+            accI[c] = MfmaT<T>::mma(cur.a2, cur.b[c], accI[c]);
+        }
+        cur = nxt;
     }
 #pragma unroll
     for (int c = 0; c < KK_CT; c++)
