@@ -116,6 +116,62 @@ def measured_traffic(kernel):
     return None
 
 
+TE_VG = ("0.565323,1.333333,1.898656,2.261292,3.594626,4.0" + ",-0.2105,-0.1322,0.0451,0.0214,0.0127,-0.0093" * 3)
+WZ_VG = ("0.378233,1.333333,1.51293,1.711566,2.846264,3.404093" + ",-0.3101,-0.1810,0.0902,0.0415,0.0254,-0.0122" * 2
+         + ",-0.2401,0.1110,0.0502,-0.0315,0.0054,0.0222" * 2)
+EIG_SWEEP = [  # (basis size N, system, command-line set-up); SOC doubles N (spinor basis)
+    (59, "Si zb, gcut sqrt12", ["-crystal", "zb", "-a", "5.43", "-vg", SI_VG, "-gcut", "3.4641016151377544"]),
+    (118, "Si zb + SOC", ["-crystal", "zb", "-a", "5.43", "-vg", SI_VG, "-gcut", "3.4641016151377544",
+                          "-us_so", "3:0.01,8:0.005,11:0.002"]),
+    (137, "Si zb, gcut sqrt24", ["-crystal", "zb", "-a", "5.43", "-vg", SI_VG, "-gcut", GCUT]),
+    (274, "Si zb + SOC", ["-crystal", "zb", "-a", "5.43", "-vg", SI_VG, "-gcut", GCUT,
+                          "-us_so", "3:0.01,8:0.005,11:0.002"]),
+    (291, "Te, gcut 4.0", ["-crystal", "te", "-a", "4.45", "-f", "1.33", "-u", "0.26", "-vg", TE_VG, "-gcut", "4.0"]),
+    (582, "Te + SOC", ["-crystal", "te", "-a", "4.45", "-f", "1.33", "-u", "0.26", "-vg", TE_VG, "-gcut", "4.0",
+                       "-us_so", "0.565323:0.0152,1.333333:-0.0061,1.898656:0.0083",
+                       "-ua_so", "0.565323:0.0117,1.333333:0.0042,1.898656:-0.0071"]),
+    (401, "wz, gcut 4.2", ["-crystal", "wz", "-a", "3.19", "-f", "1.626", "-u", "0.375", "-vg", WZ_VG, "-gcut", "4.2"]),
+    (802, "wz + SOC", ["-crystal", "wz", "-a", "3.19", "-f", "1.626", "-u", "0.375", "-vg", WZ_VG, "-gcut", "4.2",
+                       "-us_so", "0.378233:0.01,1.333333:-0.004"]),
+]
+
+
+def eig_sweep(hip, budget_s=40.0):
+    """Eigensolves/s (32 lowest spinor states, eigenvalues and vectors) at the
+    basis sizes of the five configurations: device time of the eigensolver
+    kernels (HIP events), and the wall rate with the copy to the host."""
+    out = {}
+    t_start = time.perf_counter()
+    rng = np.random.default_rng(5)
+    for n_expect, label, args in EIG_SWEEP:
+        if time.perf_counter() - t_start > budget_s:
+            break
+        try:
+            inp = capi.Input(args)
+            ctx = capi.Context(0)
+            ctx.set_system(inp.system(), inp)
+            nk = 4096 if n_expect <= 140 else (1024 if n_expect <= 300 else 128)
+            K = rng.uniform(-0.5, 0.5, (nk, 3))
+            ctx.eig(K[:8], 32, vectors=True)  # warm-up (buffers, code objects)
+            hip.sync()
+            ctx.profile(True)
+            t0 = time.perf_counter()
+            ctx.eig(K, 32, vectors=True)
+            hip.sync()
+            dt = time.perf_counter() - t0
+            rep = ctx.profile_report()
+            ctx.profile(False)
+            dev_ms = sum(v["ms"] for k, v in rep.items() if k.startswith(("tridiag", "bisect", "invit", "backtrans")))
+            n = inp.system().npw * (2 if ("-us_so" in args) else 1)
+            out[str(n)] = {"system": label, "solves": nk,
+                           "per_s": round(nk / (dev_ms * 1e-3), 1) if dev_ms > 0 else None,
+                           "per_s_with_pcie": round(nk / dt, 1)}
+            ctx.close()
+        except Exception as e:  # report, never fail the bench line
+            out[str(n_expect)] = {"system": label, "error": str(e)[:120]}
+    return out
+
+
 def cpu_baseline(workdir):
     """The reference CLI (compiled from its sources into oracle/_ref) on a
     bounded sample of the same workload: every 512th k-point of the 24^3 grid
@@ -152,6 +208,7 @@ def main():
     ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x3"])
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-eig-sweep", action="store_true", help="skip the eigensolves/s sweep over basis sizes")
     ap.add_argument("--grid", type=int, default=NGRID)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (a plain copy kernel of known "
@@ -217,6 +274,8 @@ def main():
     heev_ms = sum(v["ms"] for k, v in prof_mix.items() if k in eig_names) / opt.steps
     eigs_per_step = 2 * nk
 
+    sweep = None if opt.no_eig_sweep else eig_sweep(hip)
+
     cpu = None
     if not opt.no_cpu:
         with tempfile.TemporaryDirectory() as td:
@@ -244,6 +303,7 @@ def main():
         "elf_rel_err": elf_err,
         "im_epsM_rel_err": im_err,
         "eigensolves_per_s": round(eigs_per_step / (heev_ms * 1e-3), 1) if heev_ms > 0 else None,
+        "eigensolves_per_s_by_N": sweep,
         "kernels_ms_per_step": {k: round(v["ms"] / opt.steps, 3) for k, v in prof_mix.items()},
         "fp64_kernels_ms_per_step": {k: round(v["ms"] / opt.fp64_steps, 3) for k, v in prof_64.items()},
         "roofline": roof,
