@@ -123,6 +123,53 @@ __global__ void vtable_kernel(const int *shell, const zd *sfac, const double *vg
 
 } // namespace
 
+/* real gauge: an inversion centre s with H0_ij e^{i 2pi (G_i - G_j).s} real
+ * for every pair (no SOC only), for the current form factors.  Candidates:
+ * atoms and pair midpoints.  Sets real_gauge and phase (host only). */
+static void detect_real_gauge(pmx_ctx *c) {
+    const int n = c->npw, na = c->natom, nv = c->nv;
+    c->real_gauge = false;
+    c->phase.assign(n, 0.0);
+    if (!c->soc) {
+        std::vector<double> cand;
+        for (int a = 0; a < na; a++)
+            for (int b = a; b < na; b++)
+                for (double sg : {1.0, -1.0})
+                    for (int d = 0; d < 3; d++) cand.push_back(sg * 0.5 * (c->tau[3 * a + d] + c->tau[3 * b + d]));
+        double vmax = 0.0;
+        std::vector<zd> V((size_t)n * n, zd{0, 0});
+        for (size_t id = 0; id < V.size(); id++) {
+            int sh = c->shell[id];
+            if (sh < 0) continue;
+            double ax = 0, ay = 0;
+            for (int a = 0; a < na; a++) {
+                ax += c->sfac[id * na + a].x * c->vg[(size_t)a * nv + sh];
+                ay += c->sfac[id * na + a].y * c->vg[(size_t)a * nv + sh];
+            }
+            V[id] = zd{ax / na * PMX_RY2EV, ay / na * PMX_RY2EV};
+            vmax = std::max(vmax, std::hypot(V[id].x, V[id].y));
+        }
+        for (size_t ci = 0; ci < cand.size() / 3 && !c->real_gauge; ci++) {
+            const double *sc = &cand[3 * ci];
+            std::vector<double> ph(n);
+            for (int p = 0; p < n; p++)
+                ph[p] = 2.0 * PMX_PI * (c->G[3 * p] * sc[0] + c->G[3 * p + 1] * sc[1] + c->G[3 * p + 2] * sc[2]);
+            bool ok = true;
+            for (int i = 0; i < n && ok; i++)
+                for (int j = 0; j < n && ok; j++) {
+                    const zd &v = V[(size_t)i * n + j];
+                    double t = -(ph[i] - ph[j]);
+                    double imv = v.x * std::sin(t) + v.y * std::cos(t);
+                    if (std::fabs(imv) > 1e-12 * std::max(1.0, vmax)) ok = false;
+                }
+            if (ok) {
+                c->real_gauge = true;
+                c->phase = ph;
+            }
+        }
+    }
+}
+
 int rebuild_vtable(pmx_ctx *c) {
     const int n = c->npw;
     int rc = c->dVpair.ensure((size_t)n * n);
@@ -299,48 +346,7 @@ int pmx_set_system(pmx_ctx *c, const pmx_system *s) {
             }
         }
 
-    /* real gauge: an inversion centre s with H0_ij e^{i 2pi (G_i - G_j).s} real
-     * for every pair (no SOC only).  Candidates: atoms and pair midpoints. */
-    c->real_gauge = false;
-    c->phase.assign(n, 0.0);
-    if (!c->soc) {
-        std::vector<double> cand;
-        for (int a = 0; a < na; a++)
-            for (int b = a; b < na; b++)
-                for (double sg : {1.0, -1.0})
-                    for (int d = 0; d < 3; d++) cand.push_back(sg * 0.5 * (s->tau[3 * a + d] + s->tau[3 * b + d]));
-        double vmax = 0.0;
-        std::vector<zd> V((size_t)n * n, zd{0, 0});
-        for (size_t id = 0; id < V.size(); id++) {
-            int sh = c->shell[id];
-            if (sh < 0) continue;
-            double ax = 0, ay = 0;
-            for (int a = 0; a < na; a++) {
-                ax += c->sfac[id * na + a].x * c->vg[(size_t)a * nv + sh];
-                ay += c->sfac[id * na + a].y * c->vg[(size_t)a * nv + sh];
-            }
-            V[id] = zd{ax / na * PMX_RY2EV, ay / na * PMX_RY2EV};
-            vmax = std::max(vmax, std::hypot(V[id].x, V[id].y));
-        }
-        for (size_t ci = 0; ci < cand.size() / 3 && !c->real_gauge; ci++) {
-            const double *sc = &cand[3 * ci];
-            std::vector<double> ph(n);
-            for (int p = 0; p < n; p++)
-                ph[p] = 2.0 * PMX_PI * (c->G[3 * p] * sc[0] + c->G[3 * p + 1] * sc[1] + c->G[3 * p + 2] * sc[2]);
-            bool ok = true;
-            for (int i = 0; i < n && ok; i++)
-                for (int j = 0; j < n && ok; j++) {
-                    const zd &v = V[(size_t)i * n + j];
-                    double t = -(ph[i] - ph[j]);
-                    double imv = v.x * std::sin(t) + v.y * std::cos(t);
-                    if (std::fabs(imv) > 1e-12 * std::max(1.0, vmax)) ok = false;
-                }
-            if (ok) {
-                c->real_gauge = true;
-                c->phase = ph;
-            }
-        }
-    }
+    detect_real_gauge(c);
 
     int rc;
     if ((rc = c->dG.ensure(3 * n))) return rc;
@@ -403,6 +409,10 @@ int pmx_set_form_factors(pmx_ctx *c, const double *vg) {
     if (!c->has_system) return set_error(PMX_ESTATE, "no system set");
     PMX_CHECK_HIP(hipSetDevice(c->device));
     c->vg.assign(vg, vg + (size_t)c->natom * c->nv);
+    /* form factors that differ per atom can remove the inversion centre */
+    detect_real_gauge(c);
+    PMX_CHECK_HIP(hipMemcpyAsync(c->dPhase.p, c->phase.data(), sizeof(double) * c->npw, hipMemcpyHostToDevice,
+                                 c->stream));
     int rc = rebuild_vtable(c);
     if (rc) return rc;
     PMX_CHECK_HIP(hipStreamSynchronize(c->stream));
