@@ -152,6 +152,18 @@ int pmx_epsinv(pmx_ctx *ctx, int nq, int neps, int nfreq, const double *ReX, con
  * offset), returns gvg [natom][nv] = sum_k,n gE[k][n] dE_n(k)/dvg. */
 int pmx_grad_bands(pmx_ctx *ctx, int nk, const double *K, int nband, const double *gE, double *gvg);
 
+/* Adjoint of chi_tensor (src/pmx_chi.cpp:319-713) with respect to the form
+ * factors (no reference counterpart): given upstream gradients gRe, gIm of
+ * the ReX/ImX that pmx_chi returns for the same p (same layout), returns
+ * gvg [natom][nv] = d/dvg sum (gRe ReX + gIm ImX) of the exact windowed sum
+ * in FP64, whatever p->precision says.  The energy zero p->offset is held
+ * fixed (it only classifies states).  Spatial solve only (no SOC, even
+ * nband) and Gaussian or Lorentzian broadening; PMX_EINVAL otherwise.
+ * Eigenvector derivatives by first-order perturbation over the whole basis,
+ * pairs closer than 1e-8 eV masked; all N eigenvectors of every k and k+q are
+ * held on the device (16 N^2 bytes per k-point each). */
+int pmx_grad_chi(pmx_ctx *ctx, const pmx_chi_params *p, const double *gRe, const double *gIm, double *gvg);
+
 /* --------------------------------------------------------- host set-up
  * The reference's CLI set-up (inputParser src/pmx.hpp:190-245, inputProcess
  * src/pmx_io.cpp:423-503, init_crystal :108-310, init_mater :315-417,

@@ -7,6 +7,13 @@ Hellmann-Feynman adjoint pmx_grad_bands, including the offset's own
 dependence on the form factors.  Eigenvectors are recomputed per batch of
 `batch` k-points in the backward pass rather than stored.
 
+`ChiResponse.apply(vg, model, spec)` returns (ReX, ImX) of pmx_chi (the
+reference's chi_tensor output, src/pmx_chi.cpp:319-713) for the arguments in
+`spec` (a ChiSpec); its backward is pmx_grad_chi, the adjoint through the
+oscillator strengths, the broadening and the eigenvectors (first-order
+perturbation over the whole basis).  The energy zero is held fixed: it only
+decides which states are valence or conduction.
+
 The forward and backward run on the GPU through the C-ABI; torch is only the
 autograd tape here (tensors stay on the CPU, the arrays cross as numpy).
 """
@@ -80,3 +87,40 @@ class BandEnergies(torch.autograd.Function):
 
 def band_energies(vg, model, K, nband):
     return BandEnergies.apply(vg, model, K, nband)
+
+
+class ChiSpec:
+    """Arguments of one pmx_chi call (see capi.Context.chi)."""
+
+    def __init__(self, K, KW, Q, nband, offset, delta, epsilon, nfreq, dfreq, kk=0, precision="fp64"):
+        self.K = np.asarray(K, float).reshape(-1, 3)
+        self.KW = np.asarray(KW, float).reshape(-1)
+        self.Q = np.asarray(Q, float).reshape(-1, 3)
+        self.nband, self.offset, self.delta = int(nband), float(offset), int(delta)
+        self.epsilon, self.nfreq, self.dfreq = float(epsilon), int(nfreq), float(dfreq)
+        self.kk, self.precision = int(kk), precision
+
+    def common(self):
+        return (self.K, self.KW, self.Q, self.nband, self.offset, self.delta, self.epsilon, self.nfreq, self.dfreq)
+
+
+class ChiResponse(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, vg, model, spec):
+        model.set_vg(vg.detach().cpu().numpy())
+        re, im = model.ctx.chi(*spec.common(), kk=spec.kk, precision=spec.precision)
+        ctx.model, ctx.spec = model, spec
+        ctx.save_for_backward(vg)
+        return torch.from_numpy(re), torch.from_numpy(im)
+
+    @staticmethod
+    def backward(ctx, gRe, gIm):
+        vg, = ctx.saved_tensors
+        model, spec = ctx.model, ctx.spec
+        model.set_vg(vg.detach().cpu().numpy())
+        g = model.ctx.grad_chi(*spec.common(), gRe.detach().cpu().numpy(), gIm.detach().cpu().numpy(), kk=spec.kk)
+        return torch.from_numpy(g).reshape(vg.shape).to(vg.dtype), None, None
+
+
+def chi_response(vg, model, spec):
+    return ChiResponse.apply(vg, model, spec)

@@ -21,7 +21,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 # every symbol include/pmx.h declares
 EXPORTS = ["pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_set_system",
            "pmx_set_form_factors", "pmx_get_loci", "pmx_ref_energy", "pmx_bands", "pmx_eig_batch",
-           "pmx_chi", "pmx_chi_device", "pmx_profile", "pmx_profile_report", "pmx_kk", "pmx_epsinv", "pmx_grad_bands", "pmx_input_parse", "pmx_input_free",
+           "pmx_chi", "pmx_chi_device", "pmx_profile", "pmx_profile_report", "pmx_kk", "pmx_epsinv", "pmx_grad_bands", "pmx_grad_chi", "pmx_input_parse", "pmx_input_free",
            "pmx_input_system", "pmx_input_info", "pmx_generate_gpt", "pmx_run"]
 
 
@@ -86,6 +86,7 @@ def lib():
     L.pmx_kk.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp]
     L.pmx_epsinv.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
     L.pmx_grad_bands.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
+    L.pmx_grad_chi.argtypes = [vp, ctypes.POINTER(PmxChiParams), _dp, _dp, _dp]
     L.pmx_input_parse.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(vp)]
     L.pmx_input_free.argtypes = [vp]
     L.pmx_input_free.restype = None
@@ -272,6 +273,19 @@ class Context:
         gE = _f64(gE, (len(K), nband))
         out = np.zeros((self.natom, self.nv))
         _check(lib().pmx_grad_bands(self.h, len(K), _p(K), int(nband), _p(gE), _p(out)))
+        return out
+
+    def grad_chi(self, K, KW, Q, nband, offset, delta, epsilon, nfreq, dfreq, gRe, gIm, kk=0):
+        """d/dvg of sum(gRe * ReX + gIm * ImX) for the chi() of the same arguments."""
+        p = self.chi_params(K, KW, Q, nband, offset, delta, epsilon, nfreq, dfreq, kk, "fp64")
+        ne2 = self.neps * (self.neps + 1) // 2
+        n = 9 * p.nq * ne2 * nfreq
+        gRe = _f64(gRe).reshape(-1)
+        gIm = _f64(gIm).reshape(-1)
+        if gRe.size != n or gIm.size != n:
+            raise PmxError("grad_chi: upstream gradients must have %d entries" % n)
+        out = np.zeros((self.natom, self.nv))
+        _check(lib().pmx_grad_chi(self.h, ctypes.byref(p), _p(gRe), _p(gIm), _p(out)))
         return out
 
     def run(self, inp, precision="fp64"):

@@ -1234,6 +1234,77 @@ void chi_work_free(ChiWork *w) { delete w; }
 
 using namespace pmx;
 
+/* SCALEFACTOR of chi_tensor (src/pmx_chi.cpp:631-634) */
+static double chi_scale(const pmx_ctx *c) {
+    return 4.0 * c->bz * std::pow(PMX_PI, 5) * std::pow(PMX_HBAR, 4) * std::pow(PMX_E, 2) /
+           (std::pow(PMX_EV, 3) * std::pow(PMX_EMASS, 2) * std::pow(c->a * PMX_ANGSTROM, 5));
+}
+
+/* per-q tables of rho: uc[m][i][d] = conj(u_{m,i,d}) of uvec_LT
+ * (src/pmx_basis.cpp:830-874), wA[m][i][p] = conj(u_mi).(G_p + Q/2 + G_m/2)
+ * and, with SOC, wS[m][i][p] = conj(u_mi).socCurrent(G_p, G_loci) */
+static void q_tables(const pmx_ctx *c, double Qx, double Qy, double Qz, bool spatial, std::vector<zd> &uc,
+                     std::vector<zd> &wA, std::vector<zd> &wS) {
+    const int neps = c->neps, npw = c->npw;
+    uc.assign((size_t)neps * 9, zd{0, 0});
+    wA.assign((size_t)neps * 3 * npw, zd{0, 0});
+    wS.clear();
+    if (!spatial) wS.assign((size_t)neps * 3 * npw, zd{0, 0});
+    for (int m = 0; m < neps; m++) {
+        const double *Gm = &c->G[3 * m];
+        double vx = Qx + Gm[0], vy = Qy + Gm[1], vz = Qz + Gm[2];
+        double nv = std::sqrt(vx * vx + vy * vy + vz * vz);
+        double L[3] = {0, 0, 1};
+        if (nv != 0) { L[0] = vx / nv; L[1] = vy / nv; L[2] = vz / nv; }
+        double T[3];
+        if (vy != 0) { T[0] = -vy; T[1] = vx; T[2] = 0; }
+        else if (vz != 0) { T[0] = vz; T[1] = 0; T[2] = -vx; }
+        else { T[0] = 0; T[1] = 1; T[2] = 0; }
+        double nt = std::sqrt(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]);
+        for (double &x : T) x /= nt;
+        double lt = L[0] * T[0] + L[1] * T[1] + L[2] * T[2];
+        for (int d = 0; d < 3; d++) T[d] -= lt * L[d];
+        nt = std::sqrt(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]);
+        for (double &x : T) x /= nt;
+        double P[3] = {L[1] * T[2] - L[2] * T[1], L[2] * T[0] - L[0] * T[2], L[0] * T[1] - L[1] * T[0]};
+        double np_ = std::sqrt(P[0] * P[0] + P[1] * P[1] + P[2] * P[2]);
+        for (double &x : P) x /= np_;
+        const double r2 = std::sqrt(2.0);
+        std::complex<double> u[3][3];
+        for (int d = 0; d < 3; d++) {
+            u[0][d] = L[d];
+            u[1][d] = std::complex<double>(T[d], P[d]) / r2;
+            u[2][d] = std::complex<double>(T[d], -P[d]) / r2;
+        }
+        for (int i = 0; i < 3; i++)
+            for (int d = 0; d < 3; d++) uc[((size_t)m * 3 + i) * 3 + d] = zd{u[i][d].real(), -u[i][d].imag()};
+        for (int pp = 0; pp < npw; pp++) {
+            const double *Gp = &c->G[3 * pp];
+            double o[3] = {Gp[0] + 0.5 * Qx + 0.5 * Gm[0], Gp[1] + 0.5 * Qy + 0.5 * Gm[1], Gp[2] + 0.5 * Qz + 0.5 * Gm[2]};
+            for (int i = 0; i < 3; i++) {
+                std::complex<double> acc = 0.0;
+                for (int d = 0; d < 3; d++) acc += std::conj(u[i][d]) * o[d];
+                wA[((size_t)m * 3 + i) * npw + pp] = zd{acc.real(), acc.imag()};
+            }
+            if (!spatial) {
+                int l = c->loci_h[(size_t)m * npw + pp];
+                if (l < 0) continue;
+                /* socCurrent(G_p, G_l) with the map lambda_S (src/pmx_chi.cpp:203-266) */
+                const double *Gl = &c->G[3 * l];
+                double qx = Gl[0] - Gp[0], qy = Gl[1] - Gp[1], qz = Gl[2] - Gp[2];
+                std::complex<double> vso = c->vso_chi_at(qx, qy, qz);
+                std::complex<double> I(0, 1);
+                std::complex<double> sv[3] = {I * vso * (qy + I * qz), I * vso * (qz - qx), I * vso * (-I * qx - qy)};
+                for (int i = 0; i < 3; i++) {
+                    std::complex<double> acc = 0.0;
+                    for (int d = 0; d < 3; d++) acc += std::conj(u[i][d]) * sv[d];
+                    wS[((size_t)m * 3 + i) * npw + pp] = zd{acc.real(), acc.imag()};
+                }
+            }
+        }
+    }
+}
+
 static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm) {
     if (!c || !p || !dRe || !dIm) return set_error(PMX_EINVAL, "null pointer");
     if (!c->has_system) return set_error(PMX_ESTATE, "no system set");
@@ -1251,8 +1322,7 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
     const int N = basis_size(c);
     const int nb_s = spatial ? p->nband / 2 : p->nband;
     const int nch = spatial ? 1 : 4;
-    const double SF = 4.0 * c->bz * std::pow(PMX_PI, 5) * std::pow(PMX_HBAR, 4) * std::pow(PMX_E, 2) /
-                      (std::pow(PMX_EV, 3) * std::pow(PMX_EMASS, 2) * std::pow(c->a * PMX_ANGSTROM, 5));
+    const double SF = chi_scale(c);
     const double w0 = 0.0, df = p->dfreq;
     const int wh = window_half(p->delta, p->epsilon, df);
     const int nbins = p->nfreq + 2 * wh;
@@ -1310,61 +1380,8 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
     for (int q = 0; q < p->nq; q++) {
         const double Qx = p->Q[3 * q], Qy = p->Q[3 * q + 1], Qz = p->Q[3 * q + 2];
         /* ---- per-q tables: uvec_LT (src/pmx_basis.cpp:830-874), weights */
-        std::vector<zd> uc((size_t)neps * 9), wA((size_t)neps * 3 * npw), wS;
-        if (!spatial) wS.assign((size_t)neps * 3 * npw, zd{0, 0});
-        for (int m = 0; m < neps; m++) {
-            const double *Gm = &c->G[3 * m];
-            double vx = Qx + Gm[0], vy = Qy + Gm[1], vz = Qz + Gm[2];
-            double nv = std::sqrt(vx * vx + vy * vy + vz * vz);
-            double L[3] = {0, 0, 1};
-            if (nv != 0) { L[0] = vx / nv; L[1] = vy / nv; L[2] = vz / nv; }
-            double T[3];
-            if (vy != 0) { T[0] = -vy; T[1] = vx; T[2] = 0; }
-            else if (vz != 0) { T[0] = vz; T[1] = 0; T[2] = -vx; }
-            else { T[0] = 0; T[1] = 1; T[2] = 0; }
-            double nt = std::sqrt(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]);
-            for (double &x : T) x /= nt;
-            double lt = L[0] * T[0] + L[1] * T[1] + L[2] * T[2];
-            for (int d = 0; d < 3; d++) T[d] -= lt * L[d];
-            nt = std::sqrt(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]);
-            for (double &x : T) x /= nt;
-            double P[3] = {L[1] * T[2] - L[2] * T[1], L[2] * T[0] - L[0] * T[2], L[0] * T[1] - L[1] * T[0]};
-            double np_ = std::sqrt(P[0] * P[0] + P[1] * P[1] + P[2] * P[2]);
-            for (double &x : P) x /= np_;
-            const double r2 = std::sqrt(2.0);
-            std::complex<double> u[3][3];
-            for (int d = 0; d < 3; d++) {
-                u[0][d] = L[d];
-                u[1][d] = std::complex<double>(T[d], P[d]) / r2;
-                u[2][d] = std::complex<double>(T[d], -P[d]) / r2;
-            }
-            for (int i = 0; i < 3; i++)
-                for (int d = 0; d < 3; d++) uc[((size_t)m * 3 + i) * 3 + d] = zd{u[i][d].real(), -u[i][d].imag()};
-            for (int pp = 0; pp < npw; pp++) {
-                const double *Gp = &c->G[3 * pp];
-                double o[3] = {Gp[0] + 0.5 * Qx + 0.5 * Gm[0], Gp[1] + 0.5 * Qy + 0.5 * Gm[1], Gp[2] + 0.5 * Qz + 0.5 * Gm[2]};
-                for (int i = 0; i < 3; i++) {
-                    std::complex<double> acc = 0.0;
-                    for (int d = 0; d < 3; d++) acc += std::conj(u[i][d]) * o[d];
-                    wA[((size_t)m * 3 + i) * npw + pp] = zd{acc.real(), acc.imag()};
-                }
-                if (!spatial) {
-                    int l = c->loci_h[(size_t)m * npw + pp];
-                    if (l < 0) continue;
-                    /* socCurrent(G_p, G_l) with the map lambda_S (src/pmx_chi.cpp:203-266) */
-                    const double *Gl = &c->G[3 * l];
-                    double qx = Gl[0] - Gp[0], qy = Gl[1] - Gp[1], qz = Gl[2] - Gp[2];
-                    std::complex<double> vso = c->vso_chi_at(qx, qy, qz);
-                    std::complex<double> I(0, 1);
-                    std::complex<double> sv[3] = {I * vso * (qy + I * qz), I * vso * (qz - qx), I * vso * (-I * qx - qy)};
-                    for (int i = 0; i < 3; i++) {
-                        std::complex<double> acc = 0.0;
-                        for (int d = 0; d < 3; d++) acc += std::conj(u[i][d]) * sv[d];
-                        wS[((size_t)m * 3 + i) * npw + pp] = zd{acc.real(), acc.imag()};
-                    }
-                }
-            }
-        }
+        std::vector<zd> uc, wA, wS;
+        q_tables(c, Qx, Qy, Qz, spatial, uc, wA, wS);
         if ((rc = upload(st, w.ucd, uc)) || (rc = upload(st, w.wAd, wA))) return rc;
         if (!spatial && (rc = upload(st, w.wSd, wS))) return rc;
 
@@ -1690,6 +1707,450 @@ extern "C" int pmx_grad_bands(pmx_ctx *c, int nk, const double *K, int nband, co
                            N, c->natom, c->nv, nb_s, c->soc ? 1 : 0, vec.p, c->dShell.p, c->dSfac.p, gwd.p, out.p);
         PMX_CHECK_HIP(hipGetLastError());
     }
+    PMX_CHECK_HIP(hipMemcpyAsync(gvg, out.p, sizeof(double) * c->natom * c->nv, hipMemcpyDeviceToHost, st));
+    PMX_CHECK_HIP(hipStreamSynchronize(st));
+    return PMX_OK;
+}
+
+/* ==================================================================
+ * Adjoint of chi with respect to the form factors (no reference
+ * counterpart; for fitting vg to measured spectra).  Spatial solve, FP64,
+ * Gaussian or Lorentzian broadening: the gradient of the exact windowed sum
+ * (src/pmx_chi.cpp:573-634) that the FP64 and direct paths compute.
+ *
+ * Forward, per q and element e = (ij, m, n<=m), a = 3m+i, b = 3n+j:
+ *   Z_e(f) = sum_t w_t delta(dE_t - w_f) X_ta conj(X_tb),
+ *   Im0 = SF/w_f^2 Re Z,  Re0 = -SF/w_f^2 Im Z,  then KK (linear in Re0, Im0).
+ * With gz = dL/dRe z + i dL/dIm z for a complex z (dL = Re conj(gz) dz):
+ *   (1) KK^T: the same GEMM with the Kramers-Kronig matrix transposed;
+ *   (2) gZ_e(f) = SF/w_f^2 (gIm0 - i gRe0);
+ *   (3) per transition, Gam_e = w sum_f delta(x_f) gZ_e(f) and Gam'_e with
+ *       delta' (x_f = dE - w_f, window as the forward):
+ *       g_dE = Re sum_e conj(Gam'_e) X_a conj(X_b),
+ *       gX_a += Gam_e X_b,  gX_b += conj(Gam_e) X_a;
+ *   (4) X_ta = sum_p conj(Ck_c[p]) Cq_v[l_m(p)] W_mi(p), W = wA + conj(u_mi).K:
+ *       gCk_c[p] += conj(gX_a) Cq_v[l_m(p)] W_mi(p),
+ *       gCq_v[l] += gX_a Ck_c[p] conj(W_mi(p)) with p = l_m^{-1}(l),
+ *       gE_c(k) += g_dE,  gE_v(k+q) -= g_dE;
+ *   (5) eigenvector adjoint (first-order perturbation theory; pairs closer
+ *       than tol are masked, the phase gauge drops out of chi):
+ *       y_n = sum_{m, |E_m - E_n| > tol} psi_m (psi_m^H g_n)/(E_n - E_m) + gE_n psi_n,
+ *       so that dL = Re sum_n y_n^H dH psi_n;
+ *   (6) dH_ij/dvg[s][sh] (i > j, shell_ij = sh) = Ry2eV sfac_ijs / NATOM and
+ *       dH_ji its conjugate (src/pmx_epm.cpp:171-203), hence
+ *       dL/dvg[s][sh] = Ry2eV/NATOM Re sum_{i>j} sfac_ijs (conj(y_i) psi_j + conj(psi_i) y_j).
+ * The wave vectors at k serve every q: their gradients accumulate over q and
+ * (5)-(6) run once for them at the end.
+ * ================================================================== */
+namespace {
+
+__device__ double ddelta_fn(int code, double eps, double x) {
+    if (code == 0) return -x / (eps * eps) * delta_fn(0, eps, x);
+    const double d = eps * eps + x * x;
+    return -(eps / PMX_PI) * 2.0 * x / (d * d);
+}
+
+struct AdjTArgs {
+    int D, nact, nfreq, wh, delta;
+    double eps, w0, df;
+    const zd *X;         /* [ntr][D] rho in transition order */
+    const double *dE, *wt;
+    const int *act;      /* [nact][2]: rows a, b of the active elements */
+    const zd *gZ;        /* [nact][nfreq] */
+    zd *gX;              /* [ntr][D] */
+    double *gdE;         /* [ntr] */
+};
+
+/* step (3): one block per transition; threads over the active elements */
+__global__ void __launch_bounds__(256) chi_adj_trans_kernel(AdjTArgs a) {
+    const int t = blockIdx.x;
+    extern __shared__ double adj_sm[];
+    zd *xs = reinterpret_cast<zd *>(adj_sm); /* [D] */
+    double *gr = adj_sm + 2 * a.D, *gi = gr + a.D;
+    __shared__ double red[256];
+    for (int i = threadIdx.x; i < a.D; i += 256) {
+        xs[i] = a.X[(size_t)t * a.D + i];
+        gr[i] = 0.0;
+        gi[i] = 0.0;
+    }
+    __syncthreads();
+    const double dE = a.dE[t], w = a.wt[t];
+    const int f0 = int((dE - a.w0) / a.df);
+    const int fmin = max(0, f0 - a.wh), fmax = min(a.nfreq - 1, f0 + a.wh);
+    double gde = 0.0;
+    for (int e = threadIdx.x; e < a.nact && fmin <= fmax; e += 256) {
+        const int ea = a.act[2 * e], eb = a.act[2 * e + 1];
+        const zd *G = a.gZ + (size_t)e * a.nfreq;
+        zd g0{0, 0}, g1{0, 0};
+        for (int f = fmin; f <= fmax; f++) {
+            const double x = dE - (a.w0 + a.df * f);
+            const double d0 = delta_fn(a.delta, a.eps, x), d1 = ddelta_fn(a.delta, a.eps, x);
+            const zd gz = G[f];
+            g0.x += d0 * gz.x; g0.y += d0 * gz.y;
+            g1.x += d1 * gz.x; g1.y += d1 * gz.y;
+        }
+        g0.x *= w; g0.y *= w;
+        g1.x *= w; g1.y *= w;
+        const zd xa = xs[ea], xb = xs[eb];
+        const zd O = cmulc(xa, xb);
+        gde += g1.x * O.x + g1.y * O.y;
+        const zd ga = cmul(g0, xb), gb = cmulc(xa, g0);
+        atomicAdd(&gr[ea], ga.x);
+        atomicAdd(&gi[ea], ga.y);
+        atomicAdd(&gr[eb], gb.x);
+        atomicAdd(&gi[eb], gb.y);
+    }
+    red[threadIdx.x] = gde;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < a.D; i += 256) a.gX[(size_t)t * a.D + i] = zd{gr[i], gi[i]};
+    if (threadIdx.x == 0) a.gdE[t] = red[0];
+}
+
+struct AdjVArgs {
+    int N, npw, neps, nband, D, nbs;
+    const int *nc, *nv, *tbase; /* spinor counts, transition base per k */
+    const zd *vk, *vkq;         /* [nk][N][N] all states */
+    const int *loci, *invl;     /* [neps][npw] and its inverse per m */
+    const zd *wA, *uc;
+    const double *K;
+    const zd *gX;
+    const double *gdE;
+    zd *gk, *gkq;               /* [nk][nbs][N], accumulated */
+    double *gEk, *gEkq;         /* [nk][nbs], accumulated */
+};
+
+/* step (4): one block per k (it owns row k of every output) */
+__global__ void __launch_bounds__(256) chi_adj_evec_kernel(AdjVArgs a) {
+    const int k = blockIdx.x;
+    const int ncs = a.nc[k] / 2, nvs = a.nv[k] / 2;
+    if (ncs * nvs == 0) return;
+    extern __shared__ double adjv_sm[];
+    zd *bk = reinterpret_cast<zd *>(adjv_sm); /* [neps*3] conj(u_mi).K */
+    const double Kx = a.K[3 * k], Ky = a.K[3 * k + 1], Kz = a.K[3 * k + 2];
+    for (int mi = threadIdx.x; mi < 3 * a.neps; mi += 256) {
+        const zd *u = a.uc + (size_t)mi * 3;
+        bk[mi] = zd{u[0].x * Kx + u[1].x * Ky + u[2].x * Kz, u[0].y * Kx + u[1].y * Ky + u[2].y * Kz};
+    }
+    __syncthreads();
+    const size_t tb = a.tbase[k];
+    const zd *vk = a.vk + (size_t)k * a.N * a.N, *vkq = a.vkq + (size_t)k * a.N * a.N;
+    for (int idx = threadIdx.x; idx < ncs * a.npw; idx += 256) {
+        const int c = idx / a.npw, p = idx % a.npw;
+        const int cst = a.nband / 2 - 1 - c;
+        zd s{0, 0};
+        for (int v = 0; v < nvs; v++) {
+            const zd *gx = a.gX + (tb + (size_t)c * nvs + v) * a.D;
+            const zd *cq = vkq + (size_t)v * a.N;
+            for (int m = 0; m < a.neps; m++) {
+                const int l = a.loci[(size_t)m * a.npw + p];
+                if (l < 0) continue;
+                const zd q = cq[l];
+                for (int i = 0; i < 3; i++) {
+                    const zd wa = a.wA[((size_t)m * 3 + i) * a.npw + p], b = bk[3 * m + i];
+                    const zd W{wa.x + b.x, wa.y + b.y};
+                    const zd r = cmul(cconj(gx[3 * m + i]), cmul(q, W));
+                    s.x += r.x; s.y += r.y;
+                }
+            }
+        }
+        zd &o = a.gk[((size_t)k * a.nbs + cst) * a.N + p];
+        o.x += s.x; o.y += s.y;
+    }
+    for (int idx = threadIdx.x; idx < nvs * a.npw; idx += 256) {
+        const int v = idx / a.npw, l = idx % a.npw;
+        zd s{0, 0};
+        for (int c = 0; c < ncs; c++) {
+            const int cst = a.nband / 2 - 1 - c;
+            const zd *gx = a.gX + (tb + (size_t)c * nvs + v) * a.D;
+            const zd *ck = vk + (size_t)cst * a.N;
+            for (int m = 0; m < a.neps; m++) {
+                const int p = a.invl[(size_t)m * a.npw + l];
+                if (p < 0) continue;
+                const zd cp = ck[p];
+                for (int i = 0; i < 3; i++) {
+                    const zd wa = a.wA[((size_t)m * 3 + i) * a.npw + p], b = bk[3 * m + i];
+                    const zd Wc{wa.x + b.x, -(wa.y + b.y)};
+                    const zd r = cmul(gx[3 * m + i], cmul(cp, Wc));
+                    s.x += r.x; s.y += r.y;
+                }
+            }
+        }
+        zd &o = a.gkq[((size_t)k * a.nbs + v) * a.N + l];
+        o.x += s.x; o.y += s.y;
+    }
+    for (int c = threadIdx.x; c < ncs; c += 256) {
+        double s = 0.0;
+        for (int v = 0; v < nvs; v++) s += a.gdE[tb + (size_t)c * nvs + v];
+        a.gEk[(size_t)k * a.nbs + a.nband / 2 - 1 - c] += s;
+    }
+    for (int v = threadIdx.x; v < nvs; v += 256) {
+        double s = 0.0;
+        for (int c = 0; c < ncs; c++) s += a.gdE[tb + (size_t)c * nvs + v];
+        a.gEkq[(size_t)k * a.nbs + v] -= s;
+    }
+}
+
+/* step (5): one block per (k, state n < nbs); psi [nk][N][N] state rows */
+__global__ void __launch_bounds__(256) adj_y_kernel(int N, int nbs, double tol, const zd *psi, const double *ev,
+                                                    const zd *g, const double *gE, zd *y, int *live) {
+    const int k = blockIdx.x / nbs, n = blockIdx.x % nbs;
+    extern __shared__ double ysm[];
+    zd *coef = reinterpret_cast<zd *>(ysm); /* [N] */
+    const zd *P = psi + (size_t)k * N * N;
+    const zd *gn = g + ((size_t)k * nbs + n) * N;
+    const double En = ev[(size_t)k * N + n], gEn = gE[(size_t)k * nbs + n];
+    int nz = gEn != 0.0;
+    for (int p = threadIdx.x; p < N; p += 256) nz |= (gn[p].x != 0.0 || gn[p].y != 0.0);
+    nz = __syncthreads_or(nz);
+    if (!nz) {
+        if (threadIdx.x == 0) live[blockIdx.x] = 0;
+        return;
+    }
+    for (int m = threadIdx.x; m < N; m += 256) {
+        zd cf{0, 0};
+        if (m == n) {
+            cf.x = gEn;
+        } else {
+            const double d = En - ev[(size_t)k * N + m];
+            if (fabs(d) > tol) {
+                zd s{0, 0};
+                const zd *pm = P + (size_t)m * N;
+                for (int p = 0; p < N; p++) {
+                    const zd r = cmul(cconj(pm[p]), gn[p]);
+                    s.x += r.x; s.y += r.y;
+                }
+                cf = zd{s.x / d, s.y / d};
+            }
+        }
+        coef[m] = cf;
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < N; p += 256) {
+        zd s{0, 0};
+        for (int m = 0; m < N; m++) {
+            const zd r = cmul(P[(size_t)m * N + p], coef[m]);
+            s.x += r.x; s.y += r.y;
+        }
+        y[((size_t)k * nbs + n) * N + p] = s;
+    }
+    if (threadIdx.x == 0) live[blockIdx.x] = 1;
+}
+
+/* step (6): one block per (k, n); the adjoint form of hf_grad_kernel */
+__global__ void __launch_bounds__(256) hf_adj_kernel(int npw, int N, int natom, int nv, int nbs, const zd *psi,
+                                                     const zd *y, const int *live, const int *shell,
+                                                     const zd *sfac, double *gvg) {
+    if (!live[blockIdx.x]) return;
+    extern __shared__ double acc[]; /* [natom * nv] */
+    const int k = blockIdx.x / nbs, n = blockIdx.x % nbs;
+    for (int i = threadIdx.x; i < natom * nv; i += blockDim.x) acc[i] = 0.0;
+    __syncthreads();
+    const zd *ps = psi + ((size_t)k * N + n) * N;
+    const zd *yn = y + ((size_t)k * nbs + n) * N;
+    const size_t npair = (size_t)npw * npw;
+    for (size_t id = threadIdx.x; id < npair; id += blockDim.x) {
+        const int i = id / npw, j = id % npw;
+        if (i <= j) continue;
+        const int sh = shell[id];
+        if (sh < 0) continue;
+        const zd a = cmul(cconj(yn[i]), ps[j]), b = cmul(cconj(ps[i]), yn[j]);
+        const zd pr{a.x + b.x, a.y + b.y};
+        for (int sa = 0; sa < natom; sa++) {
+            const zd f = sfac[id * natom + sa];
+            atomicAdd(&acc[sa * nv + sh], pr.x * f.x - pr.y * f.y);
+        }
+    }
+    __syncthreads();
+    const double scale = PMX_RY2EV / natom;
+    for (int i = threadIdx.x; i < natom * nv; i += blockDim.x)
+        if (acc[i] != 0.0) atomicAdd(&gvg[i], scale * acc[i]);
+}
+
+/* step (2): gZ for the active elements from the compact gRe0/gIm0 [row][f] */
+__global__ void adj_gz_kernel(int nact, int nfreq, double SF, double w0, double df, const int *arow,
+                              const double *gRe0, const double *gIm0, zd *gZ) {
+    const size_t total = (size_t)nact * nfreq;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int e = idx / nfreq, f = idx % nfreq;
+        const size_t src = (size_t)arow[e] * nfreq + f;
+        const double w = w0 + df * f;
+        const double inv = (f == 0) ? 0.0 : 1.0 / (w * w);
+        gZ[idx] = zd{SF * inv * gIm0[src], -SF * inv * gRe0[src]};
+    }
+}
+
+} // namespace
+
+extern "C" int pmx_grad_chi(pmx_ctx *c, const pmx_chi_params *p, const double *gRe, const double *gIm, double *gvg) {
+    if (!c || !p || !gRe || !gIm || !gvg) return set_error(PMX_EINVAL, "null pointer");
+    if (!c->has_system) return set_error(PMX_ESTATE, "no system set");
+    if (c->soc) return set_error(PMX_EINVAL, "pmx_grad_chi: the spin-orbit solve has no adjoint here");
+    if (p->delta < 0 || p->delta > 1)
+        return set_error(PMX_EINVAL, "pmx_grad_chi: Gaussian (0) or Lorentzian (1) broadening only");
+    if (p->nband <= 0 || p->nband % 2 || p->nband > 2 * c->npw)
+        return set_error(PMX_EINVAL, "pmx_grad_chi: nband must be even and in [2, 2 npw]");
+    if (p->nfreq <= 0 || !(p->dfreq > 0) || p->nk < 0 || p->nq < 0) return set_error(PMX_EINVAL, "bad sizes");
+    PMX_CHECK_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int neps = c->neps, npw = c->npw, D = 3 * neps, N = npw;
+    const int ne2 = neps * (neps + 1) / 2, nel = 9 * ne2, nfreq = p->nfreq, nk = p->nk;
+    const int nbs = p->nband / 2;
+    const double SF = chi_scale(c), w0 = 0.0, df = p->dfreq;
+    const int wh = window_half(p->delta, p->epsilon, df);
+    const double tol = 1e-8; /* eV: pairs closer than this count as degenerate */
+    std::fill(gvg, gvg + (size_t)c->natom * c->nv, 0.0);
+    if (nk == 0 || p->nq == 0) return PMX_OK;
+    if (sizeof(double) * 4 * (size_t)D > c->lds_max) return set_error(PMX_EINVAL, "NEPS too large for the adjoint");
+
+    /* inverse loci: invl[m][l] = p with loci[m][p] = l (G_p -> G_p + G_m is one to one) */
+    std::vector<int> invl((size_t)neps * npw, -1);
+    for (int m = 0; m < neps; m++)
+        for (int q = 0; q < npw; q++) {
+            const int l = c->loci_h[(size_t)m * npw + q];
+            if (l >= 0) invl[(size_t)m * npw + l] = q;
+        }
+    std::vector<double> Kh(p->K, p->K + 3 * (size_t)nk), KWh(p->KW, p->KW + nk);
+    DBuf<double> Kd, KQd, KWd, evk, evkq, dEd, wtd, gdE, gEk, gEkq, gr, gi, gr0, gi0, KkT, out;
+    DBuf<zd> vk, vkq, X, gX, gk, gkq, Y, ucd, wAd, gZ;
+    DBuf<int> invd, nc, nv, tbase, pos, bad, act, arow, live;
+    int rc;
+    if ((rc = upload(st, Kd, Kh)) || (rc = upload(st, KWd, KWh)) || (rc = upload(st, invd, invl))) return rc;
+    if ((rc = evk.ensure((size_t)nk * N)) || (rc = vk.ensure((size_t)nk * N * N)) || (rc = evkq.ensure((size_t)nk * N)) ||
+        (rc = vkq.ensure((size_t)nk * N * N)) || (rc = gk.ensure((size_t)nk * nbs * N)) ||
+        (rc = gkq.ensure((size_t)nk * nbs * N)) || (rc = Y.ensure((size_t)nk * nbs * N)) ||
+        (rc = gEk.ensure((size_t)nk * nbs)) || (rc = gEkq.ensure((size_t)nk * nbs)) ||
+        (rc = live.ensure((size_t)nk * nbs)) || (rc = nc.ensure(nk)) || (rc = nv.ensure(nk)) ||
+        (rc = tbase.ensure(nk + 1)) || (rc = bad.ensure(1)) || (rc = out.ensure((size_t)c->natom * c->nv)) ||
+        (rc = gr.ensure((size_t)nel * nfreq)) || (rc = gi.ensure((size_t)nel * nfreq)))
+        return rc;
+    PMX_CHECK_HIP(hipMemsetAsync(out.p, 0, sizeof(double) * c->natom * c->nv, st));
+    PMX_CHECK_HIP(hipMemsetAsync(gk.p, 0, sizeof(zd) * (size_t)nk * nbs * N, st));
+    PMX_CHECK_HIP(hipMemsetAsync(gEk.p, 0, sizeof(double) * (size_t)nk * nbs, st));
+    /* all N states at k: the perturbation sum runs over the whole basis */
+    if ((rc = eig_device(c, nk, Kd.p, N, N, evk.p, vk.p))) return rc;
+    if (p->kk) {
+        /* KkT[f][s] = Kk[s][f] of kkmat_kernel */
+        std::vector<double> kt((size_t)nfreq * nfreq);
+        const double ds = df * 2.0 / PMX_PI;
+        for (int s = 0; s < nfreq; s++)
+            for (int f = 0; f < nfreq; f++) {
+                const double ws = df * s, wf = df * f;
+                kt[(size_t)f * nfreq + s] = (s == f) ? 0.0 : ds * (ws / (ws * ws - wf * wf));
+            }
+        if ((rc = upload(st, KkT, kt)) || (rc = gr0.ensure((size_t)nel * nfreq)) || (rc = gi0.ensure((size_t)nel * nfreq)))
+            return rc;
+    }
+    const size_t lds_y = sizeof(zd) * (size_t)N;
+    const size_t lds_hf = sizeof(double) * c->natom * c->nv;
+    std::vector<double> cre((size_t)nel * nfreq), cim((size_t)nel * nfreq);
+    for (int q = 0; q < p->nq; q++) {
+        /* upstream gradient of this q, compact [ij][e2][f], and its active elements */
+        std::vector<int> rows, pairs;
+        for (int r = 0; r < nel; r++) {
+            const int ij = r / ne2, e2 = r % ne2;
+            const size_t src = (((size_t)ij * p->nq + q) * ne2 + e2) * nfreq;
+            bool any = false;
+            for (int f = 0; f < nfreq; f++) {
+                cre[(size_t)r * nfreq + f] = gRe[src + f];
+                cim[(size_t)r * nfreq + f] = gIm[src + f];
+                any |= gRe[src + f] != 0.0 || gIm[src + f] != 0.0;
+            }
+            if (!any) continue;
+            int m = 0;
+            while ((m + 1) * (m + 2) / 2 <= e2) m++;
+            const int n = e2 - m * (m + 1) / 2;
+            rows.push_back(r);
+            pairs.push_back(3 * m + ij / 3);
+            pairs.push_back(3 * n + ij % 3);
+        }
+        const int nact = (int)rows.size();
+        if (nact == 0) continue;
+        if ((rc = upload(st, arow, rows)) || (rc = upload(st, act, pairs)) || (rc = gZ.ensure((size_t)nact * nfreq)))
+            return rc;
+        PMX_CHECK_HIP(hipMemcpyAsync(gr.p, cre.data(), sizeof(double) * cre.size(), hipMemcpyHostToDevice, st));
+        PMX_CHECK_HIP(hipMemcpyAsync(gi.p, cim.data(), sizeof(double) * cim.size(), hipMemcpyHostToDevice, st));
+        const double *g_re = gr.p, *g_im = gi.p;
+        if (p->kk) {
+            /* (1) the transposed KK matrix in the forward's GEMM: compact in, compact out */
+            const dim3 grid(8 * (((nel + 63) / 64 + 7) / 8) * ((nfreq + 16 * KK_CT - 1) / (16 * KK_CT)));
+            hipLaunchKernelGGL(kk_gemm_kernel<double>, grid, dim3(256), 0, st, nel, nfreq, gr.p, gi.p, KkT.p, gr0.p,
+                               gi0.p, ne2, 1, 0);
+            g_re = gr0.p;
+            g_im = gi0.p;
+        }
+        hipLaunchKernelGGL(adj_gz_kernel, dim3(256), dim3(256), 0, st, nact, nfreq, SF, w0, df, arow.p, g_re, g_im, gZ.p);
+
+        /* forward quantities of this q: eigenpairs at k+q, transitions, rho in transition order */
+        const double Qx = p->Q[3 * q], Qy = p->Q[3 * q + 1], Qz = p->Q[3 * q + 2];
+        std::vector<zd> uc, wA, wS;
+        q_tables(c, Qx, Qy, Qz, true, uc, wA, wS);
+        if ((rc = upload(st, ucd, uc)) || (rc = upload(st, wAd, wA))) return rc;
+        std::vector<double> KQ(3 * (size_t)nk);
+        for (int k = 0; k < nk; k++) {
+            KQ[3 * k] = Kh[3 * k] + Qx;
+            KQ[3 * k + 1] = Kh[3 * k + 1] + Qy;
+            KQ[3 * k + 2] = Kh[3 * k + 2] + Qz;
+        }
+        if ((rc = upload(st, KQd, KQ))) return rc;
+        if ((rc = eig_device(c, nk, KQd.p, N, N, evkq.p, vkq.p))) return rc;
+        PMX_CHECK_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), st));
+        TransArgs ta{nk, N, p->nband, 1, evk.p, evkq.p, p->offset, nc.p, nv.p, bad.p};
+        hipLaunchKernelGGL(classify_kernel, dim3((nk + 255) / 256), dim3(256), 0, st, ta);
+        std::vector<int> nch_h(nk), nvh(nk), tb(nk + 1);
+        int badh = 0;
+        PMX_CHECK_HIP(hipMemcpyAsync(nch_h.data(), nc.p, sizeof(int) * nk, hipMemcpyDeviceToHost, st));
+        PMX_CHECK_HIP(hipMemcpyAsync(nvh.data(), nv.p, sizeof(int) * nk, hipMemcpyDeviceToHost, st));
+        PMX_CHECK_HIP(hipMemcpyAsync(&badh, bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        PMX_CHECK_HIP(hipStreamSynchronize(st));
+        if (badh) return set_error(PMX_EBANDS, "at some k no level at or below the energy zero lies within -nband, "
+                                               "or at some k+q none above it");
+        tb[0] = 0;
+        for (int k = 0; k < nk; k++) tb[k + 1] = tb[k] + (nch_h[k] / 2) * (nvh[k] / 2);
+        const int ntr = tb[nk];
+        if (ntr == 0) continue;
+        std::vector<int> iota(ntr);
+        for (int t = 0; t < ntr; t++) iota[t] = t;
+        if ((rc = upload(st, tbase, tb)) || (rc = upload(st, pos, iota)) || (rc = dEd.ensure(ntr)) ||
+            (rc = wtd.ensure(ntr)) || (rc = gdE.ensure(ntr)) || (rc = X.ensure((size_t)ntr * D)) ||
+            (rc = gX.ensure((size_t)ntr * D)))
+            return rc;
+        hipLaunchKernelGGL(dE_kernel, dim3(nk), dim3(256), 0, st, nk, N, p->nband, 1, evk.p, evkq.p, p->offset, nc.p,
+                           nv.p, tbase.p, KWd.p, dEd.p, wtd.p);
+        RhoArgs ra{};
+        ra.nk = nk; ra.N = N; ra.npw = npw; ra.neps = neps; ra.nb_s = N; ra.nband = p->nband;
+        ra.spatial = 1; ra.nch = 1;
+        ra.vk = vk.p; ra.vkq = vkq.p; ra.nc = nc.p; ra.nv = nv.p; ra.tbase = tbase.p; ra.pos = pos.p;
+        ra.loci = c->dLoci.p; ra.wA = wAd.p; ra.uc = ucd.p; ra.K = Kd.p; ra.X = X.p;
+        hipLaunchKernelGGL(rho_kernel, dim3(nk), dim3(256), 0, st, ra);
+
+        /* (3) per transition */
+        AdjTArgs at{D, nact, nfreq, wh, p->delta, p->epsilon, w0, df, X.p, dEd.p, wtd.p, act.p, gZ.p, gX.p, gdE.p};
+        hipLaunchKernelGGL(chi_adj_trans_kernel, dim3(ntr), dim3(256), sizeof(double) * 4 * D, st, at);
+        /* (4) onto the wave vectors */
+        PMX_CHECK_HIP(hipMemsetAsync(gkq.p, 0, sizeof(zd) * (size_t)nk * nbs * N, st));
+        PMX_CHECK_HIP(hipMemsetAsync(gEkq.p, 0, sizeof(double) * (size_t)nk * nbs, st));
+        AdjVArgs av{N, npw, neps, p->nband, D, nbs, nc.p, nv.p, tbase.p, vk.p, vkq.p, c->dLoci.p, invd.p,
+                    wAd.p, ucd.p, Kd.p, gX.p, gdE.p, gk.p, gkq.p, gEk.p, gEkq.p};
+        hipLaunchKernelGGL(chi_adj_evec_kernel, dim3(nk), dim3(256), sizeof(zd) * 3 * neps, st, av);
+        /* (5)-(6) for the states at k+q */
+        hipLaunchKernelGGL(adj_y_kernel, dim3(nk * nbs), dim3(256), lds_y, st, N, nbs, tol, vkq.p, evkq.p, gkq.p,
+                           gEkq.p, Y.p, live.p);
+        hipLaunchKernelGGL(hf_adj_kernel, dim3(nk * nbs), dim3(256), lds_hf, st, npw, N, c->natom, c->nv, nbs, vkq.p,
+                           Y.p, live.p, c->dShell.p, c->dSfac.p, out.p);
+        PMX_CHECK_HIP(hipGetLastError());
+        /* host buffers above are reused next q */
+        PMX_CHECK_HIP(hipStreamSynchronize(st));
+    }
+    /* (5)-(6) for the states at k, accumulated over q */
+    hipLaunchKernelGGL(adj_y_kernel, dim3(nk * nbs), dim3(256), lds_y, st, N, nbs, tol, vk.p, evk.p, gk.p, gEk.p, Y.p,
+                       live.p);
+    hipLaunchKernelGGL(hf_adj_kernel, dim3(nk * nbs), dim3(256), lds_hf, st, npw, N, c->natom, c->nv, nbs, vk.p, Y.p,
+                       live.p, c->dShell.p, c->dSfac.p, out.p);
+    PMX_CHECK_HIP(hipGetLastError());
     PMX_CHECK_HIP(hipMemcpyAsync(gvg, out.p, sizeof(double) * c->natom * c->nv, hipMemcpyDeviceToHost, st));
     PMX_CHECK_HIP(hipStreamSynchronize(st));
     return PMX_OK;

@@ -1,0 +1,129 @@
+"""Adjoint of chi (pmx_grad_chi through the C-ABI and the
+torch.autograd.Function) against central finite differences of the FP64
+pmx_chi, h = 1e-6 Ry, on Si (inversion centre: the real eigensolver path)
+and GaAs (complex path), with and without Kramers-Kronig, for Gaussian and
+Lorentzian broadening."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+H = 1e-6
+NEPS = 9
+NBAND = 16
+NFREQ = 64
+DF = 0.15
+EPS = 0.2
+Q = np.array([[0.01, 0.0, 0.0], [0.05, 0.1, 0.0]])
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _args(a, vg):
+    return ["-crystal", "zb", "-a", str(a), "-vg", ",".join("%.17g" % x for x in vg), "-gcut",
+            "3.4641016151377544", "-neps", str(NEPS)]
+
+
+def _kpoints(model, rng, nk, offset):
+    """Random k-points whose transition energies all sit well inside a
+    frequency bin: the windowed sum's window jumps where dE crosses a
+    multiple of dfreq, which a finite difference must not straddle."""
+    K = []
+    while len(K) < nk:
+        k = rng.uniform(-0.5, 0.5, 3)
+        ok = True
+        ek = model.ctx.bands(k[None, :], NBAND, offset)[0, ::2]
+        for q in Q:
+            eq = model.ctx.bands((k + q)[None, :], NBAND, offset)[0, ::2]
+            if not (ek > 0).any() or not (eq <= 0).any():
+                ok = False
+                break
+            dE = ek[ek > 0][:, None] - eq[eq <= 0][None, :]
+            fr = dE / DF - np.floor(dE / DF)
+            if np.minimum(fr, 1 - fr).min() < 2e-3:
+                ok = False
+                break
+        if ok:
+            K.append(k)
+    return np.array(K)
+
+
+SYSTEMS = {
+    "si": (5.43, [3, 4, 8, 11] + [-0.2241, 0.0, 0.0551, 0.0724] + [-0.2241, 0.0, 0.0551, 0.0724]),
+    "gaas": (5.65, [3, 4, 8, 11] + [-0.1407, 0.05, 0.0277, 0.0635] + [-0.2807, -0.05, 0.0277, 0.0435]),
+}
+
+
+@pytest.fixture(scope="module", params=sorted(SYSTEMS))
+def setup(request):
+    from picomax_amd import autograd
+    a, vg = SYSTEMS[request.param]
+    rng = np.random.default_rng(11)
+    model = autograd.BandModel(_args(a, vg), refpoint=(0, 0, 0), nvalence=8)
+    off = model.offset()
+    K = _kpoints(model, rng, 5, off)
+    KW = rng.uniform(0.5, 1.5, len(K))
+    return request.param, model, K, KW, off, rng
+
+
+def _upstream(model, rng):
+    ne2 = NEPS * (NEPS + 1) // 2
+    shape = (9, len(Q), ne2, NFREQ)
+    mask = rng.uniform(size=(9, len(Q), ne2, 1)) < 0.3
+    return rng.standard_normal(shape) * mask, rng.standard_normal(shape) * mask
+
+
+@pytest.mark.parametrize("delta,kk", [(1, 1), (0, 0)])
+def test_chi_adjoint_vs_fd(setup, delta, kk):
+    name, model, K, KW, off, rng = setup
+    _say("chi adjoint vs FD: %s delta %d kk %d" % (name, delta, kk))
+    gRe, gIm = _upstream(model, rng)
+    v0 = model.vg0
+    model.set_vg(v0)
+    g = model.ctx.grad_chi(K, KW, Q, NBAND, off, delta, EPS, NFREQ, DF, gRe, gIm, kk=kk)
+    g_fd = np.zeros_like(v0)
+    for a in range(v0.shape[0]):
+        for s in range(v0.shape[1]):
+            val = []
+            for sg in (1, -1):
+                v = v0.copy()
+                v[a, s] += sg * H
+                model.set_vg(v)
+                re, im = model.ctx.chi(K, KW, Q, NBAND, off, delta, EPS, NFREQ, DF, kk=kk, precision="fp64")
+                val.append((gRe.reshape(-1) * re).sum() + (gIm.reshape(-1) * im).sum())
+            g_fd[a, s] = (val[0] - val[1]) / (2 * H)
+    model.set_vg(v0)
+    err = np.abs(g - g_fd).max() / np.abs(g_fd).max()
+    _say("  |g| %.3e  rel err %.3e" % (np.abs(g_fd).max(), err))
+    assert err < 1e-6
+
+
+def test_chi_autograd_function(setup):
+    import torch
+    from picomax_amd import autograd
+    name, model, K, KW, off, rng = setup
+    _say("chi torch.autograd.Function: %s" % name)
+    spec = autograd.ChiSpec(K, KW, Q, NBAND, off, 1, EPS, NFREQ, DF, kk=1)
+    v = torch.tensor(model.vg0, dtype=torch.float64, requires_grad=True)
+    re, im = autograd.chi_response(v, model, spec)
+    wr, wi = (torch.from_numpy(x.reshape(-1)) for x in _upstream(model, rng))
+    loss = (wr * re).sum() + (wi * im).sum()
+    loss.backward()
+    g = model.ctx.grad_chi(*spec.common(), wr.numpy(), wi.numpy(), kk=1)
+    assert np.allclose(v.grad.numpy(), g, rtol=1e-12, atol=1e-12 * np.abs(g).max())
+    model.set_vg(v.detach().numpy())
+
+
+def test_chi_adjoint_rejects(setup):
+    from picomax_amd import capi
+    name, model, K, KW, off, rng = setup
+    gRe, gIm = _upstream(model, rng)
+    with pytest.raises(capi.PmxError):  # triangle broadening is not differentiable
+        model.ctx.grad_chi(K, KW, Q, NBAND, off, 2, EPS, NFREQ, DF, gRe, gIm)
+    with pytest.raises(capi.PmxError):  # odd nband splits a Kramers pair
+        model.ctx.grad_chi(K, KW, Q, NBAND - 1, off, 1, EPS, NFREQ, DF, gRe, gIm)
