@@ -518,10 +518,16 @@ void orc_kk(int nfreq, double dfreq, const double *w, double *ReX, double *ImX) 
     kk_impl(nfreq, dfreq, w, ReX, ImX);
 }
 
-int orc_chi_tensor(const orc_system *sys, const int *loci, int neps, int nk, const double *K,
-                   const double *KW, int nq, const double *Q, int nband, double offset, int delta,
-                   double epsilon, int nfreq, double dfreq, int kk, double bz_volume, double *ReX,
-                   double *ImX) {
+} // extern "C"
+
+namespace {
+
+/* chi_tensor (src/pmx_chi.cpp:319-713) when ll == 0; chi_LL (the spinor form
+ * of src/pmx_chi_copy.cpp:320-676, which src_bkp/pmx_chi.cpp:126-439 is for a
+ * spatial basis) when ll == 1 */
+int chi_impl(const orc_system *sys, const int *loci, int neps, int nk, const double *K, const double *KW,
+             int nq, const double *Q, int nband, double offset, int delta, double epsilon, int nfreq,
+             double dfreq, int kk, double bz_volume, double *ReX, double *ImX, int ll) {
     const int npw = sys->npw, N2 = 2 * npw;
     if (nband > N2) return fail("nband exceeds the spinor basis size");
     if (delta < 0 || delta > 3) return fail("delta must be 0..3");
@@ -531,6 +537,12 @@ int orc_chi_tensor(const orc_system *sys, const int *loci, int neps, int nk, con
     const double a = sys->a;
     const double SF = 4.0 * bz_volume * std::pow(PI, 5) * std::pow(HBAR, 4) * std::pow(ECHG, 2) /
                       (std::pow(EV, 3) * std::pow(EMASS, 2) * std::pow(a * ANGSTROM, 5));
+    /* chi_LL's three scale factors (src/pmx_chi_copy.cpp:351-360) */
+    const double SF_LL = 2.0 * bz_volume * std::pow(ECHG, 2) / (PI * EV * a * ANGSTROM);
+    const double SF_TT = 32.0 * bz_volume * std::pow(PI, 3) * std::pow(HBAR, 4) * std::pow(ECHG, 2) /
+                         (std::pow(EV, 3) * std::pow(EMASS, 2) * std::pow(a * ANGSTROM, 5));
+    const double SF_LT = 8.0 * bz_volume * PI * std::pow(HBAR, 2) * std::pow(ECHG, 2) /
+                         (std::pow(EV, 2) * EMASS * std::pow(a * ANGSTROM, 3));
     std::vector<double> freq(nfreq);
     for (int f = 0; f < nfreq; f++) freq[f] = dfreq * f;
     const double w0 = freq[0], df = dfreq;
@@ -612,16 +624,21 @@ int orc_chi_tensor(const orc_system *sys, const int *loci, int neps, int nk, con
         }
 
         /* phases B + C, streamed per k with thread-private accumulators */
-        std::vector<std::vector<double>> accI(nthreads), accR(nthreads);
+        std::vector<std::vector<double>> accI(nthreads), accR(nthreads), accPI(nthreads), accPR(nthreads);
+        const bool pv = ll && !kk; /* chi_LL's principal-value term (src/pmx_chi_copy.cpp:560-577) */
         int errq = 0;
 #pragma omp parallel
         {
             const int tid = omp_get_thread_num();
-            std::vector<double> &aI = accI[tid], &aR = accR[tid];
+            std::vector<double> &aI = accI[tid], &aR = accR[tid], &pI = accPI[tid], &pR = accPR[tid];
             aI.assign((size_t)nel * nfreq, 0.0);
             aR.assign((size_t)nel * nfreq, 0.0);
+            if (pv) {
+                pI.assign((size_t)nel * nfreq, 0.0);
+                pR.assign((size_t)nel * nfreq, 0.0);
+            }
             std::vector<cplx> H((size_t)N2 * N2), Z((size_t)N2 * N2);
-            std::vector<double> w(N2), dl(nfreq);
+            std::vector<double> w(N2), dl(nfreq), gLL(nfreq), gLT(nfreq), gTT(nfreq);
 #pragma omp for schedule(dynamic)
             for (int k = 0; k < nk; k++) {
                 V3 Kv = v3(K + 3 * k);
@@ -649,6 +666,9 @@ int orc_chi_tensor(const orc_system *sys, const int *loci, int neps, int nk, con
                 /* rho_ch[i][m][c][v], ch = up, dn, updn, dnup (src/pmx_chi.cpp:501-548) */
                 const size_t nrho = (size_t)NTSR * neps * nc * nvb;
                 std::vector<cplx> rup(nrho, 0.0), rdn(nrho, 0.0), rud(nrho, 0.0), rdu(nrho, 0.0);
+                /* chi_LL's single vertex: density for i = 0, (orbital + SOC) current in the
+                 * same-spin products otherwise (src/pmx_chi_copy.cpp:506-537) */
+                std::vector<cplx> rll(ll ? nrho : 0, 0.0);
                 auto ridx = [&](int i, int m, int c, int v) {
                     return (((size_t)i * neps + m) * nc + c) * nvb + v;
                 };
@@ -659,7 +679,7 @@ int orc_chi_tensor(const orc_system *sys, const int *loci, int neps, int nk, con
                         const std::array<cplx, 3> &u = uv[m][i];
                         for (int c = 0; c < nc; c++)
                             for (int v = 0; v < nvb; v++) {
-                                cplx su = 0.0, sd = 0.0, sud = 0.0, sdu = 0.0;
+                                cplx su = 0.0, sd = 0.0, sud = 0.0, sdu = 0.0, sl = 0.0;
                                 for (size_t ia = 0; ia < active[m].size(); ia++) {
                                     int p = active[m][ia];
                                     int lp = loci[(size_t)m * npw + p];
@@ -677,8 +697,10 @@ int orc_chi_tensor(const orc_system *sys, const int *loci, int neps, int nk, con
                                     sd += cd * qd * orb;
                                     sud += cu * qd * soc;
                                     sdu += cd * qu * soc;
+                                    if (ll) sl += (cu * qu + cd * qd) * (i == 0 ? cplx(1.0) : orb + soc);
                                 }
                                 size_t id = ridx(i, m, c, v);
+                                if (ll) rll[id] = sl;
                                 rup[id] = su;
                                 rdn[id] = sd;
                                 rud[id] = sud;
@@ -686,8 +708,46 @@ int orc_chi_tensor(const orc_system *sys, const int *loci, int neps, int nk, con
                             }
                     }
                 }
-                /* windowed accumulation (src/pmx_chi.cpp:573-628) */
                 const double kw = KW[k];
+                if (ll) {
+                    /* every transition at every frequency (no window) */
+                    for (int c = 0; c < nc; c++)
+                        for (int v = 0; v < nvb; v++) {
+                            const double dE = Ek[k][c] - Ekq[v];
+                            for (int f = 0; f < nfreq; f++) {
+                                dl[f] = delta_fn(delta, epsilon, dE - freq[f]);
+                                if (pv) {
+                                    const double den = dE * dE - freq[f] * freq[f];
+                                    gLL[f] = dE / den;
+                                    gLT[f] = 1.0 / den;
+                                    gTT[f] = 1.0 / dE / den;
+                                }
+                            }
+                            for (int i = 0; i < NTSR; i++)
+                                for (int j = 0; j < NTSR; j++) {
+                                    const double *g = (i == 0 && j == 0) ? gLL.data()
+                                                      : (i == 0 || j == 0) ? gLT.data() : gTT.data();
+                                    for (int m = 0; m < neps; m++)
+                                        for (int n = 0; n <= m; n++) {
+                                            const cplx O = rll[ridx(i, m, c, v)] * std::conj(rll[ridx(j, n, c, v)]);
+                                            const size_t el =
+                                                ((size_t)(i * 3 + j) * ne2 + (size_t)m * (m + 1) / 2 + n) * nfreq;
+                                            const double a1 = kw * O.real(), a2 = kw * O.imag();
+                                            for (int f = 0; f < nfreq; f++) {
+                                                aI[el + f] += a1 * dl[f];
+                                                aR[el + f] -= a2 * dl[f];
+                                            }
+                                            if (pv)
+                                                for (int f = 0; f < nfreq; f++) {
+                                                    pR[el + f] += a1 * g[f];
+                                                    pI[el + f] += a2 * g[f];
+                                                }
+                                        }
+                                }
+                        }
+                    continue;
+                }
+                /* windowed accumulation (src/pmx_chi.cpp:573-628) */
                 for (int c = 0; c < nc; c++)
                     for (int v = 0; v < nvb; v++) {
                         const double dE = Ek[k][c] - Ekq[v];
@@ -727,15 +787,51 @@ int orc_chi_tensor(const orc_system *sys, const int *loci, int neps, int nk, con
             for (int e2 = 0; e2 < ne2; e2++) {
                 size_t src = ((size_t)ij * ne2 + e2) * nfreq;
                 size_t dst = (((size_t)ij * nq + q) * ne2 + e2) * nfreq;
+                const int i = ij / 3, j = ij % 3;
+                int m = 0;
+                while ((m + 1) * (m + 2) / 2 <= e2) m++;
+                const int n = e2 - m * (m + 1) / 2;
+                const double nm = std::sqrt(sqn(Qv + v3(sys->G + 3 * m)));
+                const double nn = std::sqrt(sqn(Qv + v3(sys->G + 3 * n)));
                 for (int f = 0; f < nfreq; f++) {
-                    double si = 0.0, sr = 0.0;
+                    double si = 0.0, sr = 0.0, pi_ = 0.0, pr = 0.0;
                     for (int t = 0; t < nthreads; t++) {
                         if (accI[t].empty()) continue;
                         si += accI[t][src + f];
                         sr += accR[t][src + f];
+                        if (pv) {
+                            pi_ += accPI[t][src + f];
+                            pr += accPR[t][src + f];
+                        }
                     }
-                    ImX[dst + f] = SF * (si * inv_w2[f]);
-                    ReX[dst + f] = SF * (sr * inv_w2[f]);
+                    if (!ll) {
+                        ImX[dst + f] = SF * (si * inv_w2[f]);
+                        ReX[dst + f] = SF * (sr * inv_w2[f]);
+                        continue;
+                    }
+                    /* chi_LL's scaling and sum (src/pmx_chi_copy.cpp:579-622) */
+                    const double wf = freq[f];
+                    if (f == 0) {
+                        si = 0.0;
+                        sr = 0.0;
+                    } else {
+                        const double sc = (i == 0 && j == 0) ? SF_LL / (nm * nn)
+                                          : (i == 0)         ? SF_LT / (nm * wf)
+                                          : (j == 0)         ? SF_LT / nn
+                                                             : SF_TT / (wf * wf);
+                        si *= sc;
+                        sr *= sc;
+                    }
+                    if (pv) {
+                        const double sc = (i == 0 && j == 0) ? SF_LL / (nm * nn)
+                                          : (i == 0)         ? SF_LT / nm
+                                          : (j == 0)         ? SF_LT / nn
+                                                             : SF_TT;
+                        pi_ *= sc;
+                        pr *= sc;
+                    }
+                    ImX[dst + f] = PI * si + 2.0 * pi_;
+                    ReX[dst + f] = PI * sr + 2.0 * pr;
                 }
             }
         if (kk) {
@@ -748,6 +844,25 @@ int orc_chi_tensor(const orc_system *sys, const int *loci, int neps, int nk, con
         }
     }
     return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int orc_chi_tensor(const orc_system *sys, const int *loci, int neps, int nk, const double *K,
+                   const double *KW, int nq, const double *Q, int nband, double offset, int delta,
+                   double epsilon, int nfreq, double dfreq, int kk, double bz_volume, double *ReX,
+                   double *ImX) {
+    return chi_impl(sys, loci, neps, nk, K, KW, nq, Q, nband, offset, delta, epsilon, nfreq, dfreq, kk,
+                    bz_volume, ReX, ImX, 0);
+}
+
+int orc_chi_ll(const orc_system *sys, const int *loci, int neps, int nk, const double *K, const double *KW,
+               int nq, const double *Q, int nband, double offset, int delta, double epsilon, int nfreq,
+               double dfreq, int kk, double bz_volume, double *ReX, double *ImX) {
+    return chi_impl(sys, loci, neps, nk, K, KW, nq, Q, nband, offset, delta, epsilon, nfreq, dfreq, kk,
+                    bz_volume, ReX, ImX, 1);
 }
 
 } // extern "C"

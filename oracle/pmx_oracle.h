@@ -80,6 +80,17 @@ int orc_chi_tensor(const orc_system *sys, const int *loci, int neps,
                    int delta, double epsilon, int nfreq, double dfreq,
                    int kk, double bz_volume, double *ReX, double *ImX);
 
+/* chi_LL in the spinor basis of the current sources (src/pmx_chi_copy.cpp:
+ * 320-676; src_bkp/pmx_chi.cpp:126-439 is the same sum over a spatial basis):
+ * density vertex for i = 0, no frequency window, scale factors LL/LT/TT, the
+ * principal-value term when kk == 0, f = 0 of the delta term zeroed.  Same
+ * arguments and output layout as orc_chi_tensor. */
+int orc_chi_ll(const orc_system *sys, const int *loci, int neps,
+               int nk, const double *K, const double *KW,
+               int nq, const double *Q, int nband, double offset,
+               int delta, double epsilon, int nfreq, double dfreq,
+               int kk, double bz_volume, double *ReX, double *ImX);
+
 /* kramerskronigtransform (src/pmx_chi.cpp:272-309) on one element. */
 void orc_kk(int nfreq, double dfreq, const double *w, double *ReX, double *ImX);
 

@@ -99,7 +99,7 @@ def run(tokens):
     if p["switch"] == "band":
         return s.bands(p["Q"], p["nband"], off).ravel(), None
     re, im = s.chi(p["K"], p["KW"], p["Q"], p["nband"], off, p["delta"], p["epsilon"],
-                   p["nfreq"], p["dfreq"], p["kk"])
+                   p["nfreq"], p["dfreq"], p["kk"], ll=p["switch"] == "epsLL")
     return np.concatenate([re, im]), s.G[:p["neps"]].copy()
 
 

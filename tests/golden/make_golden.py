@@ -18,6 +18,23 @@ clean-up frees C_k rows by the valence count (src/pmx_chi.cpp:697-703) and
 aborts otherwise.  tests/test_oracle.py checks both
 conditions on every chi fixture with the oracle.
 
+The -switch epsLL cases run the older snapshot src_bkp/ (`make -C oracle ref`
+also builds oracle/_ref/picomax_ref_bkp): the current src/ main does not
+dispatch epsLL (src/pmx.cpp:171-203) and its chi_LL never allocates its output
+(src/pmx_chi_copy.cpp:363).  src_bkp differs from the current sources in
+three ways, each undone here so that its output pins the current chi_LL:
+  * its pseudopotential is a fixed fit V(|G|) (src_bkp/pmx_epm.cpp:65-108)
+    on every |dG|^2 below the last -vg key; the -vg values are ignored.  The
+    cases pass the zinc-blende keys 3,4,8,11 with the fit evaluated there
+    (coefficients read from that file), which the current H uses as given;
+  * its energy zero is e[nvalence-1] at the reference point
+    (src_bkp/pmx_epm.cpp:141-143), not the mid-point: the entry records the
+    rule ("offset": "e[nvalence-1]") and the tests pass that zero;
+  * it solves the spatial H (-nband, -nvalence count spatial states); the
+    current sources count spinor states of the spin-doubled H, whose chi_LL
+    sums each Kramers pair: twice src_bkp's chi at doubled -nband/-nvalence
+    ("args_spinor", "scale").
+
 Usage: python3 tests/golden/make_golden.py [--ref oracle/_ref/picomax_ref]
 """
 import argparse
@@ -58,6 +75,27 @@ WZ = ["-crystal", "wz", "-a", "3.19", "-f", "1.626", "-u", "0.375", "-vg", WZ_VG
 
 TE_K = [[0.1311, 0.0527, 0.2113], [-0.2471, 0.1902, -0.0817], [0.3149, -0.0733, 0.1267]]
 WZ_K = [[0.0713, 0.1429, 0.0811], [-0.1931, 0.0512, -0.1513]]
+
+
+def bkp_fit_vg():
+    """-vg for the src_bkp cases: keys 3,4,8,11 with src_bkp's V(|G|) fit
+    (src_bkp/pmx_epm.cpp:65-98) evaluated there, the same on both atoms."""
+    import math
+    import re
+    text = open(os.path.join(REF_ROOT, "src_bkp", "pmx_epm.cpp")).read()
+    c = {k: float(v) for k, v in re.findall(r"^double ([a-e][1-8]) = ([-0-9.]+)\s*;", text, re.M)}
+
+    def V(g2):
+        g = math.sqrt(g2)
+        s = sum(c["a%d" % i] * math.sin(c["b%d" % i] * g + c["c%d" % i]) for i in range(1, 9))
+        return s * math.exp(-c["d1"] * g ** c["e1"])
+    v = ["%.17g" % V(g2) for g2 in (3, 4, 8, 11)]
+    return ",".join(["3", "4", "8", "11"] + v + v)
+
+
+REF_ROOT = os.environ.get("PMX_REFERENCE", "/root/reference")
+
+ZBFIT = ["-crystal", "zb", "-a", "5.43", "-vg", "{bkp_vg}", "-gcut", "3.4641016151377544"]
 
 CASES = {
     # C1 band, exactly as BASELINE's first config (H4: offset at Gamma, nvalence 4)
@@ -141,7 +179,37 @@ CASES = {
                     "-neps", "5", "-nfreq", "160", "-dfreq", "0.1", "-delta", "0",
                     "-epsilon", "0.2", "-kk", "1"],
                    kfile=("kpoints_wz_2.txt", WZ_K), keep=[".dat", ".gpt"]),
+    # chi_LL (src_bkp/pmx_chi.cpp:126-439): Gaussian, principal-value real part (kk 0)
+    "zbfit_ll_gauss_pv": dict(args=["-switch", "epsLL"] + ZBFIT +
+                              ["-nband", "14", "-nvalence", "4", "-refpoint", "0.1,0.2,0.3",
+                               "-qvec", "0.01,0,0", "-kpointfile", "{golden}/kpoints_zb_4.txt",
+                               "-neps", "4", "-nfreq", "150", "-dfreq", "0.1", "-delta", "0",
+                               "-epsilon", "0.2", "-kk", "0"],
+                              ref="bkp", kgrid=("zb", 4), keep=[".dat", ".gpt"]),
+    # Lorentzian over every transition (no window), two q, KK on
+    "zbfit_ll_lor_kk": dict(args=["-switch", "epsLL"] + ZBFIT +
+                            ["-nband", "14", "-nvalence", "4", "-refpoint", "0.1,0.2,0.3",
+                             "-qvec", "0.01,0,0,0.0,0.02,0.01", "-kpointfile", "{golden}/kpoints_zb_4.txt",
+                             "-neps", "4", "-nfreq", "200", "-dfreq", "0.08", "-delta", "1",
+                             "-epsilon", "0.1", "-kk", "1"],
+                            ref="bkp", kgrid=("zb", 4), keep=[".dat", ".gpt"]),
+    # triangle broadening with the principal-value term
+    "zbfit_ll_tri_pv": dict(args=["-switch", "epsLL"] + ZBFIT +
+                            ["-nband", "18", "-nvalence", "4", "-refpoint", "0.1,0.2,0.3",
+                             "-qvec", "0.02,0.01,0", "-kpointfile", "{golden}/kpoints_zb_4.txt",
+                             "-neps", "3", "-nfreq", "100", "-dfreq", "0.15", "-delta", "2",
+                             "-epsilon", "0.3", "-kk", "0"],
+                            ref="bkp", kgrid=("zb", 4), keep=[".dat", ".gpt"]),
 }
+
+
+def spinor_args(args):
+    """src_bkp's spatial -nband/-nvalence -> the current sources' spinor counts."""
+    out = list(args)
+    for opt in ("-nband", "-nvalence"):
+        i = out.index(opt)
+        out[i + 1] = str(2 * int(out[i + 1]))
+    return out
 
 
 def sha256(path):
@@ -168,6 +236,7 @@ def prepare_kfiles(case):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default=os.path.join(ROOT, "oracle", "_ref", "picomax_ref"))
+    ap.add_argument("--ref-bkp", default=os.path.join(ROOT, "oracle", "_ref", "picomax_ref_bkp"))
     ap.add_argument("--only", nargs="*")
     opt = ap.parse_args()
     if not os.path.exists(opt.ref):
@@ -179,13 +248,23 @@ def main():
         if opt.only and name not in opt.only:
             continue
         prepare_kfiles(case)
+        if case.get("ref") == "bkp":
+            case = dict(case, args=[a.replace("{bkp_vg}", bkp_fit_vg()) for a in case["args"]])
         args = [a.replace("{golden}", HERE) for a in case["args"]]
+        ref = opt.ref_bkp if case.get("ref") == "bkp" else opt.ref
+        if not os.path.exists(ref):
+            sys.exit("reference binary missing: run `make -C oracle ref`")
         with tempfile.TemporaryDirectory() as tmp:
-            cmd = [opt.ref] + args + ["-wdir", tmp, "-outputfile", name]
+            cmd = [ref] + args + ["-wdir", tmp, "-outputfile", name]
             r = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
             if r.returncode != 0:
                 sys.exit("%s failed:\n%s" % (name, r.stdout[-2000:]))
             entry = {"args": [a.replace(HERE, "{golden}") for a in case["args"]], "files": {}}
+            if case.get("ref") == "bkp":
+                entry["ref"] = "src_bkp"
+                entry["args_spinor"] = spinor_args(entry["args"])
+                entry["scale"] = 2.0
+                entry["offset"] = "e[nvalence-1]"
             for ext in case["keep"]:
                 src = os.path.join(tmp, name + ext)
                 dst = os.path.join(HERE, name + ext)

@@ -60,6 +60,7 @@ def lib():
                                      _dp, _dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_double, _dp, _dp]
+        L.orc_chi_ll.argtypes = L.orc_chi_tensor.argtypes
         L.orc_kk.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
         L.orc_delta.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.orc_delta.restype = ctypes.c_double
@@ -264,7 +265,8 @@ class System:
         _check(lib().orc_bands(ctypes.byref(self.c), len(Q), _p(Q), int(nband), float(offset), _p(eb)))
         return eb
 
-    def chi(self, K, KW, Q, nband, offset, delta, epsilon, nfreq, dfreq, kk):
+    def chi(self, K, KW, Q, nband, offset, delta, epsilon, nfreq, dfreq, kk, ll=False):
+        """chi_tensor, or chi_LL with ll=True; [i][j][q][m][n<=m][f] Re and Im."""
         K = np.ascontiguousarray(np.array(K, float).reshape(-1, 3))
         KW = np.ascontiguousarray(np.array(KW, float))
         Q = np.ascontiguousarray(np.array(Q, float).reshape(-1, 3))
@@ -272,7 +274,8 @@ class System:
         n = 9 * len(Q) * ne2 * nfreq
         re = np.zeros(n)
         im = np.zeros(n)
-        _check(lib().orc_chi_tensor(ctypes.byref(self.c), self.loci.ctypes.data_as(_ip), self.neps,
+        fn = lib().orc_chi_ll if ll else lib().orc_chi_tensor
+        _check(fn(ctypes.byref(self.c), self.loci.ctypes.data_as(_ip), self.neps,
                                     len(K), _p(K), _p(KW), len(Q), _p(Q), int(nband), float(offset),
                                     int(delta), float(epsilon), int(nfreq), float(dfreq), int(kk),
                                     float(self.bz), _p(re), _p(im)))
