@@ -126,6 +126,16 @@ int pmx_chi(pmx_ctx *ctx, const pmx_chi_params *p, double *ReX, double *ImX);
  * layout and sizes): nothing crosses PCIe but the k / q lists. */
 int pmx_chi_device(pmx_ctx *ctx, const pmx_chi_params *p, double *dReX, double *dImX);
 
+/* chi_LL, the -switch epsLL tensor (src/pmx_chi_copy.cpp:320-676 in the
+ * spinor basis of the current sources; src_bkp/pmx_chi.cpp:126-439 is the
+ * same sum over a spatial basis): density vertex for i = 0, every transition
+ * at every frequency (no window), LL/LT/TT scale factors over |q+G_m|,
+ * |q+G_n| and w, the delta term zeroed at f = 0, and with kk == 0 the
+ * principal-value term (per transition, FP64).  Same parameters and output
+ * layout as pmx_chi; the _device form takes device output pointers. */
+int pmx_chi_ll(pmx_ctx *ctx, const pmx_chi_params *p, double *ReX, double *ImX);
+int pmx_chi_ll_device(pmx_ctx *ctx, const pmx_chi_params *p, double *dReX, double *dImX);
+
 /* Per-kernel timing with HIP events on the context's stream (no reference
  * counterpart).  enable != 0 starts recording (and clears), 0 stops.
  * pmx_profile_report writes a JSON object

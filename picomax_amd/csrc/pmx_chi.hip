@@ -106,6 +106,7 @@ struct RhoArgs {
     const double *K;          /* [nk][3] */
     zd *X;                    /* FP64 rows (or null) */
     float2 *Xf;               /* FP32 rows (or null) */
+    int ll;                   /* chi_LL's vertex: density for i = 0, one channel with SOC */
 };
 
 __device__ __forceinline__ void put_item(const RhoArgs &a, size_t row, int ch, int col, zd v) {
@@ -168,6 +169,30 @@ __global__ void __launch_bounds__(256) rho_kernel(RhoArgs a) {
             }
             zd R[3] = {R0, R1, R2};
             for (int i = 0; i < 3; i++) {
+                const zd *u = a.uc + ((size_t)m * 3 + i) * 3;
+                zd bk{u[0].x * Kx + u[1].x * Ky + u[2].x * Kz, u[0].y * Kx + u[1].y * Ky + u[2].y * Kz};
+                zd bs = cmul(bk, S);
+                put_item(a, row, 0, 3 * m + i, (a.ll && i == 0) ? S : zd{R[i].x + bs.x, R[i].y + bs.y});
+            }
+        } else if (a.ll) {
+            /* chi_LL with SOC (src/pmx_chi_copy.cpp:506-537): same-spin products only,
+             * density for i = 0, u_i.(orbital + socCurrent) otherwise; one channel */
+            const zd *wS = a.wS + (size_t)m * 3 * a.npw;
+            zd S{0, 0}, R[3] = {{0, 0}, {0, 0}, {0, 0}};
+            for (int p = 0; p < a.npw; p++) {
+                const int l = lm[p];
+                if (l < 0) continue;
+                const zd puu = cmul(cconj(ck[2 * p]), cq[2 * l]), pdd = cmul(cconj(ck[2 * p + 1]), cq[2 * l + 1]);
+                const zd pr{puu.x + pdd.x, puu.y + pdd.y};
+                S.x += pr.x; S.y += pr.y;
+                for (int i = 1; i < 3; i++) {
+                    const zd wa = wA[i * a.npw + p], sw = wS[i * a.npw + p];
+                    const zd t = cmul(pr, zd{wa.x + sw.x, wa.y + sw.y});
+                    R[i].x += t.x; R[i].y += t.y;
+                }
+            }
+            put_item(a, row, 0, 3 * m, S);
+            for (int i = 1; i < 3; i++) {
                 const zd *u = a.uc + ((size_t)m * 3 + i) * 3;
                 zd bk{u[0].x * Kx + u[1].x * Ky + u[2].x * Kz, u[0].y * Kx + u[1].y * Ky + u[2].y * Kz};
                 zd bs = cmul(bk, S);
@@ -342,7 +367,7 @@ template <class T> __global__ void __launch_bounds__(256) rho_mfma_kernel(RhoArg
                 const zd *u = a.uc + ((size_t)m * 3 + i) * 3;
                 zd bk{u[0].x * Kx + u[1].x * Ky + u[2].x * Kz, u[0].y * Kx + u[1].y * Ky + u[2].y * Kz};
                 zd bs = cmul(bk, S);
-                put_item(a, row, 0, 3 * m + i, zd{R.x + bs.x, R.y + bs.y});
+                put_item(a, row, 0, 3 * m + i, (a.ll && i == 0) ? S : zd{R.x + bs.x, R.y + bs.y});
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -777,6 +802,8 @@ struct DirArgs {
     const double *es, *ws;    /* sorted dE and weights */
     const int *start;
     zd *Z;                    /* [nfreq][npairs][256] */
+    int off_b;                /* bin = f0 + off_b */
+    int pv;                   /* chi_LL's principal-value kernels instead of delta */
 };
 
 __device__ double delta_fn(int code, double eps, double x) {
@@ -790,6 +817,13 @@ __device__ double delta_fn(int code, double eps, double x) {
     return 0.0;
 }
 
+/* chi_LL's principal-value kernels (src/pmx_chi_copy.cpp:560-577) by the
+ * class of the element: 0 = LL (i = j = 0), 1 = LT (one of them 0), 2 = TT */
+__device__ __forceinline__ double pv_fn(int cls, double dE, double w) {
+    const double den = dE * dE - w * w;
+    return cls == 0 ? dE / den : (cls == 1 ? 1.0 / den : 1.0 / dE / den);
+}
+
 __global__ void __launch_bounds__(256) direct_kernel(DirArgs a) {
     const int pairi = blockIdx.x, b = blockIdx.y;
     const int t0 = a.start[b], t1 = a.start[b + 1];
@@ -798,10 +832,11 @@ __global__ void __launch_bounds__(256) direct_kernel(DirArgs a) {
     tile_pair(pairi, ti, tj);
     const int r = threadIdx.x / TILE, cc = threadIdx.x % TILE;
     const int ar = ti * TILE + r, ac = tj * TILE + cc;
-    const int f0 = b - a.wh;
+    const int f0 = b - a.off_b;
     const int fmin = max(0, f0 - a.wh), fmax = min(a.nfreq - 1, f0 + a.wh);
+    const int cls = (ar % 3 == 0 && ac % 3 == 0) ? 0 : ((ar % 3 == 0 || ac % 3 == 0) ? 1 : 2);
     __shared__ zd xa[32][TILE], xb[32][TILE];
-    __shared__ double ws[32];
+    __shared__ double ws[3][32];
     const int per = (a.nch == 1) ? 32 : 16;
     for (int f = fmin; f <= fmax; f++) {
         zd acc{0, 0};
@@ -820,22 +855,24 @@ __global__ void __launch_bounds__(256) direct_kernel(DirArgs a) {
                 }
                 if (which == 0) xa[item][col] = v; else xb[item][col] = v;
             }
-            for (int e = threadIdx.x; e < 32; e += blockDim.x) {
-                int tl = (a.nch == 1) ? e : e / 2;
+            for (int e = threadIdx.x; e < 32 * 3; e += blockDim.x) {
+                const int it = e % 32, k = e / 32;
+                int tl = (a.nch == 1) ? it : it / 2;
                 double w = 0.0;
-                if (tl < cnt) {
+                if (tl < cnt && (a.pv || k == 0)) {
                     w = a.ws[base + tl];
-                    if (a.nch != 1 && (e & 1)) w = -w;
-                    w *= delta_fn(a.delta, a.eps, a.es[base + tl] - wf);
+                    if (a.nch != 1 && (it & 1)) w = -w;
+                    w *= a.pv ? pv_fn(k, a.es[base + tl], wf) : delta_fn(a.delta, a.eps, a.es[base + tl] - wf);
                 }
-                ws[e] = w;
+                ws[k][it] = w;
             }
             __syncthreads();
             const int nitems = (a.nch == 1) ? cnt : 2 * cnt;
+            const double *wsel = ws[a.pv ? cls : 0];
             for (int it = 0; it < nitems; it++) {
                 zd p = cmulc(xa[it][r], xb[it][cc]);
-                acc.x += p.x * ws[it];
-                acc.y += p.y * ws[it];
+                acc.x += p.x * wsel[it];
+                acc.y += p.y * wsel[it];
             }
         }
         if (ar < a.D && ac < a.D) {
@@ -852,14 +889,15 @@ __global__ void __launch_bounds__(256) direct_kernel(DirArgs a) {
  * one block per (tile pair, block of 16 frequencies)
  * ------------------------------------------------------------------ */
 struct ConvArgs {
-    int npairs, nbins, NT, wh, nfreq;
+    int npairs, nbins, NT, R0, W, nfreq;
     const void *M;    /* [nbins][NT][npairs][256] complex of T */
-    const void *Kt;   /* [2wh+1][NT] of T */
+    const void *Kt;   /* [W][NT] of T: row b - f + R0 */
     zd *Z;            /* [nfreq][npairs][256] */
+    int kt_lds;       /* stage Kt in LDS (else read it from global memory) */
 };
 
-/* Z(f, e) = sum_{b, n} Kt[b - wh - f][n] M[b][n][e]: a GEMM whose A operand
- * is the banded Taylor-kernel table.  A wave owns 16 elements and 4 blocks of
+/* Z(f, e) = sum_{b, n} Kt[b - f + R0][n] M[b][n][e] over the rows 0..W-1 of
+ * the table: a GEMM whose A operand is the banded Taylor-kernel table.  A wave owns 16 elements and 4 blocks of
  * 16 frequencies, so every M value it loads feeds 4 MFMA pairs; k steps that
  * miss a block's band are skipped for that block.  Kt is staged in LDS.  T =
  * double (FP64 MFMA 16x16x4) or float (FP32 MFMA 16x16x4, the mixed paths). */
@@ -868,10 +906,14 @@ template <class T> __global__ void __launch_bounds__(256) conv_mfma_kernel(ConvA
     typedef typename MfmaT<T>::V V;
     typedef typename MfmaT<T>::C C;
     extern __shared__ double smem_c[];
-    T *kt = reinterpret_cast<T *>(smem_c);
-    const int W = 2 * a.wh + 1;
-    for (int i = threadIdx.x; i < W * a.NT; i += 256) kt[i] = reinterpret_cast<const T *>(a.Kt)[i];
-    __syncthreads();
+    const int W = a.W;
+    const T *kt = reinterpret_cast<const T *>(a.Kt);
+    if (a.kt_lds) {
+        T *ks = reinterpret_cast<T *>(smem_c);
+        for (int i = threadIdx.x; i < W * a.NT; i += 256) ks[i] = kt[i];
+        __syncthreads();
+        kt = ks;
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int E = a.npairs * 256;
     const int e0 = (blockIdx.x * 4 + wave) * 16;
@@ -879,8 +921,9 @@ template <class T> __global__ void __launch_bounds__(256) conv_mfma_kernel(ConvA
     const int fb0 = blockIdx.y * 16 * CONV_FB;
     const int eb = e0 + (lane & 15); /* B column: element */
     const int kq = lane >> 4;
-    const int b_lo = fb0, b_hi = min(a.nbins - 1, fb0 + 16 * CONV_FB - 1 + 2 * a.wh);
-    const int kcount = (b_hi - b_lo + 1) * a.NT;
+    /* bins under this block's frequencies: b - f + R0 in [0, W) */
+    const int b_lo = max(0, fb0 - a.R0), b_hi = min(a.nbins - 1, fb0 + 16 * CONV_FB - 1 - a.R0 + W - 1);
+    const int kcount = max(0, b_hi - b_lo + 1) * a.NT;
     const C *M = reinterpret_cast<const C *>(a.M);
     V accR[CONV_FB], accI[CONV_FB];
 #pragma unroll
@@ -907,9 +950,9 @@ template <class T> __global__ void __launch_bounds__(256) conv_mfma_kernel(ConvA
 #pragma unroll
         for (int q = 0; q < CONV_FB; q++) {
             const int fq = fb0 + 16 * q;
-            if (bs1 < fq || bs0 > fq + 15 + 2 * a.wh) continue; /* outside this block's band */
-            const int fa = fq + (lane & 15);                      /* A row: frequency */
-            const int ti = kk + (b_lo - fa) * a.NT;
+            if (bs1 < fq - a.R0 || bs0 > fq + 15 - a.R0 + W - 1) continue; /* outside this block's band */
+            const int fa = fq + (lane & 15);                                  /* A row: frequency */
+            const int ti = kk + (b_lo - fa + a.R0) * a.NT;
             const T av = (kk < kcount && fa < a.nfreq && ti >= 0 && ti < WN) ? kt[ti] : T(0);
             accR[q] = MfmaT<T>::mma(av, br, accR[q]);
             accI[q] = MfmaT<T>::mma(av, bi, accI[q]);
@@ -971,6 +1014,72 @@ __global__ void pack_kernel(PackArgs a) {
         a.ImX[dst] = a.SF * (imv * inv);
         a.ReX[dst] = a.SF * (rev * inv);
     }
+}
+
+/* chi_LL's scaling and sum (src/pmx_chi_copy.cpp:579-622): the delta sums
+ * (Z1) scaled by LL/LT/TT factors over |q+G_m|, |q+G_n| and w (zero at
+ * f = 0), plus twice the principal-value sums (Zp) with their own factors;
+ * Im <- pi tmp_imag_1 + 2 tmp_imag_2, Re <- pi tmp_real_1 + 2 tmp_real_2. */
+struct PackLLArgs {
+    int D, neps, npairs, nfreq, pv;
+    double SF_LL, SF_LT, SF_TT, dfreq, w0;
+    const double *qgn; /* [neps] |q + G_m| */
+    const zd *Z1, *Zp;
+    double *ReX, *ImX; /* compact [ij][e2][f] */
+};
+
+__global__ void pack_ll_kernel(PackLLArgs a) {
+    const int ne2 = a.neps * (a.neps + 1) / 2;
+    const size_t total = (size_t)9 * ne2 * a.nfreq;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int f = idx % a.nfreq;
+        const size_t el = idx / a.nfreq;
+        const int ij = el / ne2, e2 = el % ne2;
+        const int i = ij / 3, j = ij % 3;
+        int m = (int)((sqrt(8.0 * e2 + 1.0) - 1.0) * 0.5);
+        while ((m + 1) * (m + 2) / 2 <= e2) m++;
+        while (m * (m + 1) / 2 > e2) m--;
+        const int n = e2 - m * (m + 1) / 2;
+        const double nm = a.qgn[m], nn = a.qgn[n], w = a.w0 + a.dfreq * f;
+        const zd z1 = zfetch(a.Z1, f, a.npairs, 3 * m + i, 3 * n + j);
+        double ti = z1.x, tr = -z1.y;
+        if (f == 0) {
+            ti = 0.0;
+            tr = 0.0;
+        } else {
+            const double sc = (i == 0 && j == 0) ? a.SF_LL / (nm * nn)
+                              : (i == 0)         ? a.SF_LT / (nm * w)
+                              : (j == 0)         ? a.SF_LT / nn
+                                                 : a.SF_TT / (w * w);
+            ti *= sc;
+            tr *= sc;
+        }
+        double pi_ = 0.0, pr = 0.0;
+        if (a.pv) {
+            const zd zp = zfetch(a.Zp, f, a.npairs, 3 * m + i, 3 * n + j);
+            const double sc = (i == 0 && j == 0) ? a.SF_LL / (nm * nn)
+                              : (i == 0)         ? a.SF_LT / nm
+                              : (j == 0)         ? a.SF_LT / nn
+                                                 : a.SF_TT;
+            pr = zp.x * sc;
+            pi_ = zp.y * sc;
+        }
+        a.ImX[idx] = PMX_PI * ti + 2.0 * pi_;
+        a.ReX[idx] = PMX_PI * tr + 2.0 * pr;
+    }
+}
+
+/* largest f0 = int((dE - w0)/df) over the transitions (dE > 0) */
+__global__ void __launch_bounds__(256) f0max_kernel(int nt, const double *dE, double w0, double df, int *out) {
+    __shared__ int red[256];
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    red[threadIdx.x] = t < nt ? max(0, int((dE[t] - w0) / df)) : 0;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, red[0]);
 }
 
 /* kramerskronigtransform (src/pmx_chi.cpp:272-309), one thread per (element, f):
@@ -1156,14 +1265,14 @@ template <class T> __global__ void kkmat_kernel(int nfreq, double dfreq, T *Kk) 
     }
 }
 
-/* host: Taylor kernel table K_n(D) = h^n delta^(n)((D+1/2) df)/n! */
-void taylor_table(int delta, double eps, double df, int wh, int NT, std::vector<double> &Kt) {
-    const int W = 2 * wh + 1;
+/* host: Taylor kernel table K_n(D) = h^n delta^(n)((D+1/2) df)/n! for the
+ * W offsets D = f0 - f = Dlo .. Dlo + W - 1 (row D - Dlo) */
+void taylor_table(int delta, double eps, double df, int Dlo, int W, int NT, std::vector<double> &Kt) {
     Kt.assign((size_t)W * NT, 0.0);
     const double h = 0.5 * df;
-    for (int Dd = -wh; Dd <= wh; Dd++) {
+    for (int Dd = Dlo; Dd < Dlo + W; Dd++) {
         const double x = (Dd + 0.5) * df; /* c_b - w_f */
-        double *row = &Kt[(size_t)(Dd + wh) * NT];
+        double *row = &Kt[(size_t)(Dd - Dlo) * NT];
         if (delta == 1) {
             /* (1/pi) Im[(-1)^n / (x - i eps)^(n+1)] h^n */
             std::complex<double> z(x, -eps), inv = 1.0 / z, pw = inv;
@@ -1221,7 +1330,7 @@ template <class T> int upload(hipStream_t s, DBuf<T> &b, const std::vector<T> &h
 
 struct ChiWork {
     DBuf<double> Kd, KQd, KWd, evk, evkq, dEd, wtd, Ktd, Kkd, Re0, Im0, ReD, ImD;
-    DBuf<double> ws, ss, es;
+    DBuf<double> ws, ss, es, qgn;
     DBuf<zd> vk, vkq, Xd, M1, M2, Z1, Z2, wAd, ucd, wSd;
     DBuf<float2> Xf;
     DBuf<float> Ktf, Kkf;
@@ -1305,7 +1414,7 @@ static void q_tables(const pmx_ctx *c, double Qx, double Qy, double Qz, bool spa
     }
 }
 
-static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm) {
+static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm, bool ll) {
     if (!c || !p || !dRe || !dIm) return set_error(PMX_EINVAL, "null pointer");
     if (!c->has_system) return set_error(PMX_ESTATE, "no system set");
     if (p->nfreq <= 0 || !(p->dfreq > 0)) return set_error(PMX_EINVAL, "nfreq > 0 and dfreq > 0 required");
@@ -1321,13 +1430,22 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
     if (!c->soc && !spatial) return set_error(PMX_EINVAL, "odd -nband without SOC splits a Kramers pair; use an even -nband");
     const int N = basis_size(c);
     const int nb_s = spatial ? p->nband / 2 : p->nband;
-    const int nch = spatial ? 1 : 4;
+    const int nch = (spatial || ll) ? 1 : 4;
     const double SF = chi_scale(c);
+    /* chi_LL's scale factors (src/pmx_chi_copy.cpp:351-360) */
+    const double SF_LL = 2.0 * c->bz * std::pow(PMX_E, 2) / (PMX_PI * PMX_EV * c->a * PMX_ANGSTROM);
+    const double SF_TT = 32.0 * c->bz * std::pow(PMX_PI, 3) * std::pow(PMX_HBAR, 4) * std::pow(PMX_E, 2) /
+                         (std::pow(PMX_EV, 3) * std::pow(PMX_EMASS, 2) * std::pow(c->a * PMX_ANGSTROM, 5));
+    const double SF_LT = 8.0 * c->bz * PMX_PI * std::pow(PMX_HBAR, 2) * std::pow(PMX_E, 2) /
+                         (std::pow(PMX_EV, 2) * PMX_EMASS * std::pow(c->a * PMX_ANGSTROM, 3));
     const double w0 = 0.0, df = p->dfreq;
-    const int wh = window_half(p->delta, p->epsilon, df);
-    const int nbins = p->nfreq + 2 * wh;
+    /* epsij: window f0 - wh .. f0 + wh, bin = f0 + wh.  epsLL has no window
+     * (every transition at every frequency): bin = f0, sized per q. */
+    int wh = window_half(p->delta, p->epsilon, df);
+    int off_b = wh, nbins = p->nfreq + 2 * wh, R0 = 0, W = 2 * wh + 1;
     int NT = -1;
-    bool direct = p->precision == PMX_PREC_DIRECT || p->delta >= 2;
+    /* chi_LL's principal-value term (kk == 0) is not smooth at dE = w_f: per transition */
+    bool direct = p->precision == PMX_PREC_DIRECT || p->delta >= 2 || (ll && !p->kk);
     if (!direct) {
         NT = taylor_order(p->delta, p->epsilon, df,
                           (p->precision == PMX_PREC_FP64) ? 1e-11 : 1e-6);
@@ -1356,19 +1474,25 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
     if ((rc = w.nc.ensure(p->nk)) || (rc = w.nv.ensure(p->nk)) || (rc = w.tbase.ensure(p->nk + 1)) ||
         (rc = w.bad.ensure(1)))
         return rc;
-    if ((rc = w.M1.ensure((size_t)nbins * std::max(NT, 1) * npairs * 256))) return rc;
-    if ((rc = w.Z1.ensure((size_t)p->nfreq * npairs * 256))) return rc;
-    if (!spatial) {
-        if ((rc = w.M2.ensure((size_t)nbins * std::max(NT, 1) * npairs * 256))) return rc;
-        if ((rc = w.Z2.ensure((size_t)p->nfreq * npairs * 256))) return rc;
-    }
-    std::vector<double> Kt;
-    if (!direct) {
-        taylor_table(p->delta, p->epsilon, df, wh, NT, Kt);
-        if ((rc = upload(st, w.Ktd, Kt))) return rc;
-        std::vector<float> Ktf(Kt.begin(), Kt.end());
-        if ((rc = upload(st, w.Ktf, Ktf))) return rc;
-    }
+    const int nacc = (spatial || ll) ? 1 : 2; /* rank-1 terms per transition */
+    const bool pv = ll && !p->kk;
+    /* moment buffers and the Taylor table for the current binning */
+    auto setup_bins = [&]() -> int {
+        int r;
+        if ((r = w.M1.ensure((size_t)nbins * std::max(NT, 1) * npairs * 256))) return r;
+        if ((r = w.Z1.ensure((size_t)p->nfreq * npairs * 256))) return r;
+        if (nacc == 2 && (r = w.M2.ensure((size_t)nbins * std::max(NT, 1) * npairs * 256))) return r;
+        if ((nacc == 2 || pv) && (r = w.Z2.ensure((size_t)p->nfreq * npairs * 256))) return r;
+        if (!direct) {
+            std::vector<double> Kt;
+            taylor_table(p->delta, p->epsilon, df, -(R0 + off_b), W, NT, Kt);
+            if ((r = upload(st, w.Ktd, Kt))) return r;
+            std::vector<float> Ktf(Kt.begin(), Kt.end());
+            if ((r = upload(st, w.Ktf, Ktf))) return r;
+        }
+        return 0;
+    };
+    if (!ll && (rc = setup_bins())) return rc;
     if (p->kk) {
         if ((rc = w.Kkd.ensure((size_t)p->nfreq * p->nfreq))) return rc;
         hipLaunchKernelGGL(kkmat_kernel<double>, dim3(256), dim3(256), 0, st, p->nfreq, df, w.Kkd.p);
@@ -1422,13 +1546,33 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
             return rc;
         hipLaunchKernelGGL(dE_kernel, dim3(p->nk), dim3(256), 0, st, p->nk, nb_s, p->nband, spatial ? 1 : 0, w.evk.p,
                            w.evkq.p, p->offset, w.nc.p, w.nv.p, w.tbase.p, w.KWd.p, w.dEd.p, w.wtd.p);
+        if (ll) {
+            /* bins 0 .. max f0; the table spans f0 - f = -(nfreq - 1) .. max f0 */
+            int f0max = 0;
+            if (ntr > 0) {
+                PMX_CHECK_HIP(hipMemsetAsync(w.bad.p, 0, sizeof(int), st));
+                hipLaunchKernelGGL(f0max_kernel, dim3((ntr + 255) / 256), dim3(256), 0, st, ntr, w.dEd.p, w0, df,
+                                   w.bad.p);
+                PMX_CHECK_HIP(hipMemcpyAsync(&f0max, w.bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+                PMX_CHECK_HIP(hipStreamSynchronize(st));
+            }
+            off_b = 0;
+            wh = p->nfreq + f0max; /* the direct kernel's window f0 - wh .. f0 + wh: every frequency */
+            nbins = f0max + 1;
+            R0 = p->nfreq - 1;
+            W = p->nfreq + f0max;
+            if ((rc = setup_bins())) return rc;
+            if ((rc = w.hist.ensure(nbins)) || (rc = w.start.ensure(nbins + 1))) return rc;
+        }
 
         /* ---- bin by f0 (stable counting sort on the device); rows of a bin become contiguous */
+        if (sizeof(int) * (size_t)nbins > 65536)
+            return set_error(PMX_EINVAL, "more than 16384 frequency bins (transition energies / dfreq); raise -dfreq");
         const int nchunk = std::max((ntr + SB - 1) / SB, 1);
         if ((rc = w.cnt.ensure((size_t)nchunk * nbins))) return rc;
         if (ntr > 0) {
             hipLaunchKernelGGL(bin_count_kernel, dim3(nchunk), dim3(SB), sizeof(int) * nbins, st, ntr, w.dEd.p, w0, df,
-                               wh, nbins, w.bind.p, w.cnt.p);
+                               off_b, nbins, w.bind.p, w.cnt.p);
             hipLaunchKernelGGL(bin_prefix_kernel, dim3(nbins), dim3(256), 0, st, nchunk, nbins, w.cnt.p, w.hist.p);
             hipLaunchKernelGGL(bin_start_kernel, dim3(1), dim3(256), 0, st, nbins, w.hist.p, w.start.p);
             hipLaunchKernelGGL(bin_place_kernel, dim3(nchunk), dim3(SB), 0, st, ntr, nbins, w.bind.p, w.cnt.p,
@@ -1454,6 +1598,7 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         ra.K = w.Kd.p;
         ra.X = fp32 ? nullptr : w.Xd.p;
         ra.Xf = fp32 ? w.Xf.p : nullptr;
+        ra.ll = ll ? 1 : 0;
         if (ntr > 0) {
             /* algorithmic work of the gather-GEMM (src/pmx_chi.cpp:534-544): 8 flops per complex MAC,
              * 3 weighted (+1 plain) sums per (m, p) and transition; bytes = the eigenvector rows read */
@@ -1478,7 +1623,6 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         PMX_CHECK_HIP(hipGetLastError());
 
         /* ---- accumulate */
-        const int nacc = spatial ? 1 : 2;
         for (int acc = 0; acc < nacc; acc++) {
             DBuf<zd> &Z = acc == 0 ? w.Z1 : w.Z2;
             DBuf<zd> &M = acc == 0 ? w.M1 : w.M2;
@@ -1487,8 +1631,8 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
             if (ntr == 0) continue;
             if (direct) {
                 DirArgs da{D, nch, npairs, nbins, wh, p->nfreq, p->delta, acc, p->epsilon, w0, df,
-                           w.Xd.p, w.es.p, w.ws.p, w.start.p, Z.p};
-                Prof pf(c, "direct", 8.0 * 9.0 * ne2 * ntr * (2 * wh + 1) * (spatial ? 1 : 2), 0.0);
+                           w.Xd.p, w.es.p, w.ws.p, w.start.p, Z.p, off_b, 0};
+                Prof pf(c, "direct", 8.0 * 9.0 * ne2 * ntr * std::min(2 * wh + 1, p->nfreq) * nacc, 0.0);
                 hipLaunchKernelGGL(direct_kernel, dim3(npairs, nbins), dim3(256), 0, st, da);
             } else {
                 MomArgs2 ma{D, nch, npairs, nbins, NT, (NT + 3) / 4, acc, fp32 ? (const void *)w.Xf.p : (const void *)w.Xd.p,
@@ -1515,29 +1659,51 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                         hipLaunchKernelGGL(moment_mfma_kernel<double>, dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
                 }
                 {
-                    Prof pf(c, fp32 ? "conv_f32" : "conv_f64", 4.0 * 9.0 * ne2 * p->nfreq * (2 * wh + 1) * NT, 0.0);
+                    Prof pf(c, fp32 ? "conv_f32" : "conv_f64", 4.0 * 9.0 * ne2 * p->nfreq * (double)W * NT, 0.0);
                     const dim3 grid((npairs * 256 + 63) / 64, (p->nfreq + 16 * CONV_FB - 1) / (16 * CONV_FB));
-                    const size_t W = 2 * (size_t)wh + 1;
-                    const size_t lds_c = (fp32 ? sizeof(float) : sizeof(double)) * W * NT;
-                    if (lds_c > c->lds_max) return set_error(PMX_EINVAL, "broadening window too wide for the convolution's LDS table; use -precision direct");
+                    size_t lds_c = (fp32 ? sizeof(float) : sizeof(double)) * (size_t)W * NT;
+                    const int kt_lds = lds_c <= c->lds_max; /* a wider table is read from global memory (L2) */
+                    if (!kt_lds) lds_c = 0;
                     if (lds_c > 65536) {
                         const void *kf = fp32 ? (const void *)conv_mfma_kernel<float> : (const void *)conv_mfma_kernel<double>;
                         PMX_CHECK_HIP(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_c));
                     }
                     if (fp32) {
-                        ConvArgs ca{npairs, nbins, NT, wh, p->nfreq, M.p, w.Ktf.p, Z.p};
-                        hipLaunchKernelGGL(conv_mfma_kernel<float>, grid, dim3(256), sizeof(float) * W * NT, st, ca);
+                        ConvArgs ca{npairs, nbins, NT, R0, W, p->nfreq, M.p, w.Ktf.p, Z.p, kt_lds};
+                        hipLaunchKernelGGL(conv_mfma_kernel<float>, grid, dim3(256), lds_c, st, ca);
                     } else {
-                        ConvArgs ca{npairs, nbins, NT, wh, p->nfreq, M.p, w.Ktd.p, Z.p};
-                        hipLaunchKernelGGL(conv_mfma_kernel<double>, grid, dim3(256), sizeof(double) * W * NT, st, ca);
+                        ConvArgs ca{npairs, nbins, NT, R0, W, p->nfreq, M.p, w.Ktd.p, Z.p, kt_lds};
+                        hipLaunchKernelGGL(conv_mfma_kernel<double>, grid, dim3(256), lds_c, st, ca);
                     }
                 }
             }
             PMX_CHECK_HIP(hipGetLastError());
         }
+        if (pv) {
+            /* chi_LL's principal-value sums into Z2 */
+            PMX_CHECK_HIP(hipMemsetAsync(w.Z2.p, 0, sizeof(zd) * (size_t)p->nfreq * npairs * 256, st));
+            if (ntr > 0) {
+                DirArgs da{D, nch, npairs, nbins, wh, p->nfreq, p->delta, 0, p->epsilon, w0, df,
+                           w.Xd.p, w.es.p, w.ws.p, w.start.p, w.Z2.p, off_b, 1};
+                Prof pf(c, "direct_pv", 8.0 * 9.0 * ne2 * ntr * p->nfreq, 0.0);
+                hipLaunchKernelGGL(direct_kernel, dim3(npairs, nbins), dim3(256), 0, st, da);
+            }
+            PMX_CHECK_HIP(hipGetLastError());
+        }
 
         /* ---- scale + pack (+ KK) */
-        {
+        if (ll) {
+            Prof pf(c, "pack", 0.0, 16.0 * per_q);
+            std::vector<double> qgn(neps);
+            for (int m = 0; m < neps; m++) {
+                const double x = Qx + c->G[3 * m], y = Qy + c->G[3 * m + 1], z = Qz + c->G[3 * m + 2];
+                qgn[m] = std::sqrt(x * x + y * y + z * z);
+            }
+            if ((rc = upload(st, w.qgn, qgn))) return rc;
+            PackLLArgs pa{D, neps, npairs, p->nfreq, pv ? 1 : 0, SF_LL, SF_LT, SF_TT, df, w0, w.qgn.p, w.Z1.p,
+                          w.Z2.p, w.Re0.p, w.Im0.p};
+            hipLaunchKernelGGL(pack_ll_kernel, dim3(1024), dim3(256), 0, st, pa);
+        } else {
             Prof pf(c, "pack", 0.0, 16.0 * per_q);
             PackArgs pa{D, neps, npairs, p->nfreq, 1, 0, spatial ? 0 : 1, SF, df, w0, w.Z1.p,
                         spatial ? w.Z1.p : w.Z2.p, w.Re0.p, w.Im0.p};
@@ -1564,10 +1730,14 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
 }
 
 extern "C" int pmx_chi_device(pmx_ctx *c, const pmx_chi_params *p, double *dReX, double *dImX) {
-    return chi_run(c, p, dReX, dImX);
+    return chi_run(c, p, dReX, dImX, false);
 }
 
-extern "C" int pmx_chi(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double *ImX) {
+extern "C" int pmx_chi_ll_device(pmx_ctx *c, const pmx_chi_params *p, double *dReX, double *dImX) {
+    return chi_run(c, p, dReX, dImX, true);
+}
+
+static int chi_host(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double *ImX, bool ll) {
     if (!c || !p || !ReX || !ImX) return set_error(PMX_EINVAL, "null pointer");
     if (!c->has_system) return set_error(PMX_ESTATE, "no system set");
     if (p->nq < 0 || p->nfreq <= 0) return set_error(PMX_EINVAL, "bad sizes");
@@ -1576,10 +1746,18 @@ extern "C" int pmx_chi(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double 
     DBuf<double> r, i;
     int rc;
     if ((rc = r.ensure(n)) || (rc = i.ensure(n))) return rc;
-    if ((rc = chi_run(c, p, r.p, i.p))) return rc;
+    if ((rc = chi_run(c, p, r.p, i.p, ll))) return rc;
     PMX_CHECK_HIP(hipMemcpy(ReX, r.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     PMX_CHECK_HIP(hipMemcpy(ImX, i.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     return PMX_OK;
+}
+
+extern "C" int pmx_chi(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double *ImX) {
+    return chi_host(c, p, ReX, ImX, false);
+}
+
+extern "C" int pmx_chi_ll(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double *ImX) {
+    return chi_host(c, p, ReX, ImX, true);
 }
 
 extern "C" int pmx_kk(pmx_ctx *c, int nel, int nfreq, double dfreq, double *ReX, double *ImX) {

@@ -7,7 +7,10 @@
  *   epsij  -> ${wdir}/${outputfile}.dat  Re then Im, [i][j][q][m][n<=m][f]
  *             (export_Xij, src/pmx_io.cpp:720-741) and .gpt [NEPS][3] f64
  *             (export_gpt, src/pmx_io.cpp:816-826)
- * epsLL is accepted and, like the reference's main, computes nothing.
+ *   epsLL  -> the same two files from chi_LL, dispatched as in the older
+ *             snapshot's main (src_bkp/pmx.cpp:206-224: setRefEnergy, chi_LL,
+ *             export_Xij, export_gpt); the current main has no such branch
+ *             (src/pmx.cpp:171-203)
  */
 #include "pmx_internal.h"
 
@@ -40,8 +43,8 @@ extern "C" int pmx_run(pmx_ctx *ctx, const pmx_input *pin, int precision) {
     int rc = pmx_input_system(pin, &sys);
     if (rc) return rc;
     const std::string base = in.wdir + "/" + in.outputfile;
-    const bool band = in.sw == "band", epsij = in.sw == "epsij";
-    if (!band && !epsij) return PMX_OK; /* unknown, absent or epsLL switch: nothing, exit 0 */
+    const bool band = in.sw == "band", epsij = in.sw == "epsij", epsll = in.sw == "epsLL";
+    if (!band && !epsij && !epsll) return PMX_OK; /* unknown or absent switch: nothing, exit 0 */
     if (!ctx) return pmx::set_error(PMX_EINVAL, "null context");
     if ((rc = pmx_set_system(ctx, &sys))) return rc;
 
@@ -61,10 +64,10 @@ extern "C" int pmx_run(pmx_ctx *ctx, const pmx_input *pin, int precision) {
         return write_doubles(base + ".dat", {eb.data()}, eb.size());
     }
 
-    /* epsij */
+    /* epsij / epsLL */
     std::cout << "Susceptibility tensor matrix solver" << std::endl;
     auto t0 = clk::now();
-    if (!in.has_freq) return pmx::set_error(PMX_EINVAL, "-switch epsij needs both -nfreq and -dfreq");
+    if (!in.has_freq) return pmx::set_error(PMX_EINVAL, "-switch " + in.sw + " needs both -nfreq and -dfreq");
     double off = 0.0;
     const double rp[3] = {in.refpoint.x, in.refpoint.y, in.refpoint.z};
     if ((rc = pmx_ref_energy(ctx, rp, in.nvalence, &off))) return rc;
@@ -86,7 +89,7 @@ extern "C" int pmx_run(pmx_ctx *ctx, const pmx_input *pin, int precision) {
     const size_t ne2 = (size_t)in.neps * (in.neps + 1) / 2;
     const size_t n = 9 * (size_t)p.nq * ne2 * in.nfreq;
     std::vector<double> re(n), im(n);
-    if ((rc = pmx_chi(ctx, &p, re.data(), im.data()))) return rc;
+    if ((rc = (epsll ? pmx_chi_ll : pmx_chi)(ctx, &p, re.data(), im.data()))) return rc;
     if ((rc = write_doubles(base + ".dat", {re.data(), im.data()}, n))) return rc;
     if ((rc = write_doubles(base + ".gpt", {in.G.data()}, (size_t)3 * in.neps))) return rc;
     std::cout << "  Elapsed time: " << secs(t0, clk::now()) << " s" << std::endl;

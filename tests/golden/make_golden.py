@@ -190,7 +190,7 @@ CASES = {
     "zbfit_ll_lor_kk": dict(args=["-switch", "epsLL"] + ZBFIT +
                             ["-nband", "14", "-nvalence", "4", "-refpoint", "0.1,0.2,0.3",
                              "-qvec", "0.01,0,0,0.0,0.02,0.01", "-kpointfile", "{golden}/kpoints_zb_4.txt",
-                             "-neps", "4", "-nfreq", "200", "-dfreq", "0.08", "-delta", "1",
+                             "-neps", "4", "-nfreq", "300", "-dfreq", "0.05", "-delta", "1",
                              "-epsilon", "0.1", "-kk", "1"],
                             ref="bkp", kgrid=("zb", 4), keep=[".dat", ".gpt"]),
     # triangle broadening with the principal-value term

@@ -98,8 +98,6 @@ def test_cli_exit_codes(tmp_path):
     # unknown / absent switch: nothing computed, exit 0, no file
     r = subprocess.run([exe] + base + ["-switch", "nothing"], stdout=subprocess.PIPE)
     assert r.returncode == 0 and not list(tmp_path.iterdir())
-    r = subprocess.run([exe] + base + ["-switch", "epsLL"], stdout=subprocess.PIPE)
-    assert r.returncode == 0 and not list(tmp_path.iterdir())
     # -debug 2: dry run
     r = subprocess.run([exe] + base + ["-switch", "band", "-debug", "2"], stdout=subprocess.PIPE)
     assert r.returncode == 0 and not list(tmp_path.iterdir())
