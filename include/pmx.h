@@ -195,8 +195,9 @@ typedef struct {
     double epsilon, dfreq, gcut, encut, a, bz_volume;
     double refpoint[3];
     const double *Q;       /* [nq][3]                                        */
-    const double *K;       /* [nk][3] (NULL unless a k-point file was read)  */
+    const double *K;       /* [nk][3] (NULL when there are no k-points)      */
     const double *KW;      /* [nk]                                           */
+    int kpointorder;       /* -kpointorder of the -kpoint 1/2/3 generators   */
 } pmx_run_info;
 
 int pmx_input_info(const pmx_input *in, pmx_run_info *out);

@@ -50,7 +50,8 @@ class PmxRunInfo(ctypes.Structure):
                 ("nq", ctypes.c_int), ("nk", ctypes.c_int), ("kpoint", ctypes.c_int),
                 ("epsilon", ctypes.c_double), ("dfreq", ctypes.c_double), ("gcut", ctypes.c_double),
                 ("encut", ctypes.c_double), ("a", ctypes.c_double), ("bz_volume", ctypes.c_double),
-                ("refpoint", ctypes.c_double * 3), ("Q", _dp), ("K", _dp), ("KW", _dp)]
+                ("refpoint", ctypes.c_double * 3), ("Q", _dp), ("K", _dp), ("KW", _dp),
+                ("kpointorder", ctypes.c_int)]
 
 
 class PmxError(RuntimeError):

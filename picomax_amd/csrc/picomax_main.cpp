@@ -154,7 +154,17 @@ int main(int argc, char **argv) {
               << "---------------------------------" << std::endl;
     std::cout << "Generated Q-vector\n  Total number: " << ri.nq << std::endl;
     std::cout << "Generated G-vectors\n  Number of planewaves: " << ri.npw << std::endl;
-    if (ri.kpoint == 0) std::cout << "Imported K-vectors from kpointfile: " << value(argc, argv, "-kpointfile") << std::endl;
+    /* src/pmx.cpp:141-156 */
+    if (ri.kpoint == 0) {
+        std::cout << "Imported K-vectors from kpointfile: " << value(argc, argv, "-kpointfile") << std::endl;
+    } else if (ri.kpoint == 1) {
+        std::cout << "Generated K-vectors (Full BZ)\n  Simulation size in kpoint: " << ri.kpointorder << std::endl;
+    } else if (ri.kpoint == 2) {
+        std::cout << "Generated K-vectors (Monkhorst-Pack grids)\n  Simulation size in kpoint: " << ri.kpointorder
+                  << std::endl;
+    } else if (ri.kpoint == 3) {
+        std::cout << "Generated K-vectors\n  Using Cohen-Chadi grids with order: " << ri.kpointorder << std::endl;
+    }
     std::cout << "  Number of K-vectors: " << ri.nk << std::endl;
     if (ri.debug > 1) {
         std::cout << "number of electronic bands: " << ri.nband << "\nnumber of valence electrons: " << ri.nvalence

@@ -10,6 +10,7 @@
  *   G table              generate_gpt       src/pmx_basis.cpp:29-81
  *                        generate_gpt_rect  src/pmx_basis.cpp:86-132
  *   k-point file         import_kpt         src/pmx_basis.cpp:200-245
+ *   k-point grids        generate_kpt_*     src/pmx_basis.cpp:193-197, 256-653
  *   symmetry points      highsympoint_*     src/pmx_basis.cpp:722-788
  * Errors are returned as codes with a message instead of abort(); the CLI
  * turns the reference's abort paths back into abort().
@@ -23,6 +24,8 @@
 #include <fstream>
 #include <map>
 #include <numeric>
+#include <set>
+#include <tuple>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -331,16 +334,240 @@ int input_parse(int argc, const char *const *argv, Input &in) {
     if (npw < 0) return npw;
     in.npw = npw;
 
-    /* ---- k-points */
+    /* ---- k-points (src/pmx.cpp:141-154) */
     if (in.kpoint == 0) {
         int rc = import_kpt(in.kpointfile, in.K, in.KW);
         if (rc) return rc;
-    } else if (in.kpoint > 0) {
-        return set_error(PMX_EINVAL, "-kpoint " + std::to_string(in.kpoint) +
-                                         " grid generation is not provided; pass -kpointfile "
-                                         "(picomax_amd.kpoints writes Gamma-centred grids)");
-    }
+    } else if (in.kpoint == 1) {
+        generate_kpt_bz(in);
+    } else if (in.kpoint == 3) {
+        generate_kpt_cohen_chadi(in.kpointorder, in.K, in.KW);
+    } /* -kpoint 2 (Monkhorst-Pack) is an empty function in the reference: no k-points */
     return PMX_OK;
+}
+
+/* ------------------------------------------------------------------ k grids */
+namespace {
+
+constexpr double KTOL = 1e-6;
+
+/* checkinbz_fcc / checkonbz_fcc (src/pmx_basis.cpp:655-664, 695-706), in units of 2pi/a */
+bool in_bz_fcc(const V3 &k) {
+    return !(std::fabs(k.x) > 1.0 + KTOL || std::fabs(k.y) > 1.0 + KTOL || std::fabs(k.z) > 1.0 + KTOL ||
+             std::fabs(k.x) + std::fabs(k.y) + std::fabs(k.z) > 1.5 + KTOL);
+}
+bool on_bz_fcc(const V3 &k) {
+    return std::fabs(std::fabs(k.x) - 1.0) < KTOL || std::fabs(std::fabs(k.y) - 1.0) < KTOL ||
+           std::fabs(std::fabs(k.z) - 1.0) < KTOL ||
+           std::fabs(std::fabs(k.x) + std::fabs(k.y) + std::fabs(k.z) - 1.5) < KTOL;
+}
+/* checkinbz_hex / checkinbz_hex2 (src/pmx_basis.cpp:669-691) */
+bool in_bz_hex(const V3 &k, double f) {
+    const double r3 = std::sqrt(3.0);
+    return !(std::fabs(k.z) > 0.5 / f + KTOL || std::fabs(k.y) > 1.0 / r3 + KTOL ||
+             std::fabs(k.x) + std::fabs(k.y) / r3 > 1.0 / r3 + KTOL);
+}
+bool in_bz_hex2(const V3 &k) {
+    const double r3 = std::sqrt(3.0);
+    return !(std::fabs(k.y) > 1.0 / r3 + KTOL || std::fabs(k.x) + std::fabs(k.y) / r3 > 1.0 / r3 + KTOL);
+}
+
+/* s1 b1 + s2 b2 + s3 b3, summed left to right like the reference's vector expression */
+V3 lin3(const double *R, double s1, double s2, double s3) {
+    V3 v;
+    double *o = &v.x;
+    for (int d = 0; d < 3; d++) o[d] = s1 * R[d] + s2 * R[3 + d] + s3 * R[6 + d];
+    return v;
+}
+V3 add(const V3 &a, const V3 &b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+
+/* the 27 (9 in 2D) lattice shifts, first index slowest, tried in order until
+ * the point lands in the zone (src/pmx_basis.cpp:375-399, 516-534, 572-590) */
+template <class InZone> void fold_into_zone(const double *R, int dim, V3 &k, InZone in_zone) {
+    if (in_zone(k)) return;
+    for (int c1 = -1; c1 <= 1; c1++)
+        for (int c2 = -1; c2 <= 1; c2++)
+            for (int c3 = (dim == 2 ? 0 : -1); c3 <= (dim == 2 ? 0 : 1); c3++) {
+                V3 t = k;
+                const V3 s1{c1 * R[0], c1 * R[1], c1 * R[2]}, s2{c2 * R[3], c2 * R[4], c2 * R[5]};
+                t = add(add(t, s1), s2);
+                if (dim == 3) t = add(t, V3{c3 * R[6], c3 * R[7], c3 * R[8]});
+                if (in_zone(t)) {
+                    V3 sh = add(s1, s2);
+                    if (dim == 3) sh = add(sh, V3{c3 * R[6], c3 * R[7], c3 * R[8]});
+                    k = add(k, sh);
+                    return;
+                }
+            }
+}
+
+/* generate_kpt_fcc2 (src/pmx_basis.cpp:604-653): the (2N+1)^3 points
+ * (i1 b1 + i2 b2 + i3 b3)/N in the zone, weight 1 inside and 1/2 on its
+ * boundary, normalised */
+void kpt_fcc(const double *R, int N, std::vector<V3> &K, std::vector<double> &KW) {
+    const double df = 1.0 / N;
+    std::vector<V3> cand;
+    std::vector<double> w;
+    double tot = 0.0;
+    for (int i1 = -N; i1 <= N; i1++)
+        for (int i2 = -N; i2 <= N; i2++)
+            for (int i3 = -N; i3 <= N; i3++) {
+                const V3 k = lin3(R, df * i1, df * i2, df * i3);
+                const double wk = !in_bz_fcc(k) ? 0.0 : (on_bz_fcc(k) ? 0.5 : 1.0);
+                if (wk != 0.0) {
+                    cand.push_back(k);
+                    w.push_back(wk);
+                    tot += wk;
+                }
+            }
+    K = cand;
+    KW.resize(w.size());
+    for (size_t i = 0; i < w.size(); i++) KW[i] = w[i] / tot;
+}
+
+/* symmetrize_kpoints_xy_hex (src/pmx_basis.cpp:413-497): points keyed on a
+ * 1e-9 grid; every point and its x<->y mirror share their summed weight;
+ * output in key order, weights normalised */
+void symmetrize_xy(std::vector<V3> &K, std::vector<double> &KW) {
+    typedef std::tuple<double, double, double> Key;
+    const double tol = 1e-9;
+    auto key = [&](double x, double y, double z) {
+        return Key(std::round(x / tol) * tol, std::round(y / tol) * tol, std::round(z / tol) * tol);
+    };
+    std::map<Key, double> orig, sym;
+    for (size_t i = 0; i < K.size(); i++) orig[key(K[i].x, K[i].y, K[i].z)] += KW[i];
+    std::set<Key> done;
+    for (const auto &kv : orig) {
+        const Key &k = kv.first;
+        if (done.count(k)) continue;
+        const Key ks = key(std::get<1>(k), std::get<0>(k), std::get<2>(k));
+        done.insert(k);
+        done.insert(ks);
+        if (k == ks) {
+            sym[k] += kv.second;
+        } else {
+            auto it = orig.find(ks);
+            const double both = kv.second + (it != orig.end() ? it->second : 0.0);
+            sym[k] += both / 2.0;
+            sym[ks] += both / 2.0;
+        }
+    }
+    K.clear();
+    KW.clear();
+    double tot = 0.0;
+    for (const auto &kv : sym) {
+        K.push_back(V3{std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)});
+        KW.push_back(kv.second);
+        tot += kv.second;
+    }
+    if (tot > 0)
+        for (double &w : KW) w /= tot;
+}
+
+/* generate_kpt_hex (src/pmx_basis.cpp:499-556) and generate_kpt_hex2
+ * (:558-602): the N^dim grid of the reciprocal cell folded into the zone,
+ * uniform weights; in 3D then symmetrised in x<->y */
+void kpt_hex(const double *R, int N, int dim, double f, std::vector<V3> &K, std::vector<double> &KW) {
+    const double df = 1.0 / N;
+    K.clear();
+    for (int i1 = 0; i1 < N; i1++)
+        for (int i2 = 0; i2 < N; i2++)
+            for (int i3 = 0; i3 < (dim == 2 ? 1 : N); i3++) {
+                V3 k;
+                if (dim == 2) {
+                    /* two-term sum: b3 does not enter */
+                    k = V3{df * i1 * R[0] + df * i2 * R[3], df * i1 * R[1] + df * i2 * R[4],
+                           df * i1 * R[2] + df * i2 * R[5]};
+                    fold_into_zone(R, 2, k, [](const V3 &t) { return in_bz_hex2(t); });
+                } else {
+                    k = lin3(R, df * i1, df * i2, df * i3);
+                    fold_into_zone(R, 3, k, [f](const V3 &t) { return in_bz_hex(t, f); });
+                }
+                K.push_back(k);
+            }
+    KW.assign(K.size(), 1.0 / (double)K.size());
+    if (dim == 3) symmetrize_xy(K, KW);
+}
+
+} // namespace
+
+/* generate_kpt_bz (src/pmx_basis.cpp:347-358): fcc for zb/rs, hexagonal for
+ * wz/te, 2D hexagonal for hex; any other crystal gets no k-points */
+void generate_kpt_bz(Input &in) {
+    const int N = in.kpointorder;
+    in.K.clear();
+    in.KW.clear();
+    if (N <= 0) return;
+    if (in.crystal == "zb" || in.crystal == "rs")
+        kpt_fcc(in.recip, N, in.K, in.KW);
+    else if (in.crystal == "wz" || in.crystal == "te")
+        kpt_hex(in.recip, N, 3, in.f, in.K, in.KW);
+    else if (in.crystal == "hex")
+        kpt_hex(in.recip, N, 2, in.f, in.K, in.KW);
+}
+
+/* generate_kpt_cohen_chadi (src/pmx_basis.cpp:256-338): the special points of
+ * Cohen and Chadi, Phys. Rev. B 8, 5757 (1973), for orders 1 and 2, and the
+ * general order-N family of Philos. Mag. B 81, 551 (2001) above; coordinates
+ * (2i-1)/2^(N+1), weights (3(1-d_ij) + 3(1-d_jl) + d_il)/(4 8^(N-1)).  The
+ * point count is the formula's 2^(N-1)(2^N+1)(2^(N-1)+1)/3; slots the loops
+ * leave unfilled stay at the origin with weight 0, as in the reference. */
+void generate_kpt_cohen_chadi(int N, std::vector<V3> &K, std::vector<double> &KW) {
+    const int nk = (int)(std::pow(2.0, N - 1) * (std::pow(2.0, N) + 1.0) * (std::pow(2.0, N - 1.0) + 1.0) / 3.0);
+    K.assign(std::max(nk, 0), V3{0, 0, 0});
+    KW.assign(std::max(nk, 0), 0.0);
+    if (nk <= 0) return;
+    if (N == 1) {
+        static const double t1[2][4] = {{0.75, 0.75, 0.25, 0.25}, {0.25, 0.25, 0.25, 0.25}};
+        for (int i = 0; i < 2; i++) {
+            KW[i] = t1[i][0];
+            K[i] = V3{t1[i][1], t1[i][2], t1[i][3]};
+        }
+        return;
+    }
+    if (N == 2) {
+        /* weight, k in units of 2pi/a */
+        static const double t2[10][4] = {
+            {0.18750, 0.875, 0.375, 0.125}, {0.09375, 0.875, 0.125, 0.125}, {0.09375, 0.625, 0.625, 0.125},
+            {0.09375, 0.625, 0.375, 0.375}, {0.18750, 0.625, 0.375, 0.125}, {0.09375, 0.625, 0.125, 0.125},
+            {0.03125, 0.375, 0.375, 0.375}, {0.09375, 0.375, 0.375, 0.125}, {0.09375, 0.375, 0.125, 0.125},
+            {0.03125, 0.125, 0.125, 0.125}};
+        for (int i = 0; i < 10; i++) {
+            KW[i] = t2[i][0];
+            K[i] = V3{t2[i][1], t2[i][2], t2[i][3]};
+        }
+        return;
+    }
+    const int P = 1 << (N - 1), Q = 1 << N, T = 3 * (1 << (N - 2)), S = 3 * (1 << (N - 1));
+    const double den = 4.0 * std::pow(8.0, N - 1), sc = std::pow(2.0, N + 1);
+    int n = 0;
+    auto put = [&](int i, int j, int l) {
+        if (n >= nk) return; /* the formula's count bounds the table */
+        KW[n] = (3.0 * (i != j) + 3.0 * (j != l) + (double)(i == l)) / den;
+        K[n] = V3{(2.0 * i - 1) / sc, (2.0 * j - 1) / sc, (2.0 * l - 1) / sc};
+        n++;
+    };
+    for (int i = 1; i <= P; i++)
+        for (int j = 1; j <= i; j++)
+            for (int l = 1; l <= j; l++) put(i, j, l);
+    for (int i = P + 2; i <= Q; i += 2)
+        for (int j = 1; j <= T - i / 2; j++)
+            for (int l = 1; l <= j; l++) put(i, j, l);
+    for (int i = P + 2; i <= T; i += 2)
+        for (int j = T - i / 2 + 1; j <= i; j++)
+            for (int l = 1; l <= S + 1 - i - j; l++) put(i, j, l);
+    for (int i = T + 2; i <= Q; i += 2)
+        for (int j = T - i / 2 + 1; j <= S - i; j++)
+            for (int l = 1; l <= S + 1 - i - j; l++) put(i, j, l);
+    for (int i = P + 1; i <= Q - 1; i += 2)
+        for (int j = 1; j <= T - (i - 1) / 2; j++)
+            for (int l = 1; l <= j; l++) put(i, j, l);
+    for (int i = P + 1; i <= T - 1; i += 2)
+        for (int j = T - (i - 1) / 2 + 1; j <= i; j++)
+            for (int l = 1; l <= S + 1 - i - j; l++) put(i, j, l);
+    for (int i = T + 1; i <= Q - 1; i += 2)
+        for (int j = T - (i - 1) / 2 + 1; j <= S - i; j++)
+            for (int l = 1; l <= S + 1 - i - j; l++) put(i, j, l);
 }
 
 int import_kpt(const std::string &path, std::vector<V3> &K, std::vector<double> &KW) {
@@ -444,6 +671,7 @@ int pmx_input_info(const pmx_input *pin, pmx_run_info *r) {
     r->nq = (int)in.Q.size();
     r->nk = (int)in.K.size();
     r->kpoint = in.kpoint;
+    r->kpointorder = in.kpointorder;
     r->epsilon = in.epsilon;
     r->dfreq = in.dfreq;
     r->gcut = in.gcut;

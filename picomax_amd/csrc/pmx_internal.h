@@ -93,6 +93,8 @@ struct Input {
 int input_parse(int argc, const char *const *argv, Input &in);
 int generate_gpt_host(const double *recip, int dim, double gcut, bool rect, std::vector<double> &G);
 int import_kpt(const std::string &path, std::vector<V3> &K, std::vector<double> &KW);
+void generate_kpt_bz(Input &in);
+void generate_kpt_cohen_chadi(int order, std::vector<V3> &K, std::vector<double> &KW);
 
 } // namespace pmx
 

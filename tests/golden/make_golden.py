@@ -179,6 +179,25 @@ CASES = {
                     "-neps", "5", "-nfreq", "160", "-dfreq", "0.1", "-delta", "0",
                     "-epsilon", "0.2", "-kk", "1"],
                    kfile=("kpoints_wz_2.txt", WZ_K), keep=[".dat", ".gpt"]),
+    # chi on the generated grids: fcc zone (-kpoint 1), Cohen-Chadi (-kpoint 3), hexagonal zone
+    "si_chi_kbz": dict(args=["-switch", "epsij"] + SI + GC59 +
+                       ["-nband", "28", "-nvalence", "8", "-refpoint", "0.5,0.5,0.5",
+                        "-qvec", "0.01,0,0", "-kpoint", "1", "-kpointorder", "3",
+                        "-neps", "4", "-nfreq", "150", "-dfreq", "0.1", "-delta", "0",
+                        "-epsilon", "0.15", "-kk", "1"],
+                       keep=[".dat", ".gpt"]),
+    "si_chi_kcc": dict(args=["-switch", "epsij"] + SI + GC59 +
+                       ["-nband", "28", "-nvalence", "8", "-refpoint", "0.5,0.5,0.5",
+                        "-qvec", "0.01,0,0", "-kpoint", "3", "-kpointorder", "3",
+                        "-neps", "4", "-nfreq", "150", "-dfreq", "0.1", "-delta", "1",
+                        "-epsilon", "0.15", "-kk", "1"],
+                       keep=[".dat", ".gpt"]),
+    "wz_chi_kbz": dict(args=["-switch", "epsij"] + WZ +
+                       ["-gcut", "2.2", "-nband", "48", "-nvalence", "16", "-refpoint", "0,0,0",
+                        "-qvec", "0.0,0.0,0.012", "-kpoint", "1", "-kpointorder", "2",
+                        "-neps", "3", "-nfreq", "120", "-dfreq", "0.12", "-delta", "0",
+                        "-epsilon", "0.2", "-kk", "1"],
+                       keep=[".dat", ".gpt"]),
     # chi_LL (src_bkp/pmx_chi.cpp:126-439): Gaussian, principal-value real part (kk 0)
     "zbfit_ll_gauss_pv": dict(args=["-switch", "epsLL"] + ZBFIT +
                               ["-nband", "14", "-nvalence", "4", "-refpoint", "0.1,0.2,0.3",
@@ -201,6 +220,36 @@ CASES = {
                              "-epsilon", "0.3", "-kk", "0"],
                             ref="bkp", kgrid=("zb", 4), keep=[".dat", ".gpt"]),
 }
+
+
+# k-point generators alone: the reference's -debug 3 listing of its K vectors
+# (printed to 6 significant digits; src/pmx_io.cpp:580-595)
+KGEN = {
+    "zb_bz3": SI + ["-gcut", "2", "-kpoint", "1", "-kpointorder", "3"],
+    "zb_bz4": SI + ["-gcut", "2", "-kpoint", "1", "-kpointorder", "4"],
+    "zb_cc1": SI + ["-gcut", "2", "-kpoint", "3", "-kpointorder", "1"],
+    "zb_cc2": SI + ["-gcut", "2", "-kpoint", "3", "-kpointorder", "2"],
+    "zb_cc4": SI + ["-gcut", "2", "-kpoint", "3", "-kpointorder", "4"],
+    "zb_cc5": SI + ["-gcut", "2", "-kpoint", "3", "-kpointorder", "5"],
+    "zb_mp3": SI + ["-gcut", "2", "-kpoint", "2", "-kpointorder", "3"],
+    "wz_bz3": WZ + ["-gcut", "1.5", "-kpoint", "1", "-kpointorder", "3"],
+    "te_bz4": TE + ["-gcut", "1.5", "-kpoint", "1", "-kpointorder", "4"],
+    "hex_bz4": ["-crystal", "hex", "-a", "2.46", "-vg", "1.333333,-0.2,-0.2", "-gcut", "1.5",
+                "-kpoint", "1", "-kpointorder", "4"],
+}
+
+
+def write_kgen(ref):
+    out = {}
+    for name, args in KGEN.items():
+        r = subprocess.run([ref] + args + ["-debug", "3"], stdout=subprocess.PIPE, text=True)
+        lines = r.stdout.split("\n")
+        i = [j for j, l in enumerate(lines) if l.startswith("Number of K-vector grids")][0]
+        n = int(lines[i].split(":")[1])
+        out[name] = {"args": args, "K": [[float(x) for x in lines[i + 1 + j].split()] for j in range(n)]}
+    with open(os.path.join(HERE, "kgen.json"), "w") as fh:
+        json.dump(out, fh, indent=0, sort_keys=True)
+    print("wrote kgen.json", len(out))
 
 
 def spinor_args(args):
@@ -238,7 +287,11 @@ def main():
     ap.add_argument("--ref", default=os.path.join(ROOT, "oracle", "_ref", "picomax_ref"))
     ap.add_argument("--ref-bkp", default=os.path.join(ROOT, "oracle", "_ref", "picomax_ref_bkp"))
     ap.add_argument("--only", nargs="*")
+    ap.add_argument("--kgen", action="store_true", help="only rewrite kgen.json")
     opt = ap.parse_args()
+    if opt.kgen:
+        write_kgen(opt.ref)
+        return
     if not os.path.exists(opt.ref):
         sys.exit("reference binary missing: run `make -C oracle ref`")
     man_path = os.path.join(HERE, "manifest.json")

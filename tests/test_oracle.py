@@ -16,7 +16,7 @@ import cli_args
 import orc
 
 CHI_CASES = ["si_chi_c1", "si_chi_lor", "si_chi_tri", "si_chi_rect", "te_soc_chi", "te_usoc_chi",
-             "wz_chi"]
+             "wz_chi", "si_chi_kbz", "si_chi_kcc", "wz_chi_kbz"]
 BAND_CASES = ["si_band_c1", "gaas_band", "te_soc_band", "wz_band"]
 
 
@@ -72,6 +72,28 @@ def test_chi_fixture_is_well_posed(manifest, case):
         assert np.abs(ekq).min() > 1e-6
         assert np.all(nc >= (ekq <= 0).sum(1))
     assert np.abs(eb[:, :nb]).min() > 1e-6
+
+
+def _kgen():
+    import json
+    with open(cli_args.os.path.join(cli_args.GOLDEN, "kgen.json")) as fh:
+        return json.load(fh)
+
+
+@pytest.mark.parametrize("case", sorted(_kgen()))
+def test_kpoint_generators_match_reference(case):
+    """-kpoint 1 (zone grids: fcc, hexagonal with x<->y symmetrisation, 2D
+    hexagonal), 2 (Monkhorst-Pack: empty in the reference) and 3
+    (Cohen-Chadi) against the reference's -debug 3 listing (6 digits, same
+    order).  Their weights are pinned by the chi fixtures that use them."""
+    from picomax_amd import capi
+    e = _kgen()[case]
+    K, KW = capi.Input(e["args"]).K
+    ref = np.array(e["K"], float).reshape(-1, 3)
+    assert K.shape == ref.shape
+    if len(ref):
+        assert np.abs(K - ref).max() < 1e-5 * max(1.0, np.abs(ref).max())
+        assert abs(KW.sum() - 1.0) < 1e-12 and KW.min() > 0
 
 
 LL_CASES = ["zbfit_ll_gauss_pv", "zbfit_ll_lor_kk", "zbfit_ll_tri_pv"]
