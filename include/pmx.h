@@ -136,6 +136,17 @@ int pmx_chi_device(pmx_ctx *ctx, const pmx_chi_params *p, double *dReX, double *
 int pmx_chi_ll(pmx_ctx *ctx, const pmx_chi_params *p, double *ReX, double *ImX);
 int pmx_chi_ll_device(pmx_ctx *ctx, const pmx_chi_params *p, double *dReX, double *dImX);
 
+/* chi_xyz, the -switch epsxyz tensor (src_bkp/pmx_chi.cpp:723-1000 with
+ * dispatch src_bkp/pmx.cpp:226-244; src/pmx_chi_copy.cpp:1070-1344 is the same
+ * loop but copies only NPW components of its 2 NPW spinor eigenvectors):
+ * bra = valence state at k, ket = conduction state at k+q, vertex
+ * x_i.(G_p + K + Q/2 + G_m/2) summed over equal spins, every transition at
+ * every frequency, SCALEFACTOR (the TT one of chi_LL) times (pi delta-sum/w^2,
+ * zero at f = 0, + 2 principal value with 1/dE/(dE^2 - w^2) when kk == 0).
+ * Same parameters and output layout as pmx_chi. */
+int pmx_chi_xyz(pmx_ctx *ctx, const pmx_chi_params *p, double *ReX, double *ImX);
+int pmx_chi_xyz_device(pmx_ctx *ctx, const pmx_chi_params *p, double *dReX, double *dImX);
+
 /* Per-kernel timing with HIP events on the context's stream (no reference
  * counterpart).  enable != 0 starts recording (and clears), 0 stops.
  * pmx_profile_report writes a JSON object

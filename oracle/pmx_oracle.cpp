@@ -524,7 +524,11 @@ namespace {
 
 /* chi_tensor (src/pmx_chi.cpp:319-713) when ll == 0; chi_LL (the spinor form
  * of src/pmx_chi_copy.cpp:320-676, which src_bkp/pmx_chi.cpp:126-439 is for a
- * spatial basis) when ll == 1 */
+ * spatial basis) when ll == 1; chi_xyz (src_bkp/pmx_chi.cpp:723-1000 in the
+ * spinor basis: same-spin products, src/pmx_chi_copy.cpp:1070-1344 minus its
+ * spatial-size copies of spinor vectors) when ll == 2.  chi_xyz swaps the
+ * roles: the bra is a valence state at k, the ket a conduction state at k+q,
+ * dE = E_c(k+q) - E_v(k); its vertex is x_i.(G_p + K + Q/2 + G_m/2). */
 int chi_impl(const orc_system *sys, const int *loci, int neps, int nk, const double *K, const double *KW,
              int nq, const double *Q, int nband, double offset, int delta, double epsilon, int nfreq,
              double dfreq, int kk, double bz_volume, double *ReX, double *ImX, int ll) {
@@ -565,11 +569,13 @@ int chi_impl(const orc_system *sys, const int *loci, int neps, int nk, const dou
                 err = 1;
                 continue;
             }
+            /* bra states at k: conduction from the top, or with chi_xyz valence from the bottom */
+            const bool xyz = ll == 2;
             int nc = -1;
             Ek[k].assign(nband, 0.0);
             for (int c = 0; c < nband; c++) {
-                Ek[k][c] = w[nband - 1 - c] - offset;
-                if (Ek[k][c] <= 0) {
+                Ek[k][c] = w[xyz ? c : nband - 1 - c] - offset;
+                if (xyz ? Ek[k][c] > 0 : Ek[k][c] <= 0) {
                     nc = c;
                     break;
                 }
@@ -581,11 +587,13 @@ int chi_impl(const orc_system *sys, const int *loci, int neps, int nk, const dou
             }
             ncb[k] = nc;
             Ck[k].resize((size_t)nc * npw * 2);
-            for (int c = 0; c < nc; c++)
+            for (int c = 0; c < nc; c++) {
+                const int col = xyz ? c : nband - 1 - c;
                 for (int p = 0; p < npw; p++) {
-                    Ck[k][((size_t)c * npw + p) * 2] = Z[(size_t)(nband - 1 - c) * N2 + 2 * p];
-                    Ck[k][((size_t)c * npw + p) * 2 + 1] = Z[(size_t)(nband - 1 - c) * N2 + 2 * p + 1];
+                    Ck[k][((size_t)c * npw + p) * 2] = Z[(size_t)col * N2 + 2 * p];
+                    Ck[k][((size_t)c * npw + p) * 2 + 1] = Z[(size_t)col * N2 + 2 * p + 1];
                 }
+            }
         }
     }
     if (err == 1) return fail("eigensolver failed at k");
@@ -608,7 +616,7 @@ int chi_impl(const orc_system *sys, const int *loci, int neps, int nk, const dou
             cplx u[3][3];
             uvec_lt_impl(Qv + v3(sys->G + 3 * m), u);
             for (int i = 0; i < 3; i++)
-                for (int d = 0; d < 3; d++) uv[m][i][d] = u[i][d];
+                for (int d = 0; d < 3; d++) uv[m][i][d] = ll == 2 ? cplx(i == d ? 1.0 : 0.0) : u[i][d];
         }
         /* socCurrent cache per (m, active p) -- independent of k */
         std::vector<std::vector<std::array<cplx, 3>>> vsoc(neps);
@@ -648,11 +656,13 @@ int chi_impl(const orc_system *sys, const int *loci, int neps, int nk, const dou
                     errq = 1;
                     continue;
                 }
+                /* ket states at k+q: valence from the bottom, or with chi_xyz conduction from the top */
+                const bool xyz = ll == 2;
                 std::vector<double> Ekq(nband);
                 int nvb = -1;
                 for (int v = 0; v < nband; v++) {
-                    Ekq[v] = w[v] - offset;
-                    if (Ekq[v] > 0) {
+                    Ekq[v] = w[xyz ? nband - 1 - v : v] - offset;
+                    if (xyz ? Ekq[v] <= 0 : Ekq[v] > 0) {
                         nvb = v;
                         break;
                     }
@@ -691,13 +701,15 @@ int chi_impl(const orc_system *sys, const int *loci, int neps, int nk, const dou
                                                std::conj(u[2]) * vs[2];
                                     cplx cu = std::conj(Ckk[((size_t)c * npw + p) * 2]);
                                     cplx cd = std::conj(Ckk[((size_t)c * npw + p) * 2 + 1]);
-                                    cplx qu = Z[(size_t)v * N2 + 2 * lp];
-                                    cplx qd = Z[(size_t)v * N2 + 2 * lp + 1];
+                                    const size_t kc = (size_t)(ll == 2 ? nband - 1 - v : v) * N2;
+                                    cplx qu = Z[kc + 2 * lp];
+                                    cplx qd = Z[kc + 2 * lp + 1];
                                     su += cu * qu * orb;
                                     sd += cd * qd * orb;
                                     sud += cu * qd * soc;
                                     sdu += cd * qu * soc;
-                                    if (ll) sl += (cu * qu + cd * qd) * (i == 0 ? cplx(1.0) : orb + soc);
+                                    if (ll == 1) sl += (cu * qu + cd * qd) * (i == 0 ? cplx(1.0) : orb + soc);
+                                    if (ll == 2) sl += (cu * qu + cd * qd) * orb;
                                 }
                                 size_t id = ridx(i, m, c, v);
                                 if (ll) rll[id] = sl;
@@ -713,7 +725,7 @@ int chi_impl(const orc_system *sys, const int *loci, int neps, int nk, const dou
                     /* every transition at every frequency (no window) */
                     for (int c = 0; c < nc; c++)
                         for (int v = 0; v < nvb; v++) {
-                            const double dE = Ek[k][c] - Ekq[v];
+                            const double dE = ll == 2 ? Ekq[v] - Ek[k][c] : Ek[k][c] - Ekq[v];
                             for (int f = 0; f < nfreq; f++) {
                                 dl[f] = delta_fn(delta, epsilon, dE - freq[f]);
                                 if (pv) {
@@ -725,7 +737,9 @@ int chi_impl(const orc_system *sys, const int *loci, int neps, int nk, const dou
                             }
                             for (int i = 0; i < NTSR; i++)
                                 for (int j = 0; j < NTSR; j++) {
-                                    const double *g = (i == 0 && j == 0) ? gLL.data()
+                                    /* chi_xyz uses the TT kernel for every pair (src_bkp/pmx_chi.cpp:950-956) */
+                                    const double *g = ll == 2 ? gTT.data()
+                                                      : (i == 0 && j == 0) ? gLL.data()
                                                       : (i == 0 || j == 0) ? gLT.data() : gTT.data();
                                     for (int m = 0; m < neps; m++)
                                         for (int n = 0; n <= m; n++) {
@@ -815,7 +829,8 @@ int chi_impl(const orc_system *sys, const int *loci, int neps, int nk, const dou
                         si = 0.0;
                         sr = 0.0;
                     } else {
-                        const double sc = (i == 0 && j == 0) ? SF_LL / (nm * nn)
+                        const double sc = (ll == 2)              ? SF_TT / (wf * wf)
+                                          : (i == 0 && j == 0) ? SF_LL / (nm * nn)
                                           : (i == 0)         ? SF_LT / (nm * wf)
                                           : (j == 0)         ? SF_LT / nn
                                                              : SF_TT / (wf * wf);
@@ -823,7 +838,8 @@ int chi_impl(const orc_system *sys, const int *loci, int neps, int nk, const dou
                         sr *= sc;
                     }
                     if (pv) {
-                        const double sc = (i == 0 && j == 0) ? SF_LL / (nm * nn)
+                        const double sc = (ll == 2)              ? SF_TT
+                                          : (i == 0 && j == 0) ? SF_LL / (nm * nn)
                                           : (i == 0)         ? SF_LT / nm
                                           : (j == 0)         ? SF_LT / nn
                                                              : SF_TT;
@@ -863,6 +879,13 @@ int orc_chi_ll(const orc_system *sys, const int *loci, int neps, int nk, const d
                double dfreq, int kk, double bz_volume, double *ReX, double *ImX) {
     return chi_impl(sys, loci, neps, nk, K, KW, nq, Q, nband, offset, delta, epsilon, nfreq, dfreq, kk,
                     bz_volume, ReX, ImX, 1);
+}
+
+int orc_chi_xyz(const orc_system *sys, const int *loci, int neps, int nk, const double *K, const double *KW,
+                int nq, const double *Q, int nband, double offset, int delta, double epsilon, int nfreq,
+                double dfreq, int kk, double bz_volume, double *ReX, double *ImX) {
+    return chi_impl(sys, loci, neps, nk, K, KW, nq, Q, nband, offset, delta, epsilon, nfreq, dfreq, kk,
+                    bz_volume, ReX, ImX, 2);
 }
 
 } // extern "C"

@@ -91,6 +91,16 @@ int orc_chi_ll(const orc_system *sys, const int *loci, int neps,
                int delta, double epsilon, int nfreq, double dfreq,
                int kk, double bz_volume, double *ReX, double *ImX);
 
+/* chi_xyz, the -switch epsxyz tensor (src_bkp/pmx_chi.cpp:723-1000 in the
+ * spinor basis: same-spin products): bra = valence at k, ket = conduction at
+ * k+q, vertex x_i.(G_p + K + Q/2 + G_m/2), no window, SCALEFACTOR times
+ * (pi delta-sum / w^2 + 2 principal value with 1/dE/(dE^2 - w^2)). */
+int orc_chi_xyz(const orc_system *sys, const int *loci, int neps,
+                int nk, const double *K, const double *KW,
+                int nq, const double *Q, int nband, double offset,
+                int delta, double epsilon, int nfreq, double dfreq,
+                int kk, double bz_volume, double *ReX, double *ImX);
+
 /* kramerskronigtransform (src/pmx_chi.cpp:272-309) on one element. */
 void orc_kk(int nfreq, double dfreq, const double *w, double *ReX, double *ImX);
 

@@ -21,7 +21,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 # every symbol include/pmx.h declares
 EXPORTS = ["pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_set_system",
            "pmx_set_form_factors", "pmx_get_loci", "pmx_ref_energy", "pmx_bands", "pmx_eig_batch",
-           "pmx_chi", "pmx_chi_device", "pmx_chi_ll", "pmx_chi_ll_device", "pmx_profile", "pmx_profile_report", "pmx_kk", "pmx_epsinv", "pmx_grad_bands", "pmx_grad_chi", "pmx_input_parse", "pmx_input_free",
+           "pmx_chi", "pmx_chi_device", "pmx_chi_ll", "pmx_chi_ll_device", "pmx_chi_xyz", "pmx_chi_xyz_device", "pmx_profile", "pmx_profile_report", "pmx_kk", "pmx_epsinv", "pmx_grad_bands", "pmx_grad_chi", "pmx_input_parse", "pmx_input_free",
            "pmx_input_system", "pmx_input_info", "pmx_generate_gpt", "pmx_run"]
 
 
@@ -84,6 +84,8 @@ def lib():
     L.pmx_chi_device.argtypes = [vp, ctypes.POINTER(PmxChiParams), ctypes.c_void_p, ctypes.c_void_p]
     L.pmx_chi_ll.argtypes = [vp, ctypes.POINTER(PmxChiParams), _dp, _dp]
     L.pmx_chi_ll_device.argtypes = [vp, ctypes.POINTER(PmxChiParams), ctypes.c_void_p, ctypes.c_void_p]
+    L.pmx_chi_xyz.argtypes = [vp, ctypes.POINTER(PmxChiParams), _dp, _dp]
+    L.pmx_chi_xyz_device.argtypes = [vp, ctypes.POINTER(PmxChiParams), ctypes.c_void_p, ctypes.c_void_p]
     L.pmx_profile.argtypes = [vp, ctypes.c_int]
     L.pmx_profile_report.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
     L.pmx_kk.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp]
@@ -214,7 +216,7 @@ class Context:
         return (ev, vec) if vectors else ev
 
     def chi(self, K, KW, Q, nband, offset, delta, epsilon, nfreq, dfreq, kk=0, precision="fp64", ll=False):
-        """pmx_chi, or pmx_chi_ll (the epsLL tensor) with ll=True."""
+        """pmx_chi; pmx_chi_ll (the epsLL tensor) with ll=True or 1; pmx_chi_xyz (epsxyz) with ll=2."""
         K = _f64(K).reshape(-1, 3)
         KW = _f64(KW).reshape(-1)
         Q = _f64(Q).reshape(-1, 3)
@@ -225,7 +227,8 @@ class Context:
         n = 9 * len(Q) * ne2 * nfreq
         re = np.zeros(n)
         im = np.zeros(n)
-        _check((lib().pmx_chi_ll if ll else lib().pmx_chi)(self.h, ctypes.byref(p), _p(re), _p(im)))
+        fn = {0: lib().pmx_chi, 1: lib().pmx_chi_ll, 2: lib().pmx_chi_xyz}[int(ll)]
+        _check(fn(self.h, ctypes.byref(p), _p(re), _p(im)))
         return re, im
 
     def chi_device(self, params, dre_ptr, dim_ptr):

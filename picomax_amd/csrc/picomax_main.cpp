@@ -176,7 +176,7 @@ int main(int argc, char **argv) {
     }
     const std::string sw = ri.sw;
     pmx_ctx *ctx = nullptr;
-    if (sw == "band" || sw == "epsij" || sw == "epsLL") {
+    if (sw == "band" || sw == "epsij" || sw == "epsLL" || sw == "epsxyz") {
         const char *dev = getenv("PMX_DEVICE");
         if (pmx_create(dev ? std::atoi(dev) : 0, &ctx) != PMX_OK) {
             std::cout << "ERROR: " << pmx_last_error() << std::endl;

@@ -62,24 +62,30 @@ struct TransArgs {
     double offset;
     int *nc, *nv;             /* [nk] spinor counts */
     int *bad;                 /* set when a count is undefined */
+    int swap;                 /* chi_xyz: bra = valence at k, ket = conduction at k+q */
 };
 
 /* classification of chi_tensor (src/pmx_chi.cpp:391-397, 477-483) */
 __global__ void classify_kernel(TransArgs a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.nk) return;
+    /* nc counts the bra states (conduction at k from the top; with swap
+     * valence at k from the bottom, src_bkp/pmx_chi.cpp:779-786), nv the ket
+     * states (valence at k+q from the bottom; with swap conduction at k+q
+     * from the top, :827-833) */
     int nc = -1, nv = -1;
     for (int c = 0; c < a.nband; c++) {
-        int st = a.nband - 1 - c;
+        int st = a.swap ? c : a.nband - 1 - c;
         double e = a.evk[(size_t)k * a.nb_s + (a.spatial ? st / 2 : st)] - a.offset;
-        if (e <= 0) {
+        if (a.swap ? e > 0 : e <= 0) {
             nc = c;
             break;
         }
     }
     for (int v = 0; v < a.nband; v++) {
-        double e = a.evkq[(size_t)k * a.nb_s + (a.spatial ? v / 2 : v)] - a.offset;
-        if (e > 0) {
+        int st = a.swap ? a.nband - 1 - v : v;
+        double e = a.evkq[(size_t)k * a.nb_s + (a.spatial ? st / 2 : st)] - a.offset;
+        if (a.swap ? e <= 0 : e > 0) {
             nv = v;
             break;
         }
@@ -106,8 +112,20 @@ struct RhoArgs {
     const double *K;          /* [nk][3] */
     zd *X;                    /* FP64 rows (or null) */
     float2 *Xf;               /* FP32 rows (or null) */
-    int ll;                   /* chi_LL's vertex: density for i = 0, one channel with SOC */
+    int ll;                   /* 1: chi_LL's vertex (density for i = 0); 2: chi_xyz's Cartesian one;
+                                 either way one channel with SOC (same-spin products) */
+    int swap;                 /* chi_xyz: bra rows = valence at k, ket rows = conduction at k+q */
 };
+
+/* solve-basis rows of the bra (index c) and ket (index v) states of a transition */
+__device__ __forceinline__ int bra_row(const RhoArgs &a, int c) {
+    const int nb = a.spatial ? a.nband / 2 : a.nband;
+    return a.swap ? c : nb - 1 - c;
+}
+__device__ __forceinline__ int ket_row(const RhoArgs &a, int v) {
+    const int nb = a.spatial ? a.nband / 2 : a.nband;
+    return a.swap ? nb - 1 - v : v;
+}
 
 __device__ __forceinline__ void put_item(const RhoArgs &a, size_t row, int ch, int col, zd v) {
     const size_t idx = (row * a.nch + ch) * (size_t)(3 * a.neps) + col;
@@ -118,17 +136,18 @@ __device__ __forceinline__ void put_item(const RhoArgs &a, size_t row, int ch, i
 /* transition energies dE = E_c(k) - E_v(k+q) and weights (src/pmx_chi.cpp:587-590) */
 __global__ void dE_kernel(int nk, int nb_s, int nband, int spatial, const double *evk, const double *evkq,
                           double offset, const int *nc, const int *nv, const int *tbase, const double *KW,
-                          double *dE, double *wt) {
+                          double *dE, double *wt, int swap) {
     const int k = blockIdx.x;
     const int ncs = spatial ? nc[k] / 2 : nc[k];
     const int nvs = spatial ? nv[k] / 2 : nv[k];
     const size_t tb = tbase[k];
     for (int t = threadIdx.x; t < ncs * nvs; t += blockDim.x) {
         const int c = t / nvs, v = t % nvs;
-        const int cst = spatial ? (nband / 2 - 1 - c) : (nband - 1 - c);
-        double ec = evk[(size_t)k * nb_s + cst] - offset;
-        double ev = evkq[(size_t)k * nb_s + v] - offset;
-        dE[tb + t] = ec - ev;
+        const int nb = spatial ? nband / 2 : nband;
+        const int bst = swap ? c : nb - 1 - c, kst = swap ? nb - 1 - v : v;
+        double eb = evk[(size_t)k * nb_s + bst] - offset;
+        double ek = evkq[(size_t)k * nb_s + kst] - offset;
+        dE[tb + t] = swap ? ek - eb : eb - ek; /* E_c - E_v */
         wt[tb + t] = KW[k] * (spatial ? 2.0 : 1.0);
     }
 }
@@ -149,9 +168,8 @@ __global__ void __launch_bounds__(256) rho_kernel(RhoArgs a) {
         const int row = a.pos[tb + t];
         if (row < 0) continue;
         const int c = t / nvs, v = t % nvs;
-        const int cst = a.spatial ? (a.nband / 2 - 1 - c) : (a.nband - 1 - c);
-        const zd *ck = vk + (size_t)cst * a.N;
-        const zd *cq = vkq + (size_t)v * a.N;
+        const zd *ck = vk + (size_t)bra_row(a, c) * a.N;
+        const zd *cq = vkq + (size_t)ket_row(a, v) * a.N;
         const int *lm = a.loci + (size_t)m * a.npw;
         const zd *wA = a.wA + (size_t)m * 3 * a.npw;
         if (a.spatial) {
@@ -172,12 +190,14 @@ __global__ void __launch_bounds__(256) rho_kernel(RhoArgs a) {
                 const zd *u = a.uc + ((size_t)m * 3 + i) * 3;
                 zd bk{u[0].x * Kx + u[1].x * Ky + u[2].x * Kz, u[0].y * Kx + u[1].y * Ky + u[2].y * Kz};
                 zd bs = cmul(bk, S);
-                put_item(a, row, 0, 3 * m + i, (a.ll && i == 0) ? S : zd{R[i].x + bs.x, R[i].y + bs.y});
+                put_item(a, row, 0, 3 * m + i, (a.ll == 1 && i == 0) ? S : zd{R[i].x + bs.x, R[i].y + bs.y});
             }
         } else if (a.ll) {
             /* chi_LL with SOC (src/pmx_chi_copy.cpp:506-537): same-spin products only,
              * density for i = 0, u_i.(orbital + socCurrent) otherwise; one channel */
+            /* chi_xyz (ll == 2): u_i = x_i, orbital velocity only (src/pmx_chi_copy.cpp:1211-1216) */
             const zd *wS = a.wS + (size_t)m * 3 * a.npw;
+            const int i0 = a.ll == 1 ? 1 : 0;
             zd S{0, 0}, R[3] = {{0, 0}, {0, 0}, {0, 0}};
             for (int p = 0; p < a.npw; p++) {
                 const int l = lm[p];
@@ -185,14 +205,14 @@ __global__ void __launch_bounds__(256) rho_kernel(RhoArgs a) {
                 const zd puu = cmul(cconj(ck[2 * p]), cq[2 * l]), pdd = cmul(cconj(ck[2 * p + 1]), cq[2 * l + 1]);
                 const zd pr{puu.x + pdd.x, puu.y + pdd.y};
                 S.x += pr.x; S.y += pr.y;
-                for (int i = 1; i < 3; i++) {
-                    const zd wa = wA[i * a.npw + p], sw = wS[i * a.npw + p];
+                for (int i = i0; i < 3; i++) {
+                    const zd wa = wA[i * a.npw + p], sw = a.ll == 1 ? wS[i * a.npw + p] : zd{0, 0};
                     const zd t = cmul(pr, zd{wa.x + sw.x, wa.y + sw.y});
                     R[i].x += t.x; R[i].y += t.y;
                 }
             }
-            put_item(a, row, 0, 3 * m, S);
-            for (int i = 1; i < 3; i++) {
+            if (a.ll == 1) put_item(a, row, 0, 3 * m, S);
+            for (int i = i0; i < 3; i++) {
                 const zd *u = a.uc + ((size_t)m * 3 + i) * 3;
                 zd bk{u[0].x * Kx + u[1].x * Ky + u[2].x * Kz, u[0].y * Kx + u[1].y * Ky + u[2].y * Kz};
                 zd bs = cmul(bk, S);
@@ -288,11 +308,11 @@ template <class T> __global__ void __launch_bounds__(256) rho_mfma_kernel(RhoArg
     const zd *vk = a.vk + (size_t)k * a.nb_s * a.N;
     const zd *vkq = a.vkq + (size_t)k * a.nb_s * a.N;
     for (int idx = threadIdx.x; idx < ncs * a.N; idx += 256) {
-        const zd v = vk[(size_t)(a.nband / 2 - 1 - idx / a.N) * a.N + idx % a.N];
+        const zd v = vk[(size_t)bra_row(a, idx / a.N) * a.N + idx % a.N];
         sck[idx] = CT{(T)v.x, (T)v.y};
     }
     for (int idx = threadIdx.x; idx < nvs * a.N; idx += 256) {
-        const zd v = vkq[idx];
+        const zd v = vkq[(size_t)ket_row(a, idx / a.N) * a.N + idx % a.N];
         scq[idx] = CT{(T)v.x, (T)v.y};
     }
     __syncthreads();
@@ -367,7 +387,7 @@ template <class T> __global__ void __launch_bounds__(256) rho_mfma_kernel(RhoArg
                 const zd *u = a.uc + ((size_t)m * 3 + i) * 3;
                 zd bk{u[0].x * Kx + u[1].x * Ky + u[2].x * Kz, u[0].y * Kx + u[1].y * Ky + u[2].y * Kz};
                 zd bs = cmul(bk, S);
-                put_item(a, row, 0, 3 * m + i, (a.ll && i == 0) ? S : zd{R.x + bs.x, R.y + bs.y});
+                put_item(a, row, 0, 3 * m + i, (a.ll == 1 && i == 0) ? S : zd{R.x + bs.x, R.y + bs.y});
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -803,7 +823,8 @@ struct DirArgs {
     const int *start;
     zd *Z;                    /* [nfreq][npairs][256] */
     int off_b;                /* bin = f0 + off_b */
-    int pv;                   /* chi_LL's principal-value kernels instead of delta */
+    int pv;                   /* principal-value kernels instead of delta: 1 by element class
+                                 (chi_LL), 2 the TT kernel for every element (chi_xyz) */
 };
 
 __device__ double delta_fn(int code, double eps, double x) {
@@ -834,7 +855,7 @@ __global__ void __launch_bounds__(256) direct_kernel(DirArgs a) {
     const int ar = ti * TILE + r, ac = tj * TILE + cc;
     const int f0 = b - a.off_b;
     const int fmin = max(0, f0 - a.wh), fmax = min(a.nfreq - 1, f0 + a.wh);
-    const int cls = (ar % 3 == 0 && ac % 3 == 0) ? 0 : ((ar % 3 == 0 || ac % 3 == 0) ? 1 : 2);
+    const int cls = a.pv == 2 ? 2 : ((ar % 3 == 0 && ac % 3 == 0) ? 0 : ((ar % 3 == 0 || ac % 3 == 0) ? 1 : 2));
     __shared__ zd xa[32][TILE], xb[32][TILE];
     __shared__ double ws[3][32];
     const int per = (a.nch == 1) ? 32 : 16;
@@ -1021,7 +1042,7 @@ __global__ void pack_kernel(PackArgs a) {
  * f = 0), plus twice the principal-value sums (Zp) with their own factors;
  * Im <- pi tmp_imag_1 + 2 tmp_imag_2, Re <- pi tmp_real_1 + 2 tmp_real_2. */
 struct PackLLArgs {
-    int D, neps, npairs, nfreq, pv;
+    int D, neps, npairs, nfreq, pv, xyz; /* xyz: chi_xyz's single SCALEFACTOR (the TT one) for every element */
     double SF_LL, SF_LT, SF_TT, dfreq, w0;
     const double *qgn; /* [neps] |q + G_m| */
     const zd *Z1, *Zp;
@@ -1047,7 +1068,8 @@ __global__ void pack_ll_kernel(PackLLArgs a) {
             ti = 0.0;
             tr = 0.0;
         } else {
-            const double sc = (i == 0 && j == 0) ? a.SF_LL / (nm * nn)
+            const double sc = a.xyz               ? a.SF_TT / (w * w)
+                              : (i == 0 && j == 0) ? a.SF_LL / (nm * nn)
                               : (i == 0)         ? a.SF_LT / (nm * w)
                               : (j == 0)         ? a.SF_LT / nn
                                                  : a.SF_TT / (w * w);
@@ -1057,7 +1079,8 @@ __global__ void pack_ll_kernel(PackLLArgs a) {
         double pi_ = 0.0, pr = 0.0;
         if (a.pv) {
             const zd zp = zfetch(a.Zp, f, a.npairs, 3 * m + i, 3 * n + j);
-            const double sc = (i == 0 && j == 0) ? a.SF_LL / (nm * nn)
+            const double sc = a.xyz               ? a.SF_TT
+                              : (i == 0 && j == 0) ? a.SF_LL / (nm * nn)
                               : (i == 0)         ? a.SF_LT / nm
                               : (j == 0)         ? a.SF_LT / nn
                                                  : a.SF_TT;
@@ -1353,7 +1376,7 @@ static double chi_scale(const pmx_ctx *c) {
  * (src/pmx_basis.cpp:830-874), wA[m][i][p] = conj(u_mi).(G_p + Q/2 + G_m/2)
  * and, with SOC, wS[m][i][p] = conj(u_mi).socCurrent(G_p, G_loci) */
 static void q_tables(const pmx_ctx *c, double Qx, double Qy, double Qz, bool spatial, std::vector<zd> &uc,
-                     std::vector<zd> &wA, std::vector<zd> &wS) {
+                     std::vector<zd> &wA, std::vector<zd> &wS, bool xyz = false) {
     const int neps = c->neps, npw = c->npw;
     uc.assign((size_t)neps * 9, zd{0, 0});
     wA.assign((size_t)neps * 3 * npw, zd{0, 0});
@@ -1385,6 +1408,9 @@ static void q_tables(const pmx_ctx *c, double Qx, double Qy, double Qz, bool spa
             u[1][d] = std::complex<double>(T[d], P[d]) / r2;
             u[2][d] = std::complex<double>(T[d], -P[d]) / r2;
         }
+        if (xyz) /* chi_xyz's Cartesian unit vectors (src_bkp/pmx_chi.cpp:846-850) */
+            for (int i = 0; i < 3; i++)
+                for (int d = 0; d < 3; d++) u[i][d] = (i == d) ? 1.0 : 0.0;
         for (int i = 0; i < 3; i++)
             for (int d = 0; d < 3; d++) uc[((size_t)m * 3 + i) * 3 + d] = zd{u[i][d].real(), -u[i][d].imag()};
         for (int pp = 0; pp < npw; pp++) {
@@ -1395,7 +1421,7 @@ static void q_tables(const pmx_ctx *c, double Qx, double Qy, double Qz, bool spa
                 for (int d = 0; d < 3; d++) acc += std::conj(u[i][d]) * o[d];
                 wA[((size_t)m * 3 + i) * npw + pp] = zd{acc.real(), acc.imag()};
             }
-            if (!spatial) {
+            if (!spatial && !xyz) {
                 int l = c->loci_h[(size_t)m * npw + pp];
                 if (l < 0) continue;
                 /* socCurrent(G_p, G_l) with the map lambda_S (src/pmx_chi.cpp:203-266) */
@@ -1414,7 +1440,10 @@ static void q_tables(const pmx_ctx *c, double Qx, double Qy, double Qz, bool spa
     }
 }
 
-static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm, bool ll) {
+/* mode 0: chi_tensor (epsij); 1: chi_LL (epsLL); 2: chi_xyz (epsxyz) */
+static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm, int mode) {
+    const bool ll = mode != 0; /* no frequency window, one channel, chi_LL-style packing */
+    const bool xyz = mode == 2;
     if (!c || !p || !dRe || !dIm) return set_error(PMX_EINVAL, "null pointer");
     if (!c->has_system) return set_error(PMX_ESTATE, "no system set");
     if (p->nfreq <= 0 || !(p->dfreq > 0)) return set_error(PMX_EINVAL, "nfreq > 0 and dfreq > 0 required");
@@ -1469,7 +1498,7 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
     if ((rc = upload(st, w.Kd, Kh)) || (rc = upload(st, w.KWd, KWh))) return rc;
     if ((rc = w.evk.ensure((size_t)p->nk * nb_s)) || (rc = w.vk.ensure((size_t)p->nk * nb_s * N))) return rc;
     /* rho reads conduction rows at k and valence rows at k+q only */
-    if ((rc = eig_device(c, p->nk, w.Kd.p, nb_s, nb_s, w.evk.p, w.vk.p, 1, p->offset))) return rc;
+    if ((rc = eig_device(c, p->nk, w.Kd.p, nb_s, nb_s, w.evk.p, w.vk.p, xyz ? 2 : 1, p->offset))) return rc;
     if ((rc = w.evkq.ensure((size_t)p->nk * nb_s)) || (rc = w.vkq.ensure((size_t)p->nk * nb_s * N))) return rc;
     if ((rc = w.nc.ensure(p->nk)) || (rc = w.nv.ensure(p->nk)) || (rc = w.tbase.ensure(p->nk + 1)) ||
         (rc = w.bad.ensure(1)))
@@ -1505,7 +1534,7 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         const double Qx = p->Q[3 * q], Qy = p->Q[3 * q + 1], Qz = p->Q[3 * q + 2];
         /* ---- per-q tables: uvec_LT (src/pmx_basis.cpp:830-874), weights */
         std::vector<zd> uc, wA, wS;
-        q_tables(c, Qx, Qy, Qz, spatial, uc, wA, wS);
+        q_tables(c, Qx, Qy, Qz, spatial, uc, wA, wS, xyz);
         if ((rc = upload(st, w.ucd, uc)) || (rc = upload(st, w.wAd, wA))) return rc;
         if (!spatial && (rc = upload(st, w.wSd, wS))) return rc;
 
@@ -1517,11 +1546,12 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
             KQ[3 * k + 2] = Kh[3 * k + 2] + Qz;
         }
         if ((rc = upload(st, w.KQd, KQ))) return rc;
-        if ((rc = eig_device(c, p->nk, w.KQd.p, nb_s, nb_s, w.evkq.p, w.vkq.p, 2, p->offset))) return rc;
+        if ((rc = eig_device(c, p->nk, w.KQd.p, nb_s, nb_s, w.evkq.p, w.vkq.p, xyz ? 1 : 2, p->offset))) return rc;
 
         /* ---- classification + transition offsets */
         PMX_CHECK_HIP(hipMemsetAsync(w.bad.p, 0, sizeof(int), st));
-        TransArgs ta{p->nk, nb_s, p->nband, spatial ? 1 : 0, w.evk.p, w.evkq.p, p->offset, w.nc.p, w.nv.p, w.bad.p};
+        TransArgs ta{p->nk, nb_s, p->nband, spatial ? 1 : 0, w.evk.p, w.evkq.p, p->offset, w.nc.p, w.nv.p, w.bad.p,
+                     xyz ? 1 : 0};
         hipLaunchKernelGGL(classify_kernel, dim3((p->nk + 255) / 256), dim3(256), 0, st, ta);
         std::vector<int> nch_h(p->nk), nvh(p->nk), tb(p->nk + 1);
         int bad = 0;
@@ -1545,7 +1575,7 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
             (rc = w.es.ensure(ntr1)) || (rc = w.hist.ensure(nbins)) || (rc = w.start.ensure(nbins + 1)))
             return rc;
         hipLaunchKernelGGL(dE_kernel, dim3(p->nk), dim3(256), 0, st, p->nk, nb_s, p->nband, spatial ? 1 : 0, w.evk.p,
-                           w.evkq.p, p->offset, w.nc.p, w.nv.p, w.tbase.p, w.KWd.p, w.dEd.p, w.wtd.p);
+                           w.evkq.p, p->offset, w.nc.p, w.nv.p, w.tbase.p, w.KWd.p, w.dEd.p, w.wtd.p, xyz ? 1 : 0);
         if (ll) {
             /* bins 0 .. max f0; the table spans f0 - f = -(nfreq - 1) .. max f0 */
             int f0max = 0;
@@ -1598,7 +1628,8 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         ra.K = w.Kd.p;
         ra.X = fp32 ? nullptr : w.Xd.p;
         ra.Xf = fp32 ? w.Xf.p : nullptr;
-        ra.ll = ll ? 1 : 0;
+        ra.ll = mode;
+        ra.swap = xyz ? 1 : 0;
         if (ntr > 0) {
             /* algorithmic work of the gather-GEMM (src/pmx_chi.cpp:534-544): 8 flops per complex MAC,
              * 3 weighted (+1 plain) sums per (m, p) and transition; bytes = the eigenvector rows read */
@@ -1684,7 +1715,7 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
             PMX_CHECK_HIP(hipMemsetAsync(w.Z2.p, 0, sizeof(zd) * (size_t)p->nfreq * npairs * 256, st));
             if (ntr > 0) {
                 DirArgs da{D, nch, npairs, nbins, wh, p->nfreq, p->delta, 0, p->epsilon, w0, df,
-                           w.Xd.p, w.es.p, w.ws.p, w.start.p, w.Z2.p, off_b, 1};
+                           w.Xd.p, w.es.p, w.ws.p, w.start.p, w.Z2.p, off_b, xyz ? 2 : 1};
                 Prof pf(c, "direct_pv", 8.0 * 9.0 * ne2 * ntr * p->nfreq, 0.0);
                 hipLaunchKernelGGL(direct_kernel, dim3(npairs, nbins), dim3(256), 0, st, da);
             }
@@ -1700,8 +1731,8 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                 qgn[m] = std::sqrt(x * x + y * y + z * z);
             }
             if ((rc = upload(st, w.qgn, qgn))) return rc;
-            PackLLArgs pa{D, neps, npairs, p->nfreq, pv ? 1 : 0, SF_LL, SF_LT, SF_TT, df, w0, w.qgn.p, w.Z1.p,
-                          w.Z2.p, w.Re0.p, w.Im0.p};
+            PackLLArgs pa{D, neps, npairs, p->nfreq, pv ? 1 : 0, xyz ? 1 : 0, SF_LL, SF_LT, SF_TT, df, w0, w.qgn.p,
+                          w.Z1.p, w.Z2.p, w.Re0.p, w.Im0.p};
             hipLaunchKernelGGL(pack_ll_kernel, dim3(1024), dim3(256), 0, st, pa);
         } else {
             Prof pf(c, "pack", 0.0, 16.0 * per_q);
@@ -1730,14 +1761,18 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
 }
 
 extern "C" int pmx_chi_device(pmx_ctx *c, const pmx_chi_params *p, double *dReX, double *dImX) {
-    return chi_run(c, p, dReX, dImX, false);
+    return chi_run(c, p, dReX, dImX, 0);
 }
 
 extern "C" int pmx_chi_ll_device(pmx_ctx *c, const pmx_chi_params *p, double *dReX, double *dImX) {
-    return chi_run(c, p, dReX, dImX, true);
+    return chi_run(c, p, dReX, dImX, 1);
 }
 
-static int chi_host(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double *ImX, bool ll) {
+extern "C" int pmx_chi_xyz_device(pmx_ctx *c, const pmx_chi_params *p, double *dReX, double *dImX) {
+    return chi_run(c, p, dReX, dImX, 2);
+}
+
+static int chi_host(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double *ImX, int mode) {
     if (!c || !p || !ReX || !ImX) return set_error(PMX_EINVAL, "null pointer");
     if (!c->has_system) return set_error(PMX_ESTATE, "no system set");
     if (p->nq < 0 || p->nfreq <= 0) return set_error(PMX_EINVAL, "bad sizes");
@@ -1746,18 +1781,22 @@ static int chi_host(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double *Im
     DBuf<double> r, i;
     int rc;
     if ((rc = r.ensure(n)) || (rc = i.ensure(n))) return rc;
-    if ((rc = chi_run(c, p, r.p, i.p, ll))) return rc;
+    if ((rc = chi_run(c, p, r.p, i.p, mode))) return rc;
     PMX_CHECK_HIP(hipMemcpy(ReX, r.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     PMX_CHECK_HIP(hipMemcpy(ImX, i.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     return PMX_OK;
 }
 
 extern "C" int pmx_chi(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double *ImX) {
-    return chi_host(c, p, ReX, ImX, false);
+    return chi_host(c, p, ReX, ImX, 0);
 }
 
 extern "C" int pmx_chi_ll(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double *ImX) {
-    return chi_host(c, p, ReX, ImX, true);
+    return chi_host(c, p, ReX, ImX, 1);
+}
+
+extern "C" int pmx_chi_xyz(pmx_ctx *c, const pmx_chi_params *p, double *ReX, double *ImX) {
+    return chi_host(c, p, ReX, ImX, 2);
 }
 
 extern "C" int pmx_kk(pmx_ctx *c, int nel, int nfreq, double dfreq, double *ReX, double *ImX) {
@@ -2276,7 +2315,7 @@ extern "C" int pmx_grad_chi(pmx_ctx *c, const pmx_chi_params *p, const double *g
         if ((rc = upload(st, KQd, KQ))) return rc;
         if ((rc = eig_device(c, nk, KQd.p, N, N, evkq.p, vkq.p))) return rc;
         PMX_CHECK_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), st));
-        TransArgs ta{nk, N, p->nband, 1, evk.p, evkq.p, p->offset, nc.p, nv.p, bad.p};
+        TransArgs ta{nk, N, p->nband, 1, evk.p, evkq.p, p->offset, nc.p, nv.p, bad.p, 0};
         hipLaunchKernelGGL(classify_kernel, dim3((nk + 255) / 256), dim3(256), 0, st, ta);
         std::vector<int> nch_h(nk), nvh(nk), tb(nk + 1);
         int badh = 0;
@@ -2297,7 +2336,7 @@ extern "C" int pmx_grad_chi(pmx_ctx *c, const pmx_chi_params *p, const double *g
             (rc = gX.ensure((size_t)ntr * D)))
             return rc;
         hipLaunchKernelGGL(dE_kernel, dim3(nk), dim3(256), 0, st, nk, N, p->nband, 1, evk.p, evkq.p, p->offset, nc.p,
-                           nv.p, tbase.p, KWd.p, dEd.p, wtd.p);
+                           nv.p, tbase.p, KWd.p, dEd.p, wtd.p, 0);
         RhoArgs ra{};
         ra.nk = nk; ra.N = N; ra.npw = npw; ra.neps = neps; ra.nb_s = N; ra.nband = p->nband;
         ra.spatial = 1; ra.nch = 1;

@@ -11,6 +11,7 @@
  *             snapshot's main (src_bkp/pmx.cpp:206-224: setRefEnergy, chi_LL,
  *             export_Xij, export_gpt); the current main has no such branch
  *             (src/pmx.cpp:171-203)
+ *   epsxyz -> the same two files from chi_xyz (src_bkp/pmx.cpp:226-244)
  */
 #include "pmx_internal.h"
 
@@ -43,8 +44,9 @@ extern "C" int pmx_run(pmx_ctx *ctx, const pmx_input *pin, int precision) {
     int rc = pmx_input_system(pin, &sys);
     if (rc) return rc;
     const std::string base = in.wdir + "/" + in.outputfile;
-    const bool band = in.sw == "band", epsij = in.sw == "epsij", epsll = in.sw == "epsLL";
-    if (!band && !epsij && !epsll) return PMX_OK; /* unknown or absent switch: nothing, exit 0 */
+    const bool band = in.sw == "band", epsij = in.sw == "epsij", epsll = in.sw == "epsLL",
+               epsxyz = in.sw == "epsxyz";
+    if (!band && !epsij && !epsll && !epsxyz) return PMX_OK; /* unknown or absent switch: nothing, exit 0 */
     if (!ctx) return pmx::set_error(PMX_EINVAL, "null context");
     if ((rc = pmx_set_system(ctx, &sys))) return rc;
 
@@ -64,8 +66,9 @@ extern "C" int pmx_run(pmx_ctx *ctx, const pmx_input *pin, int precision) {
         return write_doubles(base + ".dat", {eb.data()}, eb.size());
     }
 
-    /* epsij / epsLL */
-    std::cout << "Susceptibility tensor matrix solver" << std::endl;
+    /* epsij / epsLL / epsxyz */
+    std::cout << (epsxyz ? "Susceptibility tensor matrix solver (xyz-basis)" : "Susceptibility tensor matrix solver")
+              << std::endl;
     auto t0 = clk::now();
     if (!in.has_freq) return pmx::set_error(PMX_EINVAL, "-switch " + in.sw + " needs both -nfreq and -dfreq");
     double off = 0.0;
@@ -89,7 +92,7 @@ extern "C" int pmx_run(pmx_ctx *ctx, const pmx_input *pin, int precision) {
     const size_t ne2 = (size_t)in.neps * (in.neps + 1) / 2;
     const size_t n = 9 * (size_t)p.nq * ne2 * in.nfreq;
     std::vector<double> re(n), im(n);
-    if ((rc = (epsll ? pmx_chi_ll : pmx_chi)(ctx, &p, re.data(), im.data()))) return rc;
+    if ((rc = (epsxyz ? pmx_chi_xyz : epsll ? pmx_chi_ll : pmx_chi)(ctx, &p, re.data(), im.data()))) return rc;
     if ((rc = write_doubles(base + ".dat", {re.data(), im.data()}, n))) return rc;
     if ((rc = write_doubles(base + ".gpt", {in.G.data()}, (size_t)3 * in.neps))) return rc;
     std::cout << "  Elapsed time: " << secs(t0, clk::now()) << " s" << std::endl;

@@ -103,7 +103,7 @@ def run(tokens):
     if p["switch"] == "band":
         return s.bands(p["Q"], p["nband"], off).ravel(), None
     re, im = s.chi(p["K"], p["KW"], p["Q"], p["nband"], off, p["delta"], p["epsilon"],
-                   p["nfreq"], p["dfreq"], p["kk"], ll=p["switch"] == "epsLL")
+                   p["nfreq"], p["dfreq"], p["kk"], ll={"epsLL": 1, "epsxyz": 2}.get(p["switch"], 0))
     return np.concatenate([re, im]), s.G[:p["neps"]].copy()
 
 

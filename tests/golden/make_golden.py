@@ -219,6 +219,19 @@ CASES = {
                              "-neps", "3", "-nfreq", "100", "-dfreq", "0.15", "-delta", "2",
                              "-epsilon", "0.3", "-kk", "0"],
                             ref="bkp", kgrid=("zb", 4), keep=[".dat", ".gpt"]),
+    # chi_xyz (src_bkp/pmx_chi.cpp:723-1000): valence at k to conduction at k+q, Cartesian vertex
+    "zbfit_xyz_lor_kk": dict(args=["-switch", "epsxyz"] + ZBFIT +
+                             ["-nband", "14", "-nvalence", "4", "-refpoint", "0.1,0.2,0.3",
+                              "-qvec", "0.01,0,0,0.0,0.02,0.01", "-kpointfile", "{golden}/kpoints_zb_4.txt",
+                              "-neps", "4", "-nfreq", "300", "-dfreq", "0.05", "-delta", "1",
+                              "-epsilon", "0.1", "-kk", "1"],
+                             ref="bkp", kgrid=("zb", 4), keep=[".dat", ".gpt"]),
+    "zbfit_xyz_gauss_pv": dict(args=["-switch", "epsxyz"] + ZBFIT +
+                               ["-nband", "18", "-nvalence", "4", "-refpoint", "0.1,0.2,0.3",
+                                "-qvec", "0.02,0.01,0", "-kpointfile", "{golden}/kpoints_zb_4.txt",
+                                "-neps", "3", "-nfreq", "150", "-dfreq", "0.1", "-delta", "0",
+                                "-epsilon", "0.2", "-kk", "0"],
+                               ref="bkp", kgrid=("zb", 4), keep=[".dat", ".gpt"]),
 }
 
 

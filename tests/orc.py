@@ -61,6 +61,7 @@ def lib():
                                      ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_double, _dp, _dp]
         L.orc_chi_ll.argtypes = L.orc_chi_tensor.argtypes
+        L.orc_chi_xyz.argtypes = L.orc_chi_tensor.argtypes
         L.orc_kk.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
         L.orc_delta.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.orc_delta.restype = ctypes.c_double
@@ -266,7 +267,8 @@ class System:
         return eb
 
     def chi(self, K, KW, Q, nband, offset, delta, epsilon, nfreq, dfreq, kk, ll=False):
-        """chi_tensor, or chi_LL with ll=True; [i][j][q][m][n<=m][f] Re and Im."""
+        """chi_tensor, chi_LL with ll=True (or ll=1), chi_xyz with ll=2;
+        [i][j][q][m][n<=m][f] Re and Im."""
         K = np.ascontiguousarray(np.array(K, float).reshape(-1, 3))
         KW = np.ascontiguousarray(np.array(KW, float))
         Q = np.ascontiguousarray(np.array(Q, float).reshape(-1, 3))
@@ -274,7 +276,7 @@ class System:
         n = 9 * len(Q) * ne2 * nfreq
         re = np.zeros(n)
         im = np.zeros(n)
-        fn = lib().orc_chi_ll if ll else lib().orc_chi_tensor
+        fn = {0: lib().orc_chi_tensor, 1: lib().orc_chi_ll, 2: lib().orc_chi_xyz}[int(ll)]
         _check(fn(ctypes.byref(self.c), self.loci.ctypes.data_as(_ip), self.neps,
                                     len(K), _p(K), _p(KW), len(Q), _p(Q), int(nband), float(offset),
                                     int(delta), float(epsilon), int(nfreq), float(dfreq), int(kk),

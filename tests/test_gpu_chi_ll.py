@@ -1,4 +1,5 @@
-"""GPU parity of chi_LL (pmx_chi_ll, -switch epsLL).
+"""GPU parity of chi_LL (pmx_chi_ll, -switch epsLL) and chi_xyz (pmx_chi_xyz,
+-switch epsxyz).
 
 * against the src_bkp reference build's epsLL fixtures: twice its output at
   doubled band counts, with its energy zero e[nvalence-1] at the reference
@@ -27,7 +28,8 @@ from picomax_amd import capi
 
 pytestmark = pytest.mark.gpu
 
-LL_CASES = ["zbfit_ll_gauss_pv", "zbfit_ll_lor_kk", "zbfit_ll_tri_pv"]
+LL_CASES = ["zbfit_ll_gauss_pv", "zbfit_ll_lor_kk", "zbfit_ll_tri_pv", "zbfit_xyz_lor_kk", "zbfit_xyz_gauss_pv"]
+MODE = {"epsLL": 1, "epsxyz": 2}
 
 
 def _expand(args):
@@ -66,22 +68,22 @@ def test_chi_ll_matches_reference(manifest, ctx, case, precision):
     nv = p["nvalence"]
     off = ctx.bands(np.array([p["refpoint"]]), nv, 0.0)[0][nv - 1]
     re, im = ctx.chi(p["K"], p["KW"], p["Q"], p["nband"], off, p["delta"], p["epsilon"], p["nfreq"], p["dfreq"],
-                     p["kk"], precision, ll=True)
+                     p["kk"], precision, ll=MODE[p["switch"]])
     err = cli_args.slice_error(np.concatenate([re, im]), _golden(case, e["scale"]), nslice)
     _say("  slice error %.3e" % err)
     assert err < (1e-5 if p["kk"] == 0 else 1e-8)
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
-def test_chi_ll_mixed(manifest, ctx, precision):
-    case = "zbfit_ll_lor_kk"
+@pytest.mark.parametrize("case", ["zbfit_ll_lor_kk", "zbfit_xyz_lor_kk"])
+def test_chi_ll_mixed(manifest, ctx, precision, case):
     e = manifest[case]
     _say("chi_LL %s %s" % (case, precision))
     s, p, nslice = _setup(ctx, e["args_spinor"])
     nv = p["nvalence"]
     off = ctx.bands(np.array([p["refpoint"]]), nv, 0.0)[0][nv - 1]
     re, im = ctx.chi(p["K"], p["KW"], p["Q"], p["nband"], off, p["delta"], p["epsilon"], p["nfreq"], p["dfreq"],
-                     p["kk"], precision, ll=True)
+                     p["kk"], precision, ll=MODE[p["switch"]])
     got, ref = np.concatenate([re, im]), _golden(case, e["scale"])
     nq, nf, ne2 = len(p["Q"]), p["nfreq"], p["neps"] * (p["neps"] + 1) // 2
     g, r = got.reshape(2, 9, nq, ne2, nf), ref.reshape(2, 9, nq, ne2, nf)
@@ -92,25 +94,27 @@ def test_chi_ll_mixed(manifest, ctx, precision):
 
 
 @pytest.mark.parametrize("kk", [1, 0])
-def test_chi_ll_soc_against_oracle(manifest, ctx, kk):
-    """Spinor basis with spin-orbit maps: same-spin products, u.(orbital +
-    socCurrent) vertex for the transverse components."""
+@pytest.mark.parametrize("switch", ["epsLL", "epsxyz"])
+def test_chi_ll_soc_against_oracle(manifest, ctx, kk, switch):
+    """Spinor basis with spin-orbit maps: same-spin products; chi_LL's
+    u.(orbital + socCurrent) vertex for the transverse components, chi_xyz's
+    orbital x_i vertex."""
     args = [a for a in manifest["te_soc_chi"]["args"]]
-    args[args.index("-switch") + 1] = "epsLL"
+    args[args.index("-switch") + 1] = switch
     args[args.index("-kk") + 1] = str(kk)
-    _say("chi_LL Te SOC kk %d" % kk)
+    _say("%s Te SOC kk %d" % (switch, kk))
     s, p, nslice = _setup(ctx, args)
     off = ctx.ref_energy(p["refpoint"], p["nvalence"])
     re, im = ctx.chi(p["K"], p["KW"], p["Q"], p["nband"], off, p["delta"], p["epsilon"], p["nfreq"], p["dfreq"],
-                     kk, "fp64", ll=True)
+                     kk, "fp64", ll=MODE[switch])
     r0, i0 = s.chi(p["K"], p["KW"], p["Q"], p["nband"], off, p["delta"], p["epsilon"], p["nfreq"], p["dfreq"], kk,
-                   ll=True)
+                   ll=MODE[switch])
     err = cli_args.slice_error(np.concatenate([re, im]), np.concatenate([r0, i0]), nslice)
     _say("  slice error %.3e" % err)
     assert err < (1e-5 if kk == 0 else 1e-8)
 
 
-@pytest.mark.parametrize("case", ["zbfit_ll_lor_kk", "zbfit_ll_gauss_pv"])
+@pytest.mark.parametrize("case", ["zbfit_ll_lor_kk", "zbfit_ll_gauss_pv", "zbfit_xyz_lor_kk"])
 def test_cli_epsll(manifest, case, tmp_path):
     """`picomax -switch epsLL` writes .dat/.gpt like -switch epsij; the energy
     zero is the current sources' mid-point one, so the oracle gives the bar."""

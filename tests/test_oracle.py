@@ -96,7 +96,8 @@ def test_kpoint_generators_match_reference(case):
         assert abs(KW.sum() - 1.0) < 1e-12 and KW.min() > 0
 
 
-LL_CASES = ["zbfit_ll_gauss_pv", "zbfit_ll_lor_kk", "zbfit_ll_tri_pv"]
+LL_CASES = ["zbfit_ll_gauss_pv", "zbfit_ll_lor_kk", "zbfit_ll_tri_pv", "zbfit_xyz_lor_kk", "zbfit_xyz_gauss_pv"]
+MODE = {"epsLL": 1, "epsxyz": 2}
 
 
 def ll_offset(s, p):
@@ -108,14 +109,14 @@ def ll_offset(s, p):
 
 @pytest.mark.parametrize("case", LL_CASES)
 def test_chi_ll_matches_reference(manifest, case):
-    """chi_LL against src_bkp's epsLL: twice its output at doubled band counts.
+    """chi_LL / chi_xyz against src_bkp's epsLL / epsxyz: twice its output at doubled band counts.
     With the principal-value term (kk 0) a transition within a hair of a
     frequency point makes the sum as sensitive to the eigenvalues' last bits
     (Eigen's against this solver's) as 1/|dE - w_f|: the bound is looser."""
     e = manifest[case]
     s, p = cli_args.build(e["args_spinor"])
     re, im = s.chi(p["K"], p["KW"], p["Q"], p["nband"], ll_offset(s, p), p["delta"], p["epsilon"],
-                   p["nfreq"], p["dfreq"], p["kk"], ll=True)
+                   p["nfreq"], p["dfreq"], p["kk"], ll=MODE[p["switch"]])
     ref = golden(case, ".dat") * e["scale"]
     nslice = 9 * len(p["Q"]) * p["neps"] * (p["neps"] + 1) // 2
     assert cli_args.slice_error(np.concatenate([re, im]), ref, nslice) < (1e-5 if p["kk"] == 0 else 1e-8)
