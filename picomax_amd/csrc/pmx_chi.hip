@@ -1352,9 +1352,9 @@ template <class T> int upload(hipStream_t s, DBuf<T> &b, const std::vector<T> &h
 } // namespace
 
 struct ChiWork {
-    DBuf<double> Kd, KQd, KWd, evk, evkq, dEd, wtd, Ktd, Kkd, Re0, Im0, ReD, ImD;
+    DBuf<double> Kd, KQd, KWd, evk, dEd, wtd, Ktd, Kkd, Re0, Im0, ReD, ImD; /* evk, vk: [k; k+q] halves */
     DBuf<double> ws, ss, es, qgn;
-    DBuf<zd> vk, vkq, Xd, M1, M2, Z1, Z2, wAd, ucd, wSd;
+    DBuf<zd> vk, Xd, M1, M2, Z1, Z2, wAd, ucd, wSd;
     DBuf<float2> Xf;
     DBuf<float> Ktf, Kkf;
     DBuf<int> nc, nv, tbase, bind, hist, start, cnt, pos, bad;
@@ -1492,14 +1492,25 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         PMX_CHECK_HIP(hipStreamSynchronize(st));
         return PMX_OK;
     }
-    /* ---- phase A: eigenpairs at k (src/pmx_chi.cpp:379-413) */
+    /* ---- phase A: eigenpairs at k (src/pmx_chi.cpp:379-413), solved in one
+     * batch with those at k+q of the first q: [K; K + Q_0].  Eigenpairs at k
+     * fill the first half of evk/vk, those at k+q the second. */
     std::vector<double> Kh(p->K, p->K + 3 * p->nk);
     std::vector<double> KWh(p->KW, p->KW + p->nk);
-    if ((rc = upload(st, w.Kd, Kh)) || (rc = upload(st, w.KWd, KWh))) return rc;
-    if ((rc = w.evk.ensure((size_t)p->nk * nb_s)) || (rc = w.vk.ensure((size_t)p->nk * nb_s * N))) return rc;
-    /* rho reads conduction rows at k and valence rows at k+q only */
-    if ((rc = eig_device(c, p->nk, w.Kd.p, nb_s, nb_s, w.evk.p, w.vk.p, xyz ? 2 : 1, p->offset))) return rc;
-    if ((rc = w.evkq.ensure((size_t)p->nk * nb_s)) || (rc = w.vkq.ensure((size_t)p->nk * nb_s * N))) return rc;
+    std::vector<double> K2(Kh);
+    if (p->nq > 0)
+        for (int k = 0; k < p->nk; k++)
+            for (int d = 0; d < 3; d++) K2.push_back(Kh[3 * k + d] + p->Q[d]);
+    if ((rc = upload(st, w.Kd, K2)) || (rc = upload(st, w.KWd, KWh))) return rc;
+    if ((rc = w.evk.ensure((size_t)2 * p->nk * nb_s)) || (rc = w.vk.ensure((size_t)2 * p->nk * nb_s * N))) return rc;
+    double *evk_p = w.evk.p, *evkq_p = w.evk.p + (size_t)p->nk * nb_s;
+    zd *vk_p = w.vk.p, *vkq_p = w.vk.p + (size_t)p->nk * nb_s * N;
+    /* rho reads bra rows at k (conduction; chi_xyz: valence) and ket rows at
+     * k+q (valence; chi_xyz: conduction) only */
+    const int vsel_k = xyz ? 2 : 1, vsel_kq = xyz ? 1 : 2;
+    if ((rc = eig_device(c, p->nq > 0 ? 2 * p->nk : p->nk, w.Kd.p, nb_s, nb_s, evk_p, vk_p, vsel_k, p->offset,
+                         p->nk, vsel_kq)))
+        return rc;
     if ((rc = w.nc.ensure(p->nk)) || (rc = w.nv.ensure(p->nk)) || (rc = w.tbase.ensure(p->nk + 1)) ||
         (rc = w.bad.ensure(1)))
         return rc;
@@ -1545,12 +1556,14 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
             KQ[3 * k + 1] = Kh[3 * k + 1] + Qy;
             KQ[3 * k + 2] = Kh[3 * k + 2] + Qz;
         }
-        if ((rc = upload(st, w.KQd, KQ))) return rc;
-        if ((rc = eig_device(c, p->nk, w.KQd.p, nb_s, nb_s, w.evkq.p, w.vkq.p, xyz ? 1 : 2, p->offset))) return rc;
+        if (q > 0) {
+            if ((rc = upload(st, w.KQd, KQ))) return rc;
+            if ((rc = eig_device(c, p->nk, w.KQd.p, nb_s, nb_s, evkq_p, vkq_p, vsel_kq, p->offset))) return rc;
+        }
 
         /* ---- classification + transition offsets */
         PMX_CHECK_HIP(hipMemsetAsync(w.bad.p, 0, sizeof(int), st));
-        TransArgs ta{p->nk, nb_s, p->nband, spatial ? 1 : 0, w.evk.p, w.evkq.p, p->offset, w.nc.p, w.nv.p, w.bad.p,
+        TransArgs ta{p->nk, nb_s, p->nband, spatial ? 1 : 0, evk_p, evkq_p, p->offset, w.nc.p, w.nv.p, w.bad.p,
                      xyz ? 1 : 0};
         hipLaunchKernelGGL(classify_kernel, dim3((p->nk + 255) / 256), dim3(256), 0, st, ta);
         std::vector<int> nch_h(p->nk), nvh(p->nk), tb(p->nk + 1);
@@ -1574,8 +1587,8 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
             (rc = w.pos.ensure(ntr1)) || (rc = w.ws.ensure(ntr1)) || (rc = w.ss.ensure(ntr1)) ||
             (rc = w.es.ensure(ntr1)) || (rc = w.hist.ensure(nbins)) || (rc = w.start.ensure(nbins + 1)))
             return rc;
-        hipLaunchKernelGGL(dE_kernel, dim3(p->nk), dim3(256), 0, st, p->nk, nb_s, p->nband, spatial ? 1 : 0, w.evk.p,
-                           w.evkq.p, p->offset, w.nc.p, w.nv.p, w.tbase.p, w.KWd.p, w.dEd.p, w.wtd.p, xyz ? 1 : 0);
+        hipLaunchKernelGGL(dE_kernel, dim3(p->nk), dim3(256), 0, st, p->nk, nb_s, p->nband, spatial ? 1 : 0, evk_p,
+                           evkq_p, p->offset, w.nc.p, w.nv.p, w.tbase.p, w.KWd.p, w.dEd.p, w.wtd.p, xyz ? 1 : 0);
         if (ll) {
             /* bins 0 .. max f0; the table spans f0 - f = -(nfreq - 1) .. max f0 */
             int f0max = 0;
@@ -1622,7 +1635,7 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         RhoArgs ra{};
         ra.nk = p->nk; ra.N = N; ra.npw = npw; ra.neps = neps; ra.nb_s = nb_s; ra.nband = p->nband;
         ra.spatial = spatial ? 1 : 0; ra.nch = nch;
-        ra.vk = w.vk.p; ra.vkq = w.vkq.p;
+        ra.vk = vk_p; ra.vkq = vkq_p;
         ra.nc = w.nc.p; ra.nv = w.nv.p; ra.tbase = w.tbase.p; ra.pos = w.pos.p; ra.loci = c->dLoci.p;
         ra.wA = w.wAd.p; ra.uc = w.ucd.p; ra.wS = spatial ? nullptr : w.wSd.p;
         ra.K = w.Kd.p;

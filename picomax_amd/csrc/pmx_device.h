@@ -131,9 +131,11 @@ struct Prof {
 /* eigensolve of nmat H(K) on the context's stream; K is a device array.
  * Eigenvalues of the lowest nb states; eigenvectors of the lowest nvec, or,
  * with vsel = 1 / 2, only of those above / at or below voff (the others are
- * left unwritten; honoured on the real path, else all nvec are formed) */
+ * left unwritten; honoured on the real path, else all nvec are formed).
+ * With nsplit >= 0 the matrices from index nsplit on select by vsel2 (1 or
+ * 2) instead: two batches with different selections in one launch. */
 int eig_device(pmx_ctx *ctx, int nmat, const double *dKpts, int nb, int nvec, double *dEvals, zd *dEvecs,
-               int vsel = 0, double voff = 0.0);
+               int vsel = 0, double voff = 0.0, int nsplit = -1, int vsel2 = 0);
 /* basis size of the solve (NPW spatial or 2 NPW spinor) */
 inline int basis_size(const pmx_ctx *c) { return c->soc ? 2 * c->npw : c->npw; }
 int rebuild_vtable(pmx_ctx *ctx);

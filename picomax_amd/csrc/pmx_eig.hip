@@ -775,9 +775,11 @@ __global__ void __launch_bounds__(256) bisect_kernel(int N, int nmat, int nb, in
  * vectors are wanted: vsel 1 keeps lambda > voff (conduction states), vsel 2
  * keeps lambda <= voff (valence states); the eigenvalues are ascending, so
  * either set is contiguous */
-__global__ void vrange_kernel(int nmat, int nvec, int vsel, double voff, const double *lamS, int *vr) {
+__global__ void vrange_kernel(int nmat, int nvec, int vsel, double voff, const double *lamS, int *vr, int m0,
+                              int nsplit, int vsel2) {
     const int mat = blockIdx.x * blockDim.x + threadIdx.x;
     if (mat >= nmat) return;
+    if (nsplit >= 0 && m0 + mat >= nsplit) vsel = vsel2; /* the batch's second part */
     int nbelow = 0;
     for (int j = 0; j < nvec; j++) nbelow += (lamS[(size_t)j * nmat + mat] <= voff);
     vr[2 * mat] = (vsel == 1) ? nbelow : 0;
@@ -1215,7 +1217,7 @@ template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, c
 } // namespace
 
 int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, double *dEvals, zd *dEvecs, int vsel,
-               double voff) {
+               double voff, int nsplit, int vsel2) {
     if (nmat <= 0) return 0;
     const int N = basis_size(c);
     if (nb > N) nb = N;
@@ -1228,10 +1230,10 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
     const bool lds_matrix = lds_bytes(N, elem, true) <= c->lds_max;
     const size_t lds = lds_bytes(N, elem, lds_matrix);
     if (lds > c->lds_max) return set_error(PMX_EINVAL, "basis too large for the eigensolver's LDS work arrays");
-    /* chunk the batch so that reflectors + SoA scratch stay near 4 GB */
+    /* chunk the batch so that reflectors + SoA scratch stay near 12 GB (of 288) */
     const size_t per_mat = (nvec > 0 ? P * elem + N * elem : 0) + sizeof(double) * (size_t)N * (2 + nb) +
                            sizeof(double) * (size_t)nvec * N * 4 + 64;
-    int chunk = (int)std::max<size_t>(64, std::min<size_t>((size_t)nmat, (size_t)4e9 / per_mat));
+    int chunk = (int)std::max<size_t>(64, std::min<size_t>((size_t)nmat, (size_t)12e9 / per_mat));
     const size_t slab = lds_matrix ? 0 : ((P * elem + 255) / 256) * 256;
     const int grid_max = c->num_cu * (lds_matrix ? 1 : 2) * 2;
     int rc;
@@ -1313,7 +1315,7 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
                 Prof pf(c, "invit", 0.0, 0.0);
                 if (vr)
                     hipLaunchKernelGGL(vrange_kernel, dim3((nm + 255) / 256), dim3(256), 0, c->stream, nm, nvec, vsel,
-                                       voff, lamS, vr);
+                                       voff, lamS, vr, m0, nsplit, vsel2);
                 const size_t nt = (size_t)nm * nvec;
                 hipLaunchKernelGGL(invit_kernel, dim3((nt + 255) / 256), dim3(256), 0, c->stream, N, nm, nvec, dS, eS,
                                    gb, lamS, vr, Y, work);
