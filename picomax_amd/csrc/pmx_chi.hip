@@ -962,19 +962,29 @@ template <class T> __global__ void __launch_bounds__(256) conv_mfma_kernel(ConvA
     };
     C m0 = mload(kq), m1 = mload(4 + kq);
     const int WN = W * a.NT;
+    /* per frequency block: the K rows of its band (wave-uniform, no division
+     * in the loop) and the table offset of this lane's frequency */
+    int klo[CONV_FB], khi[CONV_FB], toff[CONV_FB];
+    bool fok[CONV_FB];
+#pragma unroll
+    for (int q = 0; q < CONV_FB; q++) {
+        const int fq = fb0 + 16 * q;
+        const int blo = max(b_lo, fq - a.R0), bhi = min(b_hi, fq + 15 - a.R0 + W - 1);
+        klo[q] = (blo - b_lo) * a.NT;
+        khi[q] = (bhi - b_lo + 1) * a.NT - 1; /* below klo when the band misses the bins */
+        const int fa = fq + (lane & 15);      /* A row: frequency */
+        toff[q] = (b_lo - fa + a.R0) * a.NT;
+        fok[q] = fa < a.nfreq;
+    }
     for (int k0 = 0; k0 < kcount; k0 += 4) {
         const int kk = k0 + kq;
         const C m2 = mload(k0 + 8 + kq);
         const T br = m0.x, bi = m0.y;
-        /* bins of this k step (wave-uniform range) */
-        const int bs0 = b_lo + k0 / a.NT, bs1 = b_lo + min(k0 + 3, kcount - 1) / a.NT;
 #pragma unroll
         for (int q = 0; q < CONV_FB; q++) {
-            const int fq = fb0 + 16 * q;
-            if (bs1 < fq - a.R0 || bs0 > fq + 15 - a.R0 + W - 1) continue; /* outside this block's band */
-            const int fa = fq + (lane & 15);                                  /* A row: frequency */
-            const int ti = kk + (b_lo - fa + a.R0) * a.NT;
-            const T av = (kk < kcount && fa < a.nfreq && ti >= 0 && ti < WN) ? kt[ti] : T(0);
+            if (k0 + 3 < klo[q] || k0 > khi[q]) continue; /* outside this block's band */
+            const int ti = kk + toff[q];
+            const T av = (kk < kcount && fok[q] && ti >= 0 && ti < WN) ? kt[ti] : T(0);
             accR[q] = MfmaT<T>::mma(av, br, accR[q]);
             accI[q] = MfmaT<T>::mma(av, bi, accI[q]);
         }
