@@ -127,3 +127,46 @@ def test_chi_adjoint_rejects(setup):
         model.ctx.grad_chi(K, KW, Q, NBAND, off, 2, EPS, NFREQ, DF, gRe, gIm)
     with pytest.raises(capi.PmxError):  # odd nband splits a Kramers pair
         model.ctx.grad_chi(K, KW, Q, NBAND - 1, off, 1, EPS, NFREQ, DF, gRe, gIm)
+
+
+def test_eps_m_fit_end_to_end(setup):
+    """A loss on eps_M (local fields included: eps = 1 + chi over the i = j = 0
+    block, inverted per frequency in torch) back-propagates through
+    ChiResponse to the form factors; checked against central differences."""
+    import torch
+    from picomax_amd import autograd
+    name, model, K, KW, off, rng = setup
+    _say("eps_M loss end to end: %s" % name)
+    spec = autograd.ChiSpec(K, KW, Q[:1], NBAND, off, 1, EPS, NFREQ, DF, kk=1)
+    ne2 = NEPS * (NEPS + 1) // 2
+    iu = [(m, n) for m in range(NEPS) for n in range(m + 1)]
+    target = torch.from_numpy(rng.uniform(0.5, 1.5, NFREQ))
+
+    def eps_m_loss(v):
+        re, im = autograd.chi_response(v, model, spec)
+        blk = torch.complex(re, im).reshape(9, 1, ne2, NFREQ)[0, 0]  # i = j = 0 block
+        E = torch.zeros((NFREQ, NEPS, NEPS), dtype=torch.complex128)
+        for e, (m, n) in enumerate(iu):
+            E[:, m, n] = blk[e]
+            if m != n:
+                E[:, n, m] = blk[e].conj()
+        E = E + torch.eye(NEPS, dtype=torch.complex128)
+        epsM = 1.0 / torch.linalg.inv(E)[:, 0, 0]
+        return ((epsM.imag - target) ** 2).sum()
+
+    v = torch.tensor(model.vg0, dtype=torch.float64, requires_grad=True)
+    loss = eps_m_loss(v)
+    loss.backward()
+    g = v.grad.numpy().copy()
+    g_fd = np.zeros_like(g)
+    with torch.no_grad():
+        for a in range(g.shape[0]):
+            for s in range(g.shape[1]):
+                vp, vm = v.detach().clone(), v.detach().clone()
+                vp[a, s] += H
+                vm[a, s] -= H
+                g_fd[a, s] = (float(eps_m_loss(vp)) - float(eps_m_loss(vm))) / (2 * H)
+    model.set_vg(v.detach().numpy())
+    err = np.abs(g - g_fd).max() / np.abs(g_fd).max()
+    _say("  rel err %.3e" % err)
+    assert err < 1e-6
