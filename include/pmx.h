@@ -178,11 +178,13 @@ int pmx_grad_bands(pmx_ctx *ctx, int nk, const double *K, int nband, const doubl
  * the ReX/ImX that pmx_chi returns for the same p (same layout), returns
  * gvg [natom][nv] = d/dvg sum (gRe ReX + gIm ImX) of the exact windowed sum
  * in FP64, whatever p->precision says.  The energy zero p->offset is held
- * fixed (it only classifies states).  Spatial solve only (no SOC, even
- * nband) and Gaussian or Lorentzian broadening; PMX_EINVAL otherwise.
- * Eigenvector derivatives by first-order perturbation over the whole basis,
- * pairs closer than 1e-8 eV masked; all N eigenvectors of every k and k+q are
- * held on the device (16 N^2 bytes per k-point each). */
+ * fixed (it only classifies states).  Gaussian or Lorentzian broadening;
+ * nband even for the spatial solve (a Kramers pair is one solve state), any
+ * value up to 2 npw with spin-orbit maps; PMX_EINVAL otherwise.  Eigenvector
+ * derivatives by first-order perturbation over the whole basis (N = npw, or
+ * 2 npw spinor components with SOC), pairs closer than 1e-8 eV masked; all N
+ * eigenvectors of every k and k+q are held on the device (16 N^2 bytes per
+ * k-point each). */
 int pmx_grad_chi(pmx_ctx *ctx, const pmx_chi_params *p, const double *gRe, const double *gIm, double *gvg);
 
 /* --------------------------------------------------------- host set-up

@@ -1988,31 +1988,42 @@ extern "C" int pmx_grad_bands(pmx_ctx *c, int nk, const double *K, int nband, co
 
 /* ==================================================================
  * Adjoint of chi with respect to the form factors (no reference
- * counterpart; for fitting vg to measured spectra).  Spatial solve, FP64,
- * Gaussian or Lorentzian broadening: the gradient of the exact windowed sum
+ * counterpart; for fitting vg to measured spectra).  FP64, Gaussian or
+ * Lorentzian broadening: the gradient of the exact windowed sum
  * (src/pmx_chi.cpp:573-634) that the FP64 and direct paths compute.
  *
- * Forward, per q and element e = (ij, m, n<=m), a = 3m+i, b = 3n+j:
- *   Z_e(f) = sum_t w_t delta(dE_t - w_f) X_ta conj(X_tb),
- *   Im0 = SF/w_f^2 Re Z,  Re0 = -SF/w_f^2 Im Z,  then KK (linear in Re0, Im0).
+ * Forward, per q and element e = (ij, m, n<=m), a = 3m+i, b = 3n+j, with the
+ * rank-1 terms of each transition (spatial: X with weight 2 kw; SOC: the
+ * channels A, B, C, E of rho_kernel with signs +, -, +, -):
+ *   Z1_e(f) = sum_t w_t delta(dE_t - w_f) (A_a conj(A_b) - B_a conj(B_b)),
+ *   Z2_e(f) = sum_t w_t delta(dE_t - w_f) (C_a conj(C_b) - E_a conj(E_b)),
+ *   Im0 = SF/w_f^2 Re Z1,  Re0 = -SF/w_f^2 (Im Z1 + Im Z2)/2 (spatial: -Im Z1,
+ *   no Z2), then KK (linear in Re0, Im0).
  * With gz = dL/dRe z + i dL/dIm z for a complex z (dL = Re conj(gz) dz):
  *   (1) KK^T: the same GEMM with the Kramers-Kronig matrix transposed;
- *   (2) gZ_e(f) = SF/w_f^2 (gIm0 - i gRe0);
- *   (3) per transition, Gam_e = w sum_f delta(x_f) gZ_e(f) and Gam'_e with
- *       delta' (x_f = dE - w_f, window as the forward):
- *       g_dE = Re sum_e conj(Gam'_e) X_a conj(X_b),
- *       gX_a += Gam_e X_b,  gX_b += conj(Gam_e) X_a;
- *   (4) X_ta = sum_p conj(Ck_c[p]) Cq_v[l_m(p)] W_mi(p), W = wA + conj(u_mi).K:
- *       gCk_c[p] += conj(gX_a) Cq_v[l_m(p)] W_mi(p),
- *       gCq_v[l] += gX_a Ck_c[p] conj(W_mi(p)) with p = l_m^{-1}(l),
+ *   (2) G1_e(f) = SF/w_f^2 (gIm0 - i gRe0) spatial, SF/w_f^2 (gIm0 - i gRe0/2)
+ *       with SOC, and G2_e(f) = -i SF/w_f^2 gRe0/2;
+ *   (3) per transition, Gam_e = w sum_f delta(x_f) G_e(f) and Gam'_e with
+ *       delta' (x_f = dE - w_f, window as the forward); for a term
+ *       s Y_a conj(Y_b) (s = +-1) of Z:
+ *       g_dE += s Re conj(Gam'_e) Y_a conj(Y_b),
+ *       gY_a += s Gam_e Y_b,  gY_b += s conj(Gam_e) Y_a;
+ *   (4) rho = sum_p conj(c[p]) v[l_m(p)] weight(p) per spin pair:
+ *       spatial X with W = wA + conj(u_mi).K; SOC up, dn with W on equal
+ *       spins, ud (up-bra, down-ket), du with S = wS, and A = up + dn,
+ *       B = ud + du, C = up - dn, E = ud - du, so g_up = gA + gC,
+ *       g_dn = gA - gC, g_ud = gB + gE, g_du = gB - gE; then
+ *       gc[p] += conj(g_rho) v[l_m(p)] weight(p),
+ *       gv[l] += g_rho c[p] conj(weight(p)) with p = l_m^{-1}(l),
  *       gE_c(k) += g_dE,  gE_v(k+q) -= g_dE;
  *   (5) eigenvector adjoint (first-order perturbation theory; pairs closer
  *       than tol are masked, the phase gauge drops out of chi):
  *       y_n = sum_{m, |E_m - E_n| > tol} psi_m (psi_m^H g_n)/(E_n - E_m) + gE_n psi_n,
  *       so that dL = Re sum_n y_n^H dH psi_n;
- *   (6) dH_ij/dvg[s][sh] (i > j, shell_ij = sh) = Ry2eV sfac_ijs / NATOM and
- *       dH_ji its conjugate (src/pmx_epm.cpp:171-203), hence
- *       dL/dvg[s][sh] = Ry2eV/NATOM Re sum_{i>j} sfac_ijs (conj(y_i) psi_j + conj(psi_i) y_j).
+ *   (6) dH_ij/dvg[s][sh] (i > j, shell_ij = sh) = Ry2eV sfac_ijs / NATOM times
+ *       the spin identity, dH_ji its conjugate (src/pmx_epm.cpp:84, 171-203;
+ *       the SOC term does not depend on vg), hence
+ *       dL/dvg[s][sh] = Ry2eV/NATOM Re sum_{i>j} sfac_ijs sum_spin (conj(y_i) psi_j + conj(psi_i) y_j).
  * The wave vectors at k serve every q: their gradients accumulate over q and
  * (5)-(6) run once for them at the end.
  * ================================================================== */
@@ -2025,13 +2036,13 @@ __device__ double ddelta_fn(int code, double eps, double x) {
 }
 
 struct AdjTArgs {
-    int D, nact, nfreq, wh, delta;
+    int D, nch, nact, nfreq, wh, delta;
     double eps, w0, df;
-    const zd *X;         /* [ntr][D] rho in transition order */
+    const zd *X;         /* [ntr][nch][D] rho channels in transition order */
     const double *dE, *wt;
     const int *act;      /* [nact][2]: rows a, b of the active elements */
-    const zd *gZ;        /* [nact][nfreq] */
-    zd *gX;              /* [ntr][D] */
+    const zd *gZ;        /* [nact][nfreq] G1, then (SOC) [nact][nfreq] G2 */
+    zd *gX;              /* [ntr][nch][D] */
     double *gdE;         /* [ntr] */
 };
 
@@ -2039,11 +2050,12 @@ struct AdjTArgs {
 __global__ void __launch_bounds__(256) chi_adj_trans_kernel(AdjTArgs a) {
     const int t = blockIdx.x;
     extern __shared__ double adj_sm[];
-    zd *xs = reinterpret_cast<zd *>(adj_sm); /* [D] */
-    double *gr = adj_sm + 2 * a.D, *gi = gr + a.D;
+    const int CD = a.nch * a.D;
+    zd *xs = reinterpret_cast<zd *>(adj_sm); /* [nch][D] */
+    double *gr = adj_sm + 2 * CD, *gi = gr + CD;
     __shared__ double red[256];
-    for (int i = threadIdx.x; i < a.D; i += 256) {
-        xs[i] = a.X[(size_t)t * a.D + i];
+    for (int i = threadIdx.x; i < CD; i += 256) {
+        xs[i] = a.X[(size_t)t * CD + i];
         gr[i] = 0.0;
         gi[i] = 0.0;
     }
@@ -2051,28 +2063,37 @@ __global__ void __launch_bounds__(256) chi_adj_trans_kernel(AdjTArgs a) {
     const double dE = a.dE[t], w = a.wt[t];
     const int f0 = int((dE - a.w0) / a.df);
     const int fmin = max(0, f0 - a.wh), fmax = min(a.nfreq - 1, f0 + a.wh);
+    const int nz = a.nch == 1 ? 1 : 2; /* G1 (channels A, B), G2 (C, E) */
     double gde = 0.0;
     for (int e = threadIdx.x; e < a.nact && fmin <= fmax; e += 256) {
         const int ea = a.act[2 * e], eb = a.act[2 * e + 1];
-        const zd *G = a.gZ + (size_t)e * a.nfreq;
-        zd g0{0, 0}, g1{0, 0};
-        for (int f = fmin; f <= fmax; f++) {
-            const double x = dE - (a.w0 + a.df * f);
-            const double d0 = delta_fn(a.delta, a.eps, x), d1 = ddelta_fn(a.delta, a.eps, x);
-            const zd gz = G[f];
-            g0.x += d0 * gz.x; g0.y += d0 * gz.y;
-            g1.x += d1 * gz.x; g1.y += d1 * gz.y;
+        for (int z = 0; z < nz; z++) {
+            const zd *G = a.gZ + ((size_t)z * a.nact + e) * a.nfreq;
+            zd g0{0, 0}, g1{0, 0};
+            for (int f = fmin; f <= fmax; f++) {
+                const double x = dE - (a.w0 + a.df * f);
+                const double d0 = delta_fn(a.delta, a.eps, x), d1 = ddelta_fn(a.delta, a.eps, x);
+                const zd gz = G[f];
+                g0.x += d0 * gz.x; g0.y += d0 * gz.y;
+                g1.x += d1 * gz.x; g1.y += d1 * gz.y;
+            }
+            g0.x *= w; g0.y *= w;
+            g1.x *= w; g1.y *= w;
+            /* the channels this G multiplies: spatial X; SOC A (+), B (-) or C (+), E (-) */
+            const int nsub = a.nch == 1 ? 1 : 2;
+            for (int u = 0; u < nsub; u++) {
+                const int ch = a.nch == 1 ? 0 : 2 * z + u;
+                const double sg = u == 0 ? 1.0 : -1.0;
+                const zd xa = xs[ch * a.D + ea], xb = xs[ch * a.D + eb];
+                const zd O = cmulc(xa, xb);
+                gde += sg * (g1.x * O.x + g1.y * O.y);
+                const zd ga = cmul(g0, xb), gb = cmulc(xa, g0);
+                atomicAdd(&gr[ch * a.D + ea], sg * ga.x);
+                atomicAdd(&gi[ch * a.D + ea], sg * ga.y);
+                atomicAdd(&gr[ch * a.D + eb], sg * gb.x);
+                atomicAdd(&gi[ch * a.D + eb], sg * gb.y);
+            }
         }
-        g0.x *= w; g0.y *= w;
-        g1.x *= w; g1.y *= w;
-        const zd xa = xs[ea], xb = xs[eb];
-        const zd O = cmulc(xa, xb);
-        gde += g1.x * O.x + g1.y * O.y;
-        const zd ga = cmul(g0, xb), gb = cmulc(xa, g0);
-        atomicAdd(&gr[ea], ga.x);
-        atomicAdd(&gi[ea], ga.y);
-        atomicAdd(&gr[eb], gb.x);
-        atomicAdd(&gi[eb], gb.y);
     }
     red[threadIdx.x] = gde;
     __syncthreads();
@@ -2080,28 +2101,39 @@ __global__ void __launch_bounds__(256) chi_adj_trans_kernel(AdjTArgs a) {
         if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
-    for (int i = threadIdx.x; i < a.D; i += 256) a.gX[(size_t)t * a.D + i] = zd{gr[i], gi[i]};
+    for (int i = threadIdx.x; i < CD; i += 256) a.gX[(size_t)t * CD + i] = zd{gr[i], gi[i]};
     if (threadIdx.x == 0) a.gdE[t] = red[0];
 }
 
 struct AdjVArgs {
-    int N, npw, neps, nband, D, nbs;
+    int N, npw, neps, nband, D, nbs, soc;
     const int *nc, *nv, *tbase; /* spinor counts, transition base per k */
     const zd *vk, *vkq;         /* [nk][N][N] all states */
     const int *loci, *invl;     /* [neps][npw] and its inverse per m */
-    const zd *wA, *uc;
+    const zd *wA, *uc, *wS;     /* wS with SOC only */
     const double *K;
-    const zd *gX;
+    const zd *gX;               /* [ntr][nch][D] */
     const double *gdE;
     zd *gk, *gkq;               /* [nk][nbs][N], accumulated */
     double *gEk, *gEkq;         /* [nk][nbs], accumulated */
 };
 
+/* gradients of the spin-pair overlaps rho_up, rho_dn, rho_ud (up bra, down
+ * ket), rho_du of a transition from its channel gradients */
+__device__ __forceinline__ void spin_grads(const zd *gx, int D, int col, zd &gup, zd &gdn, zd &gud, zd &gdu) {
+    const zd gA = gx[col], gB = gx[D + col], gC = gx[2 * D + col], gE = gx[3 * D + col];
+    gup = zd{gA.x + gC.x, gA.y + gC.y};
+    gdn = zd{gA.x - gC.x, gA.y - gC.y};
+    gud = zd{gB.x + gE.x, gB.y + gE.y};
+    gdu = zd{gB.x - gE.x, gB.y - gE.y};
+}
+
 /* step (4): one block per k (it owns row k of every output) */
 __global__ void __launch_bounds__(256) chi_adj_evec_kernel(AdjVArgs a) {
     const int k = blockIdx.x;
-    const int ncs = a.nc[k] / 2, nvs = a.nv[k] / 2;
+    const int ncs = a.soc ? a.nc[k] : a.nc[k] / 2, nvs = a.soc ? a.nv[k] : a.nv[k] / 2;
     if (ncs * nvs == 0) return;
+    const int nch = a.soc ? 4 : 1, CD = nch * a.D;
     extern __shared__ double adjv_sm[];
     zd *bk = reinterpret_cast<zd *>(adjv_sm); /* [neps*3] conj(u_mi).K */
     const double Kx = a.K[3 * k], Ky = a.K[3 * k + 1], Kz = a.K[3 * k + 2];
@@ -2112,54 +2144,92 @@ __global__ void __launch_bounds__(256) chi_adj_evec_kernel(AdjVArgs a) {
     __syncthreads();
     const size_t tb = a.tbase[k];
     const zd *vk = a.vk + (size_t)k * a.N * a.N, *vkq = a.vkq + (size_t)k * a.N * a.N;
+    const int nst = a.soc ? a.nband : a.nband / 2; /* states in the solve basis */
+    /* bra (conduction at k): component (p, spin) */
     for (int idx = threadIdx.x; idx < ncs * a.npw; idx += 256) {
         const int c = idx / a.npw, p = idx % a.npw;
-        const int cst = a.nband / 2 - 1 - c;
-        zd s{0, 0};
+        const int cst = nst - 1 - c;
+        zd su{0, 0}, sd{0, 0};
         for (int v = 0; v < nvs; v++) {
-            const zd *gx = a.gX + (tb + (size_t)c * nvs + v) * a.D;
+            const zd *gx = a.gX + (tb + (size_t)c * nvs + v) * CD;
             const zd *cq = vkq + (size_t)v * a.N;
             for (int m = 0; m < a.neps; m++) {
                 const int l = a.loci[(size_t)m * a.npw + p];
                 if (l < 0) continue;
-                const zd q = cq[l];
                 for (int i = 0; i < 3; i++) {
-                    const zd wa = a.wA[((size_t)m * 3 + i) * a.npw + p], b = bk[3 * m + i];
+                    const int col = 3 * m + i;
+                    const zd wa = a.wA[((size_t)m * 3 + i) * a.npw + p], b = bk[col];
                     const zd W{wa.x + b.x, wa.y + b.y};
-                    const zd r = cmul(cconj(gx[3 * m + i]), cmul(q, W));
-                    s.x += r.x; s.y += r.y;
+                    if (!a.soc) {
+                        const zd r = cmul(cconj(gx[col]), cmul(cq[l], W));
+                        su.x += r.x; su.y += r.y;
+                    } else {
+                        zd gup, gdn, gud, gdu;
+                        spin_grads(gx, a.D, col, gup, gdn, gud, gdu);
+                        const zd S = a.wS[((size_t)m * 3 + i) * a.npw + p];
+                        const zd qu = cq[2 * l], qd = cq[2 * l + 1];
+                        /* up bra: rho_up (W, up ket), rho_ud (S, down ket); down bra: rho_dn, rho_du */
+                        const zd r1 = cmul(cconj(gup), cmul(qu, W)), r2 = cmul(cconj(gud), cmul(qd, S));
+                        const zd r3 = cmul(cconj(gdn), cmul(qd, W)), r4 = cmul(cconj(gdu), cmul(qu, S));
+                        su.x += r1.x + r2.x; su.y += r1.y + r2.y;
+                        sd.x += r3.x + r4.x; sd.y += r3.y + r4.y;
+                    }
                 }
             }
         }
-        zd &o = a.gk[((size_t)k * a.nbs + cst) * a.N + p];
-        o.x += s.x; o.y += s.y;
+        zd *o = a.gk + ((size_t)k * a.nbs + cst) * a.N;
+        if (!a.soc) {
+            o[p].x += su.x; o[p].y += su.y;
+        } else {
+            o[2 * p].x += su.x; o[2 * p].y += su.y;
+            o[2 * p + 1].x += sd.x; o[2 * p + 1].y += sd.y;
+        }
     }
+    /* ket (valence at k+q): component (l, spin) */
     for (int idx = threadIdx.x; idx < nvs * a.npw; idx += 256) {
         const int v = idx / a.npw, l = idx % a.npw;
-        zd s{0, 0};
+        zd su{0, 0}, sd{0, 0};
         for (int c = 0; c < ncs; c++) {
-            const int cst = a.nband / 2 - 1 - c;
-            const zd *gx = a.gX + (tb + (size_t)c * nvs + v) * a.D;
+            const int cst = nst - 1 - c;
+            const zd *gx = a.gX + (tb + (size_t)c * nvs + v) * CD;
             const zd *ck = vk + (size_t)cst * a.N;
             for (int m = 0; m < a.neps; m++) {
                 const int p = a.invl[(size_t)m * a.npw + l];
                 if (p < 0) continue;
-                const zd cp = ck[p];
                 for (int i = 0; i < 3; i++) {
-                    const zd wa = a.wA[((size_t)m * 3 + i) * a.npw + p], b = bk[3 * m + i];
+                    const int col = 3 * m + i;
+                    const zd wa = a.wA[((size_t)m * 3 + i) * a.npw + p], b = bk[col];
                     const zd Wc{wa.x + b.x, -(wa.y + b.y)};
-                    const zd r = cmul(gx[3 * m + i], cmul(cp, Wc));
-                    s.x += r.x; s.y += r.y;
+                    if (!a.soc) {
+                        const zd r = cmul(gx[col], cmul(ck[p], Wc));
+                        su.x += r.x; su.y += r.y;
+                    } else {
+                        zd gup, gdn, gud, gdu;
+                        spin_grads(gx, a.D, col, gup, gdn, gud, gdu);
+                        const zd Sv = a.wS[((size_t)m * 3 + i) * a.npw + p];
+                        const zd Sc{Sv.x, -Sv.y};
+                        const zd cu = ck[2 * p], cd = ck[2 * p + 1];
+                        /* up ket: rho_up (up bra, W), rho_du (down bra, S); down ket: rho_dn, rho_ud */
+                        const zd r1 = cmul(gup, cmul(cu, Wc)), r2 = cmul(gdu, cmul(cd, Sc));
+                        const zd r3 = cmul(gdn, cmul(cd, Wc)), r4 = cmul(gud, cmul(cu, Sc));
+                        su.x += r1.x + r2.x; su.y += r1.y + r2.y;
+                        sd.x += r3.x + r4.x; sd.y += r3.y + r4.y;
+                    }
                 }
             }
         }
-        zd &o = a.gkq[((size_t)k * a.nbs + v) * a.N + l];
-        o.x += s.x; o.y += s.y;
+        zd *o = a.gkq + ((size_t)k * a.nbs + v) * a.N;
+        if (!a.soc) {
+            o[l].x += su.x; o[l].y += su.y;
+        } else {
+            o[2 * l].x += su.x; o[2 * l].y += su.y;
+            o[2 * l + 1].x += sd.x; o[2 * l + 1].y += sd.y;
+        }
     }
     for (int c = threadIdx.x; c < ncs; c += 256) {
         double s = 0.0;
         for (int v = 0; v < nvs; v++) s += a.gdE[tb + (size_t)c * nvs + v];
-        a.gEk[(size_t)k * a.nbs + a.nband / 2 - 1 - c] += s;
+        a.gEk[(size_t)k * a.nbs + nst - 1 - c] += s;
     }
     for (int v = threadIdx.x; v < nvs; v += 256) {
         double s = 0.0;
@@ -2215,8 +2285,8 @@ __global__ void __launch_bounds__(256) adj_y_kernel(int N, int nbs, double tol, 
 }
 
 /* step (6): one block per (k, n); the adjoint form of hf_grad_kernel */
-__global__ void __launch_bounds__(256) hf_adj_kernel(int npw, int N, int natom, int nv, int nbs, const zd *psi,
-                                                     const zd *y, const int *live, const int *shell,
+__global__ void __launch_bounds__(256) hf_adj_kernel(int npw, int N, int natom, int nv, int nbs, int soc,
+                                                     const zd *psi, const zd *y, const int *live, const int *shell,
                                                      const zd *sfac, double *gvg) {
     if (!live[blockIdx.x]) return;
     extern __shared__ double acc[]; /* [natom * nv] */
@@ -2231,8 +2301,13 @@ __global__ void __launch_bounds__(256) hf_adj_kernel(int npw, int N, int natom, 
         if (i <= j) continue;
         const int sh = shell[id];
         if (sh < 0) continue;
-        const zd a = cmul(cconj(yn[i]), ps[j]), b = cmul(cconj(ps[i]), yn[j]);
-        const zd pr{a.x + b.x, a.y + b.y};
+        zd pr{0, 0};
+        for (int s = 0; s <= soc; s++) {
+            const int ii = soc ? 2 * i + s : i, jj = soc ? 2 * j + s : j;
+            const zd a = cmul(cconj(yn[ii]), ps[jj]), b = cmul(cconj(ps[ii]), yn[jj]);
+            pr.x += a.x + b.x;
+            pr.y += a.y + b.y;
+        }
         for (int sa = 0; sa < natom; sa++) {
             const zd f = sfac[id * natom + sa];
             atomicAdd(&acc[sa * nv + sh], pr.x * f.x - pr.y * f.y);
@@ -2244,16 +2319,22 @@ __global__ void __launch_bounds__(256) hf_adj_kernel(int npw, int N, int natom, 
         if (acc[i] != 0.0) atomicAdd(&gvg[i], scale * acc[i]);
 }
 
-/* step (2): gZ for the active elements from the compact gRe0/gIm0 [row][f] */
-__global__ void adj_gz_kernel(int nact, int nfreq, double SF, double w0, double df, const int *arow,
+/* step (2): G1 (and with SOC G2) for the active elements from the compact
+ * gRe0/gIm0 [row][f] */
+__global__ void adj_gz_kernel(int nact, int nfreq, int soc, double SF, double w0, double df, const int *arow,
                               const double *gRe0, const double *gIm0, zd *gZ) {
     const size_t total = (size_t)nact * nfreq;
     for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
         const int e = idx / nfreq, f = idx % nfreq;
         const size_t src = (size_t)arow[e] * nfreq + f;
         const double w = w0 + df * f;
-        const double inv = (f == 0) ? 0.0 : 1.0 / (w * w);
-        gZ[idx] = zd{SF * inv * gIm0[src], -SF * inv * gRe0[src]};
+        const double sc = SF * ((f == 0) ? 0.0 : 1.0 / (w * w));
+        if (!soc) {
+            gZ[idx] = zd{sc * gIm0[src], -sc * gRe0[src]};
+        } else {
+            gZ[idx] = zd{sc * gIm0[src], -0.5 * sc * gRe0[src]};
+            gZ[total + idx] = zd{0.0, -0.5 * sc * gRe0[src]};
+        }
     }
 }
 
@@ -2262,23 +2343,27 @@ __global__ void adj_gz_kernel(int nact, int nfreq, double SF, double w0, double 
 extern "C" int pmx_grad_chi(pmx_ctx *c, const pmx_chi_params *p, const double *gRe, const double *gIm, double *gvg) {
     if (!c || !p || !gRe || !gIm || !gvg) return set_error(PMX_EINVAL, "null pointer");
     if (!c->has_system) return set_error(PMX_ESTATE, "no system set");
-    if (c->soc) return set_error(PMX_EINVAL, "pmx_grad_chi: the spin-orbit solve has no adjoint here");
     if (p->delta < 0 || p->delta > 1)
         return set_error(PMX_EINVAL, "pmx_grad_chi: Gaussian (0) or Lorentzian (1) broadening only");
-    if (p->nband <= 0 || p->nband % 2 || p->nband > 2 * c->npw)
-        return set_error(PMX_EINVAL, "pmx_grad_chi: nband must be even and in [2, 2 npw]");
+    const bool soc = c->soc != 0;
+    if (p->nband <= 0 || p->nband > 2 * c->npw || (!soc && p->nband % 2))
+        return set_error(PMX_EINVAL, soc ? "pmx_grad_chi: nband must be in [1, 2 npw]"
+                                         : "pmx_grad_chi: nband must be even and in [2, 2 npw]");
     if (p->nfreq <= 0 || !(p->dfreq > 0) || p->nk < 0 || p->nq < 0) return set_error(PMX_EINVAL, "bad sizes");
     PMX_CHECK_HIP(hipSetDevice(c->device));
     hipStream_t st = c->stream;
-    const int neps = c->neps, npw = c->npw, D = 3 * neps, N = npw;
+    /* spatial: the npw-dimensional solve, Kramers pairs folded into weight 2 kw;
+     * SOC: the 2 npw spinor solve (rows 2p up, 2p+1 down), four rho channels */
+    const int neps = c->neps, npw = c->npw, D = 3 * neps, N = soc ? 2 * npw : npw, nch = soc ? 4 : 1;
     const int ne2 = neps * (neps + 1) / 2, nel = 9 * ne2, nfreq = p->nfreq, nk = p->nk;
-    const int nbs = p->nband / 2;
+    const int nbs = soc ? p->nband : p->nband / 2;
     const double SF = chi_scale(c), w0 = 0.0, df = p->dfreq;
     const int wh = window_half(p->delta, p->epsilon, df);
     const double tol = 1e-8; /* eV: pairs closer than this count as degenerate */
     std::fill(gvg, gvg + (size_t)c->natom * c->nv, 0.0);
     if (nk == 0 || p->nq == 0) return PMX_OK;
-    if (sizeof(double) * 4 * (size_t)D > c->lds_max) return set_error(PMX_EINVAL, "NEPS too large for the adjoint");
+    const size_t lds_t = sizeof(double) * 4 * (size_t)nch * D;
+    if (lds_t > c->lds_max) return set_error(PMX_EINVAL, "NEPS too large for the adjoint");
 
     /* inverse loci: invl[m][l] = p with loci[m][p] = l (G_p -> G_p + G_m is one to one) */
     std::vector<int> invl((size_t)neps * npw, -1);
@@ -2289,7 +2374,7 @@ extern "C" int pmx_grad_chi(pmx_ctx *c, const pmx_chi_params *p, const double *g
         }
     std::vector<double> Kh(p->K, p->K + 3 * (size_t)nk), KWh(p->KW, p->KW + nk);
     DBuf<double> Kd, KQd, KWd, evk, evkq, dEd, wtd, gdE, gEk, gEkq, gr, gi, gr0, gi0, KkT, out;
-    DBuf<zd> vk, vkq, X, gX, gk, gkq, Y, ucd, wAd, gZ;
+    DBuf<zd> vk, vkq, X, gX, gk, gkq, Y, ucd, wAd, wSd, gZ;
     DBuf<int> invd, nc, nv, tbase, pos, bad, act, arow, live;
     int rc;
     if ((rc = upload(st, Kd, Kh)) || (rc = upload(st, KWd, KWh)) || (rc = upload(st, invd, invl))) return rc;
@@ -2343,7 +2428,8 @@ extern "C" int pmx_grad_chi(pmx_ctx *c, const pmx_chi_params *p, const double *g
         }
         const int nact = (int)rows.size();
         if (nact == 0) continue;
-        if ((rc = upload(st, arow, rows)) || (rc = upload(st, act, pairs)) || (rc = gZ.ensure((size_t)nact * nfreq)))
+        if ((rc = upload(st, arow, rows)) || (rc = upload(st, act, pairs)) ||
+            (rc = gZ.ensure((size_t)(soc ? 2 : 1) * nact * nfreq)))
             return rc;
         PMX_CHECK_HIP(hipMemcpyAsync(gr.p, cre.data(), sizeof(double) * cre.size(), hipMemcpyHostToDevice, st));
         PMX_CHECK_HIP(hipMemcpyAsync(gi.p, cim.data(), sizeof(double) * cim.size(), hipMemcpyHostToDevice, st));
@@ -2356,13 +2442,14 @@ extern "C" int pmx_grad_chi(pmx_ctx *c, const pmx_chi_params *p, const double *g
             g_re = gr0.p;
             g_im = gi0.p;
         }
-        hipLaunchKernelGGL(adj_gz_kernel, dim3(256), dim3(256), 0, st, nact, nfreq, SF, w0, df, arow.p, g_re, g_im, gZ.p);
+        hipLaunchKernelGGL(adj_gz_kernel, dim3(256), dim3(256), 0, st, nact, nfreq, soc ? 1 : 0, SF, w0, df, arow.p,
+                           g_re, g_im, gZ.p);
 
         /* forward quantities of this q: eigenpairs at k+q, transitions, rho in transition order */
         const double Qx = p->Q[3 * q], Qy = p->Q[3 * q + 1], Qz = p->Q[3 * q + 2];
         std::vector<zd> uc, wA, wS;
-        q_tables(c, Qx, Qy, Qz, true, uc, wA, wS);
-        if ((rc = upload(st, ucd, uc)) || (rc = upload(st, wAd, wA))) return rc;
+        q_tables(c, Qx, Qy, Qz, !soc, uc, wA, wS);
+        if ((rc = upload(st, ucd, uc)) || (rc = upload(st, wAd, wA)) || (soc && (rc = upload(st, wSd, wS)))) return rc;
         std::vector<double> KQ(3 * (size_t)nk);
         for (int k = 0; k < nk; k++) {
             KQ[3 * k] = Kh[3 * k] + Qx;
@@ -2372,7 +2459,7 @@ extern "C" int pmx_grad_chi(pmx_ctx *c, const pmx_chi_params *p, const double *g
         if ((rc = upload(st, KQd, KQ))) return rc;
         if ((rc = eig_device(c, nk, KQd.p, N, N, evkq.p, vkq.p))) return rc;
         PMX_CHECK_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), st));
-        TransArgs ta{nk, N, p->nband, 1, evk.p, evkq.p, p->offset, nc.p, nv.p, bad.p, 0};
+        TransArgs ta{nk, N, p->nband, soc ? 0 : 1, evk.p, evkq.p, p->offset, nc.p, nv.p, bad.p, 0};
         hipLaunchKernelGGL(classify_kernel, dim3((nk + 255) / 256), dim3(256), 0, st, ta);
         std::vector<int> nch_h(nk), nvh(nk), tb(nk + 1);
         int badh = 0;
@@ -2383,38 +2470,38 @@ extern "C" int pmx_grad_chi(pmx_ctx *c, const pmx_chi_params *p, const double *g
         if (badh) return set_error(PMX_EBANDS, "at some k no level at or below the energy zero lies within -nband, "
                                                "or at some k+q none above it");
         tb[0] = 0;
-        for (int k = 0; k < nk; k++) tb[k + 1] = tb[k] + (nch_h[k] / 2) * (nvh[k] / 2);
+        for (int k = 0; k < nk; k++) tb[k + 1] = tb[k] + (soc ? nch_h[k] * nvh[k] : (nch_h[k] / 2) * (nvh[k] / 2));
         const int ntr = tb[nk];
         if (ntr == 0) continue;
         std::vector<int> iota(ntr);
         for (int t = 0; t < ntr; t++) iota[t] = t;
         if ((rc = upload(st, tbase, tb)) || (rc = upload(st, pos, iota)) || (rc = dEd.ensure(ntr)) ||
-            (rc = wtd.ensure(ntr)) || (rc = gdE.ensure(ntr)) || (rc = X.ensure((size_t)ntr * D)) ||
-            (rc = gX.ensure((size_t)ntr * D)))
+            (rc = wtd.ensure(ntr)) || (rc = gdE.ensure(ntr)) || (rc = X.ensure((size_t)ntr * nch * D)) ||
+            (rc = gX.ensure((size_t)ntr * nch * D)))
             return rc;
-        hipLaunchKernelGGL(dE_kernel, dim3(nk), dim3(256), 0, st, nk, N, p->nband, 1, evk.p, evkq.p, p->offset, nc.p,
-                           nv.p, tbase.p, KWd.p, dEd.p, wtd.p, 0);
+        hipLaunchKernelGGL(dE_kernel, dim3(nk), dim3(256), 0, st, nk, N, p->nband, soc ? 0 : 1, evk.p, evkq.p,
+                           p->offset, nc.p, nv.p, tbase.p, KWd.p, dEd.p, wtd.p, 0);
         RhoArgs ra{};
         ra.nk = nk; ra.N = N; ra.npw = npw; ra.neps = neps; ra.nb_s = N; ra.nband = p->nband;
-        ra.spatial = 1; ra.nch = 1;
+        ra.spatial = soc ? 0 : 1; ra.nch = nch;
         ra.vk = vk.p; ra.vkq = vkq.p; ra.nc = nc.p; ra.nv = nv.p; ra.tbase = tbase.p; ra.pos = pos.p;
-        ra.loci = c->dLoci.p; ra.wA = wAd.p; ra.uc = ucd.p; ra.K = Kd.p; ra.X = X.p;
+        ra.loci = c->dLoci.p; ra.wA = wAd.p; ra.uc = ucd.p; ra.wS = soc ? wSd.p : nullptr; ra.K = Kd.p; ra.X = X.p;
         hipLaunchKernelGGL(rho_kernel, dim3(nk), dim3(256), 0, st, ra);
 
         /* (3) per transition */
-        AdjTArgs at{D, nact, nfreq, wh, p->delta, p->epsilon, w0, df, X.p, dEd.p, wtd.p, act.p, gZ.p, gX.p, gdE.p};
-        hipLaunchKernelGGL(chi_adj_trans_kernel, dim3(ntr), dim3(256), sizeof(double) * 4 * D, st, at);
+        AdjTArgs at{D, nch, nact, nfreq, wh, p->delta, p->epsilon, w0, df, X.p, dEd.p, wtd.p, act.p, gZ.p, gX.p, gdE.p};
+        hipLaunchKernelGGL(chi_adj_trans_kernel, dim3(ntr), dim3(256), lds_t, st, at);
         /* (4) onto the wave vectors */
         PMX_CHECK_HIP(hipMemsetAsync(gkq.p, 0, sizeof(zd) * (size_t)nk * nbs * N, st));
         PMX_CHECK_HIP(hipMemsetAsync(gEkq.p, 0, sizeof(double) * (size_t)nk * nbs, st));
-        AdjVArgs av{N, npw, neps, p->nband, D, nbs, nc.p, nv.p, tbase.p, vk.p, vkq.p, c->dLoci.p, invd.p,
-                    wAd.p, ucd.p, Kd.p, gX.p, gdE.p, gk.p, gkq.p, gEk.p, gEkq.p};
+        AdjVArgs av{N, npw, neps, p->nband, D, nbs, soc ? 1 : 0, nc.p, nv.p, tbase.p, vk.p, vkq.p, c->dLoci.p,
+                    invd.p, wAd.p, ucd.p, soc ? wSd.p : nullptr, Kd.p, gX.p, gdE.p, gk.p, gkq.p, gEk.p, gEkq.p};
         hipLaunchKernelGGL(chi_adj_evec_kernel, dim3(nk), dim3(256), sizeof(zd) * 3 * neps, st, av);
         /* (5)-(6) for the states at k+q */
         hipLaunchKernelGGL(adj_y_kernel, dim3(nk * nbs), dim3(256), lds_y, st, N, nbs, tol, vkq.p, evkq.p, gkq.p,
                            gEkq.p, Y.p, live.p);
-        hipLaunchKernelGGL(hf_adj_kernel, dim3(nk * nbs), dim3(256), lds_hf, st, npw, N, c->natom, c->nv, nbs, vkq.p,
-                           Y.p, live.p, c->dShell.p, c->dSfac.p, out.p);
+        hipLaunchKernelGGL(hf_adj_kernel, dim3(nk * nbs), dim3(256), lds_hf, st, npw, N, c->natom, c->nv, nbs,
+                           soc ? 1 : 0, vkq.p, Y.p, live.p, c->dShell.p, c->dSfac.p, out.p);
         PMX_CHECK_HIP(hipGetLastError());
         /* host buffers above are reused next q */
         PMX_CHECK_HIP(hipStreamSynchronize(st));
@@ -2422,8 +2509,8 @@ extern "C" int pmx_grad_chi(pmx_ctx *c, const pmx_chi_params *p, const double *g
     /* (5)-(6) for the states at k, accumulated over q */
     hipLaunchKernelGGL(adj_y_kernel, dim3(nk * nbs), dim3(256), lds_y, st, N, nbs, tol, vk.p, evk.p, gk.p, gEk.p, Y.p,
                        live.p);
-    hipLaunchKernelGGL(hf_adj_kernel, dim3(nk * nbs), dim3(256), lds_hf, st, npw, N, c->natom, c->nv, nbs, vk.p, Y.p,
-                       live.p, c->dShell.p, c->dSfac.p, out.p);
+    hipLaunchKernelGGL(hf_adj_kernel, dim3(nk * nbs), dim3(256), lds_hf, st, npw, N, c->natom, c->nv, nbs, soc ? 1 : 0,
+                       vk.p, Y.p, live.p, c->dShell.p, c->dSfac.p, out.p);
     PMX_CHECK_HIP(hipGetLastError());
     PMX_CHECK_HIP(hipMemcpyAsync(gvg, out.p, sizeof(double) * c->natom * c->nv, hipMemcpyDeviceToHost, st));
     PMX_CHECK_HIP(hipStreamSynchronize(st));

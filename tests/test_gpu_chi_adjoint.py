@@ -1,8 +1,9 @@
 """Adjoint of chi (pmx_grad_chi through the C-ABI and the
 torch.autograd.Function) against central finite differences of the FP64
-pmx_chi, h = 1e-6 Ry, on Si (inversion centre: the real eigensolver path)
-and GaAs (complex path), with and without Kramers-Kronig, for Gaussian and
-Lorentzian broadening."""
+pmx_chi, h = 1e-6 Ry, on Si (inversion centre: the real eigensolver path),
+GaAs (complex path) and trigonal Te with spin-orbit maps (the spinor solve
+and its four rho channels), with and without Kramers-Kronig, for Gaussian
+and Lorentzian broadening."""
 import sys
 
 import numpy as np
@@ -11,12 +12,10 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 H = 1e-6
-NEPS = 9
-NBAND = 16
 NFREQ = 64
 DF = 0.15
 EPS = 0.2
-Q = np.array([[0.01, 0.0, 0.0], [0.05, 0.1, 0.0]])
+Q = np.array([[0.01, 0.0, 0.0], [0.05, 0.1, 0.02]])
 
 
 def _say(msg):
@@ -24,23 +23,27 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def _args(a, vg):
+def _zb(a, vg):
     return ["-crystal", "zb", "-a", str(a), "-vg", ",".join("%.17g" % x for x in vg), "-gcut",
-            "3.4641016151377544", "-neps", str(NEPS)]
+            "3.4641016151377544", "-neps", "9"]
 
 
-def _kpoints(model, rng, nk, offset):
+def _kpoints(model, rng, nk, offset, nband, step):
     """Random k-points whose transition energies all sit well inside a
     frequency bin: the windowed sum's window jumps where dE crosses a
-    multiple of dfreq, which a finite difference must not straddle."""
+    multiple of dfreq, which a finite difference must not straddle, and
+    the split of levels at the energy zero must not change under it either
+    (at k+q some level lies above zero, at k some at or below it).  step 2
+    takes one level of each Kramers pair (spatial solve), 1 every level."""
     K = []
     while len(K) < nk:
         k = rng.uniform(-0.5, 0.5, 3)
         ok = True
-        ek = model.ctx.bands(k[None, :], NBAND, offset)[0, ::2]
+        ek = model.ctx.bands(k[None, :], nband, offset)[0, ::step]
         for q in Q:
-            eq = model.ctx.bands((k + q)[None, :], NBAND, offset)[0, ::2]
-            if not (ek > 0).any() or not (eq <= 0).any():
+            eq = model.ctx.bands((k + q)[None, :], nband, offset)[0, ::step]
+            if not (ek > 0).any() or not (eq <= 0).any() or not (eq > 0).any() or not (ek <= 0).any() \
+                    or min(np.abs(ek).min(), np.abs(eq).min()) < 1e-3:
                 ok = False
                 break
             dE = ek[ek > 0][:, None] - eq[eq <= 0][None, :]
@@ -53,39 +56,55 @@ def _kpoints(model, rng, nk, offset):
     return np.array(K)
 
 
+TE = ["-crystal", "te", "-a", "4.45", "-f", "1.33", "-u", "0.26", "-vg",
+      "0.565323,1.333333,1.898656,2.261292,3.594626,4.0," + ",".join(["-0.2105,-0.1322,0.0451,0.0214,0.0127,-0.0093"] * 3),
+      "-gcut", "2.0", "-us_so", "0.565323:0.0152,1.333333:-0.0061,1.898656:0.0083",
+      "-ua_so", "0.565323:0.0117,1.333333:0.0042,1.898656:-0.0071", "-ua_so_val_g0", "0.0031", "-neps", "3"]
+
+# name: (arguments, nvalence, nband, NEPS, Kramers step)
 SYSTEMS = {
-    "si": (5.43, [3, 4, 8, 11] + [-0.2241, 0.0, 0.0551, 0.0724] + [-0.2241, 0.0, 0.0551, 0.0724]),
-    "gaas": (5.65, [3, 4, 8, 11] + [-0.1407, 0.05, 0.0277, 0.0635] + [-0.2807, -0.05, 0.0277, 0.0435]),
+    "si": (_zb(5.43, [3, 4, 8, 11] + [-0.2241, 0.0, 0.0551, 0.0724] + [-0.2241, 0.0, 0.0551, 0.0724]), 8, 16, 9, 2),
+    "gaas": (_zb(5.65, [3, 4, 8, 11] + [-0.1407, 0.05, 0.0277, 0.0635] + [-0.2807, -0.05, 0.0277, 0.0435]),
+             8, 16, 9, 2),
+    "te_soc": (TE, 18, 32, 3, 1),
 }
+
+
+class Setup:
+    pass
 
 
 @pytest.fixture(scope="module", params=sorted(SYSTEMS))
 def setup(request):
     from picomax_amd import autograd
-    a, vg = SYSTEMS[request.param]
-    rng = np.random.default_rng(11)
-    model = autograd.BandModel(_args(a, vg), refpoint=(0, 0, 0), nvalence=8)
-    off = model.offset()
-    K = _kpoints(model, rng, 5, off)
-    KW = rng.uniform(0.5, 1.5, len(K))
-    return request.param, model, K, KW, off, rng
+    args, nval, nband, neps, step = SYSTEMS[request.param]
+    S = Setup()
+    S.name, S.nband, S.neps = request.param, nband, neps
+    S.rng = np.random.default_rng(11)
+    S.model = autograd.BandModel(args, refpoint=(0, 0, 0), nvalence=nval)
+    S.off = S.model.offset()
+    S.K = _kpoints(S.model, S.rng, 5, S.off, nband, step)
+    S.KW = S.rng.uniform(0.5, 1.5, len(S.K))
+    return S
 
 
-def _upstream(model, rng):
-    ne2 = NEPS * (NEPS + 1) // 2
+def _upstream(S):
+    ne2 = S.neps * (S.neps + 1) // 2
     shape = (9, len(Q), ne2, NFREQ)
-    mask = rng.uniform(size=(9, len(Q), ne2, 1)) < 0.3
-    return rng.standard_normal(shape) * mask, rng.standard_normal(shape) * mask
+    mask = S.rng.uniform(size=(9, len(Q), ne2, 1)) < 0.3
+    return S.rng.standard_normal(shape) * mask, S.rng.standard_normal(shape) * mask
 
 
 @pytest.mark.parametrize("delta,kk", [(1, 1), (0, 0)])
 def test_chi_adjoint_vs_fd(setup, delta, kk):
-    name, model, K, KW, off, rng = setup
-    _say("chi adjoint vs FD: %s delta %d kk %d" % (name, delta, kk))
-    gRe, gIm = _upstream(model, rng)
+    S = setup
+    model = S.model
+    _say("chi adjoint vs FD: %s delta %d kk %d" % (S.name, delta, kk))
+    gRe, gIm = _upstream(S)
     v0 = model.vg0
     model.set_vg(v0)
-    g = model.ctx.grad_chi(K, KW, Q, NBAND, off, delta, EPS, NFREQ, DF, gRe, gIm, kk=kk)
+    common = (S.K, S.KW, Q, S.nband, S.off, delta, EPS, NFREQ, DF)
+    g = model.ctx.grad_chi(*common, gRe, gIm, kk=kk)
     g_fd = np.zeros_like(v0)
     for a in range(v0.shape[0]):
         for s in range(v0.shape[1]):
@@ -94,7 +113,7 @@ def test_chi_adjoint_vs_fd(setup, delta, kk):
                 v = v0.copy()
                 v[a, s] += sg * H
                 model.set_vg(v)
-                re, im = model.ctx.chi(K, KW, Q, NBAND, off, delta, EPS, NFREQ, DF, kk=kk, precision="fp64")
+                re, im = model.ctx.chi(*common, kk=kk, precision="fp64")
                 val.append((gRe.reshape(-1) * re).sum() + (gIm.reshape(-1) * im).sum())
             g_fd[a, s] = (val[0] - val[1]) / (2 * H)
     model.set_vg(v0)
@@ -106,12 +125,13 @@ def test_chi_adjoint_vs_fd(setup, delta, kk):
 def test_chi_autograd_function(setup):
     import torch
     from picomax_amd import autograd
-    name, model, K, KW, off, rng = setup
-    _say("chi torch.autograd.Function: %s" % name)
-    spec = autograd.ChiSpec(K, KW, Q, NBAND, off, 1, EPS, NFREQ, DF, kk=1)
+    S = setup
+    model = S.model
+    _say("chi torch.autograd.Function: %s" % S.name)
+    spec = autograd.ChiSpec(S.K, S.KW, Q, S.nband, S.off, 1, EPS, NFREQ, DF, kk=1)
     v = torch.tensor(model.vg0, dtype=torch.float64, requires_grad=True)
     re, im = autograd.chi_response(v, model, spec)
-    wr, wi = (torch.from_numpy(x.reshape(-1)) for x in _upstream(model, rng))
+    wr, wi = (torch.from_numpy(x.reshape(-1)) for x in _upstream(S))
     loss = (wr * re).sum() + (wi * im).sum()
     loss.backward()
     g = model.ctx.grad_chi(*spec.common(), wr.numpy(), wi.numpy(), kk=1)
@@ -121,12 +141,16 @@ def test_chi_autograd_function(setup):
 
 def test_chi_adjoint_rejects(setup):
     from picomax_amd import capi
-    name, model, K, KW, off, rng = setup
-    gRe, gIm = _upstream(model, rng)
+    S = setup
+    gRe, gIm = _upstream(S)
+    common = (S.K, S.KW, Q)
     with pytest.raises(capi.PmxError):  # triangle broadening is not differentiable
-        model.ctx.grad_chi(K, KW, Q, NBAND, off, 2, EPS, NFREQ, DF, gRe, gIm)
-    with pytest.raises(capi.PmxError):  # odd nband splits a Kramers pair
-        model.ctx.grad_chi(K, KW, Q, NBAND - 1, off, 1, EPS, NFREQ, DF, gRe, gIm)
+        S.model.ctx.grad_chi(*common, S.nband, S.off, 2, EPS, NFREQ, DF, gRe, gIm)
+    if S.name != "te_soc":
+        with pytest.raises(capi.PmxError):  # odd nband splits a Kramers pair of the spatial solve
+            S.model.ctx.grad_chi(*common, S.nband - 1, S.off, 1, EPS, NFREQ, DF, gRe, gIm)
+    with pytest.raises(capi.PmxError):  # more levels than the basis holds
+        S.model.ctx.grad_chi(*common, 2 * S.model.sys.npw + 2, S.off, 1, EPS, NFREQ, DF, gRe, gIm)
 
 
 def test_eps_m_fit_end_to_end(setup):
@@ -135,22 +159,23 @@ def test_eps_m_fit_end_to_end(setup):
     ChiResponse to the form factors; checked against central differences."""
     import torch
     from picomax_amd import autograd
-    name, model, K, KW, off, rng = setup
-    _say("eps_M loss end to end: %s" % name)
-    spec = autograd.ChiSpec(K, KW, Q[:1], NBAND, off, 1, EPS, NFREQ, DF, kk=1)
-    ne2 = NEPS * (NEPS + 1) // 2
-    iu = [(m, n) for m in range(NEPS) for n in range(m + 1)]
-    target = torch.from_numpy(rng.uniform(0.5, 1.5, NFREQ))
+    S = setup
+    model, neps = S.model, S.neps
+    _say("eps_M loss end to end: %s" % S.name)
+    spec = autograd.ChiSpec(S.K, S.KW, Q[:1], S.nband, S.off, 1, EPS, NFREQ, DF, kk=1)
+    ne2 = neps * (neps + 1) // 2
+    iu = [(m, n) for m in range(neps) for n in range(m + 1)]
+    target = torch.from_numpy(S.rng.uniform(0.5, 1.5, NFREQ))
 
     def eps_m_loss(v):
         re, im = autograd.chi_response(v, model, spec)
         blk = torch.complex(re, im).reshape(9, 1, ne2, NFREQ)[0, 0]  # i = j = 0 block
-        E = torch.zeros((NFREQ, NEPS, NEPS), dtype=torch.complex128)
+        E = torch.zeros((NFREQ, neps, neps), dtype=torch.complex128)
         for e, (m, n) in enumerate(iu):
             E[:, m, n] = blk[e]
             if m != n:
                 E[:, n, m] = blk[e].conj()
-        E = E + torch.eye(NEPS, dtype=torch.complex128)
+        E = E + torch.eye(neps, dtype=torch.complex128)
         epsM = 1.0 / torch.linalg.inv(E)[:, 0, 0]
         return ((epsM.imag - target) ** 2).sum()
 
