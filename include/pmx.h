@@ -168,6 +168,41 @@ int pmx_kk(pmx_ctx *ctx, int nel, int nfreq, double dfreq, double *ReX, double *
 int pmx_epsinv(pmx_ctx *ctx, int nq, int neps, int nfreq, const double *ReX, const double *ImX,
                double *epsM, double *elf, double *epsinv);
 
+/* The MATLAB wrapper's macroscopic dielectric functions
+ * (include/matlab_wrapper/+picomax/+eps/macroscopic.m:1-89), from all nine
+ * blocks of a chi_LL result (pmx_chi_ll's ReX/ImX for the same nq, Q, neps,
+ * nfreq, dfreq; Q [nq][3] in 2 pi / a units; G are the context's first neps
+ * plane waves, the .gpt rows).  eps_L = I + chi[L][L], eps_T = I + chi[T+][T+]
+ * (lower triangle, conjugate fill); with nq = 1 and Q = 0 the head of eps_L is
+ * eps_T's (:18-20).  eps_T's diagonal gets -c0^2 |q + G_m|^2 / w^2 with the
+ * MATLAB's constants (:51-53, 70-78): c0 = 3e8 m/s, w = omega / 4.135667696e-15
+ * (eV s), |q + G| = 2 pi |Q + G| / (a * 1e-10 m), and |q + G|^2 = 0 taken as
+ * 1e-15.  The MATLAB takes G from a fixed list of nine fcc vectors (:41-50);
+ * here they are the system's own.
+ * Outputs, complex [nq][nfreq]: eps00 = eps_L[0][0], epsM = 1/[eps_L^-1]_00,
+ * epsM_T = 1/[eps_T^-1]_00 (NaN at f = 0, where w = 0). */
+int pmx_eps_macro(pmx_ctx *ctx, int nq, const double *Q, int neps, int nfreq, double dfreq,
+                  const double *ReX, const double *ImX, double *eps00, double *epsM, double *epsM_T);
+
+/* The MATLAB wrapper's polarization bands
+ * (include/matlab_wrapper/@picomax/polarizationband.m:1-121): per (q, f) the
+ * eigenvalues of eps = I + chi[blk][blk] (blk 0 = L, 1 = T+, 2 = T-; stored
+ * lower triangle, conjugate fill above, complex diagonal as stored, so a
+ * general complex eigenproblem), tracked across f as eigenshuffle does
+ * (include/matlab_wrapper/+picomax/+util/eigenshuffle.m): at f = 0 the
+ * first q in descending real part and each next q matched to the one before
+ * (:12-30, 58-66), then along f each frequency matched to the previous one
+ * (:33-57), a match being the minimum-cost assignment of
+ * (1 - |v_i^H v_j|) |d_i - d_j|.  Where an f = 0 eigenvalue is repeated the
+ * MATLAB's 1e-10 lookup (:62) copies one track over its partner; here both
+ * tracks are kept.  ReX/ImX: all
+ * nine blocks as pmx_chi/pmx_chi_ll return them.  alpha [nq][nfreq][neps]
+ * complex; vec [nq][nfreq][neps][neps] complex (vector s of slot s, unit
+ * norm, largest component real, eigenshuffle's sign rule along f) may be NULL.
+ * PMX_EINVAL if an eigenvalue iteration does not converge. */
+int pmx_polarization_bands(pmx_ctx *ctx, int nq, int neps, int nfreq, int blk, const double *ReX,
+                           const double *ImX, double *alpha, double *vec);
+
 /* Hellmann-Feynman adjoint of the band energies (no reference counterpart):
  * given upstream gradients gE [nk][nband] of eband (band energies before the
  * offset), returns gvg [natom][nv] = sum_k,n gE[k][n] dE_n(k)/dvg. */

@@ -20,7 +20,7 @@ BIN_DIR = os.path.join(PKG, "bin")
 OBJ_DIR = os.path.join(ROOT, "build", "obj")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-LIB_SOURCES = ["pmx_input.cpp", "pmx_api.hip", "pmx_eig.hip", "pmx_chi.hip", "pmx_run.cpp"]
+LIB_SOURCES = ["pmx_input.cpp", "pmx_api.hip", "pmx_eig.hip", "pmx_chi.hip", "pmx_post.hip", "pmx_run.cpp"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-result", "-Wno-unused-function",

@@ -21,7 +21,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 # every symbol include/pmx.h declares
 EXPORTS = ["pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_set_system",
            "pmx_set_form_factors", "pmx_get_loci", "pmx_ref_energy", "pmx_bands", "pmx_eig_batch",
-           "pmx_chi", "pmx_chi_device", "pmx_chi_ll", "pmx_chi_ll_device", "pmx_chi_xyz", "pmx_chi_xyz_device", "pmx_profile", "pmx_profile_report", "pmx_kk", "pmx_epsinv", "pmx_grad_bands", "pmx_grad_chi", "pmx_input_parse", "pmx_input_free",
+           "pmx_chi", "pmx_chi_device", "pmx_chi_ll", "pmx_chi_ll_device", "pmx_chi_xyz", "pmx_chi_xyz_device", "pmx_profile", "pmx_profile_report", "pmx_kk", "pmx_epsinv", "pmx_eps_macro", "pmx_polarization_bands", "pmx_grad_bands", "pmx_grad_chi", "pmx_input_parse", "pmx_input_free",
            "pmx_input_system", "pmx_input_info", "pmx_generate_gpt", "pmx_run"]
 
 
@@ -90,6 +90,10 @@ def lib():
     L.pmx_profile_report.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
     L.pmx_kk.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp]
     L.pmx_epsinv.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.pmx_eps_macro.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp,
+                                _dp, _dp]
+    L.pmx_polarization_bands.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                         _dp]
     L.pmx_grad_bands.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
     L.pmx_grad_chi.argtypes = [vp, ctypes.POINTER(PmxChiParams), _dp, _dp, _dp]
     L.pmx_input_parse.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(vp)]
@@ -274,6 +278,35 @@ class Context:
                                 _p(eM.view(np.float64)), _p(elf),
                                 _p(inv.view(np.float64)) if full else None))
         return (eM, elf, inv) if full else (eM, elf)
+
+    def eps_macro(self, ReX, ImX, Q, nfreq, dfreq):
+        """(eps00, epsM, epsM_T), complex [nq][nfreq], of macroscopic.m from a chi_LL result."""
+        Q = _f64(Q).reshape(-1, 3)
+        nq = len(Q)
+        ReX = _f64(ReX).reshape(-1)
+        ImX = _f64(ImX).reshape(-1)
+        if ReX.size != 9 * nq * self.neps * (self.neps + 1) // 2 * nfreq or ImX.size != ReX.size:
+            raise PmxError("eps_macro: ReX/ImX must hold all nine blocks")
+        out = [np.zeros((nq, nfreq), complex) for _ in range(3)]
+        _check(lib().pmx_eps_macro(self.h, nq, _p(Q), self.neps, int(nfreq), float(dfreq), _p(ReX), _p(ImX),
+                                   *(_p(o.view(np.float64)) for o in out)))
+        return tuple(out)
+
+    def polarization_bands(self, ReX, ImX, nq, nfreq, blk=0, vectors=False, neps=None):
+        """Tracked eigenvalues alpha [nq][nfreq][neps] of eps = I + chi[blk][blk]
+        (polarizationband.m), and the eigenvectors [nq][nfreq][neps][neps] if asked.
+        neps defaults to the system's."""
+        ReX = _f64(ReX).reshape(-1)
+        ImX = _f64(ImX).reshape(-1)
+        n = self.neps if neps is None else int(neps)
+        if ReX.size != 9 * nq * n * (n + 1) // 2 * nfreq or ImX.size != ReX.size:
+            raise PmxError("polarization_bands: ReX/ImX must hold all nine blocks")
+        alpha = np.zeros((nq, nfreq, n), complex)
+        vec = np.zeros((nq, nfreq, n, n), complex) if vectors else None
+        _check(lib().pmx_polarization_bands(self.h, int(nq), n, int(nfreq), int(blk), _p(ReX), _p(ImX),
+                                            _p(alpha.view(np.float64)),
+                                            _p(vec.view(np.float64)) if vectors else None))
+        return (alpha, vec) if vectors else alpha
 
     def grad_bands(self, K, nband, gE):
         K = _f64(K).reshape(-1, 3)

@@ -1232,15 +1232,30 @@ __global__ void place_kernel(int nel, int nfreq, int ne2, int nq, int q, const d
     }
 }
 
-/* eps_mn = delta_mn + chi^{00}_mn (n <= m stored, eps_nm = conj(eps_mn)),
- * Gauss-Jordan with partial pivoting on [eps | I], one block per (q, f);
- * epsM = 1/[eps^-1]_00, ELF = -Im [eps^-1]_00 (macroscopic.m:23-38, 87-89) */
+/* eps_mn = delta_mn + chi_mn of one diagonal block [q][m][n <= m][f] (n <= m
+ * stored, eps_nm = conj(eps_mn)), Gauss-Jordan with partial pivoting on
+ * [eps | I], one block per (q, f); epsM = 1/[eps^-1]_00, ELF = -Im [eps^-1]_00
+ * (macroscopic.m:23-38, 87-89).  With qg2 ([nq][N], eV^2) the diagonal gets
+ * -qg2[q][m] / w_f^2, w_f = f dfreq (macroscopic.m:70-78); f = 0 then gives NaN.
+ * head ([nq][nfreq] complex, or null) replaces element (0, 0) (macroscopic.m:18-20). */
 __global__ void __launch_bounds__(256) epsinv_kernel(int N, int nq, int nfreq, const double *ReX, const double *ImX,
-                                                     zd *scratch, zd *epsM, double *elf, zd *inv_out) {
+                                                     const double *qg2, double dfreq, const zd *head, zd *scratch,
+                                                     zd *epsM, double *elf, zd *inv_out) {
     const int mat = blockIdx.x;
     const int q = mat / nfreq, f = mat % nfreq;
     const int ne2 = N * (N + 1) / 2;
     const int W = 2 * N;
+    if (qg2 && f == 0) {
+        /* w = 0: an infinite diagonal, no finite inverse */
+        if (threadIdx.x == 0) {
+            epsM[(size_t)q * nfreq + f] = zd{NAN, NAN};
+            elf[(size_t)q * nfreq + f] = NAN;
+        }
+        if (inv_out)
+            for (int e = threadIdx.x; e < N * N; e += blockDim.x) inv_out[(size_t)mat * N * N + e] = zd{NAN, NAN};
+        return;
+    }
+    const double iw2 = qg2 ? 1.0 / ((dfreq * f) * (dfreq * f)) : 0.0;
     zd *A = scratch + (size_t)blockIdx.x * N * W;
     __shared__ int piv;
     __shared__ double best[256];
@@ -1252,9 +1267,11 @@ __global__ void __launch_bounds__(256) epsinv_kernel(int N, int nq, int nfreq, c
             int m = r, n = cidx;
             bool cj = false;
             if (n > m) { int t = m; m = n; n = t; cj = true; }
-            const size_t src = ((size_t)q * ne2 + (size_t)m * (m + 1) / 2 + n) * nfreq + f; /* ij = 0 block */
+            const size_t src = ((size_t)q * ne2 + (size_t)m * (m + 1) / 2 + n) * nfreq + f;
             v = zd{ReX[src], cj ? -ImX[src] : ImX[src]};
-            if (r == cidx) v.x += 1.0;
+            if (head && r == 0 && cidx == 0) v = head[(size_t)q * nfreq + f];
+            else if (r == cidx) v.x += 1.0;
+            if (qg2 && r == cidx) v.x -= qg2[(size_t)q * N + r] * iw2;
         } else if (cidx - N == r) {
             v = zd{1.0, 0.0};
         }
@@ -1900,8 +1917,8 @@ extern "C" int pmx_epsinv(pmx_ctx *c, int nq, int neps, int nfreq, const double 
     PMX_CHECK_HIP(hipMemcpyAsync(im.p, ImX, sizeof(double) * nblk, hipMemcpyHostToDevice, st));
     {
         Prof pf(c, "epsinv", 8.0 * neps * neps * 2.0 * neps * nmat, 0.0);
-        hipLaunchKernelGGL(epsinv_kernel, dim3(nmat), dim3(256), 0, st, neps, nq, nfreq, re.p, im.p, scr.p, em.p, el.p,
-                           epsinv ? inv.p : nullptr);
+        hipLaunchKernelGGL(epsinv_kernel, dim3(nmat), dim3(256), 0, st, neps, nq, nfreq, re.p, im.p, nullptr, 0.0,
+                           nullptr, scr.p, em.p, el.p, epsinv ? inv.p : nullptr);
     }
     PMX_CHECK_HIP(hipGetLastError());
     PMX_CHECK_HIP(hipMemcpyAsync(epsM, em.p, sizeof(zd) * nmat, hipMemcpyDeviceToHost, st));
@@ -1909,6 +1926,73 @@ extern "C" int pmx_epsinv(pmx_ctx *c, int nq, int neps, int nfreq, const double 
     if (epsinv)
         PMX_CHECK_HIP(hipMemcpyAsync(epsinv, inv.p, sizeof(zd) * nmat * neps * neps, hipMemcpyDeviceToHost, st));
     PMX_CHECK_HIP(hipStreamSynchronize(st));
+    return PMX_OK;
+}
+
+/* macroscopic.m's longitudinal and transverse quantities from chi_LL's nine
+ * blocks: eps_L from block (L, L), eps_T from block (T+, T+) with its
+ * -c0^2 |q + G_m|^2 / w^2 diagonal (include/matlab_wrapper/+picomax/+eps/macroscopic.m:40-82) */
+extern "C" int pmx_eps_macro(pmx_ctx *c, int nq, const double *Q, int neps, int nfreq, double dfreq,
+                             const double *ReX, const double *ImX, double *eps00, double *epsM, double *epsM_T) {
+    if (!c || !Q || !ReX || !ImX || !eps00 || !epsM || !epsM_T) return set_error(PMX_EINVAL, "null pointer");
+    if (!c->has_system) return set_error(PMX_ESTATE, "no system set");
+    if (nq <= 0 || neps <= 0 || nfreq <= 0 || !(dfreq > 0)) return set_error(PMX_EINVAL, "bad sizes");
+    if (neps > c->npw) return set_error(PMX_EINVAL, "neps exceeds the plane-wave count");
+    PMX_CHECK_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const size_t ne2 = (size_t)neps * (neps + 1) / 2;
+    const size_t nblk = (size_t)nq * ne2 * nfreq;
+    const int nmat = nq * nfreq;
+    /* the MATLAB's constants (macroscopic.m:51-53): c0 = 3e8 m/s and w = omega / 4.135667696e-15,
+     * so c0^2 |q+G|^2 / w^2 = (c0 h |q+G|)^2 / omega^2 with omega in eV */
+    const double c0 = 3e8, h_eV = 4.135667696e-15, twopi_a = 2.0 * PMX_PI / (c->a * 1e-10);
+    std::vector<double> qg2((size_t)nq * neps);
+    bool q_zero = nq == 1;
+    for (int q = 0; q < nq; q++) {
+        if (Q[3 * q] != 0.0 || Q[3 * q + 1] != 0.0 || Q[3 * q + 2] != 0.0) q_zero = false;
+        for (int m = 0; m < neps; m++) {
+            double n2 = 0.0;
+            for (int d = 0; d < 3; d++) {
+                const double v = (Q[3 * q + d] + c->G[3 * m + d]) * twopi_a;
+                n2 += v * v;
+            }
+            if (n2 == 0.0) n2 = 1e-15;
+            qg2[(size_t)q * neps + m] = c0 * c0 * h_eV * h_eV * n2;
+        }
+    }
+    DBuf<double> reL, imL, reT, imT, el, qgd;
+    DBuf<zd> scr, emL, emT, hd;
+    int rc;
+    if ((rc = reL.ensure(nblk)) || (rc = imL.ensure(nblk)) || (rc = reT.ensure(nblk)) || (rc = imT.ensure(nblk)) ||
+        (rc = el.ensure(nmat)) || (rc = emL.ensure(nmat)) || (rc = emT.ensure(nmat)) ||
+        (rc = scr.ensure((size_t)nmat * neps * 2 * neps)) || (rc = upload(st, qgd, qg2)))
+        return rc;
+    const size_t tblk = 4 * nblk; /* block (i, j) = (1, 1) */
+    PMX_CHECK_HIP(hipMemcpyAsync(reL.p, ReX, sizeof(double) * nblk, hipMemcpyHostToDevice, st));
+    PMX_CHECK_HIP(hipMemcpyAsync(imL.p, ImX, sizeof(double) * nblk, hipMemcpyHostToDevice, st));
+    PMX_CHECK_HIP(hipMemcpyAsync(reT.p, ReX + tblk, sizeof(double) * nblk, hipMemcpyHostToDevice, st));
+    PMX_CHECK_HIP(hipMemcpyAsync(imT.p, ImX + tblk, sizeof(double) * nblk, hipMemcpyHostToDevice, st));
+    /* head of eps_L (after the q = 0 substitution of eps_T's head, macroscopic.m:18-20) */
+    std::vector<zd> head((size_t)nmat);
+    const double *hr = q_zero ? ReX + tblk : ReX, *hi = q_zero ? ImX + tblk : ImX;
+    for (int q = 0; q < nq; q++)
+        for (int f = 0; f < nfreq; f++) {
+            const size_t src = (size_t)q * ne2 * nfreq + f; /* element (0, 0) */
+            head[(size_t)q * nfreq + f] = zd{1.0 + hr[src], hi[src]};
+        }
+    if ((rc = upload(st, hd, head))) return rc;
+    {
+        Prof pf(c, "epsinv", 2.0 * 8.0 * neps * neps * 2.0 * neps * nmat, 0.0);
+        hipLaunchKernelGGL(epsinv_kernel, dim3(nmat), dim3(256), 0, st, neps, nq, nfreq, reL.p, imL.p, nullptr, 0.0,
+                           hd.p, scr.p, emL.p, el.p, nullptr);
+        hipLaunchKernelGGL(epsinv_kernel, dim3(nmat), dim3(256), 0, st, neps, nq, nfreq, reT.p, imT.p, qgd.p, dfreq,
+                           nullptr, scr.p, emT.p, el.p, nullptr);
+    }
+    PMX_CHECK_HIP(hipGetLastError());
+    PMX_CHECK_HIP(hipMemcpyAsync(epsM, emL.p, sizeof(zd) * nmat, hipMemcpyDeviceToHost, st));
+    PMX_CHECK_HIP(hipMemcpyAsync(epsM_T, emT.p, sizeof(zd) * nmat, hipMemcpyDeviceToHost, st));
+    PMX_CHECK_HIP(hipStreamSynchronize(st));
+    std::memcpy(eps00, head.data(), sizeof(zd) * nmat);
     return PMX_OK;
 }
 

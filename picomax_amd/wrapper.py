@@ -162,7 +162,7 @@ class picomax:  # noqa: N801  (the reference class name)
         filename = os.path.join(self.wdir, self.outputfile)
         if _SWITCH[switch] == "band":
             self.load(filename + ".dat", "eband")
-        elif _SWITCH[switch] == "epsij":
+        elif _SWITCH[switch] in ("epsij", "epsLL"):
             self.load(filename + ".gpt", "gpt")
             self.load(filename + ".dat", "Xij")
         print("Finished importing the calculation results!")
@@ -180,11 +180,54 @@ class picomax:  # noqa: N801  (the reference class name)
         if _SWITCH[switch] == "band":
             self.load(filename + ".dat", "eband")
             return self.eband
-        if _SWITCH[switch] == "epsij":
+        if _SWITCH[switch] in ("epsij", "epsLL"):
             self.load(filename + ".gpt", "gpt")
             self.load(filename + ".dat", "Xij")
             return self.Xij
         return None
+
+    # ---- the MATLAB wrapper's post-processing, on the loaded chi of an epsLL run
+    def _chi_blocks(self):
+        """Xij (nomega, nqpt, ne, 3, 3) back to the C-ABI layout [i][j][q][e][f]."""
+        X = np.asarray(self.Xij)
+        if X.ndim != 5:
+            raise RuntimeError("no chi loaded: run or compute 'epsLL' first")
+        t = np.transpose(X, (3, 4, 1, 2, 0))
+        return np.ascontiguousarray(t.real).reshape(-1), np.ascontiguousarray(t.imag).reshape(-1)
+
+    def _context(self, device):
+        inp = capi.Input(self.args("epsLL"))
+        ctx = capi.Context(device)
+        ctx.set_system(inp.system(), inp)
+        return ctx, inp
+
+    def macroscopic(self, device=0):
+        """macroscopic.m (include/matlab_wrapper/+picomax/+eps/macroscopic.m): sets
+        self.eps = {"eps00", "epsM", "epsM_T"}, each (nomega, nqpt)."""
+        re, im = self._chi_blocks()
+        ctx, inp = self._context(device)
+        try:
+            out = ctx.eps_macro(re, im, inp.Q, int(self.nomega), float(self.domega))
+        finally:
+            ctx.close()
+        self.eps = {k: v.T.copy() for k, v in zip(("eps00", "epsM", "epsM_T"), out)}
+        return self.eps
+
+    def polarizationband(self, device=0):
+        """polarizationband.m (include/matlab_wrapper/@picomax/polarizationband.m):
+        sets self.alphaL and self.alphaT, each (nomega, nqpt, nchi), the tracked
+        eigenvalues of eps_L = 1 + chi[L][L] and eps_T = 1 + chi[T+][T+]."""
+        re, im = self._chi_blocks()
+        ctx, inp = self._context(device)
+        try:
+            nq = len(inp.Q)
+            aL = ctx.polarization_bands(re, im, nq, int(self.nomega), blk=0)
+            aT = ctx.polarization_bands(re, im, nq, int(self.nomega), blk=1)
+        finally:
+            ctx.close()
+        self.alphaL = np.transpose(aL, (1, 0, 2)).copy()
+        self.alphaT = np.transpose(aT, (1, 0, 2)).copy()
+        return self.alphaL, self.alphaT
 
     # ---- loaders (include/python_wrapper/picomax.py:153-210)
     def load(self, filename, switch):

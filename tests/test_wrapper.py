@@ -64,3 +64,16 @@ def test_load_xij_reshape(tmp_path):
     # element (i=1, j=2, q=0, e=1, f=3) of the file order [i][j][q][e][f]
     idx = (((1 * 3 + 2) * 1 + 0) * ne + 1) * 5 + 3
     assert w.Xij[3, 0, 1, 1, 2] == data[idx] + 1j * data[N + idx]
+
+
+def test_chi_blocks_round_trip(tmp_path):
+    """The post-processing methods hand Xij back to the C-ABI in file order."""
+    w = picomax()
+    w.nchi, w.nomega, w.nqpt = 3, 4, 2
+    N = 9 * 6 * 2 * 4
+    data = np.random.default_rng(0).standard_normal(2 * N)
+    f = tmp_path / "x.dat"
+    data.tofile(str(f))
+    w.load_Xij(str(f))
+    re, im = w._chi_blocks()
+    assert np.array_equal(re, data[:N]) and np.array_equal(im, data[N:])
