@@ -15,6 +15,13 @@ FP64 path is timed in the same run for the ratio and as the reference for the
 eps_M error.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--precision bf16x3|fp32]
+
+`--config c3` / `--config c5` run the two larger workloads instead (one JSON
+line each, same fields, no CPU leg): the trigonal-Te spin-orbit plasmon list
+(32 q along Gamma-A, 582 x 582 spinor Hamiltonians, 16x16x12 k, 89 G, 800 w,
+FP32 chi) and the wurtzite local-field run (401 plane waves, 32x32x20 k,
+8 -> 24 bands, 181 G, 2000 w, split-BF16), with form factors drawn from a
+fixed seed.  `--grid n1,n2,n3` shrinks their k-grids.
 """
 import argparse
 import ctypes
@@ -200,6 +207,102 @@ def cpu_baseline(workdir):
                       "%.1f s wall" % (len(K), NGRID ** 3, t1 - t0)}
 
 
+def seeded_system(cfg):
+    """Command-line set-up of the C3 / C5 systems with form factors drawn from
+    numpy's default_rng(seed): C3 symmetric values in +-0.3 Ry on the first six
+    Te shells (three identical atoms) and antisymmetric SOC values in +-0.02 Ry
+    on the same keys; C5 per-species values in +-0.4 Ry on the first six wz
+    shells (two atoms of each species)."""
+    rng = np.random.default_rng(cfg["seed"])
+    if cfg["name"] == "c3":
+        keys = TE_VG.split(",")[:6]
+        v = rng.uniform(-0.3, 0.3, 6)
+        ua = rng.uniform(-0.02, 0.02, 6)
+        vg = ",".join(keys + ["%.6f" % x for x in v] * 3)
+        return ["-crystal", "te", "-a", "4.45", "-f", "1.33", "-u", "0.26", "-vg", vg, "-gcut", "4.0",
+                "-ua_so", ",".join("%s:%.6f" % (k, x) for k, x in zip(keys, ua)), "-neps", str(cfg["neps"]),
+                "-qpath", "G,A", "-qnum", "31", "-gammazero", "0,0,1e-3"]
+    keys = WZ_VG.split(",")[:6]
+    va, vb = rng.uniform(-0.4, 0.4, 6), rng.uniform(-0.4, 0.4, 6)
+    vg = ",".join(keys + ["%.6f" % x for x in va] * 2 + ["%.6f" % x for x in vb] * 2)
+    return ["-crystal", "wz", "-a", "3.19", "-f", "1.626", "-u", "0.375", "-vg", vg, "-gcut", "4.2",
+            "-neps", str(cfg["neps"]), "-qvec", "0.01,0,0"]
+
+
+CONFIGS = {
+    "c3": dict(name="c3", seed=3, crystal="te", f=1.33, grid=(16, 16, 12), nband=36, nvalence=18, neps=89,
+               nfreq=800, dfreq=0.025, delta=1, eps=0.05, kk=1, precision="fp32",
+               workload="Te SOC a=4.45 c/a=1.33 u=0.26, 291 PW (582x582 spinor H), 16x16x12 Gamma k-grid, "
+                        "32 q G->A (one call), nband 36 (18v->18c spinor), NEPS 89, 800 w (df 0.025), "
+                        "Lorentzian 0.05, KK, seed-drawn form factors"),
+    "c5": dict(name="c5", seed=5, crystal="wz", f=1.626, grid=(32, 32, 20), nband=64, nvalence=16, neps=181,
+               nfreq=2000, dfreq=0.015, delta=1, eps=0.05, kk=1, precision="bf16x3",
+               workload="wz a=3.19 c/a=1.626 u=0.375, 401 PW, 32x32x20 Gamma k-grid, q=0.01[100], nband 64 "
+                        "(8v->24c spatial), NEPS 181, 2000 w (df 0.015), Lorentzian 0.05, KK, seed-drawn form "
+                        "factors"),
+}
+
+
+def run_config(opt):
+    """One JSON line for the C3 or C5 workload: the mixed path's k-points/s
+    (k-points x q-vectors per second for the plasmon list) against FP64."""
+    cfg = CONFIGS[opt.config]
+    args = seeded_system(cfg)
+    inp = capi.Input(args)
+    ctx = capi.Context(0)
+    ctx.set_system(inp.system(), inp)
+    grid = tuple(int(x) for x in opt.grid.split(",")) if opt.grid else cfg["grid"]
+    K = kpoints.gamma_grid(cfg["crystal"], *grid, f=cfg["f"])
+    KW = np.full(len(K), 1.0 / len(K))
+    Q = inp.Q
+    nk, nq = len(K), len(Q)
+    off = ctx.ref_energy([0.0, 0.0, 0.0], cfg["nvalence"])
+    hip = Hip()
+    ne2 = cfg["neps"] * (cfg["neps"] + 1) // 2
+    n = 9 * nq * ne2 * cfg["nfreq"]
+    dre, dim = hip.malloc(8 * n), hip.malloc(8 * n)
+    common = (K, KW, Q, cfg["nband"], off, cfg["delta"], cfg["eps"], cfg["nfreq"], cfg["dfreq"], cfg["kk"])
+    prec = cfg["precision"]
+    print("%s: %d k x %d q, mixed (%s) step" % (opt.config, nk, nq, prec), file=sys.stderr, flush=True)
+    t_mix, prof_mix = run_steps(ctx, hip, ctx.chi_params(*common, prec), dre, dim, opt.steps, opt.warmup)
+    re_m, im_m = eps_block(hip, dre, dim, nq * ne2 * cfg["nfreq"])
+    t_64, prof_64 = None, {}
+    if opt.fp64_steps > 0:
+        print("%s: FP64 step" % opt.config, file=sys.stderr, flush=True)
+        t_64, prof_64 = run_steps(ctx, hip, ctx.chi_params(*common, "fp64"), dre, dim, opt.fp64_steps, 0)
+    re_d, im_d = eps_block(hip, dre, dim, nq * ne2 * cfg["nfreq"])
+    hip.free(dre)
+    hip.free(dim)
+    errs = {}
+    if t_64 is not None:
+        eM_m, elf_m = ctx.epsinv(re_m, im_m, nq, cfg["nfreq"])
+        eM_d, elf_d = ctx.epsinv(re_d, im_d, nq, cfg["nfreq"])
+        sel = (slice(None), slice(1, None))
+        errs = {"epsM_rel_err": float(np.max(np.abs(eM_m[sel] - eM_d[sel]) / np.abs(eM_d[sel]))),
+                "elf_rel_err": float(np.max(np.abs(elf_m[sel] - elf_d[sel])) / np.max(np.abs(elf_d[sel]))),
+                "im_epsM_rel_err": float(np.max(np.abs(eM_m[sel].imag - eM_d[sel].imag))
+                                         / np.max(np.abs(eM_d[sel].imag)))}
+    out = {
+        "metric": "chi(q,w) k-pts/s mixed vs FP64 GPU at eps_M rel err <=1e-4; eigensolves/s",
+        "value": round(nk * nq / t_mix, 2),
+        "unit": "k-pts/s" if nq == 1 else "(k-pt, q) pairs/s",
+        "n_gpus": 1, "steps": opt.steps, "warmup": opt.warmup, "ms_per_step": round(1e3 * t_mix, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": ("f32" if prec == "fp32" else "bf16x3") + " rank update + f64 eigensolve",
+        "data": "synthetic (Gamma-centred k-grid, form factors from default_rng(%d))" % cfg["seed"],
+        "config": {"workload": cfg["workload"], "precision": prec, "nk": nk, "nq": nq, "grid": list(grid),
+                   "npw": inp.system().npw},
+        "fp64_value": round(nk * nq / t_64, 2) if t_64 else None,
+        "fp64_ms_per_step": round(1e3 * t_64, 3) if t_64 else None,
+        "speedup_vs_fp64": round(t_64 / t_mix, 3) if t_64 else None,
+        **errs,
+        "kernels_ms_per_step": {k: round(v["ms"] / opt.steps, 3) for k, v in prof_mix.items()},
+        "fp64_kernels_ms_per_step": {k: round(v["ms"] / opt.fp64_steps, 3) for k, v in prof_64.items()},
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,7 +312,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eig-sweep", action="store_true", help="skip the eigensolves/s sweep over basis sizes")
-    ap.add_argument("--grid", type=int, default=NGRID)
+    ap.add_argument("--grid", default=None, help="k-grid: N for the C2 cube, n1,n2,n3 for --config c3/c5")
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c5"])
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (a plain copy kernel of known "
                          "size, to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -219,6 +323,10 @@ def main():
               "in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if opt.config != "c2":
+        run_config(opt)
+        return
+    grid = int(opt.grid) if opt.grid else NGRID
     inp = capi.Input(cli_args())
     ctx = capi.Context(0)
     if opt.copy_bytes > 0:
@@ -230,7 +338,7 @@ def main():
         hc.free(src)
         hc.free(dst)
     ctx.set_system(inp.system(), inp)
-    K = kpoints.gamma_grid("zb", opt.grid)
+    K = kpoints.gamma_grid("zb", grid)
     KW = np.full(len(K), 1.0 / len(K))
     off = ctx.ref_energy(REFPOINT, NVALENCE)
     hip = Hip()
