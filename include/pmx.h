@@ -155,6 +155,11 @@ int pmx_chi_xyz_device(pmx_ctx *ctx, const pmx_chi_params *p, double *dReX, doub
 int pmx_profile(pmx_ctx *ctx, int enable);
 int pmx_profile_report(pmx_ctx *ctx, char *buf, int len);
 
+/* The autotuner's decisions so far on this context, as a JSON list: per
+ * kernel shape the candidates' times on the sample it ran and the variant it
+ * keeps (no reference counterpart). */
+int pmx_tune_report(pmx_ctx *ctx, char *buf, int len);
+
 /* kramerskronigtransform (src/pmx_chi.cpp:272-309) applied in place to nel
  * contiguous elements of nfreq points each (w_f = f * dfreq). */
 int pmx_kk(pmx_ctx *ctx, int nel, int nfreq, double dfreq, double *ReX, double *ImX);

@@ -404,6 +404,16 @@ int pmx_profile_report(pmx_ctx *c, char *buf, int len) {
     return PMX_OK;
 }
 
+int pmx_tune_report(pmx_ctx *c, char *buf, int len) {
+    if (!c || !buf || len <= 0) return set_error(PMX_EINVAL, "null pointer");
+    std::string js = "[";
+    for (size_t i = 0; i < c->tune_log.size(); i++) js += (i ? ", " : "") + c->tune_log[i];
+    js += "]";
+    if ((int)js.size() >= len) return set_error(PMX_EINVAL, "pmx_tune_report: buffer too small");
+    std::snprintf(buf, len, "%s", js.c_str());
+    return PMX_OK;
+}
+
 int pmx_set_form_factors(pmx_ctx *c, const double *vg) {
     if (!c || !vg) return set_error(PMX_EINVAL, "null pointer");
     if (!c->has_system) return set_error(PMX_ESTATE, "no system set");

@@ -108,6 +108,9 @@ struct pmx_ctx {
     };
     bool prof = false;
     std::vector<ProfEvent> prof_pending;
+    /* autotuner: the chosen variant per kernel shape, and one JSON record per decision */
+    std::map<std::string, int> tune_choice;
+    std::vector<std::string> tune_log;
     std::map<std::string, ProfAcc> prof_acc;
 
     /* chi work buffers, kept between calls (allocated on first use) */

@@ -35,12 +35,16 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 
 namespace pmx {
 
 namespace {
 
 constexpr int EIG_THREADS = 256;
+constexpr int TU = 4; /* column loads a wave keeps in flight in the generic tridiagonalisation */
 constexpr int EIG_WAVES = EIG_THREADS / 64;
 
 /* ---------------------------------------------------------- arithmetic */
@@ -250,18 +254,28 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_kernel(EigArgs a) {
             if (!active) continue;
             for (int i = tid; i < L * EIG_WAVES; i += EIG_THREADS) part[i] = zero<T>();
             __syncthreads();
-            /* p = tau * A22 v, column pass: every stored element read once */
+            /* p = tau * A22 v, column pass: every stored element read once.  A wave
+             * issues TU loads of its column before using any, so that TU
+             * global-memory round trips overlap when the matrix is not in LDS. */
             for (int cc = wave; cc < L; cc += EIG_WAVES) {
                 const int c = k + 1 + cc;
                 const T vc = vv[cc];
+                const T *colp = A + (pk(c, c, N) - cc); /* element (k+1+rr, c) at colp[rr] */
                 T dot = zero<T>();
-                for (int rr = cc + lane; rr < L; rr += 64) {
-                    const T arc = A[pk(k + 1 + rr, c, N)];
-                    if (rr == cc) {
-                        dot += arc * vc;
-                    } else {
-                        dot += cj(arc) * vv[rr];
-                        part[wave * L + rr] += arc * vc;
+                for (int r0 = cc + lane; r0 < L; r0 += 64 * TU) {
+                    T av[TU];
+#pragma unroll
+                    for (int u = 0; u < TU; u++) av[u] = (r0 + 64 * u < L) ? colp[r0 + 64 * u] : zero<T>();
+#pragma unroll
+                    for (int u = 0; u < TU; u++) {
+                        const int rr = r0 + 64 * u;
+                        if (rr >= L) break;
+                        if (rr == cc) {
+                            dot += av[u] * vc;
+                        } else {
+                            dot += cj(av[u]) * vv[rr];
+                            part[wave * L + rr] += av[u] * vc;
+                        }
                     }
                 }
                 double dr = wave_sum(re(dot));
@@ -309,13 +323,20 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_kernel(EigArgs a) {
                 for (int i = tid; i < L; i += EIG_THREADS) ww[i] = ww[i] + al * vv[i];
             }
             __syncthreads();
-            /* A22 -= v w^H + w v^H (lower triangle) */
+            /* A22 -= v w^H + w v^H (lower triangle), TU loads in flight per wave */
             for (int cc = wave; cc < L; cc += EIG_WAVES) {
                 const int c = k + 1 + cc;
                 const T vc = cj(vv[cc]), wc = cj(ww[cc]);
-                for (int rr = cc + lane; rr < L; rr += 64) {
-                    T &x = A[pk(k + 1 + rr, c, N)];
-                    x -= vv[rr] * wc + ww[rr] * vc;
+                T *colp = A + (pk(c, c, N) - cc);
+                for (int r0 = cc + lane; r0 < L; r0 += 64 * TU) {
+                    T av[TU];
+#pragma unroll
+                    for (int u = 0; u < TU; u++) av[u] = (r0 + 64 * u < L) ? colp[r0 + 64 * u] : zero<T>();
+#pragma unroll
+                    for (int u = 0; u < TU; u++) {
+                        const int rr = r0 + 64 * u;
+                        if (rr < L) colp[rr] = av[u] - (vv[rr] * wc + ww[rr] * vc);
+                    }
                 }
             }
             __syncthreads();
@@ -328,6 +349,217 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_kernel(EigArgs a) {
             a.dS[(size_t)i * a.nmat + mat] = d[i];
             a.eS[(size_t)i * a.nmat + mat] = e[i];
             if (a.tauv) reinterpret_cast<T *>(a.tauv)[(size_t)mat * N + i] = (i + 1 < N) ? tauv[i] : zero<T>();
+        }
+        if (tid == 0) {
+            double gl = d[0], gu = d[0], emax2 = 0.0;
+            for (int i = 0; i < N; i++) {
+                double off = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < N ? fabs(e[i]) : 0.0);
+                gl = fmin(gl, d[i] - off);
+                gu = fmax(gu, d[i] + off);
+                if (i + 1 < N) emax2 = fmax(emax2, e[i] * e[i]);
+            }
+            double tnorm = fmax(fabs(gl), fabs(gu));
+            gl -= 2.0 * DBL_EPSILON * tnorm * N + 1e-300;
+            gu += 2.0 * DBL_EPSILON * tnorm * N + 1e-300;
+            a.gb[4 * mat] = gl;
+            a.gb[4 * mat + 1] = gu;
+            a.gb[4 * mat + 2] = DBL_MIN * fmax(1.0, emax2);
+            a.gb[4 * mat + 3] = tnorm;
+        }
+        __syncthreads();
+    }
+}
+
+/* The same reduction with the rank-2 update of step k-1 carried into the
+ * mat-vec sweep of step k, so that each stored element of the trailing
+ * matrix is read and written once per step (two matrix streams instead of
+ * three: the symv read, the update read and the update write).  Step k:
+ *   (a) column k takes the pending update A -= v w^H + w v^H of step k-1;
+ *   (b) the reflector of column k, v_k and tau_k;
+ *   (c) one sweep over columns k+1..N-1: x = a - v_{k-1} w_{k-1}^H - w_{k-1} v_{k-1}^H
+ *       is stored and enters p = tau_k A22 v_k;
+ *   (d) w_k = p - (tau_k/2)(p^H v_k) v_k becomes the pending pair.
+ * The last pair is applied to element (N-1, N-1) after the loop.  v and w
+ * are double-buffered in LDS by absolute row; tau goes straight to global
+ * memory.  Same outputs as tridiag_kernel. */
+template <class T, bool LDS>
+__global__ void __launch_bounds__(EIG_THREADS) tridiag_fused_kernel(EigArgs a) {
+    extern __shared__ double smem_d[];
+    const int N = a.N;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const size_t P = (size_t)N * (N + 1) / 2;
+
+    /* LDS: [A packed (LDS case)] [d N][e N] [v0 N][w0 N][v1 N][w1 N][part WAVES*N] (T) [red] */
+    char *sp = reinterpret_cast<char *>(smem_d);
+    T *A;
+    if (LDS) {
+        A = reinterpret_cast<T *>(sp);
+        sp += ((P * sizeof(T) + 15) / 16) * 16;
+    } else {
+        A = reinterpret_cast<T *>(a.scratch + (size_t)blockIdx.x * a.slab);
+    }
+    double *d = reinterpret_cast<double *>(sp); sp += sizeof(double) * N;
+    double *e = reinterpret_cast<double *>(sp); sp += sizeof(double) * N;
+    T *vb0 = reinterpret_cast<T *>(sp); sp += sizeof(T) * N;
+    T *wb0 = reinterpret_cast<T *>(sp); sp += sizeof(T) * N;
+    T *vb1 = reinterpret_cast<T *>(sp); sp += sizeof(T) * N;
+    T *wb1 = reinterpret_cast<T *>(sp); sp += sizeof(T) * N;
+    T *part = reinterpret_cast<T *>(sp); sp += sizeof(T) * N * EIG_WAVES;
+    double *red = reinterpret_cast<double *>(sp); sp += sizeof(double) * 4 * EIG_WAVES;
+    __shared__ double sh_scal[8];
+    __shared__ int sh_int[4];
+
+    for (int mat = blockIdx.x; mat < a.nmat; mat += gridDim.x) {
+        double Kv[3] = {a.K[3 * mat], a.K[3 * mat + 1], a.K[3 * mat + 2]};
+        T *vref = a.vref ? reinterpret_cast<T *>(a.vref) + (size_t)mat * P : nullptr;
+        T *tau_out = a.tauv ? reinterpret_cast<T *>(a.tauv) + (size_t)mat * N : nullptr;
+        for (int c = wave; c < N; c += EIG_WAVES)
+            for (int r = c + lane; r < N; r += 64) A[pk(r, c, N)] = h_elem<T>(a, Kv, r, c);
+        __syncthreads();
+
+        for (int k = 0; k + 1 < N; k++) {
+            const int L = N - k - 1;
+            T *vC = (k & 1) ? vb1 : vb0, *wC = (k & 1) ? wb1 : wb0; /* this step's pair */
+            const T *vP = (k & 1) ? vb0 : vb1, *wP = (k & 1) ? wb0 : wb1; /* step k-1's pair */
+            const bool hasP = k > 0;
+            /* (a) pending update of column k, rows k..N-1 */
+            if (hasP) {
+                const T vk = cj(vP[k]), wk = cj(wP[k]);
+                for (int r = k + tid; r < N; r += EIG_THREADS) A[pk(r, k, N)] -= vP[r] * wk + wP[r] * vk;
+                __syncthreads();
+            }
+            /* (b) reflector of column k */
+            double s = 0.0;
+            for (int r = k + 2 + tid; r < N; r += EIG_THREADS) s += nrm2(A[pk(r, k, N)]);
+            s = wave_sum(s);
+            if (lane == 0) red[wave] = s;
+            __syncthreads();
+            if (tid == 0) {
+                double xn2 = 0.0;
+                for (int w = 0; w < EIG_WAVES; w++) xn2 += red[w];
+                T alpha = A[pk(k + 1, k, N)];
+                double beta;
+                T tau, scal;
+                if (xn2 == 0.0 && im(alpha) == 0.0) {
+                    beta = re(alpha);
+                    tau = zero<T>();
+                    scal = zero<T>();
+                } else {
+                    beta = -copysign(sqrt(nrm2(alpha) + xn2), re(alpha));
+                    tau = cdiv(from_real<T>(beta) - alpha, from_real<T>(beta));
+                    scal = cdiv(from_real<T>(1.0), alpha - from_real<T>(beta));
+                }
+                e[k] = beta;
+                if (tau_out) tau_out[k] = tau;
+                sh_scal[0] = re(tau);
+                sh_scal[1] = im(tau);
+                sh_scal[2] = re(scal);
+                sh_scal[3] = im(scal);
+                sh_int[0] = (re(tau) == 0.0 && im(tau) == 0.0) ? 0 : 1;
+            }
+            __syncthreads();
+            T tau, scal;
+            if constexpr (sizeof(T) == sizeof(double)) {
+                tau = sh_scal[0];
+                scal = sh_scal[2];
+            } else {
+                tau = zd{sh_scal[0], sh_scal[1]};
+                scal = zd{sh_scal[2], sh_scal[3]};
+            }
+            const bool active = sh_int[0] != 0;
+            for (int i = tid; i < L; i += EIG_THREADS) {
+                T x = from_real<T>(1.0);
+                if (i > 0) {
+                    x = A[pk(k + 1 + i, k, N)] * scal;
+                    A[pk(k + 1 + i, k, N)] = x;
+                    if (vref) vref[pk(k + 1 + i, k, N)] = x;
+                }
+                vC[k + 1 + i] = active ? x : zero<T>();
+            }
+            for (int i = tid; i < L * EIG_WAVES; i += EIG_THREADS) part[i] = zero<T>();
+            __syncthreads();
+            /* (c) one sweep: pending update of columns k+1.., then p contributions */
+            for (int cc = wave; cc < L; cc += EIG_WAVES) {
+                const int c = k + 1 + cc;
+                const T vcC = vC[c];
+                const T pv = hasP ? cj(vP[c]) : zero<T>(), pw = hasP ? cj(wP[c]) : zero<T>();
+                T *colp = A + (pk(c, c, N) - cc);
+                T dot = zero<T>();
+                for (int r0 = cc + lane; r0 < L; r0 += 64 * TU) {
+                    T av[TU];
+#pragma unroll
+                    for (int u = 0; u < TU; u++) av[u] = (r0 + 64 * u < L) ? colp[r0 + 64 * u] : zero<T>();
+#pragma unroll
+                    for (int u = 0; u < TU; u++) {
+                        const int rr = r0 + 64 * u;
+                        if (rr >= L) break;
+                        const int r = k + 1 + rr;
+                        T x = av[u];
+                        if (hasP) {
+                            x = x - (vP[r] * pw + wP[r] * pv);
+                            colp[rr] = x;
+                        }
+                        if (rr == cc) {
+                            dot += x * vcC;
+                        } else {
+                            dot += cj(x) * vC[r];
+                            part[wave * L + rr] += x * vcC;
+                        }
+                    }
+                }
+                double dr = wave_sum(re(dot));
+                double di = wave_sum(im(dot));
+                if (lane == 0) {
+                    T t;
+                    if constexpr (sizeof(T) == sizeof(double)) t = dr; else t = zd{dr, di};
+                    part[wave * L + cc] += t;
+                }
+            }
+            __syncthreads();
+            /* (d) w = p - (tau/2)(p^H v) v, the pending pair of the next step */
+            double pr = 0.0, pi = 0.0;
+            for (int i = tid; i < L; i += EIG_THREADS) {
+                T p = zero<T>();
+                for (int w = 0; w < EIG_WAVES; w++) p += part[w * L + i];
+                p = tau * p;
+                wC[k + 1 + i] = p;
+                T cv = cj(p) * vC[k + 1 + i];
+                pr += re(cv);
+                pi += im(cv);
+            }
+            pr = wave_sum(pr);
+            pi = wave_sum(pi);
+            if (lane == 0) {
+                red[wave] = pr;
+                red[EIG_WAVES + wave] = pi;
+            }
+            __syncthreads();
+            double sr = 0, si = 0;
+            for (int w = 0; w < EIG_WAVES; w++) {
+                sr += red[w];
+                si += red[EIG_WAVES + w];
+            }
+            T pvs;
+            if constexpr (sizeof(T) == sizeof(double)) pvs = sr; else pvs = zd{sr, si};
+            const T al = (-0.5) * (tau * pvs);
+            for (int i = tid; i < L; i += EIG_THREADS) wC[k + 1 + i] = wC[k + 1 + i] + al * vC[k + 1 + i];
+            __syncthreads();
+        }
+        if (N >= 2 && tid == 0) {
+            /* the last pair (step N-2) acts on element (N-1, N-1) only */
+            const T *vL = ((N - 2) & 1) ? vb1 : vb0, *wL = ((N - 2) & 1) ? wb1 : wb0;
+            A[pk(N - 1, N - 1, N)] -= vL[N - 1] * cj(wL[N - 1]) + wL[N - 1] * cj(vL[N - 1]);
+        }
+        __syncthreads();
+        for (int i = tid; i < N; i += EIG_THREADS) d[i] = re(A[pk(i, i, N)]);
+        if (tid == 0) {
+            e[N - 1] = 0.0;
+            if (tau_out) tau_out[N - 1] = zero<T>();
+        }
+        __syncthreads();
+        for (int i = tid; i < N; i += EIG_THREADS) {
+            a.dS[(size_t)i * a.nmat + mat] = d[i];
+            a.eS[(size_t)i * a.nmat + mat] = e[i];
         }
         if (tid == 0) {
             double gl = d[0], gu = d[0], emax2 = 0.0;
@@ -1181,6 +1413,85 @@ template <class T, bool LDS> int launch_tridiag(pmx_ctx *c, EigArgs &a, int grid
     return 0;
 }
 
+size_t lds_fused_bytes(int N, size_t elem, bool lds_matrix) {
+    size_t P = (size_t)N * (N + 1) / 2;
+    size_t b = lds_matrix ? ((P * elem + 15) / 16) * 16 : 0;
+    b += sizeof(double) * 2 * N;
+    b += elem * (4 * (size_t)N + (size_t)N * EIG_WAVES);
+    b += sizeof(double) * 4 * EIG_WAVES;
+    return b;
+}
+
+template <class T, bool LDS> int launch_tridiag_fused(pmx_ctx *c, EigArgs &a, int grid, size_t lds) {
+    auto kern = tridiag_fused_kernel<T, LDS>;
+    if (lds > 65536)
+        PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(EIG_THREADS), lds, c->stream, a);
+    PMX_CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+/* The generic tridiagonalisation (any N, real or complex, matrix in LDS or in
+ * global memory) has two forms: three matrix streams per step
+ * (tridiag_kernel) or two with the update carried into the next mat-vec
+ * (tridiag_fused_kernel, 2N more vectors of LDS, so possibly fewer resident
+ * workgroups).  Which is faster depends on N, the element type and how much
+ * of the matrices the caches hold, so the first batch of at least
+ * 2 x num_cu matrices of a given (type, N, placement) times both on its first
+ * min(nmat, 2 * grid) matrices and keeps the faster; later batches reuse the
+ * choice.  PMX_AUTOTUNE=0 keeps the three-stream form without timing;
+ * PMX_TRIDIAG_FORM=fused or three_stream forces one form. */
+int tridiag_generic(pmx_ctx *c, EigArgs &a, bool real, bool lds_matrix, int grid, size_t lds) {
+    const int N = a.N;
+    const size_t ldsf = lds_fused_bytes(N, real ? sizeof(double) : sizeof(zd), lds_matrix);
+    auto run = [&](int variant, EigArgs &aa, int g) -> int {
+        if (variant == 0) {
+            if (real) return lds_matrix ? launch_tridiag<double, true>(c, aa, g, lds) : launch_tridiag<double, false>(c, aa, g, lds);
+            return lds_matrix ? launch_tridiag<zd, true>(c, aa, g, lds) : launch_tridiag<zd, false>(c, aa, g, lds);
+        }
+        if (real)
+            return lds_matrix ? launch_tridiag_fused<double, true>(c, aa, g, ldsf)
+                              : launch_tridiag_fused<double, false>(c, aa, g, ldsf);
+        return lds_matrix ? launch_tridiag_fused<zd, true>(c, aa, g, ldsf) : launch_tridiag_fused<zd, false>(c, aa, g, ldsf);
+    };
+    if (ldsf > c->lds_max) return run(0, a, grid);
+    const char *form = std::getenv("PMX_TRIDIAG_FORM");
+    if (form && std::strcmp(form, "fused") == 0) return run(1, a, grid);
+    if (form && std::strcmp(form, "three_stream") == 0) return run(0, a, grid);
+    const std::string key = std::string(real ? "real" : "complex") + (lds_matrix ? " lds N=" : " global N=") +
+                            std::to_string(N);
+    auto it = c->tune_choice.find(key);
+    if (it != c->tune_choice.end()) return run(it->second, a, grid);
+    const char *env = std::getenv("PMX_AUTOTUNE");
+    if ((env && env[0] == '0') || a.nmat < 2 * c->num_cu) return run(0, a, grid);
+    EigArgs s = a;
+    s.nmat = std::min(a.nmat, 2 * grid);
+    const int gs = std::min(grid, s.nmat);
+    hipEvent_t ev[3];
+    for (auto &x : ev) PMX_CHECK_HIP(hipEventCreate(&x));
+    int rc = 0;
+    float ms[2] = {0.0f, 0.0f};
+    PMX_CHECK_HIP(hipEventRecord(ev[0], c->stream));
+    if (!rc) rc = run(0, s, gs);
+    PMX_CHECK_HIP(hipEventRecord(ev[1], c->stream));
+    if (!rc) rc = run(1, s, gs);
+    PMX_CHECK_HIP(hipEventRecord(ev[2], c->stream));
+    PMX_CHECK_HIP(hipEventSynchronize(ev[2]));
+    PMX_CHECK_HIP(hipEventElapsedTime(&ms[0], ev[0], ev[1]));
+    PMX_CHECK_HIP(hipEventElapsedTime(&ms[1], ev[1], ev[2]));
+    for (auto &x : ev) PMX_CHECK_HIP(hipEventDestroy(x));
+    if (rc) return rc;
+    const int best = ms[1] < ms[0] ? 1 : 0;
+    c->tune_choice[key] = best;
+    char line[256];
+    std::snprintf(line, sizeof(line),
+                  "{\"kernel\": \"tridiag %s\", \"sample\": %d, \"three_stream_ms\": %.4f, \"fused_ms\": %.4f, "
+                  "\"chosen\": \"%s\"}",
+                  key.c_str(), s.nmat, ms[0], ms[1], best ? "fused" : "three_stream");
+    c->tune_log.push_back(line);
+    return run(best, a, grid);
+}
+
 bool wy_usable(pmx_ctx *c, int N, int nvec) { return N >= 2 && N <= 16 * BT_VL && wy_lds_bytes(N, nvec) <= c->lds_max; }
 
 template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, const double *Y, const void *vref,
@@ -1293,8 +1604,9 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
                 hipLaunchKernelGGL(kern, dim3(std::min(nm, c->num_cu * 2)), dim3(EIG_THREADS), lds_full, c->stream, a);
                 PMX_CHECK_HIP(hipGetLastError());
                 rc = 0;
-            } else if (real) rc = lds_matrix ? launch_tridiag<double, true>(c, a, grid, lds) : launch_tridiag<double, false>(c, a, grid, lds);
-            else rc = lds_matrix ? launch_tridiag<zd, true>(c, a, grid, lds) : launch_tridiag<zd, false>(c, a, grid, lds);
+            } else {
+                rc = tridiag_generic(c, a, real, lds_matrix, grid, lds);
+            }
             if (rc) return rc;
         }
         {

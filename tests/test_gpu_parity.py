@@ -97,6 +97,36 @@ def test_eigenpairs_against_oracle_hamiltonian(manifest, ctx, case):
         assert np.abs(V.conj().T @ V - np.eye(nb)).max() < 1e-10
 
 
+LARGE = {  # bases that keep the packed matrix in global memory (the C3 / C5 sizes)
+    "te_soc_582": ["-crystal", "te", "-a", "4.45", "-f", "1.33", "-u", "0.26", "-vg",
+                   "0.565323,1.333333,1.898656,2.261292,3.594626,4.0" + ",-0.2105,-0.1322,0.0451,0.0214,0.0127,-0.0093" * 3,
+                   "-gcut", "4.0", "-us_so", "0.565323:0.0152,1.333333:-0.0061,1.898656:0.0083",
+                   "-ua_so", "0.565323:0.0117,1.333333:0.0042,1.898656:-0.0071"],
+    "wz_401": ["-crystal", "wz", "-a", "3.19", "-f", "1.626", "-u", "0.375", "-vg",
+               "0.378233,1.333333,1.51293,1.711566,2.846264,3.404093" + ",-0.3101,-0.1810,0.0902,0.0415,0.0254,-0.0122" * 2
+               + ",-0.2401,0.1110,0.0502,-0.0315,0.0054,0.0222" * 2, "-gcut", "4.2"],
+}
+
+
+@pytest.mark.parametrize("case", sorted(LARGE))
+def test_eigenpairs_large_basis(ctx, case):
+    args = LARGE[case]
+    _say("eigenpairs %s" % case)
+    inp = capi.Input(args)
+    ctx.set_system(inp.system(), inp)
+    s, p = cli_args.build(["-switch", "band"] + args)
+    K = np.random.default_rng(9).uniform(-0.5, 0.5, (3, 3))
+    nb = 40
+    ev, vec = ctx.eig(K, nb, vectors=True)
+    for k in range(len(K)):
+        H = s.hamiltonian(K[k])
+        w = np.linalg.eigvalsh(H)[:nb]
+        assert np.abs(ev[k] - w).max() < 1e-10
+        V = vec[k].T
+        assert np.abs(H @ V - V * ev[k]).max() < 1e-9
+        assert np.abs(V.conj().T @ V - np.eye(nb)).max() < 1e-10
+
+
 def test_spatial_path_equals_spinor_oracle_random(ctx):
     """Random zinc-blende form factors, no inversion centre: the complex
     spatial solve + factor 2 equals the oracle's full spinor chi."""
@@ -168,3 +198,50 @@ def test_epsinv_against_numpy(manifest, ctx):
             assert np.abs(inv[q, f] - Ei).max() < 1e-12 * max(1.0, np.abs(Ei).max())
             assert abs(eM[q, f] - 1.0 / Ei[0, 0]) < 1e-12 * abs(eM[q, f])
             assert abs(elf[q, f] + Ei[0, 0].imag) < 1e-13 * max(1.0, abs(Ei[0, 0]))
+
+
+@pytest.mark.parametrize("form", ["fused", "three_stream"])
+@pytest.mark.parametrize("case", ["gaas_band", "te_soc_band"])
+def test_tridiag_forms_against_oracle(manifest, case, form, monkeypatch):
+    """Both forms of the generic tridiagonalisation (matrix in LDS here) give
+    the oracle's eigenpairs; the autotuner may pick either."""
+    monkeypatch.setenv("PMX_TRIDIAG_FORM", form)
+    args = _expand(manifest[case]["args"])
+    _say("tridiag form %s %s" % (form, case))
+    c = capi.Context(0)
+    try:
+        inp = capi.Input(args)
+        c.set_system(inp.system(), inp)
+        s, p = cli_args.build(manifest[case]["args"])
+        K = np.random.default_rng(3).uniform(-0.6, 0.6, (4, 3))
+        nb = min(16, 2 * s.npw)
+        ev, vec = c.eig(K, nb, vectors=True)
+        for k in range(len(K)):
+            H = s.hamiltonian(K[k])
+            assert np.abs(ev[k] - np.linalg.eigvalsh(H)[:nb]).max() < 1e-10
+            V = vec[k].T
+            assert np.abs(H @ V - V * ev[k]).max() < 1e-9
+    finally:
+        c.close()
+
+
+@pytest.mark.parametrize("form", ["fused", "three_stream"])
+def test_tridiag_forms_large_basis(form, monkeypatch):
+    """Both forms with the matrix in global memory (wz, 401 plane waves)."""
+    monkeypatch.setenv("PMX_TRIDIAG_FORM", form)
+    args = LARGE["wz_401"]
+    _say("tridiag form %s wz_401" % form)
+    c = capi.Context(0)
+    try:
+        inp = capi.Input(args)
+        c.set_system(inp.system(), inp)
+        s, p = cli_args.build(["-switch", "band"] + args)
+        K = np.random.default_rng(4).uniform(-0.5, 0.5, (2, 3))
+        ev, vec = c.eig(K, 24, vectors=True)
+        for k in range(len(K)):
+            H = s.hamiltonian(K[k])
+            assert np.abs(ev[k] - np.linalg.eigvalsh(H)[:24]).max() < 1e-10
+            V = vec[k].T
+            assert np.abs(H @ V - V * ev[k]).max() < 1e-9
+    finally:
+        c.close()
