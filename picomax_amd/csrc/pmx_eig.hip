@@ -38,6 +38,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
 
 namespace pmx {
 
@@ -1431,65 +1434,90 @@ template <class T, bool LDS> int launch_tridiag_fused(pmx_ctx *c, EigArgs &a, in
     return 0;
 }
 
+/* Autotuner for the tridiagonalisation: several kernels reduce the same
+ * matrices to the same tridiagonal form (to rounding), and which is fastest
+ * depends on N, the element type and how much of the matrices the caches
+ * hold.  The first batch of at least 2 x num_cu matrices of a shape (key)
+ * runs every candidate on its first `sample` matrices under HIP events and
+ * keeps the fastest (by at least 3% over the first candidate, the default)
+ * for the context; later batches reuse the choice, and the
+ * decision is logged for pmx_tune_report.  Smaller batches use the first
+ * candidate without timing.  PMX_AUTOTUNE=0 disables the timing (first
+ * candidate); PMX_TRIDIAG_FORM=<name> forces a candidate by name. */
+struct TuneCand {
+    const char *name;
+    std::function<int(EigArgs &)> run;
+};
+
+int tune_run(pmx_ctx *c, const std::string &key, EigArgs &a, int sample, const std::vector<TuneCand> &cands) {
+    const char *form = std::getenv("PMX_TRIDIAG_FORM");
+    if (form)
+        for (const auto &cd : cands)
+            if (std::strcmp(form, cd.name) == 0) return cd.run(a);
+    if (cands.size() == 1) return cands[0].run(a);
+    auto it = c->tune_choice.find(key);
+    if (it != c->tune_choice.end()) return cands[it->second].run(a);
+    const char *env = std::getenv("PMX_AUTOTUNE");
+    if ((env && env[0] == '0') || a.nmat < 2 * c->num_cu) return cands[0].run(a);
+    EigArgs s = a;
+    s.nmat = std::min(a.nmat, sample);
+    const int nc = (int)cands.size();
+    std::vector<hipEvent_t> ev(nc + 1);
+    for (auto &x : ev) PMX_CHECK_HIP(hipEventCreate(&x));
+    int rc = 0;
+    PMX_CHECK_HIP(hipEventRecord(ev[0], c->stream));
+    for (int i = 0; i < nc && !rc; i++) {
+        rc = cands[i].run(s);
+        PMX_CHECK_HIP(hipEventRecord(ev[i + 1], c->stream));
+    }
+    PMX_CHECK_HIP(hipEventSynchronize(ev[nc]));
+    std::vector<float> ms(nc, 0.0f);
+    for (int i = 0; i < nc; i++) PMX_CHECK_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+    for (auto &x : ev) PMX_CHECK_HIP(hipEventDestroy(x));
+    if (rc) return rc;
+    /* a later candidate must beat the current choice by 3%: timing noise
+     * does not move the choice away from the first (default) candidate */
+    int best = 0;
+    for (int i = 1; i < nc; i++)
+        if (ms[i] < 0.97f * ms[best]) best = i;
+    c->tune_choice[key] = best;
+    std::string rec = "{\"kernel\": \"tridiag " + key + "\", \"sample\": " + std::to_string(s.nmat) + ", \"ms\": {";
+    char num[64];
+    for (int i = 0; i < nc; i++) {
+        std::snprintf(num, sizeof(num), "%.4f", ms[i]);
+        rec += std::string(i ? ", " : "") + "\"" + cands[i].name + "\": " + num;
+    }
+    rec += std::string("}, \"chosen\": \"") + cands[best].name + "\"}";
+    c->tune_log.push_back(rec);
+    return cands[best].run(a);
+}
+
 /* The generic tridiagonalisation (any N, real or complex, matrix in LDS or in
- * global memory) has two forms: three matrix streams per step
+ * global memory) in its two forms: three matrix streams per step
  * (tridiag_kernel) or two with the update carried into the next mat-vec
  * (tridiag_fused_kernel, 2N more vectors of LDS, so possibly fewer resident
- * workgroups).  Which is faster depends on N, the element type and how much
- * of the matrices the caches hold, so the first batch of at least
- * 2 x num_cu matrices of a given (type, N, placement) times both on its first
- * min(nmat, 2 * grid) matrices and keeps the faster; later batches reuse the
- * choice.  PMX_AUTOTUNE=0 keeps the three-stream form without timing;
- * PMX_TRIDIAG_FORM=fused or three_stream forces one form. */
+ * workgroups). */
 int tridiag_generic(pmx_ctx *c, EigArgs &a, bool real, bool lds_matrix, int grid, size_t lds) {
     const int N = a.N;
     const size_t ldsf = lds_fused_bytes(N, real ? sizeof(double) : sizeof(zd), lds_matrix);
-    auto run = [&](int variant, EigArgs &aa, int g) -> int {
-        if (variant == 0) {
-            if (real) return lds_matrix ? launch_tridiag<double, true>(c, aa, g, lds) : launch_tridiag<double, false>(c, aa, g, lds);
-            return lds_matrix ? launch_tridiag<zd, true>(c, aa, g, lds) : launch_tridiag<zd, false>(c, aa, g, lds);
-        }
-        if (real)
-            return lds_matrix ? launch_tridiag_fused<double, true>(c, aa, g, ldsf)
-                              : launch_tridiag_fused<double, false>(c, aa, g, ldsf);
-        return lds_matrix ? launch_tridiag_fused<zd, true>(c, aa, g, ldsf) : launch_tridiag_fused<zd, false>(c, aa, g, ldsf);
-    };
-    if (ldsf > c->lds_max) return run(0, a, grid);
-    const char *form = std::getenv("PMX_TRIDIAG_FORM");
-    if (form && std::strcmp(form, "fused") == 0) return run(1, a, grid);
-    if (form && std::strcmp(form, "three_stream") == 0) return run(0, a, grid);
+    std::vector<TuneCand> cands;
+    cands.push_back({"three_stream", [=](EigArgs &aa) {
+                         const int g = std::min(grid, aa.nmat);
+                         if (real) return lds_matrix ? launch_tridiag<double, true>(c, aa, g, lds)
+                                                     : launch_tridiag<double, false>(c, aa, g, lds);
+                         return lds_matrix ? launch_tridiag<zd, true>(c, aa, g, lds) : launch_tridiag<zd, false>(c, aa, g, lds);
+                     }});
+    if (ldsf <= c->lds_max)
+        cands.push_back({"fused", [=](EigArgs &aa) {
+                             const int g = std::min(grid, aa.nmat);
+                             if (real) return lds_matrix ? launch_tridiag_fused<double, true>(c, aa, g, ldsf)
+                                                         : launch_tridiag_fused<double, false>(c, aa, g, ldsf);
+                             return lds_matrix ? launch_tridiag_fused<zd, true>(c, aa, g, ldsf)
+                                               : launch_tridiag_fused<zd, false>(c, aa, g, ldsf);
+                         }});
     const std::string key = std::string(real ? "real" : "complex") + (lds_matrix ? " lds N=" : " global N=") +
                             std::to_string(N);
-    auto it = c->tune_choice.find(key);
-    if (it != c->tune_choice.end()) return run(it->second, a, grid);
-    const char *env = std::getenv("PMX_AUTOTUNE");
-    if ((env && env[0] == '0') || a.nmat < 2 * c->num_cu) return run(0, a, grid);
-    EigArgs s = a;
-    s.nmat = std::min(a.nmat, 2 * grid);
-    const int gs = std::min(grid, s.nmat);
-    hipEvent_t ev[3];
-    for (auto &x : ev) PMX_CHECK_HIP(hipEventCreate(&x));
-    int rc = 0;
-    float ms[2] = {0.0f, 0.0f};
-    PMX_CHECK_HIP(hipEventRecord(ev[0], c->stream));
-    if (!rc) rc = run(0, s, gs);
-    PMX_CHECK_HIP(hipEventRecord(ev[1], c->stream));
-    if (!rc) rc = run(1, s, gs);
-    PMX_CHECK_HIP(hipEventRecord(ev[2], c->stream));
-    PMX_CHECK_HIP(hipEventSynchronize(ev[2]));
-    PMX_CHECK_HIP(hipEventElapsedTime(&ms[0], ev[0], ev[1]));
-    PMX_CHECK_HIP(hipEventElapsedTime(&ms[1], ev[1], ev[2]));
-    for (auto &x : ev) PMX_CHECK_HIP(hipEventDestroy(x));
-    if (rc) return rc;
-    const int best = ms[1] < ms[0] ? 1 : 0;
-    c->tune_choice[key] = best;
-    char line[256];
-    std::snprintf(line, sizeof(line),
-                  "{\"kernel\": \"tridiag %s\", \"sample\": %d, \"three_stream_ms\": %.4f, \"fused_ms\": %.4f, "
-                  "\"chosen\": \"%s\"}",
-                  key.c_str(), s.nmat, ms[0], ms[1], best ? "fused" : "three_stream");
-    c->tune_log.push_back(line);
-    return run(best, a, grid);
+    return tune_run(c, key, a, 2 * grid, cands);
 }
 
 bool wy_usable(pmx_ctx *c, int N, int nvec) { return N >= 2 && N <= 16 * BT_VL && wy_lds_bytes(N, nvec) <= c->lds_max; }
@@ -1587,26 +1615,39 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
                 return sizeof(double) * ((3 + tile_threads(tt) / 64) * (size_t)nb_ * ts +
                                          (size_t)nb_ * (nb_ | 1) * ts + 3 * (size_t)N + 8 * (size_t)nb_ * tt);
             };
-            if (real && tile_threads(7) <= TILE_MAX_THREADS) {
-                hipLaunchKernelGGL((tridiag_tile_real_kernel<7, 3>), dim3(nm), dim3(tile_threads(7)), tile_lds(7),
-                                   c->stream, a, (N + 6) / 7);
-                PMX_CHECK_HIP(hipGetLastError());
-                rc = 0;
-            } else if (real && tile_threads(8) <= TILE_MAX_THREADS) {
-                hipLaunchKernelGGL((tridiag_tile_real_kernel<8, 2>), dim3(nm), dim3(tile_threads(8)), tile_lds(8),
-                                   c->stream, a, (N + 7) / 8);
-                PMX_CHECK_HIP(hipGetLastError());
-                rc = 0;
-            } else if (real && N <= EIG_THREADS && lds_full <= c->lds_max) {
-                auto kern = tridiag_full_real_kernel;
-                PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                  (int)lds_full));
-                hipLaunchKernelGGL(kern, dim3(std::min(nm, c->num_cu * 2)), dim3(EIG_THREADS), lds_full, c->stream, a);
-                PMX_CHECK_HIP(hipGetLastError());
-                rc = 0;
-            } else {
+            /* real path: the register-tiled kernel (7x7 or 8x8 tiles) and the
+             * full-LDS kernel where they fit, chosen by the autotuner */
+            std::vector<TuneCand> rc_cands;
+            if (real && tile_threads(7) <= TILE_MAX_THREADS)
+                rc_cands.push_back({"tile7", [&](EigArgs &aa) {
+                                        hipLaunchKernelGGL((tridiag_tile_real_kernel<7, 3>), dim3(aa.nmat),
+                                                           dim3(tile_threads(7)), tile_lds(7), c->stream, aa, (N + 6) / 7);
+                                        PMX_CHECK_HIP(hipGetLastError());
+                                        return 0;
+                                    }});
+            if (real && tile_threads(8) <= TILE_MAX_THREADS)
+                rc_cands.push_back({"tile8", [&](EigArgs &aa) {
+                                        hipLaunchKernelGGL((tridiag_tile_real_kernel<8, 2>), dim3(aa.nmat),
+                                                           dim3(tile_threads(8)), tile_lds(8), c->stream, aa, (N + 7) / 8);
+                                        PMX_CHECK_HIP(hipGetLastError());
+                                        return 0;
+                                    }});
+            if (real && N <= EIG_THREADS && lds_full <= c->lds_max)
+                rc_cands.push_back({"full_lds", [&](EigArgs &aa) {
+                                        auto kern = tridiag_full_real_kernel;
+                                        PMX_CHECK_HIP(hipFuncSetAttribute(
+                                            (const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_full));
+                                        hipLaunchKernelGGL(kern, dim3(std::min(aa.nmat, c->num_cu * 2)), dim3(EIG_THREADS),
+                                                           lds_full, c->stream, aa);
+                                        PMX_CHECK_HIP(hipGetLastError());
+                                        return 0;
+                                    }});
+            if (!rc_cands.empty())
+                /* one matrix per workgroup: 16 x num_cu matrices are several rounds of
+                 * full residency, so occupancy differences show in the sample */
+                rc = tune_run(c, "real tiled N=" + std::to_string(N), a, 16 * c->num_cu, rc_cands);
+            else
                 rc = tridiag_generic(c, a, real, lds_matrix, grid, lds);
-            }
             if (rc) return rc;
         }
         {
