@@ -222,9 +222,10 @@ int pmx_grad_bands(pmx_ctx *ctx, int nk, const double *K, int nband, const doubl
  * nband even for the spatial solve (a Kramers pair is one solve state), any
  * value up to 2 npw with spin-orbit maps; PMX_EINVAL otherwise.  Eigenvector
  * derivatives by first-order perturbation over the whole basis (N = npw, or
- * 2 npw spinor components with SOC), pairs closer than 1e-8 eV masked; all N
- * eigenvectors of every k and k+q are held on the device (16 N^2 bytes per
- * k-point each). */
+ * 2 npw spinor components with SOC), pairs closer than 1e-8 eV masked.  The
+ * k-points go in batches of at most 512 (fewer for large N, about 4 GB of
+ * eigenvectors at k and k+q per batch) whose eigenpairs are recomputed there,
+ * so device memory does not grow with the grid. */
 int pmx_grad_chi(pmx_ctx *ctx, const pmx_chi_params *p, const double *gRe, const double *gIm, double *gvg);
 
 /* --------------------------------------------------------- host set-up

@@ -2424,6 +2424,16 @@ __global__ void adj_gz_kernel(int nact, int nfreq, int soc, double SF, double w0
 
 } // namespace
 
+/* upstream rows of one q, compact [ij][e2][f] = [r][f], from the full [ij][q][e2][f] arrays */
+static __global__ void gather_q_kernel(int nel, int ne2, int nq, int q, int nfreq, const double *src, double *dst) {
+    const size_t total = (size_t)nel * nfreq;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int r = idx / nfreq, f = idx % nfreq;
+        const int ij = r / ne2, e2 = r % ne2;
+        dst[idx] = src[(((size_t)ij * nq + q) * ne2 + e2) * nfreq + f];
+    }
+}
+
 extern "C" int pmx_grad_chi(pmx_ctx *c, const pmx_chi_params *p, const double *gRe, const double *gIm, double *gvg) {
     if (!c || !p || !gRe || !gIm || !gvg) return set_error(PMX_EINVAL, "null pointer");
     if (!c->has_system) return set_error(PMX_ESTATE, "no system set");
@@ -2439,15 +2449,20 @@ extern "C" int pmx_grad_chi(pmx_ctx *c, const pmx_chi_params *p, const double *g
     /* spatial: the npw-dimensional solve, Kramers pairs folded into weight 2 kw;
      * SOC: the 2 npw spinor solve (rows 2p up, 2p+1 down), four rho channels */
     const int neps = c->neps, npw = c->npw, D = 3 * neps, N = soc ? 2 * npw : npw, nch = soc ? 4 : 1;
-    const int ne2 = neps * (neps + 1) / 2, nel = 9 * ne2, nfreq = p->nfreq, nk = p->nk;
+    const int ne2 = neps * (neps + 1) / 2, nel = 9 * ne2, nfreq = p->nfreq, nk = p->nk, nq = p->nq;
     const int nbs = soc ? p->nband : p->nband / 2;
     const double SF = chi_scale(c), w0 = 0.0, df = p->dfreq;
     const int wh = window_half(p->delta, p->epsilon, df);
     const double tol = 1e-8; /* eV: pairs closer than this count as degenerate */
     std::fill(gvg, gvg + (size_t)c->natom * c->nv, 0.0);
-    if (nk == 0 || p->nq == 0) return PMX_OK;
+    if (nk == 0 || nq == 0) return PMX_OK;
     const size_t lds_t = sizeof(double) * 4 * (size_t)nch * D;
     if (lds_t > c->lds_max) return set_error(PMX_EINVAL, "NEPS too large for the adjoint");
+    /* k-points go in batches of at most 512 whose two full eigenvector sets
+     * (at k and at k+q, 16 N^2 bytes per k-point each) stay near 4 GB: the
+     * gradient is a sum over k, so each batch recomputes its eigenpairs and
+     * adds its share, and device memory does not grow with the grid */
+    const int KB = (int)std::max<size_t>(16, std::min<size_t>(512, (size_t)4e9 / (32 * (size_t)N * N + 1)));
 
     /* inverse loci: invl[m][l] = p with loci[m][p] = l (G_p -> G_p + G_m is one to one) */
     std::vector<int> invl((size_t)neps * npw, -1);
@@ -2456,25 +2471,39 @@ extern "C" int pmx_grad_chi(pmx_ctx *c, const pmx_chi_params *p, const double *g
             const int l = c->loci_h[(size_t)m * npw + q];
             if (l >= 0) invl[(size_t)m * npw + l] = q;
         }
-    std::vector<double> Kh(p->K, p->K + 3 * (size_t)nk), KWh(p->KW, p->KW + nk);
-    DBuf<double> Kd, KQd, KWd, evk, evkq, dEd, wtd, gdE, gEk, gEkq, gr, gi, gr0, gi0, KkT, out;
+    /* active elements of each q: rows r = ij ne2 + e2 with a non-zero upstream gradient */
+    std::vector<std::vector<int>> rows_q(nq), pairs_q(nq);
+    for (int q = 0; q < nq; q++)
+        for (int r = 0; r < nel; r++) {
+            const int ij = r / ne2, e2 = r % ne2;
+            const size_t src = (((size_t)ij * nq + q) * ne2 + e2) * nfreq;
+            bool any = false;
+            for (int f = 0; f < nfreq && !any; f++) any = gRe[src + f] != 0.0 || gIm[src + f] != 0.0;
+            if (!any) continue;
+            int m = 0;
+            while ((m + 1) * (m + 2) / 2 <= e2) m++;
+            const int n = e2 - m * (m + 1) / 2;
+            rows_q[q].push_back(r);
+            pairs_q[q].push_back(3 * m + ij / 3);
+            pairs_q[q].push_back(3 * n + ij % 3);
+        }
+    const size_t nup = (size_t)nel * nq * nfreq;
+    DBuf<double> Kd, KQd, KWd, evk, evkq, dEd, wtd, gdE, gEk, gEkq, gr, gi, gr0, gi0, KkT, out, gReD, gImD;
     DBuf<zd> vk, vkq, X, gX, gk, gkq, Y, ucd, wAd, wSd, gZ;
     DBuf<int> invd, nc, nv, tbase, pos, bad, act, arow, live;
     int rc;
-    if ((rc = upload(st, Kd, Kh)) || (rc = upload(st, KWd, KWh)) || (rc = upload(st, invd, invl))) return rc;
-    if ((rc = evk.ensure((size_t)nk * N)) || (rc = vk.ensure((size_t)nk * N * N)) || (rc = evkq.ensure((size_t)nk * N)) ||
-        (rc = vkq.ensure((size_t)nk * N * N)) || (rc = gk.ensure((size_t)nk * nbs * N)) ||
-        (rc = gkq.ensure((size_t)nk * nbs * N)) || (rc = Y.ensure((size_t)nk * nbs * N)) ||
-        (rc = gEk.ensure((size_t)nk * nbs)) || (rc = gEkq.ensure((size_t)nk * nbs)) ||
-        (rc = live.ensure((size_t)nk * nbs)) || (rc = nc.ensure(nk)) || (rc = nv.ensure(nk)) ||
-        (rc = tbase.ensure(nk + 1)) || (rc = bad.ensure(1)) || (rc = out.ensure((size_t)c->natom * c->nv)) ||
+    if ((rc = upload(st, invd, invl)) || (rc = gReD.ensure(nup)) || (rc = gImD.ensure(nup))) return rc;
+    PMX_CHECK_HIP(hipMemcpyAsync(gReD.p, gRe, sizeof(double) * nup, hipMemcpyHostToDevice, st));
+    PMX_CHECK_HIP(hipMemcpyAsync(gImD.p, gIm, sizeof(double) * nup, hipMemcpyHostToDevice, st));
+    if ((rc = evk.ensure((size_t)KB * N)) || (rc = vk.ensure((size_t)KB * N * N)) || (rc = evkq.ensure((size_t)KB * N)) ||
+        (rc = vkq.ensure((size_t)KB * N * N)) || (rc = gk.ensure((size_t)KB * nbs * N)) ||
+        (rc = gkq.ensure((size_t)KB * nbs * N)) || (rc = Y.ensure((size_t)KB * nbs * N)) ||
+        (rc = gEk.ensure((size_t)KB * nbs)) || (rc = gEkq.ensure((size_t)KB * nbs)) ||
+        (rc = live.ensure((size_t)KB * nbs)) || (rc = nc.ensure(KB)) || (rc = nv.ensure(KB)) ||
+        (rc = tbase.ensure(KB + 1)) || (rc = bad.ensure(1)) || (rc = out.ensure((size_t)c->natom * c->nv)) ||
         (rc = gr.ensure((size_t)nel * nfreq)) || (rc = gi.ensure((size_t)nel * nfreq)))
         return rc;
     PMX_CHECK_HIP(hipMemsetAsync(out.p, 0, sizeof(double) * c->natom * c->nv, st));
-    PMX_CHECK_HIP(hipMemsetAsync(gk.p, 0, sizeof(zd) * (size_t)nk * nbs * N, st));
-    PMX_CHECK_HIP(hipMemsetAsync(gEk.p, 0, sizeof(double) * (size_t)nk * nbs, st));
-    /* all N states at k: the perturbation sum runs over the whole basis */
-    if ((rc = eig_device(c, nk, Kd.p, N, N, evk.p, vk.p))) return rc;
     if (p->kk) {
         /* KkT[f][s] = Kk[s][f] of kkmat_kernel */
         std::vector<double> kt((size_t)nfreq * nfreq);
@@ -2489,113 +2518,104 @@ extern "C" int pmx_grad_chi(pmx_ctx *c, const pmx_chi_params *p, const double *g
     }
     const size_t lds_y = sizeof(zd) * (size_t)N;
     const size_t lds_hf = sizeof(double) * c->natom * c->nv;
-    std::vector<double> cre((size_t)nel * nfreq), cim((size_t)nel * nfreq);
-    for (int q = 0; q < p->nq; q++) {
-        /* upstream gradient of this q, compact [ij][e2][f], and its active elements */
-        std::vector<int> rows, pairs;
-        for (int r = 0; r < nel; r++) {
-            const int ij = r / ne2, e2 = r % ne2;
-            const size_t src = (((size_t)ij * p->nq + q) * ne2 + e2) * nfreq;
-            bool any = false;
-            for (int f = 0; f < nfreq; f++) {
-                cre[(size_t)r * nfreq + f] = gRe[src + f];
-                cim[(size_t)r * nfreq + f] = gIm[src + f];
-                any |= gRe[src + f] != 0.0 || gIm[src + f] != 0.0;
+    const unsigned ggrid = (unsigned)std::min<size_t>(((size_t)nel * nfreq + 255) / 256, 8192);
+    for (int k0 = 0; k0 < nk; k0 += KB) {
+        const int nkb = std::min(KB, nk - k0);
+        std::vector<double> Kh(p->K + 3 * (size_t)k0, p->K + 3 * (size_t)(k0 + nkb)), KWh(p->KW + k0, p->KW + k0 + nkb);
+        if ((rc = upload(st, Kd, Kh)) || (rc = upload(st, KWd, KWh))) return rc;
+        PMX_CHECK_HIP(hipMemsetAsync(gk.p, 0, sizeof(zd) * (size_t)nkb * nbs * N, st));
+        PMX_CHECK_HIP(hipMemsetAsync(gEk.p, 0, sizeof(double) * (size_t)nkb * nbs, st));
+        /* all N states at k: the perturbation sum runs over the whole basis */
+        if ((rc = eig_device(c, nkb, Kd.p, N, N, evk.p, vk.p))) return rc;
+        for (int q = 0; q < nq; q++) {
+            const int nact = (int)rows_q[q].size();
+            if (nact == 0) continue;
+            if ((rc = upload(st, arow, rows_q[q])) || (rc = upload(st, act, pairs_q[q])) ||
+                (rc = gZ.ensure((size_t)(soc ? 2 : 1) * nact * nfreq)))
+                return rc;
+            hipLaunchKernelGGL(gather_q_kernel, dim3(ggrid), dim3(256), 0, st, nel, ne2, nq, q, nfreq, gReD.p, gr.p);
+            hipLaunchKernelGGL(gather_q_kernel, dim3(ggrid), dim3(256), 0, st, nel, ne2, nq, q, nfreq, gImD.p, gi.p);
+            const double *g_re = gr.p, *g_im = gi.p;
+            if (p->kk) {
+                /* (1) the transposed KK matrix in the forward's GEMM: compact in, compact out */
+                const dim3 grid(8 * (((nel + 63) / 64 + 7) / 8) * ((nfreq + 16 * KK_CT - 1) / (16 * KK_CT)));
+                hipLaunchKernelGGL(kk_gemm_kernel<double>, grid, dim3(256), 0, st, nel, nfreq, gr.p, gi.p, KkT.p,
+                                   gr0.p, gi0.p, ne2, 1, 0);
+                g_re = gr0.p;
+                g_im = gi0.p;
             }
-            if (!any) continue;
-            int m = 0;
-            while ((m + 1) * (m + 2) / 2 <= e2) m++;
-            const int n = e2 - m * (m + 1) / 2;
-            rows.push_back(r);
-            pairs.push_back(3 * m + ij / 3);
-            pairs.push_back(3 * n + ij % 3);
-        }
-        const int nact = (int)rows.size();
-        if (nact == 0) continue;
-        if ((rc = upload(st, arow, rows)) || (rc = upload(st, act, pairs)) ||
-            (rc = gZ.ensure((size_t)(soc ? 2 : 1) * nact * nfreq)))
-            return rc;
-        PMX_CHECK_HIP(hipMemcpyAsync(gr.p, cre.data(), sizeof(double) * cre.size(), hipMemcpyHostToDevice, st));
-        PMX_CHECK_HIP(hipMemcpyAsync(gi.p, cim.data(), sizeof(double) * cim.size(), hipMemcpyHostToDevice, st));
-        const double *g_re = gr.p, *g_im = gi.p;
-        if (p->kk) {
-            /* (1) the transposed KK matrix in the forward's GEMM: compact in, compact out */
-            const dim3 grid(8 * (((nel + 63) / 64 + 7) / 8) * ((nfreq + 16 * KK_CT - 1) / (16 * KK_CT)));
-            hipLaunchKernelGGL(kk_gemm_kernel<double>, grid, dim3(256), 0, st, nel, nfreq, gr.p, gi.p, KkT.p, gr0.p,
-                               gi0.p, ne2, 1, 0);
-            g_re = gr0.p;
-            g_im = gi0.p;
-        }
-        hipLaunchKernelGGL(adj_gz_kernel, dim3(256), dim3(256), 0, st, nact, nfreq, soc ? 1 : 0, SF, w0, df, arow.p,
-                           g_re, g_im, gZ.p);
+            hipLaunchKernelGGL(adj_gz_kernel, dim3(256), dim3(256), 0, st, nact, nfreq, soc ? 1 : 0, SF, w0, df, arow.p,
+                               g_re, g_im, gZ.p);
 
-        /* forward quantities of this q: eigenpairs at k+q, transitions, rho in transition order */
-        const double Qx = p->Q[3 * q], Qy = p->Q[3 * q + 1], Qz = p->Q[3 * q + 2];
-        std::vector<zd> uc, wA, wS;
-        q_tables(c, Qx, Qy, Qz, !soc, uc, wA, wS);
-        if ((rc = upload(st, ucd, uc)) || (rc = upload(st, wAd, wA)) || (soc && (rc = upload(st, wSd, wS)))) return rc;
-        std::vector<double> KQ(3 * (size_t)nk);
-        for (int k = 0; k < nk; k++) {
-            KQ[3 * k] = Kh[3 * k] + Qx;
-            KQ[3 * k + 1] = Kh[3 * k + 1] + Qy;
-            KQ[3 * k + 2] = Kh[3 * k + 2] + Qz;
-        }
-        if ((rc = upload(st, KQd, KQ))) return rc;
-        if ((rc = eig_device(c, nk, KQd.p, N, N, evkq.p, vkq.p))) return rc;
-        PMX_CHECK_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), st));
-        TransArgs ta{nk, N, p->nband, soc ? 0 : 1, evk.p, evkq.p, p->offset, nc.p, nv.p, bad.p, 0};
-        hipLaunchKernelGGL(classify_kernel, dim3((nk + 255) / 256), dim3(256), 0, st, ta);
-        std::vector<int> nch_h(nk), nvh(nk), tb(nk + 1);
-        int badh = 0;
-        PMX_CHECK_HIP(hipMemcpyAsync(nch_h.data(), nc.p, sizeof(int) * nk, hipMemcpyDeviceToHost, st));
-        PMX_CHECK_HIP(hipMemcpyAsync(nvh.data(), nv.p, sizeof(int) * nk, hipMemcpyDeviceToHost, st));
-        PMX_CHECK_HIP(hipMemcpyAsync(&badh, bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
-        PMX_CHECK_HIP(hipStreamSynchronize(st));
-        if (badh) return set_error(PMX_EBANDS, "at some k no level at or below the energy zero lies within -nband, "
-                                               "or at some k+q none above it");
-        tb[0] = 0;
-        for (int k = 0; k < nk; k++) tb[k + 1] = tb[k] + (soc ? nch_h[k] * nvh[k] : (nch_h[k] / 2) * (nvh[k] / 2));
-        const int ntr = tb[nk];
-        if (ntr == 0) continue;
-        std::vector<int> iota(ntr);
-        for (int t = 0; t < ntr; t++) iota[t] = t;
-        if ((rc = upload(st, tbase, tb)) || (rc = upload(st, pos, iota)) || (rc = dEd.ensure(ntr)) ||
-            (rc = wtd.ensure(ntr)) || (rc = gdE.ensure(ntr)) || (rc = X.ensure((size_t)ntr * nch * D)) ||
-            (rc = gX.ensure((size_t)ntr * nch * D)))
-            return rc;
-        hipLaunchKernelGGL(dE_kernel, dim3(nk), dim3(256), 0, st, nk, N, p->nband, soc ? 0 : 1, evk.p, evkq.p,
-                           p->offset, nc.p, nv.p, tbase.p, KWd.p, dEd.p, wtd.p, 0);
-        RhoArgs ra{};
-        ra.nk = nk; ra.N = N; ra.npw = npw; ra.neps = neps; ra.nb_s = N; ra.nband = p->nband;
-        ra.spatial = soc ? 0 : 1; ra.nch = nch;
-        ra.vk = vk.p; ra.vkq = vkq.p; ra.nc = nc.p; ra.nv = nv.p; ra.tbase = tbase.p; ra.pos = pos.p;
-        ra.loci = c->dLoci.p; ra.wA = wAd.p; ra.uc = ucd.p; ra.wS = soc ? wSd.p : nullptr; ra.K = Kd.p; ra.X = X.p;
-        hipLaunchKernelGGL(rho_kernel, dim3(nk), dim3(256), 0, st, ra);
+            /* forward quantities of this q: eigenpairs at k+q, transitions, rho in transition order */
+            const double Qx = p->Q[3 * q], Qy = p->Q[3 * q + 1], Qz = p->Q[3 * q + 2];
+            std::vector<zd> uc, wA, wS;
+            q_tables(c, Qx, Qy, Qz, !soc, uc, wA, wS);
+            if ((rc = upload(st, ucd, uc)) || (rc = upload(st, wAd, wA)) || (soc && (rc = upload(st, wSd, wS)))) return rc;
+            std::vector<double> KQ(3 * (size_t)nkb);
+            for (int k = 0; k < nkb; k++) {
+                KQ[3 * k] = Kh[3 * k] + Qx;
+                KQ[3 * k + 1] = Kh[3 * k + 1] + Qy;
+                KQ[3 * k + 2] = Kh[3 * k + 2] + Qz;
+            }
+            if ((rc = upload(st, KQd, KQ))) return rc;
+            if ((rc = eig_device(c, nkb, KQd.p, N, N, evkq.p, vkq.p))) return rc;
+            PMX_CHECK_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), st));
+            TransArgs ta{nkb, N, p->nband, soc ? 0 : 1, evk.p, evkq.p, p->offset, nc.p, nv.p, bad.p, 0};
+            hipLaunchKernelGGL(classify_kernel, dim3((nkb + 255) / 256), dim3(256), 0, st, ta);
+            std::vector<int> nch_h(nkb), nvh(nkb), tb(nkb + 1);
+            int badh = 0;
+            PMX_CHECK_HIP(hipMemcpyAsync(nch_h.data(), nc.p, sizeof(int) * nkb, hipMemcpyDeviceToHost, st));
+            PMX_CHECK_HIP(hipMemcpyAsync(nvh.data(), nv.p, sizeof(int) * nkb, hipMemcpyDeviceToHost, st));
+            PMX_CHECK_HIP(hipMemcpyAsync(&badh, bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            PMX_CHECK_HIP(hipStreamSynchronize(st));
+            if (badh) return set_error(PMX_EBANDS, "at some k no level at or below the energy zero lies within -nband, "
+                                                   "or at some k+q none above it");
+            tb[0] = 0;
+            for (int k = 0; k < nkb; k++) tb[k + 1] = tb[k] + (soc ? nch_h[k] * nvh[k] : (nch_h[k] / 2) * (nvh[k] / 2));
+            const int ntr = tb[nkb];
+            if (ntr == 0) continue;
+            std::vector<int> iota(ntr);
+            for (int t = 0; t < ntr; t++) iota[t] = t;
+            if ((rc = upload(st, tbase, tb)) || (rc = upload(st, pos, iota)) || (rc = dEd.ensure(ntr)) ||
+                (rc = wtd.ensure(ntr)) || (rc = gdE.ensure(ntr)) || (rc = X.ensure((size_t)ntr * nch * D)) ||
+                (rc = gX.ensure((size_t)ntr * nch * D)))
+                return rc;
+            hipLaunchKernelGGL(dE_kernel, dim3(nkb), dim3(256), 0, st, nkb, N, p->nband, soc ? 0 : 1, evk.p, evkq.p,
+                               p->offset, nc.p, nv.p, tbase.p, KWd.p, dEd.p, wtd.p, 0);
+            RhoArgs ra{};
+            ra.nk = nkb; ra.N = N; ra.npw = npw; ra.neps = neps; ra.nb_s = N; ra.nband = p->nband;
+            ra.spatial = soc ? 0 : 1; ra.nch = nch;
+            ra.vk = vk.p; ra.vkq = vkq.p; ra.nc = nc.p; ra.nv = nv.p; ra.tbase = tbase.p; ra.pos = pos.p;
+            ra.loci = c->dLoci.p; ra.wA = wAd.p; ra.uc = ucd.p; ra.wS = soc ? wSd.p : nullptr; ra.K = Kd.p; ra.X = X.p;
+            hipLaunchKernelGGL(rho_kernel, dim3(nkb), dim3(256), 0, st, ra);
 
-        /* (3) per transition */
-        AdjTArgs at{D, nch, nact, nfreq, wh, p->delta, p->epsilon, w0, df, X.p, dEd.p, wtd.p, act.p, gZ.p, gX.p, gdE.p};
-        hipLaunchKernelGGL(chi_adj_trans_kernel, dim3(ntr), dim3(256), lds_t, st, at);
-        /* (4) onto the wave vectors */
-        PMX_CHECK_HIP(hipMemsetAsync(gkq.p, 0, sizeof(zd) * (size_t)nk * nbs * N, st));
-        PMX_CHECK_HIP(hipMemsetAsync(gEkq.p, 0, sizeof(double) * (size_t)nk * nbs, st));
-        AdjVArgs av{N, npw, neps, p->nband, D, nbs, soc ? 1 : 0, nc.p, nv.p, tbase.p, vk.p, vkq.p, c->dLoci.p,
-                    invd.p, wAd.p, ucd.p, soc ? wSd.p : nullptr, Kd.p, gX.p, gdE.p, gk.p, gkq.p, gEk.p, gEkq.p};
-        hipLaunchKernelGGL(chi_adj_evec_kernel, dim3(nk), dim3(256), sizeof(zd) * 3 * neps, st, av);
-        /* (5)-(6) for the states at k+q */
-        hipLaunchKernelGGL(adj_y_kernel, dim3(nk * nbs), dim3(256), lds_y, st, N, nbs, tol, vkq.p, evkq.p, gkq.p,
-                           gEkq.p, Y.p, live.p);
-        hipLaunchKernelGGL(hf_adj_kernel, dim3(nk * nbs), dim3(256), lds_hf, st, npw, N, c->natom, c->nv, nbs,
-                           soc ? 1 : 0, vkq.p, Y.p, live.p, c->dShell.p, c->dSfac.p, out.p);
+            /* (3) per transition */
+            AdjTArgs at{D, nch, nact, nfreq, wh, p->delta, p->epsilon, w0, df, X.p, dEd.p, wtd.p, act.p, gZ.p, gX.p, gdE.p};
+            hipLaunchKernelGGL(chi_adj_trans_kernel, dim3(ntr), dim3(256), lds_t, st, at);
+            /* (4) onto the wave vectors */
+            PMX_CHECK_HIP(hipMemsetAsync(gkq.p, 0, sizeof(zd) * (size_t)nkb * nbs * N, st));
+            PMX_CHECK_HIP(hipMemsetAsync(gEkq.p, 0, sizeof(double) * (size_t)nkb * nbs, st));
+            AdjVArgs av{N, npw, neps, p->nband, D, nbs, soc ? 1 : 0, nc.p, nv.p, tbase.p, vk.p, vkq.p, c->dLoci.p,
+                        invd.p, wAd.p, ucd.p, soc ? wSd.p : nullptr, Kd.p, gX.p, gdE.p, gk.p, gkq.p, gEk.p, gEkq.p};
+            hipLaunchKernelGGL(chi_adj_evec_kernel, dim3(nkb), dim3(256), sizeof(zd) * 3 * neps, st, av);
+            /* (5)-(6) for the states at k+q */
+            hipLaunchKernelGGL(adj_y_kernel, dim3(nkb * nbs), dim3(256), lds_y, st, N, nbs, tol, vkq.p, evkq.p, gkq.p,
+                               gEkq.p, Y.p, live.p);
+            hipLaunchKernelGGL(hf_adj_kernel, dim3(nkb * nbs), dim3(256), lds_hf, st, npw, N, c->natom, c->nv, nbs,
+                               soc ? 1 : 0, vkq.p, Y.p, live.p, c->dShell.p, c->dSfac.p, out.p);
+            PMX_CHECK_HIP(hipGetLastError());
+            /* host buffers above are reused next q */
+            PMX_CHECK_HIP(hipStreamSynchronize(st));
+        }
+        /* (5)-(6) for the states at k of this batch, accumulated over q */
+        hipLaunchKernelGGL(adj_y_kernel, dim3(nkb * nbs), dim3(256), lds_y, st, N, nbs, tol, vk.p, evk.p, gk.p, gEk.p,
+                           Y.p, live.p);
+        hipLaunchKernelGGL(hf_adj_kernel, dim3(nkb * nbs), dim3(256), lds_hf, st, npw, N, c->natom, c->nv, nbs,
+                           soc ? 1 : 0, vk.p, Y.p, live.p, c->dShell.p, c->dSfac.p, out.p);
         PMX_CHECK_HIP(hipGetLastError());
-        /* host buffers above are reused next q */
         PMX_CHECK_HIP(hipStreamSynchronize(st));
     }
-    /* (5)-(6) for the states at k, accumulated over q */
-    hipLaunchKernelGGL(adj_y_kernel, dim3(nk * nbs), dim3(256), lds_y, st, N, nbs, tol, vk.p, evk.p, gk.p, gEk.p, Y.p,
-                       live.p);
-    hipLaunchKernelGGL(hf_adj_kernel, dim3(nk * nbs), dim3(256), lds_hf, st, npw, N, c->natom, c->nv, nbs, soc ? 1 : 0,
-                       vk.p, Y.p, live.p, c->dShell.p, c->dSfac.p, out.p);
-    PMX_CHECK_HIP(hipGetLastError());
     PMX_CHECK_HIP(hipMemcpyAsync(gvg, out.p, sizeof(double) * c->natom * c->nv, hipMemcpyDeviceToHost, st));
     PMX_CHECK_HIP(hipStreamSynchronize(st));
     return PMX_OK;

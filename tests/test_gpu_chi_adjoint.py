@@ -195,3 +195,26 @@ def test_eps_m_fit_end_to_end(setup):
     err = np.abs(g - g_fd).max() / np.abs(g_fd).max()
     _say("  rel err %.3e" % err)
     assert err < 1e-6
+
+
+def test_chi_adjoint_batches_add_up():
+    """More k-points than one internal batch (512 at this basis size): the
+    gradient equals the sum of the gradients of two disjoint halves."""
+    from picomax_amd import autograd
+    model = autograd.BandModel(_zb(5.43, [3, 4, 8, 11] + [-0.2241, 0.0, 0.0551, 0.0724] * 2), refpoint=(0.5, 0.5, 0.5),
+                               nvalence=8)
+    off = model.offset()
+    rng = np.random.default_rng(21)
+    K = rng.uniform(-0.5, 0.5, (700, 3))
+    KW = rng.uniform(0.5, 1.5, len(K)) / len(K)
+    ne2 = 9 * 10 // 2
+    shape = (9, 1, ne2, NFREQ)
+    gRe, gIm = rng.standard_normal(shape), rng.standard_normal(shape)
+    q = Q[:1]
+    _say("chi adjoint over 700 k-points in batches")
+    g = model.ctx.grad_chi(K, KW, q, 16, off, 1, EPS, NFREQ, DF, gRe, gIm, kk=1)
+    g1 = model.ctx.grad_chi(K[:350], KW[:350], q, 16, off, 1, EPS, NFREQ, DF, gRe, gIm, kk=1)
+    g2 = model.ctx.grad_chi(K[350:], KW[350:], q, 16, off, 1, EPS, NFREQ, DF, gRe, gIm, kk=1)
+    err = np.abs(g - (g1 + g2)).max() / np.abs(g).max()
+    _say("  rel diff %.3e" % err)
+    assert err < 1e-11
