@@ -1227,14 +1227,15 @@ __global__ void __launch_bounds__(EIG_THREADS) backtrans_kernel(int N, int nmat,
 typedef double dx4 __attribute__((ext_vector_type(4)));
 constexpr int BW = 16;
 constexpr int VS = BW + 1;
-constexpr int BT_VL = 16; /* reflector values per thread: Np <= 256 */
+constexpr int BT_VL_MAX = 16; /* reflector values per thread: Np <= 256 */
 
 
 __device__ __forceinline__ dx4 mma64(double a, double b, dx4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-__global__ void __launch_bounds__(256) backtrans_wy_kernel(int N, int nmat, int nvec, int NV, const double *Y,
+template <int VL, int WPE>
+__global__ void __launch_bounds__(256, WPE) backtrans_wy_kernel(int N, int nmat, int nvec, int NV, const double *Y,
                                                             const double *vref_all, const double *tau_all,
                                                             const double *phase, const int *vr, zd *evecs) {
     extern __shared__ double smem_d[];
@@ -1268,13 +1269,13 @@ __global__ void __launch_bounds__(256) backtrans_wy_kernel(int N, int nmat, int 
         }
     }
     const int nref = N - 1, nct = (cnt + 15) / 16, nrt = Np / 16; /* column tiles holding wanted vectors */
-    /* the reflector block bi is held in registers (BT_VL values per thread,
-     * Np * BW <= 256 * BT_VL) while block bi + 1 is applied */
-    double vreg[BT_VL];
+    /* the reflector block bi is held in registers (VL values per thread,
+     * Np * BW <= 256 * VL) while block bi + 1 is applied */
+    double vreg[VL];
     double treg = 0.0;
     auto load_v = [&](int kb) {
 #pragma unroll
-        for (int u = 0; u < BT_VL; u++) {
+        for (int u = 0; u < VL; u++) {
             const int idx = u * 256 + tid;
             const int r = idx % Np, kk = idx / Np, k = kb + kk;
             double v = 0.0;
@@ -1288,7 +1289,7 @@ __global__ void __launch_bounds__(256) backtrans_wy_kernel(int N, int nmat, int 
     };
     auto store_v = [&]() {
 #pragma unroll
-        for (int u = 0; u < BT_VL; u++) {
+        for (int u = 0; u < VL; u++) {
             const int idx = u * 256 + tid;
             if (idx < Np * BW) Vs[(idx % Np) * VS + idx / Np] = vreg[u];
         }
@@ -1444,30 +1445,31 @@ template <class T, bool LDS> int launch_tridiag_fused(pmx_ctx *c, EigArgs &a, in
  * decision is logged for pmx_tune_report.  Smaller batches use the first
  * candidate without timing.  PMX_AUTOTUNE=0 disables the timing (first
  * candidate); PMX_TRIDIAG_FORM=<name> forces a candidate by name. */
+/* a candidate processes the first nm matrices of the batch (nm = the batch
+ * size for the real run, the sample size while timing) */
 struct TuneCand {
     const char *name;
-    std::function<int(EigArgs &)> run;
+    std::function<int(int)> run;
 };
 
-int tune_run(pmx_ctx *c, const std::string &key, EigArgs &a, int sample, const std::vector<TuneCand> &cands) {
+int tune_run(pmx_ctx *c, const std::string &key, int nmat, int sample, const std::vector<TuneCand> &cands) {
     const char *form = std::getenv("PMX_TRIDIAG_FORM");
     if (form)
         for (const auto &cd : cands)
-            if (std::strcmp(form, cd.name) == 0) return cd.run(a);
-    if (cands.size() == 1) return cands[0].run(a);
+            if (std::strcmp(form, cd.name) == 0) return cd.run(nmat);
+    if (cands.size() == 1) return cands[0].run(nmat);
     auto it = c->tune_choice.find(key);
-    if (it != c->tune_choice.end()) return cands[it->second].run(a);
+    if (it != c->tune_choice.end()) return cands[it->second].run(nmat);
     const char *env = std::getenv("PMX_AUTOTUNE");
-    if ((env && env[0] == '0') || a.nmat < 2 * c->num_cu) return cands[0].run(a);
-    EigArgs s = a;
-    s.nmat = std::min(a.nmat, sample);
+    if ((env && env[0] == '0') || nmat < 2 * c->num_cu) return cands[0].run(nmat);
+    const int ns = std::min(nmat, sample);
     const int nc = (int)cands.size();
     std::vector<hipEvent_t> ev(nc + 1);
     for (auto &x : ev) PMX_CHECK_HIP(hipEventCreate(&x));
     int rc = 0;
     PMX_CHECK_HIP(hipEventRecord(ev[0], c->stream));
     for (int i = 0; i < nc && !rc; i++) {
-        rc = cands[i].run(s);
+        rc = cands[i].run(ns);
         PMX_CHECK_HIP(hipEventRecord(ev[i + 1], c->stream));
     }
     PMX_CHECK_HIP(hipEventSynchronize(ev[nc]));
@@ -1481,7 +1483,7 @@ int tune_run(pmx_ctx *c, const std::string &key, EigArgs &a, int sample, const s
     for (int i = 1; i < nc; i++)
         if (ms[i] < 0.97f * ms[best]) best = i;
     c->tune_choice[key] = best;
-    std::string rec = "{\"kernel\": \"tridiag " + key + "\", \"sample\": " + std::to_string(s.nmat) + ", \"ms\": {";
+    std::string rec = "{\"kernel\": \"" + key + "\", \"sample\": " + std::to_string(ns) + ", \"ms\": {";
     char num[64];
     for (int i = 0; i < nc; i++) {
         std::snprintf(num, sizeof(num), "%.4f", ms[i]);
@@ -1489,7 +1491,7 @@ int tune_run(pmx_ctx *c, const std::string &key, EigArgs &a, int sample, const s
     }
     rec += std::string("}, \"chosen\": \"") + cands[best].name + "\"}";
     c->tune_log.push_back(rec);
-    return cands[best].run(a);
+    return cands[best].run(nmat);
 }
 
 /* The generic tridiagonalisation (any N, real or complex, matrix in LDS or in
@@ -1501,39 +1503,55 @@ int tridiag_generic(pmx_ctx *c, EigArgs &a, bool real, bool lds_matrix, int grid
     const int N = a.N;
     const size_t ldsf = lds_fused_bytes(N, real ? sizeof(double) : sizeof(zd), lds_matrix);
     std::vector<TuneCand> cands;
-    cands.push_back({"three_stream", [=](EigArgs &aa) {
-                         const int g = std::min(grid, aa.nmat);
+    cands.push_back({"three_stream", [=](int nm) {
+                         EigArgs aa = a;
+                         aa.nmat = nm;
+                         const int g = std::min(grid, nm);
                          if (real) return lds_matrix ? launch_tridiag<double, true>(c, aa, g, lds)
                                                      : launch_tridiag<double, false>(c, aa, g, lds);
                          return lds_matrix ? launch_tridiag<zd, true>(c, aa, g, lds) : launch_tridiag<zd, false>(c, aa, g, lds);
                      }});
     if (ldsf <= c->lds_max)
-        cands.push_back({"fused", [=](EigArgs &aa) {
-                             const int g = std::min(grid, aa.nmat);
+        cands.push_back({"fused", [=](int nm) {
+                             EigArgs aa = a;
+                             aa.nmat = nm;
+                             const int g = std::min(grid, nm);
                              if (real) return lds_matrix ? launch_tridiag_fused<double, true>(c, aa, g, ldsf)
                                                          : launch_tridiag_fused<double, false>(c, aa, g, ldsf);
                              return lds_matrix ? launch_tridiag_fused<zd, true>(c, aa, g, ldsf)
                                                : launch_tridiag_fused<zd, false>(c, aa, g, ldsf);
                          }});
-    const std::string key = std::string(real ? "real" : "complex") + (lds_matrix ? " lds N=" : " global N=") +
-                            std::to_string(N);
-    return tune_run(c, key, a, 2 * grid, cands);
+    const std::string key = std::string(real ? "tridiag real" : "tridiag complex") +
+                            (lds_matrix ? " lds N=" : " global N=") + std::to_string(N);
+    return tune_run(c, key, a.nmat, 2 * grid, cands);
 }
 
-bool wy_usable(pmx_ctx *c, int N, int nvec) { return N >= 2 && N <= 16 * BT_VL && wy_lds_bytes(N, nvec) <= c->lds_max; }
+bool wy_usable(pmx_ctx *c, int N, int nvec) { return N >= 2 && N <= 16 * BT_VL_MAX && wy_lds_bytes(N, nvec) <= c->lds_max; }
 
 template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, const double *Y, const void *vref,
                                         const void *tau, const double *phase, const int *vr, zd *evecs) {
     if (sizeof(T) == sizeof(double) && wy_usable(c, N, nvec)) {
         const size_t lds = wy_lds_bytes(N, nvec);
-        if (lds > 65536)
-            PMX_CHECK_HIP(hipFuncSetAttribute((const void *)backtrans_wy_kernel,
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(backtrans_wy_kernel, dim3(nmat), dim3(256), lds, c->stream, N, nmat, nvec,
-                           (nvec + 15) & ~15, Y, reinterpret_cast<const double *>(vref),
-                           reinterpret_cast<const double *>(tau), phase, vr, evecs);
-        PMX_CHECK_HIP(hipGetLastError());
-        return 0;
+        /* two register budgets: 16 reflector values per thread (any Np <= 256,
+         * three waves per SIMD) or 10 (Np <= 160, capped at 128 VGPRs for four
+         * waves per SIMD); the autotuner picks per shape.  Each run rewrites
+         * the same vectors from the same inputs, so sample runs are harmless. */
+        auto launch = [&](auto kern, int g) -> int {
+            if (lds > 65536)
+                PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(kern, dim3(g), dim3(256), lds, c->stream, N, nmat, nvec, (nvec + 15) & ~15, Y,
+                               reinterpret_cast<const double *>(vref), reinterpret_cast<const double *>(tau), phase,
+                               vr, evecs);
+            PMX_CHECK_HIP(hipGetLastError());
+            return 0;
+        };
+        const int Np = (N + 15) & ~15;
+        std::vector<TuneCand> cands;
+        cands.push_back({"wy_vl16", [&](int g) { return launch(backtrans_wy_kernel<16, 1>, g); }});
+        if (Np * BW <= 256 * 10)
+            cands.push_back({"wy_vl10", [&](int g) { return launch(backtrans_wy_kernel<10, 4>, g); }});
+        return tune_run(c, "backtrans real N=" + std::to_string(N) + " nvec=" + std::to_string(nvec), nmat,
+                        16 * c->num_cu, cands);
     }
     /* staging depth and vector group size that fit the LDS */
     const size_t cap = c->lds_max;
@@ -1619,21 +1637,27 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
              * full-LDS kernel where they fit, chosen by the autotuner */
             std::vector<TuneCand> rc_cands;
             if (real && tile_threads(7) <= TILE_MAX_THREADS)
-                rc_cands.push_back({"tile7", [&](EigArgs &aa) {
+                rc_cands.push_back({"tile7", [&](int nms) {
+                                        EigArgs aa = a;
+                                        aa.nmat = nms;
                                         hipLaunchKernelGGL((tridiag_tile_real_kernel<7, 3>), dim3(aa.nmat),
                                                            dim3(tile_threads(7)), tile_lds(7), c->stream, aa, (N + 6) / 7);
                                         PMX_CHECK_HIP(hipGetLastError());
                                         return 0;
                                     }});
             if (real && tile_threads(8) <= TILE_MAX_THREADS)
-                rc_cands.push_back({"tile8", [&](EigArgs &aa) {
+                rc_cands.push_back({"tile8", [&](int nms) {
+                                        EigArgs aa = a;
+                                        aa.nmat = nms;
                                         hipLaunchKernelGGL((tridiag_tile_real_kernel<8, 2>), dim3(aa.nmat),
                                                            dim3(tile_threads(8)), tile_lds(8), c->stream, aa, (N + 7) / 8);
                                         PMX_CHECK_HIP(hipGetLastError());
                                         return 0;
                                     }});
             if (real && N <= EIG_THREADS && lds_full <= c->lds_max)
-                rc_cands.push_back({"full_lds", [&](EigArgs &aa) {
+                rc_cands.push_back({"full_lds", [&](int nms) {
+                                        EigArgs aa = a;
+                                        aa.nmat = nms;
                                         auto kern = tridiag_full_real_kernel;
                                         PMX_CHECK_HIP(hipFuncSetAttribute(
                                             (const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_full));
@@ -1645,7 +1669,7 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
             if (!rc_cands.empty())
                 /* one matrix per workgroup: 16 x num_cu matrices are several rounds of
                  * full residency, so occupancy differences show in the sample */
-                rc = tune_run(c, "real tiled N=" + std::to_string(N), a, 16 * c->num_cu, rc_cands);
+                rc = tune_run(c, "tridiag real tiled N=" + std::to_string(N), a.nmat, 16 * c->num_cu, rc_cands);
             else
                 rc = tridiag_generic(c, a, real, lds_matrix, grid, lds);
             if (rc) return rc;
