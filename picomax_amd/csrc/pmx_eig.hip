@@ -1444,7 +1444,7 @@ template <class T, bool LDS> int launch_tridiag_fused(pmx_ctx *c, EigArgs &a, in
  * for the context; later batches reuse the choice, and the
  * decision is logged for pmx_tune_report.  Smaller batches use the first
  * candidate without timing.  PMX_AUTOTUNE=0 disables the timing (first
- * candidate); PMX_TRIDIAG_FORM=<name> forces a candidate by name. */
+ * candidate); PMX_FORCE_KERNEL=<name> forces a candidate by name. */
 /* a candidate processes the first nm matrices of the batch (nm = the batch
  * size for the real run, the sample size while timing) */
 struct TuneCand {
@@ -1453,7 +1453,7 @@ struct TuneCand {
 };
 
 int tune_run(pmx_ctx *c, const std::string &key, int nmat, int sample, const std::vector<TuneCand> &cands) {
-    const char *form = std::getenv("PMX_TRIDIAG_FORM");
+    const char *form = std::getenv("PMX_FORCE_KERNEL");
     if (form)
         for (const auto &cd : cands)
             if (std::strcmp(form, cd.name) == 0) return cd.run(nmat);
@@ -1464,19 +1464,26 @@ int tune_run(pmx_ctx *c, const std::string &key, int nmat, int sample, const std
     if ((env && env[0] == '0') || nmat < 2 * c->num_cu) return cands[0].run(nmat);
     const int ns = std::min(nmat, sample);
     const int nc = (int)cands.size();
-    std::vector<hipEvent_t> ev(nc + 1);
-    for (auto &x : ev) PMX_CHECK_HIP(hipEventCreate(&x));
+    /* events are released on every exit path */
+    struct Events {
+        std::vector<hipEvent_t> e;
+        ~Events() {
+            for (auto &x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    ev.e.assign(nc + 1, nullptr);
+    for (auto &x : ev.e) PMX_CHECK_HIP(hipEventCreate(&x));
     int rc = 0;
-    PMX_CHECK_HIP(hipEventRecord(ev[0], c->stream));
+    PMX_CHECK_HIP(hipEventRecord(ev.e[0], c->stream));
     for (int i = 0; i < nc && !rc; i++) {
         rc = cands[i].run(ns);
-        PMX_CHECK_HIP(hipEventRecord(ev[i + 1], c->stream));
+        PMX_CHECK_HIP(hipEventRecord(ev.e[i + 1], c->stream));
     }
-    PMX_CHECK_HIP(hipEventSynchronize(ev[nc]));
-    std::vector<float> ms(nc, 0.0f);
-    for (int i = 0; i < nc; i++) PMX_CHECK_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
-    for (auto &x : ev) PMX_CHECK_HIP(hipEventDestroy(x));
+    PMX_CHECK_HIP(hipEventSynchronize(ev.e[nc]));
     if (rc) return rc;
+    std::vector<float> ms(nc, 0.0f);
+    for (int i = 0; i < nc; i++) PMX_CHECK_HIP(hipEventElapsedTime(&ms[i], ev.e[i], ev.e[i + 1]));
     /* a later candidate must beat the current choice by 3%: timing noise
      * does not move the choice away from the first (default) candidate */
     int best = 0;

@@ -205,7 +205,7 @@ def test_epsinv_against_numpy(manifest, ctx):
 def test_tridiag_forms_against_oracle(manifest, case, form, monkeypatch):
     """Both forms of the generic tridiagonalisation (matrix in LDS here) give
     the oracle's eigenpairs; the autotuner may pick either."""
-    monkeypatch.setenv("PMX_TRIDIAG_FORM", form)
+    monkeypatch.setenv("PMX_FORCE_KERNEL", form)
     args = _expand(manifest[case]["args"])
     _say("tridiag form %s %s" % (form, case))
     c = capi.Context(0)
@@ -228,7 +228,7 @@ def test_tridiag_forms_against_oracle(manifest, case, form, monkeypatch):
 @pytest.mark.parametrize("form", ["fused", "three_stream"])
 def test_tridiag_forms_large_basis(form, monkeypatch):
     """Both forms with the matrix in global memory (wz, 401 plane waves)."""
-    monkeypatch.setenv("PMX_TRIDIAG_FORM", form)
+    monkeypatch.setenv("PMX_FORCE_KERNEL", form)
     args = LARGE["wz_401"]
     _say("tridiag form %s wz_401" % form)
     c = capi.Context(0)
