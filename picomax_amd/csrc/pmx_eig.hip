@@ -180,14 +180,18 @@ __device__ __forceinline__ double hash_unit(unsigned a, unsigned b) {
     return (double)(h & 0xFFFFFF) / 8388608.0 - 1.0;
 }
 
-template <class T, bool LDS>
+/* PG (large bases): the per-wave mat-vec partials and tau live in the
+ * workgroup's global slab after the matrix, so the LDS holds only d, e, v,
+ * w (2N doubles + 2N elements: N up to about 3,400 complex) */
+template <class T, bool LDS, bool PG = false>
 __global__ void __launch_bounds__(EIG_THREADS) tridiag_kernel(EigArgs a) {
     extern __shared__ double smem_d[];
     const int N = a.N;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t P = (size_t)N * (N + 1) / 2;
 
-    /* LDS: [A packed (LDS case)] [d N][e N] [tau N][v N][w N][part WAVES*N] (T) [red] */
+    /* LDS: [A packed (LDS case)] [d N][e N] [tau N][v N][w N][part WAVES*N] (T) [red]; with PG
+     * tau and part are in the slab instead */
     char *sp = reinterpret_cast<char *>(smem_d);
     T *A;
     if (LDS) {
@@ -198,10 +202,20 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_kernel(EigArgs a) {
     }
     double *d = reinterpret_cast<double *>(sp); sp += sizeof(double) * N;
     double *e = reinterpret_cast<double *>(sp); sp += sizeof(double) * N;
-    T *tauv = reinterpret_cast<T *>(sp); sp += sizeof(T) * N;
+    T *tauv, *part;
+    if (PG) {
+        part = reinterpret_cast<T *>(a.scratch + (size_t)blockIdx.x * a.slab + ((P * sizeof(T) + 255) / 256) * 256);
+        tauv = part + (size_t)N * EIG_WAVES;
+    } else {
+        tauv = reinterpret_cast<T *>(sp);
+        sp += sizeof(T) * N;
+    }
     T *vv = reinterpret_cast<T *>(sp); sp += sizeof(T) * N;
     T *ww = reinterpret_cast<T *>(sp); sp += sizeof(T) * N;
-    T *part = reinterpret_cast<T *>(sp); sp += sizeof(T) * N * EIG_WAVES;
+    if (!PG) {
+        part = reinterpret_cast<T *>(sp);
+        sp += sizeof(T) * N * EIG_WAVES;
+    }
     double *red = reinterpret_cast<double *>(sp); sp += sizeof(double) * 4 * EIG_WAVES;
     __shared__ double sh_scal[8];
     __shared__ int sh_int[4];
@@ -1408,8 +1422,8 @@ size_t lds_bytes(int N, size_t elem, bool lds_matrix) {
     return b;
 }
 
-template <class T, bool LDS> int launch_tridiag(pmx_ctx *c, EigArgs &a, int grid, size_t lds) {
-    auto kern = tridiag_kernel<T, LDS>;
+template <class T, bool LDS, bool PG = false> int launch_tridiag(pmx_ctx *c, EigArgs &a, int grid, size_t lds) {
+    auto kern = tridiag_kernel<T, LDS, PG>;
     if (lds > 65536)
         PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(EIG_THREADS), lds, c->stream, a);
@@ -1592,14 +1606,19 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
     const size_t elem = real ? sizeof(double) : sizeof(zd);
     const size_t P = (size_t)N * (N + 1) / 2;
     const bool lds_matrix = lds_bytes(N, elem, true) <= c->lds_max;
-    const size_t lds = lds_bytes(N, elem, lds_matrix);
+    size_t lds = lds_bytes(N, elem, lds_matrix);
+    /* large bases: the three-stream kernel with its partials and tau in the slab */
+    const bool pg = !lds_matrix && lds > c->lds_max;
+    if (pg) lds = sizeof(double) * 2 * (size_t)N + elem * 2 * (size_t)N + sizeof(double) * 4 * EIG_WAVES;
     if (lds > c->lds_max) return set_error(PMX_EINVAL, "basis too large for the eigensolver's LDS work arrays");
     /* chunk the batch so that reflectors + SoA scratch stay near 12 GB (of 288) */
     const size_t per_mat = (nvec > 0 ? P * elem + N * elem : 0) + sizeof(double) * (size_t)N * (2 + nb) +
                            sizeof(double) * (size_t)nvec * N * 4 + 64;
     int chunk = (int)std::max<size_t>(64, std::min<size_t>((size_t)nmat, (size_t)12e9 / per_mat));
-    const size_t slab = lds_matrix ? 0 : ((P * elem + 255) / 256) * 256;
-    const int grid_max = c->num_cu * (lds_matrix ? 1 : 2) * 2;
+    const size_t slab = lds_matrix ? 0 : ((P * elem + 255) / 256) * 256 + (pg ? elem * (size_t)N * (EIG_WAVES + 1) : 0);
+    /* the slabs of all resident workgroups stay within about 8 GB */
+    int grid_max = c->num_cu * (lds_matrix ? 1 : 2) * 2;
+    if (slab > 0) grid_max = (int)std::max<size_t>(64, std::min<size_t>(grid_max, (size_t)8e9 / slab));
     int rc;
     const size_t cs = (size_t)chunk;
     const size_t need = cs * N * 2 + cs * 4 + cs * nb + (nvec > 0 ? cs * nvec * N * 4 : 0);
@@ -1673,7 +1692,10 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
                                         PMX_CHECK_HIP(hipGetLastError());
                                         return 0;
                                     }});
-            if (!rc_cands.empty())
+            if (pg)
+                rc = real ? launch_tridiag<double, false, true>(c, a, grid, lds)
+                          : launch_tridiag<zd, false, true>(c, a, grid, lds);
+            else if (!rc_cands.empty())
                 /* one matrix per workgroup: 16 x num_cu matrices are several rounds of
                  * full residency, so occupancy differences show in the sample */
                 rc = tune_run(c, "tridiag real tiled N=" + std::to_string(N), a.nmat, 16 * c->num_cu, rc_cands);

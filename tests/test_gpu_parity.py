@@ -108,6 +108,10 @@ LARGE = {  # bases that keep the packed matrix in global memory (the C3 / C5 siz
 }
 
 
+LARGE["te_soc_1370"] = LARGE["te_soc_582"][:LARGE["te_soc_582"].index("-gcut")] + ["-gcut", "5.2"] + \
+    LARGE["te_soc_582"][LARGE["te_soc_582"].index("-gcut") + 2:]  # 685 PW: partials and tau in global memory
+
+
 @pytest.mark.parametrize("case", sorted(LARGE))
 def test_eigenpairs_large_basis(ctx, case):
     args = LARGE[case]
