@@ -155,6 +155,16 @@ int pmx_chi_xyz_device(pmx_ctx *ctx, const pmx_chi_params *p, double *dReX, doub
 int pmx_profile(pmx_ctx *ctx, int enable);
 int pmx_profile_report(pmx_ctx *ctx, char *buf, int len);
 
+/* Eigensolve batches as HIP graphs (no reference counterpart).  With graphs
+ * on (the default), the first eigensolve batch of a shape runs eagerly, the
+ * second identical one (the same sizes, system and buffer addresses) is
+ * captured into a graph, and later ones replay it: the k+q solves of every q
+ * after the second in a chi q list, and repeated pmx_eig_batch calls.
+ * Profiling (pmx_profile) and PMX_GRAPHS=0 keep every batch eager.
+ * enable: 1 on, 0 off (drops the graphs held), -1 leaves the setting.
+ * stats (may be NULL): {graphs held, captures, replays} so far. */
+int pmx_graphs(pmx_ctx *ctx, int enable, long *stats);
+
 /* The autotuner's decisions so far on this context, as a JSON list: per
  * kernel shape the candidates' times on the sample it ran and the variant it
  * keeps (no reference counterpart). */

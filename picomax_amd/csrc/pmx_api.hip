@@ -275,6 +275,7 @@ void pmx_destroy(pmx_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     pmx::chi_work_free(c->chi);
     c->chi = nullptr;
+    c->graphs_clear();
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -373,6 +374,24 @@ int pmx_set_system(pmx_ctx *c, const pmx_system *s) {
     PMX_CHECK_HIP(hipMemcpyAsync(c->loci_h.data(), c->dLoci.p, sizeof(int) * s->neps * n, hipMemcpyDeviceToHost, c->stream));
     PMX_CHECK_HIP(hipStreamSynchronize(c->stream));
     c->has_system = true;
+    return PMX_OK;
+}
+
+int pmx_graphs(pmx_ctx *c, int enable, long *stats) {
+    if (!c) return set_error(PMX_EINVAL, "null context");
+    PMX_CHECK_HIP(hipSetDevice(c->device));
+    if (enable >= 0) {
+        c->graphs = enable != 0;
+        if (!c->graphs) {
+            PMX_CHECK_HIP(hipStreamSynchronize(c->stream));
+            c->graphs_clear();
+        }
+    }
+    if (stats) {
+        stats[0] = (long)c->eig_graphs.size();
+        stats[1] = c->graph_captures;
+        stats[2] = c->graph_replays;
+    }
     return PMX_OK;
 }
 

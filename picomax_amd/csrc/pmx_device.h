@@ -120,6 +120,19 @@ struct pmx_ctx {
     pmx::DBuf<double> dK, dEval, dScratch, dEigWork, dEigRef;
     pmx::DBuf<pmx::zd> dEvec, dZscr;
     pmx::DBuf<int> dStatus, dVrange;
+
+    /* eigensolve graphs: the first call of a shape (every kernel argument and
+     * buffer address) runs eagerly and settles the autotuner's choices, the
+     * second is captured into a HIP graph, later ones replay it */
+    bool graphs = true, capturing = false;
+    std::map<std::string, hipGraphExec_t> eig_graphs;
+    std::map<std::string, int> eig_seen;
+    long graph_captures = 0, graph_replays = 0;
+    void graphs_clear() {
+        for (auto &kv : eig_graphs) (void)hipGraphExecDestroy(kv.second);
+        eig_graphs.clear();
+        eig_seen.clear();
+    }
 };
 
 namespace pmx {

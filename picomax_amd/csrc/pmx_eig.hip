@@ -1475,7 +1475,8 @@ int tune_run(pmx_ctx *c, const std::string &key, int nmat, int sample, const std
     auto it = c->tune_choice.find(key);
     if (it != c->tune_choice.end()) return cands[it->second].run(nmat);
     const char *env = std::getenv("PMX_AUTOTUNE");
-    if ((env && env[0] == '0') || nmat < 2 * c->num_cu) return cands[0].run(nmat);
+    /* no timing under graph capture (its synchronisation is not capturable) */
+    if ((env && env[0] == '0') || nmat < 2 * c->num_cu || c->capturing) return cands[0].run(nmat);
     const int ns = std::min(nmat, sample);
     const int nc = (int)cands.size();
     /* events are released on every exit path */
@@ -1594,8 +1595,8 @@ template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, c
 
 } // namespace
 
-int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, double *dEvals, zd *dEvecs, int vsel,
-               double voff, int nsplit, int vsel2) {
+static int eig_device_run(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, double *dEvals, zd *dEvecs,
+                          int vsel, double voff, int nsplit, int vsel2) {
     if (nmat <= 0) return 0;
     const int N = basis_size(c);
     if (nb > N) nb = N;
@@ -1737,6 +1738,84 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
             }
         }
     }
+    return 0;
+}
+
+/* Everything an eigensolve batch bakes into its launches: the call's
+ * arguments, the system's by-value scalars, and the addresses and sizes of
+ * every buffer the kernels touch.  Two calls with equal keys launch the same
+ * kernels with the same arguments, so a graph captured from one replays the
+ * other (the buffers' contents are read at replay time). */
+struct EigGraphKey {
+    int nmat, nb, nvec, vsel, nsplit, vsel2, npw, soc, real_gauge, num_cu;
+    double voff, kin;
+    const void *ptr[12];
+    size_t len[4];
+};
+
+int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, double *dEvals, zd *dEvecs, int vsel,
+               double voff, int nsplit, int vsel2) {
+    if (nmat <= 0) return 0;
+    const char *env = std::getenv("PMX_GRAPHS");
+    if (!c->graphs || c->prof || c->capturing || (env && env[0] == '0'))
+        return eig_device_run(c, nmat, dKpts, nb, nvec, dEvals, dEvecs, vsel, voff, nsplit, vsel2);
+    EigGraphKey k;
+    std::memset(&k, 0, sizeof(k));
+    k.nmat = nmat; k.nb = nb; k.nvec = nvec; k.vsel = vsel; k.nsplit = nsplit; k.vsel2 = vsel2;
+    k.npw = c->npw; k.soc = c->soc; k.real_gauge = c->real_gauge; k.num_cu = c->num_cu;
+    k.voff = voff; k.kin = c->kin;
+    const void *ptrs[12] = {dKpts, dEvals, dEvecs, c->dG.p, c->dVpair.p, c->dVreal.p, c->dVso.p, c->dPhase.p,
+                            c->dEigWork.p, c->dEigRef.p, c->dScratch.p, c->dVrange.p};
+    std::memcpy(k.ptr, ptrs, sizeof(ptrs));
+    k.len[0] = c->dEigWork.n; k.len[1] = c->dEigRef.n; k.len[2] = c->dScratch.n; k.len[3] = c->dVrange.n;
+    const std::string key(reinterpret_cast<const char *>(&k), sizeof(k));
+
+    auto g = c->eig_graphs.find(key);
+    if (g != c->eig_graphs.end()) {
+        PMX_CHECK_HIP(hipGraphLaunch(g->second, c->stream));
+        c->graph_replays++;
+        return 0;
+    }
+    auto seen = c->eig_seen.find(key);
+    if (seen == c->eig_seen.end() || seen->second < 0) {
+        /* first call of this shape (or one whose capture failed): eager */
+        const int rc = eig_device_run(c, nmat, dKpts, nb, nvec, dEvals, dEvecs, vsel, voff, nsplit, vsel2);
+        if (!rc && seen == c->eig_seen.end()) {
+            if (c->eig_seen.size() >= 64) c->eig_seen.clear();
+            c->eig_seen[key] = 1;
+        }
+        return rc;
+    }
+    /* second call: capture.  A shape's buffers were sized by its first call,
+     * so nothing is allocated while capturing. */
+    PMX_CHECK_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    c->capturing = true;
+    const int rc = eig_device_run(c, nmat, dKpts, nb, nvec, dEvals, dEvecs, vsel, voff, nsplit, vsel2);
+    c->capturing = false;
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+    hipGraphExec_t exec = nullptr;
+    /* a buffer that grew while capturing leaves the graph on other addresses than the key's */
+    const bool same_bufs = c->dEigWork.p == ptrs[8] && c->dEigRef.p == ptrs[9] && c->dScratch.p == ptrs[10] &&
+                           c->dVrange.p == ptrs[11] && c->dEigWork.n == k.len[0] && c->dEigRef.n == k.len[1] &&
+                           c->dScratch.n == k.len[2] && c->dVrange.n == k.len[3];
+    const bool ok = !rc && ec == hipSuccess && graph && same_bufs &&
+                    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    if (!ok) {
+        /* the capture did not hold: this shape stays eager */
+        (void)hipGetLastError();
+        seen->second = -1;
+        return eig_device_run(c, nmat, dKpts, nb, nvec, dEvals, dEvecs, vsel, voff, nsplit, vsel2);
+    }
+    if (c->eig_graphs.size() >= 16) {
+        for (auto &kv : c->eig_graphs) (void)hipGraphExecDestroy(kv.second);
+        c->eig_graphs.clear();
+    }
+    c->eig_graphs[key] = exec;
+    c->graph_captures++;
+    PMX_CHECK_HIP(hipGraphLaunch(exec, c->stream));
     return 0;
 }
 

@@ -1,0 +1,102 @@
+"""Eigensolve batches replayed from HIP graphs give the results of eager runs.
+
+A shape's first batch runs eagerly, the second is captured, later ones replay
+the graph; the buffers' contents (the k list, the form factors) are read at
+replay time, so a replay with new k-points or new form factors must equal an
+eager solve of them.  Eigenvalues and eigenvectors are compared bit for bit
+(the same kernels on the same inputs); chi over a q list, whose k+q solves
+replay from the third q on, is compared with an eager run to 1e-12 of its
+largest value (the moment accumulation's summation order may vary).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from picomax_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+SI = ["-crystal", "zb", "-a", "5.43", "-vg", "3,4,8,11,-0.21,0,0.04,0.08,-0.21,0,0.04,0.08",
+      "-gcut", "4.898979485566356"]  # 137 PW, real symmetric H
+TE = ["-crystal", "te", "-a", "4.45", "-f", "1.33", "-u", "0.26", "-vg",
+      "0.565323,1.333333,1.898656,2.261292,3.594626,4.0" + ",-0.2105,-0.1322,0.0451,0.0214,0.0127,-0.0093" * 3,
+      "-gcut", "2.6", "-us_so", "0.565323:0.0152,1.333333:-0.0061,1.898656:0.0083",
+      "-ua_so", "0.565323:0.0117,1.333333:0.0042,1.898656:-0.0071"]  # spinor H
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _ctx(args, graphs):
+    c = capi.Context(0)
+    inp = capi.Input(args)
+    c.set_system(inp.system(), inp)
+    c.graphs(1 if graphs else 0)
+    return c
+
+
+@pytest.mark.parametrize("case,nk", [("si", 600), ("te", 96)])
+def test_eig_batch_replay_equals_eager(case, nk):
+    args = SI if case == "si" else TE
+    _say("graph replay eig %s nk=%d" % (case, nk))
+    rng = np.random.default_rng(21)
+    K1 = rng.uniform(-0.5, 0.5, (nk, 3))
+    K2 = rng.uniform(-0.5, 0.5, (nk, 3))
+    nb = 24
+    g = _ctx(args, True)
+    e = _ctx(args, False)
+    try:
+        a1 = g.eig(K1, nb, vectors=True)   # eager, first sighting
+        b1 = g.eig(K1, nb, vectors=True)   # eager, sizes settled
+        c2 = g.eig(K2, nb, vectors=True)   # captured
+        d1 = g.eig(K1, nb, vectors=True)   # replayed
+        st = g.graphs()
+        assert st["captures"] == 1 and st["replays"] >= 1, st
+        ref1 = e.eig(K1, nb, vectors=True)
+        ref2 = e.eig(K2, nb, vectors=True)
+        assert e.graphs()["captures"] == 0
+        for got in (a1, b1, d1):
+            assert np.array_equal(got[0], ref1[0]) and np.array_equal(got[1], ref1[1])
+        assert np.array_equal(c2[0], ref2[0]) and np.array_equal(c2[1], ref2[1])
+        # new form factors in the same buffers: the replay reads them
+        inp = capi.Input(args)
+        sy = inp.system()
+        new = np.ctypeslib.as_array(sy.vg, (sy.natom, sy.nv)).copy() * 1.05
+        g.set_form_factors(new)
+        e.set_form_factors(new)
+        r_g = g.eig(K2, nb, vectors=True)
+        r_e = e.eig(K2, nb, vectors=True)
+        assert np.array_equal(r_g[0], r_e[0]) and np.array_equal(r_g[1], r_e[1])
+        assert not np.array_equal(r_g[0], ref2[0])
+    finally:
+        g.close()
+        e.close()
+
+
+def test_chi_q_list_replay_equals_eager():
+    """Six q along [100]: the k+q batches of q 3..6 replay one graph."""
+    _say("graph replay chi q list")
+    from picomax_amd import kpoints
+    K = kpoints.gamma_grid("zb", 6)
+    KW = np.full(len(K), 1.0 / len(K))
+    Q = [[0.01 * (i + 1), 0.0, 0.0] for i in range(6)]
+    args = SI[:-2] + ["-gcut", "3.4641016151377544", "-neps", "9"]
+    out = []
+    for graphs in (True, False):
+        c = _ctx(args, graphs)
+        try:
+            off = c.ref_energy((0.0, 0.0, 0.0), 8)
+            out.append(c.chi(K, KW, Q, 16, off, 1, 0.05, 200, 0.05, 1, "fp64"))
+            st = c.graphs()
+            if graphs:
+                assert st["captures"] >= 1 and st["replays"] >= 3, st
+            else:
+                assert st["captures"] == 0
+        finally:
+            c.close()
+    scale = max(np.abs(out[1][0]).max(), np.abs(out[1][1]).max())
+    assert np.abs(out[0][0] - out[1][0]).max() <= 1e-12 * scale
+    assert np.abs(out[0][1] - out[1][1]).max() <= 1e-12 * scale
