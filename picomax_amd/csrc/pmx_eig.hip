@@ -1249,9 +1249,10 @@ __device__ __forceinline__ dx4 mma64(double a, double b, dx4 c) {
 }
 
 template <int VL, int WPE>
-__global__ void __launch_bounds__(256, WPE) backtrans_wy_kernel(int N, int nmat, int nvec, int NV, const double *Y,
-                                                            const double *vref_all, const double *tau_all,
-                                                            const double *phase, const int *vr, zd *evecs) {
+__global__ void __launch_bounds__(256, WPE) backtrans_wy_kernel(int N, int nmat, int nvec, int NV, int clo, int chi,
+                                                            const double *Y, const double *vref_all,
+                                                            const double *tau_all, const double *phase, const int *vr,
+                                                            zd *evecs) {
     extern __shared__ double smem_d[];
     const int Np = (N + 15) & ~15, ZS = NV + 1;
     double *Zs = smem_d;                  /* [Np][ZS] */
@@ -1265,8 +1266,10 @@ __global__ void __launch_bounds__(256, WPE) backtrans_wy_kernel(int N, int nmat,
     const size_t P = (size_t)N * (N + 1) / 2;
     const double *vref = vref_all + (size_t)mat * P;
     const double *tau = tau_all + (size_t)mat * N;
-    /* the wanted vectors j0 .. j0 + cnt - 1 become columns 0 .. cnt - 1 of Z */
+    /* the wanted vectors j0 .. j0 + cnt - 1 become columns 0 .. cnt - 1 of Z;
+     * this launch takes the matrices with clo < cnt <= chi (cnt <= NV) */
     const int j0 = vr ? vr[2 * mat] : 0, cnt = vr ? vr[2 * mat + 1] : nvec;
+    if (cnt <= clo || cnt > chi) return;
     /* Y -> Z in chunks of 16 loads per thread, all in flight before the stores */
     for (int base = 0; base < Np * NV; base += 16 * 256) {
         double y[16];
@@ -1553,19 +1556,31 @@ bool wy_usable(pmx_ctx *c, int N, int nvec) { return N >= 2 && N <= 16 * BT_VL_M
 template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, const double *Y, const void *vref,
                                         const void *tau, const double *phase, const int *vr, zd *evecs) {
     if (sizeof(T) == sizeof(double) && wy_usable(c, N, nvec)) {
-        const size_t lds = wy_lds_bytes(N, nvec);
         /* two register budgets: 16 reflector values per thread (any Np <= 256,
          * three waves per SIMD) or 10 (Np <= 160, capped at 128 VGPRs for four
          * waves per SIMD); the autotuner picks per shape.  Each run rewrites
-         * the same vectors from the same inputs, so sample runs are harmless. */
+         * the same vectors from the same inputs, so sample runs are harmless.
+         * With a selection of more than 16 vectors, the matrices that want at
+         * most 16 (the usual case: the conduction or the valence states of
+         * one solve) run in a launch whose Z holds 16 columns, which leaves
+         * LDS for more workgroups per CU; the rest run in a second launch. */
+        const int NVf = (nvec + 15) & ~15;
         auto launch = [&](auto kern, int g) -> int {
-            if (lds > 65536)
-                PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL(kern, dim3(g), dim3(256), lds, c->stream, N, nmat, nvec, (nvec + 15) & ~15, Y,
-                               reinterpret_cast<const double *>(vref), reinterpret_cast<const double *>(tau), phase,
-                               vr, evecs);
-            PMX_CHECK_HIP(hipGetLastError());
-            return 0;
+            auto pass = [&](int NV, int clo, int chi) -> int {
+                const size_t lds = wy_lds_bytes(N, NV);
+                if (lds > 65536)
+                    PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                hipLaunchKernelGGL(kern, dim3(g), dim3(256), lds, c->stream, N, nmat, nvec, NV, clo, chi, Y,
+                                   reinterpret_cast<const double *>(vref), reinterpret_cast<const double *>(tau),
+                                   phase, vr, evecs);
+                PMX_CHECK_HIP(hipGetLastError());
+                return 0;
+            };
+            if (vr && NVf > 16) {
+                const int r = pass(16, -1, 16);
+                return r ? r : pass(NVf, 16, nvec);
+            }
+            return pass(NVf, -1, nvec);
         };
         const int Np = (N + 15) & ~15;
         std::vector<TuneCand> cands;
