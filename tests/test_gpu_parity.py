@@ -249,3 +249,38 @@ def test_tridiag_forms_large_basis(form, monkeypatch):
             assert np.abs(H @ V - V * ev[k]).max() < 1e-9
     finally:
         c.close()
+
+
+SI_REAL = {  # real symmetric spatial H (inversion centre): the register-tiled kernels
+    "si_59": ["-crystal", "zb", "-a", "5.43", "-vg", "3,4,8,11,-0.21,0,0.04,0.08,-0.21,0,0.04,0.08",
+              "-gcut", "3.4641016151377544"],
+    "si_137": ["-crystal", "zb", "-a", "5.43", "-vg", "3,4,8,11,-0.21,0,0.04,0.08,-0.21,0,0.04,0.08",
+               "-gcut", "4.898979485566356"],
+}
+
+
+@pytest.mark.parametrize("form", ["tile7", "tile8", "full_lds"])
+@pytest.mark.parametrize("case", sorted(SI_REAL))
+def test_tridiag_real_forms_against_oracle(case, form, monkeypatch):
+    """Every real tridiagonalisation candidate the autotuner may pick (7x7 or
+    8x8 register tiles, full LDS) gives the oracle's eigenpairs."""
+    monkeypatch.setenv("PMX_FORCE_KERNEL", form)
+    args = SI_REAL[case]
+    _say("tridiag real form %s %s" % (form, case))
+    c = capi.Context(0)
+    try:
+        inp = capi.Input(args)
+        c.set_system(inp.system(), inp)
+        s, p = cli_args.build(["-switch", "band"] + args)
+        K = np.random.default_rng(5).uniform(-0.6, 0.6, (6, 3))
+        K[0] = 0.0  # Gamma: degenerate levels
+        nb = 36
+        ev, vec = c.eig(K, nb, vectors=True)
+        for k in range(len(K)):
+            H = s.hamiltonian(K[k])
+            assert np.abs(ev[k] - np.linalg.eigvalsh(H)[:nb]).max() < 1e-10
+            V = vec[k].T
+            assert np.abs(H @ V - V * ev[k]).max() < 1e-9
+            assert np.abs(V.conj().T @ V - np.eye(nb)).max() < 1e-10
+    finally:
+        c.close()
