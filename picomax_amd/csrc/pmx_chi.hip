@@ -115,6 +115,8 @@ struct RhoArgs {
     int ll;                   /* 1: chi_LL's vertex (density for i = 0); 2: chi_xyz's Cartesian one;
                                  either way one channel with SOC (same-spin products) */
     int swap;                 /* chi_xyz: bra rows = valence at k, ket rows = conduction at k+q */
+    const double *G;          /* [npw][3] (rho_cart_kernel) */
+    double Qx, Qy, Qz;        /* q of this step (rho_cart_kernel) */
 };
 
 /* solve-basis rows of the bra (index c) and ket (index v) states of a transition */
@@ -425,6 +427,108 @@ template <class T> __global__ void __launch_bounds__(256) rho_mfma_kernel(RhoArg
             }
         }
         __builtin_amdgcn_wave_barrier();
+    }
+}
+
+/* ------------------------------------------------------------------
+ * rho for the spatial basis on MFMA, Cartesian form.  The weight of channel
+ * i, w_{m,i}(p) = conj(u_mi).(G_p + Q/2 + G_m/2), is linear in G_p, so
+ *   rho_i = sum_d conj(u_mi,d) T_{m,d} + conj(u_mi).(K + Q/2 + G_m/2) S_m,
+ *   S_m = sum_p conj(Ck[c][p]) Ckq[v][loci(m,p)],  T_{m,d}: the same with G_p,d
+ * (src/pmx_chi.cpp:534-544 regrouped).  G_p,d is real and independent of m,
+ * so it goes on the A side: four row channels (S, Tx, Ty, Tz) from one LDS
+ * load of the conduction row at k.  The B side is the plain loci gather of
+ * the valence rows at k+q; its 16 columns are 16 (m, v) pairs.  One wave per
+ * (16 conduction rows, 16 (m, v) columns) runs 8 MFMA chains (4 channels,
+ * re / im) and combines the channels in registers: the K loop has no weight
+ * loads and no complex multiplies.
+ * ------------------------------------------------------------------ */
+template <class T> __global__ void __launch_bounds__(256) rho_cart_kernel(RhoArgs a) {
+    typedef typename RhoT<T>::V V;
+    typedef typename Cplx<T>::type CT;
+    const int k = blockIdx.x;
+    const int ncs = a.nc[k] / 2, nvs = a.nv[k] / 2;
+    if (ncs * nvs == 0) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    extern __shared__ double rho_dyn[];
+    CT *sck = reinterpret_cast<CT *>(rho_dyn); /* [ncs][N]: conduction rows at k, c = 0 is the highest */
+    CT *scq = sck + (size_t)ncs * a.N;          /* [nvs][N]: valence rows at k+q */
+    T *sg = reinterpret_cast<T *>(scq + (size_t)nvs * a.N); /* [npw][3]: G */
+    const size_t tb = a.tbase[k];
+    const double Kx = a.K[3 * k], Ky = a.K[3 * k + 1], Kz = a.K[3 * k + 2];
+    const zd *vk = a.vk + (size_t)k * a.nb_s * a.N;
+    const zd *vkq = a.vkq + (size_t)k * a.nb_s * a.N;
+    for (int idx = threadIdx.x; idx < ncs * a.N; idx += 256) {
+        const zd v = vk[(size_t)bra_row(a, idx / a.N) * a.N + idx % a.N];
+        sck[idx] = CT{(T)v.x, (T)v.y};
+    }
+    for (int idx = threadIdx.x; idx < nvs * a.N; idx += 256) {
+        const zd v = vkq[(size_t)ket_row(a, idx / a.N) * a.N + idx % a.N];
+        scq[idx] = CT{(T)v.x, (T)v.y};
+    }
+    for (int idx = threadIdx.x; idx < 3 * a.npw; idx += 256) sg[idx] = (T)a.G[idx];
+    __syncthreads();
+    const int ncol = a.neps * nvs;            /* (m, v) columns, v fastest */
+    const int nct = (ncs + 15) / 16, njt = (ncol + 15) / 16;
+    const int K2 = 2 * a.npw, kq = lane >> 4;
+    for (int job = wave; job < nct * njt; job += 4) {
+        const int ct = job / njt, jt = job % njt;
+        const int crow = ct * 16 + (lane & 15); /* A row */
+        const int col = jt * 16 + (lane & 15);  /* B column */
+        const int m = col / nvs, vv = col % nvs;
+        const bool crow_ok = crow < ncs, col_ok = m < a.neps;
+        const CT *ck = sck + (size_t)(crow_ok ? crow : 0) * a.N;
+        const CT *cq = scq + (size_t)(col_ok ? vv : 0) * a.N;
+        const int *lm = a.loci + (size_t)(col_ok ? m : 0) * a.npw;
+        V sr = {0, 0, 0, 0}, si = {0, 0, 0, 0}, xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
+        V yr = {0, 0, 0, 0}, yi = {0, 0, 0, 0}, zr = {0, 0, 0, 0}, zi = {0, 0, 0, 0};
+        int l_n = (col_ok && kq < K2) ? lm[kq >> 1] : -1; /* loci one K step ahead */
+        for (int k0 = 0; k0 < K2; k0 += 4) {
+            const int kk = k0 + kq;
+            const int l = l_n;
+            if (col_ok && kk + 4 < K2) l_n = lm[(kk + 4) >> 1];
+            T av = 0, gx = 0, gy = 0, gz = 0, bre = 0, bim = 0;
+            if (kk < K2) {
+                const int p = kk >> 1, part = kk & 1;
+                if (crow_ok) av = part ? ck[p].y : ck[p].x;
+                gx = sg[3 * p];
+                gy = sg[3 * p + 1];
+                gz = sg[3 * p + 2];
+                if (l >= 0) {
+                    const CT q = cq[l];
+                    bre = part ? q.y : q.x;
+                    bim = part ? -q.x : q.y;
+                }
+            }
+            const T ax = av * gx, ay = av * gy, az = av * gz;
+            sr = RhoT<T>::mma(av, bre, sr);
+            si = RhoT<T>::mma(av, bim, si);
+            xr = RhoT<T>::mma(ax, bre, xr);
+            xi = RhoT<T>::mma(ax, bim, xi);
+            yr = RhoT<T>::mma(ay, bre, yr);
+            yi = RhoT<T>::mma(ay, bim, yi);
+            zr = RhoT<T>::mma(az, bre, zr);
+            zi = RhoT<T>::mma(az, bim, zi);
+        }
+        if (!col_ok) continue;
+        const double *Gm = a.G + 3 * m;
+        const double ox = Kx + 0.5 * (a.Qx + Gm[0]), oy = Ky + 0.5 * (a.Qy + Gm[1]), oz = Kz + 0.5 * (a.Qz + Gm[2]);
+        for (int r = 0; r < 4; r++) {
+            const int c = ct * 16 + RhoT<T>::drow(lane, r);
+            if (c >= ncs) continue;
+            const int row = a.pos[tb + (size_t)c * nvs + vv];
+            if (row < 0) continue;
+            const zd S{(double)sr[r], (double)si[r]};
+            const zd Tx{(double)xr[r], (double)xi[r]}, Ty{(double)yr[r], (double)yi[r]}, Tz{(double)zr[r], (double)zi[r]};
+            for (int i = 0; i < 3; i++) {
+                const zd *u = a.uc + ((size_t)m * 3 + i) * 3;
+                const zd t0 = cmul(u[0], Tx), t1 = cmul(u[1], Ty), t2 = cmul(u[2], Tz);
+                const zd bo{u[0].x * ox + u[1].x * oy + u[2].x * oz, u[0].y * ox + u[1].y * oy + u[2].y * oz};
+                const zd bs = cmul(bo, S);
+                put_item(a, row, 0, 3 * m + i,
+                         (a.ll == 1 && i == 0) ? S : zd{t0.x + t1.x + t2.x + bs.x, t0.y + t1.y + t2.y + bs.y});
+            }
+        }
     }
 }
 
@@ -1704,6 +1808,8 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         ra.Xf = fp32 ? w.Xf.p : nullptr;
         ra.ll = mode;
         ra.swap = xyz ? 1 : 0;
+        ra.G = c->dG.p;
+        ra.Qx = Qx; ra.Qy = Qy; ra.Qz = Qz;
         if (ntr > 0) {
             /* algorithmic work of the gather-GEMM (src/pmx_chi.cpp:534-544): 8 flops per complex MAC,
              * 3 weighted (+1 plain) sums per (m, p) and transition; bytes = the eigenvector rows read */
@@ -1718,7 +1824,14 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                 const void *kf = fp32 ? (const void *)rho_mfma_kernel<float> : (const void *)rho_mfma_kernel<double>;
                 PMX_CHECK_HIP(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_r));
             }
-            if (rho_lds && fp32)
+            /* the Cartesian form: its LDS is the rows plus the G table */
+            const size_t lds_c = lds_r + (fp32 ? sizeof(float) : sizeof(double)) * 3 * (size_t)npw;
+            if (rho_lds && fp32 && lds_c <= c->lds_max) {
+                if (lds_c > 65536)
+                    PMX_CHECK_HIP(hipFuncSetAttribute((const void *)rho_cart_kernel<float>,
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_c));
+                hipLaunchKernelGGL(rho_cart_kernel<float>, dim3(p->nk), dim3(256), lds_c, st, ra);
+            } else if (rho_lds && fp32)
                 hipLaunchKernelGGL(rho_mfma_kernel<float>, dim3(p->nk), dim3(256), lds_r, st, ra);
             else if (rho_lds)
                 hipLaunchKernelGGL(rho_mfma_kernel<double>, dim3(p->nk), dim3(256), lds_r, st, ra);
