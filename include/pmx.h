@@ -117,6 +117,18 @@ int pmx_bands(pmx_ctx *ctx, int nq, const double *Q, int nband, double offset, d
  * interleaved (2p = up, 2p+1 = down). */
 int pmx_eig_batch(pmx_ctx *ctx, int nk, const double *K, int nband, double *evals, double *evecs);
 
+/* pmx_eig_batch for several systems in one launch (no reference counterpart):
+ * batch i solves nk[i] k-points K[i] of context ctxs[i]'s system, SOC or not,
+ * any basis size, into evals[i] [nk[i]][nband[i]] and, when evecs and
+ * evecs[i] are not NULL, evecs[i] [nk[i]][nband[i]][2*npw_i] (the layouts of
+ * pmx_eig_batch).  The contexts must be distinct and on one device.  The
+ * batches run side by side, each on its own context's stream, forked from
+ * and joined to the first context's stream; with graphs on (pmx_graphs, kept
+ * by the first context) the second call with the same shapes is captured as
+ * one HIP graph with a branch per batch, and later calls replay it. */
+int pmx_eig_batch_multi(int nsys, pmx_ctx *const *ctxs, const int *nk, const double *const *K, const int *nband,
+                        double *const *evals, double *const *evecs);
+
 /* chi_tensor (src/pmx_chi.cpp:319-713) including the optional KK pass.
  * ReX and ImX each hold 9*nq*neps*(neps+1)/2*nfreq doubles in the order
  * [i][j][q][m][n<=m][f] of export_Xij (src/pmx_io.cpp:720-741). */

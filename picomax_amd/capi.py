@@ -20,7 +20,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 
 # every symbol include/pmx.h declares
 EXPORTS = ["pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_set_system",
-           "pmx_set_form_factors", "pmx_get_loci", "pmx_ref_energy", "pmx_bands", "pmx_eig_batch",
+           "pmx_set_form_factors", "pmx_get_loci", "pmx_ref_energy", "pmx_bands", "pmx_eig_batch", "pmx_eig_batch_multi",
            "pmx_chi", "pmx_chi_device", "pmx_chi_ll", "pmx_chi_ll_device", "pmx_chi_xyz", "pmx_chi_xyz_device", "pmx_profile", "pmx_profile_report", "pmx_tune_report", "pmx_graphs", "pmx_kk", "pmx_epsinv", "pmx_eps_macro", "pmx_polarization_bands", "pmx_grad_bands", "pmx_grad_chi", "pmx_input_parse", "pmx_input_free",
            "pmx_input_system", "pmx_input_info", "pmx_generate_gpt", "pmx_run"]
 
@@ -80,6 +80,9 @@ def lib():
     L.pmx_ref_energy.argtypes = [vp, _dp, ctypes.c_int, _dp]
     L.pmx_bands.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, _dp]
     L.pmx_eig_batch.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
+    L.pmx_eig_batch_multi.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(_dp), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_dp),
+                                      ctypes.POINTER(_dp)]
     L.pmx_chi.argtypes = [vp, ctypes.POINTER(PmxChiParams), _dp, _dp]
     L.pmx_chi_device.argtypes = [vp, ctypes.POINTER(PmxChiParams), ctypes.c_void_p, ctypes.c_void_p]
     L.pmx_chi_ll.argtypes = [vp, ctypes.POINTER(PmxChiParams), _dp, _dp]
@@ -354,3 +357,20 @@ def generate_gpt(recip, dim, gcut):
     if n < 0:
         _check(n)
     return out[:n].copy()
+
+
+def eig_multi(ctxs, Ks, nbands, vectors=False):
+    """pmx_eig_batch_multi: one eigensolve batch per context (its own system),
+    run side by side in one launch.  Returns [(evals, evecs or None), ...]."""
+    n = len(ctxs)
+    Ks = [_f64(K).reshape(-1, 3) for K in Ks]
+    evs = [np.zeros((len(K), nb)) for K, nb in zip(Ks, nbands)]
+    vecs = [np.zeros((len(K), nb, 2 * c.npw), complex) if vectors else None for K, nb, c in zip(Ks, nbands, ctxs)]
+    hs = (ctypes.c_void_p * n)(*[c.h for c in ctxs])
+    nk = (ctypes.c_int * n)(*[len(K) for K in Ks])
+    nb = (ctypes.c_int * n)(*[int(b) for b in nbands])
+    kp = (_dp * n)(*[_p(K) for K in Ks])
+    ep = (_dp * n)(*[_p(e) for e in evs])
+    vp_ = (_dp * n)(*[_p(v.view(np.float64)) for v in vecs]) if vectors else None
+    _check(lib().pmx_eig_batch_multi(n, hs, nk, kp, nb, ep, vp_))
+    return list(zip(evs, vecs))

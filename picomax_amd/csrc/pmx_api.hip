@@ -510,6 +510,86 @@ int pmx_bands(pmx_ctx *c, int nq, const double *Q, int nband, double offset, dou
     return PMX_OK;
 }
 
+/* eigenpairs of the solve basis -> the C-ABI's spinor layout */
+static void unpack_eig(const pmx_ctx *c, int nk, int nband, int nb_solve, const std::vector<double> &ev,
+                       const std::vector<zd> &vec, double *evals, double *evecs) {
+    const int N = basis_size(c), n2 = 2 * c->npw;
+    for (int k = 0; k < nk; k++)
+        for (int b = 0; b < nband; b++) {
+            evals[(size_t)k * nband + b] = c->soc ? ev[(size_t)k * nb_solve + b] : ev[(size_t)k * nb_solve + b / 2];
+            if (!evecs) continue;
+            double *out = evecs + 2 * ((size_t)k * nband + b) * n2;
+            if (c->soc) {
+                std::memcpy(out, &vec[((size_t)k * nb_solve + b) * N], sizeof(zd) * N);
+            } else {
+                /* spinor partner states of H0 (x) I2: (psi, 0) and (0, psi) */
+                const zd *src = &vec[((size_t)k * nb_solve + b / 2) * N];
+                const int s = b & 1;
+                for (int p = 0; p < c->npw; p++) {
+                    out[2 * (2 * p + s)] = src[p].x;
+                    out[2 * (2 * p + s) + 1] = src[p].y;
+                    out[2 * (2 * p + 1 - s)] = 0.0;
+                    out[2 * (2 * p + 1 - s) + 1] = 0.0;
+                }
+            }
+        }
+}
+
+int pmx_eig_batch_multi(int nsys, pmx_ctx *const *ctxs, const int *nk, const double *const *K, const int *nband,
+                        double *const *evals, double *const *evecs) {
+    if (nsys <= 0 || !ctxs || !nk || !K || !nband || !evals) return set_error(PMX_EINVAL, "null pointer or no system");
+    for (int i = 0; i < nsys; i++) {
+        pmx_ctx *c = ctxs[i];
+        if (!c || (nk[i] > 0 && (!K[i] || !evals[i]))) return set_error(PMX_EINVAL, "null pointer");
+        if (!c->has_system) return set_error(PMX_ESTATE, "no system set");
+        if (c->device != ctxs[0]->device) return set_error(PMX_EINVAL, "contexts on different devices");
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == c) return set_error(PMX_EINVAL, "a context appears twice");
+        if (nk[i] < 0) return set_error(PMX_EINVAL, "negative k count");
+        if (nband[i] <= 0 || nband[i] > 2 * c->npw) return set_error(PMX_EINVAL, "nband must be in [1, 2 npw]");
+    }
+    PMX_CHECK_HIP(hipSetDevice(ctxs[0]->device));
+    std::vector<int> nm(nsys), nbs(nsys), nvs(nsys);
+    std::vector<const double *> dK(nsys);
+    std::vector<double *> dE(nsys);
+    std::vector<zd *> dV(nsys);
+    for (int i = 0; i < nsys; i++) {
+        pmx_ctx *c = ctxs[i];
+        const int N = basis_size(c);
+        const bool vecs = evecs && evecs[i];
+        nm[i] = nk[i];
+        nbs[i] = c->soc ? nband[i] : (nband[i] + 1) / 2;
+        nvs[i] = vecs ? nbs[i] : 0;
+        int rc;
+        if ((rc = c->dK.ensure((size_t)3 * std::max(nk[i], 1)))) return rc;
+        if ((rc = c->dEval.ensure((size_t)std::max(nk[i], 1) * nbs[i]))) return rc;
+        if (vecs && (rc = c->dEvec.ensure((size_t)std::max(nk[i], 1) * nbs[i] * N))) return rc;
+        if (nk[i] > 0)
+            PMX_CHECK_HIP(hipMemcpyAsync(c->dK.p, K[i], sizeof(double) * 3 * nk[i], hipMemcpyHostToDevice, c->stream));
+        /* the batches start from the first context's stream: uploads done first */
+        PMX_CHECK_HIP(hipStreamSynchronize(c->stream));
+        dK[i] = c->dK.p;
+        dE[i] = c->dEval.p;
+        dV[i] = vecs ? c->dEvec.p : nullptr;
+    }
+    int rc = eig_device_multi(nsys, ctxs, nm.data(), dK.data(), nbs.data(), nvs.data(), dE.data(), dV.data());
+    if (rc) return rc;
+    PMX_CHECK_HIP(hipStreamSynchronize(ctxs[0]->stream));
+    for (int i = 0; i < nsys; i++) {
+        pmx_ctx *c = ctxs[i];
+        if (nk[i] == 0) continue;
+        std::vector<double> ev((size_t)nk[i] * nbs[i]);
+        std::vector<zd> vec((size_t)nk[i] * nvs[i] * basis_size(c));
+        PMX_CHECK_HIP(hipMemcpyAsync(ev.data(), c->dEval.p, sizeof(double) * ev.size(), hipMemcpyDeviceToHost, c->stream));
+        if (nvs[i] > 0)
+            PMX_CHECK_HIP(hipMemcpyAsync(vec.data(), c->dEvec.p, sizeof(zd) * vec.size(), hipMemcpyDeviceToHost,
+                                         c->stream));
+        PMX_CHECK_HIP(hipStreamSynchronize(c->stream));
+        unpack_eig(c, nk[i], nband[i], nbs[i], ev, vec, evals[i], nvs[i] > 0 ? evecs[i] : nullptr);
+    }
+    return PMX_OK;
+}
+
 int pmx_eig_batch(pmx_ctx *c, int nk, const double *K, int nband, double *evals, double *evecs) {
     if (!c || !K || !evals) return set_error(PMX_EINVAL, "null pointer");
     if (!c->has_system) return set_error(PMX_ESTATE, "no system set");
@@ -529,25 +609,7 @@ int pmx_eig_batch(pmx_ctx *c, int nk, const double *K, int nband, double *evals,
     PMX_CHECK_HIP(hipMemcpyAsync(ev.data(), c->dEval.p, sizeof(double) * ev.size(), hipMemcpyDeviceToHost, c->stream));
     PMX_CHECK_HIP(hipMemcpyAsync(vec.data(), c->dEvec.p, sizeof(zd) * vec.size(), hipMemcpyDeviceToHost, c->stream));
     PMX_CHECK_HIP(hipStreamSynchronize(c->stream));
-    const int n2 = 2 * c->npw;
-    for (int k = 0; k < nk; k++)
-        for (int b = 0; b < nband; b++) {
-            evals[(size_t)k * nband + b] = c->soc ? ev[(size_t)k * nb_solve + b] : ev[(size_t)k * nb_solve + b / 2];
-            double *out = evecs + 2 * ((size_t)k * nband + b) * n2;
-            if (c->soc) {
-                std::memcpy(out, &vec[((size_t)k * nb_solve + b) * N], sizeof(zd) * N);
-            } else {
-                /* spinor partner states of H0 (x) I2: (psi, 0) and (0, psi) */
-                const zd *src = &vec[((size_t)k * nb_solve + b / 2) * N];
-                const int s = b & 1;
-                for (int p = 0; p < c->npw; p++) {
-                    out[2 * (2 * p + s)] = src[p].x;
-                    out[2 * (2 * p + s) + 1] = src[p].y;
-                    out[2 * (2 * p + 1 - s)] = 0.0;
-                    out[2 * (2 * p + 1 - s) + 1] = 0.0;
-                }
-            }
-        }
+    unpack_eig(c, nk, nband, nb_solve, ev, vec, evals, evecs);
     return PMX_OK;
 }
 

@@ -126,11 +126,14 @@ struct pmx_ctx {
      * second is captured into a HIP graph, later ones replay it */
     bool graphs = true, capturing = false;
     std::map<std::string, hipGraphExec_t> eig_graphs;
+    std::map<std::string, hipGraphExec_t> multi_graphs; /* several contexts' batches, one branch each */
     std::map<std::string, int> eig_seen;
     long graph_captures = 0, graph_replays = 0;
     void graphs_clear() {
         for (auto &kv : eig_graphs) (void)hipGraphExecDestroy(kv.second);
+        for (auto &kv : multi_graphs) (void)hipGraphExecDestroy(kv.second);
         eig_graphs.clear();
+        multi_graphs.clear();
         eig_seen.clear();
     }
 };
@@ -152,6 +155,10 @@ struct Prof {
  * 2) instead: two batches with different selections in one launch. */
 int eig_device(pmx_ctx *ctx, int nmat, const double *dKpts, int nb, int nvec, double *dEvals, zd *dEvecs,
                int vsel = 0, double voff = 0.0, int nsplit = -1, int vsel2 = 0);
+/* n contexts' batches (all eigenvectors of the lowest nvec[i], no selection)
+ * side by side on their own streams, joined to the first context's stream */
+int eig_device_multi(int n, pmx_ctx *const *cs, const int *nmat, const double *const *dK, const int *nb,
+                     const int *nvec, double *const *dEvals, zd *const *dEvecs);
 /* basis size of the solve (NPW spatial or 2 NPW spinor) */
 inline int basis_size(const pmx_ctx *c) { return c->soc ? 2 * c->npw : c->npw; }
 int rebuild_vtable(pmx_ctx *ctx);

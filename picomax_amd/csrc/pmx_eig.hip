@@ -1768,12 +1768,8 @@ struct EigGraphKey {
     size_t len[4];
 };
 
-int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, double *dEvals, zd *dEvecs, int vsel,
-               double voff, int nsplit, int vsel2) {
-    if (nmat <= 0) return 0;
-    const char *env = std::getenv("PMX_GRAPHS");
-    if (!c->graphs || c->prof || c->capturing || (env && env[0] == '0'))
-        return eig_device_run(c, nmat, dKpts, nb, nvec, dEvals, dEvecs, vsel, voff, nsplit, vsel2);
+static EigGraphKey eig_key(const pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, const double *dEvals,
+                           const zd *dEvecs, int vsel, double voff, int nsplit, int vsel2) {
     EigGraphKey k;
     std::memset(&k, 0, sizeof(k));
     k.nmat = nmat; k.nb = nb; k.nvec = nvec; k.vsel = vsel; k.nsplit = nsplit; k.vsel2 = vsel2;
@@ -1783,6 +1779,27 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
                             c->dEigWork.p, c->dEigRef.p, c->dScratch.p, c->dVrange.p};
     std::memcpy(k.ptr, ptrs, sizeof(ptrs));
     k.len[0] = c->dEigWork.n; k.len[1] = c->dEigRef.n; k.len[2] = c->dScratch.n; k.len[3] = c->dVrange.n;
+    return k;
+}
+
+/* the scratch buffers still have the addresses and sizes the key recorded */
+static bool eig_bufs_match(const pmx_ctx *c, const EigGraphKey &k) {
+    return c->dEigWork.p == k.ptr[8] && c->dEigRef.p == k.ptr[9] && c->dScratch.p == k.ptr[10] &&
+           c->dVrange.p == k.ptr[11] && c->dEigWork.n == k.len[0] && c->dEigRef.n == k.len[1] &&
+           c->dScratch.n == k.len[2] && c->dVrange.n == k.len[3];
+}
+
+static bool graphs_allowed(const pmx_ctx *c) {
+    const char *env = std::getenv("PMX_GRAPHS");
+    return c->graphs && !c->prof && !c->capturing && !(env && env[0] == '0');
+}
+
+int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, double *dEvals, zd *dEvecs, int vsel,
+               double voff, int nsplit, int vsel2) {
+    if (nmat <= 0) return 0;
+    if (!graphs_allowed(c))
+        return eig_device_run(c, nmat, dKpts, nb, nvec, dEvals, dEvecs, vsel, voff, nsplit, vsel2);
+    const EigGraphKey k = eig_key(c, nmat, dKpts, nb, nvec, dEvals, dEvecs, vsel, voff, nsplit, vsel2);
     const std::string key(reinterpret_cast<const char *>(&k), sizeof(k));
 
     auto g = c->eig_graphs.find(key);
@@ -1811,9 +1828,7 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
     const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
     hipGraphExec_t exec = nullptr;
     /* a buffer that grew while capturing leaves the graph on other addresses than the key's */
-    const bool same_bufs = c->dEigWork.p == ptrs[8] && c->dEigRef.p == ptrs[9] && c->dScratch.p == ptrs[10] &&
-                           c->dVrange.p == ptrs[11] && c->dEigWork.n == k.len[0] && c->dEigRef.n == k.len[1] &&
-                           c->dScratch.n == k.len[2] && c->dVrange.n == k.len[3];
+    const bool same_bufs = eig_bufs_match(c, k);
     const bool ok = !rc && ec == hipSuccess && graph && same_bufs &&
                     hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
     if (graph) (void)hipGraphDestroy(graph);
@@ -1831,6 +1846,105 @@ int eig_device(pmx_ctx *c, int nmat, const double *dKpts, int nb, int nvec, doub
     c->eig_graphs[key] = exec;
     c->graph_captures++;
     PMX_CHECK_HIP(hipGraphLaunch(exec, c->stream));
+    return 0;
+}
+
+/* Eigensolve batches of several contexts (systems) on one device in one
+ * launch: each batch runs on its own context's stream, forked from and
+ * joined to the first context's stream, so SOC and non-SOC bases (complex
+ * and real kernels, any sizes) run side by side.  The call pattern is the
+ * single-context one: the first call of a set of shapes runs eagerly, the
+ * second is captured into one graph with a branch per batch (kept by the
+ * first context), later ones replay it. */
+int eig_device_multi(int n, pmx_ctx *const *cs, const int *nmat, const double *const *dK, const int *nb,
+                     const int *nvec, double *const *dEvals, zd *const *dEvecs) {
+    pmx_ctx *c0 = cs[0];
+    bool graphs = true;
+    std::string key;
+    for (int i = 0; i < n; i++) {
+        graphs = graphs && graphs_allowed(cs[i]);
+        const EigGraphKey k = eig_key(cs[i], nmat[i], dK[i], nb[i], nvec[i], dEvals[i], dEvecs[i], 0, 0.0, -1, 0);
+        key.append(reinterpret_cast<const char *>(&k), sizeof(k));
+    }
+    auto run_all = [&]() -> int {
+        for (int i = 0; i < n; i++) {
+            const int rc = eig_device_run(cs[i], nmat[i], dK[i], nb[i], nvec[i], dEvals[i], dEvecs[i], 0, 0.0, -1, 0);
+            if (rc) return rc;
+        }
+        return 0;
+    };
+    /* fork: the other streams wait for the first; join: the first waits for them */
+    std::vector<hipEvent_t> ev(2 * (size_t)n, nullptr);
+    struct Release {
+        std::vector<hipEvent_t> &e;
+        ~Release() {
+            for (auto &x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } release{ev};
+    for (auto &x : ev) PMX_CHECK_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+    auto fork = [&]() -> int {
+        PMX_CHECK_HIP(hipEventRecord(ev[0], c0->stream));
+        for (int i = 1; i < n; i++) PMX_CHECK_HIP(hipStreamWaitEvent(cs[i]->stream, ev[0], 0));
+        return 0;
+    };
+    auto join = [&]() -> int {
+        for (int i = 1; i < n; i++) {
+            PMX_CHECK_HIP(hipEventRecord(ev[n + i], cs[i]->stream));
+            PMX_CHECK_HIP(hipStreamWaitEvent(c0->stream, ev[n + i], 0));
+        }
+        return 0;
+    };
+    int rc;
+    if (graphs) {
+        auto g = c0->multi_graphs.find(key);
+        if (g != c0->multi_graphs.end()) {
+            PMX_CHECK_HIP(hipGraphLaunch(g->second, c0->stream));
+            c0->graph_replays++;
+            return 0;
+        }
+    }
+    auto seen = c0->eig_seen.find(key);
+    if (!graphs || seen == c0->eig_seen.end() || seen->second < 0) {
+        if ((rc = fork()) || (rc = run_all()) || (rc = join())) return rc;
+        if (graphs && seen == c0->eig_seen.end()) {
+            if (c0->eig_seen.size() >= 64) c0->eig_seen.clear();
+            c0->eig_seen[key] = 1;
+        }
+        return 0;
+    }
+    PMX_CHECK_HIP(hipStreamBeginCapture(c0->stream, hipStreamCaptureModeRelaxed));
+    for (int i = 0; i < n; i++) cs[i]->capturing = true;
+    rc = fork();
+    if (!rc) rc = run_all();
+    if (!rc) rc = join();
+    for (int i = 0; i < n; i++) cs[i]->capturing = false;
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c0->stream, &graph);
+    bool same_bufs = true;
+    for (int i = 0; i < n; i++) {
+        EigGraphKey k;
+        std::memcpy(&k, key.data() + i * sizeof(EigGraphKey), sizeof(k));
+        same_bufs = same_bufs && eig_bufs_match(cs[i], k);
+    }
+    hipGraphExec_t exec = nullptr;
+    const bool ok = !rc && ec == hipSuccess && graph && same_bufs &&
+                    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    if (!ok) {
+        (void)hipGetLastError();
+        seen->second = -1;
+        if ((rc = fork()) || (rc = run_all()) || (rc = join())) return rc;
+        return 0;
+    }
+    if (c0->multi_graphs.size() >= 16) {
+        for (auto &kv : c0->multi_graphs) (void)hipGraphExecDestroy(kv.second);
+        c0->multi_graphs.clear();
+    }
+    c0->multi_graphs[key] = exec;
+    c0->graph_captures++;
+    PMX_CHECK_HIP(hipGraphLaunch(exec, c0->stream));
     return 0;
 }
 

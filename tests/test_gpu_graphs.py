@@ -100,3 +100,37 @@ def test_chi_q_list_replay_equals_eager():
     scale = max(np.abs(out[1][0]).max(), np.abs(out[1][1]).max())
     assert np.abs(out[0][0] - out[1][0]).max() <= 1e-12 * scale
     assert np.abs(out[0][1] - out[1][1]).max() <= 1e-12 * scale
+
+
+TE_NOSOC = TE[:TE.index("-us_so")]  # the same Te crystal without spin-orbit: complex spatial H
+
+
+def test_eig_batch_multi_mixed_soc_equals_single():
+    """SOC (spinor, complex) and non-SOC (real and complex spatial) batches in
+    one launch, four calls (two eager: the first also sizes the scratch;
+    then captured, replayed): every batch equals pmx_eig_batch on its own
+    context bit for bit."""
+    _say("eig batch multi: Te SOC + Te + Si real")
+    cases = [(TE, 64, 24), (TE_NOSOC, 96, 20), (SI, 300, 16)]
+    ctxs = [_ctx(a, True) for a, _, _ in cases]
+    refs = [_ctx(a, False) for a, _, _ in cases]
+    try:
+        rng = np.random.default_rng(31)
+        Ks = [rng.uniform(-0.5, 0.5, (nk, 3)) for _, nk, _ in cases]
+        nbs = [nb for _, _, nb in cases]
+        want = [r.eig(K, nb, vectors=True) for r, K, nb in zip(refs, Ks, nbs)]
+        for call in range(4):
+            got = capi.eig_multi(ctxs, Ks, nbs, vectors=True)
+            for (ev, vec), (ev0, vec0) in zip(got, want):
+                assert np.array_equal(ev, ev0) and np.array_equal(vec, vec0), call
+        st = ctxs[0].graphs()
+        assert st["captures"] == 1 and st["replays"] == 1, st
+        # eigenvalues only, new k in the same buffers
+        Ks2 = [rng.uniform(-0.5, 0.5, (nk, 3)) for _, nk, _ in cases]
+        for call in range(3):
+            got = capi.eig_multi(ctxs, Ks2, nbs, vectors=False)
+            for (ev, _), r, K, nb in zip(got, refs, Ks2, nbs):
+                assert np.array_equal(ev, r.eig(K, nb)), call
+    finally:
+        for c in ctxs + refs:
+            c.close()
