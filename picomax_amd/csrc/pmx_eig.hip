@@ -39,6 +39,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -972,6 +973,224 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, WPE) tridiag_tile_real_kerne
     }
 }
 
+/* Register-tiled Householder tridiagonalisation of a complex Hermitian H
+ * (spatial without an inversion centre, or the spinor SOC basis): the tiled
+ * real kernel's scheme with complex tiles, for bases whose lower triangle
+ * fits the registers of one workgroup (N up to 22 TT).  Tile (I, J), I >= J,
+ * holds block A_IJ; the diagonal tiles hold their whole Hermitian block.
+ * The reflector convention is tridiag_kernel<zd>'s (H = I - tau v v^H with
+ * v_{k+1} = 1, p = tau A v, w = p - (tau/2)(p^H v) v, A -= v w^H + w v^H),
+ * so the back-transformation is shared.  Per step:
+ *   v, tau, beta from x (every wave forms them itself, bit-identically);
+ *   tile (I, J) writes A_IJ v_J into slot (I, J) and A_IJ^H v_I into (J, I);
+ *   row r sums its slots: p = tau A v;  w = p - (tau/2)(p^H v) v;
+ *   A_IJ -= v_I w_J^H + w_I v_J^H; the tiles of column k+1 write it out.
+ * Reflector entries go straight from each thread's row of v to HBM. */
+/* element (r, c) of the Hermitian H from its lower triangle; out of line, so
+ * the unrolled tile load does not hold many assemblies live at once */
+__device__ __noinline__ zd herm_elem(const EigArgs &a, const double *Kv, int r, int c) {
+    return (r >= c) ? h_elem<zd>(a, Kv, r, c) : cj(h_elem<zd>(a, Kv, c, r));
+}
+
+template <int TT, int WPE>
+__global__ void __launch_bounds__(TILE_MAX_THREADS, WPE) tridiag_tile_cplx_kernel(EigArgs a, int nblk) {
+    constexpr int TS = TT + 1;
+    auto tvi = [](int r) { return (r / TT) * TS + (r % TT); };
+    extern __shared__ double smem_d[];
+    const int N = a.N, NR = nblk * TT;
+    const int nvp = nblk * TS, nblkp = nblk | 1;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    zd *xk0 = reinterpret_cast<zd *>(smem_d), *xk1 = xk0 + nvp, *pp = xk1 + nvp;
+    zd *vw = pp + nvp + (size_t)wave * nvp;                        /* this wave's copy of v */
+    zd *part = pp + nvp + (size_t)(blockDim.x >> 6) * nvp;         /* [contributing block C][row block R][TS] */
+    zd *tt = part + (size_t)nblk * nblkp * TS;                     /* [N] */
+    double *dd = reinterpret_cast<double *>(tt + N), *ee = dd + N;  /* [N] each */
+    const int ntile = nblk * (nblk + 1) / 2;
+    const int mat = blockIdx.x;
+    const size_t P = (size_t)N * (N + 1) / 2;
+
+    /* tile of this thread: the m-th group holds block column J = nblk-1-m, rows I = J..nblk-1 */
+    int I = 0, J = 0;
+    const bool has_tile = tid < ntile;
+    if (has_tile) {
+        int m = (int)((sqrt(8.0 * tid + 1.0) - 1.0) * 0.5);
+        while (m * (m + 1) / 2 > tid) m--;
+        while ((m + 1) * (m + 2) / 2 <= tid) m++;
+        J = nblk - 1 - m;
+        I = J + (tid - m * (m + 1) / 2);
+    }
+    const bool diag = I == J;
+    const int r0 = I * TT, c0 = J * TT;
+    const int ib = I * TS, jb = J * TS;
+    const double Kv[3] = {a.K[3 * mat], a.K[3 * mat + 1], a.K[3 * mat + 2]};
+    zd *vref = a.vref ? reinterpret_cast<zd *>(a.vref) + (size_t)mat * P : nullptr;
+
+    zd A[TT][TT];
+#pragma unroll
+    for (int i = 0; i < TT; i++)
+#pragma unroll
+        for (int j = 0; j < TT; j++) {
+            const int r = r0 + i, c = c0 + j;
+            zd h = zd{0.0, 0.0};
+            if (has_tile && r < N && c < N) h = herm_elem(a, Kv, r, c);
+            A[i][j] = h;
+        }
+    if (has_tile && J == 0)
+#pragma unroll
+        for (int i = 0; i < TT; i++) xk0[(ib + i)] = A[i][0];
+    __syncthreads();
+    for (int kb1 = 0; kb1 < N; kb1 += TT) {
+#pragma unroll
+    for (int s = 0; s < TT; s++) {
+        const int k = kb1 + s - 1;
+        if (k < 0 || k + 1 >= N) continue;
+        const int cur = (TT % 2 == 0) ? ((s + 1) & 1) : (k & 1);
+        const zd *x = cur ? xk1 : xk0;
+        zd *xn = cur ? xk0 : xk1;
+        double s2 = 0.0;
+        for (int r = k + 2 + lane; r < N; r += 64) s2 += nrm2(x[tvi(r)]);
+        s2 = wave_sum_full(s2);
+        const zd alpha = x[tvi(k + 1)];
+        double beta;
+        zd tau, scal;
+        if (s2 == 0.0 && alpha.y == 0.0) {
+            beta = alpha.x;
+            tau = zd{0.0, 0.0};
+            scal = zd{0.0, 0.0};
+        } else {
+            beta = -copysign(sqrt(nrm2(alpha) + s2), alpha.x);
+            tau = cdiv(zd{beta, 0.0} - alpha, zd{beta, 0.0});
+            scal = cdiv(zd{1.0, 0.0}, alpha - zd{beta, 0.0});
+        }
+        const bool tnz = tau.x != 0.0 || tau.y != 0.0;
+        const int cmin = (k + 1) / TT;
+        const bool active = has_tile && J >= cmin && tnz;
+        if (tnz) {
+            for (int r = cmin * TT + lane; r < NR; r += 64) {
+                const zd xr = x[tvi(r)];
+                vw[tvi(r)] = (r == k + 1) ? zd{1.0, 0.0} : ((r > k + 1 && r < N) ? xr * scal : zd{0.0, 0.0});
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (active) {
+            /* two passes, one vector live at a time: A_IJ v_J, then A_IJ^H v_I */
+            {
+                zd vJ[TT];
+#pragma unroll
+                for (int j = 0; j < TT; j++) vJ[j] = vw[(jb + j)];
+                zd *sI = part + ((size_t)J * nblkp + I) * TS;
+#pragma unroll
+                for (int i = 0; i < TT; i++) {
+                    zd pi = zd{0.0, 0.0};
+#pragma unroll
+                    for (int j = 0; j < TT; j++) pi += A[i][j] * vJ[j];
+                    sI[i] = pi;
+                }
+            }
+            if (!diag) {
+                zd vI[TT];
+#pragma unroll
+                for (int i = 0; i < TT; i++) vI[i] = vw[(ib + i)];
+                zd *sJ = part + ((size_t)I * nblkp + J) * TS;
+#pragma unroll
+                for (int j = 0; j < TT; j++) {
+                    zd pj = zd{0.0, 0.0};
+#pragma unroll
+                    for (int i = 0; i < TT; i++) pj += cj(A[i][j]) * vI[i];
+                    sJ[j] = pj;
+                }
+            }
+        }
+        __syncthreads(); /* (1) partials */
+        if (tnz && tid < NR) {
+            const int r = tid, R = r / TT, i = r % TT;
+            zd p = zd{0.0, 0.0};
+            if (r > k && r < N) {
+                const zd *col = part + (size_t)R * TS + i;
+                for (int C = cmin; C < nblk; C++) p += col[(size_t)C * nblkp * TS];
+                p = tau * p;
+            }
+            pp[tvi(r)] = p;
+        }
+        /* reflector of this step (zero when tau = 0: the step is the identity) */
+        if (vref && tid >= k + 2 && tid < N) vref[pk(tid, k, N)] = tnz ? vw[tvi(tid)] : zd{0.0, 0.0};
+        if (tid == 0) {
+            dd[k] = x[tvi(k)].x;
+            ee[k] = beta;
+            tt[k] = tau;
+        }
+        __syncthreads(); /* (2) p */
+        if (tnz) {
+            double pr = 0.0, pim = 0.0;
+            for (int r = k + 1 + lane; r < N; r += 64) {
+                const zd cv = cj(pp[tvi(r)]) * vw[tvi(r)];
+                pr += cv.x;
+                pim += cv.y;
+            }
+            pr = wave_sum_full(pr);
+            pim = wave_sum_full(pim);
+            const zd al = (-0.5) * (tau * zd{pr, pim});
+            if (active) {
+                /* two rank-1 halves: A -= v_I w_J^H, then A -= w_I v_J^H */
+                {
+                    zd u[TT], z[TT];
+#pragma unroll
+                    for (int j = 0; j < TT; j++) z[j] = cj(pp[(jb + j)] + al * vw[(jb + j)]);
+#pragma unroll
+                    for (int i = 0; i < TT; i++) u[i] = vw[(ib + i)];
+#pragma unroll
+                    for (int i = 0; i < TT; i++)
+#pragma unroll
+                        for (int j = 0; j < TT; j++) A[i][j] -= u[i] * z[j];
+                }
+                {
+                    zd u[TT], z[TT];
+#pragma unroll
+                    for (int j = 0; j < TT; j++) z[j] = cj(vw[(jb + j)]);
+#pragma unroll
+                    for (int i = 0; i < TT; i++) u[i] = pp[(ib + i)] + al * vw[(ib + i)];
+#pragma unroll
+                    for (int i = 0; i < TT; i++)
+#pragma unroll
+                        for (int j = 0; j < TT; j++) A[i][j] -= u[i] * z[j];
+                }
+            }
+        }
+        /* column k+1 = kb1 + s for the next step */
+        if (has_tile && c0 == kb1)
+#pragma unroll
+            for (int i = 0; i < TT; i++) xn[(ib + i)] = A[i][s];
+        __syncthreads(); /* (3) next column */
+    }
+    }
+    if (tid == 0) dd[N - 1] = ((N & 1) ? xk0 : xk1)[tvi(N - 1)].x; /* column N-1 was written at step N-2 */
+    __syncthreads();
+    for (int i = tid; i < N; i += blockDim.x) {
+        a.dS[(size_t)i * a.nmat + mat] = dd[i];
+        a.eS[(size_t)i * a.nmat + mat] = (i + 1 < N) ? ee[i] : 0.0;
+        if (a.tauv) reinterpret_cast<zd *>(a.tauv)[(size_t)mat * N + i] = (i + 1 < N) ? tt[i] : zd{0.0, 0.0};
+    }
+    if (tid == 0) {
+        double gl = dd[0], gu = dd[0], emax2 = 0.0;
+        for (int i = 0; i < N; i++) {
+            const double di = dd[i];
+            double off = (i > 0 ? fabs(ee[i - 1]) : 0.0) + (i + 1 < N ? fabs(ee[i]) : 0.0);
+            gl = fmin(gl, di - off);
+            gu = fmax(gu, di + off);
+            if (i + 1 < N) emax2 = fmax(emax2, ee[i] * ee[i]);
+        }
+        double tnorm = fmax(fabs(gl), fabs(gu));
+        gl -= 2.0 * DBL_EPSILON * tnorm * N + 1e-300;
+        gu += 2.0 * DBL_EPSILON * tnorm * N + 1e-300;
+        a.gb[4 * mat] = gl;
+        a.gb[4 * mat + 1] = gu;
+        a.gb[4 * mat + 2] = DBL_MIN * fmax(1.0, emax2);
+        a.gb[4 * mat + 3] = tnorm;
+    }
+}
+
 /* lowest nb eigenvalues of every T by bisection on Sturm counts, one thread
  * per (matrix, index).  A workgroup takes mpw consecutive matrices and
  * stages their d and e^2 in LDS once, so the ~50 counts per eigenvalue read
@@ -1546,6 +1765,33 @@ int tridiag_generic(pmx_ctx *c, EigArgs &a, bool real, bool lds_matrix, int grid
                              return lds_matrix ? launch_tridiag_fused<zd, true>(c, aa, g, ldsf)
                                                : launch_tridiag_fused<zd, false>(c, aa, g, ldsf);
                          }});
+    /* complex bases whose lower triangle fits one workgroup's registers
+     * (N <= 132): the register-tiled kernel with 6x6 complex tiles */
+    if (!real) {
+        auto tiles = [&](int tt) {
+            const int nb_ = (N + tt - 1) / tt;
+            const int thr = (std::max(nb_ * (nb_ + 1) / 2, nb_ * tt) + 63) / 64 * 64;
+            const size_t ts = tt + 1;
+            const size_t lds_t = sizeof(zd) * ((3 + thr / 64) * (size_t)nb_ * ts + (size_t)nb_ * (nb_ | 1) * ts + N) +
+                                 sizeof(double) * 2 * (size_t)N;
+            return std::make_tuple(nb_, thr, lds_t);
+        };
+        auto add = [&](const char *name, int tt, auto kern) {
+            const auto [nb_, thr, lds_t] = tiles(tt);
+            if (thr > TILE_MAX_THREADS || lds_t > c->lds_max) return;
+            cands.push_back({name, [=](int nm) {
+                                 EigArgs aa = a;
+                                 aa.nmat = nm;
+                                 if (lds_t > 65536)
+                                     PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern,
+                                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_t));
+                                 hipLaunchKernelGGL(kern, dim3(nm), dim3(thr), lds_t, c->stream, aa, nb_);
+                                 PMX_CHECK_HIP(hipGetLastError());
+                                 return 0;
+                             }});
+        };
+        add("ctile6", 6, tridiag_tile_cplx_kernel<6, 2>);
+    }
     const std::string key = std::string(real ? "tridiag real" : "tridiag complex") +
                             (lds_matrix ? " lds N=" : " global N=") + std::to_string(N);
     return tune_run(c, key, a.nmat, 2 * grid, cands);

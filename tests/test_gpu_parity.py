@@ -284,3 +284,40 @@ def test_tridiag_real_forms_against_oracle(case, form, monkeypatch):
             assert np.abs(V.conj().T @ V - np.eye(nb)).max() < 1e-10
     finally:
         c.close()
+
+
+CPLX_SMALL = {  # complex bases of at most 132: the register-tiled complex kernel applies
+    "gaas_59": None,  # from the manifest (complex spatial, 59 PW)
+    "si_soc_118": ["-crystal", "zb", "-a", "5.43", "-vg", "3,4,8,11,-0.21,0,0.04,0.08,-0.21,0,0.04,0.08",
+                   "-gcut", "3.4641016151377544", "-us_so", "3:0.012,4:0.005,8:-0.004,11:0.002"],
+}
+
+
+@pytest.mark.parametrize("form", ["ctile6", "three_stream", "fused"])
+@pytest.mark.parametrize("case", sorted(CPLX_SMALL))
+def test_tridiag_complex_forms_against_oracle(manifest, case, form, monkeypatch):
+    """Every complex tridiagonalisation candidate for bases up to 132 (6x6
+    complex register tiles, the LDS kernel in its two forms), spatial and
+    spinor, gives the oracle's eigenpairs."""
+    monkeypatch.setenv("PMX_FORCE_KERNEL", form)
+    args = manifest["gaas_band"]["args"] if case == "gaas_59" else CPLX_SMALL[case]
+    args = _expand(args)
+    _say("tridiag complex form %s %s" % (form, case))
+    c = capi.Context(0)
+    try:
+        inp = capi.Input(args)
+        c.set_system(inp.system(), inp)
+        s, p = cli_args.build(args if "-switch" in args else ["-switch", "band"] + args)
+        K = np.random.default_rng(6).uniform(-0.6, 0.6, (6, 3))
+        K[0] = 0.0
+        nb = 24
+        ev, vec = c.eig(K, nb, vectors=True)
+        for k in range(len(K)):
+            H = s.hamiltonian(K[k])
+            assert H.shape[0] <= 2 * 132
+            assert np.abs(ev[k] - np.linalg.eigvalsh(H)[:nb]).max() < 1e-10
+            V = vec[k].T
+            assert np.abs(H @ V - V * ev[k]).max() < 1e-9
+            assert np.abs(V.conj().T @ V - np.eye(nb)).max() < 1e-10
+    finally:
+        c.close()
