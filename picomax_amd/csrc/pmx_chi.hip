@@ -443,7 +443,7 @@ template <class T> __global__ void __launch_bounds__(256) rho_mfma_kernel(RhoArg
  * re / im) and combines the channels in registers: the K loop has no weight
  * loads and no complex multiplies.
  * ------------------------------------------------------------------ */
-template <class T> __global__ void __launch_bounds__(256) rho_cart_kernel(RhoArgs a) {
+template <class T> __global__ void __launch_bounds__(256, 4) rho_cart_kernel(RhoArgs a) {
     typedef typename RhoT<T>::V V;
     typedef typename Cplx<T>::type CT;
     const int k = blockIdx.x;
