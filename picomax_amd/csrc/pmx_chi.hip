@@ -799,7 +799,7 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
 constexpr int MOM_MAXNT = 12; /* largest order count of the split-BF16 kernel */
 /* NI items per row: 1 (spatial), 2 (SOC: + and - channel); MAXNT >= NT orders */
 template <int NI, int MAXNT>
-__global__ void __launch_bounds__(256) moment_bf16x3_kernel(MomArgs2 a) {
+__global__ void __launch_bounds__(256, 3) moment_bf16x3_kernel(MomArgs2 a) {
     __shared__ fx4 red[2][64][2 * ((MAXNT + 1) / 2)];
     /* workgroup l runs on XCD l % 8: bins go round-robin over the XCDs and all
      * tile pairs of a bin run on the same one, sharing its rows in that L2 */
