@@ -1061,7 +1061,7 @@ struct ConvArgs {
  * miss a block's band are skipped for that block.  Kt is staged in LDS.  T =
  * double (FP64 MFMA 16x16x4) or float (FP32 MFMA 16x16x4, the mixed paths). */
 constexpr int CONV_FB = 4; /* frequency blocks of 16 per wave */
-template <class T> __global__ void __launch_bounds__(256) conv_mfma_kernel(ConvArgs a) {
+template <class T> __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 8 : 1) conv_mfma_kernel(ConvArgs a) {
     typedef typename MfmaT<T>::V V;
     typedef typename MfmaT<T>::C C;
     extern __shared__ double smem_c[];
@@ -1263,7 +1263,7 @@ __global__ void __launch_bounds__(256) f0max_kernel(int nt, const double *dE, do
  * double (FP64 MFMA) or float (FP32 MFMA, mixed paths; Kk then in FP32). */
 constexpr int KK_CT = 4;
 template <class T>
-__global__ void __launch_bounds__(256) kk_gemm_kernel(int M, int NF, const double *Re0, const double *Im0,
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 8 : 1) kk_gemm_kernel(int M, int NF, const double *Re0, const double *Im0,
                                                       const T *Kk, double *Re, double *Im, int ne2, int nq, int q) {
     typedef typename MfmaT<T>::V V;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
