@@ -1803,8 +1803,9 @@ template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, c
                                         const void *tau, const double *phase, const int *vr, zd *evecs) {
     if (sizeof(T) == sizeof(double) && wy_usable(c, N, nvec)) {
         /* two register budgets: 16 reflector values per thread (any Np <= 256,
-         * three waves per SIMD) or 10 (Np <= 160, capped at 128 VGPRs for four
-         * waves per SIMD); the autotuner picks per shape.  Each run rewrites
+         * two waves per SIMD) or 10 (Np <= 160, capped at 168 VGPRs for three
+         * waves per SIMD, which the 16-column LDS also allows); the autotuner
+         * picks per shape.  Each run rewrites
          * the same vectors from the same inputs, so sample runs are harmless.
          * With a selection of more than 16 vectors, the matrices that want at
          * most 16 (the usual case: the conduction or the valence states of
@@ -1832,7 +1833,7 @@ template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, c
         std::vector<TuneCand> cands;
         cands.push_back({"wy_vl16", [&](int g) { return launch(backtrans_wy_kernel<16, 1>, g); }});
         if (Np * BW <= 256 * 10)
-            cands.push_back({"wy_vl10", [&](int g) { return launch(backtrans_wy_kernel<10, 4>, g); }});
+            cands.push_back({"wy_vl10", [&](int g) { return launch(backtrans_wy_kernel<10, 3>, g); }});
         return tune_run(c, "backtrans real N=" + std::to_string(N) + " nvec=" + std::to_string(nvec), nmat,
                         16 * c->num_cu, cands);
     }
