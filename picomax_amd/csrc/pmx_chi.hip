@@ -713,7 +713,7 @@ __device__ __forceinline__ MomFetch<T> mom_fetch(const MomArgs2 &a, const C *X, 
     return f;
 }
 
-template <class T> __global__ void __launch_bounds__(256) moment_mfma_kernel(MomArgs2 a) {
+template <class T> __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 8 : 1) moment_mfma_kernel(MomArgs2 a) {
     typedef typename MfmaT<T>::V V;
     typedef typename MfmaT<T>::C C;
     constexpr int MAXW = 5;
