@@ -278,14 +278,26 @@ typedef struct {
 
 int pmx_input_info(const pmx_input *in, pmx_run_info *out);
 
+/* The reference main's set-up report on stdout (src/pmx.cpp:84-160): banner,
+ * host, parameters, the Q path, the G count and the k-point source with
+ * their set-up times; with -debug > 1 also printInput (src/pmx_io.cpp:510-614:
+ * options, form factors, SOC maps, atomic basis, and with -debug > 2 every G,
+ * K and Q vector in Eigen's row format).  precision != PMX_PREC_FP64 adds one
+ * "precision" line to the parameters (the option the reference does not have). */
+int pmx_print_setup(const pmx_input *in, int precision);
+
 /* generate_gpt on its own: up to max_out G vectors sorted like the reference;
  * returns NPW (>= 0) or an error code. */
 int pmx_generate_gpt(const double *recip, int dim, double gcut, int max_out, double *G_out);
 
-/* The CLI body after parsing: runs -switch band | epsij (epsLL: nothing, as
- * the reference's main) on ctx, prints the reference's progress lines to
- * stdout and writes ${wdir}/${outputfile}.dat (+ .gpt for epsij).
- * precision selects the chi path (PMX_PREC_*). */
+/* The CLI body after parsing (src/pmx.cpp:171-203): runs -switch band |
+ * epsij on ctx, and epsLL / epsxyz as the older snapshot's main dispatches
+ * them (src_bkp/pmx.cpp:206-244; the current main has no such branches),
+ * prints the reference's solver progress lines to stdout (chi_tensor's
+ * per-phase and per-q lines with their times, src/pmx_chi.cpp:370-657) and
+ * writes ${wdir}/${outputfile}.dat (+ .gpt for the chi switches).  An absent
+ * or unknown -switch does nothing and returns 0.  precision selects the chi
+ * path (PMX_PREC_*). */
 int pmx_run(pmx_ctx *ctx, const pmx_input *in, int precision);
 
 #ifdef __cplusplus

@@ -22,7 +22,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 EXPORTS = ["pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_set_system",
            "pmx_set_form_factors", "pmx_get_loci", "pmx_ref_energy", "pmx_bands", "pmx_eig_batch", "pmx_eig_batch_multi",
            "pmx_chi", "pmx_chi_device", "pmx_chi_ll", "pmx_chi_ll_device", "pmx_chi_xyz", "pmx_chi_xyz_device", "pmx_profile", "pmx_profile_report", "pmx_tune_report", "pmx_graphs", "pmx_kk", "pmx_epsinv", "pmx_eps_macro", "pmx_polarization_bands", "pmx_grad_bands", "pmx_grad_chi", "pmx_input_parse", "pmx_input_free",
-           "pmx_input_system", "pmx_input_info", "pmx_generate_gpt", "pmx_run"]
+           "pmx_input_system", "pmx_input_info", "pmx_print_setup", "pmx_generate_gpt", "pmx_run"]
 
 
 class PmxSystem(ctypes.Structure):
@@ -106,6 +106,7 @@ def lib():
     L.pmx_input_free.restype = None
     L.pmx_input_system.argtypes = [vp, ctypes.POINTER(PmxSystem)]
     L.pmx_input_info.argtypes = [vp, ctypes.POINTER(PmxRunInfo)]
+    L.pmx_print_setup.argtypes = [vp, ctypes.c_int]
     L.pmx_generate_gpt.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]
     L.pmx_run.argtypes = [vp, vp, ctypes.c_int]
     _lib = L

@@ -11,15 +11,11 @@
  */
 #include "pmx.h"
 
-#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <ctime>
-#include <fstream>
 #include <iostream>
 #include <string>
-#include <thread>
 
 static void doc() {
     std::cout << "============================================================\n"
@@ -81,23 +77,7 @@ static const char *value(int argc, char **argv, const char *o) {
     return "";
 }
 
-static unsigned meminfo(const char *key) {
-    std::ifstream f("/proc/meminfo");
-    std::string tok;
-    while (f >> tok) {
-        if (tok == key) {
-            unsigned v = 0;
-            f >> v;
-            return v;
-        }
-        std::getline(f, tok);
-    }
-    return 0;
-}
-
 int main(int argc, char **argv) {
-    auto t_init = std::chrono::system_clock::now();
-    std::time_t dt = std::chrono::system_clock::to_time_t(t_init);
     if (has(argc, argv, "-h") || has(argc, argv, "-help")) {
         doc();
         return 0;
@@ -131,46 +111,10 @@ int main(int argc, char **argv) {
             abort();
         }
     }
-    std::cout << "*********************************\n"
-              << "***PicoMax 0.1 progress output***\n"
-              << "*********************************\n"
-              << std::ctime(&dt) << "PicoMax v0.1\n"
-              << "  Running on " << pmx_version() << "\n"
-              << "  Using " << std::thread::hardware_concurrency() << " cores\n"
-              << "  Total memory: " << (meminfo("MemTotal:") / 1000) << " MB\n"
-              << "  Avail memory: " << (meminfo("MemFree:") / 1000) << " MB\n"
-              << "  Working dir: " << ri.wdir << "\n"
-              << "  Output file: " << ri.outputfile << ".dat\n"
-              << "---------------------------------\n"
-              << "Parameters:\n"
-              << "  encut = " << ri.encut << " eV\n"
-              << "  gcut = " << ri.gcut << "\n"
-              << "  nchi = " << ri.neps << "\n"
-              << "  nband = " << ri.nband << "\n"
-              << "  nfreq = " << (ri.nfreq ? ri.nfreq : 1) << "\n"
-              << "  dfreq = " << ri.dfreq << "\n"
-              << "  epsilon = " << ri.epsilon << "\n"
-              << "  precision = " << prec << "\n"
-              << "---------------------------------" << std::endl;
-    std::cout << "Generated Q-vector\n  Total number: " << ri.nq << std::endl;
-    std::cout << "Generated G-vectors\n  Number of planewaves: " << ri.npw << std::endl;
-    /* src/pmx.cpp:141-156 */
-    if (ri.kpoint == 0) {
-        std::cout << "Imported K-vectors from kpointfile: " << value(argc, argv, "-kpointfile") << std::endl;
-    } else if (ri.kpoint == 1) {
-        std::cout << "Generated K-vectors (Full BZ)\n  Simulation size in kpoint: " << ri.kpointorder << std::endl;
-    } else if (ri.kpoint == 2) {
-        std::cout << "Generated K-vectors (Monkhorst-Pack grids)\n  Simulation size in kpoint: " << ri.kpointorder
-                  << std::endl;
-    } else if (ri.kpoint == 3) {
-        std::cout << "Generated K-vectors\n  Using Cohen-Chadi grids with order: " << ri.kpointorder << std::endl;
-    }
-    std::cout << "  Number of K-vectors: " << ri.nk << std::endl;
+    /* banner, parameters, Q / G / K set-up lines (src/pmx.cpp:84-160) and,
+     * with -debug > 1, printInput and exit before any solve (src/pmx.cpp:164-167) */
+    pmx_print_setup(in, precision);
     if (ri.debug > 1) {
-        std::cout << "number of electronic bands: " << ri.nband << "\nnumber of valence electrons: " << ri.nvalence
-                  << "\nsize of dielectric matrix: " << ri.neps << "\nlattice constant [angstrom]: " << ri.a
-                  << "\nnumber of plane waves: " << ri.npw << "\nNumber of K-vector grids: " << ri.nk
-                  << "\nNumber of q-grids: " << ri.nq << std::endl;
         pmx_input_free(in);
         return 0;
     }

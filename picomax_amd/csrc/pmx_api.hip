@@ -269,6 +269,14 @@ int pmx_create(int device, pmx_ctx **out) {
     return PMX_OK;
 }
 
+extern "C++" {
+namespace pmx {
+void set_progress(pmx_ctx *c, bool on) {
+    if (c) c->progress = on;
+}
+} // namespace pmx
+}
+
 void pmx_destroy(pmx_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);

@@ -36,8 +36,10 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <iostream>
 
 namespace pmx {
 
@@ -1657,6 +1659,26 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         PMX_CHECK_HIP(hipStreamSynchronize(st));
         return PMX_OK;
     }
+    /* the reference's progress lines on stdout when the context asks for them
+     * (the CLI): chi_tensor's (src/pmx_chi.cpp:370-657), whose structure
+     * chi_LL and chi_xyz share (src_bkp/pmx_chi.cpp:179-397, 777-955); each
+     * "Elapsed time" waits for the stream, so it times the phase's kernels */
+    auto t_phase = std::chrono::system_clock::now();
+    auto head = [&](const std::string &line) {
+        if (!c->progress) return;
+        std::cout << line << std::endl;
+        t_phase = std::chrono::system_clock::now();
+    };
+    auto elapsed = [&](const char *indent) -> int {
+        if (!c->progress) return 0;
+        PMX_CHECK_HIP(hipStreamSynchronize(st));
+        std::cout << indent << "Elapsed time: "
+                  << std::chrono::duration<double>(std::chrono::system_clock::now() - t_phase).count() << " s"
+                  << std::endl;
+        return 0;
+    };
+    head("  Solving for |k,v>...");
+
     /* ---- phase A: eigenpairs at k (src/pmx_chi.cpp:379-413), solved in one
      * batch with those at k+q of the first q: [K; K + Q_0].  Eigenpairs at k
      * fill the first half of evk/vk, those at k+q the second. */
@@ -1676,6 +1698,9 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
     if ((rc = eig_device(c, p->nq > 0 ? 2 * p->nk : p->nk, w.Kd.p, nb_s, nb_s, evk_p, vk_p, vsel_k, p->offset,
                          p->nk, vsel_kq)))
         return rc;
+    if ((rc = elapsed("  "))) return rc;
+    head(std::string("  Solving for susceptibility tensor matrix elements") +
+         (xyz ? " (xyz-basis)" : ll ? " (alternative form)" : "") + "...");
     if ((rc = w.nc.ensure(p->nk)) || (rc = w.nv.ensure(p->nk)) || (rc = w.tbase.ensure(p->nk + 1)) ||
         (rc = w.bad.ensure(1)))
         return rc;
@@ -1708,6 +1733,8 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
 
     for (int q = 0; q < p->nq; q++) {
         const double Qx = p->Q[3 * q], Qy = p->Q[3 * q + 1], Qz = p->Q[3 * q + 2];
+        if (c->progress) std::cout << "  (" << (q + 1) << "/" << p->nq << ")..." << std::endl;
+        head("    Solving for overlap integrals...");
         /* ---- per-q tables: uvec_LT (src/pmx_basis.cpp:830-874), weights */
         std::vector<zd> uc, wA, wS;
         q_tables(c, Qx, Qy, Qz, spatial, uc, wA, wS, xyz);
@@ -1839,6 +1866,8 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                 hipLaunchKernelGGL(rho_kernel, dim3(p->nk), dim3(256), 0, st, ra);
         }
         PMX_CHECK_HIP(hipGetLastError());
+        if ((rc = elapsed("    "))) return rc;
+        head("    Solving for Xijmn...");
 
         /* ---- accumulate */
         for (int acc = 0; acc < nacc; acc++) {
@@ -1927,7 +1956,9 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                         spatial ? w.Z1.p : w.Z2.p, w.Re0.p, w.Im0.p};
             hipLaunchKernelGGL(pack_kernel, dim3(1024), dim3(256), 0, st, pa);
         }
+        if ((rc = elapsed("    "))) return rc;
         if (p->kk) {
+            head("    Kramers-Kronig transform...");
             Prof pf(c, "kk", 4.0 * per_q * p->nfreq, 8.0 * 4 * per_q);
             const int M = 9 * ne2;
             const dim3 grid(8 * (((M + 63) / 64 + 7) / 8) * ((p->nfreq + 16 * KK_CT - 1) / (16 * KK_CT)));
@@ -1937,6 +1968,8 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
             else
                 hipLaunchKernelGGL(kk_gemm_kernel<double>, grid, dim3(256), 0, st, M, p->nfreq, w.Re0.p, w.Im0.p,
                                    w.Kkd.p, dRe, dIm, ne2, p->nq, q);
+            PMX_CHECK_HIP(hipGetLastError());
+            if ((rc = elapsed("    "))) return rc;
         } else {
             hipLaunchKernelGGL(place_kernel, dim3(1024), dim3(256), 0, st, 9 * ne2, p->nfreq, ne2, p->nq, q, w.Re0.p,
                                w.Im0.p, dRe, dIm);

@@ -113,6 +113,9 @@ struct pmx_ctx {
     std::vector<std::string> tune_log;
     std::map<std::string, ProfAcc> prof_acc;
 
+    /* chi_tensor's progress lines on stdout (src/pmx_chi.cpp:370-657), for the CLI */
+    bool progress = false;
+
     /* chi work buffers, kept between calls (allocated on first use) */
     pmx::ChiWork *chi = nullptr;
 

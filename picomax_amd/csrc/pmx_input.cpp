@@ -18,16 +18,23 @@
 #include "pmx_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <ctime>
 #include <filesystem>
 #include <fstream>
+#include <iostream>
+#include <limits>
 #include <map>
 #include <numeric>
 #include <set>
 #include <tuple>
 #include <sstream>
 #include <stdexcept>
+#include <thread>
+
+#include <sched.h>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -330,9 +337,12 @@ int input_parse(int argc, const char *const *argv, Input &in) {
     }
 
     /* ---- generate_gpt / generate_gpt_rect */
+    auto t0 = std::chrono::system_clock::now();
     int npw = generate_gpt_host(in.recip, in.dim, in.gcut, in.gsym == 0, in.G);
     if (npw < 0) return npw;
     in.npw = npw;
+    auto t1 = std::chrono::system_clock::now();
+    in.t_gpt = std::chrono::duration<double>(t1 - t0).count();
 
     /* ---- k-points (src/pmx.cpp:141-154) */
     if (in.kpoint == 0) {
@@ -343,8 +353,161 @@ int input_parse(int argc, const char *const *argv, Input &in) {
     } else if (in.kpoint == 3) {
         generate_kpt_cohen_chadi(in.kpointorder, in.K, in.KW);
     } /* -kpoint 2 (Monkhorst-Pack) is an empty function in the reference: no k-points */
+    in.t_kpt = std::chrono::duration<double>(std::chrono::system_clock::now() - t1).count();
     return PMX_OK;
 }
+
+/* ------------------------------------------------------------ set-up report */
+namespace {
+
+/* a 3-vector as Eigen prints a row vector (include/eigen-3.4.0/Eigen/src/Core/IO.h,
+ * print_matrix with the default IOFormat): every coefficient right-aligned to
+ * the widest one at the stream's precision, separated by one space */
+void eigen_row(std::ostream &os, double x, double y, double z) {
+    const double v[3] = {x, y, z};
+    std::streamsize width = 0;
+    for (double c : v) {
+        std::stringstream ss;
+        ss.copyfmt(os);
+        ss << c;
+        width = std::max<std::streamsize>(width, (std::streamsize)ss.str().length());
+    }
+    for (int i = 0; i < 3; i++) {
+        if (i) os << ' ';
+        os.width(width);
+        os << v[i];
+    }
+}
+
+/* getCPUmodel (src/pmx_io.cpp:681-697): the first "Processor" or "model
+ * name" line among the lines it reads (it skips every other line) */
+std::string cpu_model() {
+    std::ifstream f("/proc/cpuinfo");
+    std::string line;
+    while (std::getline(f, line)) {
+        if (line.compare(0, 9, "Processor") == 0 || line.compare(0, 10, "model name") == 0) {
+            const std::string pre = "model name\t: ";
+            const size_t at = line.find(pre);
+            if (at != std::string::npos) line.erase(at, pre.size());
+            return line;
+        }
+        f.ignore(std::numeric_limits<std::streamsize>::max(), '\n');
+    }
+    return "unknown";
+}
+
+/* getMEMtotal / getMEMfree (src/pmx_io.cpp:618-654): kB from /proc/meminfo */
+unsigned meminfo_kb(const char *key) {
+    std::ifstream f("/proc/meminfo");
+    std::string tok;
+    while (f >> tok) {
+        if (tok == key) {
+            unsigned v = 0;
+            return (f >> v) ? v : 0;
+        }
+        f.ignore(std::numeric_limits<std::streamsize>::max(), '\n');
+    }
+    return 0;
+}
+
+/* omp_get_num_procs: the processors this process may run on */
+int num_procs() {
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) return CPU_COUNT(&set);
+    return (int)std::thread::hardware_concurrency();
+}
+
+/* printInput (src/pmx_io.cpp:510-614) */
+void print_input(const Input &in) {
+    const int natom = (int)in.tau.size(), nv = (int)in.g2.size();
+    std::cout << "working directory: " << in.wdir << "\n"
+              << "outputfile: " << in.outputfile << "\n"
+              << "logfile: " << in.logfile << "\n"
+              << "number of electronic bands: " << in.nband << "\n"
+              << "number of valence electrons: " << in.nvalence << "\n"
+              << "size of dielectric matrix: " << in.neps << "\n"
+              << "Material properties" << "\n"
+              << "lattice constant [angstrom]: " << in.a << "\n"
+              << "Empirical pseudopotential form factors, Vg:" << "\n"
+              << "g2: \t";
+    if (nv > 0) {
+        for (int i = 0; i < nv; i++) std::cout << in.g2[i] << "\t";
+        std::cout << "\n";
+        for (int i = 0; i < natom; i++) {
+            std::cout << "atom " << (i + 1) << ": ";
+            for (int j = 0; j < nv; j++) std::cout << in.vg[(size_t)i * nv + j] << "\t";
+            std::cout << "\n";
+        }
+    } else {
+        std::cout << "(Not provided or not applicable)\n";
+    }
+    std::cout << "Symmetric Spin-Orbit form factors (Us_SO_Ry):" << "\n";
+    if (in.use_uniform_us) {
+        std::cout << "  Using uniform Us_SO value: " << in.uniform_us << " Ry for all G^2\n";
+    } else if (!in.us.empty()) {
+        for (const auto &kv : in.us) std::cout << "  G^2 = " << kv.first << ": " << kv.second << " Ry\n";
+    } else {
+        std::cout << "  (Not provided or default values used)\n";
+    }
+    std::cout << "Antisymmetric Spin-Orbit form factor for G^2=0 (Ua_SO_Ry[0]):" << "\n";
+    auto g0 = in.ua.find(0.0);
+    if (g0 != in.ua.end())
+        std::cout << "  G^2 = 0: " << g0->second << " Ry\n";
+    else
+        std::cout << "  (Not provided, effectively 0 for G^2=0 and other G^2 values)\n";
+    std::cout << "Antisymmetric Spin-Orbit form factors (Ua_SO_Ry):" << "\n";
+    if (!in.ua.empty()) {
+        for (const auto &kv : in.ua) std::cout << "  G^2 = " << kv.first << ": " << kv.second << " Ry\n";
+    } else {
+        std::cout << "  (Not provided or default values used)\n";
+    }
+    std::cout << "atomic basis vectors:\n";
+    for (const V3 &t : in.tau) {
+        std::cout << " ";
+        eigen_row(std::cout, t.x, t.y, t.z);
+        std::cout << "\n";
+    }
+    std::cout << std::endl;
+    std::cout << "planewave energy cutoff [eV]: " << in.encut << std::endl;
+    std::cout << "planewave gpoint cutoff [2*pi/a]: " << in.gcut << std::endl;
+    std::cout << "number of plane waves: " << in.npw << std::endl;
+    std::cout << "Number of gpoint grids: " << in.npw << std::endl;
+    if (in.debug > 2) {
+        for (int g = 0; g < in.npw; g++) {
+            eigen_row(std::cout, in.G[3 * g], in.G[3 * g + 1], in.G[3 * g + 2]);
+            std::cout << "\n";
+        }
+        std::cout << std::endl;
+    }
+    if (in.kpoint == 0)
+        std::cout << "KPOINT imported from kpointfile: " << in.kpointfile << std::endl;
+    else if (in.kpoint == 1)
+        std::cout << "KPOINT generated for full BZ with order: " << in.kpointorder << std::endl;
+    else if (in.kpoint == 2)
+        std::cout << "KPOINT generated for Monkhorst-Pack grids with order: " << in.kpointorder << std::endl;
+    else if (in.kpoint == 3)
+        std::cout << "KPOINT generated for Cohen-Chadi grids with order: " << in.kpointorder << std::endl;
+    std::cout << "Number of K-vector grids: " << in.K.size() << std::endl;
+    if (in.debug > 2) {
+        for (const V3 &k : in.K) {
+            eigen_row(std::cout, k.x, k.y, k.z);
+            std::cout << "\n";
+        }
+        std::cout << std::endl;
+    }
+    std::cout << "Number of q-grids: " << in.Q.size() << std::endl;
+    if (in.debug > 2) {
+        for (const V3 &q : in.Q) {
+            eigen_row(std::cout, q.x, q.y, q.z);
+            std::cout << "\n";
+        }
+        std::cout << std::endl;
+    }
+    for (int i = 0; i < 8; i++) std::cout << "" << std::endl;
+}
+
+} // namespace
 
 /* ------------------------------------------------------------------ k grids */
 namespace {
@@ -649,6 +812,62 @@ int pmx_input_system(const pmx_input *pin, pmx_system *s) {
     s->uniform_us = in.uniform_us;
     s->a = in.a;
     s->bz_volume = in.bz_volume;
+    return PMX_OK;
+}
+
+int pmx_print_setup(const pmx_input *pin, int precision) {
+    if (!pin) return pmx::set_error(PMX_EINVAL, "null input");
+    const pmx::Input &in = *reinterpret_cast<const pmx::Input *>(pin);
+    std::time_t now = std::time(nullptr);
+    std::cout << "*********************************\n"
+              << "***PicoMax 0.1 progress output***\n"
+              << "*********************************\n"
+              << std::ctime(&now) << "PicoMax v" << 0.1f << "\n"
+              << "  Running on " << pmx::cpu_model() << "\n"
+              << "  Using " << pmx::num_procs() << " cores\n"
+              << "  Total memory: " << (pmx::meminfo_kb("MemTotal:") / 1000) << " MB\n"
+              << "  Avail memory: " << (pmx::meminfo_kb("MemFree:") / 1000) << " MB\n"
+              << "  Working dir: " << in.wdir << "\n"
+              << "  Output file: " << in.outputfile << ".dat\n"
+              << "---------------------------------\n"
+              << "Parameters:\n"
+              << "  encut = " << in.encut << " eV" << "\n"
+              << "  gcut = " << in.gcut << "\n"
+              << "  nchi = " << in.neps << "\n"
+              << "  nband = " << in.nband << "\n"
+              << "  nfreq = " << (in.has_freq ? in.nfreq : 1) << "\n"
+              << "  dfreq = " << (in.has_freq ? in.dfreq : 0.0) << "\n"
+              << "  epsilon = " << in.epsilon << "\n";
+    if (precision != PMX_PREC_FP64) /* the one added option; absent from the reference's report */
+        std::cout << "  precision = "
+                  << (precision == PMX_PREC_FP32 ? "fp32" : precision == PMX_PREC_BF16X3 ? "bf16x3" : "direct") << "\n";
+    std::cout << "---------------------------------" << std::endl;
+    std::cout << "Generated Q-vector\n"
+              << "  Path: ";
+    for (const std::string &s : in.qpath) std::cout << s << ",";
+    std::cout << "\n"
+              << "  Qnum: ";
+    for (int n : in.qnum) std::cout << n << ",";
+    std::cout << "\n"
+              << "  Total number: " << in.Q.size() << std::endl;
+    std::cout << "Generated G-vectors\n"
+              << "  Number of planewaves: " << in.npw << "\n"
+              << "  Elapsed time: " << in.t_gpt << " s" << std::endl;
+    if (in.kpoint == 0) {
+        std::cout << "Imported K-vectors from kpointfile: " << in.kpointfile << std::endl;
+    } else if (in.kpoint == 1) {
+        std::cout << "Generated K-vectors (Full BZ)\n";
+        std::cout << "  Simulation size in kpoint: " << in.kpointorder << std::endl;
+    } else if (in.kpoint == 2) {
+        std::cout << "Generated K-vectors (Monkhorst-Pack grids)\n";
+        std::cout << "  Simulation size in kpoint: " << in.kpointorder << std::endl;
+    } else if (in.kpoint == 3) {
+        std::cout << "Generated K-vectors\n";
+        std::cout << "  Using Cohen-Chadi grids with order: " << in.kpointorder << std::endl;
+    }
+    std::cout << "  Number of K-vectors: " << in.K.size() << "\n"
+              << "  Elapsed time: " << in.t_kpt << " s" << std::endl;
+    if (in.debug > 1) pmx::print_input(in);
     return PMX_OK;
 }
 

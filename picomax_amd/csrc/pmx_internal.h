@@ -74,6 +74,9 @@ struct Input {
     std::vector<double> G;
     std::vector<V3> K;
     std::vector<double> KW;
+    /* wall seconds of the G table and of the k-point set-up (printed as the
+     * reference's main prints them, src/pmx.cpp:128-160) */
+    double t_gpt = 0.0, t_kpt = 0.0;
     /* flat copies handed out through the C-ABI */
     std::vector<double> tau_flat, Q_flat, K_flat, us_k, us_v, ua_k, ua_v;
 
@@ -90,6 +93,8 @@ struct Input {
     }
 };
 
+/* chi_run prints the reference's per-phase progress lines while on (the CLI) */
+void set_progress(pmx_ctx *c, bool on);
 int input_parse(int argc, const char *const *argv, Input &in);
 int generate_gpt_host(const double *recip, int dim, double gcut, bool rect, std::vector<double> &G);
 int import_kpt(const std::string &path, std::vector<V3> &K, std::vector<double> &KW);

@@ -92,7 +92,10 @@ extern "C" int pmx_run(pmx_ctx *ctx, const pmx_input *pin, int precision) {
     const size_t ne2 = (size_t)in.neps * (in.neps + 1) / 2;
     const size_t n = 9 * (size_t)p.nq * ne2 * in.nfreq;
     std::vector<double> re(n), im(n);
-    if ((rc = (epsxyz ? pmx_chi_xyz : epsll ? pmx_chi_ll : pmx_chi)(ctx, &p, re.data(), im.data()))) return rc;
+    pmx::set_progress(ctx, true);
+    rc = (epsxyz ? pmx_chi_xyz : epsll ? pmx_chi_ll : pmx_chi)(ctx, &p, re.data(), im.data());
+    pmx::set_progress(ctx, false);
+    if (rc) return rc;
     if ((rc = write_doubles(base + ".dat", {re.data(), im.data()}, n))) return rc;
     if ((rc = write_doubles(base + ".gpt", {in.G.data()}, (size_t)3 * in.neps))) return rc;
     std::cout << "  Elapsed time: " << secs(t0, clk::now()) << " s" << std::endl;
