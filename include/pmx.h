@@ -195,6 +195,20 @@ int pmx_kk(pmx_ctx *ctx, int nel, int nfreq, double dfreq, double *ReX, double *
 int pmx_epsinv(pmx_ctx *ctx, int nq, int neps, int nfreq, const double *ReX, const double *ImX,
                double *epsM, double *elf, double *epsinv);
 
+/* pmx_epsinv with no device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  dReX / dImX are
+ * device arrays whose leading [nq][neps(neps+1)/2][nfreq] doubles are the
+ * i = j = 0 block (pmx_chi_device's output qualifies); dEpsM [nq][nfreq]
+ * complex and dElf [nq][nfreq] are device outputs, as pmx_epsinv's epsM /
+ * elf.  The Gauss-Jordan workspace is a fixed 32 MiB device array per GPU,
+ * set up by pmx_create; neps above 1024 (more than 32 MiB for one matrix) is
+ * rejected with PMX_EINVAL.  All calls on one GPU share that workspace:
+ * calls that overlap on one GPU (several host threads, several contexts)
+ * race.  Synchronous on the context's stream. */
+int pmx_epsinv_noalloc(pmx_ctx *ctx, int nq, int neps, int nfreq, const double *dReX, const double *dImX,
+                       double *dEpsM, double *dElf);
+
 /* The MATLAB wrapper's macroscopic dielectric functions
  * (include/matlab_wrapper/+picomax/+eps/macroscopic.m:1-89), from all nine
  * blocks of a chi_LL result (pmx_chi_ll's ReX/ImX for the same nq, Q, neps,

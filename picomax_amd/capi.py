@@ -21,7 +21,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 # every symbol include/pmx.h declares
 EXPORTS = ["pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_set_system",
            "pmx_set_form_factors", "pmx_get_loci", "pmx_ref_energy", "pmx_bands", "pmx_eig_batch", "pmx_eig_batch_multi",
-           "pmx_chi", "pmx_chi_device", "pmx_chi_ll", "pmx_chi_ll_device", "pmx_chi_xyz", "pmx_chi_xyz_device", "pmx_profile", "pmx_profile_report", "pmx_tune_report", "pmx_graphs", "pmx_kk", "pmx_epsinv", "pmx_eps_macro", "pmx_polarization_bands", "pmx_grad_bands", "pmx_grad_chi", "pmx_input_parse", "pmx_input_free",
+           "pmx_chi", "pmx_chi_device", "pmx_chi_ll", "pmx_chi_ll_device", "pmx_chi_xyz", "pmx_chi_xyz_device", "pmx_profile", "pmx_profile_report", "pmx_tune_report", "pmx_graphs", "pmx_kk", "pmx_epsinv", "pmx_epsinv_noalloc", "pmx_eps_macro", "pmx_polarization_bands", "pmx_grad_bands", "pmx_grad_chi", "pmx_input_parse", "pmx_input_free",
            "pmx_input_system", "pmx_input_info", "pmx_print_setup", "pmx_generate_gpt", "pmx_run"]
 
 
@@ -95,6 +95,8 @@ def lib():
     L.pmx_graphs.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long)]
     L.pmx_kk.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp]
     L.pmx_epsinv.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.pmx_epsinv_noalloc.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p]
     L.pmx_eps_macro.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp,
                                 _dp, _dp]
     L.pmx_polarization_bands.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
@@ -296,6 +298,12 @@ class Context:
                                 _p(eM.view(np.float64)), _p(elf),
                                 _p(inv.view(np.float64)) if full else None))
         return (eM, elf, inv) if full else (eM, elf)
+
+    def epsinv_noalloc(self, nq, neps, nfreq, dre_ptr, dim_ptr, deps_ptr, delf_ptr):
+        """pmx_epsinv_noalloc on device pointers (chi block in, epsM / ELF out)."""
+        _check(lib().pmx_epsinv_noalloc(self.h, int(nq), int(neps), int(nfreq), ctypes.c_void_p(dre_ptr),
+                                        ctypes.c_void_p(dim_ptr), ctypes.c_void_p(deps_ptr),
+                                        ctypes.c_void_p(delf_ptr)))
 
     def eps_macro(self, ReX, ImX, Q, nfreq, dfreq):
         """(eps00, epsM, epsM_T), complex [nq][nfreq], of macroscopic.m from a chi_LL result."""

@@ -265,6 +265,11 @@ int pmx_create(int device, pmx_ctx **out) {
         delete c;
         return set_error(PMX_EHIP, "hipStreamCreate failed");
     }
+    if (int rc = pmx::noalloc_init(device)) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
     *out = c;
     return PMX_OK;
 }

@@ -37,9 +37,11 @@
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <iostream>
+#include <mutex>
 
 namespace pmx {
 
@@ -1345,9 +1347,9 @@ __global__ void place_kernel(int nel, int nfreq, int ne2, int nq, int q, const d
  * -qg2[q][m] / w_f^2, w_f = f dfreq (macroscopic.m:70-78); f = 0 then gives NaN.
  * head ([nq][nfreq] complex, or null) replaces element (0, 0) (macroscopic.m:18-20). */
 __global__ void __launch_bounds__(256) epsinv_kernel(int N, int nq, int nfreq, const double *ReX, const double *ImX,
-                                                     const double *qg2, double dfreq, const zd *head, zd *scratch,
-                                                     zd *epsM, double *elf, zd *inv_out) {
-    const int mat = blockIdx.x;
+                                                     const double *qg2, double dfreq, const zd *head, zd *work,
+                                                     zd *epsM, double *elf, zd *inv_out, int mat0) {
+    const int mat = mat0 + blockIdx.x; /* matrix (q, f); workspace slot blockIdx.x */
     const int q = mat / nfreq, f = mat % nfreq;
     const int ne2 = N * (N + 1) / 2;
     const int W = 2 * N;
@@ -1362,7 +1364,7 @@ __global__ void __launch_bounds__(256) epsinv_kernel(int N, int nq, int nfreq, c
         return;
     }
     const double iw2 = qg2 ? 1.0 / ((dfreq * f) * (dfreq * f)) : 0.0;
-    zd *A = scratch + (size_t)blockIdx.x * N * W;
+    zd *A = work + (size_t)blockIdx.x * N * W;
     __shared__ int piv;
     __shared__ double best[256];
     __shared__ int bidx[256];
@@ -1528,6 +1530,37 @@ struct ChiWork {
 };
 
 void chi_work_free(ChiWork *w) { delete w; }
+
+/* Workspace of pmx_epsinv_noalloc: device memory that exists from module load
+ * to process end, one copy per GPU, so that entry point calls no allocator.
+ * Host code never uses this name as an address (on the host it is a stand-in
+ * symbol); it uses the per-device pointer from hipGetSymbolAddress below. */
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) zd scratch[NOALLOC_SCRATCH_BYTES / sizeof(zd)];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex noalloc_mu;
+static zd *noalloc_ptr[NOALLOC_MAX_DEVICES]; /* device address of scratch per device, once resolved */
+
+int noalloc_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return set_error(PMX_EINVAL, "device index beyond the workspace table");
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (noalloc_ptr[device]) return PMX_OK;
+    int cur = -1;
+    PMX_CHECK_HIP(hipGetDevice(&cur));
+    if (cur != device) return set_error(PMX_EHIP, "noalloc_init: the device must be current");
+    void *ptr = nullptr;
+    PMX_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return set_error(PMX_EHIP, "hipGetSymbolAddress returned a null workspace address");
+    noalloc_ptr[device] = reinterpret_cast<zd *>(ptr);
+    return PMX_OK;
+}
+
+static zd *noalloc_workspace(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    return noalloc_ptr[device];
+}
 
 } // namespace pmx
 
@@ -2064,13 +2097,44 @@ extern "C" int pmx_epsinv(pmx_ctx *c, int nq, int neps, int nfreq, const double 
     {
         Prof pf(c, "epsinv", 8.0 * neps * neps * 2.0 * neps * nmat, 0.0);
         hipLaunchKernelGGL(epsinv_kernel, dim3(nmat), dim3(256), 0, st, neps, nq, nfreq, re.p, im.p, nullptr, 0.0,
-                           nullptr, scr.p, em.p, el.p, epsinv ? inv.p : nullptr);
+                           nullptr, scr.p, em.p, el.p, epsinv ? inv.p : nullptr, 0);
     }
     PMX_CHECK_HIP(hipGetLastError());
     PMX_CHECK_HIP(hipMemcpyAsync(epsM, em.p, sizeof(zd) * nmat, hipMemcpyDeviceToHost, st));
     PMX_CHECK_HIP(hipMemcpyAsync(elf, el.p, sizeof(double) * nmat, hipMemcpyDeviceToHost, st));
     if (epsinv)
         PMX_CHECK_HIP(hipMemcpyAsync(epsinv, inv.p, sizeof(zd) * nmat * neps * neps, hipMemcpyDeviceToHost, st));
+    PMX_CHECK_HIP(hipStreamSynchronize(st));
+    return PMX_OK;
+}
+
+/* pmx_epsinv on device arrays with no device allocation: the Gauss-Jordan
+ * workspace is the file-scope scratch array (NOALLOC_SCRATCH_BYTES per GPU),
+ * and the (q, f) matrices run in chunks of as many as fit it.  The kernel
+ * writes every workspace entry before reading it, so nothing is cleared. */
+extern "C" int pmx_epsinv_noalloc(pmx_ctx *c, int nq, int neps, int nfreq, const double *dReX, const double *dImX,
+                                  double *dEpsM, double *dElf) {
+    if (!c || !dReX || !dImX || !dEpsM || !dElf) return set_error(PMX_EINVAL, "null pointer");
+    if (nq <= 0 || neps <= 0 || nfreq <= 0) return set_error(PMX_EINVAL, "bad sizes");
+    const size_t per_mat = (size_t)neps * 2 * neps; /* zd entries of one [eps | I] workspace */
+    const size_t cap = NOALLOC_SCRATCH_BYTES / sizeof(zd);
+    if (per_mat > cap)
+        return set_error(PMX_EINVAL, "pmx_epsinv_noalloc: neps = " + std::to_string(neps) + " needs " +
+                                         std::to_string(per_mat * sizeof(zd)) + " bytes of workspace per matrix, more than its " +
+                                         std::to_string(NOALLOC_SCRATCH_BYTES) + "-byte maximum (neps <= 1024)");
+    if ((long long)nq * nfreq > INT_MAX) return set_error(PMX_EINVAL, "nq * nfreq too large");
+    PMX_CHECK_HIP(hipSetDevice(c->device));
+    zd *ws = noalloc_workspace(c->device);
+    if (!ws) return set_error(PMX_ESTATE, "pmx_epsinv_noalloc: the workspace of this device was not set up");
+    hipStream_t st = c->stream;
+    const int nmat = nq * nfreq;
+    const int chunk = (int)std::min<size_t>(cap / per_mat, (size_t)nmat);
+    for (int m0 = 0; m0 < nmat; m0 += chunk) {
+        const int nm = std::min(chunk, nmat - m0);
+        hipLaunchKernelGGL(epsinv_kernel, dim3(nm), dim3(256), 0, st, neps, nq, nfreq, dReX, dImX, nullptr, 0.0,
+                           nullptr, ws, reinterpret_cast<zd *>(dEpsM), dElf, nullptr, m0);
+        PMX_CHECK_HIP(hipGetLastError());
+    }
     PMX_CHECK_HIP(hipStreamSynchronize(st));
     return PMX_OK;
 }
@@ -2130,9 +2194,9 @@ extern "C" int pmx_eps_macro(pmx_ctx *c, int nq, const double *Q, int neps, int 
     {
         Prof pf(c, "epsinv", 2.0 * 8.0 * neps * neps * 2.0 * neps * nmat, 0.0);
         hipLaunchKernelGGL(epsinv_kernel, dim3(nmat), dim3(256), 0, st, neps, nq, nfreq, reL.p, imL.p, nullptr, 0.0,
-                           hd.p, scr.p, emL.p, el.p, nullptr);
+                           hd.p, scr.p, emL.p, el.p, nullptr, 0);
         hipLaunchKernelGGL(epsinv_kernel, dim3(nmat), dim3(256), 0, st, neps, nq, nfreq, reT.p, imT.p, qgd.p, dfreq,
-                           nullptr, scr.p, emT.p, el.p, nullptr);
+                           nullptr, scr.p, emT.p, el.p, nullptr, 0);
     }
     PMX_CHECK_HIP(hipGetLastError());
     PMX_CHECK_HIP(hipMemcpyAsync(epsM, emL.p, sizeof(zd) * nmat, hipMemcpyDeviceToHost, st));

@@ -165,6 +165,9 @@ int eig_device_multi(int n, pmx_ctx *const *cs, const int *nmat, const double *c
 /* basis size of the solve (NPW spatial or 2 NPW spinor) */
 inline int basis_size(const pmx_ctx *c) { return c->soc ? 2 * c->npw : c->npw; }
 int rebuild_vtable(pmx_ctx *ctx);
+/* resolves the device address of pmx_epsinv_noalloc's workspace on `device`
+ * (which must be current); once per device, from pmx_create */
+int noalloc_init(int device);
 
 /* Workgroup b of a 1-D grid of G is dispatched to XCD b % 8; this maps it to
  * a work index so that each XCD takes one contiguous range of indices, and
