@@ -784,6 +784,32 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, WPE) tridiag_tile_real_kerne
     }
     const bool diag = I == J;
     const int r0 = I * TT, c0 = J * TT;
+    /* Row-sum and reflector rows go in reverse (thread t takes row NR-1-t), so
+     * the rows that stay in the trailing matrix longest sit in wave 0 with the
+     * tiles that do.  A wave has work at step k while one of its tiles is
+     * active (block column J >= (k+1)/TT, J <= Jmax of its first tile) or
+     * one of its rows is not yet eliminated (row NR-1-64w >= k); after that it
+     * only meets the barriers, so retired waves cost no VALU time. */
+    const int rrow = NR - 1 - tid; /* this thread's row-sum / reflector row (< 0: none) */
+    int wave_last;
+    {
+        const int t0 = wave * 64; /* first (largest-J) tile of the wave */
+        int jmax = -1;
+        if (t0 < ntile) {
+            int m = (int)((sqrt(8.0 * t0 + 1.0) - 1.0) * 0.5);
+            while (m * (m + 1) / 2 > t0) m--;
+            while ((m + 1) * (m + 2) / 2 <= t0) m++;
+            jmax = nblk - 1 - m;
+        }
+        const int last_tile = jmax >= 0 ? TT * (jmax + 1) - 2 : -1;
+        const int last_row = NR - 1 - t0; /* the wave's largest row (row r is written until step r) */
+        wave_last = max(last_tile, last_row);
+    }
+    /* the vector rows a lane touches in the per-step loops: r = lane + 64 q
+     * (NR <= 192), with their LDS offsets computed once */
+    int voff[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) voff[q] = tvi(lane + 64 * q);
     const int ib = I * TS, jb = J * TS; /* LDS offsets of blocks I and J of a vector */
     const double Kv[3] = {a.K[3 * mat], a.K[3 * mat + 1], a.K[3 * mat + 2]};
     double *vref = a.vref ? reinterpret_cast<double *>(a.vref) + (size_t)mat * P : nullptr;
@@ -815,11 +841,21 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, WPE) tridiag_tile_real_kerne
         const int cur = (TT % 2 == 0) ? ((s + 1) & 1) : (k & 1); /* k & 1, a constant when TT is even */
         const double *x = cur ? xk1 : xk0;
         double *xn = cur ? xk0 : xk1;
+        if (k > wave_last) { /* retired wave: the step's barriers only */
+            __syncthreads();
+            __syncthreads();
+            __syncthreads();
+            continue;
+        }
         /* Householder vector of column k; every wave forms it in its own copy */
         double s2 = 0.0;
-        for (int r = k + 2 + lane; r < N; r += 64) {
-            const double t = x[tvi(r)];
-            s2 += t * t;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int r = lane + 64 * q;
+            if (r >= k + 2 && r < N) {
+                const double t = x[voff[q]];
+                s2 = fma(t, t, s2);
+            }
         }
         s2 = wave_sum_full(s2);
         const double alpha = x[tvi(k + 1)];
@@ -837,9 +873,13 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, WPE) tridiag_tile_real_kerne
         const bool active = has_tile && J >= cmin && tau != 0.0;
         if (tau != 0.0) {
             /* rows of blocks >= cmin hold column k (rows below are stale) */
-            for (int r = cmin * TT + lane; r < NR; r += 64) {
-                const double xr = x[tvi(r)];
-                vw[tvi(r)] = (r == k + 1) ? 1.0 : ((r > k + 1 && r < N) ? xr * scal : 0.0);
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const int r = lane + 64 * q;
+                if (r >= cmin * TT && r < NR) {
+                    const double xr = x[voff[q]];
+                    vw[voff[q]] = (r == k + 1) ? 1.0 : ((r > k + 1 && r < N) ? xr * scal : 0.0);
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -871,23 +911,21 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, WPE) tridiag_tile_real_kerne
             }
         }
         __syncthreads(); /* (1) partials */
-        if (tau != 0.0 && tid < NR) {
-            const int r = tid, R = r / TT, i = r % TT;
+        if (tau != 0.0 && rrow >= 0) {
+            const int r = rrow, R = r / TT, i = r % TT;
             double p = 0.0;
             if (r > k && r < N) {
                 /* every slot is read (independent loads); those of retired blocks hold stale data */
+                /* the live blocks only (slots of retired blocks hold stale data) */
                 const double *col = part + (size_t)R * TS + i;
-#pragma unroll 6
-                for (int C = 0; C < nblk; C++) {
-                    const double t = col[(size_t)C * nblkp * TS];
-                    p += (C >= cmin) ? t : 0.0;
-                }
+#pragma unroll 4
+                for (int C = cmin; C < nblk; C++) p += col[(size_t)C * nblkp * TS];
                 p *= tau;
             }
             pp[tvi(r)] = p;
         }
         /* reflector of this step (zero when tau = 0: the step is the identity) */
-        if (vref && tid >= k + 2 && tid < N) ring[(k & 7) * NR + tid] = (tau != 0.0) ? vw[tvi(tid)] : 0.0;
+        if (vref && rrow >= k + 2 && rrow < N) ring[(k & 7) * NR + rrow] = (tau != 0.0) ? vw[tvi(rrow)] : 0.0;
         if (tid == 0) {
             dd[k] = x[tvi(k)];
             ee[k] = beta;
@@ -896,7 +934,11 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, WPE) tridiag_tile_real_kerne
         __syncthreads(); /* (2) p */
         if (tau != 0.0) {
             double pv = 0.0;
-            for (int r = k + 1 + lane; r < N; r += 64) pv = fma(pp[tvi(r)], vw[tvi(r)], pv);
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const int r = lane + 64 * q;
+                if (r >= k + 1 && r < N) pv = fma(pp[voff[q]], vw[voff[q]], pv);
+            }
             pv = wave_sum_full(pv);
             const double al = -0.5 * tau * pv;
             if (active) {
@@ -936,13 +978,14 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, WPE) tridiag_tile_real_kerne
             }
         }
         __syncthreads(); /* (3) next column */
-        if (vref && ((k & 7) == 7 || k + 2 == N)) {
-            /* reflector columns k0..k (rows kk+2..N-1 of column kk are contiguous) */
+        if (vref && wave == 0 && ((k & 7) == 7 || k + 2 == N)) {
+            /* reflector columns k0..k (rows kk+2..N-1 of column kk are contiguous);
+             * wave 0 is active at every step */
             const int k0 = k & ~7;
             for (int kk = k0; kk <= k; kk++) {
                 const int len = N - kk - 2;
                 double *dst = vref + pk(kk + 2, kk, N);
-                for (int i = tid; i < len; i += blockDim.x) dst[i] = ring[(kk & 7) * NR + kk + 2 + i];
+                for (int i = lane; i < len; i += 64) dst[i] = ring[(kk & 7) * NR + kk + 2 + i];
             }
         }
     }
@@ -1191,10 +1234,19 @@ __global__ void __launch_bounds__(TILE_MAX_THREADS, WPE) tridiag_tile_cplx_kerne
     }
 }
 
-/* lowest nb eigenvalues of every T by bisection on Sturm counts, one thread
- * per (matrix, index).  A workgroup takes mpw consecutive matrices and
- * stages their d and e^2 in LDS once, so the ~50 counts per eigenvalue read
- * LDS rather than re-streaming T from L2. */
+/* lowest nb eigenvalues of every T, one thread per (matrix, index).  A
+ * workgroup takes mpw consecutive matrices and stages their d and e^2 in LDS
+ * once, so the Sturm passes read LDS rather than re-streaming T from L2.
+ *
+ * Each pass over T at a shift x gives the Sturm count (the number of
+ * eigenvalues below x: negative pivots q_i of the LDL^T factorisation of
+ * T - xI) and, from the same recurrence, s = d/dx log|det(T - xI)| =
+ * sum_i q_i'/q_i with q_i' = -1 + e_{i-1}^2 q_{i-1}'/q_{i-1}^2.  Bisection on
+ * the count runs until the bracket holds eigenvalue j alone (count(lo) = j,
+ * count(hi) = j + 1); from there Newton's step x - 1/s on det(T - xI), kept
+ * inside the bracket (else the midpoint), converges quadratically, and every
+ * probe's count still shrinks the bracket.  It stops when the bracket or the
+ * Newton step falls below 2 eps |x| + 4 pivmin (bisection's own tolerance). */
 __global__ void __launch_bounds__(256) bisect_kernel(int N, int nmat, int nb, int mpw, const double *dS,
                                                      const double *eS, const double *gb, double *lamS, double *evals) {
     extern __shared__ double smem_d[];
@@ -1215,26 +1267,61 @@ __global__ void __launch_bounds__(256) bisect_kernel(int N, int nmat, int nb, in
     const double *d = dl + ml * N, *e2 = e2l + ml * N;
     double lo = gb[4 * mat], hi = gb[4 * mat + 1];
     const double pivmin = gb[4 * mat + 2];
-    for (int it = 0; it < 80; it++) {
-        const double mid = 0.5 * (lo + hi);
-        if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin || mid == lo || mid == hi) break;
+    int clo = 0, chi = N; /* counts below lo and hi (the Gershgorin bounds hold every eigenvalue) */
+    /* 1/q from v_rcp_f64 and one Newton step (a few ulp; the count is that of
+     * a T perturbed at that level) */
+    auto rcp = [](double q) {
+        double r = __builtin_amdgcn_rcp(q);
+        return fma(fma(-q, r, 1.0), r, r);
+    };
+    double x = 0.5 * (lo + hi), lam = x;
+    for (int it = 0; it < 160; it++) {
+        const double tol = 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin;
+        if (hi - lo <= tol) {
+            lam = 0.5 * (lo + hi);
+            break;
+        }
         int cnt = 0;
-        double q = d[0] - mid;
+        double q = d[0] - x;
         if (fabs(q) < pivmin) q = -pivmin;
         cnt += q < 0.0;
+        double r = rcp(q), qp = -1.0, sd = qp * r;
         for (int i = 1; i < N; i++) {
-            /* e^2 / q as e^2 * (1/q), the reciprocal from v_rcp_f64 and one
-             * Newton step (a few ulp; the count is that of a T perturbed at
-             * that level) */
-            double r = __builtin_amdgcn_rcp(q);
-            r = fma(fma(-q, r, 1.0), r, r);
-            q = fma(-e2[i], r, d[i] - mid);
+            const double t = e2[i] * r; /* e_{i-1}^2 / q_{i-1} */
+            q = (d[i] - x) - t;
+            qp = fma(t * r, qp, -1.0);
             if (fabs(q) < pivmin) q = -pivmin;
             cnt += q < 0.0;
+            r = rcp(q);
+            sd = fma(qp, r, sd);
         }
-        if (cnt <= j) lo = mid; else hi = mid;
+        if (cnt <= j) {
+            lo = x;
+            clo = cnt;
+        } else {
+            hi = x;
+            chi = cnt;
+        }
+        const double mid = 0.5 * (lo + hi);
+        double xn = mid;
+        if (clo == j && chi == j + 1) {
+            /* eigenvalue j alone in (lo, hi]: Newton on det(T - xI) */
+            const double step = 1.0 / sd;
+            const double cand = x - step;
+            if (fabs(step) <= 2.0 * DBL_EPSILON * fabs(x) + 4.0 * pivmin && cand >= lo && cand <= hi) {
+                lam = cand;
+                break;
+            }
+            if (cand > lo && cand < hi) xn = cand; /* NaN and inf fail this and bisect */
+        }
+        if (xn == lo || xn == hi) { /* no representable point strictly inside */
+            lam = mid;
+            break;
+        }
+        x = xn;
+        lam = mid;
     }
-    const double lam = (j < N) ? 0.5 * (lo + hi) : 0.0;
+    if (j >= N) lam = 0.0;
     lamS[(size_t)j * nmat + mat] = lam;
     evals[(size_t)mat * nb + j] = lam;
 }
@@ -1282,14 +1369,23 @@ __global__ void invit_kernel(int N, int nmat, int nvec, const double *dS, const 
         double *y = Y + (size_t)jj * N * nmat + mat; /* element i at y[i * nmat] */
         auto Yv = [&](int i) -> double & { return y[(size_t)i * nmat]; };
         const double lm = lam(jj);
-        /* two solves from a deterministic pseudo-random start (generated in
-         * the first forward pass); the growth per solve is 1/|lambda error| */
+        /* up to two solves from a deterministic pseudo-random start b
+         * (generated in the first forward pass); the growth per solve is
+         * 1/|lambda error|.  After the first, x solves (T - lm I) x = b, so the
+         * unit vector x/|x| has residual |b|/|x| exactly (up to the solve's
+         * own rounding); when that is already at the rounding level of T
+         * (4 N eps ||T||, the level the second solve reaches) the vector is
+         * kept.  Cluster members after the head always take both solves
+         * (Gram-Schmidt between them). */
+        double bb = 0.0;
         for (int it = 0; it < 2; it++) {
             /* forward: factor T - lm I with partial pivoting, transform y */
             double cd = D(0) - lm, cdu = (N > 1) ? E(0) : 0.0, cb = it ? Yv(0) : hash_unit(jj + 1, 1);
+            if (!it) bb = cb * cb;
             for (int i = 0; i + 1 < N; i++) {
                 const double ndl = E(i), nd = D(i + 1) - lm, ndu = (i + 2 < N) ? E(i + 1) : 0.0;
                 const double nb_ = it ? Yv(i + 1) : hash_unit(jj + 1, i + 2);
+                if (!it) bb = fma(nb_, nb_, bb);
                 if (fabs(cd) >= fabs(ndl)) {
                     if (cd == 0.0) cd = eps_piv;
                     const double f = ndl / cd;
@@ -1330,7 +1426,8 @@ __global__ void invit_kernel(int N, int nmat, int nvec, const double *dS, const 
                 nn = 0.0;
                 for (int i = 0; i < N; i++) nn += Yv(i) * Yv(i);
             }
-            if (!(nn < 1e280)) { /* rescale by the largest entry first */
+            const bool huge = !(nn < 1e280); /* growth beyond any residual test */
+            if (huge) { /* rescale by the largest entry first */
                 double mx = 0.0;
                 for (int i = 0; i < N; i++) mx = fmax(mx, fabs(Yv(i)));
                 const double sc = 1.0 / mx;
@@ -1341,8 +1438,11 @@ __global__ void invit_kernel(int N, int nmat, int nvec, const double *dS, const 
                     nn += v * v;
                 }
             }
+            const double rtol = 4.0 * N * DBL_EPSILON * tnorm;
+            const bool done = !it && jj == j && (huge || bb <= nn * rtol * rtol);
             const double sc2 = 1.0 / sqrt(nn);
             for (int i = 0; i < N; i++) Yv(i) *= sc2;
+            if (done) break;
         }
     }
 }
