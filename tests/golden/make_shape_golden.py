@@ -12,7 +12,9 @@ broadening, KK):
       df 0.025, Lorentzian 0.05, KK, nband 36; q-path G -> A with one step
       (q = 0 + gammazero, and A); 3 points of the 16x16x12 grid
   c5  wurtzite, 401 PW, NEPS 181, 2000 w at df 0.015, Lorentzian 0.05, KK,
-      nband 64, q = 0.01 [100]; 2 points of the 32x32x20 grid
+      nband 64, q = 0.01 [100]; 2 points of the 32x32x20 grid (not Gamma,
+      where -nband 64 cuts a fourfold level and the reference's chi would
+      depend on Eigen's vectors inside it)
 
 The form factors are the seeded ones of bench.py's c3 / c5 workloads
 (bench.seeded_system).  The outputs (0.25, 0.9 and 4.7 GB) are too large to
@@ -61,7 +63,7 @@ def shape_cases():
         "c3": dict(system=c3, grid=("te", (16, 16, 12), 1.33), pick=[0, 1031, 2062],
                    run=["-nband", "36", "-nvalence", "18", "-refpoint", "0,0,0", "-nfreq", "800",
                         "-dfreq", "0.025", "-delta", "1", "-epsilon", "0.05", "-kk", "1"]),
-        "c5": dict(system=c5, grid=("wz", (32, 32, 20), 1.626), pick=[0, 10241],
+        "c5": dict(system=c5, grid=("wz", (32, 32, 20), 1.626), pick=[1029, 10241],
                    run=["-nband", "64", "-nvalence", "16", "-refpoint", "0,0,0", "-nfreq", "2000",
                         "-dfreq", "0.015", "-delta", "1", "-epsilon", "0.05", "-kk", "1"]),
     }

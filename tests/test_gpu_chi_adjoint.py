@@ -1,6 +1,6 @@
 """Adjoint of chi (pmx_grad_chi through the C-ABI and the
 torch.autograd.Function) against central finite differences of the FP64
-pmx_chi, h = 1e-6 Ry, on Si (inversion centre: the real eigensolver path),
+pmx_chi and of the FP64 oracle's chi, h = 1e-6 Ry, on Si (inversion centre: the real eigensolver path),
 GaAs (complex path) and trigonal Te with spin-orbit maps (the spinor solve
 and its four rho channels), with and without Kramers-Kronig, for Gaussian
 and Lorentzian broadening."""
@@ -119,6 +119,43 @@ def test_chi_adjoint_vs_fd(setup, delta, kk):
     model.set_vg(v0)
     err = np.abs(g - g_fd).max() / np.abs(g_fd).max()
     _say("  |g| %.3e  rel err %.3e" % (np.abs(g_fd).max(), err))
+    assert err < 1e-6
+
+
+@pytest.mark.parametrize("delta,kk", [(1, 1), (0, 0)])
+def test_chi_adjoint_vs_oracle_fd(setup, delta, kk):
+    """The same adjoint against central differences of the FP64 oracle's chi
+    (tests/orc.py: the reference's loops restated on the CPU, pinned to the
+    reference's golden outputs), so the gradient is held to an independent
+    forward rather than to the GPU path it differentiates."""
+    import cli_args
+    S = setup
+    model = S.model
+    args = SYSTEMS[S.name][0]
+    _say("chi adjoint vs oracle FD: %s delta %d kk %d" % (S.name, delta, kk))
+    gRe, gIm = _upstream(S)
+    v0 = model.vg0
+    model.set_vg(v0)
+    g = model.ctx.grad_chi(S.K, S.KW, Q, S.nband, S.off, delta, EPS, NFREQ, DF, gRe, gIm, kk=kk)
+    i_vg = args.index("-vg") + 1
+    keys = args[i_vg].split(",")[:v0.shape[1]]
+
+    def oracle_loss(v):
+        a2 = list(args)
+        a2[i_vg] = ",".join(keys + ["%.17g" % x for x in v.ravel()])
+        s, _ = cli_args.build(a2)
+        re, im = s.chi(S.K, S.KW, Q, S.nband, S.off, delta, EPS, NFREQ, DF, kk)
+        return (gRe.reshape(-1) * re).sum() + (gIm.reshape(-1) * im).sum()
+
+    g_fd = np.zeros_like(v0)
+    for a in range(v0.shape[0]):
+        for sh in range(v0.shape[1]):
+            vp, vm = v0.copy(), v0.copy()
+            vp[a, sh] += H
+            vm[a, sh] -= H
+            g_fd[a, sh] = (oracle_loss(vp) - oracle_loss(vm)) / (2 * H)
+    err = np.abs(g - g_fd).max() / np.abs(g_fd).max()
+    _say("  |g| %.3e  rel err vs oracle %.3e" % (np.abs(g_fd).max(), err))
     assert err < 1e-6
 
 

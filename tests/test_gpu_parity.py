@@ -286,6 +286,34 @@ def test_tridiag_real_forms_against_oracle(case, form, monkeypatch):
         c.close()
 
 
+@pytest.mark.parametrize("nb", [28, 36])
+@pytest.mark.parametrize("form", ["wy_reg", "wy_vl16", "wy_vl10"])
+@pytest.mark.parametrize("case", sorted(SI_REAL))
+def test_backtrans_real_forms_against_oracle(case, form, nb, monkeypatch):
+    """Every real back-transformation candidate (Z in registers, or in LDS with
+    16 or 10 reflector values per thread) gives the oracle's eigenvectors; 14
+    spatial vectors fill one 16-column tile, 18 take two."""
+    monkeypatch.setenv("PMX_FORCE_KERNEL", form)
+    args = SI_REAL[case]
+    _say("backtrans real form %s %s nb %d" % (form, case, nb))
+    c = capi.Context(0)
+    try:
+        inp = capi.Input(args)
+        c.set_system(inp.system(), inp)
+        s, p = cli_args.build(["-switch", "band"] + args)
+        K = np.random.default_rng(7).uniform(-0.6, 0.6, (6, 3))
+        K[0] = 0.0
+        ev, vec = c.eig(K, nb, vectors=True)
+        for k in range(len(K)):
+            H = s.hamiltonian(K[k])
+            assert np.abs(ev[k] - np.linalg.eigvalsh(H)[:nb]).max() < 1e-10
+            V = vec[k].T
+            assert np.abs(H @ V - V * ev[k]).max() < 1e-9
+            assert np.abs(V.conj().T @ V - np.eye(nb)).max() < 1e-10
+    finally:
+        c.close()
+
+
 CPLX_SMALL = {  # complex bases of at most 132: the register-tiled complex kernel applies
     "gaas_59": None,  # from the manifest (complex spatial, 59 PW)
     "si_soc_118": ["-crystal", "zb", "-a", "5.43", "-vg", "3,4,8,11,-0.21,0,0.04,0.08,-0.21,0,0.04,0.08",

@@ -42,3 +42,21 @@ def test_c2_shape_case_is_well_posed(manifest):
         eb = s.bands(Kset, nb + 2, off)
         assert (eb[:, nb] - eb[:, nb - 2]).min() > 1e-3  # spin-degenerate: pairs
         assert np.abs(eb[:, :nb]).min() > 1e-6
+
+
+def test_shape_cases_are_well_posed(manifest):
+    """Every workload-shape fixture: -nband cuts no degenerate multiplet at
+    any of its k or k+q (spin pairs count as one level without spin-orbit
+    maps) and no level sits at the energy zero, so the reference's chi does
+    not depend on Eigen's choice of vectors inside a multiplet."""
+    for name in ("shape_c2", "shape_c3", "shape_c5"):
+        s, p = cli_args.build(manifest[name]["args"])
+        off = s.ref_energy(p["refpoint"], p["nvalence"])
+        nb = p["nband"]
+        pair = not (s.us or s.ua or s.use_uniform)
+        for q in p["Q"]:
+            for Kset in (p["K"], p["K"] + np.asarray(q)):
+                eb = s.bands(Kset, nb + 2, off)
+                gap = eb[:, nb] - eb[:, nb - (2 if pair else 1)]
+                assert gap.min() > 1e-2, name
+                assert np.abs(eb[:, :nb]).min() > 1e-3, name
