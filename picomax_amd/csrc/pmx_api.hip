@@ -235,6 +235,23 @@ std::complex<double> pmx_ctx::vso_chi_at(double qx, double qy, double qz) const 
     return std::complex<double>(ls * PMX_RY2EV * cs, la * PMX_RY2EV * sn);
 }
 
+namespace {
+/* the real-gauge phases and their (cos, sin), which the back-transformation
+ * multiplies onto the eigenvectors */
+int upload_phase(pmx_ctx *c) {
+    const size_t n = c->phase.size();
+    std::vector<pmx::zd> cs(n);
+    for (size_t i = 0; i < n; i++) cs[i] = pmx::zd{std::cos(c->phase[i]), std::sin(c->phase[i])};
+    if (int rc = c->dPhase.ensure(n)) return rc;
+    if (int rc = c->dPhaseCS.ensure(n)) return rc;
+    PMX_CHECK_HIP(hipMemcpyAsync(c->dPhase.p, c->phase.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    PMX_CHECK_HIP(hipMemcpyAsync(c->dPhaseCS.p, cs.data(), sizeof(pmx::zd) * n, hipMemcpyHostToDevice, c->stream));
+    /* cs is a local: finish the copy before it goes */
+    PMX_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return PMX_OK;
+}
+} // namespace
+
 extern "C" {
 
 const char *pmx_last_error(void) { return g_last_error.c_str(); }
@@ -364,14 +381,13 @@ int pmx_set_system(pmx_ctx *c, const pmx_system *s) {
 
     int rc;
     if ((rc = c->dG.ensure(3 * n))) return rc;
-    if ((rc = c->dPhase.ensure(n))) return rc;
     if ((rc = c->dVg.ensure((size_t)na * nv))) return rc;
     if ((rc = c->dShell.ensure((size_t)n * n))) return rc;
     if ((rc = c->dSfac.ensure((size_t)n * n * na))) return rc;
     if ((rc = c->dLoci.ensure((size_t)s->neps * n))) return rc;
     if (c->soc && (rc = c->dVso.ensure((size_t)n * n))) return rc;
     PMX_CHECK_HIP(hipMemcpyAsync(c->dG.p, c->G.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
-    PMX_CHECK_HIP(hipMemcpyAsync(c->dPhase.p, c->phase.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    if ((rc = upload_phase(c))) return rc;
     PMX_CHECK_HIP(hipMemcpyAsync(c->dShell.p, c->shell.data(), sizeof(int) * n * n, hipMemcpyHostToDevice, c->stream));
     PMX_CHECK_HIP(hipMemcpyAsync(c->dSfac.p, c->sfac.data(), sizeof(zd) * n * n * na, hipMemcpyHostToDevice, c->stream));
     if (c->soc)
@@ -453,9 +469,9 @@ int pmx_set_form_factors(pmx_ctx *c, const double *vg) {
     c->vg.assign(vg, vg + (size_t)c->natom * c->nv);
     /* form factors that differ per atom can remove the inversion centre */
     detect_real_gauge(c);
-    PMX_CHECK_HIP(hipMemcpyAsync(c->dPhase.p, c->phase.data(), sizeof(double) * c->npw, hipMemcpyHostToDevice,
-                                 c->stream));
-    int rc = rebuild_vtable(c);
+    int rc = upload_phase(c);
+    if (rc) return rc;
+    rc = rebuild_vtable(c);
     if (rc) return rc;
     PMX_CHECK_HIP(hipStreamSynchronize(c->stream));
     return PMX_OK;

@@ -94,6 +94,7 @@ struct pmx_ctx {
 
     pmx::DBuf<double> dG, dPhase, dVreal, dVg;
     pmx::DBuf<pmx::zd> dVpair, dVso, dSfac;
+    pmx::DBuf<pmx::zd> dPhaseCS; /* [npw] (cos, sin) of the real-gauge phases, for the back-transformation */
     pmx::DBuf<int> dShell, dLoci;
 
     /* per-kernel HIP-event timing (pmx_profile) */

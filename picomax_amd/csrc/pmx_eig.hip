@@ -1457,7 +1457,7 @@ constexpr int BT_MAXV = 8; /* vectors per wave per group */
 template <class T>
 __global__ void __launch_bounds__(EIG_THREADS) backtrans_kernel(int N, int nmat, int nvec, int kc, int gsz,
                                                                  const double *Y, const void *vref_,
-                                                                 const void *tau_, const double *phase, zd *evecs) {
+                                                                 const void *tau_, const zd *phcs, zd *evecs) {
     extern __shared__ double smem_d[];
     T *Z = reinterpret_cast<T *>(smem_d);
     T *Vc = Z + (size_t)gsz * N;           /* [kc][N] staged reflector columns (row index k+1+i) */
@@ -1536,9 +1536,8 @@ __global__ void __launch_bounds__(EIG_THREADS) backtrans_kernel(int N, int nmat,
                 const int jj = idx / N, i = idx % N;
                 zd o;
                 if constexpr (sizeof(T) == sizeof(double)) {
-                    double sn, cs;
-                    sincos(phase[i], &sn, &cs);
-                    o = zd{cs * Z[idx], sn * Z[idx]};
+                    const zd ph = phcs[i];
+                    o = zd{ph.x * Z[idx], ph.y * Z[idx]};
                 } else {
                     o = Z[idx];
                 }
@@ -1570,7 +1569,7 @@ __device__ __forceinline__ dx4 mma64(double a, double b, dx4 c) {
 template <int VL, int WPE>
 __global__ void __launch_bounds__(256, WPE) backtrans_wy_kernel(int N, int nmat, int nvec, int NV, int clo, int chi,
                                                             const double *Y, const double *vref_all,
-                                                            const double *tau_all, const double *phase, const int *vr,
+                                                            const double *tau_all, const zd *phcs, const int *vr,
                                                             zd *evecs) {
     extern __shared__ double smem_d[];
     const int Np = (N + 15) & ~15, ZS = NV + 1;
@@ -1723,11 +1722,178 @@ __global__ void __launch_bounds__(256, WPE) backtrans_wy_kernel(int N, int nmat,
     __syncthreads();
     for (int idx = tid; idx < cnt * N; idx += 256) {
         const int j = idx / N, i = idx % N;
-        double sn, cs;
-        sincos(phase[i], &sn, &cs);
+        const zd ph = phcs[i];
         const double z = Zs[i * ZS + j];
-        evecs[((size_t)mat * nvec + j0 + j) * N + i] = zd{cs * z, sn * z};
+        evecs[((size_t)mat * nvec + j0 + j) * N + i] = zd{ph.x * z, ph.y * z};
     }
+}
+
+/* The WY back-transformation with Z in registers (real path, Np <= 192,
+ * at most 32 wanted vectors).  One workgroup (4 waves) per matrix; row tile
+ * rt (16 rows) of Z belongs to wave rt % 4 and stays in that wave's registers
+ * in the MFMA accumulator layout (lane l, value v: row 16 rt + l/16 + 4v,
+ * column l%16 of a 16-column tile), which is also the B-operand layout of
+ * V^T Z: K step s of tile rt (rows 16 rt + 4s .. +3) takes value s.
+ *
+ * Blocks of 16 reflectors run last to first.  Block b (reflectors 16b ..
+ * 16b+15) is zero above row 16b + 1, so only row tiles rt >= b take part.
+ * Per block, with V_b staged in LDS (three buffers: V_{b-1} is loaded while
+ * block b runs, so loads stay off the critical path):
+ *   1. every wave: its row tiles' share of W = V^T Z and of G = V^T V (MFMA),
+ *      partials to LDS;                                          barrier
+ *   2. wave 3, lanes 0..15: T from the summed G and the taus, forward
+ *      recurrence T_ii = tau_i, T_ji = -tau_i sum_m T_jm G_mi, right-looking
+ *      (each known T_jm is folded into all later sums at once);  barrier
+ *   3. every wave: W = -(T sum_w W_w) (MFMA), then its tiles Z += V W (MFMA).
+ * The phases are applied and the selected vectors stored at the end. */
+template <int NCT>
+__global__ void __launch_bounds__(256, 2) backtrans_wyr_kernel(int N, int nmat, int nvec, int clo, int chi,
+                                                               const double *Y, const double *vref_all,
+                                                               const double *tau_all, const zd *phcs, const int *vr,
+                                                               zd *evecs) {
+    constexpr int MAXRT = 3; /* row tiles per wave: Np <= 192 */
+    extern __shared__ double smem_d[];
+    const int Np = (N + 15) & ~15, nrt = Np / 16;
+    double *Vs = smem_d;                                  /* [3][Np][VS] */
+    double *Wp = Vs + 3 * (size_t)Np * VS;                /* [4 waves][NCT][4][64] */
+    double *Gp = Wp + 4 * NCT * 256;                      /* [4 waves][4][64] */
+    double *Ts = Gp + 4 * 256;                            /* [16][VS] */
+    double *taus = Ts + 16 * VS;                          /* [3][16] */
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, l16 = lane & 15, l4 = lane >> 4;
+    const int mat = xcd_remap(blockIdx.x, gridDim.x);
+    const size_t P = (size_t)N * (N + 1) / 2;
+    const double *vref = vref_all + (size_t)mat * P;
+    const double *tau = tau_all + (size_t)mat * N;
+    const int j0 = vr ? vr[2 * mat] : 0, cnt = vr ? vr[2 * mat + 1] : nvec;
+    if (cnt <= clo || cnt > chi) return;
+    const int nref = N - 1, nbw = (nref + 15) / 16;
+
+    /* Z: own row tiles rt = wave + 4q */
+    dx4 z[MAXRT][NCT];
+#pragma unroll
+    for (int q = 0; q < MAXRT; q++)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = 16 * (wave + 4 * q) + l4 + 4 * v, col = 16 * ct + l16;
+                z[q][ct][v] = (row < N && col < cnt) ? Y[((size_t)(j0 + col) * N + row) * nmat + mat] : 0.0;
+            }
+    /* stage block b into buffer b % 3: rows 16b .. Np-1, columns 16b .. 16b+15
+     * (the implicit unit entry at row k+1, zero above it and past N) */
+    auto stage = [&](int b) {
+        double *V = Vs + (size_t)(b % 3) * Np * VS;
+        const int r0 = 16 * b, nr = Np - r0;
+        for (int idx = tid; idx < nr * 16; idx += 256) {
+            const int kk = idx / nr, r = r0 + idx % nr, k = 16 * b + kk;
+            double v = 0.0;
+            if (k < nref) {
+                if (r == k + 1) v = 1.0;
+                else if (r > k + 1 && r < N) v = vref[pk(r, k, N)];
+            }
+            V[r * VS + kk] = v;
+        }
+        if (tid < 16) taus[(b % 3) * 16 + tid] = (16 * b + tid < nref) ? tau[16 * b + tid] : 0.0;
+    };
+    stage(nbw - 1);
+    if (nbw >= 2) stage(nbw - 2);
+    __syncthreads();
+    for (int b = nbw - 1; b >= 0; b--) {
+        const double *V = Vs + (size_t)(b % 3) * Np * VS;
+        /* 1. partial W = V^T Z and G = V^T V over this wave's tiles rt >= b */
+        dx4 w[NCT], g = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) w[ct] = dx4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < MAXRT; q++) {
+            const int rt = wave + 4 * q;
+            if (rt < b || rt >= nrt) continue;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const double av = V[(16 * rt + 4 * s + l4) * VS + l16];
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++) w[ct] = mma64(av, z[q][ct][s], w[ct]);
+                g = mma64(av, av, g);
+            }
+        }
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) Wp[((wave * NCT + ct) * 4 + v) * 64 + lane] = w[ct][v];
+#pragma unroll
+        for (int v = 0; v < 4; v++) Gp[(wave * 4 + v) * 64 + lane] = g[v];
+        /* the block after next streams into the buffer block b + 1 used */
+        if (b >= 2) stage(b - 2);
+        __syncthreads(); /* (1) partials; block b - 2 staged */
+        /* 2. T of block b (wave 3, lanes 0..15: row j = lane) */
+        if (wave == 3 && lane < 16) {
+            const double *tb = taus + (b % 3) * 16;
+            const int j = lane;
+            double sacc[16], trow[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) sacc[i] = 0.0;
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                /* T_jm is final once the sums over m' < m are in */
+                trow[m] = (m == j) ? tb[m] : ((m > j) ? -tb[m] * sacc[m] : 0.0);
+                /* G_mi, element (m, i) of the 16x16 sum: D layout of lane 16 * (m % 4) + i, value m / 4 */
+#pragma unroll
+                for (int i = m + 1; i < 16; i++) {
+                    const int gl = 16 * (m % 4) + i, gv = m / 4;
+                    const double gmi = (Gp[(0 * 4 + gv) * 64 + gl] + Gp[(1 * 4 + gv) * 64 + gl]) +
+                                       (Gp[(2 * 4 + gv) * 64 + gl] + Gp[(3 * 4 + gv) * 64 + gl]);
+                    sacc[i] = fma(trow[m], gmi, sacc[i]);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 16; i++) Ts[j * VS + i] = trow[i];
+        }
+        __syncthreads(); /* (2) T */
+        /* 3. W = -(T sum_w W_w), then Z += V W on this wave's tiles */
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) {
+            dx4 ws;
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                ws[v] = (Wp[((0 * NCT + ct) * 4 + v) * 64 + lane] + Wp[((1 * NCT + ct) * 4 + v) * 64 + lane]) +
+                        (Wp[((2 * NCT + ct) * 4 + v) * 64 + lane] + Wp[((3 * NCT + ct) * 4 + v) * 64 + lane]);
+            dx4 tw = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < 4; s++) tw = mma64(Ts[l16 * VS + 4 * s + l4], ws[s], tw);
+#pragma unroll
+            for (int v = 0; v < 4; v++) w[ct][v] = -tw[v];
+        }
+#pragma unroll
+        for (int q = 0; q < MAXRT; q++) {
+            const int rt = wave + 4 * q;
+            if (rt < b || rt >= nrt) continue;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const double av = V[(16 * rt + l16) * VS + 4 * s + l4];
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++) z[q][ct] = mma64(av, w[ct][s], z[q][ct]);
+            }
+        }
+        __syncthreads(); /* (3) partials, T and buffer b % 3 free for reuse */
+    }
+    /* phases and store: element (row, col) of the accumulator layout */
+#pragma unroll
+    for (int q = 0; q < MAXRT; q++)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = 16 * (wave + 4 * q) + l4 + 4 * v, col = 16 * ct + l16;
+                if (row < N && col < cnt) {
+                    const zd ph = phcs[row];
+                    evecs[((size_t)mat * nvec + j0 + col) * N + row] = zd{ph.x * z[q][ct][v], ph.y * z[q][ct][v]};
+                }
+            }
+}
+
+size_t wyr_lds_bytes(int N, int nct) {
+    const size_t Np = (N + 15) & ~15;
+    return sizeof(double) * (3 * Np * VS + 4 * (size_t)nct * 256 + 4 * 256 + 16 * VS + 3 * 16);
 }
 
 size_t wy_lds_bytes(int N, int nvec) {
@@ -1900,7 +2066,7 @@ int tridiag_generic(pmx_ctx *c, EigArgs &a, bool real, bool lds_matrix, int grid
 bool wy_usable(pmx_ctx *c, int N, int nvec) { return N >= 2 && N <= 16 * BT_VL_MAX && wy_lds_bytes(N, nvec) <= c->lds_max; }
 
 template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, const double *Y, const void *vref,
-                                        const void *tau, const double *phase, const int *vr, zd *evecs) {
+                                        const void *tau, const zd *phcs, const int *vr, zd *evecs) {
     if (sizeof(T) == sizeof(double) && wy_usable(c, N, nvec)) {
         /* two register budgets: 16 reflector values per thread (any Np <= 256,
          * two waves per SIMD) or 10 (Np <= 160, capped at 168 VGPRs for three
@@ -1919,7 +2085,7 @@ template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, c
                     PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
                 hipLaunchKernelGGL(kern, dim3(g), dim3(256), lds, c->stream, N, nmat, nvec, NV, clo, chi, Y,
                                    reinterpret_cast<const double *>(vref), reinterpret_cast<const double *>(tau),
-                                   phase, vr, evecs);
+                                   phcs, vr, evecs);
                 PMX_CHECK_HIP(hipGetLastError());
                 return 0;
             };
@@ -1930,7 +2096,28 @@ template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, c
             return pass(NVf, -1, nvec);
         };
         const int Np = (N + 15) & ~15;
+        /* Z in registers: 16 (NCT 1) or 32 (NCT 2) columns, same split of the selection */
+        auto launch_reg = [&](int g) -> int {
+            auto pass = [&](auto kern, int nct, int clo, int chi) -> int {
+                const size_t lds = wyr_lds_bytes(N, nct);
+                if (lds > 65536)
+                    PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                hipLaunchKernelGGL(kern, dim3(g), dim3(256), lds, c->stream, N, nmat, nvec, clo, chi, Y,
+                                   reinterpret_cast<const double *>(vref), reinterpret_cast<const double *>(tau),
+                                   phcs, vr, evecs);
+                PMX_CHECK_HIP(hipGetLastError());
+                return 0;
+            };
+            if (NVf <= 16) return pass(backtrans_wyr_kernel<1>, 1, -1, nvec);
+            if (vr) {
+                const int r = pass(backtrans_wyr_kernel<1>, 1, -1, 16);
+                return r ? r : pass(backtrans_wyr_kernel<2>, 2, 16, nvec);
+            }
+            return pass(backtrans_wyr_kernel<2>, 2, -1, nvec);
+        };
         std::vector<TuneCand> cands;
+        if (Np <= 192 && NVf <= 32 && wyr_lds_bytes(N, NVf / 16) <= c->lds_max)
+            cands.push_back({"wy_reg", [&](int g) { return launch_reg(g); }});
         cands.push_back({"wy_vl16", [&](int g) { return launch(backtrans_wy_kernel<16, 1>, g); }});
         if (Np * BW <= 256 * 10)
             cands.push_back({"wy_vl10", [&](int g) { return launch(backtrans_wy_kernel<10, 3>, g); }});
@@ -1950,7 +2137,7 @@ template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, c
         PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int grid = std::min(nmat, c->num_cu * 4);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(EIG_THREADS), lds, c->stream, N, nmat, nvec, kc, gsz, Y, vref, tau,
-                       phase, evecs);
+                       phcs, evecs);
     PMX_CHECK_HIP(hipGetLastError());
     return 0;
 }
@@ -2092,9 +2279,9 @@ static int eig_device_run(pmx_ctx *c, int nmat, const double *dKpts, int nb, int
             }
             {
                 Prof pf(c, real ? "backtrans_real" : "backtrans_complex", nm * (real ? 2.0 : 8.0) * n2 * nvec, 0.0);
-                rc = real ? launch_backtrans<double>(c, N, nm, nvec, Y, vref, tauv, c->dPhase.p, vr,
+                rc = real ? launch_backtrans<double>(c, N, nm, nvec, Y, vref, tauv, c->dPhaseCS.p, vr,
                                                      dEvecs + (size_t)m0 * nvec * N)
-                          : launch_backtrans<zd>(c, N, nm, nvec, Y, vref, tauv, c->dPhase.p, nullptr,
+                          : launch_backtrans<zd>(c, N, nm, nvec, Y, vref, tauv, c->dPhaseCS.p, nullptr,
                                                  dEvecs + (size_t)m0 * nvec * N);
                 if (rc) return rc;
             }
