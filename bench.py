@@ -86,6 +86,37 @@ class Hip:
             raise RuntimeError("hipMemcpy failed")
 
 
+def eps_errors(ctx, re, im, re_d, im_d, nq, nfreq):
+    """eps_M / ELF / Im eps_M of a mixed-path chi block against the FP64 one
+    (relative per frequency, relative to the max, relative to the max)."""
+    eM_m, elf_m = ctx.epsinv(re, im, nq, nfreq)
+    eM_d, elf_d = ctx.epsinv(re_d, im_d, nq, nfreq)
+    sel = (slice(None), slice(1, None))  # w = 0 is identically 1 + 0i in both
+    return {"epsM_rel_err": float(np.max(np.abs(eM_m[sel] - eM_d[sel]) / np.abs(eM_d[sel]))),
+            "elf_rel_err": float(np.max(np.abs(elf_m[sel] - elf_d[sel])) / np.max(np.abs(elf_d[sel]))),
+            "im_epsM_rel_err": float(np.max(np.abs(eM_m[sel].imag - eM_d[sel].imag))
+                                     / np.max(np.abs(eM_d[sel].imag)))}
+
+
+def ksum_errors(ctx, hip, common, dre, dim, n_block, re_d, im_d, nq, nfreq, precisions):
+    """The mixed paths' errors with the FP64 cross-k reduction of the moment
+    partials (the default) and with those partials summed in FP32
+    (PMX_KSUM=fp32), one untimed call each."""
+    out = {}
+    for prec in precisions:
+        for mode in ("fp64", "fp32"):
+            if mode == "fp32":
+                os.environ["PMX_KSUM"] = "fp32"
+            try:
+                ctx.chi_device(ctx.chi_params(*common, prec), dre, dim)
+                hip.sync()
+            finally:
+                os.environ.pop("PMX_KSUM", None)
+            re, im = eps_block(hip, dre, dim, n_block)
+            out["%s_ksum_%s" % (prec, mode)] = eps_errors(ctx, re, im, re_d, im_d, nq, nfreq)
+    return out
+
+
 def run_steps(ctx, hip, params, dre, dim, steps, warmup):
     for _ in range(warmup):
         ctx.chi_device(params, dre, dim)
@@ -305,6 +336,9 @@ def run_config(opt):
         print("%s: FP64 step" % opt.config, file=sys.stderr, flush=True)
         t_64, prof_64 = run_steps(ctx, hip, ctx.chi_params(*common, "fp64"), dre, dim, opt.fp64_steps, 0)
     re_d, im_d = eps_block(hip, dre, dim, nq * ne2 * cfg["nfreq"])
+    ksum = None
+    if t_64 is not None:
+        ksum = ksum_errors(ctx, hip, common, dre, dim, nq * ne2 * cfg["nfreq"], re_d, im_d, nq, cfg["nfreq"], (prec,))
     hip.free(dre)
     hip.free(dim)
     errs = {}
@@ -330,6 +364,7 @@ def run_config(opt):
         "fp64_ms_per_step": round(1e3 * t_64, 3) if t_64 else None,
         "speedup_vs_fp64": round(t_64 / t_mix, 3) if t_64 else None,
         **errs,
+        "cross_k_reduction": ksum,
         "kernels_ms_per_step": {k: round(v["ms"] / opt.steps, 3) for k, v in prof_mix.items()},
         "fp64_kernels_ms_per_step": {k: round(v["ms"] / opt.fp64_steps, 3) for k, v in prof_64.items()},
         "autotune": ctx.tune_report(),
@@ -393,6 +428,8 @@ def main():
     p64 = ctx.chi_params(K, KW, [QVEC], NBAND, off, DELTA, EPS, NFREQ, DFREQ, KK, "fp64")
     t_64, prof_64 = run_steps(ctx, hip, p64, dre, dim, opt.fp64_steps, 1)
     re_d, im_d = eps_block(hip, dre, dim, ne2 * NFREQ)
+    ksum = ksum_errors(ctx, hip, (K, KW, [QVEC], NBAND, off, DELTA, EPS, NFREQ, DFREQ, KK), dre, dim, ne2 * NFREQ,
+                       re_d, im_d, 1, NFREQ, ("bf16x3", "fp32"))
     hip.free(dre)
     hip.free(dim)
 
@@ -447,6 +484,7 @@ def main():
         "epsM_rel_err": eps_err,
         "elf_rel_err": elf_err,
         "im_epsM_rel_err": im_err,
+        "cross_k_reduction": ksum,
         "eigensolves_per_s": round(eigs_per_step / (heev_ms * 1e-3), 1) if heev_ms > 0 else None,
         "eigensolves_per_s_by_N": sweep,
         "kernels_ms_per_step": {k: round(v["ms"] / opt.steps, 3) for k, v in prof_mix.items()},

@@ -39,6 +39,7 @@
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <iostream>
 #include <mutex>
@@ -686,6 +687,7 @@ struct MomArgs2 {
     const int *start;
     void *M;                /* [nbins][NT][npairs][256] complex: FP64 (zd), or FP32 (float2) if m32 */
     int m32;
+    int f64sum;             /* FP32 paths: sum the k-range partials in FP64 (else in FP32) */
 };
 
 template <class T> struct MomFetch {
@@ -778,6 +780,86 @@ template <class T> __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 8 : 1
     }
 }
 
+/* The FP32 path's rank update in the split-BF16 kernel's layout, on FP32
+ * MFMA (16x16x4): each of the 4 waves takes a contiguous quarter of the
+ * bin's rows (k-major transition order) for every order n, so each K entry's
+ * operands are loaded once and the orders scale its B side, B_n = s^n w X_b.
+ * Each quarter accumulates in FP32; the four quarters (four ranges of
+ * k-points) are summed in FP64 in LDS, RCH orders at a time. */
+template <int MAXNT, int OCC> __global__ void __launch_bounds__(256, OCC) moment_f32q_kernel(MomArgs2 a) {
+    typedef MfmaT<float>::V V;
+    typedef MfmaT<float>::C C;
+    constexpr int RCH = MAXNT < 6 ? MAXNT : 6;
+    __shared__ fx4 red[4][64][2 * RCH];
+    /* workgroup l runs on XCD l % 8: bins go round-robin over the XCDs and all
+     * tile pairs of a bin run on the same one, sharing its rows in that L2 */
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int pairi = slot % a.npairs, b = xcd + 8 * (slot / a.npairs);
+    if (b >= a.nbins) return;
+    const int r0 = a.start[b], r1 = a.start[b + 1]; /* an empty bin writes zero moments */
+    int ti, tj;
+    tile_pair(pairi, ti, tj);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int NT = a.NT;
+    const int ia = ti * TILE + (lane & 15), ib = tj * TILE + (lane & 15);
+    const int kq = lane >> 4;
+    const C *X = reinterpret_cast<const C *>(a.X);
+    const int K2 = (a.nch == 1) ? 2 : 4; /* K entries per row: Re/Im, times 2 channels with SOC */
+    const int kend = (r1 - r0) * K2;
+    V accr[MAXNT], acci[MAXNT];
+#pragma unroll
+    for (int j = 0; j < MAXNT; j++) {
+        accr[j] = V{0, 0, 0, 0};
+        acci[j] = V{0, 0, 0, 0};
+    }
+    const int rq = (r1 - r0 + 3) / 4; /* rows per quarter */
+    const int kw0 = min(kend, wave * rq * K2), kw1 = min(kend, (wave + 1) * rq * K2);
+    /* software pipeline: the next K step's operands load while this one's
+     * MFMAs run */
+    MomFetch<float> cur = mom_fetch<float, C>(a, X, r0, kw0 + kq, kw1, K2, ia, ib, 0);
+    for (int kb = kw0; kb < kw1; kb += 4) {
+        const MomFetch<float> nxt = mom_fetch<float, C>(a, X, r0, kb + 4 + kq, kw1, K2, ia, ib, 0);
+        float bre = cur.bre, bim = cur.bim;
+#pragma unroll
+        for (int j = 0; j < MAXNT; j++) {
+            if (j < NT) {
+                accr[j] = MfmaT<float>::mma(cur.av, bre, accr[j]);
+                acci[j] = MfmaT<float>::mma(cur.av, bim, acci[j]);
+                bre *= cur.sv;
+                bim *= cur.sv;
+            }
+        }
+        cur = nxt;
+    }
+#pragma unroll
+    for (int h0 = 0; h0 < MAXNT; h0 += RCH) {
+        if (h0 >= NT) break;
+        const int hn = min(RCH, NT - h0);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RCH; j++)
+            if (h0 + j < MAXNT && j < hn) {
+                red[wave][lane][2 * j] = accr[h0 + j];
+                red[wave][lane][2 * j + 1] = acci[h0 + j];
+            }
+        __syncthreads();
+        for (int sl = wave; sl < 2 * hn; sl += 4) {
+            const int j = h0 + sl / 2, im = sl & 1;
+            const size_t o = (((size_t)b * NT + j) * a.npairs + pairi) * 256;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const float p0 = red[0][lane][sl][v], p1 = red[1][lane][sl][v];
+                const float p2 = red[2][lane][sl][v], p3 = red[3][lane][sl][v];
+                const float t = a.f64sum ? (float)(((double)p0 + (double)p1) + ((double)p2 + (double)p3))
+                                         : (p0 + p1) + (p2 + p3);
+                const int row = MfmaT<float>::drow(lane, v), col = lane & 15;
+                reinterpret_cast<float *>(a.M)[2 * (o + row * TILE + col) + im] = t;
+            }
+        }
+    }
+}
+
+
 /* ------------------------------------------------------------------
  * split-BF16 rank update (PMX_PREC_BF16X3): every operand x = hi + lo with
  * hi = bf16(x), lo = bf16(x - hi); a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi,
@@ -796,15 +878,17 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
     lo = (__bf16)(x - (float)hi);
 }
 
-/* one workgroup per (tile pair, bin); the 4 waves take interleaved K steps
- * of the bin and every order n, so each K element is loaded and its B side
- * split once; the orders scale the A side, A_n = s^n a, re-split per order.
- * The wave partials are summed in LDS at the end. */
+/* one workgroup per (tile pair, bin); each of the 4 waves takes a contiguous
+ * quarter of the bin's rows (which are in k-major transition order) for
+ * every order n, so each K element is loaded and its B side split once; the
+ * orders scale the A side, A_n = s^n a, re-split per order.  Each quarter is
+ * accumulated in FP32 on MFMA; the four quarters (four ranges of k-points)
+ * are summed in FP64 in LDS at the end. */
 constexpr int MOM_MAXNT = 12; /* largest order count of the split-BF16 kernel */
 /* NI items per row: 1 (spatial), 2 (SOC: + and - channel); MAXNT >= NT orders */
 template <int NI, int MAXNT>
 __global__ void __launch_bounds__(256, 3) moment_bf16x3_kernel(MomArgs2 a) {
-    __shared__ fx4 red[2][64][2 * ((MAXNT + 1) / 2)];
+    __shared__ fx4 red[4][64][2 * ((MAXNT + 1) / 2)];
     /* workgroup l runs on XCD l % 8: bins go round-robin over the XCDs and all
      * tile pairs of a bin run on the same one, sharing its rows in that L2 */
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
@@ -857,10 +941,15 @@ __global__ void __launch_bounds__(256, 3) moment_bf16x3_kernel(MomArgs2 a) {
             }
         }
     };
+    /* wave w takes a contiguous quarter of the bin's rows, which are in
+     * transition (k-major) order: its FP32 sums cover one range of k-points,
+     * and the four quarters are added in FP64 below */
+    const int rq = ((r1 - r0 + 63) / 64) * 16; /* rows per quarter, a multiple of 16 */
+    const int kw0 = min(kend, wave * rq * K2), kw1 = min(kend, (wave + 1) * rq * K2);
     Raw cur, nxt;
-    fetch(32 * wave, cur);
-    for (int kb = 32 * wave; kb < kend; kb += 128) {
-        fetch(kb + 128, nxt);
+    fetch(kw0, cur);
+    for (int kb = kw0; kb < kw1; kb += 32) {
+        fetch(kb + 32 < kw1 ? kb + 32 : kend, nxt);
         float av[8], sv[8];
         bf8 brh, brl, bih, bil;
 #pragma unroll
@@ -908,46 +997,36 @@ __global__ void __launch_bounds__(256, 3) moment_bf16x3_kernel(MomArgs2 a) {
         }
         cur = nxt;
     }
-    /* sum the four waves' partials: waves 2, 3 -> 0, 1, then 1 -> 0, in
-     * halves of the orders to bound the LDS */
+    /* the four quarters' FP32 partials summed in FP64 (fixed order), in
+     * halves of the orders to bound the LDS, and stored */
     constexpr int HALF = (MAXNT + 1) / 2;
 #pragma unroll
     for (int h0 = 0; h0 < MAXNT; h0 += HALF) { /* h0 is a constant in each copy: the accumulators stay in registers */
         if (h0 >= NT) break;
         const int hn = min(HALF, NT - h0);
-        for (int stage = 2; stage >= 1; stage >>= 1) {
-            __syncthreads();
-            if (wave >= stage && wave < 2 * stage) {
+        __syncthreads(); /* the previous half's sums are read */
 #pragma unroll
-                for (int j = 0; j < HALF; j++)
-                    if (j < hn) {
-                        red[wave - stage][lane][2 * j] = accr[h0 + j];
-                        red[wave - stage][lane][2 * j + 1] = acci[h0 + j];
-                    }
+        for (int j = 0; j < HALF; j++)
+            if (j < hn) {
+                red[wave][lane][2 * j] = accr[h0 + j];
+                red[wave][lane][2 * j + 1] = acci[h0 + j];
             }
-            __syncthreads();
-            if (wave < stage) {
-#pragma unroll
-                for (int j = 0; j < HALF; j++)
-                    if (j < hn) {
-                        accr[h0 + j] += red[wave][lane][2 * j];
-                        acci[h0 + j] += red[wave][lane][2 * j + 1];
-                    }
-            }
-        }
-    }
-    if (wave != 0) return;
-#pragma unroll
-    for (int j = 0; j < MAXNT; j++) {
-        if (j < NT) {
+        __syncthreads();
+        /* thread (g = wave, L = lane) sums slot pairs s = g, g + 4, ... of lane L */
+        for (int sl = wave; sl < 2 * hn; sl += 4) {
+            const int j = h0 + sl / 2, im = sl & 1;
             const size_t o = (((size_t)b * NT + j) * a.npairs + pairi) * 256;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
+                const double t = a.f64sum ? ((double)red[0][lane][sl][v] + (double)red[1][lane][sl][v]) +
+                                                ((double)red[2][lane][sl][v] + (double)red[3][lane][sl][v])
+                                          : (double)((red[0][lane][sl][v] + red[1][lane][sl][v]) +
+                                                     (red[2][lane][sl][v] + red[3][lane][sl][v]));
                 const int row = 4 * (lane >> 4) + v, col = lane & 15;
                 if (a.m32)
-                    reinterpret_cast<float2 *>(a.M)[o + row * TILE + col] = float2{accr[j][v], acci[j][v]};
+                    reinterpret_cast<float *>(a.M)[2 * (o + row * TILE + col) + im] = (float)t;
                 else
-                    reinterpret_cast<zd *>(a.M)[o + row * TILE + col] = zd{(double)accr[j][v], (double)acci[j][v]};
+                    reinterpret_cast<double *>(a.M)[2 * (o + row * TILE + col) + im] = t;
             }
         }
     }
@@ -1915,8 +1994,12 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                 Prof pf(c, "direct", 8.0 * 9.0 * ne2 * ntr * std::min(2 * wh + 1, p->nfreq) * nacc, 0.0);
                 hipLaunchKernelGGL(direct_kernel, dim3(npairs, nbins), dim3(256), 0, st, da);
             } else {
+                /* PMX_KSUM=fp32 sums the k-range partials of the FP32 paths in FP32
+                 * (to measure what the FP64 cross-k reduction buys) */
+                const char *ks = std::getenv("PMX_KSUM");
+                const int f64sum = !(ks && !std::strcmp(ks, "fp32"));
                 MomArgs2 ma{D, nch, npairs, nbins, NT, (NT + 3) / 4, acc, fp32 ? (const void *)w.Xf.p : (const void *)w.Xd.p,
-                            w.ws.p, w.ss.p, w.start.p, M.p, fp32 ? 1 : 0};
+                            w.ws.p, w.ss.p, w.start.p, M.p, fp32 ? 1 : 0, f64sum};
                 {
                     /* algorithmic: 8 flops per complex MAC for each stored element
                      * (i,j,m,n<=m), order n and rank-1 term (1 per transition, 2 with SOC) */
@@ -1933,8 +2016,12 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                         hipLaunchKernelGGL((moment_bf16x3_kernel<2, 9>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
                     else if (bf)
                         hipLaunchKernelGGL((moment_bf16x3_kernel<2, MOM_MAXNT>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
+                    else if (fp32 && NT <= 9)
+                        hipLaunchKernelGGL((moment_f32q_kernel<9, 3>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
+                    else if (fp32 && NT <= 12)
+                        hipLaunchKernelGGL((moment_f32q_kernel<12, 3>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
                     else if (fp32)
-                        hipLaunchKernelGGL(moment_mfma_kernel<float>, dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
+                        hipLaunchKernelGGL((moment_f32q_kernel<MAX_NT, 2>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
                     else
                         hipLaunchKernelGGL(moment_mfma_kernel<double>, dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
                 }

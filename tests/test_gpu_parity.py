@@ -349,3 +349,4 @@ def test_tridiag_complex_forms_against_oracle(manifest, case, form, monkeypatch)
             assert np.abs(V.conj().T @ V - np.eye(nb)).max() < 1e-10
     finally:
         c.close()
+
