@@ -1737,7 +1737,7 @@ __global__ void __launch_bounds__(256, WPE) backtrans_wy_kernel(int N, int nmat,
  *
  * Blocks of 16 reflectors run last to first.  Block b (reflectors 16b ..
  * 16b+15) is zero above row 16b + 1, so only row tiles rt >= b take part.
- * Per block, with V_b staged in LDS (three buffers: V_{b-1} is loaded while
+ * Per block, with V_b staged in LDS (two buffers: V_{b-1} is loaded while
  * block b runs, so loads stay off the critical path):
  *   1. every wave: its row tiles' share of W = V^T Z and of G = V^T V (MFMA),
  *      partials to LDS;                                          barrier
@@ -1754,11 +1754,12 @@ __global__ void __launch_bounds__(256, 2) backtrans_wyr_kernel(int N, int nmat, 
     constexpr int MAXRT = 3; /* row tiles per wave: Np <= 192 */
     extern __shared__ double smem_d[];
     const int Np = (N + 15) & ~15, nrt = Np / 16;
-    double *Vs = smem_d;                                  /* [3][Np][VS] */
-    double *Wp = Vs + 3 * (size_t)Np * VS;                /* [4 waves][NCT][4][64] */
+    double *Vs = smem_d;                                  /* [2][Np][VS] */
+    double *Wp = Vs + 2 * (size_t)Np * VS;                /* [4 waves][NCT][4][64] */
     double *Gp = Wp + 4 * NCT * 256;                      /* [4 waves][4][64] */
     double *Ts = Gp + 4 * 256;                            /* [16][VS] */
-    double *taus = Ts + 16 * VS;                          /* [3][16] */
+    double *Gs = Ts + 16 * VS;                            /* [16][VS]: the summed G */
+    double *taus = Gs + 16 * VS;                          /* [2][16] */
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, l16 = lane & 15, l4 = lane >> 4;
     const int mat = xcd_remap(blockIdx.x, gridDim.x);
     const size_t P = (size_t)N * (N + 1) / 2;
@@ -1779,27 +1780,29 @@ __global__ void __launch_bounds__(256, 2) backtrans_wyr_kernel(int N, int nmat, 
                 const int row = 16 * (wave + 4 * q) + l4 + 4 * v, col = 16 * ct + l16;
                 z[q][ct][v] = (row < N && col < cnt) ? Y[((size_t)(j0 + col) * N + row) * nmat + mat] : 0.0;
             }
-    /* stage block b into buffer b % 3: rows 16b .. Np-1, columns 16b .. 16b+15
+    /* stage block b into buffer b % 2: rows 16b .. Np-1, columns 16b .. 16b+15
      * (the implicit unit entry at row k+1, zero above it and past N) */
     auto stage = [&](int b) {
-        double *V = Vs + (size_t)(b % 3) * Np * VS;
-        const int r0 = 16 * b, nr = Np - r0;
-        for (int idx = tid; idx < nr * 16; idx += 256) {
-            const int kk = idx / nr, r = r0 + idx % nr, k = 16 * b + kk;
-            double v = 0.0;
-            if (k < nref) {
-                if (r == k + 1) v = 1.0;
-                else if (r > k + 1 && r < N) v = vref[pk(r, k, N)];
+        double *V = Vs + (size_t)(b % 2) * Np * VS;
+        const int r0 = 16 * b;
+        /* column by column: consecutive threads read consecutive packed rows */
+        for (int kk = 0; kk < 16; kk++) {
+            const int k = 16 * b + kk;
+            for (int r = r0 + tid; r < Np; r += 256) {
+                double v = 0.0;
+                if (k < nref) {
+                    if (r == k + 1) v = 1.0;
+                    else if (r > k + 1 && r < N) v = vref[pk(r, k, N)];
+                }
+                V[r * VS + kk] = v;
             }
-            V[r * VS + kk] = v;
         }
-        if (tid < 16) taus[(b % 3) * 16 + tid] = (16 * b + tid < nref) ? tau[16 * b + tid] : 0.0;
+        if (tid < 16) taus[(b % 2) * 16 + tid] = (16 * b + tid < nref) ? tau[16 * b + tid] : 0.0;
     };
     stage(nbw - 1);
-    if (nbw >= 2) stage(nbw - 2);
     __syncthreads();
     for (int b = nbw - 1; b >= 0; b--) {
-        const double *V = Vs + (size_t)(b % 3) * Np * VS;
+        const double *V = Vs + (size_t)(b % 2) * Np * VS;
         /* 1. partial W = V^T Z and G = V^T V over this wave's tiles rt >= b */
         dx4 w[NCT], g = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -1822,31 +1825,34 @@ __global__ void __launch_bounds__(256, 2) backtrans_wyr_kernel(int N, int nmat, 
             for (int v = 0; v < 4; v++) Wp[((wave * NCT + ct) * 4 + v) * 64 + lane] = w[ct][v];
 #pragma unroll
         for (int v = 0; v < 4; v++) Gp[(wave * 4 + v) * 64 + lane] = g[v];
-        /* the block after next streams into the buffer block b + 1 used */
-        if (b >= 2) stage(b - 2);
-        __syncthreads(); /* (1) partials; block b - 2 staged */
-        /* 2. T of block b (wave 3, lanes 0..15: row j = lane) */
-        if (wave == 3 && lane < 16) {
-            const double *tb = taus + (b % 3) * 16;
-            const int j = lane;
-            double sacc[16], trow[16];
+        /* the next block streams into the buffer block b + 1 used */
+        if (b >= 1) stage(b - 1);
+        __syncthreads(); /* (1) partials; block b - 1 staged */
+        /* 2. T of block b on wave 3: the four partials of G summed into Gs
+         * (element (l/16 + 4v, l%16) per lane and value), then lanes 0..15
+         * run the forward recurrence for row j = lane */
+        if (wave == 3) {
 #pragma unroll
-            for (int i = 0; i < 16; i++) sacc[i] = 0.0;
+            for (int v = 0; v < 4; v++)
+                Gs[(l4 + 4 * v) * VS + l16] = (Gp[(0 * 4 + v) * 64 + lane] + Gp[(1 * 4 + v) * 64 + lane]) +
+                                              (Gp[(2 * 4 + v) * 64 + lane] + Gp[(3 * 4 + v) * 64 + lane]);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane < 16) {
+                const double *tb = taus + (b % 2) * 16;
+                const int j = lane;
+                double trow[16];
 #pragma unroll
-            for (int m = 0; m < 16; m++) {
-                /* T_jm is final once the sums over m' < m are in */
-                trow[m] = (m == j) ? tb[m] : ((m > j) ? -tb[m] * sacc[m] : 0.0);
-                /* G_mi, element (m, i) of the 16x16 sum: D layout of lane 16 * (m % 4) + i, value m / 4 */
+                for (int i = 0; i < 16; i++) {
+                    double sacc = 0.0;
 #pragma unroll
-                for (int i = m + 1; i < 16; i++) {
-                    const int gl = 16 * (m % 4) + i, gv = m / 4;
-                    const double gmi = (Gp[(0 * 4 + gv) * 64 + gl] + Gp[(1 * 4 + gv) * 64 + gl]) +
-                                       (Gp[(2 * 4 + gv) * 64 + gl] + Gp[(3 * 4 + gv) * 64 + gl]);
-                    sacc[i] = fma(trow[m], gmi, sacc[i]);
+                    for (int m = 0; m < i; m++) sacc = fma(trow[m], Gs[m * VS + i], sacc);
+                    trow[i] = (i == j) ? tb[i] : ((i > j) ? -tb[i] * sacc : 0.0);
                 }
-            }
 #pragma unroll
-            for (int i = 0; i < 16; i++) Ts[j * VS + i] = trow[i];
+                for (int i = 0; i < 16; i++) Ts[j * VS + i] = trow[i];
+            }
         }
         __syncthreads(); /* (2) T */
         /* 3. W = -(T sum_w W_w), then Z += V W on this wave's tiles */
@@ -1874,7 +1880,7 @@ __global__ void __launch_bounds__(256, 2) backtrans_wyr_kernel(int N, int nmat, 
                 for (int ct = 0; ct < NCT; ct++) z[q][ct] = mma64(av, w[ct][s], z[q][ct]);
             }
         }
-        __syncthreads(); /* (3) partials, T and buffer b % 3 free for reuse */
+        __syncthreads(); /* (3) partials, T and buffer b % 2 free for reuse */
     }
     /* phases and store: element (row, col) of the accumulator layout */
 #pragma unroll
@@ -1893,7 +1899,7 @@ __global__ void __launch_bounds__(256, 2) backtrans_wyr_kernel(int N, int nmat, 
 
 size_t wyr_lds_bytes(int N, int nct) {
     const size_t Np = (N + 15) & ~15;
-    return sizeof(double) * (3 * Np * VS + 4 * (size_t)nct * 256 + 4 * 256 + 16 * VS + 3 * 16);
+    return sizeof(double) * (2 * Np * VS + 4 * (size_t)nct * 256 + 4 * 256 + 2 * 16 * VS + 2 * 16);
 }
 
 size_t wy_lds_bytes(int N, int nvec) {
