@@ -259,7 +259,7 @@ SI_REAL = {  # real symmetric spatial H (inversion centre): the register-tiled k
 }
 
 
-@pytest.mark.parametrize("form", ["tile7", "tile8", "full_lds"])
+@pytest.mark.parametrize("form", ["tile7", "tile7w3", "tile8", "full_lds"])
 @pytest.mark.parametrize("case", sorted(SI_REAL))
 def test_tridiag_real_forms_against_oracle(case, form, monkeypatch):
     """Every real tridiagonalisation candidate the autotuner may pick (7x7 or
