@@ -538,6 +538,168 @@ template <class T> __global__ void __launch_bounds__(256, 4) rho_cart_kernel(Rho
 }
 
 /* ------------------------------------------------------------------
+ * rho for the spinor (SOC) basis on MFMA, the Cartesian form for the orbital
+ * current.  Spinor rows hold (up, down) at 2p, 2p+1.  For one k, rows c of
+ * the conduction rows at k and columns (m, v) of the valence rows at k+q
+ * gathered over l = loci(m, p):
+ *   S_s   = sum_p conj(C_s[c][p]) Q_s[v][l]            s = u, d
+ *   T_s,d = sum_p G_p,d conj(C_s[c][p]) Q_s[v][l]      d = x, y, z
+ *   Qud_i = sum_p conj(C_u[c][p]) Q_d[v][l] wS_mi(p),  Qdu_i with u, d swapped
+ * then up_i = sum_d uc_mid T_u,d + uc_mi.(K + Q/2 + G_m/2) S_u (dn_i with d)
+ * and the four channels A = up + dn, B = Qud + Qdu, C = up - dn,
+ * E = Qud - Qdu of src/pmx_chi.cpp:501-548 (the orbital weight
+ * conj(u_mi).(G_p + Q/2 + G_m/2) is linear in G_p; the spin-current weight
+ * wS is not, so it multiplies the gathered B side).  The valence rows at k+q
+ * are staged in LDS; the conduction rows are read from global memory, where
+ * one k's rows stay in L2, one K step ahead.  One wave per (16 c, 16 (m, v))
+ * tile runs 28 MFMA chains: 14 complex sums, re / im.
+ * ------------------------------------------------------------------ */
+template <class T> struct SocFetch {
+    zd cu, cd, w0, w1, w2;
+    int l;
+};
+
+template <class T>
+__device__ __forceinline__ SocFetch<T> soc_fetch(const zd *ck, const int *lm, const zd *ws, int npw, int kk, int KP,
+                                                 bool crow_ok, bool col_ok) {
+    SocFetch<T> f{{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, -1};
+    if (kk < KP) {
+        const int p = kk >> 1;
+        if (crow_ok) {
+            f.cu = ck[2 * p];
+            f.cd = ck[2 * p + 1];
+        }
+        if (col_ok) {
+            f.l = lm[p];
+            f.w0 = ws[p];
+            f.w1 = ws[npw + p];
+            f.w2 = ws[2 * npw + p];
+        }
+    }
+    return f;
+}
+
+template <class T> __global__ void __launch_bounds__(256, 2) rho_soc_kernel(RhoArgs a) {
+    typedef typename RhoT<T>::V V;
+    typedef typename Cplx<T>::type CT;
+    const int k = blockIdx.x;
+    const int ncs = a.nc[k], nvs = a.nv[k];
+    if (ncs * nvs == 0) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    extern __shared__ double rho_dyn[];
+    CT *scq = reinterpret_cast<CT *>(rho_dyn);                 /* [nvs][N]: valence spinor rows at k+q */
+    T *sg = reinterpret_cast<T *>(scq + (size_t)nvs * a.N); /* [npw][3]: G */
+    const size_t tb = a.tbase[k];
+    const double Kx = a.K[3 * k], Ky = a.K[3 * k + 1], Kz = a.K[3 * k + 2];
+    const zd *vk = a.vk + (size_t)k * a.nb_s * a.N;
+    const zd *vkq = a.vkq + (size_t)k * a.nb_s * a.N;
+    for (int idx = threadIdx.x; idx < nvs * a.N; idx += 256) {
+        const zd v = vkq[(size_t)ket_row(a, idx / a.N) * a.N + idx % a.N];
+        scq[idx] = CT{(T)v.x, (T)v.y};
+    }
+    for (int idx = threadIdx.x; idx < 3 * a.npw; idx += 256) sg[idx] = (T)a.G[idx];
+    __syncthreads();
+    const int ncol = a.neps * nvs; /* (m, v) columns, v fastest */
+    const int nct = (ncs + 15) / 16, njt = (ncol + 15) / 16;
+    const int KP = 2 * a.npw, kq = lane >> 4;
+    for (int job = wave; job < nct * njt; job += 4) {
+        const int ct = job / njt, jt = job % njt;
+        const int crow = ct * 16 + (lane & 15); /* A row */
+        const int col = jt * 16 + (lane & 15);  /* B column */
+        const int m = col / nvs, vv = col % nvs;
+        const bool crow_ok = crow < ncs, col_ok = m < a.neps;
+        const zd *ck = vk + (size_t)bra_row(a, crow_ok ? crow : 0) * a.N;
+        const CT *cq = scq + (size_t)(col_ok ? vv : 0) * a.N;
+        const int *lm = a.loci + (size_t)(col_ok ? m : 0) * a.npw;
+        const zd *ws = a.wS + (size_t)(col_ok ? m : 0) * 3 * a.npw;
+        /* acc[2 j] / acc[2 j + 1]: re / im of sum j; j = 0..3: S_u, T_u,x..z;
+         * 4..7: S_d, T_d,x..z; 8..10: Qud_i; 11..13: Qdu_i */
+        V acc[28];
+#pragma unroll
+        for (int j = 0; j < 28; j++) acc[j] = V{0, 0, 0, 0};
+        SocFetch<T> cur = soc_fetch<T>(ck, lm, ws, a.npw, kq, KP, crow_ok, col_ok);
+        for (int k0 = 0; k0 < KP; k0 += 4) {
+            const int kk = k0 + kq;
+            const SocFetch<T> nxt = soc_fetch<T>(ck, lm, ws, a.npw, kk + 4, KP, crow_ok, col_ok);
+            const int part = kk & 1;
+            /* conj(x) y in the real embedding: A = (Re x, Im x) over the
+             * part, B_re = (Re y, Im y), B_im = (Im y, -Re y) */
+            auto bre = [&](T x, T y) { return part ? y : x; };
+            auto bim = [&](T x, T y) { return part ? -x : y; };
+            T au = 0, ad = 0, gx = 0, gy = 0, gz = 0;
+            T bur = 0, bui = 0, bdr = 0, bdi = 0;
+            T budr[3] = {0, 0, 0}, budi[3] = {0, 0, 0}, bdur[3] = {0, 0, 0}, bdui[3] = {0, 0, 0};
+            if (kk < KP) {
+                const int p = kk >> 1;
+                au = (T)(part ? cur.cu.y : cur.cu.x);
+                ad = (T)(part ? cur.cd.y : cur.cd.x);
+                gx = sg[3 * p];
+                gy = sg[3 * p + 1];
+                gz = sg[3 * p + 2];
+                if (cur.l >= 0) {
+                    const CT qu = cq[2 * cur.l], qd = cq[2 * cur.l + 1];
+                    bur = bre(qu.x, qu.y);
+                    bui = bim(qu.x, qu.y);
+                    bdr = bre(qd.x, qd.y);
+                    bdi = bim(qd.x, qd.y);
+                    const zd wv[3] = {cur.w0, cur.w1, cur.w2};
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        const T wx = (T)wv[i].x, wy = (T)wv[i].y;
+                        const T x1 = qd.x * wx - qd.y * wy, y1 = qd.x * wy + qd.y * wx; /* Q_d wS */
+                        const T x2 = qu.x * wx - qu.y * wy, y2 = qu.x * wy + qu.y * wx; /* Q_u wS */
+                        budr[i] = bre(x1, y1);
+                        budi[i] = bim(x1, y1);
+                        bdur[i] = bre(x2, y2);
+                        bdui[i] = bim(x2, y2);
+                    }
+                }
+            }
+            const T as[8] = {au, au * gx, au * gy, au * gz, ad, ad * gx, ad * gy, ad * gz};
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                acc[2 * j] = RhoT<T>::mma(as[j], j < 4 ? bur : bdr, acc[2 * j]);
+                acc[2 * j + 1] = RhoT<T>::mma(as[j], j < 4 ? bui : bdi, acc[2 * j + 1]);
+            }
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                acc[16 + 2 * i] = RhoT<T>::mma(au, budr[i], acc[16 + 2 * i]);
+                acc[17 + 2 * i] = RhoT<T>::mma(au, budi[i], acc[17 + 2 * i]);
+                acc[22 + 2 * i] = RhoT<T>::mma(ad, bdur[i], acc[22 + 2 * i]);
+                acc[23 + 2 * i] = RhoT<T>::mma(ad, bdui[i], acc[23 + 2 * i]);
+            }
+            cur = nxt;
+        }
+        if (!col_ok) continue;
+        const double *Gm = a.G + 3 * m;
+        const double ox = Kx + 0.5 * (a.Qx + Gm[0]), oy = Ky + 0.5 * (a.Qy + Gm[1]), oz = Kz + 0.5 * (a.Qz + Gm[2]);
+        for (int r = 0; r < 4; r++) {
+            const int c = ct * 16 + RhoT<T>::drow(lane, r);
+            if (c >= ncs) continue;
+            const int row = a.pos[tb + (size_t)c * nvs + vv];
+            if (row < 0) continue;
+            auto z = [&](int j) { return zd{(double)acc[2 * j][r], (double)acc[2 * j + 1][r]}; };
+            for (int i = 0; i < 3; i++) {
+                const zd *u = a.uc + ((size_t)m * 3 + i) * 3;
+                const zd bo{u[0].x * ox + u[1].x * oy + u[2].x * oz, u[0].y * ox + u[1].y * oy + u[2].y * oz};
+                zd sp[2];
+                for (int s = 0; s < 2; s++) {
+                    const zd t0 = cmul(u[0], z(4 * s + 1)), t1 = cmul(u[1], z(4 * s + 2)), t2 = cmul(u[2], z(4 * s + 3));
+                    const zd bs = cmul(bo, z(4 * s));
+                    sp[s] = zd{t0.x + t1.x + t2.x + bs.x, t0.y + t1.y + t2.y + bs.y};
+                }
+                const zd up = sp[0], dn = sp[1], ud = z(8 + i), du = z(11 + i);
+                const int cl = 3 * m + i;
+                put_item(a, row, 0, cl, zd{up.x + dn.x, up.y + dn.y}); /* A */
+                put_item(a, row, 1, cl, zd{ud.x + du.x, ud.y + du.y}); /* B */
+                put_item(a, row, 2, cl, zd{up.x - dn.x, up.y - dn.y}); /* C */
+                put_item(a, row, 3, cl, zd{ud.x - du.x, ud.y - du.y}); /* E */
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------
  * binning: f0 = int((dE - w0)/df) exactly as the reference, bin = f0 + wh;
  * transitions whose window misses [0, nfreq) are dropped.
  * ------------------------------------------------------------------ */
@@ -1963,6 +2125,10 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                 const void *kf = fp32 ? (const void *)rho_mfma_kernel<float> : (const void *)rho_mfma_kernel<double>;
                 PMX_CHECK_HIP(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_r));
             }
+            /* SOC: the valence spinor rows at k+q and the G table */
+            int maxnv = 0;
+            for (int kk = 0; kk < p->nk; kk++) maxnv = std::max(maxnv, nvh[kk]);
+            const size_t lds_soc = sizeof(float2) * (size_t)maxnv * N + sizeof(float) * 3 * (size_t)npw;
             /* the Cartesian form: its LDS is the rows plus the G table */
             const size_t lds_c = lds_r + (fp32 ? sizeof(float) : sizeof(double)) * 3 * (size_t)npw;
             if (rho_lds && fp32 && lds_c <= c->lds_max) {
@@ -1974,7 +2140,12 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                 hipLaunchKernelGGL(rho_mfma_kernel<float>, dim3(p->nk), dim3(256), lds_r, st, ra);
             else if (rho_lds)
                 hipLaunchKernelGGL(rho_mfma_kernel<double>, dim3(p->nk), dim3(256), lds_r, st, ra);
-            else
+            else if (!spatial && !ll && fp32 && lds_soc <= c->lds_max) {
+                if (lds_soc > 65536)
+                    PMX_CHECK_HIP(hipFuncSetAttribute((const void *)rho_soc_kernel<float>,
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_soc));
+                hipLaunchKernelGGL(rho_soc_kernel<float>, dim3(p->nk), dim3(256), lds_soc, st, ra);
+            } else
                 hipLaunchKernelGGL(rho_kernel, dim3(p->nk), dim3(256), 0, st, ra);
         }
         PMX_CHECK_HIP(hipGetLastError());

@@ -156,11 +156,13 @@ def test_spatial_path_equals_spinor_oracle_random(ctx):
         assert err < 1e-8
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("bf16x3", 5e-3), ("direct", 1e-8)])
-def test_chi_precision_modes(manifest, ctx, precision, tol):
+@pytest.mark.parametrize("case,precision,tol", [("si_chi_c1", "fp32", 1e-4), ("si_chi_c1", "bf16x3", 5e-3),
+                                                ("si_chi_c1", "direct", 1e-8), ("te_soc_chi", "fp32", 1e-4),
+                                                ("te_soc_chi", "bf16x3", 5e-3), ("te_usoc_chi", "fp32", 1e-4)])
+def test_chi_precision_modes(manifest, ctx, case, precision, tol):
     """The mixed FP32 path within 1e-4 of each slice's max, the direct FP64
-    windowed sum within 1e-8, on the C1 fixture."""
-    case = "si_chi_c1"
+    windowed sum within 1e-8, on the C1 fixture; with spin-orbit maps (the
+    spinor rho on FP32 MFMA in the mixed modes) on the Te fixtures."""
     args = manifest[case]["args"]
     _say("chi %s on %s" % (precision, case))
     inp = capi.Input(_expand(args))
