@@ -98,13 +98,17 @@ def eps_errors(ctx, re, im, re_d, im_d, nq, nfreq):
                                      / np.max(np.abs(eM_d[sel].imag)))}
 
 
-def ksum_errors(ctx, hip, common, dre, dim, n_block, re_d, im_d, nq, nfreq, precisions):
+def ksum_errors(ctx, hip, common, dre, dim, n_block, re_d, im_d, nq, nfreq, precisions, have=None):
     """The mixed paths' errors with the FP64 cross-k reduction of the moment
     partials (the default) and with those partials summed in FP32
-    (PMX_KSUM=fp32), one untimed call each."""
+    (PMX_KSUM=fp32), one untimed call each; have: {precision: (re, im)} of
+    default-mode results already computed."""
     out = {}
     for prec in precisions:
         for mode in ("fp64", "fp32"):
+            if mode == "fp64" and have and prec in have:
+                out["%s_ksum_%s" % (prec, mode)] = eps_errors(ctx, *have[prec], re_d, im_d, nq, nfreq)
+                continue
             if mode == "fp32":
                 os.environ["PMX_KSUM"] = "fp32"
             try:
@@ -331,6 +335,18 @@ def run_config(opt):
     print("%s: %d k x %d q, mixed (%s) step" % (opt.config, nk, nq, prec), file=sys.stderr, flush=True)
     t_mix, prof_mix = run_steps(ctx, hip, ctx.chi_params(*common, prec), dre, dim, opt.steps, opt.warmup)
     re_m, im_m = eps_block(hip, dre, dim, nq * ne2 * cfg["nfreq"])
+    t_unprof, graphs = None, None
+    if nq >= 3:
+        # one step without the per-kernel events: q steps 3.. replay one captured HIP graph
+        print("%s: unprofiled step (q-step graphs)" % opt.config, file=sys.stderr, flush=True)
+        g0 = ctx.graphs()
+        hip.sync()
+        t0 = time.perf_counter()
+        ctx.chi_device(ctx.chi_params(*common, prec), dre, dim)
+        hip.sync()
+        t_unprof = time.perf_counter() - t0
+        g1 = ctx.graphs()
+        graphs = {"captures": g1["captures"] - g0["captures"], "replays": g1["replays"] - g0["replays"]}
     t_64, prof_64 = None, {}
     if opt.fp64_steps > 0:
         print("%s: FP64 step" % opt.config, file=sys.stderr, flush=True)
@@ -338,7 +354,8 @@ def run_config(opt):
     re_d, im_d = eps_block(hip, dre, dim, nq * ne2 * cfg["nfreq"])
     ksum = None
     if t_64 is not None:
-        ksum = ksum_errors(ctx, hip, common, dre, dim, nq * ne2 * cfg["nfreq"], re_d, im_d, nq, cfg["nfreq"], (prec,))
+        ksum = ksum_errors(ctx, hip, common, dre, dim, nq * ne2 * cfg["nfreq"], re_d, im_d, nq, cfg["nfreq"], (prec,),
+                           have={prec: (re_m, im_m)})
     hip.free(dre)
     hip.free(dim)
     errs = {}
@@ -365,6 +382,8 @@ def run_config(opt):
         "speedup_vs_fp64": round(t_64 / t_mix, 3) if t_64 else None,
         **errs,
         "cross_k_reduction": ksum,
+        "ms_per_step_unprofiled": round(1e3 * t_unprof, 3) if t_unprof else None,
+        "q_step_graphs": graphs,
         "kernels_ms_per_step": {k: round(v["ms"] / opt.steps, 3) for k, v in prof_mix.items()},
         "fp64_kernels_ms_per_step": {k: round(v["ms"] / opt.fp64_steps, 3) for k, v in prof_64.items()},
         "autotune": ctx.tune_report(),

@@ -177,6 +177,17 @@ int pmx_profile_report(pmx_ctx *ctx, char *buf, int len);
  * stats (may be NULL): {graphs held, captures, replays} so far. */
 int pmx_graphs(pmx_ctx *ctx, int enable, long *stats);
 
+/* Whole q steps of a chi q list as HIP graphs (no reference counterpart; the
+ * loop it replaces is src/pmx_chi.cpp:462-658).  With graphs on and
+ * profiling off, a pmx_chi / pmx_chi_device call over three or more q (the
+ * binned moment path) runs its first two q steps eagerly and captures the
+ * third — k+q eigensolve, classification, binning, rho, rank update,
+ * convolution, KK — into one graph that reads its q from device memory;
+ * every later q replays it with no host readback.  A replayed step whose
+ * conduction / valence counts differ from the captured ones is run again
+ * eagerly.  stats: {captures, replays, steps run again} so far. */
+int pmx_qgraph_stats(pmx_ctx *ctx, long *stats);
+
 /* The autotuner's decisions so far on this context, as a JSON list: per
  * kernel shape the candidates' times on the sample it ran and the variant it
  * keeps (no reference counterpart). */

@@ -21,7 +21,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 # every symbol include/pmx.h declares
 EXPORTS = ["pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_set_system",
            "pmx_set_form_factors", "pmx_get_loci", "pmx_ref_energy", "pmx_bands", "pmx_eig_batch", "pmx_eig_batch_multi",
-           "pmx_chi", "pmx_chi_device", "pmx_chi_ll", "pmx_chi_ll_device", "pmx_chi_xyz", "pmx_chi_xyz_device", "pmx_profile", "pmx_profile_report", "pmx_tune_report", "pmx_graphs", "pmx_kk", "pmx_epsinv", "pmx_epsinv_noalloc", "pmx_eps_macro", "pmx_polarization_bands", "pmx_grad_bands", "pmx_grad_chi", "pmx_input_parse", "pmx_input_free",
+           "pmx_chi", "pmx_chi_device", "pmx_chi_ll", "pmx_chi_ll_device", "pmx_chi_xyz", "pmx_chi_xyz_device", "pmx_profile", "pmx_profile_report", "pmx_tune_report", "pmx_graphs", "pmx_qgraph_stats", "pmx_kk", "pmx_epsinv", "pmx_epsinv_noalloc", "pmx_eps_macro", "pmx_polarization_bands", "pmx_grad_bands", "pmx_grad_chi", "pmx_input_parse", "pmx_input_free",
            "pmx_input_system", "pmx_input_info", "pmx_print_setup", "pmx_generate_gpt", "pmx_run"]
 
 
@@ -93,6 +93,7 @@ def lib():
     L.pmx_profile_report.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
     L.pmx_tune_report.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
     L.pmx_graphs.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long)]
+    L.pmx_qgraph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_long)]
     L.pmx_kk.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp]
     L.pmx_epsinv.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
     L.pmx_epsinv_noalloc.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
@@ -273,6 +274,12 @@ class Context:
         st = (ctypes.c_long * 3)()
         _check(lib().pmx_graphs(self.h, int(enable), st))
         return {"graphs": st[0], "captures": st[1], "replays": st[2]}
+
+    def qgraph_stats(self):
+        """Captured chi q steps: captures, replays and steps run again eagerly."""
+        st = (ctypes.c_long * 3)()
+        _check(lib().pmx_qgraph_stats(self.h, st))
+        return {"captures": st[0], "replays": st[1], "reruns": st[2]}
 
     def tune_report(self):
         import json

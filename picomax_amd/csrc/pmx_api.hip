@@ -406,6 +406,14 @@ int pmx_set_system(pmx_ctx *c, const pmx_system *s) {
     return PMX_OK;
 }
 
+int pmx_qgraph_stats(pmx_ctx *c, long *stats) {
+    if (!c || !stats) return set_error(PMX_EINVAL, "null pointer");
+    stats[0] = c->qstep_captures;
+    stats[1] = c->qstep_replays;
+    stats[2] = c->qstep_reruns;
+    return PMX_OK;
+}
+
 int pmx_graphs(pmx_ctx *c, int enable, long *stats) {
     if (!c) return set_error(PMX_EINVAL, "null context");
     PMX_CHECK_HIP(hipSetDevice(c->device));

@@ -121,7 +121,8 @@ struct RhoArgs {
                                  either way one channel with SOC (same-spin products) */
     int swap;                 /* chi_xyz: bra rows = valence at k, ket rows = conduction at k+q */
     const double *G;          /* [npw][3] (rho_cart_kernel) */
-    double Qx, Qy, Qz;        /* q of this step (rho_cart_kernel) */
+    double Qx, Qy, Qz;        /* q of this step (rho_cart_kernel, rho_soc_kernel) */
+    const double *Qd;         /* or, when set, q read from device memory (a captured q step) */
 };
 
 /* solve-basis rows of the bra (index c) and ket (index v) states of a transition */
@@ -517,7 +518,8 @@ template <class T> __global__ void __launch_bounds__(256, 4) rho_cart_kernel(Rho
         }
         if (!col_ok) continue;
         const double *Gm = a.G + 3 * m;
-        const double ox = Kx + 0.5 * (a.Qx + Gm[0]), oy = Ky + 0.5 * (a.Qy + Gm[1]), oz = Kz + 0.5 * (a.Qz + Gm[2]);
+        const double Qx = a.Qd ? a.Qd[0] : a.Qx, Qy = a.Qd ? a.Qd[1] : a.Qy, Qz = a.Qd ? a.Qd[2] : a.Qz;
+        const double ox = Kx + 0.5 * (Qx + Gm[0]), oy = Ky + 0.5 * (Qy + Gm[1]), oz = Kz + 0.5 * (Qz + Gm[2]);
         for (int r = 0; r < 4; r++) {
             const int c = ct * 16 + RhoT<T>::drow(lane, r);
             if (c >= ncs) continue;
@@ -672,7 +674,8 @@ template <class T> __global__ void __launch_bounds__(256, 2) rho_soc_kernel(RhoA
         }
         if (!col_ok) continue;
         const double *Gm = a.G + 3 * m;
-        const double ox = Kx + 0.5 * (a.Qx + Gm[0]), oy = Ky + 0.5 * (a.Qy + Gm[1]), oz = Kz + 0.5 * (a.Qz + Gm[2]);
+        const double Qx = a.Qd ? a.Qd[0] : a.Qx, Qy = a.Qd ? a.Qd[1] : a.Qy, Qz = a.Qd ? a.Qd[2] : a.Qz;
+        const double ox = Kx + 0.5 * (Qx + Gm[0]), oy = Ky + 0.5 * (Qy + Gm[1]), oz = Kz + 0.5 * (Qz + Gm[2]);
         for (int r = 0; r < 4; r++) {
             const int c = ct * 16 + RhoT<T>::drow(lane, r);
             if (c >= ncs) continue;
@@ -1509,7 +1512,9 @@ __global__ void __launch_bounds__(256) f0max_kernel(int nt, const double *dE, do
 constexpr int KK_CT = 4;
 template <class T>
 __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 8 : 1) kk_gemm_kernel(int M, int NF, const double *Re0, const double *Im0,
-                                                      const T *Kk, double *Re, double *Im, int ne2, int nq, int q) {
+                                                      const T *Kk, double *Re, double *Im, int ne2, int nq, int q,
+                                                      const int *qp) {
+    if (qp) q = *qp; /* a captured q step: the q index in device memory */
     typedef typename MfmaT<T>::V V;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     /* workgroup l runs on XCD l % 8: row groups go round-robin over the XCDs
@@ -1569,7 +1574,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 8 : 1) kk_gemm_kernel(in
 }
 
 __global__ void place_kernel(int nel, int nfreq, int ne2, int nq, int q, const double *Re0, const double *Im0,
-                             double *Re, double *Im) {
+                             double *Re, double *Im, const int *qp) {
+    if (qp) q = *qp;
     const size_t total = (size_t)nel * nfreq;
     for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
         const int f = idx % nfreq;
@@ -1768,9 +1774,66 @@ struct ChiWork {
     DBuf<float2> Xf;
     DBuf<float> Ktf, Kkf;
     DBuf<int> nc, nv, tbase, bind, hist, start, cnt, pos, bad;
+    /* captured q steps: the tables of every q, the current q's vector, the
+     * reference counts and [q index, unused, mismatch flag per q] */
+    DBuf<zd> ucAll, wAAll, wSAll;
+    DBuf<double> KQAll, QAll, Qcur;
+    DBuf<int> ncref, nvref, qstate;
 };
 
 void chi_work_free(ChiWork *w) { delete w; }
+
+/* A q step captured into a HIP graph reads its q from device memory:
+ * qsel_kernel copies q's tables (uvec_LT, weights, k+q points, q) from the
+ * arrays of every q into the buffers the step's kernels read and clears the
+ * count flag; qcheck_kernel compares the step's conduction / valence counts
+ * with those the graph was captured for and, when any differs, flags the q
+ * (the host runs it again eagerly) and puts the captured counts back, so
+ * the rest of the step stays within the buffers sized for them;
+ * qnext_kernel advances the q index. */
+struct QSelArgs {
+    int *qstate;
+    int n_uc, n_w, n_kq; /* elements per q */
+    const zd *ucAll, *wAAll, *wSAll;
+    const double *KQAll, *QAll;
+    zd *uc, *wA, *wS;
+    double *KQ, *Qcur;
+    int *bad;
+};
+
+__global__ void qsel_kernel(QSelArgs a) {
+    const size_t q = *a.qstate;
+    const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x, nth = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = tid; i < (size_t)a.n_uc; i += nth) a.uc[i] = a.ucAll[q * a.n_uc + i];
+    for (size_t i = tid; i < (size_t)a.n_w; i += nth) {
+        a.wA[i] = a.wAAll[q * a.n_w + i];
+        if (a.wS) a.wS[i] = a.wSAll[q * a.n_w + i];
+    }
+    for (size_t i = tid; i < (size_t)a.n_kq; i += nth) a.KQ[i] = a.KQAll[q * a.n_kq + i];
+    if (tid < 3) a.Qcur[tid] = a.QAll[3 * q + tid];
+    if (tid == 0) *a.bad = 0;
+}
+
+__global__ void qcheck_kernel(int nk, int *nc, int *nv, const int *ncref, const int *nvref, const int *bad,
+                              int *qstate) {
+    __shared__ int diff;
+    if (threadIdx.x == 0) diff = *bad;
+    __syncthreads();
+    int d = 0;
+    for (int k = threadIdx.x; k < nk; k += blockDim.x) d |= (nc[k] != ncref[k]) | (nv[k] != nvref[k]);
+    if (d) atomicOr(&diff, 1);
+    __syncthreads();
+    if (diff)
+        for (int k = threadIdx.x; k < nk; k += blockDim.x) {
+            nc[k] = ncref[k];
+            nv[k] = nvref[k];
+        }
+    if (threadIdx.x == 0) qstate[2 + *qstate] = diff;
+}
+
+__global__ void qnext_kernel(int *qstate) {
+    if (threadIdx.x == 0) qstate[0] += 1;
+}
 
 /* Workspace of pmx_epsinv_noalloc: device memory that exists from module load
  * to process end, one copy per GPU, so that entry point calls no allocator.
@@ -2005,49 +2068,68 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
     }
     if ((rc = w.Re0.ensure(per_q)) || (rc = w.Im0.ensure(per_q))) return rc;
 
-    for (int q = 0; q < p->nq; q++) {
-        const double Qx = p->Q[3 * q], Qy = p->Q[3 * q + 1], Qz = p->Q[3 * q + 2];
+    /* the counts of the last eager q step (a captured step reuses them) */
+    std::vector<int> nch_h(p->nk), nvh(p->nk), tb(p->nk + 1);
+    int ntr = 0;
+    /* one q step; cap: the form captured into a graph, which reads its q,
+     * tables and k+q points from device memory and does not wait for the
+     * device (qsel / qcheck / qnext kernels above) */
+    auto qstep = [&](int q, bool cap) -> int {
+        int rc;
+        const double Qx = cap ? 0.0 : p->Q[3 * q], Qy = cap ? 0.0 : p->Q[3 * q + 1], Qz = cap ? 0.0 : p->Q[3 * q + 2];
         if (c->progress) std::cout << "  (" << (q + 1) << "/" << p->nq << ")..." << std::endl;
         head("    Solving for overlap integrals...");
-        /* ---- per-q tables: uvec_LT (src/pmx_basis.cpp:830-874), weights */
-        std::vector<zd> uc, wA, wS;
-        q_tables(c, Qx, Qy, Qz, spatial, uc, wA, wS, xyz);
-        if ((rc = upload(st, w.ucd, uc)) || (rc = upload(st, w.wAd, wA))) return rc;
-        if (!spatial && (rc = upload(st, w.wSd, wS))) return rc;
-
-        /* ---- phase B: eigenpairs at k+q */
-        std::vector<double> KQ(3 * (size_t)p->nk);
-        for (int k = 0; k < p->nk; k++) {
-            KQ[3 * k] = Kh[3 * k] + Qx;
-            KQ[3 * k + 1] = Kh[3 * k + 1] + Qy;
-            KQ[3 * k + 2] = Kh[3 * k + 2] + Qz;
-        }
-        if (q > 0) {
-            if ((rc = upload(st, w.KQd, KQ))) return rc;
+        if (cap) {
+            QSelArgs qa{w.qstate.p, neps * 9, neps * 3 * npw, 3 * p->nk, w.ucAll.p, w.wAAll.p,
+                        spatial ? nullptr : w.wSAll.p, w.KQAll.p, w.QAll.p, w.ucd.p, w.wAd.p,
+                        spatial ? nullptr : w.wSd.p, w.KQd.p, w.Qcur.p, w.bad.p};
+            hipLaunchKernelGGL(qsel_kernel, dim3(256), dim3(256), 0, st, qa);
             if ((rc = eig_device(c, p->nk, w.KQd.p, nb_s, nb_s, evkq_p, vkq_p, vsel_kq, p->offset))) return rc;
+        } else {
+            /* ---- per-q tables: uvec_LT (src/pmx_basis.cpp:830-874), weights */
+            std::vector<zd> uc, wA, wS;
+            q_tables(c, Qx, Qy, Qz, spatial, uc, wA, wS, xyz);
+            if ((rc = upload(st, w.ucd, uc)) || (rc = upload(st, w.wAd, wA))) return rc;
+            if (!spatial && (rc = upload(st, w.wSd, wS))) return rc;
+
+            /* ---- phase B: eigenpairs at k+q */
+            std::vector<double> KQ(3 * (size_t)p->nk);
+            for (int k = 0; k < p->nk; k++) {
+                KQ[3 * k] = Kh[3 * k] + Qx;
+                KQ[3 * k + 1] = Kh[3 * k + 1] + Qy;
+                KQ[3 * k + 2] = Kh[3 * k + 2] + Qz;
+            }
+            if (q > 0) {
+                if ((rc = upload(st, w.KQd, KQ))) return rc;
+                if ((rc = eig_device(c, p->nk, w.KQd.p, nb_s, nb_s, evkq_p, vkq_p, vsel_kq, p->offset))) return rc;
+            }
+            PMX_CHECK_HIP(hipMemsetAsync(w.bad.p, 0, sizeof(int), st));
         }
 
         /* ---- classification + transition offsets */
-        PMX_CHECK_HIP(hipMemsetAsync(w.bad.p, 0, sizeof(int), st));
         TransArgs ta{p->nk, nb_s, p->nband, spatial ? 1 : 0, evk_p, evkq_p, p->offset, w.nc.p, w.nv.p, w.bad.p,
                      xyz ? 1 : 0};
         hipLaunchKernelGGL(classify_kernel, dim3((p->nk + 255) / 256), dim3(256), 0, st, ta);
-        std::vector<int> nch_h(p->nk), nvh(p->nk), tb(p->nk + 1);
-        int bad = 0;
-        PMX_CHECK_HIP(hipMemcpyAsync(nch_h.data(), w.nc.p, sizeof(int) * p->nk, hipMemcpyDeviceToHost, st));
-        PMX_CHECK_HIP(hipMemcpyAsync(nvh.data(), w.nv.p, sizeof(int) * p->nk, hipMemcpyDeviceToHost, st));
-        PMX_CHECK_HIP(hipMemcpyAsync(&bad, w.bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
-        PMX_CHECK_HIP(hipStreamSynchronize(st));
-        if (bad)
-            return set_error(PMX_EBANDS, "at some k no level at or below the energy zero lies within -nband, "
-                                         "or at some k+q none above it (the reference's NBAND_C/NBAND_V are undefined)");
-        tb[0] = 0;
-        for (int k = 0; k < p->nk; k++) {
-            int a = spatial ? nch_h[k] / 2 : nch_h[k], b = spatial ? nvh[k] / 2 : nvh[k];
-            tb[k + 1] = tb[k] + a * b;
+        if (cap) {
+            hipLaunchKernelGGL(qcheck_kernel, dim3(1), dim3(256), 0, st, p->nk, w.nc.p, w.nv.p, w.ncref.p, w.nvref.p,
+                               w.bad.p, w.qstate.p);
+        } else {
+            int bad = 0;
+            PMX_CHECK_HIP(hipMemcpyAsync(nch_h.data(), w.nc.p, sizeof(int) * p->nk, hipMemcpyDeviceToHost, st));
+            PMX_CHECK_HIP(hipMemcpyAsync(nvh.data(), w.nv.p, sizeof(int) * p->nk, hipMemcpyDeviceToHost, st));
+            PMX_CHECK_HIP(hipMemcpyAsync(&bad, w.bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            PMX_CHECK_HIP(hipStreamSynchronize(st));
+            if (bad)
+                return set_error(PMX_EBANDS, "at some k no level at or below the energy zero lies within -nband, "
+                                             "or at some k+q none above it (the reference's NBAND_C/NBAND_V are undefined)");
+            tb[0] = 0;
+            for (int k = 0; k < p->nk; k++) {
+                int a = spatial ? nch_h[k] / 2 : nch_h[k], b = spatial ? nvh[k] / 2 : nvh[k];
+                tb[k + 1] = tb[k] + a * b;
+            }
+            ntr = tb[p->nk];
+            if ((rc = upload(st, w.tbase, tb))) return rc;
         }
-        const int ntr = tb[p->nk];
-        if ((rc = upload(st, w.tbase, tb))) return rc;
         const int ntr1 = std::max(ntr, 1);
         if ((rc = w.dEd.ensure(ntr1)) || (rc = w.wtd.ensure(ntr1)) || (rc = w.bind.ensure(ntr1)) ||
             (rc = w.pos.ensure(ntr1)) || (rc = w.ws.ensure(ntr1)) || (rc = w.ss.ensure(ntr1)) ||
@@ -2111,6 +2193,7 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         ra.swap = xyz ? 1 : 0;
         ra.G = c->dG.p;
         ra.Qx = Qx; ra.Qy = Qy; ra.Qz = Qz;
+        ra.Qd = cap ? w.Qcur.p : nullptr;
         if (ntr > 0) {
             /* algorithmic work of the gather-GEMM (src/pmx_chi.cpp:534-544): 8 flops per complex MAC,
              * 3 weighted (+1 plain) sums per (m, p) and transition; bytes = the eigenvector rows read */
@@ -2255,18 +2338,87 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
             const dim3 grid(8 * (((M + 63) / 64 + 7) / 8) * ((p->nfreq + 16 * KK_CT - 1) / (16 * KK_CT)));
             if (fp32)
                 hipLaunchKernelGGL(kk_gemm_kernel<float>, grid, dim3(256), 0, st, M, p->nfreq, w.Re0.p, w.Im0.p,
-                                   w.Kkf.p, dRe, dIm, ne2, p->nq, q);
+                                   w.Kkf.p, dRe, dIm, ne2, p->nq, q, cap ? w.qstate.p : nullptr);
             else
                 hipLaunchKernelGGL(kk_gemm_kernel<double>, grid, dim3(256), 0, st, M, p->nfreq, w.Re0.p, w.Im0.p,
-                                   w.Kkd.p, dRe, dIm, ne2, p->nq, q);
+                                   w.Kkd.p, dRe, dIm, ne2, p->nq, q, cap ? w.qstate.p : nullptr);
             PMX_CHECK_HIP(hipGetLastError());
             if ((rc = elapsed("    "))) return rc;
         } else {
             hipLaunchKernelGGL(place_kernel, dim3(1024), dim3(256), 0, st, 9 * ne2, p->nfreq, ne2, p->nq, q, w.Re0.p,
-                               w.Im0.p, dRe, dIm);
+                               w.Im0.p, dRe, dIm, cap ? w.qstate.p : nullptr);
         }
+        if (cap) hipLaunchKernelGGL(qnext_kernel, dim3(1), dim3(64), 0, st, w.qstate.p);
         PMX_CHECK_HIP(hipGetLastError());
+        return 0;
+    };
+
+    /* A q list of three or more runs its first two steps eagerly (the second
+     * sizes every buffer and settles the eigensolver's choices for the k+q
+     * batch) and captures the third into a HIP graph that every later q
+     * replays: no host readback inside those steps.  A replayed step whose
+     * counts differ from the captured ones is flagged and run again eagerly. */
+    const char *genv = std::getenv("PMX_GRAPHS");
+    const bool qgraph = mode == 0 && !direct && p->nq >= 3 && c->graphs && !c->prof && !c->progress &&
+                        !(genv && genv[0] == '0');
+    int q = 0;
+    for (; q < p->nq && !(qgraph && q == 2); q++)
+        if ((rc = qstep(q, false))) return rc;
+    if (q < p->nq && ntr > 0) {
+        /* the tables of every q, uploaded once */
+        std::vector<zd> ucA, wAA, wSA;
+        std::vector<double> KQA, QA(p->Q, p->Q + 3 * (size_t)p->nq);
+        for (int qq = 0; qq < p->nq; qq++) {
+            std::vector<zd> uc, wA, wS;
+            q_tables(c, p->Q[3 * qq], p->Q[3 * qq + 1], p->Q[3 * qq + 2], spatial, uc, wA, wS, xyz);
+            ucA.insert(ucA.end(), uc.begin(), uc.end());
+            wAA.insert(wAA.end(), wA.begin(), wA.end());
+            wSA.insert(wSA.end(), wS.begin(), wS.end());
+            for (int k = 0; k < p->nk; k++)
+                for (int d = 0; d < 3; d++) KQA.push_back(Kh[3 * k + d] + p->Q[3 * qq + d]);
+        }
+        std::vector<int> qs(2 + p->nq, 0);
+        qs[0] = q;
+        if ((rc = upload(st, w.ucAll, ucA)) || (rc = upload(st, w.wAAll, wAA)) || (rc = upload(st, w.KQAll, KQA)) ||
+            (rc = upload(st, w.QAll, QA)) || (rc = upload(st, w.qstate, qs)) || (rc = w.Qcur.ensure(3)) ||
+            (rc = w.ncref.ensure(p->nk)) || (rc = w.nvref.ensure(p->nk)) ||
+            (!spatial && (rc = upload(st, w.wSAll, wSA))))
+            return rc;
+        PMX_CHECK_HIP(hipMemcpyAsync(w.ncref.p, w.nc.p, sizeof(int) * p->nk, hipMemcpyDeviceToDevice, st));
+        PMX_CHECK_HIP(hipMemcpyAsync(w.nvref.p, w.nv.p, sizeof(int) * p->nk, hipMemcpyDeviceToDevice, st));
+        PMX_CHECK_HIP(hipStreamSynchronize(st));
+        PMX_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        c->capturing = true;
+        const int rc_cap = qstep(q, true);
+        c->capturing = false;
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(st, &graph);
+        hipGraphExec_t exec = nullptr;
+        const bool ok = !rc_cap && ec == hipSuccess && graph &&
+                        hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc_cap) return rc_cap;
+        if (ok) {
+            c->qstep_captures++;
+            for (int qq = q; qq < p->nq; qq++) {
+                PMX_CHECK_HIP(hipGraphLaunch(exec, st));
+                c->qstep_replays++;
+            }
+            PMX_CHECK_HIP(hipMemcpyAsync(qs.data(), w.qstate.p, sizeof(int) * qs.size(), hipMemcpyDeviceToHost, st));
+            PMX_CHECK_HIP(hipStreamSynchronize(st));
+            (void)hipGraphExecDestroy(exec);
+            for (int qq = q; qq < p->nq; qq++)
+                if (qs[2 + qq]) {
+                    c->qstep_reruns++;
+                    if ((rc = qstep(qq, false))) return rc;
+                }
+            q = p->nq;
+        } else {
+            (void)hipGetLastError(); /* the capture did not hold: the rest runs eagerly */
+        }
     }
+    for (; q < p->nq; q++)
+        if ((rc = qstep(q, false))) return rc;
     PMX_CHECK_HIP(hipStreamSynchronize(st));
     return PMX_OK;
 }
@@ -2326,7 +2478,7 @@ extern "C" int pmx_kk(pmx_ctx *c, int nel, int nfreq, double dfreq, double *ReX,
     hipLaunchKernelGGL(kkmat_kernel<double>, dim3(256), dim3(256), 0, st, nfreq, dfreq, Kk.p);
     hipLaunchKernelGGL(kk_gemm_kernel<double>,
                        dim3(8 * (((nel + 63) / 64 + 7) / 8) * ((nfreq + 16 * KK_CT - 1) / (16 * KK_CT))), dim3(256),
-                       0, st, nel, nfreq, r0.p, i0.p, Kk.p, r.p, i.p, nel, 1, 0);
+                       0, st, nel, nfreq, r0.p, i0.p, Kk.p, r.p, i.p, nel, 1, 0, nullptr);
     PMX_CHECK_HIP(hipGetLastError());
     PMX_CHECK_HIP(hipMemcpyAsync(ReX, r.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     PMX_CHECK_HIP(hipMemcpyAsync(ImX, i.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
@@ -3008,7 +3160,7 @@ extern "C" int pmx_grad_chi(pmx_ctx *c, const pmx_chi_params *p, const double *g
                 /* (1) the transposed KK matrix in the forward's GEMM: compact in, compact out */
                 const dim3 grid(8 * (((nel + 63) / 64 + 7) / 8) * ((nfreq + 16 * KK_CT - 1) / (16 * KK_CT)));
                 hipLaunchKernelGGL(kk_gemm_kernel<double>, grid, dim3(256), 0, st, nel, nfreq, gr.p, gi.p, KkT.p,
-                                   gr0.p, gi0.p, ne2, 1, 0);
+                                   gr0.p, gi0.p, ne2, 1, 0, nullptr);
                 g_re = gr0.p;
                 g_im = gi0.p;
             }

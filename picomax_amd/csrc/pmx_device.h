@@ -133,6 +133,7 @@ struct pmx_ctx {
     std::map<std::string, hipGraphExec_t> multi_graphs; /* several contexts' batches, one branch each */
     std::map<std::string, int> eig_seen;
     long graph_captures = 0, graph_replays = 0;
+    long qstep_captures = 0, qstep_replays = 0, qstep_reruns = 0; /* captured chi q steps */
     void graphs_clear() {
         for (auto &kv : eig_graphs) (void)hipGraphExecDestroy(kv.second);
         for (auto &kv : multi_graphs) (void)hipGraphExecDestroy(kv.second);

@@ -1,4 +1,5 @@
-"""Eigensolve batches replayed from HIP graphs give the results of eager runs.
+"""Eigensolve batches and chi q steps replayed from HIP graphs give the
+results of eager runs.
 
 A shape's first batch runs eagerly, the second is captured, later ones replay
 the graph; the buffers' contents (the k list, the form factors) are read at
@@ -6,7 +7,10 @@ replay time, so a replay with new k-points or new form factors must equal an
 eager solve of them.  Eigenvalues and eigenvectors are compared bit for bit
 (the same kernels on the same inputs); chi over a q list, whose k+q solves
 replay from the third q on, is compared with an eager run to 1e-12 of its
-largest value (the moment accumulation's summation order may vary).
+largest value (the moment accumulation's summation order may vary).  On the
+binned path a q list of three or more captures its third q step whole (k+q
+eigensolve through KK) and replays it for every later q; a replay whose
+counts differ from the captured ones is run again eagerly.
 """
 import sys
 
@@ -100,6 +104,71 @@ def test_chi_q_list_replay_equals_eager():
     scale = max(np.abs(out[1][0]).max(), np.abs(out[1][1]).max())
     assert np.abs(out[0][0] - out[1][0]).max() <= 1e-12 * scale
     assert np.abs(out[0][1] - out[1][1]).max() <= 1e-12 * scale
+
+
+def _chi_q_list(args, K, Q, nvalence, nband, precision, refpoint=(0.0, 0.0, 0.0), offset=None):
+    """chi over the q list with graphs on and off; Lorentzian 0.05 eV at df
+    0.02 (the binned moment path, as C2), KK on.  The energy zero is the
+    level nvalence at refpoint, or offset when given."""
+    KW = np.full(len(K), 1.0 / len(K))
+    out, stats = [], []
+    for graphs in (True, False):
+        c = _ctx(args, graphs)
+        try:
+            off = c.ref_energy(refpoint, nvalence) if offset is None else offset
+            out.append(c.chi(K, KW, Q, nband, off, 1, 0.05, 300, 0.02, 1, precision))
+            stats.append(c.qgraph_stats())
+        finally:
+            c.close()
+    scale = max(np.abs(out[1][0]).max(), np.abs(out[1][1]).max())
+    err = max(np.abs(out[0][0] - out[1][0]).max(), np.abs(out[0][1] - out[1][1]).max()) / scale
+    return err, stats
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_chi_q_steps_replay_soc(precision):
+    """Si with spin-orbit maps (118 x 118 spinor H), five q, the energy zero
+    in the gap above level 8 over every k and k+q, so every q has the
+    captured counts: q steps 3..5 replay one captured graph (k+q eigensolve,
+    spinor rho, binning, rank update, KK with the q index in device memory)
+    and none is run again; equal to the eager run to 1e-12 of the largest
+    value."""
+    from picomax_amd import kpoints
+    _say("q-step graph si soc %s" % precision)
+    K = kpoints.gamma_grid("zb", 5)
+    Q = [[0.02 * (i + 1), 0.01 * i, 0.0] for i in range(5)]
+    args = SI[:-2] + ["-gcut", "3.4641016151377544", "-us_so", "3:0.01,8:0.005,11:0.002", "-neps", "4"]
+    c = _ctx(args, False)
+    try:
+        ev = c.eig(np.concatenate([K] + [K + np.array(q) for q in Q]), 16)
+    finally:
+        c.close()
+    top, bottom = ev[:, 7].max(), ev[:, 8].min()
+    _say("  gap above level 8: %.3f .. %.3f eV" % (top, bottom))
+    assert bottom > top
+    err, (sg, se) = _chi_q_list(args, K, Q, 8, 16, precision, offset=0.5 * (top + bottom))
+    _say("  err %.3e, graphs %s" % (err, sg))
+    assert sg["captures"] == 1 and sg["replays"] == 3 and sg["reruns"] == 0, sg
+    assert se["captures"] == 0
+    assert err <= 1e-12
+
+
+def test_chi_q_steps_replay_with_changing_counts():
+    """The energy zero inside the valence bands (Si, the valence top at X, about
+    3 eV below the top at Gamma), so the valence counts at k+q change from q to
+    q: replayed steps whose counts differ from the captured ones are flagged
+    and run again eagerly; the result equals the eager run."""
+    from picomax_amd import kpoints
+    _say("q-step graph with changing counts")
+    K = kpoints.gamma_grid("zb", 5)
+    Q = [[0.05 * (i + 1), 0.03 * i, 0.0] for i in range(5)]
+    args = SI[:-2] + ["-gcut", "3.4641016151377544", "-neps", "4"]
+    err, (sg, se) = _chi_q_list(args, K, Q, 8, 16, "fp64", refpoint=(1.0, 0.0, 0.0))
+    _say("  err %.3e, graphs %s" % (err, sg))
+    assert sg["captures"] == 1 and sg["replays"] == 3, sg
+    assert sg["reruns"] >= 1, sg  # the counts did change: some replay was flagged
+    assert se["captures"] == 0
+    assert err <= 1e-12
 
 
 TE_NOSOC = TE[:TE.index("-us_so")]  # the same Te crystal without spin-orbit: complex spatial H
