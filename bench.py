@@ -339,14 +339,14 @@ def run_config(opt):
     if nq >= 3:
         # one step without the per-kernel events: q steps 3.. replay one captured HIP graph
         print("%s: unprofiled step (q-step graphs)" % opt.config, file=sys.stderr, flush=True)
-        g0 = ctx.graphs()
+        g0 = ctx.qgraph_stats()
         hip.sync()
         t0 = time.perf_counter()
         ctx.chi_device(ctx.chi_params(*common, prec), dre, dim)
         hip.sync()
         t_unprof = time.perf_counter() - t0
-        g1 = ctx.graphs()
-        graphs = {"captures": g1["captures"] - g0["captures"], "replays": g1["replays"] - g0["replays"]}
+        g1 = ctx.qgraph_stats()
+        graphs = {k: g1[k] - g0[k] for k in g1}
     t_64, prof_64 = None, {}
     if opt.fp64_steps > 0:
         print("%s: FP64 step" % opt.config, file=sys.stderr, flush=True)

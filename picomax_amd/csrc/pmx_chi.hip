@@ -717,8 +717,9 @@ template <class T> __global__ void __launch_bounds__(256, 2) rho_soc_kernel(RhoA
 constexpr int SB = 256;
 
 __global__ void __launch_bounds__(SB) bin_count_kernel(int nt, const double *dE, double w0, double df, int wh,
-                                                       int nbins, int *bin, int *cnt) {
+                                                       int nbins, int *bin, int *cnt, const int *ntp) {
     extern __shared__ int hcount[];
+    if (ntp) nt = *ntp; /* a captured q step: the count from the device transition table */
     for (int i = threadIdx.x; i < nbins; i += SB) hcount[i] = 0;
     __syncthreads();
     const int t = blockIdx.x * SB + threadIdx.x;
@@ -784,8 +785,9 @@ __global__ void __launch_bounds__(256) bin_start_kernel(int nbins, const int *to
 __global__ void __launch_bounds__(SB) bin_place_kernel(int nt, int nbins, const int *bin, const int *cnt,
                                                        const int *start, const double *dE, const double *wt,
                                                        double w0, double df, int *pos, double *ws, double *ss,
-                                                       double *es) {
+                                                       double *es, const int *ntp) {
     __shared__ int bl[SB];
+    if (ntp) nt = *ntp;
     const int t = blockIdx.x * SB + threadIdx.x;
     const int b = t < nt ? bin[t] : -1;
     bl[threadIdx.x] = b;
@@ -1778,7 +1780,7 @@ struct ChiWork {
      * reference counts and [q index, unused, mismatch flag per q] */
     DBuf<zd> ucAll, wAAll, wSAll;
     DBuf<double> KQAll, QAll, Qcur;
-    DBuf<int> ncref, nvref, qstate;
+    DBuf<int> qstate;
 };
 
 void chi_work_free(ChiWork *w) { delete w; }
@@ -1786,11 +1788,11 @@ void chi_work_free(ChiWork *w) { delete w; }
 /* A q step captured into a HIP graph reads its q from device memory:
  * qsel_kernel copies q's tables (uvec_LT, weights, k+q points, q) from the
  * arrays of every q into the buffers the step's kernels read and clears the
- * count flag; qcheck_kernel compares the step's conduction / valence counts
- * with those the graph was captured for and, when any differs, flags the q
- * (the host runs it again eagerly) and puts the captured counts back, so
- * the rest of the step stays within the buffers sized for them;
- * qnext_kernel advances the q index. */
+ * count flag; tcap_kernel builds the transition table's offsets (tbase, and
+ * the total in tbase[nk]) from the step's conduction / valence counts and,
+ * when a count or the total exceeds the capacity the buffers were sized for,
+ * flags the q (the host runs it again eagerly) and zeroes the counts, so the
+ * rest of the step does nothing; qnext_kernel advances the q index. */
 struct QSelArgs {
     int *qstate;
     int n_uc, n_w, n_kq; /* elements per q */
@@ -1814,21 +1816,48 @@ __global__ void qsel_kernel(QSelArgs a) {
     if (tid == 0) *a.bad = 0;
 }
 
-__global__ void qcheck_kernel(int nk, int *nc, int *nv, const int *ncref, const int *nvref, const int *bad,
-                              int *qstate) {
-    __shared__ int diff;
-    if (threadIdx.x == 0) diff = *bad;
+__global__ void __launch_bounds__(256) tcap_kernel(int nk, int spatial, int *nc, int *nv, int nc_cap, int nv_cap,
+                                                  int ntr_cap, const int *bad, int *tbase, int *qstate) {
+    __shared__ int part[256];
+    __shared__ int flag;
+    const int tid = threadIdx.x;
+    if (tid == 0) flag = *bad;
     __syncthreads();
-    int d = 0;
-    for (int k = threadIdx.x; k < nk; k += blockDim.x) d |= (nc[k] != ncref[k]) | (nv[k] != nvref[k]);
-    if (d) atomicOr(&diff, 1);
+    const int per = (nk + 255) / 256, k0 = min(nk, tid * per), k1 = min(nk, k0 + per);
+    int sum = 0, over = 0;
+    for (int k = k0; k < k1; k++) {
+        const int a = nc[k], b = nv[k];
+        over |= (a > nc_cap) | (b > nv_cap);
+        sum += (spatial ? a / 2 : a) * (spatial ? b / 2 : b);
+    }
+    part[tid] = sum;
+    if (over) atomicOr(&flag, 1);
     __syncthreads();
-    if (diff)
-        for (int k = threadIdx.x; k < nk; k += blockDim.x) {
-            nc[k] = ncref[k];
-            nv[k] = nvref[k];
+    if (tid == 0) {
+        int run = 0;
+        for (int i = 0; i < 256; i++) {
+            const int x = part[i];
+            part[i] = run;
+            run += x;
         }
-    if (threadIdx.x == 0) qstate[2 + *qstate] = diff;
+        if (run > ntr_cap) flag = 1;
+        tbase[nk] = flag ? 0 : run;
+    }
+    __syncthreads();
+    if (flag) {
+        for (int k = k0; k < k1; k++) {
+            nc[k] = 0;
+            nv[k] = 0;
+            tbase[k] = 0;
+        }
+    } else {
+        int run = part[tid];
+        for (int k = k0; k < k1; k++) {
+            tbase[k] = run;
+            run += (spatial ? nc[k] / 2 : nc[k]) * (spatial ? nv[k] / 2 : nv[k]);
+        }
+    }
+    if (tid == 0) qstate[2 + qstate[0]] = flag;
 }
 
 __global__ void qnext_kernel(int *qstate) {
@@ -2068,12 +2097,14 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
     }
     if ((rc = w.Re0.ensure(per_q)) || (rc = w.Im0.ensure(per_q))) return rc;
 
-    /* the counts of the last eager q step (a captured step reuses them) */
+    /* the counts of the last eager q step; the largest seen, from which a
+     * captured step's capacities are set */
     std::vector<int> nch_h(p->nk), nvh(p->nk), tb(p->nk + 1);
-    int ntr = 0;
+    int ntr = 0, ntr_max = 0, nc_max = 0, nv_max = 0;
+    int ntr_cap = 0, nc_cap = 0, nv_cap = 0;
     /* one q step; cap: the form captured into a graph, which reads its q,
      * tables and k+q points from device memory and does not wait for the
-     * device (qsel / qcheck / qnext kernels above) */
+     * device (qsel / tcap / qnext kernels above) */
     auto qstep = [&](int q, bool cap) -> int {
         int rc;
         const double Qx = cap ? 0.0 : p->Q[3 * q], Qy = cap ? 0.0 : p->Q[3 * q + 1], Qz = cap ? 0.0 : p->Q[3 * q + 2];
@@ -2111,8 +2142,8 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                      xyz ? 1 : 0};
         hipLaunchKernelGGL(classify_kernel, dim3((p->nk + 255) / 256), dim3(256), 0, st, ta);
         if (cap) {
-            hipLaunchKernelGGL(qcheck_kernel, dim3(1), dim3(256), 0, st, p->nk, w.nc.p, w.nv.p, w.ncref.p, w.nvref.p,
-                               w.bad.p, w.qstate.p);
+            hipLaunchKernelGGL(tcap_kernel, dim3(1), dim3(256), 0, st, p->nk, spatial ? 1 : 0, w.nc.p, w.nv.p, nc_cap,
+                               nv_cap, ntr_cap, w.bad.p, w.tbase.p, w.qstate.p);
         } else {
             int bad = 0;
             PMX_CHECK_HIP(hipMemcpyAsync(nch_h.data(), w.nc.p, sizeof(int) * p->nk, hipMemcpyDeviceToHost, st));
@@ -2129,8 +2160,16 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
             }
             ntr = tb[p->nk];
             if ((rc = upload(st, w.tbase, tb))) return rc;
+            ntr_max = std::max(ntr_max, ntr);
+            for (int k = 0; k < p->nk; k++) {
+                nc_max = std::max(nc_max, nch_h[k]);
+                nv_max = std::max(nv_max, nvh[k]);
+            }
         }
-        const int ntr1 = std::max(ntr, 1);
+        /* sizes: the step's transitions, or a captured step's capacity */
+        const int ntr_sz = cap ? ntr_cap : ntr;
+        const int *ntp = cap ? w.tbase.p + p->nk : nullptr;
+        const int ntr1 = std::max(ntr_sz, 1);
         if ((rc = w.dEd.ensure(ntr1)) || (rc = w.wtd.ensure(ntr1)) || (rc = w.bind.ensure(ntr1)) ||
             (rc = w.pos.ensure(ntr1)) || (rc = w.ws.ensure(ntr1)) || (rc = w.ss.ensure(ntr1)) ||
             (rc = w.es.ensure(ntr1)) || (rc = w.hist.ensure(nbins)) || (rc = w.start.ensure(nbins + 1)))
@@ -2159,15 +2198,15 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         /* ---- bin by f0 (stable counting sort on the device); rows of a bin become contiguous */
         if (sizeof(int) * (size_t)nbins > 65536)
             return set_error(PMX_EINVAL, "more than 16384 frequency bins (transition energies / dfreq); raise -dfreq");
-        const int nchunk = std::max((ntr + SB - 1) / SB, 1);
+        const int nchunk = std::max((ntr_sz + SB - 1) / SB, 1);
         if ((rc = w.cnt.ensure((size_t)nchunk * nbins))) return rc;
-        if (ntr > 0) {
+        if (ntr_sz > 0) {
             hipLaunchKernelGGL(bin_count_kernel, dim3(nchunk), dim3(SB), sizeof(int) * nbins, st, ntr, w.dEd.p, w0, df,
-                               off_b, nbins, w.bind.p, w.cnt.p);
+                               off_b, nbins, w.bind.p, w.cnt.p, ntp);
             hipLaunchKernelGGL(bin_prefix_kernel, dim3(nbins), dim3(256), 0, st, nchunk, nbins, w.cnt.p, w.hist.p);
             hipLaunchKernelGGL(bin_start_kernel, dim3(1), dim3(256), 0, st, nbins, w.hist.p, w.start.p);
             hipLaunchKernelGGL(bin_place_kernel, dim3(nchunk), dim3(SB), 0, st, ntr, nbins, w.bind.p, w.cnt.p,
-                               w.start.p, w.dEd.p, w.wtd.p, w0, df, w.pos.p, w.ws.p, w.ss.p, w.es.p);
+                               w.start.p, w.dEd.p, w.wtd.p, w0, df, w.pos.p, w.ws.p, w.ss.p, w.es.p, ntp);
         } else {
             PMX_CHECK_HIP(hipMemsetAsync(w.start.p, 0, sizeof(int) * (nbins + 1), st));
         }
@@ -2194,14 +2233,17 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         ra.G = c->dG.p;
         ra.Qx = Qx; ra.Qy = Qy; ra.Qz = Qz;
         ra.Qd = cap ? w.Qcur.p : nullptr;
-        if (ntr > 0) {
+        if (ntr_sz > 0) {
             /* algorithmic work of the gather-GEMM (src/pmx_chi.cpp:534-544): 8 flops per complex MAC,
              * 3 weighted (+1 plain) sums per (m, p) and transition; bytes = the eigenvector rows read */
             const double macs = (double)ntr * neps * npw * (spatial ? 4.0 : 16.0);
             Prof pf(c, "rho", 8.0 * macs, 16.0 * (double)N * (2 * nb_s) * p->nk);
-            /* LDS for one k's conduction and valence rows; the largest k sets the size */
+            /* LDS for one k's conduction and valence rows; the largest k sets the
+             * size (a captured step: the capacities) */
             int maxrows = 0;
-            for (int kk = 0; kk < p->nk; kk++) maxrows = std::max(maxrows, nch_h[kk] / 2 + nvh[kk] / 2);
+            if (cap) maxrows = nc_cap / 2 + nv_cap / 2;
+            else
+                for (int kk = 0; kk < p->nk; kk++) maxrows = std::max(maxrows, nch_h[kk] / 2 + nvh[kk] / 2);
             const size_t lds_r = (fp32 ? sizeof(float2) : sizeof(zd)) * (size_t)maxrows * N;
             const bool rho_lds = spatial && lds_r + sizeof(double) * 4 * 2 * 16 * 17 <= c->lds_max;
             if (rho_lds && lds_r > 65536 - sizeof(double) * 4 * 2 * 16 * 17) {
@@ -2209,8 +2251,9 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                 PMX_CHECK_HIP(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_r));
             }
             /* SOC: the valence spinor rows at k+q and the G table */
-            int maxnv = 0;
-            for (int kk = 0; kk < p->nk; kk++) maxnv = std::max(maxnv, nvh[kk]);
+            int maxnv = cap ? nv_cap : 0;
+            if (!cap)
+                for (int kk = 0; kk < p->nk; kk++) maxnv = std::max(maxnv, nvh[kk]);
             const size_t lds_soc = sizeof(float2) * (size_t)maxnv * N + sizeof(float) * 3 * (size_t)npw;
             /* the Cartesian form: its LDS is the rows plus the G table */
             const size_t lds_c = lds_r + (fp32 ? sizeof(float) : sizeof(double)) * 3 * (size_t)npw;
@@ -2239,9 +2282,9 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         for (int acc = 0; acc < nacc; acc++) {
             DBuf<zd> &Z = acc == 0 ? w.Z1 : w.Z2;
             DBuf<zd> &M = acc == 0 ? w.M1 : w.M2;
-            if (direct || ntr == 0)
+            if (direct || ntr_sz == 0)
                 PMX_CHECK_HIP(hipMemsetAsync(Z.p, 0, sizeof(zd) * (size_t)p->nfreq * npairs * 256, st));
-            if (ntr == 0) continue;
+            if (ntr_sz == 0) continue;
             if (direct) {
                 DirArgs da{D, nch, npairs, nbins, wh, p->nfreq, p->delta, acc, p->epsilon, w0, df,
                            w.Xd.p, w.es.p, w.ws.p, w.start.p, Z.p, off_b, 0};
@@ -2353,11 +2396,13 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         return 0;
     };
 
-    /* A q list of three or more runs its first two steps eagerly (the second
-     * sizes every buffer and settles the eigensolver's choices for the k+q
-     * batch) and captures the third into a HIP graph that every later q
-     * replays: no host readback inside those steps.  A replayed step whose
-     * counts differ from the captured ones is flagged and run again eagerly. */
+    /* A q list of three or more runs its first two steps eagerly (they settle
+     * the eigensolver's choices for the k+q batch and give the counts the
+     * capacities are set from) and captures the third into a HIP graph that
+     * every later q replays: no host readback inside those steps.  A replayed
+     * step whose counts exceed the capacities is flagged and run again
+     * eagerly; if the first replay is flagged, the rest of the list runs
+     * eagerly. */
     const char *genv = std::getenv("PMX_GRAPHS");
     const bool qgraph = mode == 0 && !direct && p->nq >= 3 && c->graphs && !c->prof && !c->progress &&
                         !(genv && genv[0] == '0');
@@ -2379,13 +2424,25 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         }
         std::vector<int> qs(2 + p->nq, 0);
         qs[0] = q;
+        /* capacities of a captured step: the conduction counts at k do not
+         * change with q; the valence counts at k+q may, by two levels (four
+         * spinor states) over the largest seen; the transitions by a quarter.
+         * PMX_QSTEP_MARGIN=0 sets both margins to zero (the largest seen). */
+        const char *menv = std::getenv("PMX_QSTEP_MARGIN");
+        const bool margin = !(menv && menv[0] == '0');
+        nc_cap = nc_max;
+        nv_cap = std::min(nv_max + (margin ? 4 : 0), p->nband);
+        ntr_cap = ntr_max + (margin ? ntr_max / 4 + 256 : 0);
+        const size_t ncap1 = (size_t)ntr_cap;
+        const bool fp32c = p->precision == PMX_PREC_FP32 || p->precision == PMX_PREC_BF16X3;
         if ((rc = upload(st, w.ucAll, ucA)) || (rc = upload(st, w.wAAll, wAA)) || (rc = upload(st, w.KQAll, KQA)) ||
             (rc = upload(st, w.QAll, QA)) || (rc = upload(st, w.qstate, qs)) || (rc = w.Qcur.ensure(3)) ||
-            (rc = w.ncref.ensure(p->nk)) || (rc = w.nvref.ensure(p->nk)) ||
-            (!spatial && (rc = upload(st, w.wSAll, wSA))))
+            (!spatial && (rc = upload(st, w.wSAll, wSA))) || (rc = w.dEd.ensure(ncap1)) ||
+            (rc = w.wtd.ensure(ncap1)) || (rc = w.bind.ensure(ncap1)) || (rc = w.pos.ensure(ncap1)) ||
+            (rc = w.ws.ensure(ncap1)) || (rc = w.ss.ensure(ncap1)) || (rc = w.es.ensure(ncap1)) ||
+            (rc = w.cnt.ensure((size_t)((ntr_cap + SB - 1) / SB) * nbins)) ||
+            (fp32c ? (rc = w.Xf.ensure(ncap1 * nch * D)) : (rc = w.Xd.ensure(ncap1 * nch * D))))
             return rc;
-        PMX_CHECK_HIP(hipMemcpyAsync(w.ncref.p, w.nc.p, sizeof(int) * p->nk, hipMemcpyDeviceToDevice, st));
-        PMX_CHECK_HIP(hipMemcpyAsync(w.nvref.p, w.nv.p, sizeof(int) * p->nk, hipMemcpyDeviceToDevice, st));
         PMX_CHECK_HIP(hipStreamSynchronize(st));
         PMX_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
         c->capturing = true;
@@ -2400,19 +2457,26 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         if (rc_cap) return rc_cap;
         if (ok) {
             c->qstep_captures++;
-            for (int qq = q; qq < p->nq; qq++) {
+            int qq = q;
+            PMX_CHECK_HIP(hipGraphLaunch(exec, st));
+            c->qstep_replays++;
+            qq++;
+            int first_flag = 0;
+            PMX_CHECK_HIP(hipMemcpyAsync(&first_flag, w.qstate.p + 2 + q, sizeof(int), hipMemcpyDeviceToHost, st));
+            PMX_CHECK_HIP(hipStreamSynchronize(st));
+            for (; !first_flag && qq < p->nq; qq++) {
                 PMX_CHECK_HIP(hipGraphLaunch(exec, st));
                 c->qstep_replays++;
             }
             PMX_CHECK_HIP(hipMemcpyAsync(qs.data(), w.qstate.p, sizeof(int) * qs.size(), hipMemcpyDeviceToHost, st));
             PMX_CHECK_HIP(hipStreamSynchronize(st));
             (void)hipGraphExecDestroy(exec);
-            for (int qq = q; qq < p->nq; qq++)
-                if (qs[2 + qq]) {
+            for (int r = q; r < qq; r++)
+                if (qs[2 + r]) {
                     c->qstep_reruns++;
-                    if ((rc = qstep(qq, false))) return rc;
+                    if ((rc = qstep(r, false))) return rc;
                 }
-            q = p->nq;
+            q = qq; /* the rest (after a flagged first replay) runs eagerly below */
         } else {
             (void)hipGetLastError(); /* the capture did not hold: the rest runs eagerly */
         }

@@ -153,20 +153,27 @@ def test_chi_q_steps_replay_soc(precision):
     assert err <= 1e-12
 
 
-def test_chi_q_steps_replay_with_changing_counts():
+@pytest.mark.parametrize("margin", ["default", "0"])
+def test_chi_q_steps_replay_with_changing_counts(margin, monkeypatch):
     """The energy zero inside the valence bands (Si, the valence top at X, about
     3 eV below the top at Gamma), so the valence counts at k+q change from q to
-    q: replayed steps whose counts differ from the captured ones are flagged
-    and run again eagerly; the result equals the eager run."""
+    q.  With the default capacities (two levels and a quarter of the
+    transitions over the largest counts of the eager steps) the replays hold
+    them; with PMX_QSTEP_MARGIN=0 (no margin) replays whose counts exceed the
+    eager steps' are flagged and run again eagerly.  Either way the result
+    equals the eager run."""
     from picomax_amd import kpoints
-    _say("q-step graph with changing counts")
+    if margin == "0":
+        monkeypatch.setenv("PMX_QSTEP_MARGIN", "0")
+    _say("q-step graph with changing counts, margin %s" % margin)
     K = kpoints.gamma_grid("zb", 5)
-    Q = [[0.05 * (i + 1), 0.03 * i, 0.0] for i in range(5)]
+    Q = [[0.01, 0.0, 0.0], [0.02, 0.0, 0.0], [0.3, 0.1, 0.0], [0.5, 0.2, 0.1], [0.7, 0.3, 0.2]]
     args = SI[:-2] + ["-gcut", "3.4641016151377544", "-neps", "4"]
     err, (sg, se) = _chi_q_list(args, K, Q, 8, 16, "fp64", refpoint=(1.0, 0.0, 0.0))
     _say("  err %.3e, graphs %s" % (err, sg))
-    assert sg["captures"] == 1 and sg["replays"] == 3, sg
-    assert sg["reruns"] >= 1, sg  # the counts did change: some replay was flagged
+    assert sg["captures"] == 1 and sg["replays"] >= 1, sg
+    if margin == "0":
+        assert sg["reruns"] >= 1, sg
     assert se["captures"] == 0
     assert err <= 1e-12
 
