@@ -1310,8 +1310,12 @@ struct ConvArgs {
  * 16 frequencies, so every M value it loads feeds 4 MFMA pairs; k steps that
  * miss a block's band are skipped for that block.  Kt is staged in LDS.  T =
  * double (FP64 MFMA 16x16x4) or float (FP32 MFMA 16x16x4, the mixed paths). */
-constexpr int CONV_FB = 4; /* frequency blocks of 16 per wave */
-template <class T> __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 8 : 1) conv_mfma_kernel(ConvArgs a) {
+constexpr int CONV_FB = 4; /* default frequency blocks of 16 per wave */
+/* FB frequency blocks of 16 per wave (each M value loaded feeds FB MFMA
+ * pairs), OCC waves per SIMD the registers are cut for; the autotuner picks
+ * among the built pairs */
+template <class T, int FB, int OCC> __global__ void __launch_bounds__(256, OCC) conv_mfma_kernel(ConvArgs a) {
+    constexpr int CONV_FB = FB;
     typedef typename MfmaT<T>::V V;
     typedef typename MfmaT<T>::C C;
     extern __shared__ double smem_c[];
@@ -1511,8 +1515,8 @@ __global__ void __launch_bounds__(256) f0max_kernel(int nt, const double *dE, do
  * placed in file order.  A wave computes 16 rows x KK_CT
  * column tiles of 16, so each A value it loads feeds KK_CT MFMA pairs.  T =
  * double (FP64 MFMA) or float (FP32 MFMA, mixed paths; Kk then in FP32). */
-constexpr int KK_CT = 4;
-template <class T>
+constexpr int KK_CT_DEF = 4; /* default column tiles per wave */
+template <class T, int KK_CT = KK_CT_DEF>
 __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 8 : 1) kk_gemm_kernel(int M, int NF, const double *Re0, const double *Im0,
                                                       const T *Kk, double *Re, double *Im, int ne2, int nq, int q,
                                                       const int *qp) {
@@ -2313,32 +2317,68 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                         hipLaunchKernelGGL((moment_bf16x3_kernel<2, 9>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
                     else if (bf)
                         hipLaunchKernelGGL((moment_bf16x3_kernel<2, MOM_MAXNT>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
-                    else if (fp32 && NT <= 9)
-                        hipLaunchKernelGGL((moment_f32q_kernel<9, 3>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
-                    else if (fp32 && NT <= 12)
-                        hipLaunchKernelGGL((moment_f32q_kernel<12, 3>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
-                    else if (fp32)
+                    else if (fp32 && NT <= 12) {
+                        /* waves per SIMD of the FP32 rank update, chosen by the autotuner */
+                        auto mom_cand = [&](const char *name, void (*kern)(MomArgs2)) {
+                            return TuneCand{name, [&, kern](int) {
+                                                hipLaunchKernelGGL(kern, dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0,
+                                                                   st, ma);
+                                                PMX_CHECK_HIP(hipGetLastError());
+                                                return 0;
+                                            }};
+                        };
+                        std::vector<TuneCand> mc;
+                        if (NT <= 9) {
+                            mc.push_back(mom_cand("mom_w3", moment_f32q_kernel<9, 3>));
+                            mc.push_back(mom_cand("mom_w2", moment_f32q_kernel<9, 2>));
+                        } else {
+                            mc.push_back(mom_cand("mom_w3", moment_f32q_kernel<12, 3>));
+                            mc.push_back(mom_cand("mom_w2", moment_f32q_kernel<12, 2>));
+                        }
+                        const std::string key = "moment f32 NT=" + std::to_string(NT) + " nch=" + std::to_string(nch) +
+                                                " pairs=" + std::to_string(npairs) + " nbins=" + std::to_string(nbins);
+                        if ((rc = tune_run(c, key, 1 << 30, 1 << 30, mc))) return rc;
+                    } else if (fp32)
                         hipLaunchKernelGGL((moment_f32q_kernel<MAX_NT, 2>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
                     else
                         hipLaunchKernelGGL(moment_mfma_kernel<double>, dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
                 }
                 {
                     Prof pf(c, fp32 ? "conv_f32" : "conv_f64", 4.0 * 9.0 * ne2 * p->nfreq * (double)W * NT, 0.0);
-                    const dim3 grid((npairs * 256 + 63) / 64, (p->nfreq + 16 * CONV_FB - 1) / (16 * CONV_FB));
                     size_t lds_c = (fp32 ? sizeof(float) : sizeof(double)) * (size_t)W * NT;
                     const int kt_lds = lds_c <= c->lds_max; /* a wider table is read from global memory (L2) */
                     if (!kt_lds) lds_c = 0;
-                    if (lds_c > 65536) {
-                        const void *kf = fp32 ? (const void *)conv_mfma_kernel<float> : (const void *)conv_mfma_kernel<double>;
-                        PMX_CHECK_HIP(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_c));
-                    }
+                    const ConvArgs ca{npairs, nbins, NT, R0, W, p->nfreq, M.p,
+                                      fp32 ? (const void *)w.Ktf.p : (const void *)w.Ktd.p, Z.p, kt_lds};
+                    /* frequency blocks per wave (and the waves per SIMD their
+                     * registers allow), chosen by the autotuner per shape */
+                    auto conv_cand = [&](const char *name, void (*kern)(ConvArgs), int fb) {
+                        return TuneCand{name, [&, kern, fb](int) {
+                                            if (lds_c > 65536)
+                                                PMX_CHECK_HIP(hipFuncSetAttribute(
+                                                    (const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                    (int)lds_c));
+                                            const dim3 grid((npairs * 256 + 63) / 64,
+                                                            (p->nfreq + 16 * fb - 1) / (16 * fb));
+                                            hipLaunchKernelGGL(kern, grid, dim3(256), lds_c, st, ca);
+                                            PMX_CHECK_HIP(hipGetLastError());
+                                            return 0;
+                                        }};
+                    };
+                    std::vector<TuneCand> cc;
                     if (fp32) {
-                        ConvArgs ca{npairs, nbins, NT, R0, W, p->nfreq, M.p, w.Ktf.p, Z.p, kt_lds};
-                        hipLaunchKernelGGL(conv_mfma_kernel<float>, grid, dim3(256), lds_c, st, ca);
+                        cc.push_back(conv_cand("conv_fb4", conv_mfma_kernel<float, 4, 8>, 4));
+                        cc.push_back(conv_cand("conv_fb2", conv_mfma_kernel<float, 2, 8>, 2));
+                        cc.push_back(conv_cand("conv_fb8", conv_mfma_kernel<float, 8, 4>, 8));
                     } else {
-                        ConvArgs ca{npairs, nbins, NT, R0, W, p->nfreq, M.p, w.Ktd.p, Z.p, kt_lds};
-                        hipLaunchKernelGGL(conv_mfma_kernel<double>, grid, dim3(256), lds_c, st, ca);
+                        cc.push_back(conv_cand("conv_fb4", conv_mfma_kernel<double, 4, 1>, 4));
+                        cc.push_back(conv_cand("conv_fb2", conv_mfma_kernel<double, 2, 1>, 2));
                     }
+                    const std::string key = std::string("conv ") + (fp32 ? "f32" : "f64") + " E=" +
+                                            std::to_string(npairs * 256) + " nbins=" + std::to_string(nbins) +
+                                            " W=" + std::to_string(W) + " NT=" + std::to_string(NT) +
+                                            " nfreq=" + std::to_string(p->nfreq);
+                    if ((rc = tune_run(c, key, 1 << 30, 1 << 30, cc))) return rc;
                 }
             }
             PMX_CHECK_HIP(hipGetLastError());
@@ -2378,14 +2418,29 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
             head("    Kramers-Kronig transform...");
             Prof pf(c, "kk", 4.0 * per_q * p->nfreq, 8.0 * 4 * per_q);
             const int M = 9 * ne2;
-            const dim3 grid(8 * (((M + 63) / 64 + 7) / 8) * ((p->nfreq + 16 * KK_CT - 1) / (16 * KK_CT)));
-            if (fp32)
-                hipLaunchKernelGGL(kk_gemm_kernel<float>, grid, dim3(256), 0, st, M, p->nfreq, w.Re0.p, w.Im0.p,
-                                   w.Kkf.p, dRe, dIm, ne2, p->nq, q, cap ? w.qstate.p : nullptr);
-            else
-                hipLaunchKernelGGL(kk_gemm_kernel<double>, grid, dim3(256), 0, st, M, p->nfreq, w.Re0.p, w.Im0.p,
-                                   w.Kkd.p, dRe, dIm, ne2, p->nq, q, cap ? w.qstate.p : nullptr);
-            PMX_CHECK_HIP(hipGetLastError());
+            const int *qp = cap ? w.qstate.p : nullptr;
+            /* column tiles of 16 per wave, chosen by the autotuner per shape */
+            auto kk_cand = [&](const char *name, auto kern, int ct, const auto *Kk) {
+                return TuneCand{name, [&, kern, ct, Kk](int) {
+                                    const dim3 grid(8 * (((M + 63) / 64 + 7) / 8) * ((p->nfreq + 16 * ct - 1) / (16 * ct)));
+                                    hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, M, p->nfreq, w.Re0.p, w.Im0.p, Kk,
+                                                       dRe, dIm, ne2, p->nq, q, qp);
+                                    PMX_CHECK_HIP(hipGetLastError());
+                                    return 0;
+                                }};
+            };
+            std::vector<TuneCand> kc;
+            if (fp32) {
+                kc.push_back(kk_cand("kk_ct4", kk_gemm_kernel<float, 4>, 4, (const float *)w.Kkf.p));
+                kc.push_back(kk_cand("kk_ct2", kk_gemm_kernel<float, 2>, 2, (const float *)w.Kkf.p));
+                kc.push_back(kk_cand("kk_ct8", kk_gemm_kernel<float, 8>, 8, (const float *)w.Kkf.p));
+            } else {
+                kc.push_back(kk_cand("kk_ct4", kk_gemm_kernel<double, 4>, 4, (const double *)w.Kkd.p));
+                kc.push_back(kk_cand("kk_ct2", kk_gemm_kernel<double, 2>, 2, (const double *)w.Kkd.p));
+            }
+            const std::string key = std::string("kk ") + (fp32 ? "f32" : "f64") + " M=" + std::to_string(M) +
+                                    " nfreq=" + std::to_string(p->nfreq);
+            if ((rc = tune_run(c, key, 1 << 30, 1 << 30, kc))) return rc;
             if ((rc = elapsed("    "))) return rc;
         } else {
             hipLaunchKernelGGL(place_kernel, dim3(1024), dim3(256), 0, st, 9 * ne2, p->nfreq, ne2, p->nq, q, w.Re0.p,
@@ -2541,7 +2596,7 @@ extern "C" int pmx_kk(pmx_ctx *c, int nel, int nfreq, double dfreq, double *ReX,
     PMX_CHECK_HIP(hipMemcpyAsync(i0.p, ImX, sizeof(double) * n, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(kkmat_kernel<double>, dim3(256), dim3(256), 0, st, nfreq, dfreq, Kk.p);
     hipLaunchKernelGGL(kk_gemm_kernel<double>,
-                       dim3(8 * (((nel + 63) / 64 + 7) / 8) * ((nfreq + 16 * KK_CT - 1) / (16 * KK_CT))), dim3(256),
+                       dim3(8 * (((nel + 63) / 64 + 7) / 8) * ((nfreq + 16 * KK_CT_DEF - 1) / (16 * KK_CT_DEF))), dim3(256),
                        0, st, nel, nfreq, r0.p, i0.p, Kk.p, r.p, i.p, nel, 1, 0, nullptr);
     PMX_CHECK_HIP(hipGetLastError());
     PMX_CHECK_HIP(hipMemcpyAsync(ReX, r.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
@@ -3222,7 +3277,7 @@ extern "C" int pmx_grad_chi(pmx_ctx *c, const pmx_chi_params *p, const double *g
             const double *g_re = gr.p, *g_im = gi.p;
             if (p->kk) {
                 /* (1) the transposed KK matrix in the forward's GEMM: compact in, compact out */
-                const dim3 grid(8 * (((nel + 63) / 64 + 7) / 8) * ((nfreq + 16 * KK_CT - 1) / (16 * KK_CT)));
+                const dim3 grid(8 * (((nel + 63) / 64 + 7) / 8) * ((nfreq + 16 * KK_CT_DEF - 1) / (16 * KK_CT_DEF)));
                 hipLaunchKernelGGL(kk_gemm_kernel<double>, grid, dim3(256), 0, st, nel, nfreq, gr.p, gi.p, KkT.p,
                                    gr0.p, gi0.p, ne2, 1, 0, nullptr);
                 g_re = gr0.p;

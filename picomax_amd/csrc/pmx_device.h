@@ -9,6 +9,7 @@
 #include "pmx_internal.h"
 
 #include <complex>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -146,6 +147,19 @@ struct pmx_ctx {
 namespace pmx {
 /* records [start, end) events around the launches in its scope when
  * profiling is on; flops/bytes are the algorithmic counts of that launch */
+/* Autotuner (pmx_eig.hip): the first call of a shape (key) with at least
+ * 2 x num_cu items launches every candidate once untimed, then times each on
+ * min(nmat, sample) items under HIP events, keeps the fastest (a later one
+ * must win by 3%) for the context and logs the decision for
+ * pmx_tune_report.  A candidate's run(n) processes the first n items (for
+ * whole-launch kernels n is ignored; they must be idempotent).
+ * PMX_AUTOTUNE=0 keeps the first; PMX_FORCE_KERNEL=<name> forces one. */
+struct TuneCand {
+    const char *name;
+    std::function<int(int)> run;
+};
+int tune_run(pmx_ctx *c, const std::string &key, int nmat, int sample, const std::vector<TuneCand> &cands);
+
 struct Prof {
     pmx_ctx *c;
     int idx = -1;

@@ -87,9 +87,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 /* Sum over a full wave (every lane active) by DPP within rows of 16 and
  * four lane reads; every lane gets the same bits, in every wave alike. */
 template <int CTRL> __device__ __forceinline__ double dpp_f64(double v) {
+    /* every lane reads a valid source lane for the controls used here
+     * (quad permutes, row mirrors), so no old value is needed */
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 __device__ __forceinline__ double readlane_f64(double v, int l) {
@@ -2054,12 +2056,7 @@ template <class T, bool LDS> int launch_tridiag_fused(pmx_ctx *c, EigArgs &a, in
  * decision is logged for pmx_tune_report.  Smaller batches use the first
  * candidate without timing.  PMX_AUTOTUNE=0 disables the timing (first
  * candidate); PMX_FORCE_KERNEL=<name> forces a candidate by name. */
-/* a candidate processes the first nm matrices of the batch (nm = the batch
- * size for the real run, the sample size while timing) */
-struct TuneCand {
-    const char *name;
-    std::function<int(int)> run;
-};
+} // namespace
 
 int tune_run(pmx_ctx *c, const std::string &key, int nmat, int sample, const std::vector<TuneCand> &cands) {
     const char *form = std::getenv("PMX_FORCE_KERNEL");
@@ -2105,7 +2102,9 @@ int tune_run(pmx_ctx *c, const std::string &key, int nmat, int sample, const std
     for (int i = 1; i < nc; i++)
         if (ms[i] < 0.97f * ms[best]) best = i;
     c->tune_choice[key] = best;
-    std::string rec = "{\"kernel\": \"" + key + "\", \"sample\": " + std::to_string(ns) + ", \"ms\": {";
+    /* whole-launch candidates (the chi kernels) pass a sample of 1 << 30 */
+    const std::string smp = ns >= (1 << 30) ? std::string("\"launch\"") : std::to_string(ns);
+    std::string rec = "{\"kernel\": \"" + key + "\", \"sample\": " + smp + ", \"ms\": {";
     char num[64];
     for (int i = 0; i < nc; i++) {
         std::snprintf(num, sizeof(num), "%.4f", ms[i]);
@@ -2115,6 +2114,8 @@ int tune_run(pmx_ctx *c, const std::string &key, int nmat, int sample, const std
     c->tune_log.push_back(rec);
     return cands[best].run(nmat);
 }
+
+namespace {
 
 /* The generic tridiagonalisation (any N, real or complex, matrix in LDS or in
  * global memory) in its two forms: three matrix streams per step

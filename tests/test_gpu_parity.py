@@ -177,6 +177,31 @@ def test_chi_precision_modes(manifest, ctx, case, precision, tol):
     assert err < tol
 
 
+@pytest.mark.parametrize("form,precision,tol", [("conv_fb2", "fp64", 1e-8), ("kk_ct2", "fp64", 1e-8),
+                                                ("conv_fb2", "fp32", 1e-4), ("conv_fb8", "fp32", 1e-4),
+                                                ("kk_ct2", "fp32", 1e-4), ("kk_ct8", "fp32", 1e-4),
+                                                ("mom_w2", "fp32", 1e-4)])
+def test_chi_tuned_forms_against_reference(manifest, ctx, form, precision, tol, monkeypatch):
+    """Every autotuner candidate of the convolution (frequency blocks per
+    wave), the KK GEMM (column tiles per wave) and the FP32 rank update
+    (waves per SIMD), forced, against the reference's output on the
+    Lorentzian + KK fixture: within 1e-8 per slice in FP64, 1e-4 in FP32."""
+    monkeypatch.setenv("PMX_FORCE_KERNEL", form)
+    case = "si_chi_lor"
+    args = manifest[case]["args"]
+    _say("chi form %s %s on %s" % (form, precision, case))
+    inp = capi.Input(_expand(args))
+    ctx.set_system(inp.system(), inp)
+    s, p = cli_args.build(args)
+    off = ctx.ref_energy(p["refpoint"], p["nvalence"])
+    re, im = ctx.chi(p["K"], p["KW"], p["Q"], p["nband"], off, p["delta"], p["epsilon"], p["nfreq"], p["dfreq"],
+                     p["kk"], precision)
+    ref = np.fromfile(os.path.join(cli_args.GOLDEN, case + ".dat"), dtype="<f8")
+    err = cli_args.slice_error(np.concatenate([re, im]), ref, _nslice(args))
+    _say("  slice error %.3e" % err)
+    assert err < tol
+
+
 def test_epsinv_against_numpy(manifest, ctx):
     case = "si_chi_lor"
     args = manifest[case]["args"]
