@@ -130,3 +130,34 @@ def test_cli_epsll(manifest, case, tmp_path):
     _say("  slice error %.3e" % err)
     assert err < (1e-5 if p["kk"] == 0 else 1e-8)
     assert open(str(tmp_path / (case + ".gpt")), "rb").read() == gpt.tobytes()
+
+
+@pytest.mark.parametrize("ll", [0, 1, 2])
+def test_spinor_path_with_zero_soc_equals_spatial(ll):
+    """The GPU's spinor branches (rho_kernel's four channels for chi_tensor,
+    the same-spin vertex of chi_LL / chi_xyz, the complex spinor eigensolve)
+    with every spin-orbit map present but zero reduce to its spatial branch
+    (real-gauge solve, spatial rho), which the reference's fixtures pin:
+    within 1e-9 of each slice's max."""
+    vg = "3,4,8,11,-0.21,0,0.04,0.08,-0.21,0,0.04,0.08"
+    base = ["-crystal", "zb", "-a", "5.43", "-vg", vg, "-gcut", "3.0", "-neps", "4"]
+    rng = np.random.default_rng(11)
+    K = rng.uniform(-0.5, 0.5, (6, 3))
+    KW = np.full(len(K), 1.0 / 6)
+    Q = [[0.012, 0.004, -0.003]]
+    out = []
+    for args in (base, base + ["-us_so", "3:0,8:0,11:0"]):
+        c = capi.Context(0)
+        try:
+            inp = capi.Input(args)
+            c.set_system(inp.system(), inp)
+            off = c.ref_energy((0.5, 0.5, 0.5), 8)
+            out.append((off, c.chi(K, KW, Q, 16, off, 1, 0.08, 200, 0.04, 1, "fp64", ll=ll)))
+        finally:
+            c.close()
+    (o0, (r0, i0)), (o1, (r1, i1)) = out
+    _say("spinor vs spatial, mode %d" % ll)
+    assert abs(o0 - o1) < 1e-10
+    err = cli_args.slice_error(np.concatenate([r1, i1]), np.concatenate([r0, i0]), 9 * 10)
+    _say("  slice error %.3e" % err)
+    assert err < 1e-9

@@ -240,3 +240,27 @@ def test_kk_lorentz_oscillator():
     assert np.abs(re[sel] - re_true[sel]).max() < 0.02 * np.abs(re_true).max()
     # ImX gets the transform of the original ReX (zero here): unchanged
     assert np.array_equal(im, im_true)
+
+
+@pytest.mark.parametrize("ll", [0, 1, 2])
+def test_spinor_path_with_zero_soc_equals_spatial(ll):
+    """The spinor branches of chi_tensor (channels A, B, C, E), chi_LL and
+    chi_xyz (same-spin products, spin-current vertex) with every spin-orbit
+    map present but zero reduce to the spatial branch, which the reference's
+    fixtures pin: the 2 NPW spinor solve then has Kramers pairs of the NPW
+    spatial levels, and the spin sums over each pair give the spatial
+    result's factor 2 exactly."""
+    vg = [3, 4, 8, 11, -0.21, 0, 0.04, 0.08, -0.21, 0, 0.04, 0.08]
+    s0 = orc.System("zb", a=5.43, vg=vg, gcut=3.0, neps=4)
+    s1 = orc.System("zb", a=5.43, vg=vg, gcut=3.0, neps=4, us_so={3.0: 0.0, 8.0: 0.0, 11.0: 0.0})
+    rng = np.random.default_rng(11)
+    K = rng.uniform(-0.5, 0.5, (4, 3))
+    KW = np.full(len(K), 0.25)
+    Q = [[0.012, 0.004, -0.003]]
+    off = s0.ref_energy((0.5, 0.5, 0.5), 8)
+    assert abs(s1.ref_energy((0.5, 0.5, 0.5), 8) - off) < 1e-10
+    r0, i0 = s0.chi(K, KW, Q, 16, off, 1, 0.08, 200, 0.04, 1, ll=ll)
+    r1, i1 = s1.chi(K, KW, Q, 16, off, 1, 0.08, 200, 0.04, 1, ll=ll)
+    err = cli_args.slice_error(np.concatenate([r1, i1]), np.concatenate([r0, i0]), 9 * 10)
+    assert np.abs(np.concatenate([r0, i0])).max() > 0
+    assert err < 1e-9
