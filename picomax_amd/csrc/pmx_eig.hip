@@ -601,6 +601,358 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_fused_kernel(EigArgs a) {
     }
 }
 
+template <class T> __device__ __forceinline__ T mk(double re, double im);
+template <> __device__ __forceinline__ double mk<double>(double re, double) { return re; }
+template <> __device__ __forceinline__ zd mk<zd>(double re, double im) { return zd{re, im}; }
+
+/* Blocked Householder tridiagonalisation for matrices in the global slab,
+ * lower storage, in panels of NBP columns (the scheme of LAPACK's latrd).
+ * Inside a panel the trailing matrix stays as it was at the panel's start;
+ * with V, W the panel's reflectors and update vectors so far (columns
+ * 0..j-1 of [NBP][N] arrays in the slab after the matrix), step k = k0 + j:
+ *   (a) column k is brought up to date: x -= V conj(W[k,:])^T + W conj(V[k,:])^T;
+ *   (b) the reflector of x, as in tridiag_kernel;
+ *   (c) p = tau (A22 v - V (W^H v) - W (V^H v)) with A22 the stale trailing
+ *       matrix: one read of it per step;
+ *   (d) w = p - (tau/2)(p^H v) v joins W;
+ * and after the panel one pass applies A22 -= V W^H + W V^H, so the update's
+ * read and write of the trailing matrix are paid once per panel instead of
+ * once per step.  W^H v and V^H v ride on the reduction that forms the
+ * reflector: W^H v = conj(W[k+1,:]) + scal sum_{r >= k+2} conj(W[r,:]) x_r.
+ * Every pass maps row r of a step to thread (r - k) mod 256, so a thread
+ * keeps its rows' x, v and p in registers between passes.  Same outputs as
+ * tridiag_kernel (d, e, tau, reflectors), to rounding. */
+constexpr int PANEL_MR = 4;  /* rows per thread in a step's row passes: N <= 1024 */
+constexpr int PANEL_UC = 16; /* columns per wave in the panel update */
+
+template <class T, int NBP, int NCOL, int TU>
+__global__ void __launch_bounds__(EIG_THREADS, 2) tridiag_panel_kernel(EigArgs a) {
+    extern __shared__ double smem_d[];
+    constexpr int NV = 2 * NBP;      /* staged values per column in the panel update */
+    constexpr int RS = 1 + 4 * NBP;  /* per-wave reduction slots */
+    const int N = a.N;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const size_t P = (size_t)N * (N + 1) / 2;
+
+    /* LDS: [d N][e N] (double) [v N] [part WAVES*N, or the update's staged
+     * columns 64*NV] (T) [red WAVES*RS] (double) */
+    char *sp = reinterpret_cast<char *>(smem_d);
+    double *d = reinterpret_cast<double *>(sp); sp += sizeof(double) * N;
+    double *e = reinterpret_cast<double *>(sp); sp += sizeof(double) * N;
+    T *vv = reinterpret_cast<T *>(sp); sp += sizeof(T) * N;
+    T *part = reinterpret_cast<T *>(sp), *stage = part;
+    sp += sizeof(T) * max((size_t)N * EIG_WAVES, (size_t)64 * NV);
+    double *red = reinterpret_cast<double *>(sp);
+    __shared__ double sh_alpha[2];
+
+    char *slab = a.scratch + (size_t)blockIdx.x * a.slab;
+    T *A = reinterpret_cast<T *>(slab);
+    T *Vp = reinterpret_cast<T *>(slab + ((P * sizeof(T) + 255) / 256) * 256);
+    T *Wp = Vp + (size_t)NBP * N;
+
+    for (int mat = blockIdx.x; mat < a.nmat; mat += gridDim.x) {
+        double Kv[3] = {a.K[3 * mat], a.K[3 * mat + 1], a.K[3 * mat + 2]};
+        T *vref = a.vref ? reinterpret_cast<T *>(a.vref) + (size_t)mat * P : nullptr;
+        T *tau_out = a.tauv ? reinterpret_cast<T *>(a.tauv) + (size_t)mat * N : nullptr;
+        for (int c = wave; c < N; c += EIG_WAVES)
+            for (int r = c + lane; r < N; r += 64) A[pk(r, c, N)] = h_elem<T>(a, Kv, r, c);
+        __syncthreads();
+
+        for (int k0 = 0; k0 + 1 < N; k0 += NBP) {
+            const int nb = min(NBP, N - 1 - k0);
+            for (int j = 0; j < nb; j++) {
+                const int k = k0 + j, L = N - k - 1;
+                /* (a) column k up to date; |x|^2 and conj(W)^T x, conj(V)^T x below row k+1 */
+                T cw[NBP], cv[NBP];
+#pragma unroll
+                for (int i = 0; i < NBP; i++) {
+                    cw[i] = i < j ? cj(Wp[(size_t)i * N + k]) : zero<T>();
+                    cv[i] = i < j ? cj(Vp[(size_t)i * N + k]) : zero<T>();
+                }
+                T xs[PANEL_MR], rw[NBP], rv[NBP];
+#pragma unroll
+                for (int i = 0; i < NBP; i++) rw[i] = rv[i] = zero<T>();
+                double s = 0.0;
+#pragma unroll
+                for (int m = 0; m < PANEL_MR; m++) {
+                    const int r = k + tid + EIG_THREADS * m;
+                    T x = zero<T>();
+                    if (r < N) {
+                        T vi[NBP], wi[NBP];
+#pragma unroll
+                        for (int i = 0; i < NBP; i++) {
+                            vi[i] = i < j ? Vp[(size_t)i * N + r] : zero<T>();
+                            wi[i] = i < j ? Wp[(size_t)i * N + r] : zero<T>();
+                        }
+                        x = A[pk(r, k, N)];
+#pragma unroll
+                        for (int i = 0; i < NBP; i++)
+                            if (i < j) x -= vi[i] * cw[i] + wi[i] * cv[i];
+                        if (r == k) d[k] = re(x);
+                        if (r == k + 1) {
+                            sh_alpha[0] = re(x);
+                            sh_alpha[1] = im(x);
+                        }
+                        if (r >= k + 2) {
+                            s += nrm2(x);
+#pragma unroll
+                            for (int i = 0; i < NBP; i++) {
+                                rw[i] += cj(wi[i]) * x;
+                                rv[i] += cj(vi[i]) * x;
+                            }
+                        }
+                    }
+                    xs[m] = x;
+                }
+                s = wave_sum_full(s);
+                if (lane == 0) red[wave * RS] = s;
+#pragma unroll
+                for (int i = 0; i < NBP; i++)
+                    if (i < j) {
+                        const double s0 = wave_sum_full(re(rw[i])), s1 = wave_sum_full(im(rw[i]));
+                        const double s2 = wave_sum_full(re(rv[i])), s3 = wave_sum_full(im(rv[i]));
+                        if (lane == 0) {
+                            double *q = red + wave * RS + 1 + 4 * i;
+                            q[0] = s0;
+                            q[1] = s1;
+                            q[2] = s2;
+                            q[3] = s3;
+                        }
+                    }
+                __syncthreads(); /* (1) */
+                /* (b) every thread forms the reflector's scalars from the same sums */
+                double xn2 = 0.0;
+                for (int w = 0; w < EIG_WAVES; w++) xn2 += red[w * RS];
+                const T alpha = mk<T>(sh_alpha[0], sh_alpha[1]);
+                double beta;
+                T tau, scal;
+                if (xn2 == 0.0 && im(alpha) == 0.0) {
+                    beta = re(alpha);
+                    tau = zero<T>();
+                    scal = zero<T>();
+                } else {
+                    beta = -copysign(sqrt(nrm2(alpha) + xn2), re(alpha));
+                    tau = cdiv(from_real<T>(beta) - alpha, from_real<T>(beta));
+                    scal = cdiv(from_real<T>(1.0), alpha - from_real<T>(beta));
+                }
+                const bool active = !(re(tau) == 0.0 && im(tau) == 0.0);
+                /* W^H v and V^H v over rows k+1.. (v_{k+1} = 1, v_r = scal x_r below) */
+                T cWv[NBP], cVv[NBP];
+#pragma unroll
+                for (int i = 0; i < NBP; i++) {
+                    cWv[i] = cVv[i] = zero<T>();
+                    if (i < j) {
+                        double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+                        for (int w = 0; w < EIG_WAVES; w++) {
+                            const double *q = red + w * RS + 1 + 4 * i;
+                            t0 += q[0];
+                            t1 += q[1];
+                            t2 += q[2];
+                            t3 += q[3];
+                        }
+                        cWv[i] = cj(Wp[(size_t)i * N + k + 1]) + scal * mk<T>(t0, t1);
+                        cVv[i] = cj(Vp[(size_t)i * N + k + 1]) + scal * mk<T>(t2, t3);
+                    }
+                }
+                /* the reflector: v in LDS (the mat-vec), in V, and out */
+                T vs[PANEL_MR];
+#pragma unroll
+                for (int m = 0; m < PANEL_MR; m++) {
+                    const int r = k + tid + EIG_THREADS * m;
+                    T v = zero<T>();
+                    if (active && r >= k + 1 && r < N) v = (r == k + 1) ? from_real<T>(1.0) : xs[m] * scal;
+                    vs[m] = v;
+                    if (r >= k + 1 && r < N) {
+                        vv[r] = v;
+                        Vp[(size_t)j * N + r] = v;
+                        if (vref && r >= k + 2) vref[pk(r, k, N)] = v;
+                    }
+                }
+                for (int i = tid; i < L * EIG_WAVES; i += EIG_THREADS) part[i] = zero<T>();
+                if (tid == 0) {
+                    e[k] = beta;
+                    if (tau_out) tau_out[k] = tau;
+                }
+                __syncthreads(); /* (2) */
+                /* (c) stale A22 v, column pass: every stored element read once.
+                 * A wave takes NCOL columns at a time, cc + WAVES t, and walks
+                 * their common rows together: NCOL times the loads in flight,
+                 * and one partial update per row for all of them. */
+                for (int cc = wave; cc < L; cc += NCOL * EIG_WAVES) {
+                    T vc[NCOL], dot[NCOL];
+                    const T *colp[NCOL];
+#pragma unroll
+                    for (int t = 0; t < NCOL; t++) {
+                        const int ct = cc + EIG_WAVES * t;
+                        /* element (k+1+rr, k+1+ct) at colp[t][rr], rows rr >= ct */
+                        vc[t] = ct < L ? vv[k + 1 + ct] : zero<T>();
+                        colp[t] = A + (pk(k + 1 + min(ct, L - 1), k + 1 + min(ct, L - 1), N) - min(ct, L - 1));
+                        dot[t] = zero<T>();
+                    }
+                    for (int r0 = cc + lane; r0 < L; r0 += 64 * TU) {
+                        T av[NCOL][TU];
+#pragma unroll
+                        for (int u = 0; u < TU; u++)
+#pragma unroll
+                            for (int t = 0; t < NCOL; t++) {
+                                const int rr = r0 + 64 * u;
+                                av[t][u] = (rr >= cc + EIG_WAVES * t && rr < L) ? colp[t][rr] : zero<T>();
+                            }
+                        /* v and the partials of the batch's rows, read together
+                         * (rows past L read row L-1 and are discarded) */
+                        T vr[TU], pt[TU];
+#pragma unroll
+                        for (int u = 0; u < TU; u++) {
+                            const int rq = min(r0 + 64 * u, L - 1);
+                            vr[u] = vv[k + 1 + rq];
+                            pt[u] = part[wave * L + rq];
+                        }
+#pragma unroll
+                        for (int u = 0; u < TU; u++) {
+                            const int rr = r0 + 64 * u;
+#pragma unroll
+                            for (int t = 0; t < NCOL; t++) {
+                                const int ct = cc + EIG_WAVES * t;
+                                /* av is zero outside rows ct..L-1 */
+                                dot[t] += (rr == ct) ? av[t][u] * vc[t] : cj(av[t][u]) * vr[u];
+                                if (rr != ct) pt[u] += av[t][u] * vc[t];
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < TU; u++) {
+                            const int rr = r0 + 64 * u;
+                            if (rr < L && rr != cc) part[wave * L + rr] = pt[u];
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < NCOL; t++) {
+                        const int ct = cc + EIG_WAVES * t;
+                        const double dr = wave_sum_full(re(dot[t])), di = wave_sum_full(im(dot[t]));
+                        if (lane == 0 && ct < L) part[wave * L + ct] += mk<T>(dr, di);
+                    }
+                }
+                __syncthreads(); /* (3) */
+                /* p with the panel's corrections; w = p - (tau/2)(p^H v) v */
+                T ps[PANEL_MR];
+                double pr = 0.0, pim = 0.0;
+#pragma unroll
+                for (int m = 0; m < PANEL_MR; m++) {
+                    const int r = k + tid + EIG_THREADS * m;
+                    T p = zero<T>();
+                    if (r >= k + 1 && r < N) {
+                        const int rr = r - k - 1;
+                        for (int w = 0; w < EIG_WAVES; w++) p += part[w * L + rr];
+#pragma unroll
+                        for (int i = 0; i < NBP; i++)
+                            if (i < j) p -= Vp[(size_t)i * N + r] * cWv[i] + Wp[(size_t)i * N + r] * cVv[i];
+                        p = tau * p;
+                        const T cvp = cj(p) * vs[m];
+                        pr += re(cvp);
+                        pim += im(cvp);
+                    }
+                    ps[m] = p;
+                }
+                pr = wave_sum_full(pr);
+                pim = wave_sum_full(pim);
+                if (lane == 0) {
+                    red[wave * RS] = pr;
+                    red[wave * RS + 1] = pim;
+                }
+                __syncthreads(); /* (4) */
+                double sr = 0.0, si = 0.0;
+                for (int w = 0; w < EIG_WAVES; w++) {
+                    sr += red[w * RS];
+                    si += red[w * RS + 1];
+                }
+                const T al = (-0.5) * (tau * mk<T>(sr, si));
+#pragma unroll
+                for (int m = 0; m < PANEL_MR; m++) {
+                    const int r = k + tid + EIG_THREADS * m;
+                    if (r >= k + 1 && r < N) Wp[(size_t)j * N + r] = ps[m] + al * vs[m];
+                }
+                __syncthreads(); /* (5) W visible; red, part free */
+            }
+            /* the panel's update of the trailing matrix (columns c0.., rows >= column):
+             * 64-column chunks, their conj(W), conj(V) rows staged in LDS; wave w takes
+             * 16 columns of a chunk over every row block, its rows' V, W in registers */
+            const int c0 = k0 + nb;
+            for (int cb = c0; cb < N; cb += 64) {
+                for (int idx = tid; idx < 64 * NV; idx += EIG_THREADS) {
+                    const int cl = idx / NV, i = idx % NV, c = cb + cl;
+                    T val = zero<T>();
+                    if (c < N) {
+                        if (i < NBP) {
+                            if (i < nb) val = cj(Wp[(size_t)i * N + c]);
+                        } else if (i - NBP < nb) {
+                            val = cj(Vp[(size_t)(i - NBP) * N + c]);
+                        }
+                    }
+                    stage[idx] = val;
+                }
+                __syncthreads();
+                const int cw0 = cb + PANEL_UC * wave;
+                if (cw0 < N) {
+                    for (int rb = cb; rb < N; rb += 64) {
+                        const int r = rb + lane;
+                        if (rb + 63 < cw0) continue; /* every row above these columns */
+                        T vr[NBP], wr[NBP];
+#pragma unroll
+                        for (int i = 0; i < NBP; i++) {
+                            vr[i] = (i < nb && r < N) ? Vp[(size_t)i * N + r] : zero<T>();
+                            wr[i] = (i < nb && r < N) ? Wp[(size_t)i * N + r] : zero<T>();
+                        }
+                        T av[PANEL_UC];
+#pragma unroll
+                        for (int q = 0; q < PANEL_UC; q++) {
+                            const int c = cw0 + q;
+                            av[q] = (c < N && r >= c && r < N) ? A[pk(r, c, N)] : zero<T>();
+                        }
+#pragma unroll
+                        for (int q = 0; q < PANEL_UC; q++) {
+                            const int c = cw0 + q;
+                            if (c < N && r >= c && r < N) {
+                                const T *st = stage + (size_t)(PANEL_UC * wave + q) * NV;
+                                T x = av[q];
+#pragma unroll
+                                for (int i = 0; i < NBP; i++) x -= vr[i] * st[i] + wr[i] * st[NBP + i];
+                                A[pk(r, c, N)] = x;
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (tid == 0) {
+            d[N - 1] = re(A[pk(N - 1, N - 1, N)]);
+            e[N - 1] = 0.0;
+            if (tau_out) tau_out[N - 1] = zero<T>();
+        }
+        __syncthreads();
+        for (int i = tid; i < N; i += EIG_THREADS) {
+            a.dS[(size_t)i * a.nmat + mat] = d[i];
+            a.eS[(size_t)i * a.nmat + mat] = e[i];
+        }
+        if (tid == 0) {
+            double gl = d[0], gu = d[0], emax2 = 0.0;
+            for (int i = 0; i < N; i++) {
+                double off = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < N ? fabs(e[i]) : 0.0);
+                gl = fmin(gl, d[i] - off);
+                gu = fmax(gu, d[i] + off);
+                if (i + 1 < N) emax2 = fmax(emax2, e[i] * e[i]);
+            }
+            double tnorm = fmax(fabs(gl), fabs(gu));
+            gl -= 2.0 * DBL_EPSILON * tnorm * N + 1e-300;
+            gu += 2.0 * DBL_EPSILON * tnorm * N + 1e-300;
+            a.gb[4 * mat] = gl;
+            a.gb[4 * mat + 1] = gu;
+            a.gb[4 * mat + 2] = DBL_MIN * fmax(1.0, emax2);
+            a.gb[4 * mat + 3] = tnorm;
+        }
+        __syncthreads();
+    }
+}
+
 /* Real symmetric path with the full matrix in LDS (column-major, N <= 140 so
  * that N^2 doubles fit next to the work vectors): thread r owns row r, so the
  * mat-vec p_r = sum_c A[c][r] v_c and the rank-2 update of row r need no
@@ -2046,6 +2398,24 @@ template <class T, bool LDS> int launch_tridiag_fused(pmx_ctx *c, EigArgs &a, in
     return 0;
 }
 
+/* LDS of tridiag_panel_kernel: d, e, v, the per-wave mat-vec partials (or the
+ * update's staged columns), the reduction slots */
+size_t lds_panel_bytes(int N, size_t elem, int nbp) {
+    return sizeof(double) * 2 * (size_t)N + elem * (size_t)N +
+           elem * std::max((size_t)N * EIG_WAVES, (size_t)64 * 2 * nbp) + sizeof(double) * EIG_WAVES * (1 + 4 * nbp);
+}
+/* slab bytes the panel kernel keeps after the matrix: V and W, [NBP][N] each */
+size_t panel_slab_bytes(int N, size_t elem, int nbp) { return elem * 2 * (size_t)nbp * N; }
+
+template <class T, int NBP, int NCOL, int TU> int launch_tridiag_panel(pmx_ctx *c, EigArgs &a, int grid, size_t lds) {
+    auto kern = tridiag_panel_kernel<T, NBP, NCOL, TU>;
+    if (lds > 65536)
+        PMX_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(EIG_THREADS), lds, c->stream, a);
+    PMX_CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
 /* Autotuner for the tridiagonalisation: several kernels reduce the same
  * matrices to the same tridiagonal form (to rounding), and which is fastest
  * depends on N, the element type and how much of the matrices the caches
@@ -2144,6 +2514,26 @@ int tridiag_generic(pmx_ctx *c, EigArgs &a, bool real, bool lds_matrix, int grid
                              return lds_matrix ? launch_tridiag_fused<zd, true>(c, aa, g, ldsf)
                                                : launch_tridiag_fused<zd, false>(c, aa, g, ldsf);
                          }});
+    /* matrices in the slab: the blocked form, in panels of 8 or 4 columns */
+    if (!lds_matrix) {
+        const size_t elem = real ? sizeof(double) : sizeof(zd);
+        const size_t P = (size_t)N * (N + 1) / 2, mat_bytes = ((P * elem + 255) / 256) * 256;
+        auto add_panel = [&](const char *name, int nbp, auto launch_r, auto launch_c) {
+            const size_t ldsp = lds_panel_bytes(N, elem, nbp);
+            if (ldsp > c->lds_max || a.slab < mat_bytes + panel_slab_bytes(N, elem, nbp) || N > EIG_THREADS * PANEL_MR)
+                return;
+            cands.push_back({name, [=](int nm) {
+                                 EigArgs aa = a;
+                                 aa.nmat = nm;
+                                 const int g = std::min(grid, nm);
+                                 return real ? launch_r(c, aa, g, ldsp) : launch_c(c, aa, g, ldsp);
+                             }});
+        };
+        add_panel("panel8", 8, launch_tridiag_panel<double, 8, 2, 8>, launch_tridiag_panel<zd, 8, 2, 8>);
+        add_panel("panel8c4", 8, launch_tridiag_panel<double, 8, 4, 4>, launch_tridiag_panel<zd, 8, 4, 4>);
+        add_panel("panel8c8", 8, launch_tridiag_panel<double, 8, 8, 2>, launch_tridiag_panel<zd, 8, 8, 2>);
+        add_panel("panel4", 4, launch_tridiag_panel<double, 4, 2, 8>, launch_tridiag_panel<zd, 4, 2, 8>);
+    }
     /* complex bases whose lower triangle fits one workgroup's registers
      * (N <= 132): the register-tiled kernel with 6x6 complex tiles */
     if (!real) {
@@ -2278,7 +2668,11 @@ static int eig_device_run(pmx_ctx *c, int nmat, const double *dKpts, int nb, int
     const size_t per_mat = (nvec > 0 ? P * elem + N * elem : 0) + sizeof(double) * (size_t)N * (2 + nb) +
                            sizeof(double) * (size_t)nvec * N * 4 + 64;
     int chunk = (int)std::max<size_t>(64, std::min<size_t>((size_t)nmat, (size_t)12e9 / per_mat));
-    const size_t slab = lds_matrix ? 0 : ((P * elem + 255) / 256) * 256 + (pg ? elem * (size_t)N * (EIG_WAVES + 1) : 0);
+    /* the matrix, then the partials and tau of the large-basis form or the
+     * panel vectors of the blocked form */
+    const size_t slab = lds_matrix ? 0
+                                   : ((P * elem + 255) / 256) * 256 +
+                                         (pg ? elem * (size_t)N * (EIG_WAVES + 1) : panel_slab_bytes(N, elem, 8));
     /* the slabs of all resident workgroups stay within about 8 GB */
     int grid_max = c->num_cu * (lds_matrix ? 1 : 2) * 2;
     if (slab > 0) grid_max = (int)std::max<size_t>(64, std::min<size_t>(grid_max, (size_t)8e9 / slab));
