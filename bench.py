@@ -350,7 +350,9 @@ def run_config(opt):
     t_64, prof_64 = None, {}
     if opt.fp64_steps > 0:
         print("%s: FP64 step" % opt.config, file=sys.stderr, flush=True)
-        t_64, prof_64 = run_steps(ctx, hip, ctx.chi_params(*common, "fp64"), dre, dim, opt.fp64_steps, 0)
+        # as many untimed steps as the mixed path had, so that the FP64 kernels'
+        # autotuner samples fall outside the timed step as the mixed ones do
+        t_64, prof_64 = run_steps(ctx, hip, ctx.chi_params(*common, "fp64"), dre, dim, opt.fp64_steps, opt.warmup)
     re_d, im_d = eps_block(hip, dre, dim, nq * ne2 * cfg["nfreq"])
     ksum = None
     if t_64 is not None:

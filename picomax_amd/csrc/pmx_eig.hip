@@ -2514,8 +2514,11 @@ int tridiag_generic(pmx_ctx *c, EigArgs &a, bool real, bool lds_matrix, int grid
                              return lds_matrix ? launch_tridiag_fused<zd, true>(c, aa, g, ldsf)
                                                : launch_tridiag_fused<zd, false>(c, aa, g, ldsf);
                          }});
-    /* matrices in the slab: the blocked form, in panels of 8 or 4 columns */
+    /* matrices in the slab: the blocked form, in panels of 8 or 4 columns,
+     * first in the list (the default where the batch is too small to time) */
     if (!lds_matrix) {
+        std::vector<TuneCand> unblocked;
+        unblocked.swap(cands);
         const size_t elem = real ? sizeof(double) : sizeof(zd);
         const size_t P = (size_t)N * (N + 1) / 2, mat_bytes = ((P * elem + 255) / 256) * 256;
         auto add_panel = [&](const char *name, int nbp, auto launch_r, auto launch_c) {
@@ -2529,10 +2532,10 @@ int tridiag_generic(pmx_ctx *c, EigArgs &a, bool real, bool lds_matrix, int grid
                                  return real ? launch_r(c, aa, g, ldsp) : launch_c(c, aa, g, ldsp);
                              }});
         };
-        add_panel("panel8", 8, launch_tridiag_panel<double, 8, 2, 8>, launch_tridiag_panel<zd, 8, 2, 8>);
         add_panel("panel8c4", 8, launch_tridiag_panel<double, 8, 4, 4>, launch_tridiag_panel<zd, 8, 4, 4>);
-        add_panel("panel8c8", 8, launch_tridiag_panel<double, 8, 8, 2>, launch_tridiag_panel<zd, 8, 8, 2>);
+        add_panel("panel8", 8, launch_tridiag_panel<double, 8, 2, 8>, launch_tridiag_panel<zd, 8, 2, 8>);
         add_panel("panel4", 4, launch_tridiag_panel<double, 4, 2, 8>, launch_tridiag_panel<zd, 4, 2, 8>);
+        cands.insert(cands.end(), unblocked.begin(), unblocked.end());
     }
     /* complex bases whose lower triangle fits one workgroup's registers
      * (N <= 132): the register-tiled kernel with 6x6 complex tiles */

@@ -256,13 +256,13 @@ def test_tridiag_forms_against_oracle(manifest, case, form, monkeypatch):
         c.close()
 
 
-@pytest.mark.parametrize("form", ["fused", "three_stream", "panel8", "panel4"])
+@pytest.mark.parametrize("form", ["fused", "three_stream", "panel8c4", "panel8", "panel4"])
 @pytest.mark.parametrize("case", ["wz_401", "te_soc_582"])
 def test_tridiag_forms_large_basis(case, form, monkeypatch):
     """Every form with the matrix in global memory (wz, 401 plane waves; Te
     with spin-orbit maps, 582 x 582 spinor H): the unblocked kernels with
     three or two matrix streams per step and the blocked kernel in panels
-    of 8 or 4 columns."""
+    of 8 or 4 columns, its mat-vec walking 4 or 2 columns per wave."""
     monkeypatch.setenv("PMX_FORCE_KERNEL", form)
     args = LARGE[case]
     _say("tridiag form %s %s" % (form, case))
