@@ -1802,6 +1802,7 @@ __global__ void vrange_kernel(int nmat, int nvec, int vsel, double voff, const d
  * members each iteration).  Each solve of (T - lambda I) x = y is one forward
  * pass of LU with partial pivoting fused with the forward substitution, then
  * the back substitution; all per-thread arrays are SoA [.][N][slot]. */
+constexpr int IB = 8; /* rows per block of loads in inverse iteration */
 __global__ void invit_kernel(int N, int nmat, int nvec, const double *dS, const double *eS, const double *gb,
                              const double *lamS, const int *vr, double *Y, double *work) {
     const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -1834,41 +1835,72 @@ __global__ void invit_kernel(int N, int nmat, int nvec, const double *dS, const 
          * (Gram-Schmidt between them). */
         double bb = 0.0;
         for (int it = 0; it < 2; it++) {
-            /* forward: factor T - lm I with partial pivoting, transform y */
+            /* forward: factor T - lm I with partial pivoting, transform y.
+             * Rows come in blocks of IB whose loads are issued together, so a
+             * pass waits for memory N/IB times instead of N; the arithmetic is
+             * row by row as written. */
             double cd = D(0) - lm, cdu = (N > 1) ? E(0) : 0.0, cb = it ? Yv(0) : hash_unit(jj + 1, 1);
             if (!it) bb = cb * cb;
-            for (int i = 0; i + 1 < N; i++) {
-                const double ndl = E(i), nd = D(i + 1) - lm, ndu = (i + 2 < N) ? E(i + 1) : 0.0;
-                const double nb_ = it ? Yv(i + 1) : hash_unit(jj + 1, i + 2);
-                if (!it) bb = fma(nb_, nb_, bb);
-                if (fabs(cd) >= fabs(ndl)) {
-                    if (cd == 0.0) cd = eps_piv;
-                    const double f = ndl / cd;
-                    at(dd, i) = cd; at(du, i) = cdu; at(du2, i) = 0.0; Yv(i) = cb;
-                    cd = nd - f * cdu;
-                    cdu = ndu;
-                    cb = nb_ - f * cb;
-                } else {
-                    const double f = cd / ndl;
-                    at(dd, i) = ndl; at(du, i) = nd; at(du2, i) = ndu; Yv(i) = nb_;
-                    const double ncd = cdu - f * nd, ncdu = -f * ndu, ncb = cb - f * nb_;
-                    cd = ncd;
-                    cdu = ncdu;
-                    cb = ncb;
+            for (int i0 = 0; i0 + 1 < N; i0 += IB) {
+                double ndl[IB], nd[IB], ndu[IB], nbv[IB];
+#pragma unroll
+                for (int u = 0; u < IB; u++) {
+                    const int i = i0 + u;
+                    const bool ok = i + 1 < N;
+                    ndl[u] = ok ? E(i) : 0.0;
+                    nd[u] = ok ? D(i + 1) : 0.0;
+                    ndu[u] = (i + 2 < N) ? E(i + 1) : 0.0;
+                    nbv[u] = ok ? (it ? Yv(i + 1) : hash_unit(jj + 1, i + 2)) : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < IB; u++) {
+                    const int i = i0 + u;
+                    if (i + 1 >= N) break;
+                    const double nd_ = nd[u] - lm, nb_ = nbv[u];
+                    if (!it) bb = fma(nb_, nb_, bb);
+                    if (fabs(cd) >= fabs(ndl[u])) {
+                        if (cd == 0.0) cd = eps_piv;
+                        const double f = ndl[u] / cd;
+                        at(dd, i) = cd; at(du, i) = cdu; at(du2, i) = 0.0; Yv(i) = cb;
+                        cd = nd_ - f * cdu;
+                        cdu = ndu[u];
+                        cb = nb_ - f * cb;
+                    } else {
+                        const double f = cd / ndl[u];
+                        at(dd, i) = ndl[u]; at(du, i) = nd_; at(du2, i) = ndu[u]; Yv(i) = nb_;
+                        const double ncd = cdu - f * nd_, ncdu = -f * ndu[u], ncb = cb - f * nb_;
+                        cd = ncd;
+                        cdu = ncdu;
+                        cb = ncb;
+                    }
                 }
             }
             if (cd == 0.0) cd = eps_piv;
             at(dd, N - 1) = cd;
-            /* back substitution, accumulating |x|^2 */
+            /* back substitution, accumulating |x|^2, in blocks of IB rows likewise */
             double x2 = 0.0, x1 = cb / cd;
             Yv(N - 1) = x1;
             double nn = x1 * x1;
-            for (int i = N - 2; i >= 0; i--) {
-                const double x = (Yv(i) - at(du, i) * x1 - at(du2, i) * x2) / at(dd, i);
-                Yv(i) = x;
-                nn += x * x;
-                x2 = x1;
-                x1 = x;
+            for (int i0 = N - 2; i0 >= 0; i0 -= IB) {
+                double yv[IB], ud[IB], ud2[IB], dg[IB];
+#pragma unroll
+                for (int u = 0; u < IB; u++) {
+                    const int i = max(i0 - u, 0);
+                    yv[u] = Yv(i);
+                    ud[u] = at(du, i);
+                    ud2[u] = at(du2, i);
+                    dg[u] = at(dd, i);
+                }
+#pragma unroll
+                for (int u = 0; u < IB; u++) {
+                    const int i = i0 - u;
+                    if (i < 0) break;
+                    const double x = (yv[u] - ud[u] * x1 - ud2[u] * x2) / dg[u];
+                    Yv(i) = x;
+                    nn += x * x;
+                    x2 = x1;
+                    x1 = x;
+                }
             }
             /* orthogonalise against the cluster's earlier vectors */
             if (jj > j) {
@@ -1896,11 +1928,171 @@ __global__ void invit_kernel(int N, int nmat, int nvec, const double *dS, const 
             const double rtol = 4.0 * N * DBL_EPSILON * tnorm;
             const bool done = !it && jj == j && (huge || bb <= nn * rtol * rtol);
             const double sc2 = 1.0 / sqrt(nn);
-            for (int i = 0; i < N; i++) Yv(i) *= sc2;
+            for (int i0 = 0; i0 < N; i0 += IB) {
+                double yv[IB];
+#pragma unroll
+                for (int u = 0; u < IB; u++) yv[u] = Yv(min(i0 + u, N - 1));
+#pragma unroll
+                for (int u = 0; u < IB; u++)
+                    if (i0 + u < N) Yv(i0 + u) = yv[u] * sc2;
+            }
             if (done) break;
         }
     }
 }
+
+/* invit_kernel without the factor scratch: the LU factors of T - lambda I
+ * are not stored.  The forward pass keeps only the elimination state (cd,
+ * cdu, cb) at the start of every S-row segment (4 doubles per S rows in the
+ * SoA scratch, against 3 per row), and the back substitution recomputes each
+ * segment's factors from its checkpoint, in registers, just before it needs
+ * them.  The recomputation repeats the forward pass's arithmetic exactly, so
+ * the vectors are bit-identical to invit_kernel's; the price is a second
+ * forward pass over D and E, the saving the factor round trip through HBM.
+ * The segment after the current one has been overwritten by then, so the
+ * right-hand side of each segment's last row is kept with its checkpoint. */
+template <int S>
+__global__ void __launch_bounds__(256) invit_ck_kernel(int N, int nmat, int nvec, const double *dS, const double *eS,
+                                                       const double *gb, const double *lamS, const int *vr, double *Y,
+                                                       double *work) {
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (t >= (size_t)nmat * nvec) return;
+    const int mat = t % nmat, j = t / nmat;
+    const double tnorm = gb[4 * mat + 3];
+    const double ortol = 1e-5 * tnorm;
+    auto lam = [&](int jj) { return lamS[(size_t)jj * nmat + mat]; };
+    if (j > 0 && lam(j) - lam(j - 1) <= ortol) return; /* not a cluster head */
+    int jend = j + 1;
+    while (jend < nvec && lam(jend) - lam(jend - 1) <= ortol) jend++;
+    if (vr && (jend <= vr[2 * mat] || j >= vr[2 * mat] + vr[2 * mat + 1])) return; /* no wanted vector */
+    auto D = [&](int i) { return dS[(size_t)i * nmat + mat]; };
+    auto E = [&](int i) { return eS[(size_t)i * nmat + mat]; };
+    const double eps_piv = DBL_EPSILON * tnorm;
+    for (int jj = j; jj < jend; jj++) {
+        double *y = Y + (size_t)jj * N * nmat + mat; /* element i at y[i * nmat] */
+        auto Yv = [&](int i) -> double & { return y[(size_t)i * nmat]; };
+        const double lm = lam(jj);
+        double bb = 0.0;
+        for (int it = 0; it < 2; it++) {
+            /* one segment of the forward pass from state (cd, cdu, cb): rows
+             * i0 .. i0+S-1 (those with i + 1 < N); the right-hand side of row
+             * i0 + S is rhs_last (it > 0) when given, and the factors go to
+             * f_* when wanted */
+            auto segment = [&](int i0, double &cd, double &cdu, double &cb, bool use_last, double rhs_last,
+                               double *f_dd, double *f_du, double *f_du2, double *f_y, bool acc_bb) {
+                double ndl[S], nd[S], ndu[S], nbv[S];
+#pragma unroll
+                for (int u = 0; u < S; u++) {
+                    const int i = i0 + u;
+                    const bool ok = i + 1 < N;
+                    ndl[u] = ok ? E(i) : 0.0;
+                    nd[u] = ok ? D(i + 1) : 0.0;
+                    ndu[u] = (i + 2 < N) ? E(i + 1) : 0.0;
+                    if (it == 0)
+                        nbv[u] = ok ? hash_unit(jj + 1, i + 2) : 0.0;
+                    else if (u == S - 1 && use_last)
+                        nbv[u] = rhs_last;
+                    else
+                        nbv[u] = ok ? Yv(i + 1) : 0.0;
+                }
+                /* the two pivot cases by selects, with one division: lanes
+                 * that pivot differently do not run both branches */
+#pragma unroll
+                for (int u = 0; u < S; u++) {
+                    const int i = i0 + u;
+                    if (i + 1 >= N) break;
+                    const double nd_ = nd[u] - lm, nb_ = nbv[u];
+                    if (acc_bb) bb = fma(nb_, nb_, bb);
+                    const bool keep = fabs(cd) >= fabs(ndl[u]); /* no row interchange */
+                    if (keep && cd == 0.0) cd = eps_piv;
+                    const double f = (keep ? ndl[u] : cd) / (keep ? cd : ndl[u]);
+                    if (f_dd) {
+                        f_dd[u] = keep ? cd : ndl[u];
+                        f_du[u] = keep ? cdu : nd_;
+                        f_du2[u] = keep ? 0.0 : ndu[u];
+                        f_y[u] = keep ? cb : nb_;
+                    }
+                    const double a0 = keep ? nd_ : cdu, b0 = keep ? cdu : nd_;
+                    const double a1 = keep ? nb_ : cb, b1 = keep ? cb : nb_;
+                    cd = a0 - f * b0;
+                    cdu = keep ? ndu[u] : -f * ndu[u];
+                    cb = a1 - f * b1;
+                }
+                return nbv[S - 1];
+            };
+            /* checkpoint q: [4][nseg] SoA slots of this thread */
+            const size_t slots = (size_t)nmat * nvec;
+            auto ck = [&](int q, int c) -> double & { return work[((size_t)q * 4 + c) * slots + t]; };
+            double cd = D(0) - lm, cdu = (N > 1) ? E(0) : 0.0, cb = it ? Yv(0) : hash_unit(jj + 1, 1);
+            if (!it) bb = cb * cb;
+            const int nseg = (N - 1 + S - 1) / S; /* segments holding rows 0 .. N-2 */
+            for (int sg = 0; sg < nseg; sg++) {
+                ck(sg, 0) = cd;
+                ck(sg, 1) = cdu;
+                ck(sg, 2) = cb;
+                ck(sg, 3) = segment(sg * S, cd, cdu, cb, false, 0.0, nullptr, nullptr, nullptr, nullptr, !it);
+            }
+            if (cd == 0.0) cd = eps_piv;
+            /* back substitution, a segment at a time from the last */
+            double x2 = 0.0, x1 = cb / cd;
+            Yv(N - 1) = x1;
+            double nn = x1 * x1;
+            for (int sg = nseg - 1; sg >= 0; sg--) {
+                double scd = ck(sg, 0), scdu = ck(sg, 1), scb = ck(sg, 2);
+                const double srhs = ck(sg, 3);
+                double f_dd[S], f_du[S], f_du2[S], f_y[S];
+                const int i0 = sg * S;
+                segment(i0, scd, scdu, scb, true, srhs, f_dd, f_du, f_du2, f_y, false);
+#pragma unroll
+                for (int u = S - 1; u >= 0; u--) {
+                    const int i = i0 + u;
+                    if (i > N - 2) continue;
+                    const double x = (f_y[u] - f_du[u] * x1 - f_du2[u] * x2) / f_dd[u];
+                    Yv(i) = x;
+                    nn += x * x;
+                    x2 = x1;
+                    x1 = x;
+                }
+            }
+            /* orthogonalise against the cluster's earlier vectors */
+            if (jj > j) {
+                for (int pj = j; pj < jj; pj++) {
+                    const double *yp = Y + (size_t)pj * N * nmat + mat;
+                    double s2 = 0.0;
+                    for (int i = 0; i < N; i++) s2 += yp[(size_t)i * nmat] * Yv(i);
+                    for (int i = 0; i < N; i++) Yv(i) -= s2 * yp[(size_t)i * nmat];
+                }
+                nn = 0.0;
+                for (int i = 0; i < N; i++) nn += Yv(i) * Yv(i);
+            }
+            const bool huge = !(nn < 1e280); /* growth beyond any residual test */
+            if (huge) { /* rescale by the largest entry first */
+                double mx = 0.0;
+                for (int i = 0; i < N; i++) mx = fmax(mx, fabs(Yv(i)));
+                const double sc = 1.0 / mx;
+                nn = 0.0;
+                for (int i = 0; i < N; i++) {
+                    const double v = Yv(i) * sc;
+                    Yv(i) = v;
+                    nn += v * v;
+                }
+            }
+            const double rtol = 4.0 * N * DBL_EPSILON * tnorm;
+            const bool done = !it && jj == j && (huge || bb <= nn * rtol * rtol);
+            const double sc2 = 1.0 / sqrt(nn);
+            for (int i0 = 0; i0 < N; i0 += IB) {
+                double yv[IB];
+#pragma unroll
+                for (int u = 0; u < IB; u++) yv[u] = Yv(min(i0 + u, N - 1));
+#pragma unroll
+                for (int u = 0; u < IB; u++)
+                    if (i0 + u < N) Yv(i0 + u) = yv[u] * sc2;
+            }
+            if (done) break;
+        }
+    }
+}
+constexpr int INVIT_S = 8; /* rows per checkpointed segment */
 
 /* z_j = H_0 H_1 ... H_{N-2} y_j, one workgroup per matrix.  Vectors are
  * processed in groups of gsz (<= 32) held in LDS; the reflectors are staged
@@ -2783,8 +2975,14 @@ static int eig_device_run(pmx_ctx *c, int nmat, const double *dKpts, int nb, int
                     hipLaunchKernelGGL(vrange_kernel, dim3((nm + 255) / 256), dim3(256), 0, c->stream, nm, nvec, vsel,
                                        voff, lamS, vr, m0, nsplit, vsel2);
                 const size_t nt = (size_t)nm * nvec;
-                hipLaunchKernelGGL(invit_kernel, dim3((nt + 255) / 256), dim3(256), 0, c->stream, N, nm, nvec, dS, eS,
-                                   gb, lamS, vr, Y, work);
+                /* the checkpointed form (PMX_INVIT=scratch: the stored-factor form) */
+                const char *fi = std::getenv("PMX_INVIT");
+                if (!(fi && std::strcmp(fi, "scratch") == 0))
+                    hipLaunchKernelGGL((invit_ck_kernel<INVIT_S>), dim3((nt + 255) / 256), dim3(256), 0, c->stream, N,
+                                       nm, nvec, dS, eS, gb, lamS, vr, Y, work);
+                else
+                    hipLaunchKernelGGL(invit_kernel, dim3((nt + 255) / 256), dim3(256), 0, c->stream, N, nm, nvec, dS,
+                                       eS, gb, lamS, vr, Y, work);
                 PMX_CHECK_HIP(hipGetLastError());
             }
             {
