@@ -1951,8 +1951,8 @@ __global__ void invit_kernel(int N, int nmat, int nvec, const double *dS, const 
  * forward pass over D and E, the saving the factor round trip through HBM.
  * The segment after the current one has been overwritten by then, so the
  * right-hand side of each segment's last row is kept with its checkpoint. */
-template <int S>
-__global__ void __launch_bounds__(256) invit_ck_kernel(int N, int nmat, int nvec, const double *dS, const double *eS,
+template <int S, int OCC>
+__global__ void __launch_bounds__(256, OCC) invit_ck_kernel(int N, int nmat, int nvec, const double *dS, const double *eS,
                                                        const double *gb, const double *lamS, const int *vr, double *Y,
                                                        double *work) {
     const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -2092,7 +2092,8 @@ __global__ void __launch_bounds__(256) invit_ck_kernel(int N, int nmat, int nvec
         }
     }
 }
-constexpr int INVIT_S = 8; /* rows per checkpointed segment */
+constexpr int INVIT_S = 8; /* rows per checkpointed segment; three waves per SIMD (C2: 1.19 -> 1.09 ms;
+                              4-row segments at four waves per SIMD: 1.19) */
 
 /* z_j = H_0 H_1 ... H_{N-2} y_j, one workgroup per matrix.  Vectors are
  * processed in groups of gsz (<= 32) held in LDS; the reflectors are staged
@@ -2978,8 +2979,8 @@ static int eig_device_run(pmx_ctx *c, int nmat, const double *dKpts, int nb, int
                 /* the checkpointed form (PMX_INVIT=scratch: the stored-factor form) */
                 const char *fi = std::getenv("PMX_INVIT");
                 if (!(fi && std::strcmp(fi, "scratch") == 0))
-                    hipLaunchKernelGGL((invit_ck_kernel<INVIT_S>), dim3((nt + 255) / 256), dim3(256), 0, c->stream, N,
-                                       nm, nvec, dS, eS, gb, lamS, vr, Y, work);
+                    hipLaunchKernelGGL((invit_ck_kernel<INVIT_S, 3>), dim3((nt + 255) / 256), dim3(256), 0, c->stream,
+                                       N, nm, nvec, dS, eS, gb, lamS, vr, Y, work);
                 else
                     hipLaunchKernelGGL(invit_kernel, dim3((nt + 255) / 256), dim3(256), 0, c->stream, N, nm, nvec, dS,
                                        eS, gb, lamS, vr, Y, work);
