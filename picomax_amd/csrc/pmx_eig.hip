@@ -2033,9 +2033,11 @@ __global__ void __launch_bounds__(256, OCC) invit_ck_kernel(int N, int nmat, int
                 ck(sg, 3) = segment(sg * S, cd, cdu, cb, false, 0.0, nullptr, nullptr, nullptr, nullptr, !it);
             }
             if (cd == 0.0) cd = eps_piv;
-            /* back substitution, a segment at a time from the last */
+            /* back substitution, a segment at a time from the last; row N-1's
+             * x is stored after the loop, since the last segment's
+             * recomputation reads that row's right-hand side */
             double x2 = 0.0, x1 = cb / cd;
-            Yv(N - 1) = x1;
+            const double x_end = x1;
             double nn = x1 * x1;
             for (int sg = nseg - 1; sg >= 0; sg--) {
                 double scd = ck(sg, 0), scdu = ck(sg, 1), scb = ck(sg, 2);
@@ -2054,6 +2056,7 @@ __global__ void __launch_bounds__(256, OCC) invit_ck_kernel(int N, int nmat, int
                     x1 = x;
                 }
             }
+            Yv(N - 1) = x_end;
             /* orthogonalise against the cluster's earlier vectors */
             if (jj > j) {
                 for (int pj = j; pj < jj; pj++) {
