@@ -79,10 +79,6 @@ __device__ __forceinline__ zd cdiv(zd a, zd b) {
 }
 __device__ __forceinline__ double cdiv(double a, double b) { return a / b; }
 
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 /* Sum over a full wave (every lane active) by DPP within rows of 16 and
  * four lane reads; every lane gets the same bits, in every wave alike. */
@@ -236,7 +232,7 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_kernel(EigArgs a) {
             const int L = N - k - 1;
             double s = 0.0;
             for (int r = k + 2 + tid; r < N; r += EIG_THREADS) s += nrm2(A[pk(r, k, N)]);
-            s = wave_sum(s);
+            s = wave_sum_full(s);
             if (lane == 0) red[wave] = s;
             __syncthreads();
             if (tid == 0) {
@@ -298,8 +294,8 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_kernel(EigArgs a) {
                         }
                     }
                 }
-                double dr = wave_sum(re(dot));
-                double di = wave_sum(im(dot));
+                double dr = wave_sum_full(re(dot));
+                double di = wave_sum_full(im(dot));
                 if (lane == 0) {
                     T t;
                     if constexpr (sizeof(T) == sizeof(double)) t = dr; else t = zd{dr, di};
@@ -317,8 +313,8 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_kernel(EigArgs a) {
                 pr += re(cv);
                 pi += im(cv);
             }
-            pr = wave_sum(pr);
-            pi = wave_sum(pi);
+            pr = wave_sum_full(pr);
+            pi = wave_sum_full(pi);
             if (lane == 0) {
                 red[wave] = pr;
                 red[EIG_WAVES + wave] = pi;
@@ -451,7 +447,7 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_fused_kernel(EigArgs a) {
             /* (b) reflector of column k */
             double s = 0.0;
             for (int r = k + 2 + tid; r < N; r += EIG_THREADS) s += nrm2(A[pk(r, k, N)]);
-            s = wave_sum(s);
+            s = wave_sum_full(s);
             if (lane == 0) red[wave] = s;
             __syncthreads();
             if (tid == 0) {
@@ -527,8 +523,8 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_fused_kernel(EigArgs a) {
                         }
                     }
                 }
-                double dr = wave_sum(re(dot));
-                double di = wave_sum(im(dot));
+                double dr = wave_sum_full(re(dot));
+                double di = wave_sum_full(im(dot));
                 if (lane == 0) {
                     T t;
                     if constexpr (sizeof(T) == sizeof(double)) t = dr; else t = zd{dr, di};
@@ -547,8 +543,8 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_fused_kernel(EigArgs a) {
                 pr += re(cv);
                 pi += im(cv);
             }
-            pr = wave_sum(pr);
-            pi = wave_sum(pi);
+            pr = wave_sum_full(pr);
+            pi = wave_sum_full(pi);
             if (lane == 0) {
                 red[wave] = pr;
                 red[EIG_WAVES + wave] = pi;
@@ -981,7 +977,7 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_full_real_kernel(EigArgs 
             /* ||x[1:]||^2 with x = column k below the diagonal */
             const double xr = (r < N) ? A[(size_t)k * N + r] : 0.0;
             double s2 = (r >= k + 2 && r < N) ? xr * xr : 0.0;
-            s2 = wave_sum(s2);
+            s2 = wave_sum_full(s2);
             if (lane == 0) red[wave] = s2;
             __syncthreads(); /* (1) */
             double xn2 = 0.0;
@@ -1036,7 +1032,7 @@ __global__ void __launch_bounds__(EIG_THREADS) tridiag_full_real_kernel(EigArgs 
                 for (; c < N; c++, col += N) p0 += col[0] * vv[c];
                 pr = tau * ((p0 + p1) + (p2 + p3));
             }
-            double pv = wave_sum(pr * vr);
+            double pv = wave_sum_full(pr * vr);
             if (lane == 0) red[EIG_WAVES + wave] = pv;
             __syncthreads(); /* (3) */
             double pvs = 0.0;
@@ -2163,8 +2159,8 @@ __global__ void __launch_bounds__(EIG_THREADS) backtrans_kernel(int N, int nmat,
 #pragma unroll
                     for (int u = 0; u < BT_MAXV; u++) {
                         if (wave + EIG_WAVES * u < ng) { /* wave-uniform */
-                            sr[u] = wave_sum(sr[u]);
-                            if constexpr (sizeof(T) != sizeof(double)) si[u] = wave_sum(si[u]);
+                            sr[u] = wave_sum_full(sr[u]);
+                            if constexpr (sizeof(T) != sizeof(double)) si[u] = wave_sum_full(si[u]);
                         }
                     }
                     for (int i = lane; i < L; i += 64) {
