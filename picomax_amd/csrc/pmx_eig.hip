@@ -2544,6 +2544,212 @@ __global__ void __launch_bounds__(256, 2) backtrans_wyr_kernel(int N, int nmat, 
             }
 }
 
+/* The compact-WY back-transformation of the complex path (Hermitian H
+ * reduced to real T): z = H_0 H_1 ... H_{N-2} y with H_k = I - tau_k v_k v_k^H,
+ * Z in registers.  One workgroup (NW waves) per (matrix, group of 16 wanted
+ * vectors); row tile rt of Z (16 rows, real and imaginary planes) belongs to
+ * wave rt % NW in the FP64 MFMA accumulator layout (lane l, value v: row
+ * 16 rt + l/16 + 4v, column l%16), which is also the B-operand layout of
+ * V^H Z.  Blocks of 16 reflectors run last to first; block b is zero above
+ * row 16b + 1, so only row tiles rt >= b take part.  Per block:
+ *   1. every wave: its tiles' share of W = V^H Z and G = V^H V, V read from
+ *      the packed reflectors, each complex product as four real MFMAs;
+ *      partials to LDS;                                             barrier
+ *   2. the last wave: G summed, then T from T_ii = tau_i, T_ji = -tau_i
+ *      sum_m T_jm G_mi in complex arithmetic, right-looking;        barrier
+ *   3. every wave: W = -(T sum_w W_w), then its tiles Z += V W.     barrier
+ * The same products as backtrans_kernel<zd>, grouped by blocks. */
+template <int MAXRT, int NW>
+__global__ void __launch_bounds__(64 * NW) backtrans_wyc_kernel(int N, int nmat, int nvec, const double *Y,
+                                                               const zd *vref_all, const zd *tau_all, const int *vr,
+                                                               zd *evecs) {
+    extern __shared__ double smem_d[];
+    const int Np = (N + 15) & ~15, nrt = Np / 16;
+    double *Wp = smem_d;            /* [NW waves][re, im][4][64] */
+    double *Gp = Wp + NW * 2 * 256; /* [NW waves][re, im][4][64] */
+    double *Gs = Gp + NW * 2 * 256; /* [re, im][16][VS]: the summed G */
+    double *Ts = Gs + 2 * 16 * VS; /* [re, im][16][VS] */
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, l16 = lane & 15, l4 = lane >> 4;
+    const int mat = xcd_remap(blockIdx.x, gridDim.x), c0 = 16 * blockIdx.y;
+    const size_t P = (size_t)N * (N + 1) / 2;
+    const zd *vref = vref_all + (size_t)mat * P;
+    const zd *tau = tau_all + (size_t)mat * N;
+    /* the wanted vectors j0 .. j0 + cnt - 1; this workgroup takes columns c0 .. c0 + 15 of them */
+    const int j0 = vr ? vr[2 * mat] : 0, cnt = vr ? vr[2 * mat + 1] : nvec;
+    if (c0 >= cnt) return;
+    const int nref = N - 1, nbw = (nref + 15) / 16;
+    /* element (r, k) of V: the unit entry at r = k + 1, the packed reflector below, zero above */
+    auto vel = [&](int r, int k) -> zd {
+        if (k >= nref || r <= k || r >= N) return zd{0.0, 0.0};
+        if (r == k + 1) return zd{1.0, 0.0};
+        return vref[pk(r, k, N)];
+    };
+    dx4 zr[MAXRT], zi[MAXRT];
+#pragma unroll
+    for (int q = 0; q < MAXRT; q++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = 16 * (wave + NW * q) + l4 + 4 * v, col = c0 + l16;
+            zr[q][v] = (row < N && col < cnt) ? Y[((size_t)(j0 + col) * N + row) * nmat + mat] : 0.0;
+            zi[q][v] = 0.0;
+        }
+    for (int b = nbw - 1; b >= 0; b--) {
+        /* 1. this wave's share of W = V^H Z and G = V^H V */
+        dx4 wr = {0.0, 0.0, 0.0, 0.0}, wi = wr, gr = wr, gi = wr;
+#pragma unroll
+        for (int q = 0; q < MAXRT; q++) {
+            const int rt = wave + NW * q;
+            if (rt < b || rt >= nrt) continue;
+            zd a[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) a[s] = vel(16 * rt + 4 * s + l4, 16 * b + l16);
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                /* A = conj(a) (row l%16 of V^H), B = Z or V (row 4s + l/16) */
+                wr = mma64(a[s].x, zr[q][s], wr);
+                wr = mma64(a[s].y, zi[q][s], wr);
+                wi = mma64(a[s].x, zi[q][s], wi);
+                wi = mma64(-a[s].y, zr[q][s], wi);
+                gr = mma64(a[s].x, a[s].x, gr);
+                gr = mma64(a[s].y, a[s].y, gr);
+                gi = mma64(a[s].x, a[s].y, gi);
+                gi = mma64(-a[s].y, a[s].x, gi);
+            }
+            /* one tile's loads at a time: Z's tiles already fill most registers */
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            Wp[((wave * 2 + 0) * 4 + v) * 64 + lane] = wr[v];
+            Wp[((wave * 2 + 1) * 4 + v) * 64 + lane] = wi[v];
+            Gp[((wave * 2 + 0) * 4 + v) * 64 + lane] = gr[v];
+            Gp[((wave * 2 + 1) * 4 + v) * 64 + lane] = gi[v];
+        }
+        __syncthreads(); /* (1) partials */
+        /* 2. T of block b on wave 3 */
+        if (wave == NW - 1) {
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    double g = 0.0;
+#pragma unroll
+                    for (int w = 0; w < NW; w++) g += Gp[((w * 2 + h) * 4 + v) * 64 + lane];
+                    Gs[(h * 16 + l4 + 4 * v) * VS + l16] = g;
+                    __builtin_amdgcn_sched_barrier(0); /* NW partials in flight at a time */
+                }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            /* right-looking over the columns m of T, all 64 lanes: lane
+             * (j = l%16, i-chunk l/16) keeps the sums for columns i = l/16 +
+             * 4u; at step m the lane owning (j, m) finishes T_jm into Ts,
+             * and every lane folds T_jm G_mi into its later columns */
+            const int j = l16;
+            double ar[4] = {0.0, 0.0, 0.0, 0.0}, ai[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int m = 0; m < 16; m++) {
+                const zd t = (16 * b + m < nref) ? tau[16 * b + m] : zd{0.0, 0.0};
+                if (l4 == (m & 3)) {
+                    const int u = m >> 2;
+                    double x = 0.0, y = 0.0;
+                    if (m == j) {
+                        x = t.x;
+                        y = t.y;
+                    } else if (m > j) {
+                        x = -(t.x * ar[0] - t.y * ai[0]);
+                        y = -(t.x * ai[0] + t.y * ar[0]);
+#pragma unroll
+                        for (int uu = 1; uu < 4; uu++)
+                            if (uu == u) {
+                                x = -(t.x * ar[uu] - t.y * ai[uu]);
+                                y = -(t.x * ai[uu] + t.y * ar[uu]);
+                            }
+                    }
+                    Ts[j * VS + m] = x;
+                    Ts[(16 + j) * VS + m] = y;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const double tjr = Ts[j * VS + m], tji = Ts[(16 + j) * VS + m];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int i = l4 + 4 * u;
+                    if (i > m) {
+                        const double g_r = Gs[m * VS + i], g_i = Gs[(16 + m) * VS + i];
+                        ar[u] = fma(tjr, g_r, fma(-tji, g_i, ar[u]));
+                        ai[u] = fma(tjr, g_i, fma(tji, g_r, ai[u]));
+                    }
+                }
+            }
+        }
+        __syncthreads(); /* (2) T */
+        /* 3. W = -(T sum_w W_w), then Z += V W on this wave's tiles */
+        dx4 sr_, si_;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            double x = 0.0, y = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                x += Wp[((w * 2 + 0) * 4 + v) * 64 + lane];
+                y += Wp[((w * 2 + 1) * 4 + v) * 64 + lane];
+            }
+            sr_[v] = x;
+            si_[v] = y;
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        dx4 twr = {0.0, 0.0, 0.0, 0.0}, twi = twr;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const double t_r = Ts[l16 * VS + 4 * s + l4], t_i = Ts[(16 + l16) * VS + 4 * s + l4];
+            twr = mma64(t_r, sr_[s], twr);
+            twr = mma64(-t_i, si_[s], twr);
+            twi = mma64(t_r, si_[s], twi);
+            twi = mma64(t_i, sr_[s], twi);
+        }
+        wr = -twr;
+        wi = -twi;
+#pragma unroll
+        for (int q = 0; q < MAXRT; q++) {
+            const int rt = wave + NW * q;
+            if (rt < b || rt >= nrt) continue;
+            zd a[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) a[s] = vel(16 * rt + l16, 16 * b + 4 * s + l4);
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                /* A = V (row l%16, reflector 4s + l/16), B = W (value s) */
+                zr[q] = mma64(a[s].x, wr[s], zr[q]);
+                zr[q] = mma64(-a[s].y, wi[s], zr[q]);
+                zi[q] = mma64(a[s].x, wi[s], zi[q]);
+                zi[q] = mma64(a[s].y, wr[s], zi[q]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads(); /* (3) partials and T free for reuse */
+    }
+#pragma unroll
+    for (int q = 0; q < MAXRT; q++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = 16 * (wave + NW * q) + l4 + 4 * v, col = c0 + l16;
+            if (row < N && col < cnt) evecs[((size_t)mat * nvec + j0 + col) * N + row] = zd{zr[q][v], zi[q][v]};
+        }
+}
+
+size_t wyc_lds_bytes(int nw) { return sizeof(double) * (2 * (size_t)nw * 2 * 256 + 2 * 2 * 16 * VS); }
+bool wyc_forced_off() {
+    const char *f = std::getenv("PMX_FORCE_KERNEL");
+    return f && std::strcmp(f, "bt_loop") == 0;
+}
+/* the built (row tiles per wave, waves) forms of the complex WY kernel: the
+ * smallest that holds Np = 16 nrt rows (MAXRT NW >= nrt), or 0 */
+int wyc_form(int N) {
+    const int nrt = ((N + 15) & ~15) / 16;
+    return nrt <= 16 ? 44 : (nrt <= 32 ? 48 : (nrt <= 40 ? 58 : (nrt <= 48 ? 68 : 0)));
+}
+bool wyc_usable(int N) { return N >= 2 && wyc_form(N) > 0 && !wyc_forced_off(); }
+
 size_t wyr_lds_bytes(int N, int nct) {
     const size_t Np = (N + 15) & ~15;
     return sizeof(double) * (2 * Np * VS + 4 * (size_t)nct * 256 + 4 * 256 + 2 * 16 * VS + 2 * 16);
@@ -2822,6 +3028,25 @@ template <class T> int launch_backtrans(pmx_ctx *c, int N, int nmat, int nvec, c
         return tune_run(c, "backtrans real N=" + std::to_string(N) + " nvec=" + std::to_string(nvec), nmat,
                         16 * c->num_cu, cands);
     }
+    /* complex: the compact-WY kernel with Z in registers where its row tiles
+     * fit (N <= 640), one workgroup per (matrix, 16 wanted vectors); else, or
+     * with PMX_FORCE_KERNEL=bt_loop, the reflector-at-a-time kernel below
+     * (every vector: it takes no selection) */
+    if (sizeof(T) != sizeof(double) && wyc_usable(N)) {
+        auto run = [&](auto kern, int nw) -> int {
+            const size_t lds = wyc_lds_bytes(nw);
+            hipLaunchKernelGGL(kern, dim3(nmat, (nvec + 15) / 16), dim3(64 * nw), lds, c->stream, N, nmat, nvec, Y,
+                               reinterpret_cast<const zd *>(vref), reinterpret_cast<const zd *>(tau), vr, evecs);
+            PMX_CHECK_HIP(hipGetLastError());
+            return 0;
+        };
+        switch (wyc_form(N)) {
+        case 44: return run(backtrans_wyc_kernel<4, 4>, 4);
+        case 48: return run(backtrans_wyc_kernel<4, 8>, 8);
+        case 58: return run(backtrans_wyc_kernel<5, 8>, 8);
+        default: return run(backtrans_wyc_kernel<6, 8>, 8);
+        }
+    }
     /* staging depth and vector group size that fit the LDS */
     const size_t cap = c->lds_max;
     int kc = 16;
@@ -2882,8 +3107,8 @@ static int eig_device_run(pmx_ctx *c, int nmat, const double *dKpts, int nb, int
     char *vref = reinterpret_cast<char *>(c->dEigRef.p);
     char *tauv = vref + cs * P * elem;
     const double n3 = (double)N * N * N, n2 = (double)N * N;
-    /* selected vectors only on the real path's blocked back-transformation */
-    if (!(real && wy_usable(c, N, nvec))) vsel = 0;
+    /* selected vectors only on the blocked back-transformations */
+    if (!(real ? wy_usable(c, N, nvec) : wyc_usable(N))) vsel = 0;
     if (nvec > 0 && vsel && (rc = c->dVrange.ensure(2 * cs))) return rc;
     int *vr = (nvec > 0 && vsel) ? c->dVrange.p : nullptr;
     for (int m0 = 0; m0 < nmat; m0 += chunk) {
@@ -2989,7 +3214,7 @@ static int eig_device_run(pmx_ctx *c, int nmat, const double *dKpts, int nb, int
                 Prof pf(c, real ? "backtrans_real" : "backtrans_complex", nm * (real ? 2.0 : 8.0) * n2 * nvec, 0.0);
                 rc = real ? launch_backtrans<double>(c, N, nm, nvec, Y, vref, tauv, c->dPhaseCS.p, vr,
                                                      dEvecs + (size_t)m0 * nvec * N)
-                          : launch_backtrans<zd>(c, N, nm, nvec, Y, vref, tauv, c->dPhaseCS.p, nullptr,
+                          : launch_backtrans<zd>(c, N, nm, nvec, Y, vref, tauv, c->dPhaseCS.p, vr,
                                                  dEvecs + (size_t)m0 * nvec * N);
                 if (rc) return rc;
             }

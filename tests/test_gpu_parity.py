@@ -135,6 +135,43 @@ def test_inverse_iteration_forms_agree(manifest, case, monkeypatch):
         assert np.abs(V.conj().T @ V - np.eye(nb)).max() < 1e-10
 
 
+@pytest.mark.parametrize("case", ["gaas_band", "te_soc_band", "wz_band"])
+def test_backtrans_complex_forms_agree(manifest, case, monkeypatch):
+    """The compact-WY complex back-transformation (FP64 MFMA, Z in
+    registers) against the reflector-at-a-time kernel (PMX_FORCE_KERNEL=
+    bt_loop): the same products in another order, so the vectors agree to
+    rounding, and both are eigenvectors of the oracle's H."""
+    args = _expand(manifest[case]["args"])
+    _say("complex back-transformation forms %s" % case)
+    s, p = cli_args.build(manifest[case]["args"])
+    rng = np.random.default_rng(12)
+    K = rng.uniform(-0.6, 0.6, (6, 3))
+    K[0] = 0.0
+    nb = min(40, 2 * s.npw)
+    out = {}
+    for form in ("wy", "bt_loop"):
+        if form == "bt_loop":
+            monkeypatch.setenv("PMX_FORCE_KERNEL", "bt_loop")
+        else:
+            monkeypatch.delenv("PMX_FORCE_KERNEL", raising=False)
+        c = capi.Context(0)
+        try:
+            inp = capi.Input(args)
+            c.set_system(inp.system(), inp)
+            out[form] = c.eig(K, nb, vectors=True)
+        finally:
+            c.close()
+    ev0, v0 = out["wy"]
+    ev1, v1 = out["bt_loop"]
+    assert np.array_equal(ev0, ev1)
+    assert np.abs(v0 - v1).max() < 1e-12
+    for k in range(len(K)):
+        H = s.hamiltonian(K[k])
+        V = v0[k].T
+        assert np.abs(H @ V - V * ev0[k]).max() < 1e-9
+        assert np.abs(V.conj().T @ V - np.eye(nb)).max() < 1e-10
+
+
 LARGE = {  # bases that keep the packed matrix in global memory (the C3 / C5 sizes)
     "te_soc_582": ["-crystal", "te", "-a", "4.45", "-f", "1.33", "-u", "0.26", "-vg",
                    "0.565323,1.333333,1.898656,2.261292,3.594626,4.0" + ",-0.2105,-0.1322,0.0451,0.0214,0.0127,-0.0093" * 3,
