@@ -331,7 +331,7 @@ def run_config(opt):
     n = 9 * nq * ne2 * cfg["nfreq"]
     dre, dim = hip.malloc(8 * n), hip.malloc(8 * n)
     common = (K, KW, Q, cfg["nband"], off, cfg["delta"], cfg["eps"], cfg["nfreq"], cfg["dfreq"], cfg["kk"])
-    prec = cfg["precision"]
+    prec = opt.precision or cfg["precision"]
     print("%s: %d k x %d q, mixed (%s) step" % (opt.config, nk, nq, prec), file=sys.stderr, flush=True)
     t_mix, prof_mix = run_steps(ctx, hip, ctx.chi_params(*common, prec), dre, dim, opt.steps, opt.warmup)
     re_m, im_m = eps_block(hip, dre, dim, nq * ne2 * cfg["nfreq"])
@@ -399,7 +399,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x3"])
+    ap.add_argument("--precision", default=None, choices=["fp32", "bf16x3"],
+                    help="mixed path (default: bf16x3 for c2, the configuration's own for c3/c5)")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eig-sweep", action="store_true", help="skip the eigensolves/s sweep over basis sizes")
@@ -417,6 +418,7 @@ def main():
     if opt.config != "c2":
         run_config(opt)
         return
+    opt.precision = opt.precision or "bf16x3"
     grid = int(opt.grid) if opt.grid else NGRID
     inp = capi.Input(cli_args())
     ctx = capi.Context(0)
