@@ -1516,8 +1516,9 @@ __global__ void __launch_bounds__(256) f0max_kernel(int nt, const double *dE, do
  * column tiles of 16, so each A value it loads feeds KK_CT MFMA pairs.  T =
  * double (FP64 MFMA) or float (FP32 MFMA, mixed paths; Kk then in FP32). */
 constexpr int KK_CT_DEF = 4; /* default column tiles per wave */
-template <class T, int KK_CT = KK_CT_DEF>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 8 : 1) kk_gemm_kernel(int M, int NF, const double *Re0, const double *Im0,
+/* OCC: waves per SIMD the FP32 form's registers are cut for (the FP64 form keeps the compiler's choice) */
+template <class T, int KK_CT = KK_CT_DEF, int OCC = 8>
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? OCC : 1) kk_gemm_kernel(int M, int NF, const double *Re0, const double *Im0,
                                                       const T *Kk, double *Re, double *Im, int ne2, int nq, int q,
                                                       const int *qp) {
     if (qp) q = *qp; /* a captured q step: the q index in device memory */
@@ -2433,7 +2434,8 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
             if (fp32) {
                 kc.push_back(kk_cand("kk_ct4", kk_gemm_kernel<float, 4>, 4, (const float *)w.Kkf.p));
                 kc.push_back(kk_cand("kk_ct2", kk_gemm_kernel<float, 2>, 2, (const float *)w.Kkf.p));
-                kc.push_back(kk_cand("kk_ct8", kk_gemm_kernel<float, 8>, 8, (const float *)w.Kkf.p));
+                kc.push_back(kk_cand("kk_ct8w4", kk_gemm_kernel<float, 8, 4>, 8, (const float *)w.Kkf.p));
+                kc.push_back(kk_cand("kk_ct16w2", kk_gemm_kernel<float, 16, 2>, 16, (const float *)w.Kkf.p));
             } else {
                 kc.push_back(kk_cand("kk_ct4", kk_gemm_kernel<double, 4>, 4, (const double *)w.Kkd.p));
                 kc.push_back(kk_cand("kk_ct2", kk_gemm_kernel<double, 2>, 2, (const double *)w.Kkd.p));

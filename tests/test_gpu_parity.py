@@ -254,7 +254,8 @@ def test_chi_precision_modes(manifest, ctx, case, precision, tol):
 
 @pytest.mark.parametrize("form,precision,tol", [("conv_fb2", "fp64", 1e-8), ("kk_ct2", "fp64", 1e-8),
                                                 ("conv_fb2", "fp32", 1e-4), ("conv_fb8", "fp32", 1e-4),
-                                                ("kk_ct2", "fp32", 1e-4), ("kk_ct8", "fp32", 1e-4),
+                                                ("kk_ct2", "fp32", 1e-4), ("kk_ct8w4", "fp32", 1e-4),
+                                                ("kk_ct16w2", "fp32", 1e-4),
                                                 ("mom_w2", "fp32", 1e-4)])
 def test_chi_tuned_forms_against_reference(manifest, ctx, form, precision, tol, monkeypatch):
     """Every autotuner candidate of the convolution (frequency blocks per
