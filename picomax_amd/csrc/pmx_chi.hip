@@ -539,6 +539,163 @@ template <class T> __global__ void __launch_bounds__(256, 4) rho_cart_kernel(Rho
     }
 }
 
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+/* x = hi + lo with hi = bf16(x), lo = bf16(x - hi), both rounded to nearest
+ * even by the gfx950 conversion (v_cvt_pk_bf16_f32) */
+__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
+
+/* ------------------------------------------------------------------
+ * rho for the spatial basis in split BF16 (the split-BF16 path): the sums of
+ * rho_cart_kernel with every operand x = hi + lo in BF16 and three
+ * v_mfma_f32_16x16x32_bf16 per product, FP32 accumulation.  G_p,d moves to
+ * the B side (T_{m,d} = sum_p conj(Ck[c][p]) (G_p,d Ckq[v][loci(m,p)])), so
+ * the A operand, the conduction rows at k in the real embedding (re, im of
+ * entry p at K = 2p, 2p+1), is split once when it is staged in LDS; the B
+ * side, the gathered valence entry and its three G-weighted copies, is split
+ * as it is formed.  Lane l holds K entries 8(l/16) .. +7 (four entries p) of
+ * A row l%16 and of B column l%16; D rows 4(l/16)+v, as in rho_cart_kernel.
+ * ------------------------------------------------------------------ */
+__global__ void __launch_bounds__(256, 3) rho_bf3_kernel(RhoArgs a, int K2p) {
+    const int k = blockIdx.x;
+    const int ncs = a.nc[k] / 2, nvs = a.nv[k] / 2;
+    if (ncs * nvs == 0) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    extern __shared__ double rho_dyn[];
+    bf8 *skh = reinterpret_cast<bf8 *>(rho_dyn);                  /* [ncs][K2p/8]: A rows, hi */
+    bf8 *skl = skh + (size_t)ncs * (K2p / 8);                      /* lo */
+    float2 *scq = reinterpret_cast<float2 *>(skl + (size_t)ncs * (K2p / 8)); /* [nvs][N]: valence rows at k+q */
+    float *sg = reinterpret_cast<float *>(scq + (size_t)nvs * a.N); /* [npw][3]: G */
+    const size_t tb = a.tbase[k];
+    const double Kx = a.K[3 * k], Ky = a.K[3 * k + 1], Kz = a.K[3 * k + 2];
+    const zd *vk = a.vk + (size_t)k * a.nb_s * a.N;
+    const zd *vkq = a.vkq + (size_t)k * a.nb_s * a.N;
+    const int K2 = 2 * a.npw;
+    /* A rows: entry K of row c is re (K even) / im (K odd) of Ck[c][K/2], zero past 2 npw */
+    __bf16 *ah_ = reinterpret_cast<__bf16 *>(skh), *al_ = reinterpret_cast<__bf16 *>(skl);
+    for (int idx = threadIdx.x; idx < ncs * K2p; idx += 256) {
+        const int c = idx / K2p, kk = idx % K2p;
+        float x = 0.0f;
+        if (kk < K2) {
+            const zd v = vk[(size_t)bra_row(a, c) * a.N + (kk >> 1)];
+            x = (float)((kk & 1) ? v.y : v.x);
+        }
+        __bf16 h, l;
+        split_bf16(x, h, l);
+        ah_[idx] = h;
+        al_[idx] = l;
+    }
+    for (int idx = threadIdx.x; idx < nvs * a.N; idx += 256) {
+        const zd v = vkq[(size_t)ket_row(a, idx / a.N) * a.N + idx % a.N];
+        scq[idx] = float2{(float)v.x, (float)v.y};
+    }
+    for (int idx = threadIdx.x; idx < 3 * a.npw; idx += 256) sg[idx] = (float)a.G[idx];
+    __syncthreads();
+    const int ncol = a.neps * nvs; /* (m, v) columns, v fastest */
+    const int nct = (ncs + 15) / 16, njt = (ncol + 15) / 16;
+    const int kq8 = 8 * (lane >> 4);
+    for (int job = wave; job < nct * njt; job += 4) {
+        const int ct = job / njt, jt = job % njt;
+        const int crow = ct * 16 + (lane & 15); /* A row */
+        const int col = jt * 16 + (lane & 15);  /* B column */
+        const int m = col / nvs, vv = col % nvs;
+        const bool crow_ok = crow < ncs, col_ok = m < a.neps;
+        const bf8 *rh = skh + (size_t)(crow_ok ? crow : 0) * (K2p / 8), *rl = skl + (size_t)(crow_ok ? crow : 0) * (K2p / 8);
+        const float2 *cq = scq + (size_t)(col_ok ? vv : 0) * a.N;
+        const int *lm = a.loci + (size_t)(col_ok ? m : 0) * a.npw;
+        /* channels S, Tx, Ty, Tz: real and imaginary parts */
+        fx4 accr[4], acci[4];
+#pragma unroll
+        for (int ch = 0; ch < 4; ch++) {
+            accr[ch] = fx4{0, 0, 0, 0};
+            acci[ch] = fx4{0, 0, 0, 0};
+        }
+        /* the four loci of this lane's K entries, one K step ahead */
+        int ln[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int pp = (kq8 >> 1) + e;
+            ln[e] = (col_ok && pp < a.npw) ? lm[pp] : -1;
+        }
+        for (int k0 = 0; k0 < K2p; k0 += 32) {
+            const int p0 = (k0 + kq8) >> 1;
+            int lc[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                lc[e] = ln[e];
+                const int pn = p0 + 16 + e;
+                ln[e] = (col_ok && pn < a.npw && k0 + 32 < K2p) ? lm[pn] : -1;
+            }
+            const bf8 ah = crow_ok ? rh[(k0 + kq8) / 8] : bf8{}, al = crow_ok ? rl[(k0 + kq8) / 8] : bf8{};
+            bf8 bh[4][2], bl[4][2]; /* [channel][re, im] */
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int pp = p0 + e;
+                float2 q = float2{0.0f, 0.0f};
+                float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+                if (lc[e] >= 0) q = cq[lc[e]];
+                if (pp < a.npw) {
+                    gx = sg[3 * pp];
+                    gy = sg[3 * pp + 1];
+                    gz = sg[3 * pp + 2];
+                }
+                /* K = 2pp (re of Ck): B = (q.x, q.y); K = 2pp+1 (im of Ck): B = (q.y, -q.x) */
+                const float bre0 = q.x, bim0 = q.y, bre1 = q.y, bim1 = -q.x;
+                const float gw[4] = {1.0f, gx, gy, gz};
+#pragma unroll
+                for (int ch = 0; ch < 4; ch++) {
+                    __bf16 h, l;
+                    split_bf16(ch ? gw[ch] * bre0 : bre0, h, l);
+                    bh[ch][0][2 * e] = h;
+                    bl[ch][0][2 * e] = l;
+                    split_bf16(ch ? gw[ch] * bim0 : bim0, h, l);
+                    bh[ch][1][2 * e] = h;
+                    bl[ch][1][2 * e] = l;
+                    split_bf16(ch ? gw[ch] * bre1 : bre1, h, l);
+                    bh[ch][0][2 * e + 1] = h;
+                    bl[ch][0][2 * e + 1] = l;
+                    split_bf16(ch ? gw[ch] * bim1 : bim1, h, l);
+                    bh[ch][1][2 * e + 1] = h;
+                    bl[ch][1][2 * e + 1] = l;
+                }
+            }
+#pragma unroll
+            for (int ch = 0; ch < 4; ch++) {
+                accr[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[ch][0], accr[ch], 0, 0, 0);
+                accr[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[ch][0], accr[ch], 0, 0, 0);
+                accr[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[ch][0], accr[ch], 0, 0, 0);
+                acci[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[ch][1], acci[ch], 0, 0, 0);
+                acci[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[ch][1], acci[ch], 0, 0, 0);
+                acci[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[ch][1], acci[ch], 0, 0, 0);
+            }
+        }
+        if (!col_ok) continue;
+        const double *Gm = a.G + 3 * m;
+        const double Qx = a.Qd ? a.Qd[0] : a.Qx, Qy = a.Qd ? a.Qd[1] : a.Qy, Qz = a.Qd ? a.Qd[2] : a.Qz;
+        const double ox = Kx + 0.5 * (Qx + Gm[0]), oy = Ky + 0.5 * (Qy + Gm[1]), oz = Kz + 0.5 * (Qz + Gm[2]);
+        for (int r = 0; r < 4; r++) {
+            const int c = ct * 16 + 4 * (lane >> 4) + r;
+            if (c >= ncs) continue;
+            const int row = a.pos[tb + (size_t)c * nvs + vv];
+            if (row < 0) continue;
+            const zd S{(double)accr[0][r], (double)acci[0][r]};
+            const zd Tx{(double)accr[1][r], (double)acci[1][r]}, Ty{(double)accr[2][r], (double)acci[2][r]};
+            const zd Tz{(double)accr[3][r], (double)acci[3][r]};
+            for (int i = 0; i < 3; i++) {
+                const zd *u = a.uc + ((size_t)m * 3 + i) * 3;
+                const zd t0 = cmul(u[0], Tx), t1 = cmul(u[1], Ty), t2 = cmul(u[2], Tz);
+                const zd bo{u[0].x * ox + u[1].x * oy + u[2].x * oz, u[0].y * ox + u[1].y * oy + u[2].y * oz};
+                const zd bs = cmul(bo, S);
+                put_item(a, row, 0, 3 * m + i,
+                         (a.ll == 1 && i == 0) ? S : zd{t0.x + t1.x + t2.x + bs.x, t0.y + t1.y + t2.y + bs.y});
+            }
+        }
+    }
+}
+
 /* ------------------------------------------------------------------
  * rho for the spinor (SOC) basis on MFMA, the Cartesian form for the orbital
  * current.  Spinor rows hold (up, down) at 2p, 2p+1.  For one k, rows c of
@@ -1036,14 +1193,6 @@ template <int MAXNT, int OCC> __global__ void __launch_bounds__(256, OCC) moment
  * l%16); D rows 4(l/16)+v.  The s^n scaling is applied to B in FP32 and B is
  * re-split for every order n.
  * ------------------------------------------------------------------ */
-typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
-
-/* x = hi + lo with hi = bf16(x), lo = bf16(x - hi), both rounded to nearest
- * even by the gfx950 conversion (v_cvt_pk_bf16_f32) */
-__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
-    hi = (__bf16)x;
-    lo = (__bf16)(x - (float)hi);
-}
 
 /* one workgroup per (tile pair, bin); each of the 4 waves takes a contiguous
  * quarter of the bin's rows (which are in k-major transition order) for
@@ -2262,7 +2411,25 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
             const size_t lds_soc = sizeof(float2) * (size_t)maxnv * N + sizeof(float) * 3 * (size_t)npw;
             /* the Cartesian form: its LDS is the rows plus the G table */
             const size_t lds_c = lds_r + (fp32 ? sizeof(float) : sizeof(double)) * 3 * (size_t)npw;
-            if (rho_lds && fp32 && lds_c <= c->lds_max) {
+            /* the split-BF16 form: the conduction rows split in LDS (hi, lo), the valence
+             * rows in FP32, the G table */
+            int maxnc2 = cap ? nc_cap / 2 : 0, maxnv2 = cap ? nv_cap / 2 : 0;
+            if (!cap)
+                for (int kk = 0; kk < p->nk; kk++) {
+                    maxnc2 = std::max(maxnc2, nch_h[kk] / 2);
+                    maxnv2 = std::max(maxnv2, nvh[kk] / 2);
+                }
+            const int K2p = (2 * npw + 31) & ~31;
+            const size_t lds_b = 4 * (size_t)maxnc2 * K2p + sizeof(float2) * (size_t)maxnv2 * N + sizeof(float) * 3 * (size_t)npw;
+            const char *rf = std::getenv("PMX_RHO");
+            const bool rho_bf = spatial && fp32 && p->precision == PMX_PREC_BF16X3 && N == npw &&
+                                lds_b <= c->lds_max && !(rf && std::strcmp(rf, "fp32") == 0);
+            if (rho_bf) {
+                if (lds_b > 65536)
+                    PMX_CHECK_HIP(hipFuncSetAttribute((const void *)rho_bf3_kernel,
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+                hipLaunchKernelGGL(rho_bf3_kernel, dim3(p->nk), dim3(256), lds_b, st, ra, K2p);
+            } else if (rho_lds && fp32 && lds_c <= c->lds_max) {
                 if (lds_c > 65536)
                     PMX_CHECK_HIP(hipFuncSetAttribute((const void *)rho_cart_kernel<float>,
                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_c));
