@@ -1541,6 +1541,105 @@ template <class T, int FB, int OCC> __global__ void __launch_bounds__(256, OCC) 
         }
 }
 
+/* The convolution of the split-BF16 path: the GEMM of conv_mfma_kernel<float>
+ * with each operand x = hi + lo in BF16 and three v_mfma_f32_16x16x32_bf16
+ * per product, FP32 accumulation.  The Taylor table is split once as it is
+ * staged in LDS, each entry's hi and lo in one 32-bit word; the moments are
+ * split as they are loaded.  Lane l holds K entries 8(l/16) .. +7 of A (the
+ * table row of frequency l%16) and of B (the moment rows of element l%16);
+ * D rows 4(l/16)+v.  The K axis runs over (bin, order) from the first bin
+ * under the workgroup's frequencies, in steps of 32; a block skips the steps
+ * outside its band. */
+template <int FB, int OCC> __global__ void __launch_bounds__(256, OCC) conv_bf3_kernel(ConvArgs a) {
+    extern __shared__ double smem_c[];
+    const int W = a.W, WN = W * a.NT;
+    unsigned *ks = reinterpret_cast<unsigned *>(smem_c); /* [W][NT]: bf16 hi | lo << 16 */
+    {
+        const float *kt = reinterpret_cast<const float *>(a.Kt);
+        for (int i = threadIdx.x; i < WN; i += 256) {
+            __bf16 h, l;
+            split_bf16(kt[i], h, l);
+            ks[i] = (unsigned)__builtin_bit_cast(unsigned short, h) | ((unsigned)__builtin_bit_cast(unsigned short, l) << 16);
+        }
+        __syncthreads();
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int E = a.npairs * 256;
+    const int e0 = (blockIdx.x * 4 + wave) * 16;
+    if (e0 >= E) return;
+    const int fb0 = blockIdx.y * 16 * FB;
+    const int eb = e0 + (lane & 15); /* B column: element */
+    const int kq8 = 8 * (lane >> 4);
+    const int b_lo = max(0, fb0 - a.R0), b_hi = min(a.nbins - 1, fb0 + 16 * FB - 1 - a.R0 + W - 1);
+    const int kcount = max(0, b_hi - b_lo + 1) * a.NT;
+    const float2 *M = reinterpret_cast<const float2 *>(a.M);
+    fx4 accR[FB], accI[FB];
+#pragma unroll
+    for (int q = 0; q < FB; q++) {
+        accR[q] = fx4{0, 0, 0, 0};
+        accI[q] = fx4{0, 0, 0, 0};
+    }
+    int klo[FB], khi[FB], toff[FB];
+    bool fok[FB];
+#pragma unroll
+    for (int q = 0; q < FB; q++) {
+        const int fq = fb0 + 16 * q;
+        const int blo = max(b_lo, fq - a.R0), bhi = min(b_hi, fq + 15 - a.R0 + W - 1);
+        klo[q] = (blo - b_lo) * a.NT;
+        khi[q] = (bhi - b_lo + 1) * a.NT - 1; /* below klo when the band misses the bins */
+        const int fa = fq + (lane & 15);      /* A row: frequency */
+        toff[q] = (b_lo - fa + a.R0) * a.NT;
+        fok[q] = fa < a.nfreq;
+    }
+    auto mload = [&](int kk) -> float2 {
+        return kk < kcount ? M[((size_t)b_lo * a.NT + kk) * E + eb] : float2{0.0f, 0.0f};
+    };
+    float2 mc[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) mc[e] = mload(kq8 + e);
+    for (int k0 = 0; k0 < kcount; k0 += 32) {
+        /* this step's moments split; the next step's loaded */
+        bf8 brh, brl, bih, bil;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            __bf16 h, l;
+            split_bf16(mc[e].x, h, l);
+            brh[e] = h;
+            brl[e] = l;
+            split_bf16(mc[e].y, h, l);
+            bih[e] = h;
+            bil[e] = l;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++) mc[e] = mload(k0 + 32 + kq8 + e);
+#pragma unroll
+        for (int q = 0; q < FB; q++) {
+            if (k0 + 31 < klo[q] || k0 > khi[q]) continue; /* outside this block's band */
+            bf8 ah, al;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int kk = k0 + kq8 + e, ti = kk + toff[q];
+                const unsigned wv = (kk < kcount && fok[q] && ti >= 0 && ti < WN) ? ks[ti] : 0u;
+                ah[e] = __builtin_bit_cast(__bf16, (unsigned short)(wv & 0xffffu));
+                al[e] = __builtin_bit_cast(__bf16, (unsigned short)(wv >> 16));
+            }
+            accR[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, brh, accR[q], 0, 0, 0);
+            accR[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, brl, accR[q], 0, 0, 0);
+            accR[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, brh, accR[q], 0, 0, 0);
+            accI[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bih, accI[q], 0, 0, 0);
+            accI[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bil, accI[q], 0, 0, 0);
+            accI[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bih, accI[q], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < FB; q++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int f = fb0 + 16 * q + 4 * (lane >> 4) + v;
+            if (f < a.nfreq) a.Z[(size_t)f * E + eb] = zd{(double)accR[q][v], (double)accI[q][v]};
+        }
+}
+
 /* ------------------------------------------------------------------
  * packing: Z tiles -> ReX/ImX [i][j][q][m][n<=m][f] with SCALEFACTOR/w^2
  * ------------------------------------------------------------------ */
@@ -2534,7 +2633,11 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                                         }};
                     };
                     std::vector<TuneCand> cc;
-                    if (fp32) {
+                    const bool cbf = fp32 && kt_lds && p->precision == PMX_PREC_BF16X3;
+                    if (cbf) {
+                        cc.push_back(conv_cand("convb_fb4", conv_bf3_kernel<4, 4>, 4));
+                        cc.push_back(conv_cand("convb_fb8", conv_bf3_kernel<8, 2>, 8));
+                    } else if (fp32) {
                         cc.push_back(conv_cand("conv_fb4", conv_mfma_kernel<float, 4, 8>, 4));
                         cc.push_back(conv_cand("conv_fb2", conv_mfma_kernel<float, 2, 8>, 2));
                         cc.push_back(conv_cand("conv_fb8", conv_mfma_kernel<float, 8, 4>, 8));
@@ -2542,7 +2645,7 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                         cc.push_back(conv_cand("conv_fb4", conv_mfma_kernel<double, 4, 1>, 4));
                         cc.push_back(conv_cand("conv_fb2", conv_mfma_kernel<double, 2, 1>, 2));
                     }
-                    const std::string key = std::string("conv ") + (fp32 ? "f32" : "f64") + " E=" +
+                    const std::string key = std::string("conv ") + (cbf ? "bf16x3" : (fp32 ? "f32" : "f64")) + " E=" +
                                             std::to_string(npairs * 256) + " nbins=" + std::to_string(nbins) +
                                             " W=" + std::to_string(W) + " NT=" + std::to_string(NT) +
                                             " nfreq=" + std::to_string(p->nfreq);
