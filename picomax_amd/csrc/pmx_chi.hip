@@ -1648,6 +1648,11 @@ struct PackArgs {
     double SF, dfreq, w0;
     const zd *Z1, *Z2;
     double *ReX, *ImX;
+    /* optional: the rows of ImX and ReX rounded to FP32 and split into BF16
+     * hi + lo, [element][NFp] with NFp >= nfreq (the tail zero), for the
+     * split-BF16 KK GEMM */
+    __bf16 *Ih = nullptr, *Il = nullptr, *Rh = nullptr, *Rl = nullptr;
+    int NFp = 0;
 };
 
 __device__ __forceinline__ zd zfetch(const zd *Z, int f, int npairs, int a, int b) {
@@ -1682,8 +1687,26 @@ __global__ void pack_kernel(PackArgs a) {
         const double w = a.w0 + a.dfreq * f;
         const double inv = (f == 0) ? 0.0 : 1.0 / (w * w);
         const size_t dst = (((size_t)ij * a.nq + a.q) * ne2 + e2) * a.nfreq + f; /* nq = 1, q = 0: compact */
-        a.ImX[dst] = a.SF * (imv * inv);
-        a.ReX[dst] = a.SF * (rev * inv);
+        const double iv = a.SF * (imv * inv), rv = a.SF * (rev * inv);
+        a.ImX[dst] = iv;
+        a.ReX[dst] = rv;
+        if (a.Ih) {
+            const size_t o = el * a.NFp + f;
+            __bf16 h, l;
+            split_bf16((float)iv, h, l);
+            a.Ih[o] = h;
+            a.Il[o] = l;
+            split_bf16((float)rv, h, l);
+            a.Rh[o] = h;
+            a.Rl[o] = l;
+            if (f == a.nfreq - 1)
+                for (size_t t = o + 1; t < (el + 1) * a.NFp; t++) {
+                    a.Ih[t] = (__bf16)0.0f;
+                    a.Il[t] = (__bf16)0.0f;
+                    a.Rh[t] = (__bf16)0.0f;
+                    a.Rl[t] = (__bf16)0.0f;
+                }
+        }
     }
 }
 
@@ -1828,6 +1851,89 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? OCC : 1) kk_gemm_kernel(
         }
 }
 
+/* The KK GEMM of the split-BF16 path: kk_gemm_kernel's sums with each
+ * operand x = hi + lo in BF16 and three v_mfma_f32_16x16x32_bf16 per
+ * product, FP32 accumulation.  The A rows come split from pack_kernel
+ * ([element][NFp]) and K transposed and split from kkmat_bf3_kernel
+ * ([f][NFp], NFc >= NF rows), both zero past nfreq, so the K loop has no
+ * bounds checks.  Lane l holds K entries 8(l/16) .. +7 of A row l%16 and of
+ * the K column l%16 of each column tile; D rows 4(l/16)+v. */
+template <int CT, int OCC>
+__global__ void __launch_bounds__(256, OCC) kk_bf3_kernel(int M, int NF, int NFp, const __bf16 *Ih, const __bf16 *Il,
+                                                          const __bf16 *Rh, const __bf16 *Rl, const __bf16 *KhT,
+                                                          const __bf16 *KlT, const double *Re0, const double *Im0,
+                                                          double *Re, double *Im, int ne2, int nq, int q,
+                                                          const int *qp) {
+    if (qp) q = *qp;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ncg = (NF + 16 * CT - 1) / (16 * CT);
+    const int slot = blockIdx.x >> 3;
+    const int rg = (blockIdx.x & 7) + 8 * (slot / ncg), cg = slot % ncg;
+    const int row0 = (rg * 4 + wave) * 16, col0 = cg * 16 * CT;
+    if (row0 >= M) return;
+    const int kq = lane >> 4;
+    const size_t ao = (size_t)min(row0 + (lane & 15), M - 1) * NFp + 8 * kq; /* rows past M: the last, discarded */
+    const bf8 *aIh = reinterpret_cast<const bf8 *>(Ih + ao), *aIl = reinterpret_cast<const bf8 *>(Il + ao);
+    const bf8 *aRh = reinterpret_cast<const bf8 *>(Rh + ao), *aRl = reinterpret_cast<const bf8 *>(Rl + ao);
+    const bf8 *bh[CT], *bl[CT];
+#pragma unroll
+    for (int c = 0; c < CT; c++) {
+        const size_t bo = (size_t)(col0 + 16 * c + (lane & 15)) * NFp + 8 * kq;
+        bh[c] = reinterpret_cast<const bf8 *>(KhT + bo);
+        bl[c] = reinterpret_cast<const bf8 *>(KlT + bo);
+    }
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    v4 accR[CT], accI[CT];
+#pragma unroll
+    for (int c = 0; c < CT; c++) {
+        accR[c] = v4{0, 0, 0, 0};
+        accI[c] = v4{0, 0, 0, 0};
+    }
+    struct Ops {
+        bf8 ih, il, rh, rl, kh[CT], kl[CT];
+    };
+    /* K step s covers entries 32 s .. 32 s + 31: offset 4 s in bf8 units */
+    auto load = [&](int s) {
+        Ops o;
+        o.ih = aIh[4 * s];
+        o.il = aIl[4 * s];
+        o.rh = aRh[4 * s];
+        o.rl = aRl[4 * s];
+#pragma unroll
+        for (int c = 0; c < CT; c++) {
+            o.kh[c] = bh[c][4 * s];
+            o.kl[c] = bl[c][4 * s];
+        }
+        return o;
+    };
+    const int nks = NFp / 32;
+    Ops cur = load(0);
+    for (int s = 0; s < nks; s++) {
+        const Ops nxt = load(min(s + 1, nks - 1));
+#pragma unroll
+        for (int c = 0; c < CT; c++) {
+            accR[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.ih, cur.kh[c], accR[c], 0, 0, 0);
+            accR[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.ih, cur.kl[c], accR[c], 0, 0, 0);
+            accR[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.il, cur.kh[c], accR[c], 0, 0, 0);
+            accI[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.rh, cur.kh[c], accI[c], 0, 0, 0);
+            accI[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.rh, cur.kl[c], accI[c], 0, 0, 0);
+            accI[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.rl, cur.kh[c], accI[c], 0, 0, 0);
+        }
+        cur = nxt;
+    }
+#pragma unroll
+    for (int c = 0; c < CT; c++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int r = row0 + 4 * kq + v, cidx = col0 + 16 * c + (lane & 15);
+            if (r >= M || cidx >= NF) continue;
+            const size_t ij = r / ne2, e2 = r % ne2;
+            const size_t o = ((ij * nq + q) * ne2 + e2) * NF + cidx;
+            Re[o] = Re0[(size_t)r * NF + cidx] + (double)accR[c][v];
+            Im[o] = Im0[(size_t)r * NF + cidx] + (double)accI[c][v];
+        }
+}
+
 __global__ void place_kernel(int nel, int nfreq, int ne2, int nq, int q, const double *Re0, const double *Im0,
                              double *Re, double *Im, const int *qp) {
     if (qp) q = *qp;
@@ -1959,6 +2065,23 @@ template <class T> __global__ void kkmat_kernel(int nfreq, double dfreq, T *Kk) 
     }
 }
 
+/* kkmat_kernel<float>'s matrix transposed and split for kk_bf3_kernel:
+ * KhT[f][s] + KlT[f][s] = (float)Kk[s][f], NFc x NFp, zero for f or s past
+ * nfreq */
+__global__ void kkmat_bf3_kernel(int nfreq, int NFp, int NFc, double dfreq, __bf16 *KhT, __bf16 *KlT) {
+    const size_t total = (size_t)NFc * NFp;
+    const double ds = dfreq * 2.0 / PMX_PI;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int f = idx / NFp, s = idx % NFp;
+        const double ws = dfreq * s, wf = dfreq * f;
+        const float v = (f < nfreq && s < nfreq && s != f) ? (float)(ds * (ws / (ws * ws - wf * wf))) : 0.0f;
+        __bf16 h, l;
+        split_bf16(v, h, l);
+        KhT[idx] = h;
+        KlT[idx] = l;
+    }
+}
+
 /* host: Taylor kernel table K_n(D) = h^n delta^(n)((D+1/2) df)/n! for the
  * W offsets D = f0 - f = Dlo .. Dlo + W - 1 (row D - Dlo) */
 void taylor_table(int delta, double eps, double df, int Dlo, int W, int NT, std::vector<double> &Kt) {
@@ -2028,6 +2151,7 @@ struct ChiWork {
     DBuf<zd> vk, Xd, M1, M2, Z1, Z2, wAd, ucd, wSd;
     DBuf<float2> Xf;
     DBuf<float> Ktf, Kkf;
+    DBuf<__bf16> kA, kT; /* split-BF16 KK: A rows [Ih | Il | Rh | Rl][M][NFp], K [KhT | KlT][NFc][NFp] */
     DBuf<int> nc, nv, tbase, bind, hist, start, cnt, pos, bad;
     /* captured q steps: the tables of every q, the current q's vector, the
      * reference counts and [q index, unused, mismatch flag per q] */
@@ -2347,6 +2471,17 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
         hipLaunchKernelGGL(kkmat_kernel<double>, dim3(256), dim3(256), 0, st, p->nfreq, df, w.Kkd.p);
         if ((rc = w.Kkf.ensure((size_t)p->nfreq * p->nfreq))) return rc;
         hipLaunchKernelGGL(kkmat_kernel<float>, dim3(256), dim3(256), 0, st, p->nfreq, df, w.Kkf.p);
+    }
+    /* the split-BF16 KK (PMX_KK=fp32 keeps the FP32 one on that path) */
+    const char *kkenv = std::getenv("PMX_KK");
+    const bool kkbf = p->kk && !ll && !direct && p->precision == PMX_PREC_BF16X3 &&
+                      !(kkenv && std::string(kkenv) == "fp32");
+    const int kkNFp = (p->nfreq + 31) / 32 * 32, kkNFc = (p->nfreq + 255) / 256 * 256;
+    const size_t kkM = (size_t)9 * ne2;
+    if (kkbf) {
+        if ((rc = w.kA.ensure(4 * kkM * kkNFp)) || (rc = w.kT.ensure((size_t)2 * kkNFc * kkNFp))) return rc;
+        hipLaunchKernelGGL(kkmat_bf3_kernel, dim3(256), dim3(256), 0, st, p->nfreq, kkNFp, kkNFc, df, w.kT.p,
+                           w.kT.p + (size_t)kkNFc * kkNFp);
     }
     if ((rc = w.Re0.ensure(per_q)) || (rc = w.Im0.ensure(per_q))) return rc;
 
@@ -2682,6 +2817,13 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
             Prof pf(c, "pack", 0.0, 16.0 * per_q);
             PackArgs pa{D, neps, npairs, p->nfreq, 1, 0, spatial ? 0 : 1, SF, df, w0, w.Z1.p,
                         spatial ? w.Z1.p : w.Z2.p, w.Re0.p, w.Im0.p};
+            if (kkbf) {
+                pa.Ih = w.kA.p;
+                pa.Il = pa.Ih + kkM * kkNFp;
+                pa.Rh = pa.Il + kkM * kkNFp;
+                pa.Rl = pa.Rh + kkM * kkNFp;
+                pa.NFp = kkNFp;
+            }
             hipLaunchKernelGGL(pack_kernel, dim3(1024), dim3(256), 0, st, pa);
         }
         if ((rc = elapsed("    "))) return rc;
@@ -2700,8 +2842,24 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                                     return 0;
                                 }};
             };
+            auto kkb_cand = [&](const char *name, auto kern, int ct) {
+                return TuneCand{name, [&, kern, ct](int) {
+                                    const dim3 grid(8 * (((M + 63) / 64 + 7) / 8) * ((p->nfreq + 16 * ct - 1) / (16 * ct)));
+                                    const __bf16 *Ih = w.kA.p, *Il = Ih + kkM * kkNFp, *Rh = Il + kkM * kkNFp,
+                                                 *Rl = Rh + kkM * kkNFp;
+                                    hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, M, p->nfreq, kkNFp, Ih, Il, Rh, Rl,
+                                                       w.kT.p, w.kT.p + (size_t)kkNFc * kkNFp, w.Re0.p, w.Im0.p,
+                                                       dRe, dIm, ne2, p->nq, q, qp);
+                                    PMX_CHECK_HIP(hipGetLastError());
+                                    return 0;
+                                }};
+            };
             std::vector<TuneCand> kc;
-            if (fp32) {
+            if (kkbf) {
+                kc.push_back(kkb_cand("kkb_ct4", kk_bf3_kernel<4, 4>, 4));
+                kc.push_back(kkb_cand("kkb_ct2", kk_bf3_kernel<2, 6>, 2));
+                kc.push_back(kkb_cand("kkb_ct8", kk_bf3_kernel<8, 2>, 8));
+            } else if (fp32) {
                 kc.push_back(kk_cand("kk_ct4", kk_gemm_kernel<float, 4>, 4, (const float *)w.Kkf.p));
                 kc.push_back(kk_cand("kk_ct2", kk_gemm_kernel<float, 2>, 2, (const float *)w.Kkf.p));
                 kc.push_back(kk_cand("kk_ct8w4", kk_gemm_kernel<float, 8, 4>, 8, (const float *)w.Kkf.p));
@@ -2710,7 +2868,7 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                 kc.push_back(kk_cand("kk_ct4", kk_gemm_kernel<double, 4>, 4, (const double *)w.Kkd.p));
                 kc.push_back(kk_cand("kk_ct2", kk_gemm_kernel<double, 2>, 2, (const double *)w.Kkd.p));
             }
-            const std::string key = std::string("kk ") + (fp32 ? "f32" : "f64") + " M=" + std::to_string(M) +
+            const std::string key = std::string("kk ") + (kkbf ? "bf16x3" : fp32 ? "f32" : "f64") + " M=" + std::to_string(M) +
                                     " nfreq=" + std::to_string(p->nfreq);
             if ((rc = tune_run(c, key, 1 << 30, 1 << 30, kc))) return rc;
             if ((rc = elapsed("    "))) return rc;

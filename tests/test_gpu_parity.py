@@ -257,13 +257,15 @@ def test_chi_precision_modes(manifest, ctx, case, precision, tol):
                                                 ("kk_ct2", "fp32", 1e-4), ("kk_ct8w4", "fp32", 1e-4),
                                                 ("kk_ct16w2", "fp32", 1e-4),
                                                 ("convb_fb4", "bf16x3", 5e-3), ("convb_fb8", "bf16x3", 5e-3),
+                                                ("kkb_ct2", "bf16x3", 5e-3), ("kkb_ct4", "bf16x3", 5e-3),
+                                                ("kkb_ct8", "bf16x3", 5e-3),
                                                 ("mom_w2", "fp32", 1e-4)])
 def test_chi_tuned_forms_against_reference(manifest, ctx, form, precision, tol, monkeypatch):
     """Every autotuner candidate of the convolution (frequency blocks per
     wave), the KK GEMM (column tiles per wave) and the FP32 rank update
     (waves per SIMD), forced, against the reference's output on the
     Lorentzian + KK fixture: within 1e-8 per slice in FP64, 1e-4 in FP32,
-    5e-3 in split BF16 (the split-BF16 convolution's forms)."""
+    5e-3 in split BF16 (the split-BF16 convolution's and KK GEMM's forms)."""
     monkeypatch.setenv("PMX_FORCE_KERNEL", form)
     case = "si_chi_lor"
     args = manifest[case]["args"]
@@ -278,6 +280,31 @@ def test_chi_tuned_forms_against_reference(manifest, ctx, form, precision, tol, 
     err = cli_args.slice_error(np.concatenate([re, im]), ref, _nslice(args))
     _say("  slice error %.3e" % err)
     assert err < tol
+
+
+def test_kk_split_bf16_against_fp32_kk(manifest, ctx, monkeypatch):
+    """The split-BF16 KK GEMM (A rows split by the pack kernel, K split and
+    transposed) against the FP32 one on the same split-BF16 step: the outputs
+    differ only by the KK's own rounding, within 1e-4 per slice (the FP32
+    path's bar against the reference), far inside the 5e-3 the split-BF16
+    path is held to."""
+    case = "si_chi_lor"
+    args = manifest[case]["args"]
+    _say("chi bf16x3 KK forms on %s" % case)
+    inp = capi.Input(_expand(args))
+    ctx.set_system(inp.system(), inp)
+    s, p = cli_args.build(args)
+    off = ctx.ref_energy(p["refpoint"], p["nvalence"])
+    out = {}
+    for kk in ("bf16x3", "fp32"):
+        if kk == "fp32":
+            monkeypatch.setenv("PMX_KK", "fp32")
+        re, im = ctx.chi(p["K"], p["KW"], p["Q"], p["nband"], off, p["delta"], p["epsilon"], p["nfreq"],
+                         p["dfreq"], p["kk"], "bf16x3")
+        out[kk] = np.concatenate([re, im])
+    err = cli_args.slice_error(out["bf16x3"], out["fp32"], _nslice(args))
+    _say("  bf16x3 KK vs FP32 KK slice error %.3e" % err)
+    assert 0 < err < 1e-4
 
 
 def test_epsinv_against_numpy(manifest, ctx):
