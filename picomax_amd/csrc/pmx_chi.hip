@@ -1348,6 +1348,185 @@ __global__ void __launch_bounds__(256, 3) moment_bf16x3_kernel(MomArgs2 a) {
     }
 }
 
+/* x0, x1 -> packed BF16 hi (x0 in the low half) and lo = bf16(x - hi), both
+ * rounded to nearest even by v_cvt_pk_bf16_f32; hi's FP32 values are read
+ * back from the packed word by a shift and a mask */
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split2_bf16(float x0, float x1, unsigned &hi, unsigned &lo) {
+    hi = __builtin_bit_cast(unsigned, __builtin_convertvector(f2v{x0, x1}, bf2));
+    const float h0 = __builtin_bit_cast(float, hi << 16), h1 = __builtin_bit_cast(float, hi & 0xffff0000u);
+    lo = __builtin_bit_cast(unsigned, __builtin_convertvector(f2v{x0 - h0, x1 - h1}, bf2));
+}
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+
+/* The split-BF16 rank update with two tile pairs per workgroup: (ti, tj0)
+ * and (ti, tj1 = tj0 + 1) share the A side (row tile ti), which is scaled by
+ * s^n and re-split for every order n, so that per-order split feeds twice
+ * the MFMAs of moment_bf16x3_kernel; each B side is split once per K step.
+ * A row tile with an odd count of pairs ends in a group of one.  Same K
+ * layout, quarters and FP64 cross-quarter sum as moment_bf16x3_kernel. */
+template <int NI>
+__global__ void __launch_bounds__(256, 2) moment_bf3w_kernel(MomArgs2 a, int ngroups) {
+    constexpr int MAXNT = 9, HALF = (MAXNT + 1) / 2;
+    __shared__ fx4 red[4][64][2 * HALF];
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    int gi = slot % ngroups;
+    const int b = xcd + 8 * (slot / ngroups);
+    if (b >= a.nbins) return;
+    int ti = 0;
+    while (gi >= (ti + 2) / 2) { /* row tile ti holds ceil((ti + 1) / 2) groups */
+        gi -= (ti + 2) / 2;
+        ti++;
+    }
+    const int tj0 = 2 * gi;
+    const bool two = tj0 + 1 <= ti;
+    const int r0 = a.start[b], r1 = a.start[b + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int NT = a.NT;
+    const int ia = ti * TILE + (lane & 15), ib0 = tj0 * TILE + (lane & 15), ib1 = ib0 + TILE;
+    const int kg = lane >> 4;
+    const float2 *X = reinterpret_cast<const float2 *>(a.X);
+    constexpr int K2 = 2 * NI, RPL = 8 / K2;
+    const int kend = (r1 - r0) * K2;
+    fx4 accr[2][MAXNT], acci[2][MAXNT];
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int j = 0; j < MAXNT; j++) {
+            accr[t][j] = fx4{0, 0, 0, 0};
+            acci[t][j] = fx4{0, 0, 0, 0};
+        }
+    struct Raw {
+        float w[RPL], s[RPL];
+        float2 xa[RPL][NI], xb[2][RPL][NI];
+    };
+    auto fetch = [&](int kb, Raw &r) {
+#pragma unroll
+        for (int rr = 0; rr < RPL; rr++) {
+            const int kk0 = kb + 8 * kg + rr * K2;
+            r.w[rr] = 0.f;
+            r.s[rr] = 0.f;
+#pragma unroll
+            for (int it = 0; it < NI; it++) {
+                r.xa[rr][it] = make_float2(0.f, 0.f);
+                r.xb[0][rr][it] = make_float2(0.f, 0.f);
+                r.xb[1][rr][it] = make_float2(0.f, 0.f);
+            }
+            if (kk0 < kend) {
+                const int row = r0 + kk0 / K2;
+                r.w[rr] = (float)a.ws[row];
+                r.s[rr] = (float)a.ss[row];
+#pragma unroll
+                for (int it = 0; it < NI; it++) {
+                    const int ch = (NI == 1) ? 0 : (2 * a.acc_index + it);
+                    const float2 *xr = X + ((size_t)row * a.nch + ch) * a.D;
+                    if (ia < a.D) r.xa[rr][it] = xr[ia];
+                    if (ib0 < a.D) r.xb[0][rr][it] = xr[ib0];
+                    if (two && ib1 < a.D) r.xb[1][rr][it] = xr[ib1];
+                }
+            }
+        }
+    };
+    const int rq = ((r1 - r0 + 63) / 64) * 16;
+    const int kw0 = min(kend, wave * rq * K2), kw1 = min(kend, (wave + 1) * rq * K2);
+    Raw cur, nxt;
+    fetch(kw0, cur);
+    for (int kb = kw0; kb < kw1; kb += 32) {
+        fetch(kb + 32 < kw1 ? kb + 32 : kend, nxt);
+        /* K entries e = 2 q, 2 q + 1 (q = rr NI + it) are the re and im parts
+         * of one row's value: one packed word of each operand */
+        float av[4][2], sv[4];
+        u4v brh[2], brl[2], bih[2], bil[2];
+#pragma unroll
+        for (int rr = 0; rr < RPL; rr++)
+#pragma unroll
+            for (int it = 0; it < NI; it++) {
+                const int q = rr * NI + it;
+                const float w = it ? -cur.w[rr] : cur.w[rr];
+                av[q][0] = cur.xa[rr][it].x;
+                av[q][1] = cur.xa[rr][it].y;
+                sv[q] = cur.s[rr];
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    const float2 xb = cur.xb[t][rr][it];
+                    unsigned h, l;
+                    split2_bf16(w * xb.x, w * xb.y, h, l);
+                    brh[t][q] = h;
+                    brl[t][q] = l;
+                    split2_bf16(-(w * xb.y), w * xb.x, h, l);
+                    bih[t][q] = h;
+                    bil[t][q] = l;
+                }
+            }
+#pragma unroll
+        for (int j = 0; j < MAXNT; j++) {
+            if (j < NT) {
+                u4v ah, al;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    unsigned h, l;
+                    split2_bf16(av[q][0], av[q][1], h, l);
+                    ah[q] = h;
+                    al[q] = l;
+                    av[q][0] *= sv[q];
+                    av[q][1] *= sv[q];
+                }
+                const bf8 ahi = __builtin_bit_cast(bf8, ah), alo = __builtin_bit_cast(bf8, al);
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    if (t == 1 && !two) break;
+                    const bf8 rh = __builtin_bit_cast(bf8, brh[t]), rl = __builtin_bit_cast(bf8, brl[t]);
+                    const bf8 ih = __builtin_bit_cast(bf8, bih[t]), il = __builtin_bit_cast(bf8, bil[t]);
+                    accr[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, rh, accr[t][j], 0, 0, 0);
+                    accr[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, rl, accr[t][j], 0, 0, 0);
+                    accr[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, rh, accr[t][j], 0, 0, 0);
+                    acci[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, ih, acci[t][j], 0, 0, 0);
+                    acci[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, il, acci[t][j], 0, 0, 0);
+                    acci[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, ih, acci[t][j], 0, 0, 0);
+                }
+            }
+        }
+        cur = nxt;
+    }
+    /* per tile pair: the four quarters' FP32 partials summed in FP64 and
+     * stored, in halves of the orders */
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        if (t == 1 && !two) break;
+        const int pairi = ti * (ti + 1) / 2 + tj0 + t;
+#pragma unroll
+        for (int h0 = 0; h0 < MAXNT; h0 += HALF) {
+            if (h0 >= NT) break;
+            const int hn = min(HALF, NT - h0);
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < HALF; j++)
+                if (j < hn) {
+                    red[wave][lane][2 * j] = accr[t][h0 + j];
+                    red[wave][lane][2 * j + 1] = acci[t][h0 + j];
+                }
+            __syncthreads();
+            for (int sl = wave; sl < 2 * hn; sl += 4) {
+                const int j = h0 + sl / 2, im = sl & 1;
+                const size_t o = (((size_t)b * NT + j) * a.npairs + pairi) * 256;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const double tt = a.f64sum ? ((double)red[0][lane][sl][v] + (double)red[1][lane][sl][v]) +
+                                                     ((double)red[2][lane][sl][v] + (double)red[3][lane][sl][v])
+                                               : (double)((red[0][lane][sl][v] + red[1][lane][sl][v]) +
+                                                          (red[2][lane][sl][v] + red[3][lane][sl][v]));
+                    const int row = 4 * (lane >> 4) + v, col = lane & 15;
+                    if (a.m32)
+                        reinterpret_cast<float *>(a.M)[2 * (o + row * TILE + col) + im] = (float)tt;
+                    else
+                        reinterpret_cast<double *>(a.M)[2 * (o + row * TILE + col) + im] = tt;
+                }
+            }
+        }
+    }
+}
+
 /* ------------------------------------------------------------------
  * direct windowed sum (PMX_PREC_DIRECT, triangle, rect): one block per
  * (tile pair, bin); Z[f][pair][256] += sum_t w delta(dE_t - w_f) X X^H
@@ -2711,12 +2890,38 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                                       16.0 * (double)nbins * NT * npairs * 256 * 2;
                     const bool bf = p->precision == PMX_PREC_BF16X3 && NT <= MOM_MAXNT; /* else the FP32 MFMA kernel */
                     Prof pf(c, bf ? "moment_bf16x3" : (fp32 ? "moment_mfma_f32" : "moment_mfma_f64"), fl, by);
-                    if (bf && nch == 1 && NT <= 9)
-                        hipLaunchKernelGGL((moment_bf16x3_kernel<1, 9>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
-                    else if (bf && nch == 1)
+                    if (bf && NT <= 9) {
+                        /* one tile pair per workgroup, or two sharing their row tile */
+                        int ngroups = 0;
+                        for (int t = 0; t < ntile; t++) ngroups += (t + 2) / 2;
+                        auto momb_cand = [&](const char *name, void (*kern)(MomArgs2), int ng) {
+                            return TuneCand{name, [&, kern, ng](int) {
+                                                hipLaunchKernelGGL(kern, dim3(ng * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
+                                                PMX_CHECK_HIP(hipGetLastError());
+                                                return 0;
+                                            }};
+                        };
+                        auto momw_cand = [&](const char *name, void (*kern)(MomArgs2, int)) {
+                            return TuneCand{name, [&, kern](int) {
+                                                hipLaunchKernelGGL(kern, dim3(ngroups * ((nbins + 7) & ~7)), dim3(256), 0,
+                                                                   st, ma, ngroups);
+                                                PMX_CHECK_HIP(hipGetLastError());
+                                                return 0;
+                                            }};
+                        };
+                        std::vector<TuneCand> mc;
+                        if (nch == 1) {
+                            mc.push_back(momw_cand("momb_t2", moment_bf3w_kernel<1>));
+                            mc.push_back(momb_cand("momb_t1", moment_bf16x3_kernel<1, 9>, npairs));
+                        } else {
+                            mc.push_back(momw_cand("momb_t2", moment_bf3w_kernel<2>));
+                            mc.push_back(momb_cand("momb_t1", moment_bf16x3_kernel<2, 9>, npairs));
+                        }
+                        const std::string key = "moment bf16x3 NT=" + std::to_string(NT) + " nch=" + std::to_string(nch) +
+                                                " pairs=" + std::to_string(npairs) + " nbins=" + std::to_string(nbins);
+                        if ((rc = tune_run(c, key, 1 << 30, 1 << 30, mc))) return rc;
+                    } else if (bf && nch == 1)
                         hipLaunchKernelGGL((moment_bf16x3_kernel<1, MOM_MAXNT>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
-                    else if (bf && NT <= 9)
-                        hipLaunchKernelGGL((moment_bf16x3_kernel<2, 9>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
                     else if (bf)
                         hipLaunchKernelGGL((moment_bf16x3_kernel<2, MOM_MAXNT>), dim3(npairs * ((nbins + 7) & ~7)), dim3(256), 0, st, ma);
                     else if (fp32 && NT <= 12) {

@@ -259,6 +259,7 @@ def test_chi_precision_modes(manifest, ctx, case, precision, tol):
                                                 ("convb_fb4", "bf16x3", 5e-3), ("convb_fb8", "bf16x3", 5e-3),
                                                 ("kkb_ct2", "bf16x3", 5e-3), ("kkb_ct4", "bf16x3", 5e-3),
                                                 ("kkb_ct8", "bf16x3", 5e-3),
+                                                ("momb_t1", "bf16x3", 5e-3), ("momb_t2", "bf16x3", 5e-3),
                                                 ("mom_w2", "fp32", 1e-4)])
 def test_chi_tuned_forms_against_reference(manifest, ctx, form, precision, tol, monkeypatch):
     """Every autotuner candidate of the convolution (frequency blocks per
@@ -280,6 +281,28 @@ def test_chi_tuned_forms_against_reference(manifest, ctx, form, precision, tol, 
     err = cli_args.slice_error(np.concatenate([re, im]), ref, _nslice(args))
     _say("  slice error %.3e" % err)
     assert err < tol
+
+
+@pytest.mark.parametrize("case", ["si_chi_lor", "te_soc_chi"])
+def test_moment_split_bf16_tile_forms_identical(manifest, ctx, case, monkeypatch):
+    """The split-BF16 rank update with two tile pairs per workgroup (one
+    per-order split of the shared row tile) against the one-pair form: the
+    same products accumulated in the same order, so χ agrees bit for bit, on
+    the spatial basis and on the spinor basis (two channels per row)."""
+    args = manifest[case]["args"]
+    _say("chi bf16x3 moment forms on %s" % case)
+    inp = capi.Input(_expand(args))
+    ctx.set_system(inp.system(), inp)
+    s, p = cli_args.build(args)
+    off = ctx.ref_energy(p["refpoint"], p["nvalence"])
+    out = {}
+    for form in ("momb_t1", "momb_t2"):
+        monkeypatch.setenv("PMX_FORCE_KERNEL", form)
+        re, im = ctx.chi(p["K"], p["KW"], p["Q"], p["nband"], off, p["delta"], p["epsilon"], p["nfreq"],
+                         p["dfreq"], p["kk"], "bf16x3")
+        out[form] = np.concatenate([re, im])
+    assert np.abs(out["momb_t1"]).max() > 0
+    assert np.array_equal(out["momb_t1"], out["momb_t2"])
 
 
 def test_kk_split_bf16_against_fp32_kk(manifest, ctx, monkeypatch):
