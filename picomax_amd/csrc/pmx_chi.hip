@@ -1723,22 +1723,35 @@ template <class T, int FB, int OCC> __global__ void __launch_bounds__(256, OCC) 
 /* The convolution of the split-BF16 path: the GEMM of conv_mfma_kernel<float>
  * with each operand x = hi + lo in BF16 and three v_mfma_f32_16x16x32_bf16
  * per product, FP32 accumulation.  The Taylor table is split once as it is
- * staged in LDS, each entry's hi and lo in one 32-bit word; the moments are
- * split as they are loaded.  Lane l holds K entries 8(l/16) .. +7 of A (the
- * table row of frequency l%16) and of B (the moment rows of element l%16);
- * D rows 4(l/16)+v.  The K axis runs over (bin, order) from the first bin
- * under the workgroup's frequencies, in steps of 32; a block skips the steps
- * outside its band. */
+ * staged in LDS, as two arrays of overlapping pairs, kh2[P + i] = (hi_i,
+ * hi_{i+1}) and kl2 likewise, zero-padded by P = 15 NT + 31 entries on both
+ * sides: a lane's eight K entries are four pair words at any alignment, and
+ * every index a non-skipped K step forms lies in the padded range (entries
+ * outside the window read zero), so the loads need no selects.  The moments
+ * are split as they are loaded.  Lane l holds K entries 8(l/16) .. +7 of A
+ * (the table row of frequency l%16) and of B (the moment rows of element
+ * l%16); D rows 4(l/16)+v.  The K axis runs over (bin, order) from the first
+ * bin under the workgroup's frequencies, in steps of 32; a block skips the
+ * steps outside its band.  Rows past nfreq compute values that are not
+ * stored; K entries past the band's last bin meet zero moments. */
+__host__ __device__ constexpr int conv_bf3_pad(int NT) { return 15 * NT + 31; }
+__host__ __device__ constexpr size_t conv_bf3_lds(int W, int NT) {
+    return 2 * sizeof(unsigned) * ((size_t)W * NT + 2 * conv_bf3_pad(NT));
+}
 template <int FB, int OCC> __global__ void __launch_bounds__(256, OCC) conv_bf3_kernel(ConvArgs a) {
     extern __shared__ double smem_c[];
-    const int W = a.W, WN = W * a.NT;
-    unsigned *ks = reinterpret_cast<unsigned *>(smem_c); /* [W][NT]: bf16 hi | lo << 16 */
+    const int W = a.W, WN = W * a.NT, P = conv_bf3_pad(a.NT), L = WN + 2 * P;
+    unsigned *kh2 = reinterpret_cast<unsigned *>(smem_c), *kl2 = kh2 + L;
     {
         const float *kt = reinterpret_cast<const float *>(a.Kt);
-        for (int i = threadIdx.x; i < WN; i += 256) {
-            __bf16 h, l;
-            split_bf16(kt[i], h, l);
-            ks[i] = (unsigned)__builtin_bit_cast(unsigned short, h) | ((unsigned)__builtin_bit_cast(unsigned short, l) << 16);
+        auto bits = [](__bf16 x) { return (unsigned)__builtin_bit_cast(unsigned short, x); };
+        for (int i = threadIdx.x; i < L; i += 256) {
+            const int t0 = i - P, t1 = t0 + 1;
+            __bf16 h0 = (__bf16)0.0f, l0 = (__bf16)0.0f, h1 = (__bf16)0.0f, l1 = (__bf16)0.0f;
+            if (t0 >= 0 && t0 < WN) split_bf16(kt[t0], h0, l0);
+            if (t1 >= 0 && t1 < WN) split_bf16(kt[t1], h1, l1);
+            kh2[i] = bits(h0) | (bits(h1) << 16);
+            kl2[i] = bits(l0) | (bits(l1) << 16);
         }
         __syncthreads();
     }
@@ -1759,7 +1772,6 @@ template <int FB, int OCC> __global__ void __launch_bounds__(256, OCC) conv_bf3_
         accI[q] = fx4{0, 0, 0, 0};
     }
     int klo[FB], khi[FB], toff[FB];
-    bool fok[FB];
 #pragma unroll
     for (int q = 0; q < FB; q++) {
         const int fq = fb0 + 16 * q;
@@ -1768,40 +1780,43 @@ template <int FB, int OCC> __global__ void __launch_bounds__(256, OCC) conv_bf3_
         khi[q] = (bhi - b_lo + 1) * a.NT - 1; /* below klo when the band misses the bins */
         const int fa = fq + (lane & 15);      /* A row: frequency */
         toff[q] = (b_lo - fa + a.R0) * a.NT;
-        fok[q] = fa < a.nfreq;
     }
+    /* branch-free: the index is clamped into the band (the bin clamped into
+     * the array when the band is empty) and entries past it read as zero */
+    const float2 *Mb = M + (size_t)min(b_lo, a.nbins - 1) * a.NT * E + eb;
+    const int kc1 = max(kcount - 1, 0);
     auto mload = [&](int kk) -> float2 {
-        return kk < kcount ? M[((size_t)b_lo * a.NT + kk) * E + eb] : float2{0.0f, 0.0f};
+        /* kcount E < 2^32, checked by the host */
+        const float2 v = Mb[(unsigned)min(kk, kc1) * (unsigned)E];
+        return kk < kcount ? v : float2{0.0f, 0.0f};
     };
     float2 mc[8];
 #pragma unroll
     for (int e = 0; e < 8; e++) mc[e] = mload(kq8 + e);
     for (int k0 = 0; k0 < kcount; k0 += 32) {
-        /* this step's moments split; the next step's loaded */
-        bf8 brh, brl, bih, bil;
+        /* this step's moments split, K entries (2 u, 2 u + 1) to a packed
+         * word each; the next step's loaded */
+        u4v rh, rl, ih, il;
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            __bf16 h, l;
-            split_bf16(mc[e].x, h, l);
-            brh[e] = h;
-            brl[e] = l;
-            split_bf16(mc[e].y, h, l);
-            bih[e] = h;
-            bil[e] = l;
+        for (int u = 0; u < 4; u++) {
+            unsigned h, l;
+            split2_bf16(mc[2 * u].x, mc[2 * u + 1].x, h, l);
+            rh[u] = h;
+            rl[u] = l;
+            split2_bf16(mc[2 * u].y, mc[2 * u + 1].y, h, l);
+            ih[u] = h;
+            il[u] = l;
         }
+        const bf8 brh = __builtin_bit_cast(bf8, rh), brl = __builtin_bit_cast(bf8, rl);
+        const bf8 bih = __builtin_bit_cast(bf8, ih), bil = __builtin_bit_cast(bf8, il);
 #pragma unroll
         for (int e = 0; e < 8; e++) mc[e] = mload(k0 + 32 + kq8 + e);
 #pragma unroll
         for (int q = 0; q < FB; q++) {
             if (k0 + 31 < klo[q] || k0 > khi[q]) continue; /* outside this block's band */
-            bf8 ah, al;
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const int kk = k0 + kq8 + e, ti = kk + toff[q];
-                const unsigned wv = (kk < kcount && fok[q] && ti >= 0 && ti < WN) ? ks[ti] : 0u;
-                ah[e] = __builtin_bit_cast(__bf16, (unsigned short)(wv & 0xffffu));
-                al[e] = __builtin_bit_cast(__bf16, (unsigned short)(wv >> 16));
-            }
+            const int ti = P + k0 + kq8 + toff[q]; /* in [0, L - 7) for a step in the band */
+            const bf8 ah = __builtin_bit_cast(bf8, u4v{kh2[ti], kh2[ti + 2], kh2[ti + 4], kh2[ti + 6]});
+            const bf8 al = __builtin_bit_cast(bf8, u4v{kl2[ti], kl2[ti + 2], kl2[ti + 4], kl2[ti + 6]});
             accR[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, brh, accR[q], 0, 0, 0);
             accR[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, brl, accR[q], 0, 0, 0);
             accR[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, brh, accR[q], 0, 0, 0);
@@ -2959,31 +2974,35 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                                       fp32 ? (const void *)w.Ktf.p : (const void *)w.Ktd.p, Z.p, kt_lds};
                     /* frequency blocks per wave (and the waves per SIMD their
                      * registers allow), chosen by the autotuner per shape */
-                    auto conv_cand = [&](const char *name, void (*kern)(ConvArgs), int fb) {
-                        return TuneCand{name, [&, kern, fb](int) {
-                                            if (lds_c > 65536)
+                    const size_t lds_b3 = conv_bf3_lds(W, NT);
+                    auto conv_cand = [&](const char *name, void (*kern)(ConvArgs), int fb, size_t lds) {
+                        return TuneCand{name, [&, kern, fb, lds](int) {
+                                            if (lds > 65536)
                                                 PMX_CHECK_HIP(hipFuncSetAttribute(
                                                     (const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                    (int)lds_c));
+                                                    (int)lds));
                                             const dim3 grid((npairs * 256 + 63) / 64,
                                                             (p->nfreq + 16 * fb - 1) / (16 * fb));
-                                            hipLaunchKernelGGL(kern, grid, dim3(256), lds_c, st, ca);
+                                            hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, ca);
                                             PMX_CHECK_HIP(hipGetLastError());
                                             return 0;
                                         }};
                     };
                     std::vector<TuneCand> cc;
-                    const bool cbf = fp32 && kt_lds && p->precision == PMX_PREC_BF16X3;
+                    /* the split-BF16 kernel's 32-bit moment offsets: a workgroup's band
+                     * spans at most 16 FB + W bins (FB <= 8) */
+                    const bool cbf = fp32 && lds_b3 <= c->lds_max && p->precision == PMX_PREC_BF16X3 &&
+                                     (double)(128 + W) * NT * npairs * 256 < 4294967296.0;
                     if (cbf) {
-                        cc.push_back(conv_cand("convb_fb4", conv_bf3_kernel<4, 4>, 4));
-                        cc.push_back(conv_cand("convb_fb8", conv_bf3_kernel<8, 2>, 8));
+                        cc.push_back(conv_cand("convb_fb4", conv_bf3_kernel<4, 4>, 4, lds_b3));
+                        cc.push_back(conv_cand("convb_fb8", conv_bf3_kernel<8, 2>, 8, lds_b3));
                     } else if (fp32) {
-                        cc.push_back(conv_cand("conv_fb4", conv_mfma_kernel<float, 4, 8>, 4));
-                        cc.push_back(conv_cand("conv_fb2", conv_mfma_kernel<float, 2, 8>, 2));
-                        cc.push_back(conv_cand("conv_fb8", conv_mfma_kernel<float, 8, 4>, 8));
+                        cc.push_back(conv_cand("conv_fb4", conv_mfma_kernel<float, 4, 8>, 4, lds_c));
+                        cc.push_back(conv_cand("conv_fb2", conv_mfma_kernel<float, 2, 8>, 2, lds_c));
+                        cc.push_back(conv_cand("conv_fb8", conv_mfma_kernel<float, 8, 4>, 8, lds_c));
                     } else {
-                        cc.push_back(conv_cand("conv_fb4", conv_mfma_kernel<double, 4, 1>, 4));
-                        cc.push_back(conv_cand("conv_fb2", conv_mfma_kernel<double, 2, 1>, 2));
+                        cc.push_back(conv_cand("conv_fb4", conv_mfma_kernel<double, 4, 1>, 4, lds_c));
+                        cc.push_back(conv_cand("conv_fb2", conv_mfma_kernel<double, 2, 1>, 2, lds_c));
                     }
                     const std::string key = std::string("conv ") + (cbf ? "bf16x3" : (fp32 ? "f32" : "f64")) + " E=" +
                                             std::to_string(npairs * 256) + " nbins=" + std::to_string(nbins) +
