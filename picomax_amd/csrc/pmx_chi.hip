@@ -548,6 +548,18 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
     lo = (__bf16)(x - (float)hi);
 }
 
+/* x0, x1 -> packed BF16 hi (x0 in the low half) and lo = bf16(x - hi), both
+ * rounded to nearest even by v_cvt_pk_bf16_f32; hi's FP32 values are read
+ * back from the packed word by a shift and a mask */
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split2_bf16(float x0, float x1, unsigned &hi, unsigned &lo) {
+    hi = __builtin_bit_cast(unsigned, __builtin_convertvector(f2v{x0, x1}, bf2));
+    const float h0 = __builtin_bit_cast(float, hi << 16), h1 = __builtin_bit_cast(float, hi & 0xffff0000u);
+    lo = __builtin_bit_cast(unsigned, __builtin_convertvector(f2v{x0 - h0, x1 - h1}, bf2));
+}
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+
 /* ------------------------------------------------------------------
  * rho for the spatial basis in split BF16 (the split-BF16 path): the sums of
  * rho_cart_kernel with every operand x = hi + lo in BF16 and three
@@ -630,46 +642,36 @@ __global__ void __launch_bounds__(256, 3) rho_bf3_kernel(RhoArgs a, int K2p) {
                 ln[e] = (col_ok && pn < a.npw && k0 + 32 < K2p) ? lm[pn] : -1;
             }
             const bf8 ah = crow_ok ? rh[(k0 + kq8) / 8] : bf8{}, al = crow_ok ? rl[(k0 + kq8) / 8] : bf8{};
-            bf8 bh[4][2], bl[4][2]; /* [channel][re, im] */
+            u4v bh[4][2], bl[4][2]; /* [channel][re, im]: word e holds K entries 2e, 2e+1 */
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                const int pp = p0 + e;
-                float2 q = float2{0.0f, 0.0f};
-                float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-                if (lc[e] >= 0) q = cq[lc[e]];
-                if (pp < a.npw) {
-                    gx = sg[3 * pp];
-                    gy = sg[3 * pp + 1];
-                    gz = sg[3 * pp + 2];
-                }
+                const int pp = min(p0 + e, a.npw - 1);
+                const float2 qg = cq[max(lc[e], 0)];
+                const float2 q = lc[e] >= 0 ? qg : float2{0.0f, 0.0f}; /* lc < 0 past npw */
+                const float gw[4] = {1.0f, sg[3 * pp], sg[3 * pp + 1], sg[3 * pp + 2]};
                 /* K = 2pp (re of Ck): B = (q.x, q.y); K = 2pp+1 (im of Ck): B = (q.y, -q.x) */
-                const float bre0 = q.x, bim0 = q.y, bre1 = q.y, bim1 = -q.x;
-                const float gw[4] = {1.0f, gx, gy, gz};
 #pragma unroll
                 for (int ch = 0; ch < 4; ch++) {
-                    __bf16 h, l;
-                    split_bf16(ch ? gw[ch] * bre0 : bre0, h, l);
-                    bh[ch][0][2 * e] = h;
-                    bl[ch][0][2 * e] = l;
-                    split_bf16(ch ? gw[ch] * bim0 : bim0, h, l);
-                    bh[ch][1][2 * e] = h;
-                    bl[ch][1][2 * e] = l;
-                    split_bf16(ch ? gw[ch] * bre1 : bre1, h, l);
-                    bh[ch][0][2 * e + 1] = h;
-                    bl[ch][0][2 * e + 1] = l;
-                    split_bf16(ch ? gw[ch] * bim1 : bim1, h, l);
-                    bh[ch][1][2 * e + 1] = h;
-                    bl[ch][1][2 * e + 1] = l;
+                    const float bx = ch ? gw[ch] * q.x : q.x, by = ch ? gw[ch] * q.y : q.y;
+                    unsigned h, l;
+                    split2_bf16(bx, by, h, l);
+                    bh[ch][0][e] = h;
+                    bl[ch][0][e] = l;
+                    split2_bf16(by, -bx, h, l);
+                    bh[ch][1][e] = h;
+                    bl[ch][1][e] = l;
                 }
             }
 #pragma unroll
             for (int ch = 0; ch < 4; ch++) {
-                accr[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[ch][0], accr[ch], 0, 0, 0);
-                accr[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[ch][0], accr[ch], 0, 0, 0);
-                accr[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[ch][0], accr[ch], 0, 0, 0);
-                acci[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[ch][1], acci[ch], 0, 0, 0);
-                acci[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[ch][1], acci[ch], 0, 0, 0);
-                acci[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[ch][1], acci[ch], 0, 0, 0);
+                const bf8 hr = __builtin_bit_cast(bf8, bh[ch][0]), lr = __builtin_bit_cast(bf8, bl[ch][0]);
+                const bf8 hi = __builtin_bit_cast(bf8, bh[ch][1]), li = __builtin_bit_cast(bf8, bl[ch][1]);
+                accr[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, hr, accr[ch], 0, 0, 0);
+                accr[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, lr, accr[ch], 0, 0, 0);
+                accr[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, hr, accr[ch], 0, 0, 0);
+                acci[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, hi, acci[ch], 0, 0, 0);
+                acci[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, li, acci[ch], 0, 0, 0);
+                acci[ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, hi, acci[ch], 0, 0, 0);
             }
         }
         if (!col_ok) continue;
@@ -1348,17 +1350,6 @@ __global__ void __launch_bounds__(256, 3) moment_bf16x3_kernel(MomArgs2 a) {
     }
 }
 
-/* x0, x1 -> packed BF16 hi (x0 in the low half) and lo = bf16(x - hi), both
- * rounded to nearest even by v_cvt_pk_bf16_f32; hi's FP32 values are read
- * back from the packed word by a shift and a mask */
-typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
-typedef float f2v __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void split2_bf16(float x0, float x1, unsigned &hi, unsigned &lo) {
-    hi = __builtin_bit_cast(unsigned, __builtin_convertvector(f2v{x0, x1}, bf2));
-    const float h0 = __builtin_bit_cast(float, hi << 16), h1 = __builtin_bit_cast(float, hi & 0xffff0000u);
-    lo = __builtin_bit_cast(unsigned, __builtin_convertvector(f2v{x0 - h0, x1 - h1}, bf2));
-}
-typedef unsigned u4v __attribute__((ext_vector_type(4)));
 
 /* The split-BF16 rank update with two tile pairs per workgroup: (ti, tj0)
  * and (ti, tj1 = tj0 + 1) share the A side (row tile ti), which is scaled by
