@@ -2119,6 +2119,96 @@ __global__ void __launch_bounds__(256, OCC) kk_bf3_kernel(int M, int NF, int NFp
         }
 }
 
+/* kk_bf3_kernel with the K tile staged in LDS: the four waves of a
+ * workgroup take four row tiles of one column group, so they share the
+ * group's 16 CT columns of K.  Each 32-entry K step of that tile (hi and lo,
+ * 4 CT KiB) is loaded once per workgroup into one of two LDS buffers, one
+ * step ahead, with one barrier per step.  Waves whose rows lie past M run
+ * the loop on the last row (barriers) and store nothing. */
+constexpr int KKL_RS = 40; /* LDS row stride of one K column, in bf16 (32 + 8: bank spread) */
+template <int CT, int OCC>
+__global__ void __launch_bounds__(256, OCC) kk_bf3l_kernel(int M, int NF, int NFp, const __bf16 *Ih, const __bf16 *Il,
+                                                           const __bf16 *Rh, const __bf16 *Rl, const __bf16 *KhT,
+                                                           const __bf16 *KlT, const double *Re0, const double *Im0,
+                                                           double *Re, double *Im, int ne2, int nq, int q,
+                                                           const int *qp) {
+    if (qp) q = *qp;
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2][2][16 * CT * KKL_RS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, tid = threadIdx.x;
+    const int ncg = (NF + 16 * CT - 1) / (16 * CT);
+    const int slot = blockIdx.x >> 3;
+    const int rg = (blockIdx.x & 7) + 8 * (slot / ncg), cg = slot % ncg;
+    const int row0 = (rg * 4 + wave) * 16, col0 = cg * 16 * CT;
+    if (rg * 64 >= M) return; /* the whole workgroup */
+    const int kq = lane >> 4;
+    const size_t ao = (size_t)min(row0 + (lane & 15), M - 1) * NFp + 8 * kq;
+    const bf8 *aIh = reinterpret_cast<const bf8 *>(Ih + ao), *aIl = reinterpret_cast<const bf8 *>(Il + ao);
+    const bf8 *aRh = reinterpret_cast<const bf8 *>(Rh + ao), *aRl = reinterpret_cast<const bf8 *>(Rl + ao);
+    /* the staging items of this thread: (half, column, quarter of the step) */
+    constexpr int NI = 16 * CT * 2 * 4 / 256; /* 16-byte items per thread per step */
+    const bf8 *gsrc[NI];
+    int ldst[NI];
+#pragma unroll
+    for (int u = 0; u < NI; u++) {
+        const int i = tid + 256 * u, h = i / (16 * CT * 4), col = (i / 4) % (16 * CT), qq = i % 4;
+        gsrc[u] = reinterpret_cast<const bf8 *>((h ? KlT : KhT) + (size_t)(col0 + col) * NFp + 8 * qq);
+        ldst[u] = (h * 16 * CT + col) * KKL_RS + 8 * qq;
+    }
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    v4 accR[CT], accI[CT];
+#pragma unroll
+    for (int c = 0; c < CT; c++) {
+        accR[c] = v4{0, 0, 0, 0};
+        accI[c] = v4{0, 0, 0, 0};
+    }
+    const int nks = NFp / 32;
+    bf8 st[NI];
+#pragma unroll
+    for (int u = 0; u < NI; u++) st[u] = gsrc[u][0];
+#pragma unroll
+    for (int u = 0; u < NI; u++) *reinterpret_cast<bf8 *>(&Bs[0][0][0] + ldst[u]) = st[u];
+    bf8 ih = aIh[0], il = aIl[0], rh = aRh[0], rl = aRl[0];
+    __syncthreads();
+    for (int s = 0; s < nks; s++) {
+        const int sn = min(s + 1, nks - 1);
+#pragma unroll
+        for (int u = 0; u < NI; u++) st[u] = gsrc[u][4 * sn];
+        const bf8 ih2 = aIh[4 * sn], il2 = aIl[4 * sn], rh2 = aRh[4 * sn], rl2 = aRl[4 * sn];
+        const __bf16 *bh = &Bs[s & 1][0][0], *bl = &Bs[s & 1][1][0];
+#pragma unroll
+        for (int c = 0; c < CT; c++) {
+            const int bo = (16 * c + (lane & 15)) * KKL_RS + 8 * kq;
+            const bf8 kh = *reinterpret_cast<const bf8 *>(bh + bo), kl = *reinterpret_cast<const bf8 *>(bl + bo);
+            accR[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ih, kh, accR[c], 0, 0, 0);
+            accR[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ih, kl, accR[c], 0, 0, 0);
+            accR[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(il, kh, accR[c], 0, 0, 0);
+            accI[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rh, kh, accI[c], 0, 0, 0);
+            accI[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rh, kl, accI[c], 0, 0, 0);
+            accI[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rl, kh, accI[c], 0, 0, 0);
+        }
+        /* the other buffer was last read in step s - 1, before the barrier that ended it */
+#pragma unroll
+        for (int u = 0; u < NI; u++) *reinterpret_cast<bf8 *>(&Bs[(s + 1) & 1][0][0] + ldst[u]) = st[u];
+        ih = ih2;
+        il = il2;
+        rh = rh2;
+        rl = rl2;
+        __syncthreads();
+    }
+    if (row0 >= M) return;
+#pragma unroll
+    for (int c = 0; c < CT; c++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int r = row0 + 4 * kq + v, cidx = col0 + 16 * c + (lane & 15);
+            if (r >= M || cidx >= NF) continue;
+            const size_t ij = r / ne2, e2 = r % ne2;
+            const size_t o = ((ij * nq + q) * ne2 + e2) * NF + cidx;
+            Re[o] = Re0[(size_t)r * NF + cidx] + (double)accR[c][v];
+            Im[o] = Im0[(size_t)r * NF + cidx] + (double)accI[c][v];
+        }
+}
+
 __global__ void place_kernel(int nel, int nfreq, int ne2, int nq, int q, const double *Re0, const double *Im0,
                              double *Re, double *Im, const int *qp) {
     if (qp) q = *qp;
@@ -3071,9 +3161,9 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
             };
             std::vector<TuneCand> kc;
             if (kkbf) {
+                kc.push_back(kkb_cand("kkb_l8", kk_bf3l_kernel<8, 2>, 8));
+                kc.push_back(kkb_cand("kkb_l4", kk_bf3l_kernel<4, 4>, 4));
                 kc.push_back(kkb_cand("kkb_ct4", kk_bf3_kernel<4, 4>, 4));
-                kc.push_back(kkb_cand("kkb_ct2", kk_bf3_kernel<2, 6>, 2));
-                kc.push_back(kkb_cand("kkb_ct8", kk_bf3_kernel<8, 2>, 8));
             } else if (fp32) {
                 kc.push_back(kk_cand("kk_ct4", kk_gemm_kernel<float, 4>, 4, (const float *)w.Kkf.p));
                 kc.push_back(kk_cand("kk_ct2", kk_gemm_kernel<float, 2>, 2, (const float *)w.Kkf.p));

@@ -257,8 +257,8 @@ def test_chi_precision_modes(manifest, ctx, case, precision, tol):
                                                 ("kk_ct2", "fp32", 1e-4), ("kk_ct8w4", "fp32", 1e-4),
                                                 ("kk_ct16w2", "fp32", 1e-4),
                                                 ("convb_fb4", "bf16x3", 5e-3), ("convb_fb8", "bf16x3", 5e-3),
-                                                ("kkb_ct2", "bf16x3", 5e-3), ("kkb_ct4", "bf16x3", 5e-3),
-                                                ("kkb_ct8", "bf16x3", 5e-3),
+                                                ("kkb_ct4", "bf16x3", 5e-3), ("kkb_l8", "bf16x3", 5e-3),
+                                                ("kkb_l4", "bf16x3", 5e-3),
                                                 ("momb_t1", "bf16x3", 5e-3), ("momb_t2", "bf16x3", 5e-3),
                                                 ("mom_w2", "fp32", 1e-4)])
 def test_chi_tuned_forms_against_reference(manifest, ctx, form, precision, tol, monkeypatch):
