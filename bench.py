@@ -411,8 +411,8 @@ def main():
                          "size, to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     opt = ap.parse_args()
     if opt.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (k-sharding is described "
-              "in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the chi path has no multi-GPU form (sharding by k is a "
+              "design, not code); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     if opt.config != "c2":
