@@ -1853,45 +1853,77 @@ __device__ __forceinline__ zd zfetch(const zd *Z, int f, int npairs, int a, int 
     return cj ? cconj(v) : v;
 }
 
-__global__ void pack_kernel(PackArgs a) {
-    const int ne2 = a.neps * (a.neps + 1) / 2;
-    const size_t total = (size_t)9 * ne2 * a.nfreq;
-    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
-        const int f = idx % a.nfreq;
-        const size_t el = idx / a.nfreq;
-        const int ij = el / ne2, e2 = el % ne2;
-        const int i = ij / 3, j = ij % 3;
-        int m = (int)((sqrt(8.0 * e2 + 1.0) - 1.0) * 0.5);
-        while ((m + 1) * (m + 2) / 2 <= e2) m++;
-        while (m * (m + 1) / 2 > e2) m--;
-        const int n = e2 - m * (m + 1) / 2;
-        const int ra = 3 * m + i, rb = 3 * n + j;
-        zd z1 = zfetch(a.Z1, f, a.npairs, ra, rb);
-        double imv = z1.x;
-        double rev = a.soc ? -0.5 * (z1.y + zfetch(a.Z2, f, a.npairs, ra, rb).y) : -z1.y;
-        const double w = a.w0 + a.dfreq * f;
-        const double inv = (f == 0) ? 0.0 : 1.0 / (w * w);
-        const size_t dst = (((size_t)ij * a.nq + a.q) * ne2 + e2) * a.nfreq + f; /* nq = 1, q = 0: compact */
-        const double iv = a.SF * (imv * inv), rv = a.SF * (rev * inv);
-        a.ImX[dst] = iv;
-        a.ReX[dst] = rv;
-        if (a.Ih) {
-            const size_t o = el * a.NFp + f;
-            __bf16 h, l;
-            split_bf16((float)iv, h, l);
-            a.Ih[o] = h;
-            a.Il[o] = l;
-            split_bf16((float)rv, h, l);
-            a.Rh[o] = h;
-            a.Rl[o] = l;
-            if (f == a.nfreq - 1)
-                for (size_t t = o + 1; t < (el + 1) * a.NFp; t++) {
-                    a.Ih[t] = (__bf16)0.0f;
-                    a.Il[t] = (__bf16)0.0f;
-                    a.Rh[t] = (__bf16)0.0f;
-                    a.Rl[t] = (__bf16)0.0f;
-                }
-        }
+/* one output element (row el = ij ne2 + e2, frequency f) of the packing from
+ * its Z values imv = Im-part source, rev = Re-part source */
+__device__ __forceinline__ void pack_put(const PackArgs &a, size_t el, int f, double imv, double rev) {
+    const double w = a.w0 + a.dfreq * f;
+    const double inv = (f == 0) ? 0.0 : 1.0 / (w * w);
+    const size_t dst = el * a.nfreq + f; /* nq = 1, q = 0: compact [ij][e2][f] */
+    const double iv = a.SF * (imv * inv), rv = a.SF * (rev * inv);
+    a.ImX[dst] = iv;
+    a.ReX[dst] = rv;
+    if (a.Ih) {
+        const size_t o = el * a.NFp + f;
+        __bf16 h, l;
+        split_bf16((float)iv, h, l);
+        a.Ih[o] = h;
+        a.Il[o] = l;
+        split_bf16((float)rv, h, l);
+        a.Rh[o] = h;
+        a.Rl[o] = l;
+        if (f == a.nfreq - 1)
+            for (size_t t = o + 1; t < (el + 1) * a.NFp; t++) {
+                a.Ih[t] = (__bf16)0.0f;
+                a.Il[t] = (__bf16)0.0f;
+                a.Rh[t] = (__bf16)0.0f;
+                a.Rl[t] = (__bf16)0.0f;
+            }
+    }
+}
+
+/* Packing as a transpose through LDS: a workgroup takes 32 Z
+ * entries (one tile pair's elements, [f][pair][256] layout) by 32
+ * frequencies, reads them along the entries and writes along the
+ * frequencies.  Each stored element (A, B) with A >= B (the lower triangle;
+ * a diagonal tile's upper elements are not used) gives output (i, j, m, n) =
+ * (A%3, B%3, A/3, B/3), and, when A and B lie in one 3x3 block (m = n, A > B),
+ * also (B%3, A%3, m, m) from its conjugate: every output (ij, m >= n)
+ * exactly once, with the values zfetch gives it (Z[a][b] for a >= b, else the
+ * conjugate of Z[b][a]). */
+constexpr int PKT = 32;
+__global__ void __launch_bounds__(256) pack_t_kernel(PackArgs a) {
+    __shared__ zd s1[PKT][PKT + 1], s2[PKT][PKT + 1]; /* [entry][f] */
+    const int NENT = a.npairs * 256, ne2 = a.neps * (a.neps + 1) / 2;
+    const int e0 = blockIdx.x * PKT, f0 = blockIdx.y * PKT;
+    const int tid = threadIdx.x, lo = tid % PKT, hi = tid / PKT; /* hi: 0..7 */
+#pragma unroll
+    for (int u = 0; u < PKT / 8; u++) {
+        const int fl = hi + 8 * u, e = e0 + lo, f = f0 + fl;
+        const bool ok = e < NENT && f < a.nfreq;
+        const size_t src = (size_t)f * NENT + e;
+        s1[lo][fl] = ok ? a.Z1[src] : zd{0.0, 0.0};
+        if (a.soc) s2[lo][fl] = ok ? a.Z2[src] : zd{0.0, 0.0};
+    }
+    __syncthreads();
+    const int f = f0 + lo;
+    if (f >= a.nfreq) return;
+#pragma unroll
+    for (int u = 0; u < PKT / 8; u++) {
+        const int el_ = hi + 8 * u, e = e0 + el_;
+        if (e >= NENT) continue;
+        int ti, tj;
+        tile_pair(e / 256, ti, tj);
+        const int A = ti * TILE + (e % 256) / TILE, B = tj * TILE + e % TILE;
+        if (A >= a.D || B >= a.D || A < B) continue;
+        const zd z1 = s1[el_][lo];
+        const double y2 = a.soc ? s2[el_][lo].y : 0.0;
+        const int m = A / 3, n = B / 3;
+        const size_t e2 = (size_t)m * (m + 1) / 2 + n;
+        /* output (A%3, B%3, m, n): z as stored */
+        pack_put(a, (size_t)(3 * (A % 3) + B % 3) * ne2 + e2, f, z1.x, a.soc ? -0.5 * (z1.y + y2) : -z1.y);
+        /* output (B%3, A%3, m, m) of the same 3x3 block: the conjugate */
+        if (m == n && A != B)
+            pack_put(a, (size_t)(3 * (B % 3) + A % 3) * ne2 + e2, f, z1.x, a.soc ? 0.5 * (z1.y + y2) : z1.y);
     }
 }
 
@@ -2038,7 +2070,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? OCC : 1) kk_gemm_kernel(
 
 /* The KK GEMM of the split-BF16 path: kk_gemm_kernel's sums with each
  * operand x = hi + lo in BF16 and three v_mfma_f32_16x16x32_bf16 per
- * product, FP32 accumulation.  The A rows come split from pack_kernel
+ * product, FP32 accumulation.  The A rows come split from pack_t_kernel
  * ([element][NFp]) and K transposed and split from kkmat_bf3_kernel
  * ([f][NFp], NFc >= NF rows), both zero past nfreq, so the K loop has no
  * bounds checks.  Lane l holds K entries 8(l/16) .. +7 of A row l%16 and of
@@ -3129,7 +3161,8 @@ static int chi_run(pmx_ctx *c, const pmx_chi_params *p, double *dRe, double *dIm
                 pa.Rl = pa.Rh + kkM * kkNFp;
                 pa.NFp = kkNFp;
             }
-            hipLaunchKernelGGL(pack_kernel, dim3(1024), dim3(256), 0, st, pa);
+            hipLaunchKernelGGL(pack_t_kernel, dim3((npairs * 256 + PKT - 1) / PKT, (p->nfreq + PKT - 1) / PKT), dim3(256),
+                               0, st, pa);
         }
         if ((rc = elapsed("    "))) return rc;
         if (p->kk) {
