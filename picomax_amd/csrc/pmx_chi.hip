@@ -560,6 +560,14 @@ __device__ __forceinline__ void split2_bf16(float x0, float x1, unsigned &hi, un
 }
 typedef unsigned u4v __attribute__((ext_vector_type(4)));
 
+/* From a packed hi (or lo) word of the pair (x, y) (x in the low half), the
+ * word of (y, -x) (NEG_HI) or of (-y, x) (NEG_LO): halves swapped, one sign
+ * flipped.  Round-to-nearest-even commutes with negation, so this equals
+ * splitting the permuted pair (up to the sign of a zero lo). */
+template <bool NEG_HI> __device__ __forceinline__ unsigned swapneg_bf16x2(unsigned w) {
+    return __builtin_amdgcn_alignbit(w, w, 16) ^ (NEG_HI ? 0x80000000u : 0x00008000u);
+}
+
 /* ------------------------------------------------------------------
  * rho for the spatial basis in split BF16 (the split-BF16 path): the sums of
  * rho_cart_kernel with every operand x = hi + lo in BF16 and three
@@ -657,9 +665,8 @@ __global__ void __launch_bounds__(256, 3) rho_bf3_kernel(RhoArgs a, int K2p) {
                     split2_bf16(bx, by, h, l);
                     bh[ch][0][e] = h;
                     bl[ch][0][e] = l;
-                    split2_bf16(by, -bx, h, l);
-                    bh[ch][1][e] = h;
-                    bl[ch][1][e] = l;
+                    bh[ch][1][e] = swapneg_bf16x2<true>(h); /* (by, -bx) */
+                    bl[ch][1][e] = swapneg_bf16x2<true>(l);
                 }
             }
 #pragma unroll
@@ -1445,9 +1452,8 @@ __global__ void __launch_bounds__(256, 2) moment_bf3w_kernel(MomArgs2 a, int ngr
                     split2_bf16(w * xb.x, w * xb.y, h, l);
                     brh[t][q] = h;
                     brl[t][q] = l;
-                    split2_bf16(-(w * xb.y), w * xb.x, h, l);
-                    bih[t][q] = h;
-                    bil[t][q] = l;
+                    bih[t][q] = swapneg_bf16x2<false>(h); /* (-w xb.y, w xb.x) */
+                    bil[t][q] = swapneg_bf16x2<false>(l);
                 }
             }
 #pragma unroll
