@@ -1399,37 +1399,33 @@ __global__ void __launch_bounds__(256, 2) moment_bf3w_kernel(MomArgs2 a, int ngr
         float w[RPL], s[RPL];
         float2 xa[RPL][NI], xb[2][RPL][NI];
     };
+    /* branch-free: rows past the bin read its last row with weight 0 (the B
+     * side, w x, is then zero, so they add nothing), columns past D read
+     * column D-1 (the padding rows and columns of the last tile are never
+     * packed); the loads need only a valid row, which a non-empty bin has */
+    const int rlast = r1 - 1, iac = min(ia, a.D - 1), ib0c = min(ib0, a.D - 1), ib1c = min(ib1, a.D - 1);
     auto fetch = [&](int kb, Raw &r) {
 #pragma unroll
         for (int rr = 0; rr < RPL; rr++) {
             const int kk0 = kb + 8 * kg + rr * K2;
-            r.w[rr] = 0.f;
-            r.s[rr] = 0.f;
+            const int row = min(r0 + kk0 / K2, rlast);
+            const float wv = (float)a.ws[row];
+            r.w[rr] = kk0 < kend ? wv : 0.f;
+            r.s[rr] = (float)a.ss[row];
 #pragma unroll
             for (int it = 0; it < NI; it++) {
-                r.xa[rr][it] = make_float2(0.f, 0.f);
-                r.xb[0][rr][it] = make_float2(0.f, 0.f);
-                r.xb[1][rr][it] = make_float2(0.f, 0.f);
-            }
-            if (kk0 < kend) {
-                const int row = r0 + kk0 / K2;
-                r.w[rr] = (float)a.ws[row];
-                r.s[rr] = (float)a.ss[row];
-#pragma unroll
-                for (int it = 0; it < NI; it++) {
-                    const int ch = (NI == 1) ? 0 : (2 * a.acc_index + it);
-                    const float2 *xr = X + ((size_t)row * a.nch + ch) * a.D;
-                    if (ia < a.D) r.xa[rr][it] = xr[ia];
-                    if (ib0 < a.D) r.xb[0][rr][it] = xr[ib0];
-                    if (two && ib1 < a.D) r.xb[1][rr][it] = xr[ib1];
-                }
+                const int ch = (NI == 1) ? 0 : (2 * a.acc_index + it);
+                const float2 *xr = X + ((size_t)row * a.nch + ch) * a.D;
+                r.xa[rr][it] = xr[iac];
+                r.xb[0][rr][it] = xr[ib0c];
+                r.xb[1][rr][it] = xr[ib1c];
             }
         }
     };
     const int rq = ((r1 - r0 + 63) / 64) * 16;
     const int kw0 = min(kend, wave * rq * K2), kw1 = min(kend, (wave + 1) * rq * K2);
     Raw cur, nxt;
-    fetch(kw0, cur);
+    if (kend > 0) fetch(kw0, cur); /* an empty bin stores zero moments */
     for (int kb = kw0; kb < kw1; kb += 32) {
         fetch(kb + 32 < kw1 ? kb + 32 : kend, nxt);
         /* K entries e = 2 q, 2 q + 1 (q = rr NI + it) are the re and im parts
